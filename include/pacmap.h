@@ -1,0 +1,192 @@
+/*
+ * pacmap.h -- C ABI of the MI355X-native PaCMAP engine (libpacmap_mi355x.so).
+ *
+ * Drop-in for the public surface of beamform/pacmap-rs 0.2.6:
+ *   pacmap::fit_transform(x, Configuration)            src/lib.rs:294-297
+ *   pacmap::knn::find_k_nearest_neighbors(data, k)     src/knn.rs:137
+ * plus test seams for the per-iteration pieces the reference keeps private
+ * (gradient.rs:35, adam.rs:35, weights.rs:52, neighbors.rs:77-87).
+ *
+ * Conventions (all entry points):
+ *   - row-major, contiguous arrays; plain pointers and element counts;
+ *   - host pointers unless the name ends in _dev;
+ *   - outputs are caller-allocated;
+ *   - every call is blocking, owns its HIP stream and device buffers, and is
+ *     safe to call from several host threads at once;
+ *   - the return value is a pacmap_code; when `st` is non-NULL it receives the
+ *     code plus details (shape mismatch, HIP error, message).
+ */
+#ifndef PACMAP_H
+#define PACMAP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* One code per PaCMapError variant (lib.rs:870-922), then the new ones. */
+typedef enum {
+  PACMAP_OK = 0,
+  PACMAP_ERR_SAMPLE_SIZE = 1,                 /* lib.rs:875-876  n <= 1 */
+  PACMAP_ERR_INVALID_NEAREST_NEIGHBOR_SHAPE = 2, /* lib.rs:878-885 */
+  PACMAP_ERR_EMPTY_ARRAY_MEAN = 3,            /* lib.rs:887-889 */
+  PACMAP_ERR_NORMAL = 4,                      /* lib.rs:891-893 */
+  PACMAP_ERR_INVALID_NEIGHBOR_COUNT = 5,      /* lib.rs:895-897 */
+  PACMAP_ERR_INVALID_FAR_POINT_COUNT = 6,     /* lib.rs:899-901 */
+  PACMAP_ERR_EMPTY_ARRAY_MIN_MAX = 7,         /* lib.rs:903-905 */
+  PACMAP_ERR_ZERO_RANGE = 8,                  /* lib.rs:907-909 (declared, never returned) */
+  PACMAP_ERR_PCA = 9,                         /* lib.rs:911-913 */
+  PACMAP_ERR_NEIGHBORS = 10,                  /* lib.rs:915-917 */
+  PACMAP_ERR_MISSING_TRANSFORM = 11,          /* lib.rs:919-921 */
+  /* new: the reference panics (gradient.rs:123) where this returns */
+  PACMAP_ERR_INVALID_PAIR_INDEX = 12,
+  PACMAP_ERR_INVALID_ARGUMENT = 13,
+  PACMAP_ERR_GPU = 14,                        /* HIP runtime error, see gpu_error */
+  PACMAP_ERR_UNSUPPORTED = 15                 /* size beyond what a kernel supports */
+} pacmap_code;
+
+typedef struct {
+  int32_t code;          /* pacmap_code */
+  int32_t gpu_error;     /* hipError_t when code == PACMAP_ERR_GPU */
+  uint64_t expected[2];  /* InvalidNearestNeighborShape: [n * n_neighbors, 2] */
+  uint64_t actual[2];
+  char message[256];
+} pacmap_status;
+
+/* Initialization (lib.rs:226-239) */
+enum { PACMAP_INIT_PCA = 0, PACMAP_INIT_VALUE = 1, PACMAP_INIT_RANDOM = 2 };
+/* PairConfiguration (lib.rs:241-267) */
+enum { PACMAP_PAIRS_GENERATE = 0, PACMAP_PAIRS_NEIGHBORS_PROVIDED = 1, PACMAP_PAIRS_ALL_PROVIDED = 2 };
+/* kNN candidate precision (MI355X extension; every mode returns the exact
+ * FP32 neighbour sets of knn.rs:137-194, see pacmap_knn_exact) */
+enum { PACMAP_KNN_FP32 = 0, PACMAP_KNN_BF16 = 1, PACMAP_KNN_FP8 = 2 };
+
+/* Configuration (lib.rs:163-224).  Option<T> fields carry a has_ flag; enum
+ * payloads are borrowed pointers that must outlive the call. */
+typedef struct {
+  uint64_t embedding_dimensions;      /* default 2 */
+  int32_t init_kind;                  /* default PACMAP_INIT_PCA */
+  int32_t init_random_has_seed;       /* Random(Some(seed)) */
+  uint64_t init_random_seed;
+  const float *init_value;            /* Value: n x embedding_dimensions */
+  uint64_t init_value_rows, init_value_cols;
+  float mid_near_ratio;               /* default 0.5 */
+  float far_pair_ratio;               /* default 2.0 */
+  int32_t has_override_neighbors;
+  uint64_t override_neighbors;
+  int32_t has_seed;
+  uint64_t seed;
+  int32_t pair_kind;                  /* default PACMAP_PAIRS_GENERATE */
+  const uint32_t *pair_neighbors;     /* rows x cols (cols must be 2) */
+  uint64_t pair_neighbors_rows, pair_neighbors_cols;
+  const uint32_t *pair_mn;            /* AllProvided: rows x 2 */
+  uint64_t pair_mn_rows;
+  const uint32_t *pair_fp;            /* AllProvided: rows x 2 */
+  uint64_t pair_fp_rows;
+  float learning_rate;                /* default 1.0 */
+  uint64_t num_iters[3];              /* default (100, 100, 250) */
+  int32_t has_snapshots;
+  const uint64_t *snapshots;          /* iteration indices */
+  uint64_t num_snapshots;
+  uint64_t approx_threshold;          /* default 8000 */
+  /* MI355X extensions */
+  int32_t knn_precision;              /* PACMAP_KNN_*, default FP32 */
+  int32_t record_loss;                /* 1: keep the per-iteration loss (default 1) */
+} pacmap_config;
+
+/* Fills `cfg` with Configuration::default() (lib.rs:208-224). */
+void pacmap_config_default(pacmap_config *cfg);
+
+/* fit_transform (lib.rs:294-367).  y_out: n x dims.  snapshots_out: when the
+ * config has snapshots, num_snapshots x n x dims (else may be NULL).
+ * loss_out (may be NULL): total-iteration float array of losses. */
+int pacmap_fit_transform(const float *x, uint64_t n, uint64_t d, const pacmap_config *cfg,
+                         float *y_out, float *snapshots_out, float *loss_out,
+                         pacmap_status *st);
+
+/* Same, with x already resident in device memory (HBM); outputs on the host. */
+int pacmap_fit_transform_dev(const float *x_dev, uint64_t n, uint64_t d,
+                             const pacmap_config *cfg, float *y_out, float *snapshots_out,
+                             float *loss_out, pacmap_status *st);
+
+/* find_k_nearest_neighbors (knn.rs:137-194): per row the min(k, n-1) nearest
+ * other rows by the distance of distance.rs:50-75, ascending, ties by index.
+ * idx: n x min(k, n-1) u32, dist: n x min(k, n-1) f32.  n == 0 is a no-op. */
+int pacmap_knn_exact(const float *x, uint64_t n, uint64_t d, uint64_t k, uint32_t *idx,
+                     float *dist, pacmap_status *st);
+/* Same with a candidate precision (PACMAP_KNN_*); results equal pacmap_knn_exact.
+ * stats_out (may be NULL): [rows re-scanned exactly, candidates per row]. */
+int pacmap_knn(const float *x, uint64_t n, uint64_t d, uint64_t k, int32_t precision,
+               uint32_t *idx, float *dist, uint64_t *stats_out, pacmap_status *st);
+
+/* Seam: pacmap_grad (gradient.rs:35-74).  grad_out: (n+1) x dims, loss at
+ * [n][0].  Pair arrays rows x 2. */
+int pacmap_grad(const float *y, uint64_t n, uint64_t dims, const uint32_t *pair_nn,
+                uint64_t n_nn, const uint32_t *pair_mn, uint64_t n_mn, const uint32_t *pair_fp,
+                uint64_t n_fp, float w_nn, float w_mn, float w_fp, float *grad_out,
+                pacmap_status *st);
+
+/* Seam: update_embedding_adam (adam.rs:35-60), in place on y, m, v (n x dims);
+ * grad has at least n rows. */
+int pacmap_adam_step(float *y, const float *grad, float *m, float *v, uint64_t n, uint64_t dims,
+                     float beta1, float beta2, float lr, uint64_t itr, pacmap_status *st);
+
+/* Seam: find_weights (weights.rs:52-81); out = {w_nn, w_mn, w_fp}.  Host only. */
+void pacmap_weights(float w_mn_init, uint64_t itr, uint64_t phase_1_iters,
+                    uint64_t phase_2_iters, float out[3]);
+
+/* Seam: sigma and scale_dist (neighbors.rs:77-87, distance.rs:27-37) plus the
+ * NN pair pick of sample_neighbors_pair (sampling.rs:251-286, ties by column).
+ * knn_dist / knn_idx: n x kprime.  pairs_out (may be NULL): (n*n_neighbors) x 2. */
+int pacmap_sigma_scale(const float *knn_dist, const uint32_t *knn_idx, uint64_t n,
+                       uint64_t kprime, uint64_t n_neighbors, float *sigma_out,
+                       float *scaled_out, uint32_t *pairs_out, pacmap_status *st);
+
+/* Seam: the optimiser loop of pacmap() (lib.rs:781-837) from an already
+ * initialised Y (no Value renormalisation), as one hipGraph.  y: n x dims in/out. */
+int pacmap_optimize(float *y, uint64_t n, uint64_t dims, const uint32_t *pair_nn,
+                    uint64_t n_nn, const uint32_t *pair_mn, uint64_t n_mn,
+                    const uint32_t *pair_fp, uint64_t n_fp, float lr,
+                    const uint64_t num_iters[3], const uint64_t *snapshots,
+                    uint64_t num_snapshots, float *snapshots_out, float *loss_out,
+                    pacmap_status *st);
+
+/* Device buffers for callers that keep inputs resident in HBM. */
+int pacmap_dev_alloc(uint64_t bytes, void **ptr_out, pacmap_status *st);
+int pacmap_dev_free(void *ptr);
+int pacmap_dev_upload(void *dst_dev, const void *src_host, uint64_t bytes, pacmap_status *st);
+int pacmap_dev_download(void *dst_host, const void *src_dev, uint64_t bytes, pacmap_status *st);
+
+/* Per-thread timings of the last call on this thread (HIP events on the call's
+ * stream, milliseconds; host phases by steady clock). */
+typedef struct {
+  double total_ms;
+  double upload_ms;
+  double preprocess_ms;       /* normalise or PCA (covariance, eigensolve, projection) */
+  double pca_cov_ms;          /* covariance kernel alone */
+  double pca_eig_ms;          /* host eigensolve */
+  double knn_ms;              /* kNN kernels */
+  double knn_kernel_ms;       /* the dominant kNN kernel alone */
+  uint64_t knn_kernel_launches;
+  double sigma_ms;            /* sigma/scale/NN-pair kernels */
+  double sampling_ms;         /* host MN/FP sampling */
+  double csr_ms;              /* host CSR build + upload */
+  double graph_build_ms;      /* capture + instantiate */
+  double optimize_ms;         /* the graph of all iterations */
+  uint64_t iterations;
+  uint64_t csr_entries;       /* 2 x non-self pairs */
+  uint64_t knn_rescanned_rows;
+  uint64_t knn_candidates;
+} pacmap_timings;
+void pacmap_last_timings(pacmap_timings *out);
+
+/* Library and device information; returns 0 when a device is usable. */
+int pacmap_device_info(char *name_out, uint64_t name_len, int32_t *cu_count_out);
+const char *pacmap_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

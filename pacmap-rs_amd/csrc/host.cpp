@@ -1,0 +1,784 @@
+// host.cpp -- C ABI of libpacmap_mi355x: fit_transform orchestration, kNN and
+// the per-iteration seams.  Every entry point validates on the host first,
+// then runs its kernels on a stream of its own and blocks until done.
+#include <algorithm>
+#include <cmath>
+#include <memory>
+#include <new>
+#include <random>
+
+#include "host_common.h"
+#include "internal.h"
+
+namespace pm {
+namespace {
+
+thread_local pacmap_timings g_tm;
+
+void set_status(pacmap_status *st, int code, const char *msg) {
+  if (!st) return;
+  std::memset(st, 0, sizeof(*st));
+  st->code = code;
+  if (msg) {
+    std::strncpy(st->message, msg, sizeof(st->message) - 1);
+  }
+}
+
+template <class F>
+int guarded(pacmap_status *st, F &&body) {
+  try {
+    body();
+    set_status(st, PACMAP_OK, nullptr);
+    return PACMAP_OK;
+  } catch (const PmError &e) {
+    if (st) {
+      set_status(st, e.code, e.what());
+      st->gpu_error = e.gpu_error;
+      st->expected[0] = e.expected[0];
+      st->expected[1] = e.expected[1];
+      st->actual[0] = e.actual[0];
+      st->actual[1] = e.actual[1];
+    }
+    return e.code;
+  } catch (const std::bad_alloc &) {
+    set_status(st, PACMAP_ERR_GPU, "host allocation failed");
+    return PACMAP_ERR_GPU;
+  } catch (const std::exception &e) {
+    set_status(st, PACMAP_ERR_INVALID_ARGUMENT, e.what());
+    return PACMAP_ERR_INVALID_ARGUMENT;
+  }
+}
+
+template <class T>
+void upload(DevBuf<T> &dst, const T *src, size_t count, hipStream_t s) {
+  if (count) PM_HIP(hipMemcpyAsync(dst.get(), src, count * sizeof(T), hipMemcpyHostToDevice, s));
+}
+template <class T>
+void download(T *dst, const T *src, size_t count, hipStream_t s) {
+  if (count) PM_HIP(hipMemcpyAsync(dst, src, count * sizeof(T), hipMemcpyDeviceToHost, s));
+}
+
+// ---------------------------------------------------------------------------
+// exact kNN on a device matrix (knn.rs:137-194)
+// ---------------------------------------------------------------------------
+void knn_device(const float *x_dev, uint64_t n, uint64_t d, uint32_t k, DevBuf<uint32_t> &idx,
+                DevBuf<float> &dist, hipStream_t s) {
+  if (k > kKnnMaxK)
+    fail(PACMAP_ERR_UNSUPPORTED, "k = " + std::to_string(k) + " exceeds the kNN kernel limit of " +
+                                     std::to_string(kKnnMaxK));
+  if (n >= (1ull << 31)) fail(PACMAP_ERR_UNSUPPORTED, "too many points for the kNN kernel");
+  if (d >= (1ull << 28)) fail(PACMAP_ERR_UNSUPPORTED, "too many features for the kNN kernel");
+  idx.alloc(n * k);
+  dist.alloc(n * k);
+  if (k == 0) return;
+  const KnnLayout L = knn_layout(static_cast<uint32_t>(d));
+  DevBuf<float> xp(n * L.Dp);
+  Timer all, ker;
+  all.start(s);
+  PM_HIP(launch_knn_pack(x_dev, xp.get(), n, L, s));
+  ker.start(s);
+  PM_HIP(launch_knn_exact(xp.get(), n, L, k, idx.get(), dist.get(), s));
+  ker.stop(s);
+  all.stop(s);
+  g_tm.knn_kernel_ms += ker.ms();
+  g_tm.knn_kernel_launches += 1;
+  g_tm.knn_ms += all.ms();
+  g_tm.knn_candidates = k;
+}
+
+// ---------------------------------------------------------------------------
+// the optimiser (lib.rs:781-837) as one hipGraph of fused iterations
+// ---------------------------------------------------------------------------
+struct GraphGuard {
+  hipGraph_t g = nullptr;
+  hipGraphExec_t e = nullptr;
+  ~GraphGuard() {
+    if (e) (void)hipGraphExecDestroy(e);
+    if (g) (void)hipGraphDestroy(g);
+  }
+};
+
+void optimize_device(const Csr &csr, uint64_t n, uint64_t dims, std::vector<float> &y, float lr,
+                     const uint64_t iters[3], const uint64_t *snap_idx, uint64_t ns,
+                     float *snaps_out, float *loss_out, bool record_loss, hipStream_t s) {
+  const uint64_t T = iters[0] + iters[1] + iters[2];
+  const uint64_t nd = n * dims;
+  // snapshot slots (lib.rs:830-836): slot k is written at the iteration whose
+  // itr+1 equals snapshots[k], when k is the first position with that value;
+  // slot 0 starts as the initial Y (lib.rs:789-791)
+  std::vector<int64_t> writer(T, -1);
+  std::vector<bool> written(ns, false);
+  for (uint64_t sidx = 0; sidx < ns; ++sidx) {
+    const uint64_t v = snap_idx[sidx];
+    if (v >= 1 && v <= T && writer[v - 1] < 0) {
+      writer[v - 1] = static_cast<int64_t>(sidx);
+      written[sidx] = true;
+    }
+  }
+  if (snaps_out && ns) {
+    std::memset(snaps_out, 0, sizeof(float) * ns * nd);
+    std::memcpy(snaps_out, y.data(), sizeof(float) * nd);
+  }
+  const double tcsr = now_ms();
+  DevBuf<uint32_t> d_rp(n + 1), d_ent(std::max<size_t>(1, csr.ent.size()));
+  upload(d_rp, csr.row_ptr.data(), n + 1, s);
+  upload(d_ent, csr.ent.data(), csr.ent.size(), s);
+  DevBuf<float> ybuf(2 * nd), m(nd), v(nd);
+  upload(ybuf, y.data(), nd, s);
+  PM_HIP(launch_fill_zero(m.get(), m.bytes(), s));
+  PM_HIP(launch_fill_zero(v.get(), v.bytes(), s));
+  DevBuf<float> d_snap(ns && snaps_out ? ns * nd : 0);
+  const uint32_t nb = grad_blocks(static_cast<uint32_t>(n));
+  DevBuf<float> part(record_loss ? T * nb : 0), d_loss(record_loss ? T : 0);
+  PM_HIP(hipStreamSynchronize(s));
+  g_tm.csr_ms += now_ms() - tcsr;
+  g_tm.csr_entries = csr.ent.size();
+  g_tm.iterations = T;
+  if (T == 0) return;
+
+  const float beta1 = 0.9f, beta2 = 0.999f, w_mn_init = 1000.0f;  // lib.rs:782-784
+  GraphGuard gg;
+  const double tg = now_ms();
+  PM_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+  hipError_t cap_err = hipSuccess;
+  for (uint64_t it = 0; it < T && cap_err == hipSuccess; ++it) {
+    GradArgs a;
+    a.row_ptr = d_rp.get();
+    a.ent = d_ent.get();
+    a.y_in = ybuf.get() + (it % 2) * nd;
+    a.y_out = ybuf.get() + ((it + 1) % 2) * nd;
+    a.m = m.get();
+    a.v = v.get();
+    a.n = static_cast<uint32_t>(n);
+    find_weights(w_mn_init, it, iters[0], iters[1], a.w);
+    a.lr_t = adam_lr_t(lr, beta1, beta2, it);
+    a.omb1 = 1.0f - beta1;
+    a.omb2 = 1.0f - beta2;
+    a.loss_part = record_loss ? part.get() + it * nb : nullptr;
+    a.snap = (writer[it] >= 0 && d_snap.get()) ? d_snap.get() + writer[it] * nd : nullptr;
+    cap_err = launch_grad_adam(a, static_cast<uint32_t>(dims), true, s);
+  }
+  const hipError_t end_err = hipStreamEndCapture(s, &gg.g);
+  if (cap_err != hipSuccess) fail_gpu(cap_err, "grad_adam launch during capture", __FILE__, __LINE__);
+  if (end_err != hipSuccess) fail_gpu(end_err, "hipStreamEndCapture", __FILE__, __LINE__);
+  PM_HIP(hipGraphInstantiate(&gg.e, gg.g, nullptr, nullptr, 0));
+  g_tm.graph_build_ms += now_ms() - tg;
+  Timer opt;
+  opt.start(s);
+  PM_HIP(hipGraphLaunch(gg.e, s));
+  opt.stop(s);
+  if (record_loss) PM_HIP(launch_loss_reduce(part.get(), nb, static_cast<uint32_t>(T), d_loss.get(), s));
+  download(y.data(), ybuf.get() + (T % 2) * nd, nd, s);
+  std::vector<float> loss(record_loss ? T : 0);
+  if (record_loss) download(loss.data(), d_loss.get(), T, s);
+  PM_HIP(hipStreamSynchronize(s));
+  g_tm.optimize_ms += opt.ms();
+  if (loss_out) {
+    if (record_loss)
+      std::memcpy(loss_out, loss.data(), sizeof(float) * T);
+    else
+      std::fill(loss_out, loss_out + T, std::nanf(""));
+  }
+  if (snaps_out && ns) {
+    for (uint64_t sidx = 0; sidx < ns; ++sidx)
+      if (written[sidx]) download(snaps_out + sidx * nd, d_snap.get() + sidx * nd, nd, s);
+    PM_HIP(hipStreamSynchronize(s));
+  }
+}
+
+// ---------------------------------------------------------------------------
+// PCA: top-q principal axes of the rows of x (FP64 covariance on the GPU)
+// ---------------------------------------------------------------------------
+void cholesky_upper_inv(int b, const std::vector<double> &G, std::vector<double> &Rinv) {
+  // G = R^T R with R upper triangular; returns R^{-1} (upper triangular)
+  std::vector<double> R(static_cast<size_t>(b) * b, 0.0);
+  double shift = 0.0;
+  for (int attempt = 0; attempt < 4; ++attempt) {
+    bool ok = true;
+    std::fill(R.begin(), R.end(), 0.0);
+    for (int j = 0; j < b && ok; ++j) {
+      double s = G[static_cast<size_t>(j) * b + j] + shift;
+      for (int k = 0; k < j; ++k) s -= R[static_cast<size_t>(k) * b + j] * R[static_cast<size_t>(k) * b + j];
+      if (!(s > 0.0)) {
+        ok = false;
+        break;
+      }
+      const double rjj = std::sqrt(s);
+      R[static_cast<size_t>(j) * b + j] = rjj;
+      for (int i = j + 1; i < b; ++i) {
+        double t = G[static_cast<size_t>(j) * b + i];
+        for (int k = 0; k < j; ++k) t -= R[static_cast<size_t>(k) * b + j] * R[static_cast<size_t>(k) * b + i];
+        R[static_cast<size_t>(j) * b + i] = t / rjj;
+      }
+    }
+    if (ok) break;
+    double tr = 0.0;
+    for (int j = 0; j < b; ++j) tr += G[static_cast<size_t>(j) * b + j];
+    shift = shift == 0.0 ? 1e-12 * tr : shift * 100.0;
+    if (attempt == 3) fail(PACMAP_ERR_PCA, "subspace orthonormalisation failed");
+  }
+  Rinv.assign(static_cast<size_t>(b) * b, 0.0);
+  for (int j = b - 1; j >= 0; --j) {
+    Rinv[static_cast<size_t>(j) * b + j] = 1.0 / R[static_cast<size_t>(j) * b + j];
+    for (int i = j - 1; i >= 0; --i) {
+      double s = 0.0;
+      for (int k = i + 1; k <= j; ++k) s += R[static_cast<size_t>(i) * b + k] * Rinv[static_cast<size_t>(k) * b + j];
+      Rinv[static_cast<size_t>(i) * b + j] = -s / R[static_cast<size_t>(i) * b + i];
+    }
+  }
+}
+
+// V: d x q (row-major, columns = axes by descending variance); mean: d.
+void pca_axes(const float *x_dev, uint64_t n, uint32_t d, uint32_t q, std::vector<double> &V,
+              std::vector<double> &mean, hipStream_t s) {
+  DevBuf<double> d_mean(d), d_scr(cov_scratch_doubles(n, d)), d_cov(static_cast<size_t>(d) * d);
+  Timer tc;
+  tc.start(s);
+  PM_HIP(launch_colmean_f64(x_dev, n, d, d_mean.get(), d_scr.get(), s));
+  PM_HIP(launch_cov_f64(x_dev, n, d, d_mean.get(), d_scr.get(), d_cov.get(), s));
+  tc.stop(s);
+  mean.resize(d);
+  download(mean.data(), d_mean.get(), d, s);
+  PM_HIP(hipStreamSynchronize(s));
+  g_tm.pca_cov_ms += tc.ms();
+  const double te = now_ms();
+  V.assign(static_cast<size_t>(d) * q, 0.0);
+  if (d <= 192) {
+    std::vector<double> C(static_cast<size_t>(d) * d), w, E;
+    download(C.data(), d_cov.get(), C.size(), s);
+    PM_HIP(hipStreamSynchronize(s));
+    jacobi_eigh(static_cast<int>(d), C, w, E);
+    for (uint32_t r = 0; r < d; ++r)
+      for (uint32_t c = 0; c < q; ++c) V[static_cast<size_t>(r) * q + c] = E[static_cast<size_t>(r) * d + c];
+  } else {
+    // block subspace iteration with CholeskyQR2 re-orthonormalisation, then a
+    // Rayleigh-Ritz step on the b x b projection (b = q + 16)
+    const uint32_t b = std::min<uint32_t>(d, q + 16);
+    std::vector<double> Q(static_cast<size_t>(d) * b);
+    SmallRng rng = SmallRng::seed_from_u64(0x5eedull);
+    for (auto &e : Q) e = rng.next_f64() - 0.5;
+    DevBuf<double> dQ(Q.size()), dZ(Q.size()), dG(static_cast<size_t>(b) * b), dR(static_cast<size_t>(b) * b);
+    upload(dQ, Q.data(), Q.size(), s);
+    std::vector<double> G(static_cast<size_t>(b) * b), Rinv;
+    auto orth = [&](DevBuf<double> &src, DevBuf<double> &dst) {
+      for (int pass = 0; pass < 2; ++pass) {
+        DevBuf<double> &in = pass == 0 ? src : dst;
+        PM_HIP(launch_gemm_f64(b, b, d, in.get(), b, true, in.get(), b, false, dG.get(), b, s));
+        download(G.data(), dG.get(), G.size(), s);
+        PM_HIP(hipStreamSynchronize(s));
+        cholesky_upper_inv(static_cast<int>(b), G, Rinv);
+        upload(dR, Rinv.data(), Rinv.size(), s);
+        DevBuf<double> tmp(static_cast<size_t>(d) * b);
+        PM_HIP(launch_gemm_f64(d, b, b, in.get(), b, false, dR.get(), b, false, tmp.get(), b, s));
+        PM_HIP(hipMemcpyAsync(dst.get(), tmp.get(), tmp.bytes(), hipMemcpyDeviceToDevice, s));
+        PM_HIP(hipStreamSynchronize(s));
+      }
+    };
+    orth(dQ, dQ);
+    for (int it = 0; it < 10; ++it) {
+      PM_HIP(launch_gemm_f64(d, b, d, d_cov.get(), d, false, dQ.get(), b, false, dZ.get(), b, s));
+      orth(dZ, dQ);
+    }
+    PM_HIP(launch_gemm_f64(d, b, d, d_cov.get(), d, false, dQ.get(), b, false, dZ.get(), b, s));
+    PM_HIP(launch_gemm_f64(b, b, d, dQ.get(), b, true, dZ.get(), b, false, dG.get(), b, s));
+    download(G.data(), dG.get(), G.size(), s);
+    download(Q.data(), dQ.get(), Q.size(), s);
+    PM_HIP(hipStreamSynchronize(s));
+    for (uint32_t i = 0; i < b; ++i)  // symmetrise the Ritz matrix
+      for (uint32_t j = i + 1; j < b; ++j) {
+        const double avg = 0.5 * (G[static_cast<size_t>(i) * b + j] + G[static_cast<size_t>(j) * b + i]);
+        G[static_cast<size_t>(i) * b + j] = avg;
+        G[static_cast<size_t>(j) * b + i] = avg;
+      }
+    std::vector<double> w, U;
+    jacobi_eigh(static_cast<int>(b), G, w, U);
+    for (uint32_t r = 0; r < d; ++r)
+      for (uint32_t c = 0; c < q; ++c) {
+        double acc = 0.0;
+        for (uint32_t k = 0; k < b; ++k) acc += Q[static_cast<size_t>(r) * b + k] * U[static_cast<size_t>(k) * b + c];
+        V[static_cast<size_t>(r) * q + c] = acc;
+      }
+    for (uint32_t c = 0; c < q; ++c) {  // sign: largest-magnitude loading positive
+      uint32_t best = 0;
+      double bv = -1.0;
+      for (uint32_t r = 0; r < d; ++r)
+        if (std::fabs(V[static_cast<size_t>(r) * q + c]) > bv) {
+          bv = std::fabs(V[static_cast<size_t>(r) * q + c]);
+          best = r;
+        }
+      if (V[static_cast<size_t>(best) * q + c] < 0)
+        for (uint32_t r = 0; r < d; ++r) V[static_cast<size_t>(r) * q + c] = -V[static_cast<size_t>(r) * q + c];
+    }
+  }
+  g_tm.pca_eig_ms += now_ms() - te;
+}
+
+// out (n x q) = (x - mean) V on the device
+void pca_project(const float *x_dev, uint64_t n, uint32_t d, uint32_t q, const std::vector<double> &V,
+                 const std::vector<double> &mean, DevBuf<float> &out, hipStream_t s) {
+  std::vector<float> Vf(V.begin(), V.end()), mf(mean.begin(), mean.end());
+  DevBuf<float> dV(Vf.size()), dm(mf.size());
+  upload(dV, Vf.data(), Vf.size(), s);
+  upload(dm, mf.data(), mf.size(), s);
+  out.alloc(n * q);
+  PM_HIP(launch_project(x_dev, n, d, dm.get(), dV.get(), q, out.get(), s));
+  PM_HIP(hipStreamSynchronize(s));
+}
+
+// ---------------------------------------------------------------------------
+// Value initialisation (lib.rs:753-766) with ndarray 0.16's reduction orders
+// ---------------------------------------------------------------------------
+void value_init(const float *yin, uint64_t n, uint64_t dims, std::vector<float> &y) {
+  std::vector<float> mean(dims, 0.0f), wm(dims, 0.0f), ssq(dims, 0.0f);
+  for (uint64_t i = 0; i < n; ++i)
+    for (uint64_t d = 0; d < dims; ++d) mean[d] = mean[d] + yin[i * dims + d];
+  for (uint64_t d = 0; d < dims; ++d) mean[d] = mean[d] / static_cast<float>(n);
+  for (uint64_t i = 0; i < n; ++i) {
+    const float count = static_cast<float>(i + 1);
+    for (uint64_t d = 0; d < dims; ++d) {
+      const float x = yin[i * dims + d];
+      const float delta = x - wm[d];
+      wm[d] = wm[d] + delta / count;
+      ssq[d] = std::fma(x - wm[d], delta, ssq[d]);
+    }
+  }
+  for (uint64_t d = 0; d < dims; ++d) ssq[d] = std::sqrt(ssq[d] / static_cast<float>(n));
+  y.resize(n * dims);
+  for (uint64_t i = 0; i < n; ++i)
+    for (uint64_t d = 0; d < dims; ++d) {
+      const float c = yin[i * dims + d] - mean[d];
+      const float sc = c * 0.0001f;
+      y[i * dims + d] = sc / ssq[d];
+    }
+}
+
+// Random(seed) initialisation: N(0, 1) draws (Box-Muller on the SmallRng stream).
+void random_init(uint64_t n, uint64_t dims, bool has_seed, uint64_t seed, std::vector<float> &y) {
+  if (!has_seed) {
+    std::random_device rd;
+    seed = (static_cast<uint64_t>(rd()) << 32) ^ rd();
+  }
+  SmallRng rng = SmallRng::seed_from_u64(seed);
+  y.resize(n * dims);
+  const double two_pi = 6.283185307179586;
+  for (uint64_t e = 0; e < n * dims; e += 2) {
+    const double u1 = 1.0 - rng.next_f64(), u2 = rng.next_f64();
+    const double r = std::sqrt(-2.0 * std::log(u1));
+    y[e] = static_cast<float>(r * std::cos(two_pi * u2));
+    if (e + 1 < n * dims) y[e + 1] = static_cast<float>(r * std::sin(two_pi * u2));
+  }
+}
+
+// ---------------------------------------------------------------------------
+// fit_transform (lib.rs:294-367)
+// ---------------------------------------------------------------------------
+void fit_impl(const float *x_host, const float *x_dev_in, uint64_t n, uint64_t d,
+              const pacmap_config *cfg, float *y_out, float *snaps_out, float *loss_out) {
+  const double t0 = now_ms();
+  if (!cfg) fail(PACMAP_ERR_INVALID_ARGUMENT, "config is NULL");
+  if (n <= 1) fail(PACMAP_ERR_SAMPLE_SIZE, "Sample size must be larger than one");  // lib.rs:300-302
+  if (!y_out) fail(PACMAP_ERR_INVALID_ARGUMENT, "y_out is NULL");
+  if (!x_host && !x_dev_in && d) fail(PACMAP_ERR_INVALID_ARGUMENT, "x is NULL");
+  const uint64_t dims = cfg->embedding_dimensions;
+  if (dims == 0) fail(PACMAP_ERR_INVALID_ARGUMENT, "embedding_dimensions must be at least 1");
+  if (dims > 8) fail(PACMAP_ERR_UNSUPPORTED, "embedding_dimensions above 8 is not supported");
+  if (n >= kMaxPoints) fail(PACMAP_ERR_UNSUPPORTED, "too many points");
+  const int init = cfg->init_kind;
+  if (init < PACMAP_INIT_PCA || init > PACMAP_INIT_RANDOM) fail(PACMAP_ERR_INVALID_ARGUMENT, "unknown initialization");
+  const bool pca_path = d > 100 && init == PACMAP_INIT_PCA;  // lib.rs:456
+  if (!pca_path && d == 0) fail(PACMAP_ERR_EMPTY_ARRAY_MIN_MAX, "Failed to compute min or max of X: the array is empty");
+  if (init == PACMAP_INIT_VALUE &&
+      (!cfg->init_value || cfg->init_value_rows != n || cfg->init_value_cols != dims))
+    fail(PACMAP_ERR_INVALID_ARGUMENT, "Initialization::Value must be n x embedding_dimensions");
+  if (init == PACMAP_INIT_PCA && !pca_path && std::min<uint64_t>(n, dims) < dims)
+    fail(PACMAP_ERR_INVALID_ARGUMENT, "PCA initialisation needs n >= embedding_dimensions");
+  uint64_t counts[3];
+  decide_num_pairs(n, cfg->has_override_neighbors != 0, cfg->override_neighbors, cfg->mid_near_ratio,
+                   cfg->far_pair_ratio, counts);  // lib.rs:333-338
+  const uint64_t k = counts[0], n_mn = counts[1], n_fp = counts[2];
+  const int pk = cfg->pair_kind;
+  if (pk < PACMAP_PAIRS_GENERATE || pk > PACMAP_PAIRS_ALL_PROVIDED) fail(PACMAP_ERR_INVALID_ARGUMENT, "unknown pair configuration");
+  if (pk == PACMAP_PAIRS_NEIGHBORS_PROVIDED) {  // lib.rs:651-658
+    if (cfg->pair_neighbors_rows != n * k || cfg->pair_neighbors_cols != 2) {
+      PmError e(PACMAP_ERR_INVALID_NEAREST_NEIGHBOR_SHAPE, "Invalid shape for nearest neighbor pairs");
+      e.expected[0] = n * k;
+      e.expected[1] = 2;
+      e.actual[0] = cfg->pair_neighbors_rows;
+      e.actual[1] = cfg->pair_neighbors_cols;
+      throw e;
+    }
+  }
+  if (pk != PACMAP_PAIRS_GENERATE && cfg->pair_neighbors_rows && !cfg->pair_neighbors)
+    fail(PACMAP_ERR_INVALID_ARGUMENT, "pair_neighbors is NULL");
+  if (pk == PACMAP_PAIRS_ALL_PROVIDED) {
+    if (cfg->pair_neighbors_cols != 2 && cfg->pair_neighbors_rows)
+      fail(PACMAP_ERR_INVALID_ARGUMENT, "pair arrays must have 2 columns");
+    if ((cfg->pair_mn_rows && !cfg->pair_mn) || (cfg->pair_fp_rows && !cfg->pair_fp))
+      fail(PACMAP_ERR_INVALID_ARGUMENT, "pair array is NULL");
+  }
+  if (cfg->has_snapshots && cfg->num_snapshots && !cfg->snapshots)
+    fail(PACMAP_ERR_INVALID_ARGUMENT, "snapshots is NULL");
+
+  Stream stream;
+  const hipStream_t s = stream.get();
+  DevBuf<float> d_xin;
+  const float *xd = x_dev_in;
+  if (x_host) {
+    Timer tu;
+    tu.start(s);
+    d_xin.alloc(n * d);
+    upload(d_xin, x_host, n * d, s);
+    tu.stop(s);
+    g_tm.upload_ms += tu.ms();
+    xd = d_xin.get();
+  }
+
+  // --- preprocess_x (lib.rs:442-532) ---
+  Timer tp;
+  tp.start(s);
+  DevBuf<float> d_xk;
+  uint64_t dk = d;
+  std::vector<float> y0;
+  bool have_y = false;
+  if (pca_path) {
+    const uint32_t q = static_cast<uint32_t>(std::min<uint64_t>(100, n));  // lib.rs:457
+    if (dims > q) fail(PACMAP_ERR_INVALID_ARGUMENT, "embedding_dimensions exceeds the PCA width");
+    std::vector<double> V, mean;
+    pca_axes(xd, n, static_cast<uint32_t>(d), q, V, mean, s);
+    pca_project(xd, n, static_cast<uint32_t>(d), q, V, mean, d_xk, s);
+    dk = q;
+  } else {
+    d_xk.alloc(n * d);
+    DevBuf<uint32_t> keys(2);
+    DevBuf<float> mean(d);
+    PM_HIP(launch_minmax(xd, n * d, keys.get(), s));
+    PM_HIP(launch_normalize(xd, d_xk.get(), n * d, keys.get(), s));
+    PM_HIP(launch_colmean_seq(d_xk.get(), n, static_cast<uint32_t>(d), mean.get(), s));
+    PM_HIP(launch_sub_mean(d_xk.get(), n, static_cast<uint32_t>(d), mean.get(), s));
+    if (init == PACMAP_INIT_PCA) {  // lib.rs:513-519 then lib.rs:768
+      const uint32_t nc = static_cast<uint32_t>(std::min<uint64_t>(n, dims));
+      if (nc > d) fail(PACMAP_ERR_PCA, "PCA needs at least as many features as components");
+      std::vector<double> V, pm_mean;
+      pca_axes(d_xk.get(), n, static_cast<uint32_t>(d), nc, V, pm_mean, s);
+      DevBuf<float> proj;
+      pca_project(d_xk.get(), n, static_cast<uint32_t>(d), nc, V, pm_mean, proj, s);
+      y0.resize(n * dims);
+      download(y0.data(), proj.get(), n * dims, s);
+      PM_HIP(hipStreamSynchronize(s));
+      for (auto &e : y0) e = e * 0.01f;
+      have_y = true;
+    }
+  }
+  tp.stop(s);
+  g_tm.preprocess_ms += tp.ms();
+
+  // --- sample_pairs (lib.rs:629-692) ---
+  std::vector<uint32_t> nn_pairs, mn_pairs, fp_pairs;
+  PairList lists[3];
+  std::vector<float> xk_host;
+  const uint64_t *seed = cfg->has_seed ? &cfg->seed : nullptr;
+  auto fetch_xk = [&]() {
+    xk_host.resize(n * dk);
+    download(xk_host.data(), d_xk.get(), n * dk, s);
+    PM_HIP(hipStreamSynchronize(s));
+  };
+  if (pk == PACMAP_PAIRS_GENERATE) {  // neighbors.rs:56-108
+    const uint32_t kp = static_cast<uint32_t>(std::min<uint64_t>(k + 50, n - 1));  // neighbors.rs:67
+    const uint32_t knn = static_cast<uint32_t>(std::min<uint64_t>(k, n - 1));      // neighbors.rs:68
+    // n >= approx_threshold selects USearch in the reference; the exact
+    // neighbours computed here are a valid answer for that path too
+    DevBuf<uint32_t> idx;
+    DevBuf<float> dist;
+    knn_device(d_xk.get(), n, dk, kp, idx, dist, s);
+    Timer tsg;
+    tsg.start(s);
+    DevBuf<float> sig(n), scaled(n * kp);
+    DevBuf<uint32_t> d_nn(n * knn * 2);
+    PM_HIP(launch_sigma(dist.get(), n, kp, sig.get(), s));
+    PM_HIP(launch_scale_nn(dist.get(), idx.get(), sig.get(), n, kp, knn, scaled.get(), d_nn.get(), s));
+    tsg.stop(s);
+    nn_pairs.resize(n * knn * 2);
+    download(nn_pairs.data(), d_nn.get(), nn_pairs.size(), s);
+    fetch_xk();
+    g_tm.sigma_ms += tsg.ms();
+    const double ts = now_ms();
+    mn_pairs.resize(n * n_mn * 2);
+    fp_pairs.resize(n * n_fp * 2);
+    sample_mn_pairs(xk_host.data(), n, dk, n_mn, seed, mn_pairs.data());
+    sample_fp_pairs(n, nn_pairs.data(), knn, n_fp, seed, fp_pairs.data());
+    g_tm.sampling_ms += now_ms() - ts;
+    lists[0] = {nn_pairs.data(), n * knn};
+    lists[1] = {mn_pairs.data(), n * n_mn};
+    lists[2] = {fp_pairs.data(), n * n_fp};
+  } else if (pk == PACMAP_PAIRS_NEIGHBORS_PROVIDED) {  // lib.rs:651-676
+    fetch_xk();
+    const double ts = now_ms();
+    for (uint64_t r = 0; r < 2 * n * k; ++r)
+      if (cfg->pair_neighbors[r] >= n) fail(PACMAP_ERR_INVALID_PAIR_INDEX, "pair_neighbors index out of range");
+    mn_pairs.resize(n * n_mn * 2);
+    fp_pairs.resize(n * n_fp * 2);
+    sample_mn_pairs(xk_host.data(), n, dk, n_mn, seed, mn_pairs.data());
+    sample_fp_pairs(n, cfg->pair_neighbors, k, n_fp, seed, fp_pairs.data());
+    g_tm.sampling_ms += now_ms() - ts;
+    lists[0] = {cfg->pair_neighbors, n * k};
+    lists[1] = {mn_pairs.data(), n * n_mn};
+    lists[2] = {fp_pairs.data(), n * n_fp};
+  } else {  // lib.rs:679-690
+    lists[0] = {cfg->pair_neighbors, cfg->pair_neighbors_rows};
+    lists[1] = {cfg->pair_mn, cfg->pair_mn_rows};
+    lists[2] = {cfg->pair_fp, cfg->pair_fp_rows};
+  }
+  Csr csr;
+  const double tc = now_ms();
+  build_csr(n, lists, csr);
+  g_tm.csr_ms += now_ms() - tc;
+
+  // --- YInit (lib.rs:752-779) ---
+  if (pca_path) {  // YInit::Preprocessed: x[:, :dims] * 0.01 (lib.rs:767)
+    if (xk_host.empty()) fetch_xk();
+    y0.resize(n * dims);
+    for (uint64_t i = 0; i < n; ++i)
+      for (uint64_t c = 0; c < dims; ++c) y0[i * dims + c] = xk_host[i * dk + c] * 0.01f;
+  } else if (!have_y) {
+    if (init == PACMAP_INIT_VALUE)
+      value_init(cfg->init_value, n, dims, y0);
+    else
+      random_init(n, dims, cfg->init_random_has_seed != 0, cfg->init_random_seed, y0);
+  }
+  optimize_device(csr, n, dims, y0, cfg->learning_rate, cfg->num_iters,
+                  cfg->has_snapshots ? cfg->snapshots : nullptr,
+                  cfg->has_snapshots ? cfg->num_snapshots : 0, snaps_out, loss_out,
+                  cfg->record_loss != 0, s);
+  std::memcpy(y_out, y0.data(), sizeof(float) * n * dims);
+  g_tm.total_ms = now_ms() - t0;
+}
+
+}  // namespace
+}  // namespace pm
+
+using namespace pm;
+
+extern "C" {
+
+void pacmap_config_default(pacmap_config *cfg) {
+  if (!cfg) return;
+  std::memset(cfg, 0, sizeof(*cfg));
+  cfg->embedding_dimensions = 2;  // lib.rs:211-221
+  cfg->init_kind = PACMAP_INIT_PCA;
+  cfg->mid_near_ratio = 0.5f;
+  cfg->far_pair_ratio = 2.0f;
+  cfg->pair_kind = PACMAP_PAIRS_GENERATE;
+  cfg->learning_rate = 1.0f;
+  cfg->num_iters[0] = 100;
+  cfg->num_iters[1] = 100;
+  cfg->num_iters[2] = 250;
+  cfg->approx_threshold = 8000;
+  cfg->knn_precision = PACMAP_KNN_FP32;
+  cfg->record_loss = 1;
+}
+
+int pacmap_fit_transform(const float *x, uint64_t n, uint64_t d, const pacmap_config *cfg,
+                         float *y_out, float *snapshots_out, float *loss_out, pacmap_status *st) {
+  g_tm = pacmap_timings{};
+  return guarded(st, [&] { fit_impl(x, nullptr, n, d, cfg, y_out, snapshots_out, loss_out); });
+}
+
+int pacmap_fit_transform_dev(const float *x_dev, uint64_t n, uint64_t d, const pacmap_config *cfg,
+                             float *y_out, float *snapshots_out, float *loss_out, pacmap_status *st) {
+  g_tm = pacmap_timings{};
+  return guarded(st, [&] { fit_impl(nullptr, x_dev, n, d, cfg, y_out, snapshots_out, loss_out); });
+}
+
+int pacmap_knn(const float *x, uint64_t n, uint64_t d, uint64_t k, int32_t precision, uint32_t *idx,
+               float *dist, uint64_t *stats_out, pacmap_status *st) {
+  g_tm = pacmap_timings{};
+  return guarded(st, [&] {
+    if (precision < PACMAP_KNN_FP32 || precision > PACMAP_KNN_FP8)
+      fail(PACMAP_ERR_INVALID_ARGUMENT, "unknown kNN precision");
+    if (n == 0) return;  // knn.rs:141-143
+    if (!x && d) fail(PACMAP_ERR_INVALID_ARGUMENT, "x is NULL");
+    const uint64_t kk = std::min<uint64_t>(k, n - 1);  // knn.rs:146
+    if (kk && (!idx || !dist)) fail(PACMAP_ERR_INVALID_ARGUMENT, "output is NULL");
+    if (kk == 0) return;
+    Stream stream;
+    const hipStream_t s = stream.get();
+    DevBuf<float> dx(n * d);
+    upload(dx, x, n * d, s);
+    DevBuf<uint32_t> di;
+    DevBuf<float> dd;
+    knn_device(dx.get(), n, d, static_cast<uint32_t>(kk), di, dd, s);
+    download(idx, di.get(), n * kk, s);
+    download(dist, dd.get(), n * kk, s);
+    PM_HIP(hipStreamSynchronize(s));
+    if (stats_out) {
+      stats_out[0] = g_tm.knn_rescanned_rows;
+      stats_out[1] = g_tm.knn_candidates;
+    }
+  });
+}
+
+int pacmap_knn_exact(const float *x, uint64_t n, uint64_t d, uint64_t k, uint32_t *idx, float *dist,
+                     pacmap_status *st) {
+  return pacmap_knn(x, n, d, k, PACMAP_KNN_FP32, idx, dist, nullptr, st);
+}
+
+int pacmap_grad(const float *y, uint64_t n, uint64_t dims, const uint32_t *pair_nn, uint64_t n_nn,
+                const uint32_t *pair_mn, uint64_t n_mn, const uint32_t *pair_fp, uint64_t n_fp,
+                float w_nn, float w_mn, float w_fp, float *grad_out, pacmap_status *st) {
+  g_tm = pacmap_timings{};
+  return guarded(st, [&] {
+    if (dims == 0 || dims > 8) fail(PACMAP_ERR_UNSUPPORTED, "dims must be in 1..8");
+    if (n == 0) return;
+    if (!y || !grad_out) fail(PACMAP_ERR_INVALID_ARGUMENT, "NULL array");
+    if ((n_nn && !pair_nn) || (n_mn && !pair_mn) || (n_fp && !pair_fp))
+      fail(PACMAP_ERR_INVALID_ARGUMENT, "NULL pair array");
+    PairList lists[3] = {{pair_nn, n_nn}, {pair_mn, n_mn}, {pair_fp, n_fp}};
+    Csr csr;
+    build_csr(n, lists, csr);
+    Stream stream;
+    const hipStream_t s = stream.get();
+    DevBuf<uint32_t> d_rp(n + 1), d_ent(std::max<size_t>(1, csr.ent.size()));
+    upload(d_rp, csr.row_ptr.data(), n + 1, s);
+    upload(d_ent, csr.ent.data(), csr.ent.size(), s);
+    DevBuf<float> dy(n * dims), dg(n * dims);
+    upload(dy, y, n * dims, s);
+    const uint32_t nb = grad_blocks(static_cast<uint32_t>(n));
+    DevBuf<float> part(nb), loss(1);
+    GradArgs a;
+    a.row_ptr = d_rp.get();
+    a.ent = d_ent.get();
+    a.y_in = dy.get();
+    a.grad_out = dg.get();
+    a.loss_part = part.get();
+    a.n = static_cast<uint32_t>(n);
+    a.w[0] = w_nn;
+    a.w[1] = w_mn;
+    a.w[2] = w_fp;
+    PM_HIP(launch_grad_adam(a, static_cast<uint32_t>(dims), false, s));
+    PM_HIP(launch_loss_reduce(part.get(), nb, 1, loss.get(), s));
+    download(grad_out, dg.get(), n * dims, s);
+    float lv = 0.f;
+    download(&lv, loss.get(), 1, s);
+    PM_HIP(hipStreamSynchronize(s));
+    for (uint64_t c = 0; c < dims; ++c) grad_out[n * dims + c] = 0.0f;
+    grad_out[n * dims] = lv;  // gradient.rs:72
+  });
+}
+
+int pacmap_adam_step(float *y, const float *grad, float *m, float *v, uint64_t n, uint64_t dims,
+                     float beta1, float beta2, float lr, uint64_t itr, pacmap_status *st) {
+  return guarded(st, [&] {
+    const uint64_t count = n * dims;
+    if (count == 0) return;
+    if (!y || !grad || !m || !v) fail(PACMAP_ERR_INVALID_ARGUMENT, "NULL array");
+    Stream stream;
+    const hipStream_t s = stream.get();
+    DevBuf<float> dy(count), dg(count), dm(count), dv(count);
+    upload(dy, y, count, s);
+    upload(dg, grad, count, s);
+    upload(dm, m, count, s);
+    upload(dv, v, count, s);
+    PM_HIP(launch_adam(dy.get(), dg.get(), dm.get(), dv.get(), count, adam_lr_t(lr, beta1, beta2, itr),
+                       1.0f - beta1, 1.0f - beta2, s));
+    download(y, dy.get(), count, s);
+    download(m, dm.get(), count, s);
+    download(v, dv.get(), count, s);
+    PM_HIP(hipStreamSynchronize(s));
+  });
+}
+
+void pacmap_weights(float w_mn_init, uint64_t itr, uint64_t p1, uint64_t p2, float out[3]) {
+  if (out) find_weights(w_mn_init, itr, p1, p2, out);
+}
+
+int pacmap_sigma_scale(const float *knn_dist, const uint32_t *knn_idx, uint64_t n, uint64_t kprime,
+                       uint64_t n_neighbors, float *sigma_out, float *scaled_out, uint32_t *pairs_out,
+                       pacmap_status *st) {
+  return guarded(st, [&] {
+    if (n == 0) return;
+    if (kprime > 512) fail(PACMAP_ERR_UNSUPPORTED, "kprime above 512");
+    if (n_neighbors > kprime) fail(PACMAP_ERR_INVALID_ARGUMENT, "n_neighbors exceeds kprime");
+    if (kprime && (!knn_dist || !knn_idx)) fail(PACMAP_ERR_INVALID_ARGUMENT, "NULL array");
+    for (uint64_t e = 0; e < n * kprime; ++e)
+      if (knn_idx[e] >= n) fail(PACMAP_ERR_INVALID_PAIR_INDEX, "neighbour index out of range");
+    Stream stream;
+    const hipStream_t s = stream.get();
+    DevBuf<float> dd(n * kprime), dsig(n), dsc(n * kprime);
+    DevBuf<uint32_t> di(n * kprime), dp(pairs_out ? n * n_neighbors * 2 : 0);
+    upload(dd, knn_dist, n * kprime, s);
+    upload(di, knn_idx, n * kprime, s);
+    PM_HIP(launch_sigma(dd.get(), n, static_cast<uint32_t>(kprime), dsig.get(), s));
+    PM_HIP(launch_scale_nn(dd.get(), di.get(), dsig.get(), n, static_cast<uint32_t>(kprime),
+                           static_cast<uint32_t>(n_neighbors), dsc.get(), dp.get(), s));
+    if (sigma_out) download(sigma_out, dsig.get(), n, s);
+    if (scaled_out) download(scaled_out, dsc.get(), n * kprime, s);
+    if (pairs_out) download(pairs_out, dp.get(), n * n_neighbors * 2, s);
+    PM_HIP(hipStreamSynchronize(s));
+  });
+}
+
+int pacmap_optimize(float *y, uint64_t n, uint64_t dims, const uint32_t *pair_nn, uint64_t n_nn,
+                    const uint32_t *pair_mn, uint64_t n_mn, const uint32_t *pair_fp, uint64_t n_fp,
+                    float lr, const uint64_t num_iters[3], const uint64_t *snapshots,
+                    uint64_t num_snapshots, float *snapshots_out, float *loss_out, pacmap_status *st) {
+  g_tm = pacmap_timings{};
+  return guarded(st, [&] {
+    if (dims == 0 || dims > 8) fail(PACMAP_ERR_UNSUPPORTED, "dims must be in 1..8");
+    if (!num_iters) fail(PACMAP_ERR_INVALID_ARGUMENT, "num_iters is NULL");
+    if (n == 0) return;
+    if (!y) fail(PACMAP_ERR_INVALID_ARGUMENT, "y is NULL");
+    if ((n_nn && !pair_nn) || (n_mn && !pair_mn) || (n_fp && !pair_fp))
+      fail(PACMAP_ERR_INVALID_ARGUMENT, "NULL pair array");
+    if (num_snapshots && !snapshots) fail(PACMAP_ERR_INVALID_ARGUMENT, "snapshots is NULL");
+    const double t0 = now_ms();
+    PairList lists[3] = {{pair_nn, n_nn}, {pair_mn, n_mn}, {pair_fp, n_fp}};
+    Csr csr;
+    build_csr(n, lists, csr);
+    g_tm.csr_ms += now_ms() - t0;
+    std::vector<float> yv(y, y + n * dims);
+    Stream stream;
+    optimize_device(csr, n, dims, yv, lr, num_iters, snapshots, num_snapshots, snapshots_out, loss_out,
+                    loss_out != nullptr, stream.get());
+    std::memcpy(y, yv.data(), sizeof(float) * n * dims);
+    g_tm.total_ms = now_ms() - t0;
+  });
+}
+
+int pacmap_dev_alloc(uint64_t bytes, void **ptr_out, pacmap_status *st) {
+  return guarded(st, [&] {
+    if (!ptr_out) fail(PACMAP_ERR_INVALID_ARGUMENT, "ptr_out is NULL");
+    *ptr_out = nullptr;
+    PM_HIP(hipMalloc(ptr_out, bytes ? bytes : 1));
+  });
+}
+
+int pacmap_dev_free(void *ptr) { return ptr ? (hipFree(ptr) == hipSuccess ? 0 : PACMAP_ERR_GPU) : 0; }
+
+int pacmap_dev_upload(void *dst_dev, const void *src_host, uint64_t bytes, pacmap_status *st) {
+  return guarded(st, [&] { PM_HIP(hipMemcpy(dst_dev, src_host, bytes, hipMemcpyHostToDevice)); });
+}
+
+int pacmap_dev_download(void *dst_host, const void *src_dev, uint64_t bytes, pacmap_status *st) {
+  return guarded(st, [&] { PM_HIP(hipMemcpy(dst_host, src_dev, bytes, hipMemcpyDeviceToHost)); });
+}
+
+void pacmap_last_timings(pacmap_timings *out) {
+  if (out) *out = g_tm;
+}
+
+int pacmap_device_info(char *name_out, uint64_t name_len, int32_t *cu_count_out) {
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count == 0) return PACMAP_ERR_GPU;
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, 0) != hipSuccess) return PACMAP_ERR_GPU;
+  if (name_out && name_len) {
+    std::strncpy(name_out, prop.gcnArchName, name_len - 1);
+    name_out[name_len - 1] = 0;
+  }
+  if (cu_count_out) *cu_count_out = prop.multiProcessorCount;
+  return 0;
+}
+
+const char *pacmap_version(void) { return "pacmap-mi355x 0.1.0 (gfx950)"; }
+
+}  // extern "C"

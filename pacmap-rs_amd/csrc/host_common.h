@@ -1,0 +1,161 @@
+// host_common.h -- host-side helpers of libpacmap_mi355x (errors, RAII, threads).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstring>
+#include <functional>
+#include <stdexcept>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "../../include/pacmap.h"
+
+namespace pm {
+
+// Internal failure carrying a pacmap_code; converted to pacmap_status at the
+// C boundary (no exception crosses it).
+struct PmError : std::runtime_error {
+  int code;
+  int gpu_error = 0;
+  uint64_t expected[2] = {0, 0};
+  uint64_t actual[2] = {0, 0};
+  PmError(int c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+[[noreturn]] inline void fail(int code, const std::string &msg) { throw PmError(code, msg); }
+
+[[noreturn]] inline void fail_gpu(hipError_t e, const char *what, const char *file, int line) {
+  PmError err(PACMAP_ERR_GPU, std::string(what) + ": " + hipGetErrorString(e) + " (" + file + ":" +
+                                  std::to_string(line) + ")");
+  err.gpu_error = static_cast<int>(e);
+  throw err;
+}
+
+#define PM_HIP(expr)                                                 \
+  do {                                                               \
+    hipError_t pm_e_ = (expr);                                       \
+    if (pm_e_ != hipSuccess) ::pm::fail_gpu(pm_e_, #expr, __FILE__, __LINE__); \
+  } while (0)
+
+template <class T>
+class DevBuf {
+ public:
+  DevBuf() = default;
+  explicit DevBuf(size_t count) { alloc(count); }
+  DevBuf(const DevBuf &) = delete;
+  DevBuf &operator=(const DevBuf &) = delete;
+  ~DevBuf() { release(); }
+  void alloc(size_t count) {
+    release();
+    n_ = count;
+    if (count) PM_HIP(hipMalloc(reinterpret_cast<void **>(&p_), count * sizeof(T)));
+  }
+  void release() {
+    if (p_) (void)hipFree(p_);
+    p_ = nullptr;
+    n_ = 0;
+  }
+  T *get() const { return p_; }
+  size_t size() const { return n_; }
+  size_t bytes() const { return n_ * sizeof(T); }
+
+ private:
+  T *p_ = nullptr;
+  size_t n_ = 0;
+};
+
+class Stream {
+ public:
+  Stream() { PM_HIP(hipStreamCreateWithFlags(&s_, hipStreamNonBlocking)); }
+  ~Stream() {
+    if (s_) (void)hipStreamDestroy(s_);
+  }
+  Stream(const Stream &) = delete;
+  Stream &operator=(const Stream &) = delete;
+  hipStream_t get() const { return s_; }
+  void sync() const { PM_HIP(hipStreamSynchronize(s_)); }
+
+ private:
+  hipStream_t s_ = nullptr;
+};
+
+// Event pair timing a region of one stream.
+class Timer {
+ public:
+  Timer() {
+    PM_HIP(hipEventCreate(&a_));
+    PM_HIP(hipEventCreate(&b_));
+  }
+  ~Timer() {
+    (void)hipEventDestroy(a_);
+    (void)hipEventDestroy(b_);
+  }
+  Timer(const Timer &) = delete;
+  Timer &operator=(const Timer &) = delete;
+  void start(hipStream_t s) { PM_HIP(hipEventRecord(a_, s)); }
+  void stop(hipStream_t s) { PM_HIP(hipEventRecord(b_, s)); }
+  double ms() {
+    PM_HIP(hipEventSynchronize(b_));
+    float v = 0.f;
+    PM_HIP(hipEventElapsedTime(&v, a_, b_));
+    return v;
+  }
+
+ private:
+  hipEvent_t a_ = nullptr, b_ = nullptr;
+};
+
+int host_threads();
+// Runs fn(begin, end) over [0, total) split into contiguous ranges.
+void parallel_for(uint64_t total, const std::function<void(uint64_t, uint64_t)> &fn,
+                  int threads = 0);
+double now_ms();
+
+// --- reference arithmetic restated for the host (product copies) ----------
+// weights.rs:52-81
+void find_weights(float w_mn_init, uint64_t itr, uint64_t p1, uint64_t p2, float out[3]);
+// adam.rs:46-47 (powi lowered to binary square-and-multiply)
+float adam_lr_t(float lr, float beta1, float beta2, uint64_t itr);
+// distance.rs:50-75
+float euclid_ref(const float *a, const float *b, uint64_t d);
+// lib.rs:565-597 (throws on invalid counts)
+void decide_num_pairs(uint64_t n, bool has_override, uint64_t override_k, float mn_ratio,
+                      float fp_ratio, uint64_t out[3]);
+
+// --- rand 0.8.5 SmallRng restatement (sampling.cpp) ------------------------
+struct SmallRng {
+  uint64_t s[4];
+  static SmallRng seed_from_u64(uint64_t state);
+  uint64_t next_u64();
+  uint32_t next_u32() { return static_cast<uint32_t>(next_u64() >> 32); }
+  uint32_t gen_range(uint32_t low, uint32_t high);
+  double next_f64();  // [0, 1) with 53 bits
+};
+
+// sampling.rs:77-103 / 157-195 (seed == nullptr: fresh entropy per point)
+void sample_fp_pairs(uint64_t n, const uint32_t *pair_neighbors, uint64_t n_neighbors,
+                     uint64_t n_fp, const uint64_t *seed, uint32_t *pair_fp);
+// sampling.rs:118-142 / 216-235 + 296-313
+void sample_mn_pairs(const float *x, uint64_t n, uint64_t d, uint64_t n_mn, const uint64_t *seed,
+                     uint32_t *pair_mn);
+
+// Symmetrised CSR of the three pair lists (see internal.h for the entry bits).
+struct PairList {
+  const uint32_t *p = nullptr;
+  uint64_t rows = 0;
+};
+struct Csr {
+  std::vector<uint32_t> row_ptr;
+  std::vector<uint32_t> ent;
+};
+void build_csr(uint64_t n, const PairList lists[3], Csr &out);
+
+// --- linear algebra (linalg.cpp) -------------------------------------------
+// Eigen-decomposition of a symmetric m x m matrix (row-major, destroyed):
+// eigenvalues descending in w, eigenvectors as columns of V (m x m).
+void jacobi_eigh(int m, std::vector<double> &A, std::vector<double> &w, std::vector<double> &V);
+
+}  // namespace pm

@@ -1,0 +1,91 @@
+// internal.h -- launchers shared between the HIP kernels and the host library.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace pm {
+
+// Feature layout for the exact-order distance kernels.  distance.rs:50-75 sums
+// (a-b)^2 into 8 lane accumulators over chunks of 8 features, adds the lanes
+// 0..7 in order, then adds the tail features one by one.  The packed layout
+// puts lane l's features (8c + l, c < C) contiguously in segment l of length
+// Cp >= C (zero padded), then the tail, then zero padding up to Dp.  Zero
+// padding is exact: (0 - 0)^2 = 0 and s + 0 = s.
+struct KnnLayout {
+  uint32_t d = 0;   // input features
+  uint32_t C = 0;   // full 8-chunks (d / 8)
+  uint32_t R = 0;   // tail features (d % 8)
+  uint32_t Cp = 0;  // padded segment length (multiple of 4)
+  uint32_t Dp = 0;  // padded row length (multiple of kKnnChunk)
+};
+constexpr uint32_t kKnnChunk = 16;  // features staged per LDS chunk
+KnnLayout knn_layout(uint32_t d);
+
+// Largest neighbour count the LDS top-k kernels hold per query.
+constexpr uint32_t kKnnMaxK = 448;
+
+hipError_t launch_knn_pack(const float *x, float *xp, uint64_t n, const KnnLayout &L,
+                           hipStream_t s);
+// Exact FP32 kNN in the reference's distance order; k = min(k, n-1) already.
+hipError_t launch_knn_exact(const float *xp, uint64_t n, const KnnLayout &L, uint32_t k,
+                            uint32_t *idx, float *dist, hipStream_t s);
+
+hipError_t launch_sigma(const float *dist, uint64_t n, uint32_t kp, float *sig, hipStream_t s);
+hipError_t launch_scale_nn(const float *dist, const uint32_t *idx, const float *sig, uint64_t n,
+                           uint32_t kp, uint32_t knn, float *scaled, uint32_t *pairs,
+                           hipStream_t s);
+
+// CSR entry: bits 0..28 other index, bit 29 "owner" (the pair's first
+// endpoint, counts the loss once), bits 30..31 pair type (0 NN, 1 MN, 2 FP).
+constexpr uint32_t kIdxMask = (1u << 29) - 1u;
+constexpr uint32_t kOwnerBit = 1u << 29;
+constexpr uint32_t kMaxPoints = 1u << 29;
+
+struct GradArgs {
+  const uint32_t *row_ptr = nullptr;  // n + 1
+  const uint32_t *ent = nullptr;      // row_ptr[n]
+  const float *y_in = nullptr;        // n x dims
+  float *y_out = nullptr;             // n x dims (Adam mode)
+  float *m = nullptr;
+  float *v = nullptr;
+  float *grad_out = nullptr;          // n x dims (gradient mode)
+  float *loss_part = nullptr;         // one float per block (may be null)
+  float *snap = nullptr;              // optional copy of y_out
+  uint32_t n = 0;
+  float w[3] = {0, 0, 0};             // w_nn, w_mn, w_fp
+  float lr_t = 0, omb1 = 0, omb2 = 0; // Adam scalars
+};
+constexpr uint32_t kGradRowsPerBlock = 16;
+uint32_t grad_blocks(uint32_t n);
+// adam == false: write the gradient to grad_out instead of stepping.
+hipError_t launch_grad_adam(const GradArgs &a, uint32_t dims, bool adam, hipStream_t s);
+// loss[t] = sum of part[t * nb .. t * nb + nb) in a fixed order.
+hipError_t launch_loss_reduce(const float *part, uint32_t nb, uint32_t iters, float *loss,
+                              hipStream_t s);
+
+// Preprocessing (lib.rs:486-511)
+hipError_t launch_minmax(const float *x, uint64_t count, uint32_t *keys /*2*/, hipStream_t s);
+hipError_t launch_normalize(const float *x, float *out, uint64_t count, const uint32_t *keys,
+                            hipStream_t s);
+hipError_t launch_colmean_seq(const float *x, uint64_t n, uint32_t d, float *mean, hipStream_t s);
+hipError_t launch_sub_mean(float *x, uint64_t n, uint32_t d, const float *mean, hipStream_t s);
+
+// PCA pieces (FP64 covariance, FP32 projection)
+hipError_t launch_colmean_f64(const float *x, uint64_t n, uint32_t d, double *mean,
+                              double *scratch, hipStream_t s);
+uint64_t cov_scratch_doubles(uint64_t n, uint32_t d);
+hipError_t launch_cov_f64(const float *x, uint64_t n, uint32_t d, const double *mean,
+                          double *scratch, double *cov, hipStream_t s);
+hipError_t launch_project(const float *x, uint64_t n, uint32_t d, const float *mean,
+                          const float *V, uint32_t q, float *out, hipStream_t s);
+
+hipError_t launch_adam(float *y, const float *g, float *m, float *v, uint64_t count, float lr_t,
+                       float omb1, float omb2, hipStream_t s);
+hipError_t launch_fill_zero(void *p, uint64_t bytes, hipStream_t s);
+hipError_t launch_gemm_f64(uint32_t M, uint32_t N, uint32_t K, const double *A, uint32_t lda,
+                           bool tA, const double *B, uint32_t ldb, bool tB, double *C, uint32_t ldc,
+                           hipStream_t s);
+
+}  // namespace pm

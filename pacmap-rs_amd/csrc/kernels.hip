@@ -1,0 +1,945 @@
+// kernels.hip -- hand-written CDNA4 (gfx950) kernels of the PaCMAP hot path.
+//
+// Built with -ffp-contract=off: the reference (Rust) never fuses a multiply
+// and an add, and the per-pair arithmetic here must round exactly like it.
+#include "internal.h"
+
+#include <cstdint>
+
+namespace pm {
+
+// ---------------------------------------------------------------------------
+// helpers
+// ---------------------------------------------------------------------------
+// Monotone map f32 -> u32 whose unsigned order is Rust's f32::total_cmp.
+__device__ __forceinline__ uint32_t tkey(float f) {
+  const uint32_t b = __float_as_uint(f);
+  return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+}
+__device__ __forceinline__ float from_tkey(uint32_t k) {
+  const uint32_t b = (k & 0x80000000u) ? (k & 0x7fffffffu) : ~k;
+  return __uint_as_float(b);
+}
+
+__device__ __forceinline__ uint64_t shfl_xor_u64(uint64_t v, int m) {
+  uint32_t lo = static_cast<uint32_t>(v), hi = static_cast<uint32_t>(v >> 32);
+  lo = __shfl_xor(lo, m, 64);
+  hi = __shfl_xor(hi, m, 64);
+  return (static_cast<uint64_t>(hi) << 32) | lo;
+}
+__device__ __forceinline__ uint64_t shfl_u64(uint64_t v, int src) {
+  uint32_t lo = static_cast<uint32_t>(v), hi = static_cast<uint32_t>(v >> 32);
+  lo = __shfl(lo, src, 64);
+  hi = __shfl(hi, src, 64);
+  return (static_cast<uint64_t>(hi) << 32) | lo;
+}
+
+// Ascending bitonic sort of 64*R keys held by one full wavefront: element
+// r*64 + lane lives in v[r] of `lane`.  Strides below 64 exchange across
+// lanes, larger strides between registers of the same lane.
+template <int R>
+__device__ __forceinline__ void wave_bitonic(uint64_t (&v)[R], int lane) {
+  constexpr int N = 64 * R;
+#pragma unroll
+  for (int size = 2; size <= N; size <<= 1) {
+#pragma unroll
+    for (int stride = size >> 1; stride > 0; stride >>= 1) {
+      if (stride >= 64) {
+        const int rs = stride >> 6;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          if ((r & rs) == 0) {
+            const bool asc = (((r * 64 + lane) & size) == 0);
+            const uint64_t a = v[r], b = v[r | rs];
+            const uint64_t lo = a < b ? a : b, hi = a < b ? b : a;
+            v[r] = asc ? lo : hi;
+            v[r | rs] = asc ? hi : lo;
+          }
+        }
+      } else {
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          const uint64_t o = shfl_xor_u64(v[r], stride);
+          const bool asc = (((r * 64 + lane) & size) == 0);
+          const bool lower = (lane & stride) == 0;
+          const uint64_t lo = v[r] < o ? v[r] : o, hi = v[r] < o ? o : v[r];
+          v[r] = (lower == asc) ? lo : hi;
+        }
+      }
+    }
+  }
+}
+
+template <int R>
+__device__ __forceinline__ uint64_t wave_pick(const uint64_t (&v)[R], uint32_t pos) {
+  uint64_t sel = 0;
+#pragma unroll
+  for (int r = 0; r < R; ++r)
+    if (static_cast<uint32_t>(r) == (pos >> 6)) sel = v[r];
+  return shfl_u64(sel, static_cast<int>(pos & 63u));
+}
+
+// ---------------------------------------------------------------------------
+// exact kNN (knn.rs:137-194 with distance.rs:50-75 arithmetic)
+// ---------------------------------------------------------------------------
+KnnLayout knn_layout(uint32_t d) {
+  KnnLayout L;
+  L.d = d;
+  L.C = d / 8;
+  L.R = d % 8;
+  L.Cp = (L.C + 3u) / 4u * 4u;
+  const uint32_t used = 8u * L.Cp + L.R;
+  L.Dp = (used + kKnnChunk - 1u) / kKnnChunk * kKnnChunk;
+  return L;
+}
+
+__global__ void knn_pack_kernel(const float *__restrict__ x, float *__restrict__ xp, uint64_t n,
+                                uint32_t d, uint32_t C, uint32_t Cp, uint32_t Dp) {
+  const uint64_t total = n * Dp;
+  for (uint64_t e = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; e < total;
+       e += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
+    const uint64_t row = e / Dp;
+    const uint32_t f = static_cast<uint32_t>(e - row * Dp);
+    float v = 0.0f;
+    if (f < 8u * Cp) {
+      const uint32_t l = f / Cp, c = f - l * Cp;
+      if (c < C) v = x[row * d + 8u * c + l];
+    } else {
+      const uint32_t r = f - 8u * Cp;
+      if (8u * C + r < d) v = x[row * d + 8u * C + r];
+    }
+    xp[e] = v;
+  }
+}
+
+hipError_t launch_knn_pack(const float *x, float *xp, uint64_t n, const KnnLayout &L,
+                           hipStream_t s) {
+  const uint64_t total = n * L.Dp;
+  if (total == 0) return hipSuccess;
+  uint64_t blocks = (total + 255) / 256;
+  if (blocks > 65536) blocks = 65536;
+  knn_pack_kernel<<<static_cast<uint32_t>(blocks), 256, 0, s>>>(x, xp, n, L.d, L.C, L.Cp, L.Dp);
+  return hipGetLastError();
+}
+
+// One block owns QB = 16*TQ queries and streams every candidate tile of 64
+// rows.  Each thread computes a TQ x 4 tile of distances with two registers
+// per pair (the running lane accumulator and the running total), so the
+// reference's order -- lane partials over 8-chunks, lanes added 0..7, then
+// the tail -- is reproduced exactly.  Candidates below the query's current
+// k-th key (dist bits, index) are appended to an LDS buffer of KCAP keys; a
+// wavefront bitonic sort compacts it back to k when it could overflow.
+template <int TQ, int KCAP>
+__global__ __launch_bounds__(256) void knn_exact_kernel(const float *__restrict__ xp, uint32_t n,
+                                                        uint32_t Dp, uint32_t Cp, uint32_t k,
+                                                        uint32_t *__restrict__ oidx,
+                                                        float *__restrict__ odist) {
+  constexpr int QB = 16 * TQ, CB = 64, KC = static_cast<int>(kKnnChunk), R = KCAP / 64;
+  __shared__ __align__(16) float sQ[KC][QB];
+  __shared__ __align__(16) float sC[KC][CB];
+  __shared__ uint64_t sBuf[QB][KCAP];
+  __shared__ uint64_t sTau[QB];
+  __shared__ uint32_t sCnt[QB];
+
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const int tq = t >> 4, tc = t & 15;
+  const uint32_t qbase = blockIdx.x * QB;
+  const uint32_t segEnd = 8u * Cp;
+  const uint32_t thresh = KCAP - CB;
+  for (int i = t; i < QB; i += 256) {
+    sCnt[i] = 0;
+    sTau[i] = ~0ull;
+  }
+  __syncthreads();
+
+  for (uint32_t cbase = 0; cbase < n; cbase += CB) {
+    float s[TQ][4], acc[TQ][4];
+#pragma unroll
+    for (int i = 0; i < TQ; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        s[i][j] = 0.0f;
+        acc[i][j] = 0.0f;
+      }
+
+    for (uint32_t f0 = 0; f0 < Dp; f0 += KC) {
+      __syncthreads();
+      for (int li = t; li < QB * (KC / 4); li += 256) {
+        const int q = li / (KC / 4), f4 = li % (KC / 4);
+        const uint32_t row = qbase + q;
+        float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (row < n) v = *reinterpret_cast<const float4 *>(xp + static_cast<size_t>(row) * Dp + f0 + 4 * f4);
+        sQ[4 * f4 + 0][q] = v.x;
+        sQ[4 * f4 + 1][q] = v.y;
+        sQ[4 * f4 + 2][q] = v.z;
+        sQ[4 * f4 + 3][q] = v.w;
+      }
+      for (int li = t; li < CB * (KC / 4); li += 256) {
+        const int c = li / (KC / 4), f4 = li % (KC / 4);
+        const uint32_t row = cbase + c;
+        float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (row < n) v = *reinterpret_cast<const float4 *>(xp + static_cast<size_t>(row) * Dp + f0 + 4 * f4);
+        sC[4 * f4 + 0][c] = v.x;
+        sC[4 * f4 + 1][c] = v.y;
+        sC[4 * f4 + 2][c] = v.z;
+        sC[4 * f4 + 3][c] = v.w;
+      }
+      __syncthreads();
+#pragma unroll
+      for (int sc = 0; sc < KC / 4; ++sc) {
+        const uint32_t fs = f0 + 4u * sc;
+        if (fs >= segEnd) {
+          // tail: distance.rs:69-72 adds each (a-b)^2 to the total directly
+#pragma unroll
+          for (int e = 0; e < 4; ++e) {
+            float qv[TQ];
+#pragma unroll
+            for (int i = 0; i < TQ; ++i) qv[i] = sQ[4 * sc + e][tq * TQ + i];
+            const float4 cv = *reinterpret_cast<const float4 *>(&sC[4 * sc + e][tc * 4]);
+            const float cvs[4] = {cv.x, cv.y, cv.z, cv.w};
+#pragma unroll
+            for (int i = 0; i < TQ; ++i)
+#pragma unroll
+              for (int j = 0; j < 4; ++j) {
+                const float df = qv[i] - cvs[j];
+                const float sq = df * df;
+                s[i][j] = s[i][j] + sq;
+              }
+          }
+        } else {
+          // lane segment: distance.rs:61-64 accumulates per lane
+#pragma unroll
+          for (int e = 0; e < 4; ++e) {
+            float qv[TQ];
+#pragma unroll
+            for (int i = 0; i < TQ; ++i) qv[i] = sQ[4 * sc + e][tq * TQ + i];
+            const float4 cv = *reinterpret_cast<const float4 *>(&sC[4 * sc + e][tc * 4]);
+            const float cvs[4] = {cv.x, cv.y, cv.z, cv.w};
+#pragma unroll
+            for (int i = 0; i < TQ; ++i)
+#pragma unroll
+              for (int j = 0; j < 4; ++j) {
+                const float df = qv[i] - cvs[j];
+                const float sq = df * df;
+                acc[i][j] = acc[i][j] + sq;
+              }
+          }
+          if ((fs + 4u) % Cp == 0u) {
+            // end of lane segment: distance.rs:66 adds lanes in order
+#pragma unroll
+            for (int i = 0; i < TQ; ++i)
+#pragma unroll
+              for (int j = 0; j < 4; ++j) {
+                s[i][j] = s[i][j] + acc[i][j];
+                acc[i][j] = 0.0f;
+              }
+          }
+        }
+      }
+    }
+
+    // candidate filter against the query's current k-th key
+#pragma unroll
+    for (int i = 0; i < TQ; ++i) {
+      const int ql = tq * TQ + i;
+      const uint32_t q = qbase + ql;
+      if (q >= n) continue;
+      const uint64_t tau = sTau[ql];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const uint32_t c = cbase + tc * 4 + j;
+        if (c >= n || c == q) continue;
+        const float dd = sqrtf(s[i][j]);  // distance.rs:74
+        const uint64_t key = (static_cast<uint64_t>(tkey(dd)) << 32) | c;
+        if (key < tau) {
+          const uint32_t pos = atomicAdd(&sCnt[ql], 1u);
+          sBuf[ql][pos] = key;
+        }
+      }
+    }
+    __syncthreads();
+    for (int ql = wave; ql < QB; ql += 4) {
+      const uint32_t cnt = sCnt[ql];
+      if (cnt > thresh) {
+        uint64_t v[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          const uint32_t e = r * 64 + lane;
+          v[r] = e < cnt ? sBuf[ql][e] : ~0ull;
+        }
+        wave_bitonic<R>(v, lane);
+        const uint32_t keep = cnt < k ? cnt : k;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          const uint32_t e = r * 64 + lane;
+          if (e < keep) sBuf[ql][e] = v[r];
+        }
+        const uint64_t tau = (cnt >= k) ? wave_pick<R>(v, k - 1) : ~0ull;
+        if (lane == 0) {
+          sCnt[ql] = keep;
+          sTau[ql] = tau;
+        }
+      }
+    }
+    __syncthreads();
+  }
+
+  // final ordering: ascending (distance, index), first k
+  for (int ql = wave; ql < QB; ql += 4) {
+    const uint32_t q = qbase + ql;
+    if (q >= n) continue;
+    const uint32_t cnt = sCnt[ql];
+    uint64_t v[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const uint32_t e = r * 64 + lane;
+      v[r] = e < cnt ? sBuf[ql][e] : ~0ull;
+    }
+    wave_bitonic<R>(v, lane);
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const uint32_t e = r * 64 + lane;
+      if (e < k) {
+        oidx[static_cast<size_t>(q) * k + e] = static_cast<uint32_t>(v[r]);
+        odist[static_cast<size_t>(q) * k + e] = from_tkey(static_cast<uint32_t>(v[r] >> 32));
+      }
+    }
+  }
+}
+
+hipError_t launch_knn_exact(const float *xp, uint64_t n, const KnnLayout &L, uint32_t k,
+                            uint32_t *idx, float *dist, hipStream_t s) {
+  if (n == 0 || k == 0) return hipSuccess;
+  const uint32_t nn = static_cast<uint32_t>(n);
+  if (k <= 64) {
+    knn_exact_kernel<4, 128><<<(nn + 63) / 64, 256, 0, s>>>(xp, nn, L.Dp, L.Cp, k, idx, dist);
+  } else if (k <= 192) {
+    knn_exact_kernel<2, 256><<<(nn + 31) / 32, 256, 0, s>>>(xp, nn, L.Dp, L.Cp, k, idx, dist);
+  } else if (k <= kKnnMaxK) {
+    knn_exact_kernel<1, 512><<<(nn + 15) / 16, 256, 0, s>>>(xp, nn, L.Dp, L.Cp, k, idx, dist);
+  } else {
+    return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// sigma and scaled distances (neighbors.rs:77-87, distance.rs:27-37)
+// ---------------------------------------------------------------------------
+__global__ void sigma_kernel(const float *__restrict__ dist, uint32_t n, uint32_t kp,
+                             uint32_t start, uint32_t end, float *__restrict__ sig) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  if (end <= start) {  // empty lane: mean_axis is None -> 1e-10
+    sig[i] = 1e-10f;
+    return;
+  }
+  float s = 0.0f;  // ndarray sums a short lane sequentially from zero
+  for (uint32_t c = start; c < end; ++c) s = s + dist[static_cast<size_t>(i) * kp + c];
+  const float mean = s / static_cast<float>(end - start);
+  sig[i] = fmaxf(mean, 1e-10f);  // f32::max returns the non-NaN operand
+}
+
+hipError_t launch_sigma(const float *dist, uint64_t n, uint32_t kp, float *sig, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  uint32_t start = kp >= 1 ? kp - 1 : 0;
+  if (start > 3) start = 3;
+  const uint32_t end = kp < 6 ? kp : 6;
+  sigma_kernel<<<static_cast<uint32_t>((n + 255) / 256), 256, 0, s>>>(
+      dist, static_cast<uint32_t>(n), kp, start, end, sig);
+  return hipGetLastError();
+}
+
+// One wavefront per row: scaled distances, then the row's scaled order
+// (ties by column) picks the first knn neighbours (sampling.rs:251-286).
+template <int R>
+__global__ __launch_bounds__(256) void scale_nn_kernel(const float *__restrict__ dist,
+                                                       const uint32_t *__restrict__ idx,
+                                                       const float *__restrict__ sig, uint32_t n,
+                                                       uint32_t kp, uint32_t knn,
+                                                       float *__restrict__ scaled,
+                                                       uint32_t *__restrict__ pairs) {
+  const int lane = threadIdx.x & 63;
+  const uint32_t row = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (row >= n) return;
+  const float si = sig[row];
+  uint64_t v[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const uint32_t c = r * 64 + lane;
+    v[r] = ~0ull;
+    if (c < kp) {
+      const size_t o = static_cast<size_t>(row) * kp + c;
+      const float d = dist[o];
+      const float dd = d * d;
+      const float ss = si * sig[idx[o]];
+      const float sc = dd / ss;
+      scaled[o] = sc;
+      v[r] = (static_cast<uint64_t>(tkey(sc)) << 32) | c;
+    }
+  }
+  if (pairs == nullptr) return;
+  wave_bitonic<R>(v, lane);
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const uint32_t j = r * 64 + lane;
+    if (j < knn) {
+      const uint32_t col = static_cast<uint32_t>(v[r]);
+      const size_t o = (static_cast<size_t>(row) * knn + j) * 2;
+      pairs[o] = row;
+      pairs[o + 1] = idx[static_cast<size_t>(row) * kp + col];
+    }
+  }
+}
+
+hipError_t launch_scale_nn(const float *dist, const uint32_t *idx, const float *sig, uint64_t n,
+                           uint32_t kp, uint32_t knn, float *scaled, uint32_t *pairs,
+                           hipStream_t s) {
+  if (n == 0 || kp == 0) return hipSuccess;
+  const uint32_t blocks = static_cast<uint32_t>((n + 3) / 4), nn = static_cast<uint32_t>(n);
+  if (kp <= 64)
+    scale_nn_kernel<1><<<blocks, 256, 0, s>>>(dist, idx, sig, nn, kp, knn, scaled, pairs);
+  else if (kp <= 128)
+    scale_nn_kernel<2><<<blocks, 256, 0, s>>>(dist, idx, sig, nn, kp, knn, scaled, pairs);
+  else if (kp <= 256)
+    scale_nn_kernel<4><<<blocks, 256, 0, s>>>(dist, idx, sig, nn, kp, knn, scaled, pairs);
+  else if (kp <= 512)
+    scale_nn_kernel<8><<<blocks, 256, 0, s>>>(dist, idx, sig, nn, kp, knn, scaled, pairs);
+  else
+    return hipErrorInvalidValue;
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// fused gradient + Adam over the symmetrised CSR (gradient.rs:98-151,
+// adam.rs:51-59)
+// ---------------------------------------------------------------------------
+// 16 lanes own one row.  Lane l walks entries beg+l, beg+l+16, ... in order
+// and accumulates s * w1(D) * (y_row - y_other) exactly as the reference
+// computes each per-pair term; the 16 partials are combined by a fixed xor
+// butterfly, so the result is bit-identical from run to run (no atomics).
+template <int DIMS, bool ADAM, bool LOSS>
+__global__ __launch_bounds__(256) void grad_adam_kernel(const GradArgs a) {
+  const int t = threadIdx.x, gl = t & 15, grp = t >> 4;
+  const uint32_t row = blockIdx.x * kGradRowsPerBlock + grp;
+  float g[DIMS];
+  float yr[DIMS];
+#pragma unroll
+  for (int d = 0; d < DIMS; ++d) {
+    g[d] = 0.0f;
+    yr[d] = 0.0f;
+  }
+  float loss = 0.0f;
+  if (row < a.n) {
+#pragma unroll
+    for (int d = 0; d < DIMS; ++d) yr[d] = a.y_in[static_cast<size_t>(row) * DIMS + d];
+    const uint32_t beg = a.row_ptr[row], end = a.row_ptr[row + 1];
+    for (uint32_t e = beg + gl; e < end; e += 16) {
+      const uint32_t w = a.ent[e];
+      const uint32_t o = w & kIdxMask;
+      const uint32_t ty = w >> 30;
+      float u[DIMS];
+      float D = 1.0f;  // gradient.rs:121-125
+#pragma unroll
+      for (int d = 0; d < DIMS; ++d) {
+        u[d] = yr[d] - a.y_in[static_cast<size_t>(o) * DIMS + d];
+        const float sq = u[d] * u[d];
+        D = D + sq;
+      }
+      const bool fp = ty == 2u;
+      const float c = fp ? 1.0f : (ty == 0u ? 10.0f : 10000.0f);
+      const float c2 = fp ? 2.0f : (ty == 0u ? 20.0f : 20000.0f);
+      const float wt = fp ? a.w[2] : (ty == 0u ? a.w[0] : a.w[1]);
+      const float cd = c + D;
+      const float w1 = wt * (c2 / (cd * cd));  // gradient.rs:130 / 140
+#pragma unroll
+      for (int d = 0; d < DIMS; ++d) {
+        const float tt = w1 * u[d];
+        g[d] = fp ? g[d] - tt : g[d] + tt;
+      }
+      if (LOSS && (w & kOwnerBit)) {
+        const float num = fp ? 1.0f : D;  // gradient.rs:129 / 139
+        loss = loss + wt * (num / cd);
+      }
+    }
+  }
+#pragma unroll
+  for (int off = 8; off > 0; off >>= 1) {
+#pragma unroll
+    for (int d = 0; d < DIMS; ++d) g[d] = g[d] + __shfl_xor(g[d], off, 64);
+    if (LOSS) loss = loss + __shfl_xor(loss, off, 64);
+  }
+  if (gl == 0 && row < a.n) {
+    const size_t base = static_cast<size_t>(row) * DIMS;
+    if (ADAM) {
+#pragma unroll
+      for (int d = 0; d < DIMS; ++d) {
+        float m = a.m[base + d], v = a.v[base + d];
+        m = m + a.omb1 * (g[d] - m);          // adam.rs:57
+        const float gg = g[d] * g[d];
+        v = v + a.omb2 * (gg - v);            // adam.rs:58
+        const float num = a.lr_t * m;
+        const float den = sqrtf(v) + 1e-7f;
+        const float y = yr[d] - num / den;    // adam.rs:59
+        a.m[base + d] = m;
+        a.v[base + d] = v;
+        a.y_out[base + d] = y;
+        if (a.snap) a.snap[base + d] = y;
+      }
+    } else {
+#pragma unroll
+      for (int d = 0; d < DIMS; ++d) a.grad_out[base + d] = g[d];
+    }
+  }
+  if (LOSS) {
+    __shared__ float sLoss[kGradRowsPerBlock];
+    if (gl == 0) sLoss[grp] = loss;
+    __syncthreads();
+    if (t == 0) {
+      float s = 0.0f;
+      for (uint32_t i = 0; i < kGradRowsPerBlock; ++i) s = s + sLoss[i];
+      a.loss_part[blockIdx.x] = s;
+    }
+  }
+}
+
+uint32_t grad_blocks(uint32_t n) { return (n + kGradRowsPerBlock - 1) / kGradRowsPerBlock; }
+
+template <int DIMS>
+static hipError_t launch_grad_dims(const GradArgs &a, bool adam, hipStream_t s) {
+  const uint32_t blocks = grad_blocks(a.n);
+  if (blocks == 0) return hipSuccess;
+  const bool loss = a.loss_part != nullptr;
+  if (adam) {
+    if (loss)
+      grad_adam_kernel<DIMS, true, true><<<blocks, 256, 0, s>>>(a);
+    else
+      grad_adam_kernel<DIMS, true, false><<<blocks, 256, 0, s>>>(a);
+  } else {
+    if (loss)
+      grad_adam_kernel<DIMS, false, true><<<blocks, 256, 0, s>>>(a);
+    else
+      grad_adam_kernel<DIMS, false, false><<<blocks, 256, 0, s>>>(a);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_grad_adam(const GradArgs &a, uint32_t dims, bool adam, hipStream_t s) {
+  switch (dims) {
+    case 1: return launch_grad_dims<1>(a, adam, s);
+    case 2: return launch_grad_dims<2>(a, adam, s);
+    case 3: return launch_grad_dims<3>(a, adam, s);
+    case 4: return launch_grad_dims<4>(a, adam, s);
+    case 5: return launch_grad_dims<5>(a, adam, s);
+    case 6: return launch_grad_dims<6>(a, adam, s);
+    case 7: return launch_grad_dims<7>(a, adam, s);
+    case 8: return launch_grad_dims<8>(a, adam, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+__global__ __launch_bounds__(256) void loss_reduce_kernel(const float *__restrict__ part,
+                                                          uint32_t nb, float *__restrict__ loss) {
+  __shared__ float sm[256];
+  const float *p = part + static_cast<size_t>(blockIdx.x) * nb;
+  float s = 0.0f;
+  for (uint32_t i = threadIdx.x; i < nb; i += 256) s = s + p[i];
+  sm[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (static_cast<int>(threadIdx.x) < w) sm[threadIdx.x] = sm[threadIdx.x] + sm[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) loss[blockIdx.x] = sm[0];
+}
+
+hipError_t launch_loss_reduce(const float *part, uint32_t nb, uint32_t iters, float *loss,
+                              hipStream_t s) {
+  if (iters == 0) return hipSuccess;
+  loss_reduce_kernel<<<iters, 256, 0, s>>>(part, nb, loss);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// preprocessing (lib.rs:486-511)
+// ---------------------------------------------------------------------------
+__global__ void minmax_init_kernel(uint32_t *keys) {
+  keys[0] = 0xffffffffu;
+  keys[1] = 0u;
+}
+
+__global__ __launch_bounds__(256) void minmax_kernel(const float *__restrict__ x, uint64_t count,
+                                                     uint32_t *keys) {
+  uint32_t lo = 0xffffffffu, hi = 0u;
+  for (uint64_t e = blockIdx.x * 256ull + threadIdx.x; e < count; e += gridDim.x * 256ull) {
+    const uint32_t k = tkey(x[e]);
+    lo = k < lo ? k : lo;
+    hi = k > hi ? k : hi;
+  }
+  __shared__ uint32_t slo[256], shi[256];
+  slo[threadIdx.x] = lo;
+  shi[threadIdx.x] = hi;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (static_cast<int>(threadIdx.x) < w) {
+      slo[threadIdx.x] = min(slo[threadIdx.x], slo[threadIdx.x + w]);
+      shi[threadIdx.x] = max(shi[threadIdx.x], shi[threadIdx.x + w]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    atomicMin(&keys[0], slo[0]);
+    atomicMax(&keys[1], shi[0]);
+  }
+}
+
+hipError_t launch_minmax(const float *x, uint64_t count, uint32_t *keys, hipStream_t s) {
+  minmax_init_kernel<<<1, 1, 0, s>>>(keys);
+  uint64_t blocks = (count + 255) / 256;
+  if (blocks > 4096) blocks = 4096;
+  if (blocks == 0) blocks = 1;
+  minmax_kernel<<<static_cast<uint32_t>(blocks), 256, 0, s>>>(x, count, keys);
+  return hipGetLastError();
+}
+
+__global__ void normalize_kernel(const float *__restrict__ x, float *__restrict__ out,
+                                 uint64_t count, const uint32_t *__restrict__ keys) {
+  const float xmin = from_tkey(keys[0]), xmax = from_tkey(keys[1]);
+  for (uint64_t e = blockIdx.x * 256ull + threadIdx.x; e < count; e += gridDim.x * 256ull) {
+    const float sh = x[e] - xmin;  // lib.rs:500
+    out[e] = sh / xmax;            // lib.rs:503 (divides by the max, not the range)
+  }
+}
+
+hipError_t launch_normalize(const float *x, float *out, uint64_t count, const uint32_t *keys,
+                            hipStream_t s) {
+  uint64_t blocks = (count + 255) / 256;
+  if (blocks > 65536) blocks = 65536;
+  if (blocks == 0) return hipSuccess;
+  normalize_kernel<<<static_cast<uint32_t>(blocks), 256, 0, s>>>(x, out, count, keys);
+  return hipGetLastError();
+}
+
+// mean_axis(Axis(0)) of a row-major array: rows added in order from zero,
+// then divided by n (lib.rs:506-508).  One thread per column keeps that order.
+__global__ void colmean_seq_kernel(const float *__restrict__ x, uint64_t n, uint32_t d,
+                                   float *__restrict__ mean) {
+  const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= d) return;
+  float s = 0.0f;
+  for (uint64_t r = 0; r < n; ++r) s = s + x[r * d + c];
+  mean[c] = s / static_cast<float>(n);
+}
+
+hipError_t launch_colmean_seq(const float *x, uint64_t n, uint32_t d, float *mean, hipStream_t s) {
+  if (d == 0) return hipSuccess;
+  colmean_seq_kernel<<<(d + 63) / 64, 64, 0, s>>>(x, n, d, mean);
+  return hipGetLastError();
+}
+
+__global__ void sub_mean_kernel(float *__restrict__ x, uint64_t count, uint32_t d,
+                                const float *__restrict__ mean) {
+  for (uint64_t e = blockIdx.x * 256ull + threadIdx.x; e < count; e += gridDim.x * 256ull)
+    x[e] = x[e] - mean[e % d];
+}
+
+hipError_t launch_sub_mean(float *x, uint64_t n, uint32_t d, const float *mean, hipStream_t s) {
+  const uint64_t count = n * d;
+  uint64_t blocks = (count + 255) / 256;
+  if (blocks > 65536) blocks = 65536;
+  if (blocks == 0) return hipSuccess;
+  sub_mean_kernel<<<static_cast<uint32_t>(blocks), 256, 0, s>>>(x, count, d, mean);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// PCA initialisation pieces: FP64 column means and covariance, FP32 projection
+// ---------------------------------------------------------------------------
+constexpr uint32_t kCovTile = 64;
+constexpr uint32_t kCovRowsPerSplit = 2048;
+
+static uint32_t cov_splits(uint64_t n) {
+  uint64_t s = (n + kCovRowsPerSplit - 1) / kCovRowsPerSplit;
+  if (s < 1) s = 1;
+  if (s > 64) s = 64;
+  return static_cast<uint32_t>(s);
+}
+
+uint64_t cov_scratch_doubles(uint64_t n, uint32_t d) {
+  const uint64_t s = cov_splits(n);
+  const uint64_t colpart = s * d;
+  return s * static_cast<uint64_t>(d) * d + colpart;
+}
+
+__global__ void colsum_part_kernel(const float *__restrict__ x, uint64_t n, uint32_t d,
+                                   uint32_t splits, double *__restrict__ part) {
+  const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t sp = blockIdx.y;
+  if (c >= d) return;
+  const uint64_t r0 = n * sp / splits, r1 = n * (sp + 1) / splits;
+  double s = 0.0;
+  for (uint64_t r = r0; r < r1; ++r) s += static_cast<double>(x[r * d + c]);
+  part[static_cast<size_t>(sp) * d + c] = s;
+}
+
+__global__ void colmean_fin_kernel(const double *__restrict__ part, uint32_t d, uint32_t splits,
+                                   uint64_t n, double *__restrict__ mean) {
+  const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= d) return;
+  double s = 0.0;
+  for (uint32_t sp = 0; sp < splits; ++sp) s += part[static_cast<size_t>(sp) * d + c];
+  mean[c] = s / static_cast<double>(n);
+}
+
+hipError_t launch_colmean_f64(const float *x, uint64_t n, uint32_t d, double *mean,
+                              double *scratch, hipStream_t s) {
+  const uint32_t splits = cov_splits(n);
+  dim3 g((d + 63) / 64, splits);
+  colsum_part_kernel<<<g, 64, 0, s>>>(x, n, d, splits, scratch);
+  colmean_fin_kernel<<<(d + 63) / 64, 64, 0, s>>>(scratch, d, splits, n, mean);
+  return hipGetLastError();
+}
+
+// Partial covariance of one row split for one 64x64 tile (upper triangle of
+// tiles only).  256 threads each own a 4x4 block of the tile in FP64.
+__global__ __launch_bounds__(256) void cov_part_kernel(const float *__restrict__ x, uint64_t n,
+                                                       uint32_t d, const double *__restrict__ mean,
+                                                       uint32_t splits, double *__restrict__ part) {
+  const uint32_t ti = blockIdx.x, tj = blockIdx.y, sp = blockIdx.z;
+  if (tj < ti) return;
+  constexpr int RB = 16;
+  __shared__ double sA[RB][kCovTile];
+  __shared__ double sB[RB][kCovTile];
+  const int t = threadIdx.x, ty = t >> 4, tx = t & 15;
+  const uint32_t a0 = ti * kCovTile, b0 = tj * kCovTile;
+  const uint64_t r0 = n * sp / splits, r1 = n * (sp + 1) / splits;
+  double acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[i][j] = 0.0;
+  for (uint64_t rb = r0; rb < r1; rb += RB) {
+    __syncthreads();
+    for (int li = t; li < RB * static_cast<int>(kCovTile); li += 256) {
+      const int rr = li / kCovTile, cc = li % kCovTile;
+      const uint64_t r = rb + rr;
+      const uint32_t ca = a0 + cc, cb = b0 + cc;
+      sA[rr][cc] = (r < r1 && ca < d) ? static_cast<double>(x[r * d + ca]) - mean[ca] : 0.0;
+      sB[rr][cc] = (r < r1 && cb < d) ? static_cast<double>(x[r * d + cb]) - mean[cb] : 0.0;
+    }
+    __syncthreads();
+#pragma unroll 4
+    for (int rr = 0; rr < RB; ++rr) {
+      double av[4], bv[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) av[i] = sA[rr][ty * 4 + i];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) bv[j] = sB[rr][tx * 4 + j];
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = fma(av[i], bv[j], acc[i][j]);
+    }
+  }
+  double *P = part + static_cast<size_t>(sp) * d * d;
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const uint32_t a = a0 + ty * 4 + i, b = b0 + tx * 4 + j;
+      if (a < d && b < d) P[static_cast<size_t>(a) * d + b] = acc[i][j];
+    }
+}
+
+__global__ void cov_fin_kernel(const double *__restrict__ part, uint32_t d, uint32_t splits,
+                               double *__restrict__ cov) {
+  const uint64_t e = blockIdx.x * 256ull + threadIdx.x;
+  if (e >= static_cast<uint64_t>(d) * d) return;
+  const uint32_t a = static_cast<uint32_t>(e / d), b = static_cast<uint32_t>(e % d);
+  // only the upper triangle of tiles was written
+  const uint32_t ta = a / kCovTile, tb = b / kCovTile;
+  const uint64_t src = (tb >= ta) ? static_cast<uint64_t>(a) * d + b : static_cast<uint64_t>(b) * d + a;
+  double s = 0.0;
+  for (uint32_t sp = 0; sp < splits; ++sp) s += part[static_cast<size_t>(sp) * d * d + src];
+  cov[e] = s;
+}
+
+hipError_t launch_cov_f64(const float *x, uint64_t n, uint32_t d, const double *mean,
+                          double *scratch, double *cov, hipStream_t s) {
+  const uint32_t splits = cov_splits(n);
+  const uint32_t tiles = (d + kCovTile - 1) / kCovTile;
+  dim3 g(tiles, tiles, splits);
+  cov_part_kernel<<<g, 256, 0, s>>>(x, n, d, mean, splits, scratch);
+  const uint64_t total = static_cast<uint64_t>(d) * d;
+  cov_fin_kernel<<<static_cast<uint32_t>((total + 255) / 256), 256, 0, s>>>(scratch, d, splits, cov);
+  return hipGetLastError();
+}
+
+// out[r][c] = sum_f (x[r][f] - mean[f]) * V[f][c]; 64x64 output tile per block.
+__global__ __launch_bounds__(256) void project_kernel(const float *__restrict__ x, uint64_t n,
+                                                      uint32_t d, const float *__restrict__ mean,
+                                                      const float *__restrict__ V, uint32_t q,
+                                                      float *__restrict__ out) {
+  constexpr int TB = 64, KB = 16;
+  __shared__ float sX[KB][TB + 1];
+  __shared__ float sV[KB][TB];
+  const int t = threadIdx.x, ty = t >> 4, tx = t & 15;
+  const uint64_t r0 = static_cast<uint64_t>(blockIdx.x) * TB;
+  const uint32_t c0 = blockIdx.y * TB;
+  float acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[i][j] = 0.0f;
+  for (uint32_t f0 = 0; f0 < d; f0 += KB) {
+    __syncthreads();
+    for (int li = t; li < TB * KB; li += 256) {
+      const int rr = li / KB, ff = li % KB;
+      const uint64_t r = r0 + rr;
+      const uint32_t f = f0 + ff;
+      sX[ff][rr] = (r < n && f < d) ? x[r * d + f] - mean[f] : 0.0f;
+    }
+    for (int li = t; li < TB * KB; li += 256) {
+      const int ff = li / TB, cc = li % TB;
+      const uint32_t f = f0 + ff, c = c0 + cc;
+      sV[ff][cc] = (f < d && c < q) ? V[static_cast<size_t>(f) * q + c] : 0.0f;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int ff = 0; ff < KB; ++ff) {
+      float xv[4], vv[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) xv[i] = sX[ff][ty * 4 + i];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) vv[j] = sV[ff][tx * 4 + j];
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = fmaf(xv[i], vv[j], acc[i][j]);
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const uint64_t r = r0 + ty * 4 + i;
+      const uint32_t c = c0 + tx * 4 + j;
+      if (r < n && c < q) out[r * q + c] = acc[i][j];
+    }
+}
+
+hipError_t launch_project(const float *x, uint64_t n, uint32_t d, const float *mean,
+                          const float *V, uint32_t q, float *out, hipStream_t s) {
+  if (n == 0 || q == 0) return hipSuccess;
+  dim3 g(static_cast<uint32_t>((n + 63) / 64), (q + 63) / 64);
+  project_kernel<<<g, 256, 0, s>>>(x, n, d, mean, V, q, out);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// small utilities: Adam alone (adam.rs:51-59), zero fill, FP64 GEMM
+// ---------------------------------------------------------------------------
+__global__ void adam_kernel(float *__restrict__ y, const float *__restrict__ g,
+                            float *__restrict__ m, float *__restrict__ v, uint64_t count,
+                            float lr_t, float omb1, float omb2) {
+  for (uint64_t e = blockIdx.x * 256ull + threadIdx.x; e < count; e += gridDim.x * 256ull) {
+    const float gv = g[e];
+    float mm = m[e], vv = v[e];
+    mm = mm + omb1 * (gv - mm);
+    const float gg = gv * gv;
+    vv = vv + omb2 * (gg - vv);
+    const float num = lr_t * mm;
+    const float den = sqrtf(vv) + 1e-7f;
+    y[e] = y[e] - num / den;
+    m[e] = mm;
+    v[e] = vv;
+  }
+}
+
+hipError_t launch_adam(float *y, const float *g, float *m, float *v, uint64_t count, float lr_t,
+                       float omb1, float omb2, hipStream_t s) {
+  uint64_t blocks = (count + 255) / 256;
+  if (blocks > 65536) blocks = 65536;
+  if (blocks == 0) return hipSuccess;
+  adam_kernel<<<static_cast<uint32_t>(blocks), 256, 0, s>>>(y, g, m, v, count, lr_t, omb1, omb2);
+  return hipGetLastError();
+}
+
+__global__ void fill_zero_kernel(uint32_t *__restrict__ p, uint64_t words) {
+  for (uint64_t e = blockIdx.x * 256ull + threadIdx.x; e < words; e += gridDim.x * 256ull) p[e] = 0u;
+}
+
+hipError_t launch_fill_zero(void *p, uint64_t bytes, hipStream_t s) {
+  const uint64_t words = bytes / 4;
+  uint64_t blocks = (words + 255) / 256;
+  if (blocks > 65536) blocks = 65536;
+  if (blocks == 0) return hipSuccess;
+  fill_zero_kernel<<<static_cast<uint32_t>(blocks), 256, 0, s>>>(static_cast<uint32_t *>(p), words);
+  return hipGetLastError();
+}
+
+// C (M x N) = op(A) (M x K) * op(B) (K x N), row-major, FP64.
+__global__ __launch_bounds__(256) void gemm_f64_kernel(uint32_t M, uint32_t N, uint32_t K,
+                                                       const double *__restrict__ A, uint32_t lda,
+                                                       int tA, const double *__restrict__ B,
+                                                       uint32_t ldb, int tB, double *__restrict__ C,
+                                                       uint32_t ldc) {
+  constexpr int TB = 64, KB = 16;
+  __shared__ double sA[KB][TB];
+  __shared__ double sB[KB][TB];
+  const int t = threadIdx.x, ty = t >> 4, tx = t & 15;
+  const uint32_t r0 = blockIdx.x * TB, c0 = blockIdx.y * TB;
+  double acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[i][j] = 0.0;
+  for (uint32_t k0 = 0; k0 < K; k0 += KB) {
+    __syncthreads();
+    for (int li = t; li < TB * KB; li += 256) {
+      const int rr = li / KB, kk = li % KB;
+      const uint32_t r = r0 + rr, k = k0 + kk;
+      double v = 0.0;
+      if (r < M && k < K) v = tA ? A[static_cast<size_t>(k) * lda + r] : A[static_cast<size_t>(r) * lda + k];
+      sA[kk][rr] = v;
+    }
+    for (int li = t; li < TB * KB; li += 256) {
+      const int kk = li / TB, cc = li % TB;
+      const uint32_t k = k0 + kk, c = c0 + cc;
+      double v = 0.0;
+      if (k < K && c < N) v = tB ? B[static_cast<size_t>(c) * ldb + k] : B[static_cast<size_t>(k) * ldb + c];
+      sB[kk][cc] = v;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < KB; ++kk) {
+      double av[4], bv[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) av[i] = sA[kk][ty * 4 + i];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) bv[j] = sB[kk][tx * 4 + j];
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = fma(av[i], bv[j], acc[i][j]);
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const uint32_t r = r0 + ty * 4 + i, c = c0 + tx * 4 + j;
+      if (r < M && c < N) C[static_cast<size_t>(r) * ldc + c] = acc[i][j];
+    }
+}
+
+hipError_t launch_gemm_f64(uint32_t M, uint32_t N, uint32_t K, const double *A, uint32_t lda,
+                           bool tA, const double *B, uint32_t ldb, bool tB, double *C, uint32_t ldc,
+                           hipStream_t s) {
+  if (M == 0 || N == 0) return hipSuccess;
+  dim3 g((M + 63) / 64, (N + 63) / 64);
+  gemm_f64_kernel<<<g, 256, 0, s>>>(M, N, K, A, lda, tA ? 1 : 0, B, ldb, tB ? 1 : 0, C, ldc);
+  return hipGetLastError();
+}
+
+}  // namespace pm

@@ -1,0 +1,337 @@
+"""Parity of the HIP path against the CPU oracle, through the C ABI.
+
+Bars (written in each test):
+* kNN indices and distances: bit-exact to the oracle (the reference's exact
+  algorithm with the 8-lane distance order, ties by neighbour index);
+* sigma, scaled distances and NN pair picks: bit-exact;
+* gradient, Adam, 10-iteration embeddings and losses: 1e-5 relative in FP32
+  (the reference's own chunk reduction order is not fixed, gradient.rs:53-70).
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+import pacmap_mi355x as pm
+
+pytestmark = pytest.mark.gpu
+
+REL = 1e-5  # relative tolerance for floating-point parity (FP32)
+
+
+def assert_rel(got, ref, rel=REL, floor_frac=1e-3):
+    """Norm-wise relative error <= rel and per-element error <= rel * max(|ref|, floor)."""
+    got = np.asarray(got, np.float64)
+    ref = np.asarray(ref, np.float64)
+    assert got.shape == ref.shape
+    if ref.size == 0:
+        return
+    scale = np.abs(ref).max()
+    nerr = np.linalg.norm(got - ref) / max(np.linalg.norm(ref), 1e-30)
+    assert nerr <= rel, f"norm-wise relative error {nerr:.3e} > {rel}"
+    bound = rel * np.maximum(np.abs(ref), floor_frac * scale) + 1e-30
+    worst = (np.abs(got - ref) / bound).max()
+    assert worst <= 1.0, f"element error {worst:.2f}x the bound"
+
+
+def _pairs(rng, n, rows):
+    return rng.integers(0, n, size=(rows, 2)).astype(np.uint32)
+
+
+# ------------------------------------------------------------------- kNN --
+KNN_CASES = [
+    # (n, d, k, kind, seed)
+    (500, 100, 60, "normal", 1),
+    (700, 784, 60, "uniform", 2),
+    (300, 1000, 10, "sym", 3),
+    (400, 3, 60, "normal", 4),
+    (200, 8, 5, "normal", 5),
+    (150, 7, 149, "normal", 6),
+    (333, 17, 100, "normal", 7),
+    (300, 16, 200, "normal", 8),
+    (520, 33, 448, "uniform", 9),
+    (500, 4, 60, "grid", 10),     # many exact distance ties
+    (256, 12, 30, "dups", 11),    # duplicated points: zero distances
+    (1100, 128, 60, "normal", 12),
+]
+
+
+def _data(kind, n, d, seed):
+    rng = np.random.default_rng(seed)
+    if kind == "normal":
+        return rng.normal(size=(n, d)).astype(np.float32)
+    if kind == "uniform":
+        return rng.uniform(0, 1, size=(n, d)).astype(np.float32)
+    if kind == "sym":
+        return rng.uniform(-1, 1, size=(n, d)).astype(np.float32)
+    if kind == "grid":
+        return rng.integers(0, 3, size=(n, d)).astype(np.float32)
+    if kind == "dups":
+        base = rng.normal(size=(n // 4, d)).astype(np.float32)
+        return np.repeat(base, 4, axis=0)[:n].copy()
+    raise ValueError(kind)
+
+
+@pytest.mark.parametrize("n,d,k,kind,seed", KNN_CASES)
+def test_knn_exact_bitwise(n, d, k, kind, seed):
+    x = _data(kind, n, d, seed)
+    print(f"knn case n={n} d={d} k={k} {kind}", flush=True)
+    idx, dist = pm.knn.find_k_nearest_neighbors(x, k)
+    ref_i, ref_d = O.knn_rowwise(x, k, nthreads=8)
+    assert idx.shape == ref_i.shape == (n, min(k, n - 1))
+    assert np.array_equal(dist.view(np.uint32), ref_d.view(np.uint32)), "distance bits differ"
+    assert np.array_equal(idx, ref_i), "neighbour indices differ"
+
+
+def test_knn_matches_triple_sort_reference_algorithm():
+    x = _data("grid", 300, 5, 21)
+    idx, dist = pm.knn.find_k_nearest_neighbors(x, 60)
+    ref_i, ref_d = O.knn_triple_sort(x, 60, nthreads=4)
+    assert np.array_equal(idx, ref_i) and np.array_equal(dist, ref_d)
+
+
+def test_knn_golden_kats(golden):
+    for case in golden("knn.json")["cases"]:
+        if "embeddings" in case:
+            x = np.array(case["embeddings"], np.float32).reshape(len(case["embeddings"]), -1)
+        else:
+            x = np.zeros(case["embeddings_shape"], np.float32)
+        idx, dist = pm.knn.find_k_nearest_neighbors(x, case["k"])
+        assert list(idx.shape) == case["shape"]
+        for point, exp in case.get("expected", []):
+            got = sorted(zip(idx[point].tolist(), dist[point].tolist()))
+            want = sorted((int(i), float(v)) for i, v in exp)
+            assert [g[0] for g in got] == [w[0] for w in want]
+            for (_, gd), (_, wd) in zip(got, want):
+                assert abs(gd - wd) <= max(1e-4, 1e-4 * abs(wd))
+
+
+def test_knn_zero_features_ties_by_index():
+    x = np.zeros((70, 0), np.float32)
+    idx, dist = pm.knn.find_k_nearest_neighbors(x, 5)
+    assert (dist == 0).all()
+    exp = np.array([[j for j in range(70) if j != i][:5] for i in range(70)], np.uint32)
+    assert np.array_equal(idx, exp)
+
+
+# -------------------------------------------------------- sigma / scale --
+def test_sigma_scale_and_nn_pairs_bitwise():
+    x = _data("normal", 600, 20, 31)
+    kp, knn = 60, 10
+    idx, dist = O.knn_rowwise(x, kp, nthreads=8)
+    sig, sc, pairs = pm.sigma_scale(dist, idx, knn)
+    ref_sig = O.sigma(dist)
+    ref_sc = O.scale_dist(dist, ref_sig, idx)
+    ref_pairs = O.sample_neighbors_pair(ref_sc, idx, knn)
+    assert np.array_equal(sig.view(np.uint32), ref_sig.view(np.uint32))
+    assert np.array_equal(sc.view(np.uint32), ref_sc.view(np.uint32))
+    assert np.array_equal(pairs, ref_pairs)
+
+
+def test_scale_dist_golden_on_gpu(golden):
+    c = golden("scale_dist.json")["cases"][1]
+    dist = np.array(c["knn_distances"], np.float32)
+    idx = np.array(c["neighbors"], np.uint32)
+    # the golden provides sigma directly: recover scaled values through the seam
+    # by checking sigma-independent pieces against the oracle restatement
+    sig, sc, _ = pm.sigma_scale(dist, idx, 0)
+    assert np.array_equal(sig, O.sigma(dist))
+    assert np.array_equal(sc, O.scale_dist(dist, O.sigma(dist), idx))
+
+
+# --------------------------------------------------------------- gradient --
+def test_grad_golden(golden):
+    g = golden("gradient.json")
+    out = pm.pacmap_grad(np.array(g["y_test"], np.float32), g["pair_neighbors"], g["pair_mn"],
+                         g["pair_fp"], g["w_neighbors"], g["w_mn"], g["w_fp"])
+    np.testing.assert_allclose(out, np.array(g["grad_python"], np.float32), rtol=0, atol=1e-6)
+
+
+@pytest.mark.parametrize("dims", [2, 3, 1])
+def test_grad_matches_oracle(dims):
+    rng = np.random.default_rng(40 + dims)
+    n = 3000
+    y = rng.normal(size=(n, dims)).astype(np.float32)
+    nn, mn, fp = _pairs(rng, n, 30000), _pairs(rng, n, 15000), _pairs(rng, n, 60000)
+    got = pm.pacmap_grad(y, nn, mn, fp, 2.0, 730.5, 1.0)
+    ref = O.pacmap_grad(y, nn, mn, fp, 2.0, 730.5, 1.0)
+    ref64 = O.pacmap_grad_f64(y.astype(np.float64), nn, mn, fp, 2.0, 730.5, 1.0)
+    assert_rel(got[:-1], ref[:-1])
+    assert abs(got[-1, 0] - ref64[-1, 0]) <= REL * abs(ref64[-1, 0])
+    assert np.abs(got[:-1].astype(np.float64).sum(0)).max() < 1e-3  # zero-sum invariant
+
+
+def test_grad_rejects_out_of_range_pairs():
+    y = np.zeros((10, 2), np.float32)
+    with pytest.raises(pm.InvalidPairIndex):
+        pm.pacmap_grad(y, np.array([[0, 10]]), np.zeros((0, 2)), np.zeros((0, 2)), 1, 1, 1)
+
+
+# ------------------------------------------------------------------- Adam --
+def test_adam_golden(golden):
+    a = golden("adam.json")
+    y = np.array(a["y"], np.float32)
+    m = np.zeros_like(y)
+    v = np.zeros_like(y)
+    pm.update_embedding_adam(y, np.array(a["grad"], np.float32), m, v, a["beta1"], a["beta2"],
+                             a["lr"], int(a["itr"]))
+    for got, key in ((y, "y_expected"), (m, "m_expected"), (v, "v_expected")):
+        np.testing.assert_allclose(got, np.array(a[key], np.float32), rtol=0, atol=a["epsilon"])
+
+
+def test_adam_matches_oracle():
+    rng = np.random.default_rng(50)
+    y = rng.normal(size=(5000, 2)).astype(np.float32)
+    g = rng.normal(size=(5000, 2)).astype(np.float32) * np.float32(1e-3)
+    m = rng.normal(size=(5000, 2)).astype(np.float32) * np.float32(1e-3)
+    v = np.abs(rng.normal(size=(5000, 2))).astype(np.float32) * np.float32(1e-6)
+    y2, m2, v2 = y.copy(), m.copy(), v.copy()
+    pm.update_embedding_adam(y, g, m, v, np.float32(0.9), np.float32(0.999), 1.0, 17)
+    O.update_embedding_adam(y2, g, m2, v2, np.float32(0.9), np.float32(0.999), 1.0, 17)
+    assert_rel(y, y2, rel=1e-6)
+    assert_rel(m, m2, rel=1e-6)
+    assert_rel(v, v2, rel=1e-6)
+
+
+# -------------------------------------------------------------- optimiser --
+def _problem(seed, n=1500):
+    rng = np.random.default_rng(seed)
+    y0 = O.value_init(rng.normal(size=(n, 2)).astype(np.float32))
+    return y0, _pairs(rng, n, n * 10), _pairs(rng, n, n * 10), _pairs(rng, n, n * 20)
+
+
+def test_optimize_10_iterations_matches_oracle():
+    y0, nn, mn, fp = _problem(60)
+    it = (4, 3, 3)
+    y, _, loss = pm.optimize(y0, nn, mn, fp, 1.0, it)
+    ry, _, rloss = O.optimize(y0, nn, mn, fp, 1.0, it)
+    assert_rel(y, ry)
+    assert_rel(loss, rloss)
+
+
+def test_optimize_full_schedule_deterministic():
+    y0, nn, mn, fp = _problem(61, n=2000)
+    a = pm.optimize(y0, nn, mn, fp, 1.0, (100, 100, 250))
+    b = pm.optimize(y0, nn, mn, fp, 1.0, (100, 100, 250))
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[2], b[2])
+
+
+def test_optimize_snapshots_follow_reference_slots():
+    y0, nn, mn, fp = _problem(62, n=500)
+    snaps = [3, 6, 6, 0, 99]
+    y, s, _ = pm.optimize(y0, nn, mn, fp, 1.0, (3, 2, 2), snapshots=snaps)
+    ry, rs, _ = O.optimize(y0, nn, mn, fp, 1.0, (3, 2, 2), snapshots=snaps)
+    assert_rel(s[0], rs[0])
+    assert_rel(s[1], rs[1])
+    assert not s[2].any() and not s[4].any()
+    # slot 3 (value 0) is never written but slot 0 pre-fill applies only to slot 0
+    assert not s[3].any()
+
+
+def test_optimize_snapshot_slot_zero_keeps_initial_state():
+    y0, nn, mn, fp = _problem(63, n=300)
+    _, s, _ = pm.optimize(y0, nn, mn, fp, 1.0, (2, 1, 1), snapshots=[0, 2])
+    assert np.array_equal(s[0], y0)
+
+
+# ---------------------------------------------------------- fit_transform --
+def test_fit_transform_value_allprovided_matches_oracle():
+    rng = np.random.default_rng(70)
+    n, d = 1200, 30
+    x = rng.normal(size=(n, d)).astype(np.float32)
+    v = rng.normal(size=(n, 2)).astype(np.float32)
+    nn, mn, fp = _pairs(rng, n, n * 10), _pairs(rng, n, n * 5), _pairs(rng, n, n * 20)
+    cfg = pm.Configuration(initialization=pm.Initialization.Value(v),
+                           pair_configuration=pm.PairConfiguration.AllProvided(nn, mn, fp),
+                           num_iters=(4, 3, 3))
+    y, snaps, loss = pm.fit_transform(x, cfg, return_loss=True)
+    ry, _, rloss = O.optimize(O.value_init(v), nn, mn, fp, 1.0, (4, 3, 3))
+    assert snaps is None
+    assert_rel(y, ry)
+    assert_rel(loss, rloss)
+
+
+def _oracle_fit_generate(x, k, n_mn, n_fp, seed, v, lr, iters):
+    xn = O.normalize(x)
+    n = x.shape[0]
+    kp = min(k + 50, n - 1)
+    idx, dist = O.knn_rowwise(xn, kp, nthreads=8)
+    sig = O.sigma(dist)
+    sc = O.scale_dist(dist, sig, idx)
+    nn = O.sample_neighbors_pair(sc, idx, min(k, n - 1))
+    mn = O.sample_mn_pair_deterministic(xn, n_mn, seed)
+    fp = O.sample_fp_pair_deterministic(n, nn, min(k, n - 1), n_fp, seed)
+    y, _, loss = O.optimize(O.value_init(v), nn, mn, fp, lr, iters)
+    return y, loss
+
+
+def test_fit_transform_generate_matches_oracle_pipeline():
+    """Generate + seed + Value init: kNN, sigma, NN/MN/FP pairs and 10 iterations."""
+    rng = np.random.default_rng(71)
+    n, d = 900, 40
+    centers = rng.normal(scale=3.0, size=(6, d))
+    x = (centers[rng.integers(0, 6, n)] + rng.normal(size=(n, d))).astype(np.float32)
+    v = rng.normal(size=(n, 2)).astype(np.float32)
+    it = (5, 3, 2)
+    cfg = pm.Configuration(initialization=pm.Initialization.Value(v), seed=42, num_iters=it,
+                           override_neighbors=10, mid_near_ratio=1.0)
+    y, _, loss = pm.fit_transform(x, cfg, return_loss=True)
+    ry, rloss = _oracle_fit_generate(x, 10, 10, 20, 42, v, 1.0, it)
+    assert_rel(y, ry)
+    assert_rel(loss, rloss)
+
+
+def _reference_test_data():
+    """tests.rs:24-37: 1000 x 50 U(-1, 1) plus a per-cluster offset (5 clusters)."""
+    rng = np.random.default_rng(80)
+    x = rng.uniform(-1, 1, size=(1000, 50)).astype(np.float32)
+    x += (np.arange(1000) // 200)[:, None].astype(np.float32)
+    return x
+
+
+def _structure_checks(y, snaps, cfg, n=1000):
+    """tests.rs:79-147."""
+    assert y.shape == (n, cfg.embedding_dimensions)
+    assert np.isfinite(y).all()
+    assert (np.abs(y.mean(0)) < 1.0).all()
+    assert (y.std(0) > 1e-6).all()
+    lab = np.arange(n) // 200
+    dd = np.sqrt(((y[:, None, :].astype(np.float64) - y[None, :, :]) ** 2).sum(-1))
+    iu = np.triu_indices(n, 1)
+    same = (lab[:, None] == lab[None, :])[iu]
+    assert dd[iu][same].mean() < dd[iu][~same].mean()
+    if cfg.snapshots is not None:
+        assert snaps.shape == (len(cfg.snapshots), n, cfg.embedding_dimensions)
+
+
+@pytest.mark.parametrize("which", [0, 1, 2])
+def test_fit_transform_reference_configurations(which):
+    """The three configurations of tests.rs:40-71."""
+    x = _reference_test_data()
+    cfgs = [
+        pm.Configuration(),
+        pm.Configuration(initialization=pm.Initialization.Pca(), override_neighbors=15, seed=42,
+                         snapshots=[100, 200, 300]),
+        pm.Configuration(initialization=pm.Initialization.Random(42), mid_near_ratio=0.3,
+                         far_pair_ratio=1.5, override_neighbors=10, seed=42, learning_rate=0.8,
+                         num_iters=(50, 50, 100)),
+    ]
+    y, snaps = pm.fit_transform(x, cfgs[which])
+    _structure_checks(y, snaps, cfgs[which])
+
+
+def test_fit_transform_pca_path_high_dimensional():
+    """d > 100 with PCA init: randomized-PCA path of lib.rs:456-484 (parity unpinned)."""
+    rng = np.random.default_rng(90)
+    n, d = 3000, 200
+    centers = rng.normal(scale=2.0, size=(8, d))
+    lab = rng.integers(0, 8, n)
+    x = (centers[lab] + rng.normal(size=(n, d))).astype(np.float32)
+    y, _ = pm.fit_transform(x, pm.Configuration(seed=7))
+    assert np.isfinite(y).all() and y.shape == (n, 2)
+    # 10-NN preservation against the input space is well above chance
+    from_in = O.knn_rowwise(x, 10, nthreads=8)[0]
+    from_out = O.knn_rowwise(y, 10, nthreads=8)[0]
+    pres = np.mean([len(set(a) & set(b)) / 10 for a, b in zip(from_in, from_out)])
+    assert pres > 0.2
+    # clusters separate: same-label neighbours dominate in the embedding
+    assert (lab[from_out] == lab[:, None]).mean() > 0.9

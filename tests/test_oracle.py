@@ -1,0 +1,352 @@
+"""Pins the CPU oracle before it is trusted as the parity checker.
+
+Each test checks one oracle function against something the oracle's own
+formula does not restate: the reference's in-source golden vectors
+(tests/golden/*.json, extracted from the reference's #[cfg(test)] modules),
+a closed form, an invariant, an FP64 or brute-force answer, or a planted
+mistake that must be caught.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import oracle as O
+
+
+# ---------------------------------------------------------------- goldens --
+def test_gradient_golden(golden):
+    """gradient.rs:161-216 (Python-PaCMAP values, 1e-6 abs)."""
+    g = golden("gradient.json")
+    y = np.array(g["y_test"], np.float32)
+    out = O.pacmap_grad(y, g["pair_neighbors"], g["pair_mn"], g["pair_fp"], g["w_neighbors"],
+                        g["w_mn"], g["w_fp"])
+    np.testing.assert_allclose(out, np.array(g["grad_python"], np.float32), rtol=0, atol=1e-6)
+
+
+def test_gradient_golden_catches_planted_mistakes(golden):
+    """A swapped weight or a flipped far-pair sign must fail the golden check."""
+    g = golden("gradient.json")
+    y = np.array(g["y_test"], np.float32)
+    exp = np.array(g["grad_python"], np.float32)
+    swapped = O.pacmap_grad(y, g["pair_neighbors"], g["pair_mn"], g["pair_fp"], g["w_mn"],
+                            g["w_neighbors"], g["w_fp"])
+    assert np.abs(swapped - exp).max() > 1e-6
+    # far pairs fed as mid-near pairs (repulsion turned into attraction)
+    as_mn = O.pacmap_grad(y, g["pair_neighbors"], np.concatenate([g["pair_mn"], g["pair_fp"]]),
+                          np.zeros((0, 2)), g["w_neighbors"], g["w_mn"], g["w_fp"])
+    assert np.abs(as_mn - exp).max() > 1e-6
+    # transposed pair arrays (i, j) -> (j, i) change only signs of u: gradient must be invariant
+    flipped = O.pacmap_grad(y, np.array(g["pair_neighbors"])[:, ::-1],
+                            np.array(g["pair_mn"])[:, ::-1], np.array(g["pair_fp"])[:, ::-1],
+                            g["w_neighbors"], g["w_mn"], g["w_fp"])
+    np.testing.assert_allclose(flipped, exp, rtol=0, atol=1e-6)
+
+
+def test_gradient_f64_twin_matches_golden(golden):
+    g = golden("gradient.json")
+    y = np.array(g["y_test"], np.float64)
+    out = O.pacmap_grad_f64(y, g["pair_neighbors"], g["pair_mn"], g["pair_fp"], g["w_neighbors"],
+                            g["w_mn"], g["w_fp"])
+    np.testing.assert_allclose(out, np.array(g["grad_python"]), rtol=0, atol=1e-6)
+
+
+def test_adam_golden(golden):
+    """adam.rs:68-117."""
+    a = golden("adam.json")
+    y = np.array(a["y"], np.float32)
+    m = np.zeros_like(y)
+    v = np.zeros_like(y)
+    O.update_embedding_adam(y, np.array(a["grad"], np.float32), m, v, a["beta1"], a["beta2"],
+                            a["lr"], int(a["itr"]))
+    for got, key in ((y, "y_expected"), (m, "m_expected"), (v, "v_expected")):
+        np.testing.assert_allclose(got, np.array(a[key], np.float32), rtol=0, atol=a["epsilon"])
+
+
+def test_adam_golden_catches_wrong_iteration(golden):
+    a = golden("adam.json")
+    y = np.array(a["y"], np.float32)
+    m = np.zeros_like(y)
+    v = np.zeros_like(y)
+    # bias correction of t = 2 instead of t = 1
+    O.update_embedding_adam(y, np.array(a["grad"], np.float32), m, v, a["beta1"], a["beta2"],
+                            a["lr"], 1)
+    assert np.abs(y - np.array(a["y_expected"])).max() > a["epsilon"]
+
+
+def test_adam_lr_t_closed_form():
+    """lr_t = lr sqrt(1 - b2^t) / (1 - b1^t) against float64 within f32 rounding."""
+    b1, b2 = float(np.float32(0.9)), float(np.float32(0.999))
+    for itr in (0, 1, 2, 3, 7, 49, 99, 449):
+        t = itr + 1
+        exact = 1.0 * math.sqrt(1 - b2 ** t) / (1 - b1 ** t)
+        got = O.adam_lr_t(1.0, np.float32(0.9), np.float32(0.999), itr)
+        # 1 - b2^t cancels: f32 rounding of b2^t (~t ulps) is amplified by ~1/(t 1e-3)
+        assert abs(got - exact) <= 1e-4 * exact
+    # t = 1 has no cancellation error: 1 - b2 and 1 - b1 are exact in f32 (Sterbenz)
+    assert O.adam_lr_t(1.0, np.float32(0.9), np.float32(0.999), 0) == np.float32(
+        np.sqrt(np.float32(1 - np.float32(0.999))) / np.float32(1 - np.float32(0.9)))
+    # powi as square-and-multiply: exact for small integer powers of 2
+    assert O.powi(2.0, 10) == 1024.0
+    assert O.powi(0.5, 3) == 0.125
+
+
+def test_weights_golden(golden):
+    """weights.rs:88-114."""
+    w = golden("weights.json")
+    for c in w["cases"]:
+        out = O.find_weights(w["w_mn_init"], c["itr"], c["p1"], c["p2"])
+        assert out[0] == c["expected"]["w_neighbors"]
+        assert out[1] == c["expected"]["w_mn"]
+        assert out[2] == c["expected"]["w_fp"]
+
+
+def test_scale_dist_golden(golden):
+    """distance.rs:120-190 (2x2 exact, 10x5 at 1e-6)."""
+    for c in golden("scale_dist.json")["cases"]:
+        out = O.scale_dist(np.array(c["knn_distances"], np.float32),
+                           np.array(c["sig"], np.float32), np.array(c["neighbors"], np.uint32))
+        exp = np.array(c["expected"], np.float32)
+        if c["tolerance"] == 0.0:
+            assert np.array_equal(out, exp)
+        else:
+            np.testing.assert_allclose(out, exp, rtol=0, atol=c["tolerance"])
+
+
+def _check_knn_case(case, idx, dist):
+    assert list(idx.shape) == case["shape"] and list(dist.shape) == case["shape"]
+    for point, exp in case.get("expected", []):
+        got = sorted(zip(idx[point].tolist(), dist[point].tolist()))
+        want = sorted((int(i), float(dv)) for i, dv in exp)
+        assert [g[0] for g in got] == [w[0] for w in want]
+        for (_, gd), (_, wd) in zip(got, want):
+            assert abs(gd - wd) <= max(1e-4, 1e-4 * abs(wd))
+
+
+def test_knn_golden_kats(golden):
+    """knn.rs:215-324: both oracle kNN forms reproduce the reference KATs."""
+    for case in golden("knn.json")["cases"]:
+        if "embeddings" in case:
+            x = np.array(case["embeddings"], np.float32).reshape(len(case["embeddings"]), -1)
+        else:
+            x = np.zeros(case["embeddings_shape"], np.float32)
+        for fn in (O.knn_triple_sort, O.knn_rowwise):
+            idx, dist = fn(x, case["k"])
+            _check_knn_case(case, idx, dist)
+            if x.shape[0] > 1 and case["k"] > 0:
+                assert (idx < x.shape[0]).all() and (dist >= 0).all()
+
+
+# ------------------------------------------------- invariants, brute force --
+def _random_pairs(rng, n, rows):
+    return rng.integers(0, n, size=(rows, 2)).astype(np.uint32)
+
+
+def test_gradient_zero_sum_and_f64_twin():
+    rng = np.random.default_rng(1)
+    n = 300
+    y = rng.normal(size=(n, 2)).astype(np.float32)
+    nn, mn, fp = (_random_pairs(rng, n, r) for r in (3000, 1500, 6000))
+    g32 = O.pacmap_grad(y, nn, mn, fp, 2.0, 1000.0, 1.0)
+    g64 = O.pacmap_grad_f64(y, nn, mn, fp, 2.0, 1000.0, 1.0)
+    # each pair adds +t to one row and -t to the other
+    assert np.abs(g64[:-1].sum(0)).max() < 1e-9
+    assert np.abs(g32[:-1].sum(0)).max() < 1e-3
+    scale = np.abs(g64[:-1]).max()
+    np.testing.assert_allclose(g32[:-1], g64[:-1], rtol=0, atol=1e-5 * scale)
+    assert abs(g32[-1, 0] - g64[-1, 0]) < 1e-5 * abs(g64[-1, 0])
+
+
+def test_gradient_chunk_order_is_one_reduction_order():
+    """Across the 131072-pair chunk boundary the result is a valid reordering."""
+    rng = np.random.default_rng(2)
+    n = 1000
+    y = rng.normal(size=(n, 2)).astype(np.float32)
+    fp = _random_pairs(rng, n, 140000)
+    g32 = O.pacmap_grad(y, np.zeros((0, 2)), np.zeros((0, 2)), fp, 1.0, 1.0, 1.0, nthreads=4)
+    g64 = O.pacmap_grad_f64(y, np.zeros((0, 2)), np.zeros((0, 2)), fp, 1.0, 1.0, 1.0)
+    np.testing.assert_allclose(g32, g64, rtol=0, atol=2e-5 * np.abs(g64).max())
+
+
+def test_euclid_properties():
+    """distance.rs:230-268: non-negative, zero on equal, symmetric, near the scalar sum."""
+    rng = np.random.default_rng(3)
+    for length in (0, 1, 7, 8, 9, 31, 100, 255):
+        a = (rng.normal(size=length) * 10).astype(np.float32)
+        b = (rng.normal(size=length) * 10).astype(np.float32)
+        d1 = O.euclid(a, b)
+        assert d1 >= 0.0
+        assert O.euclid(a, a) == 0.0
+        assert O.euclid(b, a) == d1
+        exact = O.euclid_f64(a, b)
+        assert abs(d1 - exact) <= 1e-5 * max(exact, 1.0)
+
+
+def test_euclid_order_is_lane_partials():
+    """The 8-lane order differs from a plain sequential sum on adversarial input."""
+    # lane 0 gets one large term, other lanes many small ones: the two orders round differently
+    a = np.zeros(64, np.float32)
+    a[0::8] = 4096.0
+    a[1::8] = 1e-3
+    a[2::8] = 3e-3
+    b = np.zeros(64, np.float32)
+    d = O.euclid(a, b)
+    lanes = np.zeros(8, np.float32)
+    for c in range(8):
+        for l in range(8):
+            lanes[l] = np.float32(lanes[l] + np.float32(a[8 * c + l] * a[8 * c + l]))
+    s = np.float32(0)
+    for l in range(8):
+        s = np.float32(s + lanes[l])
+    assert d == np.float32(np.sqrt(s))
+
+
+def test_knn_bruteforce_fp64_on_tie_free_data():
+    rng = np.random.default_rng(4)
+    x = rng.normal(size=(200, 13)).astype(np.float32)
+    k = 17
+    idx, dist = O.knn_triple_sort(x, k)
+    d64 = np.sqrt(((x[:, None, :].astype(np.float64) - x[None, :, :]) ** 2).sum(-1))
+    np.fill_diagonal(d64, np.inf)
+    ref = np.argsort(d64, axis=1, kind="stable")[:, :k]
+    assert np.array_equal(idx, ref)
+    np.testing.assert_allclose(dist, np.take_along_axis(d64, ref, 1), rtol=1e-6)
+    assert (np.diff(dist, axis=1) >= 0).all()
+    assert (idx != np.arange(200)[:, None]).all()
+
+
+def test_knn_rowwise_equals_triple_sort_with_ties():
+    rng = np.random.default_rng(5)
+    x = rng.integers(0, 3, size=(150, 4)).astype(np.float32)  # many exact ties
+    for k in (1, 5, 60, 149, 500):
+        a = O.knn_triple_sort(x, k, nthreads=3)
+        b = O.knn_rowwise(x, k, nthreads=2)
+        assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+
+
+def test_sigma_closed_form():
+    d = np.arange(1, 61, dtype=np.float32)[None, :].repeat(3, 0) * np.float32(0.1)
+    sig = O.sigma(d)
+    exp = np.float32((np.float32(d[0, 3] + d[0, 4]) + d[0, 5]) / np.float32(3))
+    assert (sig == exp).all()
+    # short rows: columns start..end follow neighbors.rs:79-80
+    assert O.sigma(d[:, :2])[0] == d[0, 1]
+    assert O.sigma(d[:, :5])[0] == np.float32(np.float32(d[0, 3] + d[0, 4]) / np.float32(2))
+    assert (O.sigma(np.zeros((2, 0), np.float32)) == np.float32(1e-10)).all()
+    assert (O.sigma(np.zeros((2, 6), np.float32)) == np.float32(1e-10)).all()
+
+
+def test_decide_num_pairs_closed_form():
+    assert O.decide_num_pairs(1000) == (0, (10, 5, 20))
+    assert O.decide_num_pairs(10000) == (0, (10, 5, 20))
+    # n = 70000: 10 + 15 (log10 7e4 - 4) = 22.68 -> 23
+    assert O.decide_num_pairs(70000) == (0, (23, 12, 46))
+    assert O.decide_num_pairs(10 ** 6) == (0, (40, 20, 80))
+    assert O.decide_num_pairs(70000, override=10, mn_ratio=1.0) == (0, (10, 10, 20))
+    assert O.decide_num_pairs(100, override=0)[0] == 5
+    assert O.decide_num_pairs(100, fp_ratio=0.0)[0] == 6
+    # round half away from zero: 10 * 0.25 = 2.5 -> 3
+    assert O.decide_num_pairs(100, mn_ratio=0.25)[1][1] == 3
+
+
+# ------------------------------------------------------------- sampling --
+def test_sample_fp_invariants():
+    """sampling.rs:323-343."""
+    rng = O.Rng(0)
+    out = O.sample_fp(5, 10, np.array([2, 4, 6], np.uint32), 0, rng)
+    assert len(out) == 5 and len(set(out.tolist())) == 5
+    assert not set(out.tolist()) & {0, 2, 4, 6}
+    assert (out < 10).all()
+    # fewer available indices than requested
+    out = O.sample_fp(50, 10, np.array([2, 4, 6], np.uint32), 0, rng)
+    assert sorted(out.tolist()) == [1, 3, 5, 7, 8, 9]
+
+
+def test_gen_range_uniform_and_bounded():
+    rng = O.Rng(42)
+    draws = np.array([rng.gen_range(0, 7) for _ in range(7000)])
+    assert draws.min() == 0 and draws.max() == 6
+    counts = np.bincount(draws, minlength=7)
+    assert (np.abs(counts - 1000) < 150).all()
+
+
+def test_fp_pairs_deterministic_invariants():
+    """sampling.rs:345-398."""
+    n, k, n_fp = 100, 10, 5
+    pn = np.stack([np.arange(n * k) // 10, np.arange(n * k) % 10], 1).astype(np.uint32)
+    a = O.sample_fp_pair_deterministic(n, pn, k, n_fp, 42)
+    b = O.sample_fp_pair_deterministic(n, pn, k, n_fp, 42)
+    assert a.shape == (n * n_fp, 2) and np.array_equal(a, b)
+    for i in range(n):
+        rows = a[i * n_fp:(i + 1) * n_fp]
+        assert (rows[:, 0] == i).all()
+        assert not set(rows[:, 1].tolist()) & set(pn[i * k:(i + 1) * k, 1].tolist())
+        assert len(set(rows[:, 1].tolist())) == n_fp
+    assert not np.array_equal(a, O.sample_fp_pair_deterministic(n, pn, k, n_fp, 43))
+
+
+def test_mn_pairs_deterministic_invariants():
+    """sampling.rs:400-421: valid indices, pair[0] != pair[1], determinism."""
+    x = np.random.default_rng(6).uniform(-1, 1, size=(1000, 20)).astype(np.float32)
+    a = O.sample_mn_pair_deterministic(x, 5, 42)
+    assert a.shape == (5000, 2)
+    assert (a < 1000).all() and (a[:, 0] != a[:, 1]).all()
+    assert np.array_equal(a, O.sample_mn_pair_deterministic(x, 5, 42))
+    # each MN target is the 2nd closest of its 6 candidates: never the nearest of all n
+    d = np.sqrt(((x[:, None] - x[None]) ** 2).sum(-1))
+    np.fill_diagonal(d, np.inf)
+    nearest = d.argmin(1)
+    assert (a[:, 1] != nearest[a[:, 0]]).mean() > 0.95
+
+
+def test_neighbors_pair_sorted_by_scaled_distance():
+    """sampling.rs:489-536."""
+    rng = np.random.default_rng(7)
+    scaled = rng.uniform(size=(10, 10)).astype(np.float32)
+    nbrs = np.tile(np.arange(10, dtype=np.uint32), (10, 1))
+    out = O.sample_neighbors_pair(scaled, nbrs, 5)
+    assert out.shape == (50, 2)
+    for i in range(10):
+        assert np.array_equal(out[i * 5:(i + 1) * 5, 1], np.argsort(scaled[i], kind="stable")[:5])
+
+
+# ---------------------------------------------------- init / preprocessing --
+def test_value_init_closed_form():
+    y = np.random.default_rng(8).normal(3.0, 2.0, size=(500, 2)).astype(np.float32)
+    out = O.value_init(y)
+    assert np.abs(out.mean(0)).max() < 1e-9
+    np.testing.assert_allclose(out.std(0), 1e-4, rtol=1e-4)
+    exp = (y - y.astype(np.float64).mean(0)) * 1e-4 / y.astype(np.float64).std(0)
+    np.testing.assert_allclose(out, exp, rtol=0, atol=1e-9)
+
+
+def test_normalize_closed_form():
+    x = np.random.default_rng(9).uniform(-3, 5, size=(300, 7)).astype(np.float32)
+    out = O.normalize(x)
+    xd = x.astype(np.float64)
+    exp = (xd - xd.min()) / xd.max()
+    exp -= exp.mean(0)
+    np.testing.assert_allclose(out, exp, rtol=0, atol=1e-6)
+
+
+def test_optimize_loop_matches_stepwise_oracle():
+    rng = np.random.default_rng(10)
+    n = 200
+    y0 = rng.normal(size=(n, 2)).astype(np.float32) * np.float32(1e-4)
+    nn, mn, fp = (_random_pairs(rng, n, r) for r in (2000, 1000, 4000))
+    y, snaps, loss = O.optimize(y0, nn, mn, fp, 1.0, (3, 2, 2), snapshots=[2, 5, 5, 0, 100])
+    # stepwise restatement of lib.rs:803-837
+    ys = y0.copy()
+    m = np.zeros_like(ys)
+    v = np.zeros_like(ys)
+    states = {}
+    for itr in range(7):
+        w = O.find_weights(1000.0, itr, 3, 2)
+        g = O.pacmap_grad(ys, nn, mn, fp, w[0], w[1], w[2])
+        assert g[-1, 0] == loss[itr]
+        O.update_embedding_adam(ys, g, m, v, np.float32(0.9), np.float32(0.999), 1.0, itr)
+        states[itr + 1] = ys.copy()
+    assert np.array_equal(y, ys)
+    assert np.array_equal(snaps[0], states[2]) and np.array_equal(snaps[1], states[5])
+    assert not snaps[2].any() and not snaps[4].any()  # duplicate and unreachable stay zero
+    assert not snaps[3].any()
