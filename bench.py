@@ -26,6 +26,7 @@ N, D, DPRIME, K_NN, N_MN, N_FP, DIMS = 70000, 784, 100, 10, 10, 20, 2
 ITERS = (100, 100, 250)
 HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3
+BF16_PEAK_TFLOPS = 2500.0
 
 
 def log(msg):
@@ -122,6 +123,14 @@ def main():
     knn_ms = float(np.mean([t["knn_kernel_ms"] for t in tms]))
     flops = 2.0 * N * N * DPRIME
     achieved = flops / (knn_ms * 1e-3) / 1e12
+    path = int(tms[-1]["knn_path"])
+    if path == 0:
+        kname, peak = "knn_exact_kernel (direct FP32 in the reference's distance order)", FP32_PEAK_TFLOPS
+    else:
+        kname = ("knn_mfma_cand_kernel (split-bf16 MFMA candidates: 3 bf16 MFMAs per "
+                 "algorithmic multiply-add, plus a 1/ss sample pass)" if path == 1 else
+                 "knn_mfma_cand_kernel (bf16 MFMA candidates)")
+        peak = BF16_PEAK_TFLOPS
     opt_ms = float(np.mean([t["optimize_ms"] for t in tms]))
     pairs = N * (K_NN + N_MN + N_FP)
     b_iter = grad_adam_bytes(N, pairs, DIMS)
@@ -138,10 +147,9 @@ def main():
                                "k=10 (60 candidates), 10 MN + 20 FP per point, 450 iterations",
                    "n": N, "d": D, "knn_dims": DPRIME, "n_neighbors": K_NN, "n_mn": N_MN,
                    "n_fp": N_FP, "iterations": it_total},
-        "roofline": {"kernel": "knn_exact_kernel (exact FP32, reference distance order)",
-                     "bound": "mfma", "achieved": round(achieved, 3),
-                     "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": round(achieved / FP32_PEAK_TFLOPS, 4), "traffic": None,
+        "roofline": {"kernel": kname, "bound": "mfma", "achieved": round(achieved, 3),
+                     "peak": peak, "unit": "TFLOP/s", "frac": round(achieved / peak, 4),
+                     "traffic": None,
                      "algorithmic": f"2*n^2*d' = {flops:.3e} flop per launch, "
                                     f"{knn_ms:.3f} ms per launch"},
         "grad_adam": {"bytes_per_iter": b_iter, "us_per_iter": round(opt_ms * 1e3 / it_total, 3),
@@ -149,6 +157,12 @@ def main():
                       "frac_hbm_peak": round(b_iter * it_total / (opt_ms * 1e-3) / 1e9 /
                                              HBM_PEAK_GBS, 4)},
         "breakdown_ms": breakdown,
+        "knn": {"path": ["direct", "split-bf16 MFMA + certified rerank",
+                         "bf16 MFMA + certified rerank"][path],
+                "rescanned_rows": int(tms[-1]["knn_rescanned_rows"]),
+                "candidates_capacity": int(tms[-1]["knn_candidates"]),
+                "rerank_ms": round(float(np.mean([t["knn_rerank_ms"] for t in tms])), 3),
+                "fallback_ms": round(float(np.mean([t["knn_fallback_ms"] for t in tms])), 3)},
     }
     if not args.no_cpu_baseline:
         log("bench: timing the CPU oracle on a bounded sample")

@@ -59,9 +59,14 @@ typedef struct {
 enum { PACMAP_INIT_PCA = 0, PACMAP_INIT_VALUE = 1, PACMAP_INIT_RANDOM = 2 };
 /* PairConfiguration (lib.rs:241-267) */
 enum { PACMAP_PAIRS_GENERATE = 0, PACMAP_PAIRS_NEIGHBORS_PROVIDED = 1, PACMAP_PAIRS_ALL_PROVIDED = 2 };
-/* kNN candidate precision (MI355X extension; every mode returns the exact
- * FP32 neighbour sets of knn.rs:137-194, see pacmap_knn_exact) */
-enum { PACMAP_KNN_FP32 = 0, PACMAP_KNN_BF16 = 1, PACMAP_KNN_FP8 = 2 };
+/* kNN path (MI355X extension).  Every mode returns the exact FP32 neighbour
+ * sets of knn.rs:137-194 (see pacmap_knn_exact):
+ *   FP32   (default) split-bf16 MFMA candidates, exact FP32 rerank, per-row
+ *          certificate, exact rescan of uncertified rows;
+ *   BF16   single-bf16 MFMA candidates, same rerank/certificate/rescan;
+ *   FP8    reserved (returns PACMAP_ERR_INVALID_ARGUMENT);
+ *   DIRECT every distance in the reference's order on the vector ALUs. */
+enum { PACMAP_KNN_FP32 = 0, PACMAP_KNN_BF16 = 1, PACMAP_KNN_FP8 = 2, PACMAP_KNN_DIRECT = 3 };
 
 /* Configuration (lib.rs:163-224).  Option<T> fields carry a has_ flag; enum
  * payloads are borrowed pointers that must outlive the call. */
@@ -177,8 +182,11 @@ typedef struct {
   double optimize_ms;         /* the graph of all iterations */
   uint64_t iterations;
   uint64_t csr_entries;       /* 2 x non-self pairs */
-  uint64_t knn_rescanned_rows;
-  uint64_t knn_candidates;
+  uint64_t knn_rescanned_rows; /* rows the certificate sent to the exact rescan */
+  uint64_t knn_candidates;     /* candidate list capacity per row */
+  double knn_rerank_ms;        /* exact rerank + certificate kernel */
+  double knn_fallback_ms;      /* exact rescan of uncertified rows */
+  uint64_t knn_path;           /* 0 direct, 1 split-bf16 MFMA, 2 bf16 MFMA */
 } pacmap_timings;
 void pacmap_last_timings(pacmap_timings *out);
 

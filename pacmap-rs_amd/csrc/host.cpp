@@ -61,8 +61,95 @@ void download(T *dst, const T *src, size_t count, hipStream_t s) {
 // ---------------------------------------------------------------------------
 // exact kNN on a device matrix (knn.rs:137-194)
 // ---------------------------------------------------------------------------
+// Relative error coefficient of the MFMA lower bound lb = a_q + a_j - 2 dot
+// (see knn_mfma.hip): bf16 input rounding (split: hi + lo leaves ~3 u^2,
+// single: 2 u, u = 2^-8), FP32 accumulation of the MFMA (counted as one
+// truncating rounding per product), the rounding of a, of the sums, and of
+// the centring; doubled for safety.
+double knn_kappa(uint32_t kpad, bool split) {
+  const double u = 1.0 / 256.0, e23 = 1.0 / 8388608.0, e24 = e23 / 2;
+  const double dotc = split ? 3.1 * u * u + (3.0 * kpad + 1.0) * e23 * 1.02
+                            : (2.0 * u + u * u) + (kpad + 1.0) * e23 * 1.02;
+  return 2.0 * (dotc + 16.0 * e24 + 4.0 * e24 * 2.0);
+}
+
+// The reference distance of an unlisted candidate is at least
+// sqrt(tau * cert_scale): (1 - gammaA) bounds the 8-lane FP32 summation
+// error, (1 - 2^-24)^2 the final sqrt rounding, 2^-20 is margin.
+double knn_cert_scale(uint32_t d) {
+  const double e24 = 1.0 / 16777216.0;
+  const double gammaA = (d / 8.0 + 16.0) * e24 * 1.1;
+  return (1.0 - gammaA) * (1.0 - e24) * (1.0 - e24) * (1.0 - 1.0 / 1048576.0);
+}
+
+bool knn_mfma_eligible(uint64_t n, uint64_t d, uint32_t k, int precision) {
+  if (precision == PACMAP_KNN_DIRECT) return false;
+  const uint64_t kch = (d + 31) / 32;
+  const uint64_t ntiles = (n + 15) / 16;
+  const uint32_t ss = std::max<uint32_t>(1, (3 * k + 15) / 16);
+  return n >= 1024 && d >= 1 && kch <= 8 && k <= 128 && ntiles >= 8ull * ss;
+}
+
+// Certified MFMA candidates + exact rerank; returns false when the input is
+// not finite (the caller then uses the direct kernel).
+bool knn_mfma_path(const float *x_dev, uint64_t n, uint64_t d, uint32_t k, const float *xp,
+                   const KnnLayout &L, bool split, DevBuf<uint32_t> &idx, DevBuf<float> &dist,
+                   hipStream_t s) {
+  const uint32_t kch = static_cast<uint32_t>((d + 31) / 32);
+  const uint64_t ntiles = (n + 15) / 16;
+  const uint64_t rows_padded = ntiles * 16;
+  DevBuf<double> mean(d), scratch(cov_scratch_doubles(n, static_cast<uint32_t>(d)));
+  PM_HIP(launch_colmean_f64(x_dev, n, static_cast<uint32_t>(d), mean.get(), scratch.get(), s));
+  const size_t frag_words = ntiles * kch * 64 * 4;  // 16 B per lane fragment
+  DevBuf<uint32_t> fhi(frag_words), flo(split ? frag_words : 4);
+  PM_HIP(launch_knn_frag(x_dev, n, static_cast<uint32_t>(d), mean.get(), kch, ntiles, fhi.get(),
+                         split ? flo.get() : nullptr, s));
+  DevBuf<float> arow(rows_padded);
+  DevBuf<uint32_t> flags(2);
+  PM_HIP(launch_fill_zero(flags.get(), flags.bytes(), s));
+  const double kappa = knn_kappa(kch * 32, split);
+  PM_HIP(launch_knn_rownorm(x_dev, n, static_cast<uint32_t>(d), mean.get(),
+                            static_cast<float>(1.0 - kappa), rows_padded, arow.get(), flags.get(), s));
+  uint32_t hflags[2] = {0, 0};
+  download(hflags, flags.get(), 2, s);
+  PM_HIP(hipStreamSynchronize(s));
+  if (hflags[0]) return false;
+  const uint32_t ss = std::max<uint32_t>(1, (3 * k + 15) / 16);
+  const uint32_t capg = (3.0 * k * 2.5 <= 512.0) ? 512 : 1024;
+  DevBuf<uint32_t> gcnt(n), gcand(n * capg), fb_list(n);
+  DevBuf<float> gtau(n);
+  Timer tk, tr;
+  tk.start(s);
+  PM_HIP(launch_knn_cand(fhi.get(), split ? flo.get() : fhi.get(), arow.get(), static_cast<uint32_t>(n),
+                         kch, static_cast<uint32_t>(ntiles), ss, capg, gcnt.get(), gcand.get(),
+                         gtau.get(), split, s));
+  tk.stop(s);
+  tr.start(s);
+  PM_HIP(launch_knn_rerank(xp, L, static_cast<uint32_t>(n), k, capg, gcnt.get(), gcand.get(), gtau.get(),
+                           knn_cert_scale(static_cast<uint32_t>(d)), idx.get(), dist.get(),
+                           flags.get() + 1, fb_list.get(), s));
+  tr.stop(s);
+  download(hflags, flags.get(), 2, s);
+  PM_HIP(hipStreamSynchronize(s));
+  g_tm.knn_kernel_ms += tk.ms();
+  g_tm.knn_kernel_launches += 1;
+  g_tm.knn_rerank_ms += tr.ms();
+  g_tm.knn_candidates = capg;
+  g_tm.knn_path = split ? 1 : 2;
+  const uint32_t nfb = hflags[1];
+  g_tm.knn_rescanned_rows += nfb;
+  if (nfb) {
+    Timer tf;
+    tf.start(s);
+    PM_HIP(launch_knn_exact(xp, n, L, k, idx.get(), dist.get(), s, fb_list.get(), nfb));
+    tf.stop(s);
+    g_tm.knn_fallback_ms += tf.ms();
+  }
+  return true;
+}
+
 void knn_device(const float *x_dev, uint64_t n, uint64_t d, uint32_t k, DevBuf<uint32_t> &idx,
-                DevBuf<float> &dist, hipStream_t s) {
+                DevBuf<float> &dist, hipStream_t s, int precision = PACMAP_KNN_FP32) {
   if (k > kKnnMaxK)
     fail(PACMAP_ERR_UNSUPPORTED, "k = " + std::to_string(k) + " exceeds the kNN kernel limit of " +
                                      std::to_string(kKnnMaxK));
@@ -73,17 +160,24 @@ void knn_device(const float *x_dev, uint64_t n, uint64_t d, uint32_t k, DevBuf<u
   if (k == 0) return;
   const KnnLayout L = knn_layout(static_cast<uint32_t>(d));
   DevBuf<float> xp(n * L.Dp);
-  Timer all, ker;
+  Timer all;
   all.start(s);
   PM_HIP(launch_knn_pack(x_dev, xp.get(), n, L, s));
-  ker.start(s);
-  PM_HIP(launch_knn_exact(xp.get(), n, L, k, idx.get(), dist.get(), s));
-  ker.stop(s);
+  bool done = false;
+  if (knn_mfma_eligible(n, d, k, precision))
+    done = knn_mfma_path(x_dev, n, d, k, xp.get(), L, precision != PACMAP_KNN_BF16, idx, dist, s);
+  if (!done) {
+    Timer ker;
+    ker.start(s);
+    PM_HIP(launch_knn_exact(xp.get(), n, L, k, idx.get(), dist.get(), s));
+    ker.stop(s);
+    g_tm.knn_kernel_ms += ker.ms();
+    g_tm.knn_kernel_launches += 1;
+    g_tm.knn_candidates = n - 1;
+    g_tm.knn_path = 0;
+  }
   all.stop(s);
-  g_tm.knn_kernel_ms += ker.ms();
-  g_tm.knn_kernel_launches += 1;
   g_tm.knn_ms += all.ms();
-  g_tm.knn_candidates = k;
 }
 
 // ---------------------------------------------------------------------------
@@ -489,7 +583,7 @@ void fit_impl(const float *x_host, const float *x_dev_in, uint64_t n, uint64_t d
     // neighbours computed here are a valid answer for that path too
     DevBuf<uint32_t> idx;
     DevBuf<float> dist;
-    knn_device(d_xk.get(), n, dk, kp, idx, dist, s);
+    knn_device(d_xk.get(), n, dk, kp, idx, dist, s, cfg->knn_precision);
     Timer tsg;
     tsg.start(s);
     DevBuf<float> sig(n), scaled(n * kp);
@@ -593,7 +687,7 @@ int pacmap_knn(const float *x, uint64_t n, uint64_t d, uint64_t k, int32_t preci
                float *dist, uint64_t *stats_out, pacmap_status *st) {
   g_tm = pacmap_timings{};
   return guarded(st, [&] {
-    if (precision < PACMAP_KNN_FP32 || precision > PACMAP_KNN_FP8)
+    if (precision < PACMAP_KNN_FP32 || precision > PACMAP_KNN_DIRECT || precision == PACMAP_KNN_FP8)
       fail(PACMAP_ERR_INVALID_ARGUMENT, "unknown kNN precision");
     if (n == 0) return;  // knn.rs:141-143
     if (!x && d) fail(PACMAP_ERR_INVALID_ARGUMENT, "x is NULL");
@@ -606,7 +700,7 @@ int pacmap_knn(const float *x, uint64_t n, uint64_t d, uint64_t k, int32_t preci
     upload(dx, x, n * d, s);
     DevBuf<uint32_t> di;
     DevBuf<float> dd;
-    knn_device(dx.get(), n, d, static_cast<uint32_t>(kk), di, dd, s);
+    knn_device(dx.get(), n, d, static_cast<uint32_t>(kk), di, dd, s, precision);
     download(idx, di.get(), n * kk, s);
     download(dist, dd.get(), n * kk, s);
     PM_HIP(hipStreamSynchronize(s));

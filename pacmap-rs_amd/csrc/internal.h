@@ -29,8 +29,24 @@ constexpr uint32_t kKnnMaxK = 448;
 hipError_t launch_knn_pack(const float *x, float *xp, uint64_t n, const KnnLayout &L,
                            hipStream_t s);
 // Exact FP32 kNN in the reference's distance order; k = min(k, n-1) already.
+// qlist (optional): only the nq listed query rows are computed.
 hipError_t launch_knn_exact(const float *xp, uint64_t n, const KnnLayout &L, uint32_t k,
-                            uint32_t *idx, float *dist, hipStream_t s);
+                            uint32_t *idx, float *dist, hipStream_t s,
+                            const uint32_t *qlist = nullptr, uint32_t nq = 0);
+
+// MFMA candidate path (knn_mfma.hip)
+hipError_t launch_knn_frag(const float *x, uint64_t n, uint32_t d, const double *mean, uint32_t kch,
+                           uint64_t ntiles, void *fhi, void *flo, hipStream_t s);
+hipError_t launch_knn_rownorm(const float *x, uint64_t n, uint32_t d, const double *mean,
+                              float one_minus_kappa, uint64_t rows_padded, float *arow,
+                              uint32_t *nonfinite, hipStream_t s);
+hipError_t launch_knn_cand(const void *fhi, const void *flo, const float *arow, uint32_t n,
+                           uint32_t kch, uint32_t ntiles, uint32_t ss, uint32_t capg,
+                           uint32_t *gcnt, uint32_t *gcand, float *gtau, bool split, hipStream_t s);
+hipError_t launch_knn_rerank(const float *xp, const KnnLayout &L, uint32_t n, uint32_t k,
+                             uint32_t capg, const uint32_t *gcnt, const uint32_t *gcand,
+                             const float *gtau, double cert_scale, uint32_t *idx, float *dist,
+                             uint32_t *fb_count, uint32_t *fb_list, hipStream_t s);
 
 hipError_t launch_sigma(const float *dist, uint64_t n, uint32_t kp, float *sig, hipStream_t s);
 hipError_t launch_scale_nn(const float *dist, const uint32_t *idx, const float *sig, uint64_t n,

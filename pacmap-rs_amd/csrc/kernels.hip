@@ -133,7 +133,9 @@ template <int TQ, int KCAP>
 __global__ __launch_bounds__(256) void knn_exact_kernel(const float *__restrict__ xp, uint32_t n,
                                                         uint32_t Dp, uint32_t Cp, uint32_t k,
                                                         uint32_t *__restrict__ oidx,
-                                                        float *__restrict__ odist) {
+                                                        float *__restrict__ odist,
+                                                        const uint32_t *__restrict__ qlist,
+                                                        uint32_t nq) {
   constexpr int QB = 16 * TQ, CB = 64, KC = static_cast<int>(kKnnChunk), R = KCAP / 64;
   __shared__ __align__(16) float sQ[KC][QB];
   __shared__ __align__(16) float sC[KC][CB];
@@ -146,6 +148,12 @@ __global__ __launch_bounds__(256) void knn_exact_kernel(const float *__restrict_
   const uint32_t qbase = blockIdx.x * QB;
   const uint32_t segEnd = 8u * Cp;
   const uint32_t thresh = KCAP - CB;
+  // query rows of this block: consecutive rows, or entries of qlist
+  auto qrow = [&](int ql) -> uint32_t {
+    const uint32_t p = qbase + static_cast<uint32_t>(ql);
+    if (qlist) return p < nq ? qlist[p] : 0xffffffffu;
+    return p < n ? p : 0xffffffffu;
+  };
   for (int i = t; i < QB; i += 256) {
     sCnt[i] = 0;
     sTau[i] = ~0ull;
@@ -166,7 +174,7 @@ __global__ __launch_bounds__(256) void knn_exact_kernel(const float *__restrict_
       __syncthreads();
       for (int li = t; li < QB * (KC / 4); li += 256) {
         const int q = li / (KC / 4), f4 = li % (KC / 4);
-        const uint32_t row = qbase + q;
+        const uint32_t row = qrow(q);
         float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
         if (row < n) v = *reinterpret_cast<const float4 *>(xp + static_cast<size_t>(row) * Dp + f0 + 4 * f4);
         sQ[4 * f4 + 0][q] = v.x;
@@ -242,7 +250,7 @@ __global__ __launch_bounds__(256) void knn_exact_kernel(const float *__restrict_
 #pragma unroll
     for (int i = 0; i < TQ; ++i) {
       const int ql = tq * TQ + i;
-      const uint32_t q = qbase + ql;
+      const uint32_t q = qrow(ql);
       if (q >= n) continue;
       const uint64_t tau = sTau[ql];
 #pragma unroll
@@ -286,7 +294,7 @@ __global__ __launch_bounds__(256) void knn_exact_kernel(const float *__restrict_
 
   // final ordering: ascending (distance, index), first k
   for (int ql = wave; ql < QB; ql += 4) {
-    const uint32_t q = qbase + ql;
+    const uint32_t q = qrow(ql);
     if (q >= n) continue;
     const uint32_t cnt = sCnt[ql];
     uint64_t v[R];
@@ -308,15 +316,18 @@ __global__ __launch_bounds__(256) void knn_exact_kernel(const float *__restrict_
 }
 
 hipError_t launch_knn_exact(const float *xp, uint64_t n, const KnnLayout &L, uint32_t k,
-                            uint32_t *idx, float *dist, hipStream_t s) {
+                            uint32_t *idx, float *dist, hipStream_t s, const uint32_t *qlist,
+                            uint32_t nq) {
   if (n == 0 || k == 0) return hipSuccess;
   const uint32_t nn = static_cast<uint32_t>(n);
+  const uint32_t rows = qlist ? nq : nn;
+  if (rows == 0) return hipSuccess;
   if (k <= 64) {
-    knn_exact_kernel<4, 128><<<(nn + 63) / 64, 256, 0, s>>>(xp, nn, L.Dp, L.Cp, k, idx, dist);
+    knn_exact_kernel<4, 128><<<(rows + 63) / 64, 256, 0, s>>>(xp, nn, L.Dp, L.Cp, k, idx, dist, qlist, nq);
   } else if (k <= 192) {
-    knn_exact_kernel<2, 256><<<(nn + 31) / 32, 256, 0, s>>>(xp, nn, L.Dp, L.Cp, k, idx, dist);
+    knn_exact_kernel<2, 256><<<(rows + 31) / 32, 256, 0, s>>>(xp, nn, L.Dp, L.Cp, k, idx, dist, qlist, nq);
   } else if (k <= kKnnMaxK) {
-    knn_exact_kernel<1, 512><<<(nn + 15) / 16, 256, 0, s>>>(xp, nn, L.Dp, L.Cp, k, idx, dist);
+    knn_exact_kernel<1, 512><<<(rows + 15) / 16, 256, 0, s>>>(xp, nn, L.Dp, L.Cp, k, idx, dist, qlist, nq);
   } else {
     return hipErrorInvalidValue;
   }

@@ -1,0 +1,492 @@
+// knn_mfma.hip -- certified exact kNN with MFMA candidate generation (gfx950).
+//
+// The reference's exact kNN (knn.rs:137-194) ranks every pair by the FP32
+// distance of distance.rs:50-75.  Recomputing all n^2 distances in that order
+// is VALU work; here the n^2 part runs on the matrix cores and only a short
+// candidate list per row is recomputed in the exact order:
+//
+//  1. prep: rows are centred (c = x - mean, distances are unchanged) and split
+//     into bf16 hi + lo parts stored in MFMA fragment order; per row
+//     a = |c|^2 (1 - kappa), where kappa bounds every rounding error of step 2
+//     relative to |c_q|^2 + |c_j|^2.
+//  2. candidates: lb(q, j) = a_q + a_j - 2 <c_q, c_j> via 3 bf16 MFMAs
+//     (hi*hi + hi*lo + lo*hi, FP32 accumulation) is a lower bound of the true
+//     squared distance.  A threshold tau_q is taken from a 1/ss sample of the
+//     candidate tiles (the 16th smallest lb), then every j with lb <= tau_q is
+//     appended to the row's list.
+//  3. rerank: the listed candidates get the exact FP32 distance of the
+//     reference, are sorted by (distance, index), and the row is certified
+//     when every unlisted candidate must be strictly farther than the k-th
+//     (sqrt(tau) with rounding margins > D_k).  Uncertified rows are redone by
+//     the exact direct kernel, so the result always equals the exact path.
+#include "internal.h"
+
+#include <cstdint>
+
+namespace pm {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ uint32_t tkey_m(float f) {
+  const uint32_t b = __float_as_uint(f);
+  return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+}
+__device__ __forceinline__ float from_tkey_m(uint32_t k) {
+  const uint32_t b = (k & 0x80000000u) ? (k & 0x7fffffffu) : ~k;
+  return __uint_as_float(b);
+}
+__device__ __forceinline__ uint64_t shfl_xor_u64m(uint64_t v, int m) {
+  uint32_t lo = static_cast<uint32_t>(v), hi = static_cast<uint32_t>(v >> 32);
+  lo = __shfl_xor(lo, m, 64);
+  hi = __shfl_xor(hi, m, 64);
+  return (static_cast<uint64_t>(hi) << 32) | lo;
+}
+__device__ __forceinline__ uint64_t shfl_u64m(uint64_t v, int src) {
+  uint32_t lo = static_cast<uint32_t>(v), hi = static_cast<uint32_t>(v >> 32);
+  lo = __shfl(lo, src, 64);
+  hi = __shfl(hi, src, 64);
+  return (static_cast<uint64_t>(hi) << 32) | lo;
+}
+
+template <int R>
+__device__ __forceinline__ void wave_bitonic_m(uint64_t (&v)[R], int lane) {
+  constexpr int N = 64 * R;
+#pragma unroll
+  for (int size = 2; size <= N; size <<= 1) {
+#pragma unroll
+    for (int stride = size >> 1; stride > 0; stride >>= 1) {
+      if (stride >= 64) {
+        const int rs = stride >> 6;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          if ((r & rs) == 0) {
+            const bool asc = (((r * 64 + lane) & size) == 0);
+            const uint64_t a = v[r], b = v[r | rs];
+            const uint64_t lo = a < b ? a : b, hi = a < b ? b : a;
+            v[r] = asc ? lo : hi;
+            v[r | rs] = asc ? hi : lo;
+          }
+        }
+      } else {
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          const uint64_t o = shfl_xor_u64m(v[r], stride);
+          const bool asc = (((r * 64 + lane) & size) == 0);
+          const bool lower = (lane & stride) == 0;
+          const uint64_t lo = v[r] < o ? v[r] : o, hi = v[r] < o ? o : v[r];
+          v[r] = (lower == asc) ? lo : hi;
+        }
+      }
+    }
+  }
+}
+
+template <int R>
+__device__ __forceinline__ uint64_t wave_pick_m(const uint64_t (&v)[R], uint32_t pos) {
+  uint64_t sel = 0;
+#pragma unroll
+  for (int r = 0; r < R; ++r)
+    if (static_cast<uint32_t>(r) == (pos >> 6)) sel = v[r];
+  return shfl_u64m(sel, static_cast<int>(pos & 63u));
+}
+
+// Ascending bitonic sort of 32 floats held by one 16-lane segment, element
+// e = reg * 16 + (lane & 15).
+__device__ __forceinline__ void seg32_sort(float &v0, float &v1, int lane) {
+  const int li = lane & 15;
+#pragma unroll
+  for (int size = 2; size <= 32; size <<= 1) {
+#pragma unroll
+    for (int stride = size >> 1; stride > 0; stride >>= 1) {
+      if (stride == 16) {
+        const bool asc0 = ((li & size) == 0);  // element li vs 16 + li (size 32: ascending)
+        const float lo = fminf(v0, v1), hi = fmaxf(v0, v1);
+        v0 = asc0 ? lo : hi;
+        v1 = asc0 ? hi : lo;
+      } else {
+        const float o0 = __shfl_xor(v0, stride, 64), o1 = __shfl_xor(v1, stride, 64);
+        const bool lower = (li & stride) == 0;
+        const bool asc0 = ((li & size) == 0), asc1 = (((16 + li) & size) == 0);
+        v0 = (lower == asc0) ? fminf(v0, o0) : fmaxf(v0, o0);
+        v1 = (lower == asc1) ? fminf(v1, o1) : fmaxf(v1, o1);
+      }
+    }
+  }
+}
+
+__device__ __forceinline__ uint16_t bf16_rn(float f) {
+  const uint32_t b = __float_as_uint(f);
+  const uint32_t r = b + 0x7fffu + ((b >> 16) & 1u);  // finite inputs only
+  return static_cast<uint16_t>(r >> 16);
+}
+__device__ __forceinline__ float bf16_to_f(uint16_t h) { return __uint_as_float(static_cast<uint32_t>(h) << 16); }
+
+// ---------------------------------------------------------------------------
+// prep: centred bf16 hi/lo fragments, a = |c|^2 (1 - kappa), finiteness flag
+// ---------------------------------------------------------------------------
+// Fragment order: [row tile t][32-feature chunk kc][lane l][8 values], with
+// row = 16 t + (l & 15) and feature = 32 kc + 8 (l >> 4) + e.  One wave-wide
+// 16-byte load per lane fetches an A or B operand of v_mfma_f32_16x16x32_bf16.
+__global__ void knn_frag_kernel(const float *__restrict__ x, uint64_t n, uint32_t d,
+                                const double *__restrict__ mean, uint32_t kch, uint64_t ntiles,
+                                uint4 *__restrict__ fhi, uint4 *__restrict__ flo) {
+  const uint64_t total = ntiles * kch * 64;
+  for (uint64_t g = blockIdx.x * 256ull + threadIdx.x; g < total; g += gridDim.x * 256ull) {
+    const uint32_t l = static_cast<uint32_t>(g & 63);
+    const uint64_t tk = g >> 6;
+    const uint32_t kc = static_cast<uint32_t>(tk % kch);
+    const uint64_t t = tk / kch;
+    const uint64_t row = t * 16 + (l & 15);
+    const uint32_t f0 = kc * 32 + 8 * (l >> 4);
+    uint16_t h[8], lo[8];
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      float c = 0.0f;
+      const uint32_t f = f0 + e;
+      if (row < n && f < d) c = static_cast<float>(static_cast<double>(x[row * d + f]) - mean[f]);
+      h[e] = bf16_rn(c);
+      lo[e] = bf16_rn(c - bf16_to_f(h[e]));
+    }
+    fhi[g] = make_uint4(h[0] | (static_cast<uint32_t>(h[1]) << 16), h[2] | (static_cast<uint32_t>(h[3]) << 16),
+                        h[4] | (static_cast<uint32_t>(h[5]) << 16), h[6] | (static_cast<uint32_t>(h[7]) << 16));
+    if (flo) flo[g] = make_uint4(lo[0] | (static_cast<uint32_t>(lo[1]) << 16), lo[2] | (static_cast<uint32_t>(lo[3]) << 16),
+                        lo[4] | (static_cast<uint32_t>(lo[5]) << 16), lo[6] | (static_cast<uint32_t>(lo[7]) << 16));
+  }
+}
+
+__global__ void knn_rownorm_kernel(const float *__restrict__ x, uint64_t n, uint32_t d,
+                                   const double *__restrict__ mean, float one_minus_kappa,
+                                   uint64_t rows_padded, float *__restrict__ arow,
+                                   uint32_t *__restrict__ nonfinite) {
+  const uint64_t row = blockIdx.x * 256ull + threadIdx.x;
+  if (row >= rows_padded) return;
+  if (row >= n) {
+    arow[row] = 0.0f;
+    return;
+  }
+  double s = 0.0;
+  bool bad = false;
+  for (uint32_t f = 0; f < d; ++f) {
+    const float xv = x[row * d + f];
+    bad = bad || !isfinite(xv);
+    const float c = static_cast<float>(static_cast<double>(xv) - mean[f]);
+    s += static_cast<double>(c) * c;
+  }
+  arow[row] = static_cast<float>(s * static_cast<double>(one_minus_kappa));
+  if (bad) atomicOr(nonfinite, 1u);
+}
+
+// ---------------------------------------------------------------------------
+// candidate kernel
+// ---------------------------------------------------------------------------
+// Block = 4 waves; each wave owns RT query tiles (16 RT rows) with their A
+// fragments in registers and streams candidate tiles staged in LDS (CT tiles
+// per stage, shared by the 4 waves).  Lane l holds, per query tile rt and
+// r = 0..3, the value for query row 16 rt + 4 (l >> 4) + r and candidate
+// 16 t + (l & 15).
+template <int KCH, int RT, bool SPLIT>
+__global__ __launch_bounds__(256) void knn_mfma_cand_kernel(
+    const uint4 *__restrict__ fhi, const uint4 *__restrict__ flo, const float *__restrict__ arow,
+    uint32_t n, uint32_t ntiles, uint32_t ss, uint32_t capg, uint32_t *__restrict__ gcnt,
+    uint32_t *__restrict__ gcand, float *__restrict__ gtau) {
+  constexpr int CT = (16 / KCH) > 0 ? (16 / KCH) : 1;  // candidate tiles per LDS stage
+  constexpr int NH = SPLIT ? 2 : 1;
+  __shared__ uint4 sB[CT][KCH][NH][64];
+  __shared__ float sA[CT][16];
+  __shared__ uint32_t sCnt[4][16 * RT];
+
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const uint32_t qtile0 = (blockIdx.x * 4 + wave) * RT;
+  const int rsub = lane >> 4, col = lane & 15;
+
+  bf16x8 ahi[RT][KCH], alo[RT][KCH];
+  float aq[RT][4];
+  uint32_t qidx[RT][4];
+#pragma unroll
+  for (int rt = 0; rt < RT; ++rt) {
+    const uint32_t tile = qtile0 + rt;
+#pragma unroll
+    for (int kc = 0; kc < KCH; ++kc) {
+      uint4 h = make_uint4(0, 0, 0, 0), lo = make_uint4(0, 0, 0, 0);
+      if (tile < ntiles) {
+        h = fhi[(static_cast<size_t>(tile) * KCH + kc) * 64 + lane];
+        if (SPLIT) lo = flo[(static_cast<size_t>(tile) * KCH + kc) * 64 + lane];
+      }
+      ahi[rt][kc] = __builtin_bit_cast(bf16x8, h);
+      alo[rt][kc] = __builtin_bit_cast(bf16x8, lo);
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const uint32_t q = tile * 16 + 4 * rsub + r;
+      qidx[rt][r] = q;
+      aq[rt][r] = q < n ? arow[q] : 0.0f;
+    }
+  }
+  for (int i = threadIdx.x; i < 4 * 16 * RT; i += 256) (&sCnt[0][0])[i] = 0;
+
+  // per-lane two smallest sampled bounds per query row (phase 1)
+  float best0[RT][4], best1[RT][4], tau[RT][4];
+#pragma unroll
+  for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      best0[rt][r] = __builtin_huge_valf();
+      best1[rt][r] = __builtin_huge_valf();
+      tau[rt][r] = __builtin_huge_valf();
+    }
+
+  const uint32_t nsample = (ntiles + ss - 1) / ss;
+  for (int phase = 0; phase < 2; ++phase) {
+    const uint32_t slots = phase == 0 ? nsample : ntiles;
+    for (uint32_t s0 = 0; s0 < slots; s0 += CT) {
+      __syncthreads();
+      // cooperative stage load: CT tiles x KCH chunks x {hi, lo} x 64 lanes
+      for (int i = threadIdx.x; i < CT * KCH * NH * 64; i += 256) {
+        const int l = i & 63, h = (i >> 6) % NH, kc = (i / (64 * NH)) % KCH, ct = i / (64 * NH * KCH);
+        const uint32_t slot = s0 + ct;
+        const uint32_t tile = phase == 0 ? slot * ss : slot;
+        uint4 v = make_uint4(0, 0, 0, 0);
+        if (slot < slots && tile < ntiles) {
+          const size_t o = (static_cast<size_t>(tile) * KCH + kc) * 64 + l;
+          v = h == 0 ? fhi[o] : flo[o];
+        }
+        sB[ct][kc][h][l] = v;
+      }
+      for (int i = threadIdx.x; i < CT * 16; i += 256) {
+        const int ct = i >> 4, cc = i & 15;
+        const uint32_t slot = s0 + ct;
+        const uint32_t tile = phase == 0 ? slot * ss : slot;
+        sA[ct][cc] = (slot < slots && tile < ntiles) ? arow[tile * 16 + cc] : 0.0f;
+      }
+      __syncthreads();
+#pragma unroll 1
+      for (int ct = 0; ct < CT; ++ct) {
+        const uint32_t slot = s0 + ct;
+        if (slot >= slots) break;
+        const uint32_t tile = phase == 0 ? slot * ss : slot;
+        if (tile >= ntiles) break;
+        floatx4 acc[RT];
+#pragma unroll
+        for (int rt = 0; rt < RT; ++rt) acc[rt] = floatx4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int kc = 0; kc < KCH; ++kc) {
+          const bf16x8 bh = __builtin_bit_cast(bf16x8, sB[ct][kc][0][lane]);
+#pragma unroll
+          for (int rt = 0; rt < RT; ++rt)
+            acc[rt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi[rt][kc], bh, acc[rt], 0, 0, 0);
+          if (SPLIT) {
+            const bf16x8 bl = __builtin_bit_cast(bf16x8, sB[ct][kc][NH - 1][lane]);
+#pragma unroll
+            for (int rt = 0; rt < RT; ++rt) {
+              acc[rt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi[rt][kc], bl, acc[rt], 0, 0, 0);
+              acc[rt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(alo[rt][kc], bh, acc[rt], 0, 0, 0);
+            }
+          }
+        }
+        const uint32_t c = tile * 16 + col;
+        const float ac = sA[ct][col];
+        const bool cvalid = c < n;
+#pragma unroll
+        for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const uint32_t q = qidx[rt][r];
+            const float lb = fmaf(-2.0f, acc[rt][r], aq[rt][r] + ac);
+            const bool ok = cvalid && q < n && c != q;
+            if (phase == 0) {
+              if (ok && lb < best1[rt][r]) {
+                const bool first = lb < best0[rt][r];
+                best1[rt][r] = first ? best0[rt][r] : lb;
+                best0[rt][r] = first ? lb : best0[rt][r];
+              }
+            } else if (ok && lb <= tau[rt][r]) {
+              const uint32_t pos = atomicAdd(&sCnt[wave][rt * 16 + 4 * rsub + r], 1u);
+              if (pos < capg) gcand[static_cast<size_t>(q) * capg + pos] = c;
+            }
+          }
+      }
+    }
+    if (phase == 0) {
+      // tau = 16th smallest of the 32 per-row values held by the row's 16 lanes
+#pragma unroll
+      for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          float v0 = best0[rt][r], v1 = best1[rt][r];
+          seg32_sort(v0, v1, lane);
+          tau[rt][r] = __shfl(v0, (lane & ~15) | 15, 64);
+        }
+    }
+  }
+  __syncthreads();
+  if (col == 0) {
+#pragma unroll
+    for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const uint32_t q = qidx[rt][r];
+        if (q < n) {
+          gcnt[q] = sCnt[wave][rt * 16 + 4 * rsub + r];
+          gtau[q] = tau[rt][r];
+        }
+      }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// exact rerank + certificate (one wavefront per query row)
+// ---------------------------------------------------------------------------
+template <int R>
+__global__ __launch_bounds__(256) void knn_rerank_kernel(
+    const float *__restrict__ xp, uint32_t Dp, uint32_t Cp, uint32_t n, uint32_t k, uint32_t capg,
+    const uint32_t *__restrict__ gcnt, const uint32_t *__restrict__ gcand,
+    const float *__restrict__ gtau, double cert_scale, uint32_t *__restrict__ oidx,
+    float *__restrict__ odist, uint32_t *__restrict__ fb_count, uint32_t *__restrict__ fb_list) {
+  extern __shared__ float sQrow[];  // 4 x Dp
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const uint32_t q = blockIdx.x * 4 + wave;
+  float *qv = sQrow + static_cast<size_t>(wave) * Dp;
+  if (q < n)
+    for (uint32_t f = lane; f < Dp; f += 64) qv[f] = xp[static_cast<size_t>(q) * Dp + f];
+  __syncthreads();
+  if (q >= n) return;
+  const uint32_t cnt = gcnt[q];
+  const bool all_listed = cnt == n - 1;
+  if (cnt > capg || cnt < k) {
+    if (lane == 0) fb_list[atomicAdd(fb_count, 1u)] = q;
+    return;
+  }
+  const uint32_t segEnd = 8u * Cp;
+  uint64_t v[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const uint32_t e = r * 64 + lane;
+    v[r] = ~0ull;
+    if (e < cnt) {
+      const uint32_t c = gcand[static_cast<size_t>(q) * capg + e];
+      const float *cr = xp + static_cast<size_t>(c) * Dp;
+      float s = 0.0f, acc = 0.0f;
+      for (uint32_t f = 0; f < Dp; f += 4) {
+        const float4 cv = *reinterpret_cast<const float4 *>(cr + f);
+        const float4 qq = *reinterpret_cast<const float4 *>(qv + f);
+        const float cvs[4] = {cv.x, cv.y, cv.z, cv.w}, qvs[4] = {qq.x, qq.y, qq.z, qq.w};
+        if (f >= segEnd) {
+#pragma unroll
+          for (int e2 = 0; e2 < 4; ++e2) {
+            const float df = qvs[e2] - cvs[e2];
+            const float sq = df * df;
+            s = s + sq;
+          }
+        } else {
+#pragma unroll
+          for (int e2 = 0; e2 < 4; ++e2) {
+            const float df = qvs[e2] - cvs[e2];
+            const float sq = df * df;
+            acc = acc + sq;
+          }
+          if ((f + 4u) % Cp == 0u) {
+            s = s + acc;
+            acc = 0.0f;
+          }
+        }
+      }
+      const float dd = sqrtf(s);
+      v[r] = (static_cast<uint64_t>(tkey_m(dd)) << 32) | c;
+    }
+  }
+  wave_bitonic_m<R>(v, lane);
+  const float dk = from_tkey_m(static_cast<uint32_t>(wave_pick_m<R>(v, k - 1) >> 32));
+  const float t = gtau[q];
+  // every unlisted candidate has a true squared distance > tau (up to the
+  // rounding factor folded in cert_scale), hence a reference distance > dk
+  const bool cert = all_listed || (t > 0.0f && sqrt(static_cast<double>(t) * cert_scale) > static_cast<double>(dk));
+  if (!cert) {
+    if (lane == 0) fb_list[atomicAdd(fb_count, 1u)] = q;
+    return;
+  }
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const uint32_t e = r * 64 + lane;
+    if (e < k) {
+      oidx[static_cast<size_t>(q) * k + e] = static_cast<uint32_t>(v[r]);
+      odist[static_cast<size_t>(q) * k + e] = from_tkey_m(static_cast<uint32_t>(v[r] >> 32));
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// host launchers
+// ---------------------------------------------------------------------------
+hipError_t launch_knn_frag(const float *x, uint64_t n, uint32_t d, const double *mean, uint32_t kch,
+                           uint64_t ntiles, void *fhi, void *flo, hipStream_t s) {
+  const uint64_t total = ntiles * kch * 64;
+  uint64_t blocks = (total + 255) / 256;
+  if (blocks > 65536) blocks = 65536;
+  if (blocks == 0) return hipSuccess;
+  knn_frag_kernel<<<static_cast<uint32_t>(blocks), 256, 0, s>>>(x, n, d, mean, kch, ntiles,
+                                                                static_cast<uint4 *>(fhi),
+                                                                static_cast<uint4 *>(flo));
+  return hipGetLastError();
+}
+
+hipError_t launch_knn_rownorm(const float *x, uint64_t n, uint32_t d, const double *mean,
+                              float one_minus_kappa, uint64_t rows_padded, float *arow,
+                              uint32_t *nonfinite, hipStream_t s) {
+  knn_rownorm_kernel<<<static_cast<uint32_t>((rows_padded + 255) / 256), 256, 0, s>>>(
+      x, n, d, mean, one_minus_kappa, rows_padded, arow, nonfinite);
+  return hipGetLastError();
+}
+
+template <int KCH>
+static hipError_t cand_dispatch(const void *fhi, const void *flo, const float *arow, uint32_t n,
+                                uint32_t ntiles, uint32_t ss, uint32_t capg, uint32_t *gcnt,
+                                uint32_t *gcand, float *gtau, bool split, hipStream_t s) {
+  constexpr int RT = KCH <= 4 ? 4 : 2;
+  const uint32_t qtiles_per_block = 4 * RT;
+  const uint32_t blocks = (ntiles + qtiles_per_block - 1) / qtiles_per_block;
+  if (split)
+    knn_mfma_cand_kernel<KCH, RT, true><<<blocks, 256, 0, s>>>(
+        static_cast<const uint4 *>(fhi), static_cast<const uint4 *>(flo), arow, n, ntiles, ss, capg,
+        gcnt, gcand, gtau);
+  else
+    knn_mfma_cand_kernel<KCH, RT, false><<<blocks, 256, 0, s>>>(
+        static_cast<const uint4 *>(fhi), static_cast<const uint4 *>(flo), arow, n, ntiles, ss, capg,
+        gcnt, gcand, gtau);
+  return hipGetLastError();
+}
+
+hipError_t launch_knn_cand(const void *fhi, const void *flo, const float *arow, uint32_t n,
+                           uint32_t kch, uint32_t ntiles, uint32_t ss, uint32_t capg,
+                           uint32_t *gcnt, uint32_t *gcand, float *gtau, bool split, hipStream_t s) {
+  switch (kch) {
+    case 1: return cand_dispatch<1>(fhi, flo, arow, n, ntiles, ss, capg, gcnt, gcand, gtau, split, s);
+    case 2: return cand_dispatch<2>(fhi, flo, arow, n, ntiles, ss, capg, gcnt, gcand, gtau, split, s);
+    case 3: return cand_dispatch<3>(fhi, flo, arow, n, ntiles, ss, capg, gcnt, gcand, gtau, split, s);
+    case 4: return cand_dispatch<4>(fhi, flo, arow, n, ntiles, ss, capg, gcnt, gcand, gtau, split, s);
+    case 5: return cand_dispatch<5>(fhi, flo, arow, n, ntiles, ss, capg, gcnt, gcand, gtau, split, s);
+    case 6: return cand_dispatch<6>(fhi, flo, arow, n, ntiles, ss, capg, gcnt, gcand, gtau, split, s);
+    case 7: return cand_dispatch<7>(fhi, flo, arow, n, ntiles, ss, capg, gcnt, gcand, gtau, split, s);
+    case 8: return cand_dispatch<8>(fhi, flo, arow, n, ntiles, ss, capg, gcnt, gcand, gtau, split, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+hipError_t launch_knn_rerank(const float *xp, const KnnLayout &L, uint32_t n, uint32_t k,
+                             uint32_t capg, const uint32_t *gcnt, const uint32_t *gcand,
+                             const float *gtau, double cert_scale, uint32_t *idx, float *dist,
+                             uint32_t *fb_count, uint32_t *fb_list, hipStream_t s) {
+  const uint32_t blocks = (n + 3) / 4;
+  const size_t shm = static_cast<size_t>(4) * L.Dp * sizeof(float);
+  if (capg == 512)
+    knn_rerank_kernel<8><<<blocks, 256, shm, s>>>(xp, L.Dp, L.Cp, n, k, capg, gcnt, gcand, gtau,
+                                                  cert_scale, idx, dist, fb_count, fb_list);
+  else if (capg == 1024)
+    knn_rerank_kernel<16><<<blocks, 256, shm, s>>>(xp, L.Dp, L.Cp, n, k, capg, gcnt, gcand, gtau,
+                                                   cert_scale, idx, dist, fb_count, fb_list);
+  else
+    return hipErrorInvalidValue;
+  return hipGetLastError();
+}
+
+}  // namespace pm

@@ -75,12 +75,13 @@ class Timings(C.Structure):
         (name, C.c_double) for name in ("sigma_ms", "sampling_ms", "csr_ms", "graph_build_ms",
                                         "optimize_ms")] + [
         (name, C.c_uint64) for name in ("iterations", "csr_entries", "knn_rescanned_rows",
-                                        "knn_candidates")]
+                                        "knn_candidates")] + [
+        ("knn_rerank_ms", C.c_double), ("knn_fallback_ms", C.c_double), ("knn_path", C.c_uint64)]
 
 
 INIT_PCA, INIT_VALUE, INIT_RANDOM = 0, 1, 2
 PAIRS_GENERATE, PAIRS_NEIGHBORS_PROVIDED, PAIRS_ALL_PROVIDED = 0, 1, 2
-KNN_PRECISIONS = {"fp32": 0, "bf16": 1, "fp8": 2}
+KNN_PRECISIONS = {"fp32": 0, "bf16": 1, "fp8": 2, "direct": 3}
 
 _P = C.c_void_p
 _u64 = C.c_uint64
