@@ -149,6 +149,15 @@ int pacmap_sigma_scale(const float *knn_dist, const uint32_t *knn_idx, uint64_t 
                        uint64_t kprime, uint64_t n_neighbors, float *sigma_out,
                        float *scaled_out, uint32_t *pairs_out, pacmap_status *st);
 
+/* Seam: mid-near and far pair sampling of generate_pairs with a seed
+ * (sampling.rs:77-142, 296-313; neighbors.rs:92-96) on the GPU.  x: n x d
+ * (the preprocessed data), pair_nn: (n*n_neighbors) x 2 (i-major NN pairs,
+ * the far-pair reject lists).  pair_mn_out: (n*n_mn) x 2, pair_fp_out:
+ * (n*n_fp) x 2 (unfilled far pairs stay (0, 0)). */
+int pacmap_sample_pairs(const float *x, uint64_t n, uint64_t d, const uint32_t *pair_nn,
+                        uint64_t n_neighbors, uint64_t n_mn, uint64_t n_fp, uint64_t seed,
+                        uint32_t *pair_mn_out, uint32_t *pair_fp_out, pacmap_status *st);
+
 /* Seam: the optimiser loop of pacmap() (lib.rs:781-837) from an already
  * initialised Y (no Value renormalisation), as one hipGraph.  y: n x dims in/out. */
 int pacmap_optimize(float *y, uint64_t n, uint64_t dims, const uint32_t *pair_nn,

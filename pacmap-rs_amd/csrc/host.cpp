@@ -139,9 +139,12 @@ bool knn_mfma_path(const float *x_dev, uint64_t n, uint64_t d, uint32_t k, const
   const uint32_t nfb = hflags[1];
   g_tm.knn_rescanned_rows += nfb;
   if (nfb) {
+    // split each listed row's candidates over S blocks, then merge
+    const uint32_t S = static_cast<uint32_t>(std::min<uint64_t>(64, std::max<uint64_t>(1, n / 2048)));
+    DevBuf<uint64_t> part(static_cast<size_t>(nfb) * S * k);
     Timer tf;
     tf.start(s);
-    PM_HIP(launch_knn_exact(xp, n, L, k, idx.get(), dist.get(), s, fb_list.get(), nfb));
+    PM_HIP(launch_knn_rescan(xp, n, L, k, fb_list.get(), nfb, part.get(), S, idx.get(), dist.get(), s));
     tf.stop(s);
     g_tm.knn_fallback_ms += tf.ms();
   }
@@ -192,7 +195,70 @@ struct GraphGuard {
   }
 };
 
-void optimize_device(const Csr &csr, uint64_t n, uint64_t dims, std::vector<float> &y, float lr,
+// Symmetrised CSR in device memory (see internal.h for the entry format).
+struct DevCsr {
+  DevBuf<uint32_t> row_ptr, ent;
+  uint64_t nnz = 0;
+};
+
+// Builds the CSR of three device pair arrays on the GPU.  Rows longer than
+// the per-row sort handles are rare (hubs); the whole CSR is then built on
+// the host from the same pairs, which keeps the layout deterministic.
+void build_csr_dev(uint64_t n, const uint32_t *const dp[3], const uint64_t rows[3], DevCsr &out,
+                   hipStream_t s) {
+  if (n >= kMaxPoints) fail(PACMAP_ERR_UNSUPPORTED, "too many points for the CSR entry format");
+  const double t0 = now_ms();
+  DevBuf<uint32_t> deg(n), flags(2), scratch((n + 1023) / 1024 + 2), cursor(n);
+  PM_HIP(launch_fill_zero(deg.get(), deg.bytes(), s));
+  PM_HIP(launch_fill_zero(flags.get(), flags.bytes(), s));
+  PM_HIP(launch_csr_count(dp, rows, static_cast<uint32_t>(n), deg.get(), flags.get(), s));
+  out.row_ptr.alloc(n + 1);
+  PM_HIP(launch_csr_scan(deg.get(), static_cast<uint32_t>(n), out.row_ptr.get(), cursor.get(), scratch.get(), s));
+  uint32_t hf[2] = {0, 0}, nnz = 0;
+  download(hf, flags.get(), 2, s);
+  download(&nnz, out.row_ptr.get() + n, 1, s);
+  PM_HIP(hipStreamSynchronize(s));
+  if (hf[0]) fail(PACMAP_ERR_INVALID_PAIR_INDEX, "pair index out of range");
+  out.nnz = nnz;
+  out.ent.alloc(std::max<uint64_t>(1, nnz));
+  PM_HIP(launch_csr_fill_sort(dp, rows, static_cast<uint32_t>(n), out.row_ptr.get(), cursor.get(),
+                              out.ent.get(), flags.get() + 1, s));
+  download(hf, flags.get(), 2, s);
+  PM_HIP(hipStreamSynchronize(s));
+  if (hf[1]) {
+    std::vector<uint32_t> hp[3];
+    PairList lists[3];
+    for (int t = 0; t < 3; ++t) {
+      hp[t].resize(2 * rows[t]);
+      download(hp[t].data(), dp[t], 2 * rows[t], s);
+      lists[t] = {hp[t].data(), rows[t]};
+    }
+    PM_HIP(hipStreamSynchronize(s));
+    Csr h;
+    build_csr(n, lists, h);
+    upload(out.row_ptr, h.row_ptr.data(), n + 1, s);
+    upload(out.ent, h.ent.data(), h.ent.size(), s);
+    PM_HIP(hipStreamSynchronize(s));
+  }
+  g_tm.csr_ms += now_ms() - t0;
+  g_tm.csr_entries = nnz;
+}
+
+// Uploads host pair arrays and builds their CSR on the device.
+void build_csr_from_host(uint64_t n, const PairList lists[3], DevCsr &out, hipStream_t s) {
+  DevBuf<uint32_t> dpair[3];
+  const uint32_t *dp[3];
+  uint64_t rows[3];
+  for (int t = 0; t < 3; ++t) {
+    rows[t] = lists[t].rows;
+    dpair[t].alloc(std::max<uint64_t>(2, 2 * rows[t]));
+    upload(dpair[t], lists[t].p, 2 * rows[t], s);
+    dp[t] = dpair[t].get();
+  }
+  build_csr_dev(n, dp, rows, out, s);
+}
+
+void optimize_device(const DevCsr &csr, uint64_t n, uint64_t dims, std::vector<float> &y, float lr,
                      const uint64_t iters[3], const uint64_t *snap_idx, uint64_t ns,
                      float *snaps_out, float *loss_out, bool record_loss, hipStream_t s) {
   const uint64_t T = iters[0] + iters[1] + iters[2];
@@ -213,10 +279,6 @@ void optimize_device(const Csr &csr, uint64_t n, uint64_t dims, std::vector<floa
     std::memset(snaps_out, 0, sizeof(float) * ns * nd);
     std::memcpy(snaps_out, y.data(), sizeof(float) * nd);
   }
-  const double tcsr = now_ms();
-  DevBuf<uint32_t> d_rp(n + 1), d_ent(std::max<size_t>(1, csr.ent.size()));
-  upload(d_rp, csr.row_ptr.data(), n + 1, s);
-  upload(d_ent, csr.ent.data(), csr.ent.size(), s);
   DevBuf<float> ybuf(2 * nd), m(nd), v(nd);
   upload(ybuf, y.data(), nd, s);
   PM_HIP(launch_fill_zero(m.get(), m.bytes(), s));
@@ -225,8 +287,6 @@ void optimize_device(const Csr &csr, uint64_t n, uint64_t dims, std::vector<floa
   const uint32_t nb = grad_blocks(static_cast<uint32_t>(n));
   DevBuf<float> part(record_loss ? T * nb : 0), d_loss(record_loss ? T : 0);
   PM_HIP(hipStreamSynchronize(s));
-  g_tm.csr_ms += now_ms() - tcsr;
-  g_tm.csr_entries = csr.ent.size();
   g_tm.iterations = T;
   if (T == 0) return;
 
@@ -237,8 +297,8 @@ void optimize_device(const Csr &csr, uint64_t n, uint64_t dims, std::vector<floa
   hipError_t cap_err = hipSuccess;
   for (uint64_t it = 0; it < T && cap_err == hipSuccess; ++it) {
     GradArgs a;
-    a.row_ptr = d_rp.get();
-    a.ent = d_ent.get();
+    a.row_ptr = csr.row_ptr.get();
+    a.ent = csr.ent.get();
     a.y_in = ybuf.get() + (it % 2) * nd;
     a.y_out = ybuf.get() + ((it + 1) % 2) * nd;
     a.m = m.get();
@@ -567,14 +627,26 @@ void fit_impl(const float *x_host, const float *x_dev_in, uint64_t n, uint64_t d
   g_tm.preprocess_ms += tp.ms();
 
   // --- sample_pairs (lib.rs:629-692) ---
-  std::vector<uint32_t> nn_pairs, mn_pairs, fp_pairs;
-  PairList lists[3];
-  std::vector<float> xk_host;
-  const uint64_t *seed = cfg->has_seed ? &cfg->seed : nullptr;
-  auto fetch_xk = [&]() {
-    xk_host.resize(n * dk);
-    download(xk_host.data(), d_xk.get(), n * dk, s);
-    PM_HIP(hipStreamSynchronize(s));
+  DevCsr csr;
+  uint64_t seed_base = cfg->seed;
+  if (!cfg->has_seed) {  // thread_rng in the reference: a fresh stream per call
+    std::random_device rd;
+    seed_base = (static_cast<uint64_t>(rd()) << 32) ^ rd();
+  }
+  // MN/FP sampling on the device from the NN pairs `d_nn` (knn per point)
+  auto sample_and_build = [&](const uint32_t *d_nn, uint32_t knn) {
+    Timer tsmp;
+    tsmp.start(s);
+    DevBuf<uint32_t> d_mn(std::max<uint64_t>(2, 2 * n * n_mn)), d_fp(std::max<uint64_t>(2, 2 * n * n_fp));
+    PM_HIP(launch_fill_zero(d_fp.get(), d_fp.bytes(), s));
+    PM_HIP(launch_sample_pairs(d_xk.get(), static_cast<uint32_t>(n), static_cast<uint32_t>(dk), d_nn, knn,
+                               static_cast<uint32_t>(n_mn), static_cast<uint32_t>(n_fp), seed_base,
+                               cfg->has_seed ? 1 : 0, d_mn.get(), d_fp.get(), s));
+    tsmp.stop(s);
+    g_tm.sampling_ms += tsmp.ms();
+    const uint32_t *dp[3] = {d_nn, d_mn.get(), d_fp.get()};
+    const uint64_t rows[3] = {n * knn, n * n_mn, n * n_fp};
+    build_csr_dev(n, dp, rows, csr, s);
   };
   if (pk == PACMAP_PAIRS_GENERATE) {  // neighbors.rs:56-108
     const uint32_t kp = static_cast<uint32_t>(std::min<uint64_t>(k + 50, n - 1));  // neighbors.rs:67
@@ -587,52 +659,32 @@ void fit_impl(const float *x_host, const float *x_dev_in, uint64_t n, uint64_t d
     Timer tsg;
     tsg.start(s);
     DevBuf<float> sig(n), scaled(n * kp);
-    DevBuf<uint32_t> d_nn(n * knn * 2);
+    DevBuf<uint32_t> d_nn(std::max<uint64_t>(2, n * knn * 2));
     PM_HIP(launch_sigma(dist.get(), n, kp, sig.get(), s));
     PM_HIP(launch_scale_nn(dist.get(), idx.get(), sig.get(), n, kp, knn, scaled.get(), d_nn.get(), s));
     tsg.stop(s);
-    nn_pairs.resize(n * knn * 2);
-    download(nn_pairs.data(), d_nn.get(), nn_pairs.size(), s);
-    fetch_xk();
     g_tm.sigma_ms += tsg.ms();
-    const double ts = now_ms();
-    mn_pairs.resize(n * n_mn * 2);
-    fp_pairs.resize(n * n_fp * 2);
-    sample_mn_pairs(xk_host.data(), n, dk, n_mn, seed, mn_pairs.data());
-    sample_fp_pairs(n, nn_pairs.data(), knn, n_fp, seed, fp_pairs.data());
-    g_tm.sampling_ms += now_ms() - ts;
-    lists[0] = {nn_pairs.data(), n * knn};
-    lists[1] = {mn_pairs.data(), n * n_mn};
-    lists[2] = {fp_pairs.data(), n * n_fp};
+    sample_and_build(d_nn.get(), knn);
   } else if (pk == PACMAP_PAIRS_NEIGHBORS_PROVIDED) {  // lib.rs:651-676
-    fetch_xk();
-    const double ts = now_ms();
     for (uint64_t r = 0; r < 2 * n * k; ++r)
       if (cfg->pair_neighbors[r] >= n) fail(PACMAP_ERR_INVALID_PAIR_INDEX, "pair_neighbors index out of range");
-    mn_pairs.resize(n * n_mn * 2);
-    fp_pairs.resize(n * n_fp * 2);
-    sample_mn_pairs(xk_host.data(), n, dk, n_mn, seed, mn_pairs.data());
-    sample_fp_pairs(n, cfg->pair_neighbors, k, n_fp, seed, fp_pairs.data());
-    g_tm.sampling_ms += now_ms() - ts;
-    lists[0] = {cfg->pair_neighbors, n * k};
-    lists[1] = {mn_pairs.data(), n * n_mn};
-    lists[2] = {fp_pairs.data(), n * n_fp};
+    DevBuf<uint32_t> d_nn(std::max<uint64_t>(2, 2 * n * k));
+    upload(d_nn, cfg->pair_neighbors, 2 * n * k, s);
+    sample_and_build(d_nn.get(), static_cast<uint32_t>(k));
   } else {  // lib.rs:679-690
-    lists[0] = {cfg->pair_neighbors, cfg->pair_neighbors_rows};
-    lists[1] = {cfg->pair_mn, cfg->pair_mn_rows};
-    lists[2] = {cfg->pair_fp, cfg->pair_fp_rows};
+    PairList lists[3] = {{cfg->pair_neighbors, cfg->pair_neighbors_rows},
+                         {cfg->pair_mn, cfg->pair_mn_rows},
+                         {cfg->pair_fp, cfg->pair_fp_rows}};
+    build_csr_from_host(n, lists, csr, s);
   }
-  Csr csr;
-  const double tc = now_ms();
-  build_csr(n, lists, csr);
-  g_tm.csr_ms += now_ms() - tc;
 
   // --- YInit (lib.rs:752-779) ---
   if (pca_path) {  // YInit::Preprocessed: x[:, :dims] * 0.01 (lib.rs:767)
-    if (xk_host.empty()) fetch_xk();
     y0.resize(n * dims);
-    for (uint64_t i = 0; i < n; ++i)
-      for (uint64_t c = 0; c < dims; ++c) y0[i * dims + c] = xk_host[i * dk + c] * 0.01f;
+    PM_HIP(hipMemcpy2DAsync(y0.data(), dims * sizeof(float), d_xk.get(), dk * sizeof(float),
+                            dims * sizeof(float), n, hipMemcpyDeviceToHost, s));
+    PM_HIP(hipStreamSynchronize(s));
+    for (auto &e : y0) e = e * 0.01f;
   } else if (!have_y) {
     if (init == PACMAP_INIT_VALUE)
       value_init(cfg->init_value, n, dims, y0);
@@ -727,20 +779,17 @@ int pacmap_grad(const float *y, uint64_t n, uint64_t dims, const uint32_t *pair_
     if ((n_nn && !pair_nn) || (n_mn && !pair_mn) || (n_fp && !pair_fp))
       fail(PACMAP_ERR_INVALID_ARGUMENT, "NULL pair array");
     PairList lists[3] = {{pair_nn, n_nn}, {pair_mn, n_mn}, {pair_fp, n_fp}};
-    Csr csr;
-    build_csr(n, lists, csr);
     Stream stream;
     const hipStream_t s = stream.get();
-    DevBuf<uint32_t> d_rp(n + 1), d_ent(std::max<size_t>(1, csr.ent.size()));
-    upload(d_rp, csr.row_ptr.data(), n + 1, s);
-    upload(d_ent, csr.ent.data(), csr.ent.size(), s);
+    DevCsr csr;
+    build_csr_from_host(n, lists, csr, s);
     DevBuf<float> dy(n * dims), dg(n * dims);
     upload(dy, y, n * dims, s);
     const uint32_t nb = grad_blocks(static_cast<uint32_t>(n));
     DevBuf<float> part(nb), loss(1);
     GradArgs a;
-    a.row_ptr = d_rp.get();
-    a.ent = d_ent.get();
+    a.row_ptr = csr.row_ptr.get();
+    a.ent = csr.ent.get();
     a.y_in = dy.get();
     a.grad_out = dg.get();
     a.loss_part = part.get();
@@ -811,6 +860,34 @@ int pacmap_sigma_scale(const float *knn_dist, const uint32_t *knn_idx, uint64_t 
   });
 }
 
+int pacmap_sample_pairs(const float *x, uint64_t n, uint64_t d, const uint32_t *pair_nn,
+                        uint64_t n_neighbors, uint64_t n_mn, uint64_t n_fp, uint64_t seed,
+                        uint32_t *pair_mn_out, uint32_t *pair_fp_out, pacmap_status *st) {
+  return guarded(st, [&] {
+    if (n == 0) return;
+    if (n >= kMaxPoints) fail(PACMAP_ERR_UNSUPPORTED, "too many points");
+    if ((n * d && !x) || (n * n_neighbors && !pair_nn) || (n * n_mn && !pair_mn_out) ||
+        (n * n_fp && !pair_fp_out))
+      fail(PACMAP_ERR_INVALID_ARGUMENT, "NULL array");
+    for (uint64_t r = 0; r < 2 * n * n_neighbors; ++r)
+      if (pair_nn[r] >= n) fail(PACMAP_ERR_INVALID_PAIR_INDEX, "pair_nn index out of range");
+    Stream stream;
+    const hipStream_t s = stream.get();
+    DevBuf<float> dx(std::max<uint64_t>(1, n * d));
+    DevBuf<uint32_t> dnn(std::max<uint64_t>(2, 2 * n * n_neighbors)), dmn(std::max<uint64_t>(2, 2 * n * n_mn)),
+        dfp(std::max<uint64_t>(2, 2 * n * n_fp));
+    upload(dx, x, n * d, s);
+    upload(dnn, pair_nn, 2 * n * n_neighbors, s);
+    PM_HIP(launch_fill_zero(dfp.get(), dfp.bytes(), s));
+    PM_HIP(launch_sample_pairs(dx.get(), static_cast<uint32_t>(n), static_cast<uint32_t>(d), dnn.get(),
+                               static_cast<uint32_t>(n_neighbors), static_cast<uint32_t>(n_mn),
+                               static_cast<uint32_t>(n_fp), seed, 1, dmn.get(), dfp.get(), s));
+    download(pair_mn_out, dmn.get(), 2 * n * n_mn, s);
+    download(pair_fp_out, dfp.get(), 2 * n * n_fp, s);
+    PM_HIP(hipStreamSynchronize(s));
+  });
+}
+
 int pacmap_optimize(float *y, uint64_t n, uint64_t dims, const uint32_t *pair_nn, uint64_t n_nn,
                     const uint32_t *pair_mn, uint64_t n_mn, const uint32_t *pair_fp, uint64_t n_fp,
                     float lr, const uint64_t num_iters[3], const uint64_t *snapshots,
@@ -826,11 +903,10 @@ int pacmap_optimize(float *y, uint64_t n, uint64_t dims, const uint32_t *pair_nn
     if (num_snapshots && !snapshots) fail(PACMAP_ERR_INVALID_ARGUMENT, "snapshots is NULL");
     const double t0 = now_ms();
     PairList lists[3] = {{pair_nn, n_nn}, {pair_mn, n_mn}, {pair_fp, n_fp}};
-    Csr csr;
-    build_csr(n, lists, csr);
-    g_tm.csr_ms += now_ms() - t0;
-    std::vector<float> yv(y, y + n * dims);
     Stream stream;
+    DevCsr csr;
+    build_csr_from_host(n, lists, csr, stream.get());
+    std::vector<float> yv(y, y + n * dims);
     optimize_device(csr, n, dims, yv, lr, num_iters, snapshots, num_snapshots, snapshots_out, loss_out,
                     loss_out != nullptr, stream.get());
     std::memcpy(y, yv.data(), sizeof(float) * n * dims);

@@ -34,6 +34,28 @@ hipError_t launch_knn_exact(const float *xp, uint64_t n, const KnnLayout &L, uin
                             uint32_t *idx, float *dist, hipStream_t s,
                             const uint32_t *qlist = nullptr, uint32_t nq = 0);
 
+// Pair sampling on the GPU (pairs.hip); pair arrays are rows x 2 u32 and
+// pair_fp must be zeroed by the caller (unfilled far pairs stay (0, 0)).
+hipError_t launch_sample_pairs(const float *x, uint32_t n, uint32_t d, const uint32_t *pair_nn,
+                               uint32_t knn, uint32_t n_mn, uint32_t n_fp, uint64_t seed, int seeded,
+                               uint32_t *pair_mn, uint32_t *pair_fp, hipStream_t s);
+// Symmetrised CSR on the GPU: count degrees (flags out-of-range indices in
+// bad[0]), scan to row offsets, fill, and sort each row by entry value (rows
+// longer than 2048 entries set too_long[0] and stay unsorted).
+hipError_t launch_csr_count(const uint32_t *const p[3], const uint64_t rows[3], uint32_t n,
+                            uint32_t *deg, uint32_t *bad, hipStream_t s);
+hipError_t launch_csr_scan(const uint32_t *deg, uint32_t n, uint32_t *row_ptr, uint32_t *cursor,
+                           uint32_t *scratch /* (n + 1023) / 1024 + 1 */, hipStream_t s);
+hipError_t launch_csr_fill_sort(const uint32_t *const p[3], const uint64_t rows[3], uint32_t n,
+                                const uint32_t *row_ptr, uint32_t *cursor, uint32_t *ent,
+                                uint32_t *too_long, hipStream_t s);
+
+// Exact rescan of the nq listed rows with the candidate range split over S
+// blocks per row; part holds nq x S x k partial keys.
+hipError_t launch_knn_rescan(const float *xp, uint64_t n, const KnnLayout &L, uint32_t k,
+                             const uint32_t *qlist, uint32_t nq, uint64_t *part, uint32_t S,
+                             uint32_t *idx, float *dist, hipStream_t s);
+
 // MFMA candidate path (knn_mfma.hip)
 hipError_t launch_knn_frag(const float *x, uint64_t n, uint32_t d, const double *mean, uint32_t kch,
                            uint64_t ntiles, void *fhi, void *flo, hipStream_t s);

@@ -135,7 +135,8 @@ __global__ __launch_bounds__(256) void knn_exact_kernel(const float *__restrict_
                                                         uint32_t *__restrict__ oidx,
                                                         float *__restrict__ odist,
                                                         const uint32_t *__restrict__ qlist,
-                                                        uint32_t nq) {
+                                                        uint32_t nq, uint32_t cchunk,
+                                                        uint64_t *__restrict__ part) {
   constexpr int QB = 16 * TQ, CB = 64, KC = static_cast<int>(kKnnChunk), R = KCAP / 64;
   __shared__ __align__(16) float sQ[KC][QB];
   __shared__ __align__(16) float sC[KC][CB];
@@ -160,7 +161,10 @@ __global__ __launch_bounds__(256) void knn_exact_kernel(const float *__restrict_
   }
   __syncthreads();
 
-  for (uint32_t cbase = 0; cbase < n; cbase += CB) {
+  // candidate range of this block (blockIdx.y splits the rows 0..n-1)
+  const uint32_t c_begin = blockIdx.y * cchunk;
+  const uint32_t c_end = (c_begin + cchunk < n) ? c_begin + cchunk : n;
+  for (uint32_t cbase = c_begin; cbase < c_end; cbase += CB) {
     float s[TQ][4], acc[TQ][4];
 #pragma unroll
     for (int i = 0; i < TQ; ++i)
@@ -256,7 +260,7 @@ __global__ __launch_bounds__(256) void knn_exact_kernel(const float *__restrict_
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
         const uint32_t c = cbase + tc * 4 + j;
-        if (c >= n || c == q) continue;
+        if (c >= c_end || c == q) continue;
         const float dd = sqrtf(s[i][j]);  // distance.rs:74
         const uint64_t key = (static_cast<uint64_t>(tkey(dd)) << 32) | c;
         if (key < tau) {
@@ -304,6 +308,15 @@ __global__ __launch_bounds__(256) void knn_exact_kernel(const float *__restrict_
       v[r] = e < cnt ? sBuf[ql][e] : ~0ull;
     }
     wave_bitonic<R>(v, lane);
+    if (part) {  // partial list of this candidate range, merged by knn_merge_kernel
+      const size_t base = (static_cast<size_t>(qbase + ql) * gridDim.y + blockIdx.y) * k;
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const uint32_t e = r * 64 + lane;
+        if (e < k) part[base + e] = v[r];
+      }
+      continue;
+    }
 #pragma unroll
     for (int r = 0; r < R; ++r) {
       const uint32_t e = r * 64 + lane;
@@ -315,6 +328,108 @@ __global__ __launch_bounds__(256) void knn_exact_kernel(const float *__restrict_
   }
 }
 
+// Final bitonic merge stage over 64*R keys (input bitonic, output ascending).
+template <int R>
+__device__ __forceinline__ void wave_bitonic_last(uint64_t (&v)[R], int lane) {
+  constexpr int N = 64 * R;
+#pragma unroll
+  for (int stride = N >> 1; stride > 0; stride >>= 1) {
+    if (stride >= 64) {
+      const int rs = stride >> 6;
+#pragma unroll
+      for (int r = 0; r < R; ++r)
+        if ((r & rs) == 0) {
+          const uint64_t a = v[r], b = v[r | rs];
+          v[r] = a < b ? a : b;
+          v[r | rs] = a < b ? b : a;
+        }
+    } else {
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const uint64_t o = shfl_xor_u64(v[r], stride);
+        const bool lower = (lane & stride) == 0;
+        v[r] = lower ? (v[r] < o ? v[r] : o) : (v[r] < o ? o : v[r]);
+      }
+    }
+  }
+}
+
+// Merges the S sorted partial lists (k keys each) of one listed query row
+// into its first k (one wavefront per row): the running list A and each
+// sorted chunk B give min(A[i], B[N-1-i]), a bitonic sequence holding the N
+// smallest of both, which the last bitonic stage sorts.
+template <int R>
+__global__ __launch_bounds__(256) void knn_merge_kernel(const uint64_t *__restrict__ part,
+                                                        uint32_t nq, uint32_t S, uint32_t k,
+                                                        const uint32_t *__restrict__ qlist,
+                                                        uint32_t *__restrict__ oidx,
+                                                        float *__restrict__ odist) {
+  const int lane = threadIdx.x & 63;
+  const uint32_t slot = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (slot >= nq) return;
+  constexpr uint32_t N = 64 * R;
+  const uint64_t *src = part + static_cast<size_t>(slot) * S * k;
+  const uint32_t total = S * k;
+  uint64_t run[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) run[r] = ~0ull;
+  for (uint32_t c0 = 0; c0 < total; c0 += N) {
+    uint64_t b[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const uint32_t e = c0 + r * 64 + lane;
+      b[r] = e < total ? src[e] : ~0ull;
+    }
+    wave_bitonic<R>(b, lane);
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const uint64_t rev = shfl_xor_u64(b[R - 1 - r], 63);  // element N-1-i
+      run[r] = run[r] < rev ? run[r] : rev;
+    }
+    wave_bitonic_last<R>(run, lane);
+  }
+  const uint32_t q = qlist[slot];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const uint32_t e = r * 64 + lane;
+    if (e < k) {
+      oidx[static_cast<size_t>(q) * k + e] = static_cast<uint32_t>(run[r]);
+      odist[static_cast<size_t>(q) * k + e] = from_tkey(static_cast<uint32_t>(run[r] >> 32));
+    }
+  }
+}
+
+hipError_t launch_knn_rescan(const float *xp, uint64_t n, const KnnLayout &L, uint32_t k,
+                             const uint32_t *qlist, uint32_t nq, uint64_t *part, uint32_t S,
+                             uint32_t *idx, float *dist, hipStream_t s) {
+  if (nq == 0 || k == 0) return hipSuccess;
+  const uint32_t nn = static_cast<uint32_t>(n);
+  const uint32_t cchunk = ((nn + S - 1) / S + 63) / 64 * 64;
+  const uint32_t blocks = (nq + 15) / 16;
+  dim3 g(blocks, S);
+  // 16 query rows per block keep the partial buffer indexable by list slot
+  if (k <= 64)
+    knn_exact_kernel<1, 128><<<g, 256, 0, s>>>(xp, nn, L.Dp, L.Cp, k, idx, dist, qlist, nq, cchunk, part);
+  else if (k <= 192)
+    knn_exact_kernel<1, 256><<<g, 256, 0, s>>>(xp, nn, L.Dp, L.Cp, k, idx, dist, qlist, nq, cchunk, part);
+  else if (k <= kKnnMaxK)
+    knn_exact_kernel<1, 512><<<g, 256, 0, s>>>(xp, nn, L.Dp, L.Cp, k, idx, dist, qlist, nq, cchunk, part);
+  else
+    return hipErrorInvalidValue;
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  const uint32_t mb = (nq + 3) / 4;
+  if (k <= 64)
+    knn_merge_kernel<1><<<mb, 256, 0, s>>>(part, nq, S, k, qlist, idx, dist);
+  else if (k <= 128)
+    knn_merge_kernel<2><<<mb, 256, 0, s>>>(part, nq, S, k, qlist, idx, dist);
+  else if (k <= 256)
+    knn_merge_kernel<4><<<mb, 256, 0, s>>>(part, nq, S, k, qlist, idx, dist);
+  else
+    knn_merge_kernel<8><<<mb, 256, 0, s>>>(part, nq, S, k, qlist, idx, dist);
+  return hipGetLastError();
+}
+
 hipError_t launch_knn_exact(const float *xp, uint64_t n, const KnnLayout &L, uint32_t k,
                             uint32_t *idx, float *dist, hipStream_t s, const uint32_t *qlist,
                             uint32_t nq) {
@@ -323,11 +438,11 @@ hipError_t launch_knn_exact(const float *xp, uint64_t n, const KnnLayout &L, uin
   const uint32_t rows = qlist ? nq : nn;
   if (rows == 0) return hipSuccess;
   if (k <= 64) {
-    knn_exact_kernel<4, 128><<<(rows + 63) / 64, 256, 0, s>>>(xp, nn, L.Dp, L.Cp, k, idx, dist, qlist, nq);
+    knn_exact_kernel<4, 128><<<(rows + 63) / 64, 256, 0, s>>>(xp, nn, L.Dp, L.Cp, k, idx, dist, qlist, nq, (nn + 63) / 64 * 64, nullptr);
   } else if (k <= 192) {
-    knn_exact_kernel<2, 256><<<(rows + 31) / 32, 256, 0, s>>>(xp, nn, L.Dp, L.Cp, k, idx, dist, qlist, nq);
+    knn_exact_kernel<2, 256><<<(rows + 31) / 32, 256, 0, s>>>(xp, nn, L.Dp, L.Cp, k, idx, dist, qlist, nq, (nn + 63) / 64 * 64, nullptr);
   } else if (k <= kKnnMaxK) {
-    knn_exact_kernel<1, 512><<<(rows + 15) / 16, 256, 0, s>>>(xp, nn, L.Dp, L.Cp, k, idx, dist, qlist, nq);
+    knn_exact_kernel<1, 512><<<(rows + 15) / 16, 256, 0, s>>>(xp, nn, L.Dp, L.Cp, k, idx, dist, qlist, nq, (nn + 63) / 64 * 64, nullptr);
   } else {
     return hipErrorInvalidValue;
   }
@@ -445,32 +560,52 @@ __global__ __launch_bounds__(256) void grad_adam_kernel(const GradArgs a) {
 #pragma unroll
     for (int d = 0; d < DIMS; ++d) yr[d] = a.y_in[static_cast<size_t>(row) * DIMS + d];
     const uint32_t beg = a.row_ptr[row], end = a.row_ptr[row + 1];
-    for (uint32_t e = beg + gl; e < end; e += 16) {
-      const uint32_t w = a.ent[e];
-      const uint32_t o = w & kIdxMask;
-      const uint32_t ty = w >> 30;
-      float u[DIMS];
-      float D = 1.0f;  // gradient.rs:121-125
+    // lane gl owns entries beg+gl, beg+gl+16, ...; four of them are loaded
+    // (entry word, then the other point's y) before any is used, so the
+    // gathers overlap; they are then consumed in the same order
+    constexpr int U = 4;
+    for (uint32_t e0 = beg + gl; e0 < end; e0 += 16 * U) {
+      uint32_t wv[U];
+      float yo[U][DIMS];
 #pragma unroll
-      for (int d = 0; d < DIMS; ++d) {
-        u[d] = yr[d] - a.y_in[static_cast<size_t>(o) * DIMS + d];
-        const float sq = u[d] * u[d];
-        D = D + sq;
+      for (int uu = 0; uu < U; ++uu) {
+        const uint32_t e = e0 + 16u * uu;
+        wv[uu] = e < end ? a.ent[e] : 0xffffffffu;
       }
-      const bool fp = ty == 2u;
-      const float c = fp ? 1.0f : (ty == 0u ? 10.0f : 10000.0f);
-      const float c2 = fp ? 2.0f : (ty == 0u ? 20.0f : 20000.0f);
-      const float wt = fp ? a.w[2] : (ty == 0u ? a.w[0] : a.w[1]);
-      const float cd = c + D;
-      const float w1 = wt * (c2 / (cd * cd));  // gradient.rs:130 / 140
 #pragma unroll
-      for (int d = 0; d < DIMS; ++d) {
-        const float tt = w1 * u[d];
-        g[d] = fp ? g[d] - tt : g[d] + tt;
+      for (int uu = 0; uu < U; ++uu) {
+        const uint32_t o = wv[uu] == 0xffffffffu ? row : (wv[uu] & kIdxMask);
+#pragma unroll
+        for (int d = 0; d < DIMS; ++d) yo[uu][d] = a.y_in[static_cast<size_t>(o) * DIMS + d];
       }
-      if (LOSS && (w & kOwnerBit)) {
-        const float num = fp ? 1.0f : D;  // gradient.rs:129 / 139
-        loss = loss + wt * (num / cd);
+#pragma unroll
+      for (int uu = 0; uu < U; ++uu) {
+        const uint32_t w = wv[uu];
+        if (w == 0xffffffffu) break;
+        const uint32_t ty = w >> 30;
+        float u[DIMS];
+        float D = 1.0f;  // gradient.rs:121-125
+#pragma unroll
+        for (int d = 0; d < DIMS; ++d) {
+          u[d] = yr[d] - yo[uu][d];
+          const float sq = u[d] * u[d];
+          D = D + sq;
+        }
+        const bool fp = ty == 2u;
+        const float c = fp ? 1.0f : (ty == 0u ? 10.0f : 10000.0f);
+        const float c2 = fp ? 2.0f : (ty == 0u ? 20.0f : 20000.0f);
+        const float wt = fp ? a.w[2] : (ty == 0u ? a.w[0] : a.w[1]);
+        const float cd = c + D;
+        const float w1 = wt * (c2 / (cd * cd));  // gradient.rs:130 / 140
+#pragma unroll
+        for (int d = 0; d < DIMS; ++d) {
+          const float tt = w1 * u[d];
+          g[d] = fp ? g[d] - tt : g[d] + tt;
+        }
+        if (LOSS && (w & kOwnerBit)) {
+          const float num = fp ? 1.0f : D;  // gradient.rs:129 / 139
+          loss = loss + wt * (num / cd);
+        }
       }
     }
   }

@@ -1,0 +1,404 @@
+// pairs.hip -- mid-near / far pair sampling and the symmetrised CSR on the GPU.
+//
+// Sampling follows sampling.rs:37-142 and 296-313 with rand 0.8.5's SmallRng
+// (Xoshiro256++ seeded through rand_core 0.6's PCG32 seed_from_u64) and
+// UniformInt<u32>::sample_single (widening multiply, zone rejection), one
+// thread per point: point i's streams depend only on (seed, i), so every
+// point is independent.  MN distances use distance.rs:50-75's order.
+//
+// The CSR holds every non-self pair in both endpoint rows.  Rows are filled
+// with integer atomics and then each row is sorted by entry value, which
+// makes the layout (and so every row's summation order) independent of the
+// atomic order.
+#include "internal.h"
+
+#include <cstdint>
+
+namespace pm {
+
+struct DevRng {
+  uint64_t s[4];
+};
+
+__device__ __forceinline__ uint64_t rotl64d(uint64_t x, int k) { return (x << k) | (x >> (64 - k)); }
+
+__device__ inline DevRng rng_seed_from_u64(uint64_t state) {
+  uint32_t w[8];
+  for (int c = 0; c < 8; ++c) {
+    state = state * 6364136223846793005ull + 11634580027462260723ull;
+    const uint64_t st = state;
+    const uint32_t xs = static_cast<uint32_t>(((st >> 18) ^ st) >> 27);
+    const uint32_t rot = static_cast<uint32_t>(st >> 59);
+    w[c] = (xs >> rot) | (xs << ((32u - rot) & 31u));
+  }
+  DevRng r;
+  bool zero = true;
+  for (int c = 0; c < 8; ++c) zero = zero && w[c] == 0;
+  if (zero) {
+    uint64_t st = 0;
+    for (int q = 0; q < 4; ++q) {
+      st += 0x9e3779b97f4a7c15ull;
+      uint64_t z = st;
+      z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+      z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+      r.s[q] = z ^ (z >> 31);
+    }
+    return r;
+  }
+  for (int q = 0; q < 4; ++q) r.s[q] = static_cast<uint64_t>(w[2 * q]) | (static_cast<uint64_t>(w[2 * q + 1]) << 32);
+  return r;
+}
+
+__device__ __forceinline__ uint64_t rng_next_u64(DevRng &r) {
+  const uint64_t result = rotl64d(r.s[0] + r.s[3], 23) + r.s[0];
+  const uint64_t t = r.s[1] << 17;
+  r.s[2] ^= r.s[0];
+  r.s[3] ^= r.s[1];
+  r.s[1] ^= r.s[2];
+  r.s[0] ^= r.s[3];
+  r.s[2] ^= t;
+  r.s[3] = rotl64d(r.s[3], 45);
+  return result;
+}
+
+__device__ __forceinline__ uint32_t rng_gen_range(DevRng &r, uint32_t range) {
+  // [0, range), range >= 1
+  const uint32_t zone = (range << __clz(range)) - 1u;
+  for (;;) {
+    const uint32_t v = static_cast<uint32_t>(rng_next_u64(r) >> 32);
+    const uint64_t m = static_cast<uint64_t>(v) * range;
+    if (static_cast<uint32_t>(m) <= zone) return static_cast<uint32_t>(m >> 32);
+  }
+}
+
+// sampling.rs:37-61: n_samples distinct draws from [0, maximum) avoiding self
+// and the reject list (reject[q * stride], q < n_reject); returns the count.
+__device__ uint32_t dev_sample_fp(uint32_t n_samples, uint32_t maximum, const uint32_t *reject,
+                                  uint32_t n_reject, uint32_t stride, uint32_t self_ind, DevRng &rng,
+                                  uint32_t *out, uint32_t out_stride) {
+  bool self_absent = true;
+  for (uint32_t q = 0; q < n_reject; ++q)
+    if (reject[q * stride] == self_ind) self_absent = false;
+  uint64_t avail = maximum;
+  avail = avail > n_reject ? avail - n_reject : 0;
+  const uint32_t sa = self_absent ? 1u : 0u;
+  avail = avail > sa ? avail - sa : 0;
+  if (n_samples > avail) n_samples = static_cast<uint32_t>(avail);
+  uint32_t len = 0;
+  while (len < n_samples) {
+    const uint32_t j = rng_gen_range(rng, maximum);
+    if (j == self_ind) continue;
+    bool dup = false;
+    for (uint32_t q = 0; q < len && !dup; ++q) dup = out[q * out_stride] == j;
+    for (uint32_t q = 0; q < n_reject && !dup; ++q) dup = reject[q * stride] == j;
+    if (!dup) out[(len++) * out_stride] = j;
+  }
+  return len;
+}
+
+__device__ __forceinline__ uint32_t tkey_p(float f) {
+  const uint32_t b = __float_as_uint(f);
+  return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+}
+
+__device__ float euclid_ref_dev(const float *__restrict__ a, const float *__restrict__ b, uint32_t d) {
+  float acc[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+  const uint32_t chunks = d / 8;
+  for (uint32_t c = 0; c < chunks; ++c) {
+#pragma unroll
+    for (int l = 0; l < 8; ++l) {
+      const float df = a[8 * c + l] - b[8 * c + l];
+      const float sq = df * df;
+      acc[l] = acc[l] + sq;
+    }
+  }
+  float s = 0.0f;
+#pragma unroll
+  for (int l = 0; l < 8; ++l) s = s + acc[l];
+  for (uint32_t e = chunks * 8; e < d; ++e) {
+    const float df = a[e] - b[e];
+    const float sq = df * df;
+    s = s + sq;
+  }
+  return sqrtf(s);
+}
+
+// Seeds: seeded runs use seed + i (sampling.rs:97, 133); unseeded runs draw
+// one base seed per call and decorrelate points by an odd multiplier.
+__device__ __forceinline__ uint64_t point_seed(uint64_t base, int seeded, uint64_t i) {
+  return seeded ? base + i : (base ^ (0x9e3779b97f4a7c15ull * (i + 1)));
+}
+
+// sampling.rs:77-103: one thread per point; rows (i, j) of pair_fp, unfilled
+// rows stay (0, 0) (callers zero the array).
+__global__ void sample_fp_kernel(uint32_t n, const uint32_t *__restrict__ pair_nn, uint32_t knn,
+                                 uint32_t n_fp, uint64_t seed, int seeded,
+                                 uint32_t *__restrict__ pair_fp) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  DevRng rng = rng_seed_from_u64(point_seed(seed, seeded, i));
+  uint32_t *rows = pair_fp + static_cast<size_t>(2) * i * n_fp;
+  const uint32_t got = dev_sample_fp(n_fp, n, pair_nn + static_cast<size_t>(2) * i * knn + 1, knn, 2, i,
+                                     rng, rows + 1, 2);
+  for (uint32_t q = 0; q < got; ++q) rows[2 * q] = i;
+}
+
+// sampling.rs:118-142 + 296-313: for each j, six distinct draws avoiding self
+// and the earlier picks, then the second closest by distance (ties keep draw
+// order; unfilled slots are (0.0, 0)).
+__global__ void sample_mn_kernel(const float *__restrict__ x, uint32_t n, uint32_t d, uint32_t n_mn,
+                                 uint64_t seed, int seeded, uint32_t *__restrict__ pair_mn) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  DevRng rng = rng_seed_from_u64(point_seed(seed, seeded, i));
+  uint32_t *rows = pair_mn + static_cast<size_t>(2) * i * n_mn;
+  const float *xi = x + static_cast<size_t>(i) * d;
+  for (uint32_t j = 0; j < n_mn; ++j) {
+    uint32_t sampled[6];
+    const uint32_t got = dev_sample_fp(6, n, rows + 1, j, 2, i, rng, sampled, 1);
+    float dist[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    uint32_t idx[6] = {0, 0, 0, 0, 0, 0};
+    for (uint32_t q = 0; q < got; ++q) {
+      dist[q] = euclid_ref_dev(xi, x + static_cast<size_t>(sampled[q]) * d, d);
+      idx[q] = sampled[q];
+    }
+    for (int p = 1; p < 6; ++p) {  // insertion sort by total order: stable
+      const float td = dist[p];
+      const uint32_t ti = idx[p];
+      int c = p - 1;
+      while (c >= 0 && tkey_p(dist[c]) > tkey_p(td)) {
+        dist[c + 1] = dist[c];
+        idx[c + 1] = idx[c];
+        --c;
+      }
+      dist[c + 1] = td;
+      idx[c + 1] = ti;
+    }
+    rows[2 * j] = i;
+    rows[2 * j + 1] = idx[1];
+  }
+}
+
+hipError_t launch_sample_pairs(const float *x, uint32_t n, uint32_t d, const uint32_t *pair_nn,
+                               uint32_t knn, uint32_t n_mn, uint32_t n_fp, uint64_t seed, int seeded,
+                               uint32_t *pair_mn, uint32_t *pair_fp, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  const uint32_t blocks = (n + 63) / 64;
+  if (n_fp) sample_fp_kernel<<<blocks, 64, 0, s>>>(n, pair_nn, knn, n_fp, seed, seeded, pair_fp);
+  if (n_mn) sample_mn_kernel<<<blocks, 64, 0, s>>>(x, n, d, n_mn, seed, seeded, pair_mn);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// symmetrised CSR
+// ---------------------------------------------------------------------------
+struct PairArrays {
+  const uint32_t *p[3];
+  uint64_t rows[3];
+};
+
+__device__ __forceinline__ bool pair_at(const PairArrays &pa, uint64_t g, uint32_t &i, uint32_t &j,
+                                        uint32_t &type) {
+#pragma unroll
+  for (int t = 0; t < 3; ++t) {
+    if (g < pa.rows[t]) {
+      i = pa.p[t][2 * g];
+      j = pa.p[t][2 * g + 1];
+      type = static_cast<uint32_t>(t);
+      return true;
+    }
+    g -= pa.rows[t];
+  }
+  return false;
+}
+
+// degree count and index validation (bad pair -> flag, pair index kept)
+__global__ void csr_degree_kernel(PairArrays pa, uint64_t total, uint32_t n, uint32_t *__restrict__ deg,
+                                  uint32_t *__restrict__ bad) {
+  for (uint64_t g = blockIdx.x * 256ull + threadIdx.x; g < total; g += gridDim.x * 256ull) {
+    uint32_t i, j, t;
+    pair_at(pa, g, i, j, t);
+    if (i >= n || j >= n) {
+      atomicOr(bad, 1u);
+      continue;
+    }
+    if (i == j) continue;  // gradient.rs:116-118
+    atomicAdd(&deg[i], 1u);
+    atomicAdd(&deg[j], 1u);
+  }
+}
+
+__global__ void csr_fill_kernel(PairArrays pa, uint64_t total, uint32_t *__restrict__ cursor,
+                                uint32_t *__restrict__ ent) {
+  for (uint64_t g = blockIdx.x * 256ull + threadIdx.x; g < total; g += gridDim.x * 256ull) {
+    uint32_t i, j, t;
+    pair_at(pa, g, i, j, t);
+    if (i == j) continue;
+    const uint32_t tb = t << 30;
+    ent[atomicAdd(&cursor[i], 1u)] = j | tb | kOwnerBit;
+    ent[atomicAdd(&cursor[j], 1u)] = i | tb;
+  }
+}
+
+// exclusive scan of n counts (three phases, 1024 elements per block)
+__global__ __launch_bounds__(256) void scan_block_kernel(const uint32_t *__restrict__ in, uint32_t n,
+                                                         uint32_t *__restrict__ out,
+                                                         uint32_t *__restrict__ block_sums) {
+  __shared__ uint32_t sm[1024];
+  const uint32_t base = blockIdx.x * 1024;
+  for (int q = 0; q < 4; ++q) {
+    const uint32_t e = base + threadIdx.x * 4 + q;
+    sm[threadIdx.x * 4 + q] = e < n ? in[e] : 0u;
+  }
+  __syncthreads();
+  uint32_t local[4], run = 0;
+  for (int q = 0; q < 4; ++q) {
+    local[q] = run;
+    run += sm[threadIdx.x * 4 + q];
+  }
+  __shared__ uint32_t ts[256];
+  ts[threadIdx.x] = run;
+  __syncthreads();
+  for (int off = 1; off < 256; off <<= 1) {  // Hillis-Steele inclusive scan of thread totals
+    const uint32_t v = threadIdx.x >= static_cast<unsigned>(off) ? ts[threadIdx.x - off] : 0u;
+    __syncthreads();
+    ts[threadIdx.x] += v;
+    __syncthreads();
+  }
+  const uint32_t prefix = threadIdx.x ? ts[threadIdx.x - 1] : 0u;
+  for (int q = 0; q < 4; ++q) {
+    const uint32_t e = base + threadIdx.x * 4 + q;
+    if (e < n) out[e] = prefix + local[q];
+  }
+  if (threadIdx.x == 255) block_sums[blockIdx.x] = ts[255];
+}
+
+__global__ void scan_sums_kernel(uint32_t *__restrict__ sums, uint32_t nb, uint32_t *__restrict__ total) {
+  if (threadIdx.x != 0) return;
+  uint32_t run = 0;
+  for (uint32_t b = 0; b < nb; ++b) {
+    const uint32_t v = sums[b];
+    sums[b] = run;
+    run += v;
+  }
+  *total = run;
+}
+
+__global__ void scan_add_kernel(uint32_t *__restrict__ out, uint32_t n, const uint32_t *__restrict__ sums,
+                                const uint32_t *__restrict__ total, uint32_t *__restrict__ cursor) {
+  const uint32_t e = blockIdx.x * 256 + threadIdx.x;
+  if (e < n) {
+    out[e] += sums[e / 1024];
+    cursor[e] = out[e];
+  }
+  if (e == n) out[n] = *total;
+}
+
+template <int R>
+__device__ __forceinline__ void wave_sort_u32(uint32_t (&v)[R], int lane) {
+  constexpr int N = 64 * R;
+#pragma unroll
+  for (int size = 2; size <= N; size <<= 1) {
+#pragma unroll
+    for (int stride = size >> 1; stride > 0; stride >>= 1) {
+      if (stride >= 64) {
+        const int rs = stride >> 6;
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+          if ((r & rs) == 0) {
+            const bool asc = (((r * 64 + lane) & size) == 0);
+            const uint32_t a = v[r], b = v[r | rs];
+            const uint32_t lo = a < b ? a : b, hi = a < b ? b : a;
+            v[r] = asc ? lo : hi;
+            v[r | rs] = asc ? hi : lo;
+          }
+      } else {
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          const uint32_t o = __shfl_xor(v[r], stride, 64);
+          const bool asc = (((r * 64 + lane) & size) == 0);
+          const bool lower = (lane & stride) == 0;
+          const uint32_t lo = v[r] < o ? v[r] : o, hi = v[r] < o ? o : v[r];
+          v[r] = (lower == asc) ? lo : hi;
+        }
+      }
+    }
+  }
+}
+
+// one wavefront per row of length <= 64 * R (longer rows are left for the
+// next size class; rows above the largest class are reported)
+template <int R>
+__global__ __launch_bounds__(256) void csr_sort_rows_kernel(const uint32_t *__restrict__ row_ptr, uint32_t n,
+                                                            uint32_t min_len, uint32_t *__restrict__ ent,
+                                                            uint32_t *__restrict__ too_long) {
+  const int lane = threadIdx.x & 63;
+  const uint32_t row = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (row >= n) return;
+  const uint32_t b = row_ptr[row], len = row_ptr[row + 1] - b;
+  if (len <= min_len || len < 2) return;
+  if (len > 64u * R) {
+    if (R == 32 && lane == 0) atomicOr(too_long, 1u);
+    return;
+  }
+  uint32_t v[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const uint32_t e = r * 64 + lane;
+    v[r] = e < len ? ent[b + e] : 0xffffffffu;
+  }
+  wave_sort_u32<R>(v, lane);
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const uint32_t e = r * 64 + lane;
+    if (e < len) ent[b + e] = v[r];
+  }
+}
+
+static uint32_t grid_for(uint64_t total) {
+  uint64_t b = (total + 255) / 256;
+  if (b > 65536) b = 65536;
+  return static_cast<uint32_t>(b ? b : 1);
+}
+
+hipError_t launch_csr_count(const uint32_t *const p[3], const uint64_t rows[3], uint32_t n,
+                            uint32_t *deg, uint32_t *bad, hipStream_t s) {
+  PairArrays pa;
+  uint64_t total = 0;
+  for (int t = 0; t < 3; ++t) {
+    pa.p[t] = p[t];
+    pa.rows[t] = rows[t];
+    total += rows[t];
+  }
+  if (total) csr_degree_kernel<<<grid_for(total), 256, 0, s>>>(pa, total, n, deg, bad);
+  return hipGetLastError();
+}
+
+hipError_t launch_csr_scan(const uint32_t *deg, uint32_t n, uint32_t *row_ptr, uint32_t *cursor,
+                           uint32_t *scratch, hipStream_t s) {
+  const uint32_t nb = (n + 1023) / 1024;
+  scan_block_kernel<<<nb ? nb : 1, 256, 0, s>>>(deg, n, row_ptr, scratch);
+  scan_sums_kernel<<<1, 64, 0, s>>>(scratch, nb, scratch + nb);
+  scan_add_kernel<<<(n + 1 + 255) / 256, 256, 0, s>>>(row_ptr, n, scratch, scratch + nb, cursor);
+  return hipGetLastError();
+}
+
+hipError_t launch_csr_fill_sort(const uint32_t *const p[3], const uint64_t rows[3], uint32_t n,
+                                const uint32_t *row_ptr, uint32_t *cursor, uint32_t *ent,
+                                uint32_t *too_long, hipStream_t s) {
+  PairArrays pa;
+  uint64_t total = 0;
+  for (int t = 0; t < 3; ++t) {
+    pa.p[t] = p[t];
+    pa.rows[t] = rows[t];
+    total += rows[t];
+  }
+  if (total) csr_fill_kernel<<<grid_for(total), 256, 0, s>>>(pa, total, cursor, ent);
+  const uint32_t blocks = (n + 3) / 4;
+  csr_sort_rows_kernel<1><<<blocks, 256, 0, s>>>(row_ptr, n, 0, ent, too_long);
+  csr_sort_rows_kernel<4><<<blocks, 256, 0, s>>>(row_ptr, n, 64, ent, too_long);
+  csr_sort_rows_kernel<32><<<blocks, 256, 0, s>>>(row_ptr, n, 256, ent, too_long);
+  return hipGetLastError();
+}
+
+}  // namespace pm

@@ -109,6 +109,8 @@ def lib():
         L.pacmap_weights.argtypes = [C.c_float, _u64, _u64, _u64, _P]
         L.pacmap_weights.restype = None
         L.pacmap_sigma_scale.argtypes = [_P, _P, _u64, _u64, _u64, _P, _P, _P, C.POINTER(Status)]
+        L.pacmap_sample_pairs.argtypes = [_P, _u64, _u64, _P, _u64, _u64, _u64, _u64, _P, _P,
+                                          C.POINTER(Status)]
         L.pacmap_optimize.argtypes = [_P, _u64, _u64, _P, _u64, _P, _u64, _P, _u64, C.c_float,
                                       _P, _P, _u64, _P, _P, C.POINTER(Status)]
         L.pacmap_dev_alloc.argtypes = [_u64, C.POINTER(C.c_void_p), C.POINTER(Status)]
@@ -419,6 +421,20 @@ def sigma_scale(knn_dist, knn_idx, n_neighbors=0):
                                   pairs.ctypes.data if n_neighbors else None, C.byref(st))
     _check(rc, st)
     return sig, sc, pairs
+
+
+def sample_pairs(x, pair_neighbors, n_neighbors, n_mn, n_fp, seed):
+    """Seam for the seeded MN/FP samplers (sampling.rs:77-142): (pair_mn, pair_fp)."""
+    x = np.ascontiguousarray(x, np.float32)
+    n, d = x.shape
+    pn = np.ascontiguousarray(pair_neighbors, np.uint32)
+    mn = np.zeros((n * n_mn, 2), np.uint32)
+    fp = np.zeros((n * n_fp, 2), np.uint32)
+    st = Status()
+    rc = lib().pacmap_sample_pairs(x.ctypes.data, n, d, pn.ctypes.data, n_neighbors, n_mn, n_fp,
+                                   seed, mn.ctypes.data, fp.ctypes.data, C.byref(st))
+    _check(rc, st)
+    return mn, fp
 
 
 def optimize(y0, pair_neighbors, pair_mn, pair_fp, lr=1.0, num_iters=(100, 100, 250),
