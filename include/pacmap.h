@@ -196,6 +196,7 @@ typedef struct {
   double knn_rerank_ms;        /* exact rerank + certificate kernel */
   double knn_fallback_ms;      /* exact rescan of uncertified rows */
   uint64_t knn_path;           /* 0 direct, 1 split-bf16 MFMA, 2 bf16 MFMA */
+  double knn_sample_ms;        /* MFMA threshold-sampling kernel */
 } pacmap_timings;
 void pacmap_last_timings(pacmap_timings *out);
 

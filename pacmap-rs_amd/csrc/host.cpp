@@ -116,23 +116,34 @@ bool knn_mfma_path(const float *x_dev, uint64_t n, uint64_t d, uint32_t k, const
   if (hflags[0]) return false;
   const uint32_t ss = std::max<uint32_t>(1, (3 * k + 15) / 16);
   const uint32_t capg = (3.0 * k * 2.5 <= 512.0) ? 512 : 1024;
-  DevBuf<uint32_t> gcnt(n), gcand(n * capg), fb_list(n);
+  // split the candidate range until there are enough (group, split) blocks
+  // to keep every CU busy with a short tail
+  const uint32_t groups = knn_cand_groups(static_cast<uint32_t>(ntiles));
+  const uint32_t nsplit = std::min<uint32_t>(16, std::max<uint32_t>(1, (4096 + groups - 1) / groups));
+  DevBuf<uint32_t> gcnt(n * nsplit), gcand(n * nsplit * capg), fb_list(n);
   DevBuf<float> gtau(n);
-  Timer tk, tr;
+  const void *flo_p = split ? flo.get() : fhi.get();
+  Timer tsamp, tk, tr;
+  tsamp.start(s);
+  PM_HIP(launch_knn_cand(0, fhi.get(), flo_p, arow.get(), static_cast<uint32_t>(n), kch,
+                         static_cast<uint32_t>(ntiles), ss, capg, nsplit, gcnt.get(), gcand.get(),
+                         gtau.get(), split, s));
+  tsamp.stop(s);
   tk.start(s);
-  PM_HIP(launch_knn_cand(fhi.get(), split ? flo.get() : fhi.get(), arow.get(), static_cast<uint32_t>(n),
-                         kch, static_cast<uint32_t>(ntiles), ss, capg, gcnt.get(), gcand.get(),
+  PM_HIP(launch_knn_cand(1, fhi.get(), flo_p, arow.get(), static_cast<uint32_t>(n), kch,
+                         static_cast<uint32_t>(ntiles), ss, capg, nsplit, gcnt.get(), gcand.get(),
                          gtau.get(), split, s));
   tk.stop(s);
   tr.start(s);
-  PM_HIP(launch_knn_rerank(xp, L, static_cast<uint32_t>(n), k, capg, gcnt.get(), gcand.get(), gtau.get(),
-                           knn_cert_scale(static_cast<uint32_t>(d)), idx.get(), dist.get(),
+  PM_HIP(launch_knn_rerank(xp, L, static_cast<uint32_t>(n), k, capg, nsplit, gcnt.get(), gcand.get(),
+                           gtau.get(), knn_cert_scale(static_cast<uint32_t>(d)), idx.get(), dist.get(),
                            flags.get() + 1, fb_list.get(), s));
   tr.stop(s);
   download(hflags, flags.get(), 2, s);
   PM_HIP(hipStreamSynchronize(s));
   g_tm.knn_kernel_ms += tk.ms();
   g_tm.knn_kernel_launches += 1;
+  g_tm.knn_sample_ms += tsamp.ms();
   g_tm.knn_rerank_ms += tr.ms();
   g_tm.knn_candidates = capg;
   g_tm.knn_path = split ? 1 : 2;
@@ -681,10 +692,11 @@ void fit_impl(const float *x_host, const float *x_dev_in, uint64_t n, uint64_t d
   // --- YInit (lib.rs:752-779) ---
   if (pca_path) {  // YInit::Preprocessed: x[:, :dims] * 0.01 (lib.rs:767)
     y0.resize(n * dims);
-    PM_HIP(hipMemcpy2DAsync(y0.data(), dims * sizeof(float), d_xk.get(), dk * sizeof(float),
-                            dims * sizeof(float), n, hipMemcpyDeviceToHost, s));
+    DevBuf<float> cols(n * dims);
+    PM_HIP(launch_take_cols(d_xk.get(), n, static_cast<uint32_t>(dk), static_cast<uint32_t>(dims), 0.01f,
+                            cols.get(), s));
+    download(y0.data(), cols.get(), n * dims, s);
     PM_HIP(hipStreamSynchronize(s));
-    for (auto &e : y0) e = e * 0.01f;
   } else if (!have_y) {
     if (init == PACMAP_INIT_VALUE)
       value_init(cfg->init_value, n, dims, y0);

@@ -62,13 +62,19 @@ hipError_t launch_knn_frag(const float *x, uint64_t n, uint32_t d, const double 
 hipError_t launch_knn_rownorm(const float *x, uint64_t n, uint32_t d, const double *mean,
                               float one_minus_kappa, uint64_t rows_padded, float *arow,
                               uint32_t *nonfinite, hipStream_t s);
-hipError_t launch_knn_cand(const void *fhi, const void *flo, const float *arow, uint32_t n,
-                           uint32_t kch, uint32_t ntiles, uint32_t ss, uint32_t capg,
-                           uint32_t *gcnt, uint32_t *gcand, float *gtau, bool split, hipStream_t s);
+// phase 0: tau per row from a 1/ss tile sample; phase 1: append candidates
+// with lb <= tau to per-(row, split) lists (gcnt: n x nsplit, gcand:
+// n x nsplit x capg).  Query tiles are grouped knn_cand_groups(ntiles) ways.
+uint32_t knn_cand_groups(uint32_t ntiles);
+hipError_t launch_knn_cand(int phase, const void *fhi, const void *flo, const float *arow,
+                           uint32_t n, uint32_t kch, uint32_t ntiles, uint32_t ss, uint32_t capg,
+                           uint32_t nsplit, uint32_t *gcnt, uint32_t *gcand, float *gtau,
+                           bool split, hipStream_t s);
 hipError_t launch_knn_rerank(const float *xp, const KnnLayout &L, uint32_t n, uint32_t k,
-                             uint32_t capg, const uint32_t *gcnt, const uint32_t *gcand,
-                             const float *gtau, double cert_scale, uint32_t *idx, float *dist,
-                             uint32_t *fb_count, uint32_t *fb_list, hipStream_t s);
+                             uint32_t capg, uint32_t nsplit, const uint32_t *gcnt,
+                             const uint32_t *gcand, const float *gtau, double cert_scale,
+                             uint32_t *idx, float *dist, uint32_t *fb_count, uint32_t *fb_list,
+                             hipStream_t s);
 
 hipError_t launch_sigma(const float *dist, uint64_t n, uint32_t kp, float *sig, hipStream_t s);
 hipError_t launch_scale_nn(const float *dist, const uint32_t *idx, const float *sig, uint64_t n,
@@ -122,6 +128,9 @@ hipError_t launch_project(const float *x, uint64_t n, uint32_t d, const float *m
 hipError_t launch_adam(float *y, const float *g, float *m, float *v, uint64_t count, float lr_t,
                        float omb1, float omb2, hipStream_t s);
 hipError_t launch_fill_zero(void *p, uint64_t bytes, hipStream_t s);
+// dst (n x cols) = src[:, :cols] * scale, src with row stride ld
+hipError_t launch_take_cols(const float *src, uint64_t n, uint32_t ld, uint32_t cols, float scale,
+                            float *dst, hipStream_t s);
 hipError_t launch_gemm_f64(uint32_t M, uint32_t N, uint32_t K, const double *A, uint32_t lda,
                            bool tA, const double *B, uint32_t ldb, bool tB, double *C, uint32_t ldc,
                            hipStream_t s);

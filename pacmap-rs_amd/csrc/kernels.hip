@@ -1015,6 +1015,25 @@ __global__ void fill_zero_kernel(uint32_t *__restrict__ p, uint64_t words) {
   for (uint64_t e = blockIdx.x * 256ull + threadIdx.x; e < words; e += gridDim.x * 256ull) p[e] = 0u;
 }
 
+// dst (n x cols) = src[:, :cols] * scale (src has row stride ld)
+__global__ void take_cols_kernel(const float *__restrict__ src, uint64_t n, uint32_t ld, uint32_t cols,
+                                 float scale, float *__restrict__ dst) {
+  const uint64_t total = n * cols;
+  for (uint64_t e = blockIdx.x * 256ull + threadIdx.x; e < total; e += gridDim.x * 256ull) {
+    const uint64_t r = e / cols;
+    dst[e] = src[r * ld + (e - r * cols)] * scale;
+  }
+}
+
+hipError_t launch_take_cols(const float *src, uint64_t n, uint32_t ld, uint32_t cols, float scale,
+                            float *dst, hipStream_t s) {
+  uint64_t blocks = (n * cols + 255) / 256;
+  if (blocks > 65536) blocks = 65536;
+  if (blocks == 0) return hipSuccess;
+  take_cols_kernel<<<static_cast<uint32_t>(blocks), 256, 0, s>>>(src, n, ld, cols, scale, dst);
+  return hipGetLastError();
+}
+
 hipError_t launch_fill_zero(void *p, uint64_t bytes, hipStream_t s) {
   const uint64_t words = bytes / 4;
   uint64_t blocks = (words + 255) / 256;

@@ -182,23 +182,31 @@ __global__ void knn_rownorm_kernel(const float *__restrict__ x, uint64_t n, uint
 // ---------------------------------------------------------------------------
 // Block = 4 waves; each wave owns RT query tiles (16 RT rows) with their A
 // fragments in registers and streams candidate tiles staged in LDS (CT tiles
-// per stage, shared by the 4 waves).  Lane l holds, per query tile rt and
-// r = 0..3, the value for query row 16 rt + 4 (l >> 4) + r and candidate
-// 16 t + (l & 15).
-template <int KCH, int RT, bool SPLIT>
-__global__ __launch_bounds__(256) void knn_mfma_cand_kernel(
+// per stage, shared by the 4 waves, double buffered: the next stage is
+// fetched into registers while the current one is multiplied, so one barrier
+// per stage remains).  Lane l holds, per query tile rt and r = 0..3, the
+// value for query row 16 rt + 4 (l >> 4) + r and candidate 16 t + (l & 15).
+//
+// PHASE 0 scans a 1/ss sample of candidate tiles and writes tau per row.
+// PHASE 1 scans the candidate tiles of split blockIdx.y (of gridDim.y) and
+// appends every candidate with lb <= tau to the row's list of that split.
+template <int KCH, int RT, bool SPLIT, int PHASE>
+__global__ __launch_bounds__(256) void knn_mfma_kernel(
     const uint4 *__restrict__ fhi, const uint4 *__restrict__ flo, const float *__restrict__ arow,
     uint32_t n, uint32_t ntiles, uint32_t ss, uint32_t capg, uint32_t *__restrict__ gcnt,
     uint32_t *__restrict__ gcand, float *__restrict__ gtau) {
-  constexpr int CT = (16 / KCH) > 0 ? (16 / KCH) : 1;  // candidate tiles per LDS stage
+  constexpr int CT = (8 / KCH) > 0 ? (8 / KCH) : 1;  // candidate tiles per LDS stage
   constexpr int NH = SPLIT ? 2 : 1;
-  __shared__ uint4 sB[CT][KCH][NH][64];
-  __shared__ float sA[CT][16];
+  constexpr int PER = CT * KCH * NH * 64;            // uint4 per stage
+  constexpr int PF = (PER + 255) / 256;              // prefetch registers per thread
+  __shared__ uint4 sB[2][CT][KCH][NH][64];
+  __shared__ float sA[2][CT][16];
   __shared__ uint32_t sCnt[4][16 * RT];
 
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const uint32_t qtile0 = (blockIdx.x * 4 + wave) * RT;
   const int rsub = lane >> 4, col = lane & 15;
+  const uint32_t split = PHASE == 0 ? 0 : blockIdx.y, nsplit = PHASE == 0 ? 1 : gridDim.y;
 
   bf16x8 ahi[RT][KCH], alo[RT][KCH];
   float aq[RT][4];
@@ -225,7 +233,6 @@ __global__ __launch_bounds__(256) void knn_mfma_cand_kernel(
   }
   for (int i = threadIdx.x; i < 4 * 16 * RT; i += 256) (&sCnt[0][0])[i] = 0;
 
-  // per-lane two smallest sampled bounds per query row (phase 1)
   float best0[RT][4], best1[RT][4], tau[RT][4];
 #pragma unroll
   for (int rt = 0; rt < RT; ++rt)
@@ -233,103 +240,131 @@ __global__ __launch_bounds__(256) void knn_mfma_cand_kernel(
     for (int r = 0; r < 4; ++r) {
       best0[rt][r] = __builtin_huge_valf();
       best1[rt][r] = __builtin_huge_valf();
-      tau[rt][r] = __builtin_huge_valf();
+      const uint32_t q = qidx[rt][r];
+      tau[rt][r] = (PHASE == 1 && q < n) ? gtau[q] : __builtin_huge_valf();
     }
 
+  // candidate slots: sample tiles (phase 0) or this split's range (phase 1)
   const uint32_t nsample = (ntiles + ss - 1) / ss;
-  for (int phase = 0; phase < 2; ++phase) {
-    const uint32_t slots = phase == 0 ? nsample : ntiles;
-    for (uint32_t s0 = 0; s0 < slots; s0 += CT) {
-      __syncthreads();
-      // cooperative stage load: CT tiles x KCH chunks x {hi, lo} x 64 lanes
-      for (int i = threadIdx.x; i < CT * KCH * NH * 64; i += 256) {
+  const uint32_t slot_lo = PHASE == 0 ? 0 : static_cast<uint32_t>(static_cast<uint64_t>(ntiles) * split / nsplit);
+  const uint32_t slot_hi = PHASE == 0 ? nsample : static_cast<uint32_t>(static_cast<uint64_t>(ntiles) * (split + 1) / nsplit);
+  auto slot_tile = [&](uint32_t slot) -> uint32_t { return PHASE == 0 ? slot * ss : slot; };
+  const uint32_t nstage = (slot_hi - slot_lo + CT - 1) / CT;
+
+  uint4 pf[PF];
+  float pa = 0.0f;
+  auto fetch = [&](uint32_t st) {
+    const uint32_t s0 = slot_lo + st * CT;
+#pragma unroll
+    for (int j = 0; j < PF; ++j) {
+      const int i = static_cast<int>(threadIdx.x) + j * 256;
+      uint4 v = make_uint4(0, 0, 0, 0);
+      if (i < PER) {
         const int l = i & 63, h = (i >> 6) % NH, kc = (i / (64 * NH)) % KCH, ct = i / (64 * NH * KCH);
         const uint32_t slot = s0 + ct;
-        const uint32_t tile = phase == 0 ? slot * ss : slot;
-        uint4 v = make_uint4(0, 0, 0, 0);
-        if (slot < slots && tile < ntiles) {
+        const uint32_t tile = slot_tile(slot);
+        if (slot < slot_hi && tile < ntiles) {
           const size_t o = (static_cast<size_t>(tile) * KCH + kc) * 64 + l;
           v = h == 0 ? fhi[o] : flo[o];
         }
-        sB[ct][kc][h][l] = v;
       }
-      for (int i = threadIdx.x; i < CT * 16; i += 256) {
-        const int ct = i >> 4, cc = i & 15;
-        const uint32_t slot = s0 + ct;
-        const uint32_t tile = phase == 0 ? slot * ss : slot;
-        sA[ct][cc] = (slot < slots && tile < ntiles) ? arow[tile * 16 + cc] : 0.0f;
-      }
-      __syncthreads();
+      pf[j] = v;
+    }
+    if (threadIdx.x < CT * 16) {
+      const uint32_t slot = s0 + threadIdx.x / 16;
+      const uint32_t tile = slot_tile(slot);
+      pa = (slot < slot_hi && tile < ntiles) ? arow[tile * 16 + (threadIdx.x & 15)] : 0.0f;
+    }
+  };
+  auto stash = [&](int buf) {
+#pragma unroll
+    for (int j = 0; j < PF; ++j) {
+      const int i = static_cast<int>(threadIdx.x) + j * 256;
+      if (i < PER) (&sB[buf][0][0][0][0])[i] = pf[j];
+    }
+    if (threadIdx.x < CT * 16) (&sA[buf][0][0])[threadIdx.x] = pa;
+  };
+
+  if (nstage) {
+    fetch(0);
+    stash(0);
+  }
+  __syncthreads();
+  for (uint32_t st = 0; st < nstage; ++st) {
+    const int buf = st & 1;
+    if (st + 1 < nstage) fetch(st + 1);  // in flight while this stage computes
 #pragma unroll 1
-      for (int ct = 0; ct < CT; ++ct) {
-        const uint32_t slot = s0 + ct;
-        if (slot >= slots) break;
-        const uint32_t tile = phase == 0 ? slot * ss : slot;
-        if (tile >= ntiles) break;
-        floatx4 acc[RT];
+    for (int ct = 0; ct < CT; ++ct) {
+      const uint32_t slot = slot_lo + st * CT + ct;
+      if (slot >= slot_hi) break;
+      const uint32_t tile = slot_tile(slot);
+      if (tile >= ntiles) break;
+      floatx4 acc[RT];
 #pragma unroll
-        for (int rt = 0; rt < RT; ++rt) acc[rt] = floatx4{0.f, 0.f, 0.f, 0.f};
+      for (int rt = 0; rt < RT; ++rt) acc[rt] = floatx4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-        for (int kc = 0; kc < KCH; ++kc) {
-          const bf16x8 bh = __builtin_bit_cast(bf16x8, sB[ct][kc][0][lane]);
-#pragma unroll
-          for (int rt = 0; rt < RT; ++rt)
-            acc[rt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi[rt][kc], bh, acc[rt], 0, 0, 0);
-          if (SPLIT) {
-            const bf16x8 bl = __builtin_bit_cast(bf16x8, sB[ct][kc][NH - 1][lane]);
-#pragma unroll
-            for (int rt = 0; rt < RT; ++rt) {
-              acc[rt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi[rt][kc], bl, acc[rt], 0, 0, 0);
-              acc[rt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(alo[rt][kc], bh, acc[rt], 0, 0, 0);
-            }
-          }
-        }
-        const uint32_t c = tile * 16 + col;
-        const float ac = sA[ct][col];
-        const bool cvalid = c < n;
+      for (int kc = 0; kc < KCH; ++kc) {
+        const bf16x8 bh = __builtin_bit_cast(bf16x8, sB[buf][ct][kc][0][lane]);
 #pragma unroll
         for (int rt = 0; rt < RT; ++rt)
+          acc[rt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi[rt][kc], bh, acc[rt], 0, 0, 0);
+        if (SPLIT) {
+          const bf16x8 bl = __builtin_bit_cast(bf16x8, sB[buf][ct][kc][NH - 1][lane]);
 #pragma unroll
-          for (int r = 0; r < 4; ++r) {
-            const uint32_t q = qidx[rt][r];
-            const float lb = fmaf(-2.0f, acc[rt][r], aq[rt][r] + ac);
-            const bool ok = cvalid && q < n && c != q;
-            if (phase == 0) {
-              if (ok && lb < best1[rt][r]) {
-                const bool first = lb < best0[rt][r];
-                best1[rt][r] = first ? best0[rt][r] : lb;
-                best0[rt][r] = first ? lb : best0[rt][r];
-              }
-            } else if (ok && lb <= tau[rt][r]) {
-              const uint32_t pos = atomicAdd(&sCnt[wave][rt * 16 + 4 * rsub + r], 1u);
-              if (pos < capg) gcand[static_cast<size_t>(q) * capg + pos] = c;
-            }
+          for (int rt = 0; rt < RT; ++rt) {
+            acc[rt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi[rt][kc], bl, acc[rt], 0, 0, 0);
+            acc[rt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(alo[rt][kc], bh, acc[rt], 0, 0, 0);
           }
+        }
       }
-    }
-    if (phase == 0) {
-      // tau = 16th smallest of the 32 per-row values held by the row's 16 lanes
+      const uint32_t c = tile * 16 + col;
+      const float ac = sA[buf][ct][col];
+      const bool cvalid = c < n;
 #pragma unroll
       for (int rt = 0; rt < RT; ++rt)
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-          float v0 = best0[rt][r], v1 = best1[rt][r];
-          seg32_sort(v0, v1, lane);
-          tau[rt][r] = __shfl(v0, (lane & ~15) | 15, 64);
+          const uint32_t q = qidx[rt][r];
+          const float lb = fmaf(-2.0f, acc[rt][r], aq[rt][r] + ac);
+          const bool ok = cvalid && q < n && c != q;
+          if (PHASE == 0) {
+            if (ok && lb < best1[rt][r]) {
+              const bool first = lb < best0[rt][r];
+              best1[rt][r] = first ? best0[rt][r] : lb;
+              best0[rt][r] = first ? lb : best0[rt][r];
+            }
+          } else if (ok && lb <= tau[rt][r]) {
+            const uint32_t pos = atomicAdd(&sCnt[wave][rt * 16 + 4 * rsub + r], 1u);
+            if (pos < capg) gcand[(static_cast<size_t>(q) * nsplit + split) * capg + pos] = c;
+          }
         }
     }
+    if (st + 1 < nstage) stash(buf ^ 1);
+    __syncthreads();
   }
-  __syncthreads();
+
+  if (PHASE == 0) {
+    // tau = 16th smallest of the 32 per-row values held by the row's 16 lanes
+#pragma unroll
+    for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        float v0 = best0[rt][r], v1 = best1[rt][r];
+        seg32_sort(v0, v1, lane);
+        tau[rt][r] = __shfl(v0, (lane & ~15) | 15, 64);
+      }
+  }
   if (col == 0) {
 #pragma unroll
     for (int rt = 0; rt < RT; ++rt)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const uint32_t q = qidx[rt][r];
-        if (q < n) {
-          gcnt[q] = sCnt[wave][rt * 16 + 4 * rsub + r];
+        if (q >= n) continue;
+        if (PHASE == 0)
           gtau[q] = tau[rt][r];
-        }
+        else
+          gcnt[static_cast<size_t>(q) * nsplit + split] = sCnt[wave][rt * 16 + 4 * rsub + r];
       }
   }
 }
@@ -340,7 +375,7 @@ __global__ __launch_bounds__(256) void knn_mfma_cand_kernel(
 template <int R>
 __global__ __launch_bounds__(256) void knn_rerank_kernel(
     const float *__restrict__ xp, uint32_t Dp, uint32_t Cp, uint32_t n, uint32_t k, uint32_t capg,
-    const uint32_t *__restrict__ gcnt, const uint32_t *__restrict__ gcand,
+    uint32_t nsplit, const uint32_t *__restrict__ gcnt, const uint32_t *__restrict__ gcand,
     const float *__restrict__ gtau, double cert_scale, uint32_t *__restrict__ oidx,
     float *__restrict__ odist, uint32_t *__restrict__ fb_count, uint32_t *__restrict__ fb_list) {
   extern __shared__ float sQrow[];  // 4 x Dp
@@ -351,9 +386,16 @@ __global__ __launch_bounds__(256) void knn_rerank_kernel(
     for (uint32_t f = lane; f < Dp; f += 64) qv[f] = xp[static_cast<size_t>(q) * Dp + f];
   __syncthreads();
   if (q >= n) return;
-  const uint32_t cnt = gcnt[q];
+  // the row's list is the concatenation of its per-split lists
+  uint32_t cnt = 0;
+  bool overflow = false;
+  for (uint32_t sp = 0; sp < nsplit; ++sp) {
+    const uint32_t c = gcnt[static_cast<size_t>(q) * nsplit + sp];
+    overflow = overflow || c > capg;
+    cnt += c;
+  }
   const bool all_listed = cnt == n - 1;
-  if (cnt > capg || cnt < k) {
+  if (overflow || cnt > capg || cnt < k) {
     if (lane == 0) fb_list[atomicAdd(fb_count, 1u)] = q;
     return;
   }
@@ -364,7 +406,9 @@ __global__ __launch_bounds__(256) void knn_rerank_kernel(
     const uint32_t e = r * 64 + lane;
     v[r] = ~0ull;
     if (e < cnt) {
-      const uint32_t c = gcand[static_cast<size_t>(q) * capg + e];
+      uint32_t sp = 0, off = e;
+      while (off >= gcnt[static_cast<size_t>(q) * nsplit + sp]) off -= gcnt[static_cast<size_t>(q) * nsplit + sp++];
+      const uint32_t c = gcand[(static_cast<size_t>(q) * nsplit + sp) * capg + off];
       const float *cr = xp + static_cast<size_t>(c) * Dp;
       float s = 0.0f, acc = 0.0f;
       for (uint32_t f = 0; f < Dp; f += 4) {
@@ -438,52 +482,70 @@ hipError_t launch_knn_rownorm(const float *x, uint64_t n, uint32_t d, const doub
   return hipGetLastError();
 }
 
-template <int KCH>
-static hipError_t cand_dispatch(const void *fhi, const void *flo, const float *arow, uint32_t n,
-                                uint32_t ntiles, uint32_t ss, uint32_t capg, uint32_t *gcnt,
-                                uint32_t *gcand, float *gtau, bool split, hipStream_t s) {
-  constexpr int RT = KCH <= 4 ? 4 : 2;
-  const uint32_t qtiles_per_block = 4 * RT;
-  const uint32_t blocks = (ntiles + qtiles_per_block - 1) / qtiles_per_block;
-  if (split)
-    knn_mfma_cand_kernel<KCH, RT, true><<<blocks, 256, 0, s>>>(
-        static_cast<const uint4 *>(fhi), static_cast<const uint4 *>(flo), arow, n, ntiles, ss, capg,
-        gcnt, gcand, gtau);
+constexpr int kCandRT = 2;  // query tiles per wave in the candidate kernels
+
+uint32_t knn_cand_groups(uint32_t ntiles) { return (ntiles + 4 * kCandRT - 1) / (4 * kCandRT); }
+
+template <int KCH, bool SPLIT>
+static hipError_t cand_dispatch(int phase, const void *fhi, const void *flo, const float *arow,
+                                uint32_t n, uint32_t ntiles, uint32_t ss, uint32_t capg,
+                                uint32_t nsplit, uint32_t *gcnt, uint32_t *gcand, float *gtau,
+                                hipStream_t s) {
+  const uint32_t groups = knn_cand_groups(ntiles);
+  const uint4 *h = static_cast<const uint4 *>(fhi), *l = static_cast<const uint4 *>(flo);
+  if (phase == 0)
+    knn_mfma_kernel<KCH, kCandRT, SPLIT, 0><<<dim3(groups, 1), 256, 0, s>>>(h, l, arow, n, ntiles, ss,
+                                                                           capg, gcnt, gcand, gtau);
   else
-    knn_mfma_cand_kernel<KCH, RT, false><<<blocks, 256, 0, s>>>(
-        static_cast<const uint4 *>(fhi), static_cast<const uint4 *>(flo), arow, n, ntiles, ss, capg,
-        gcnt, gcand, gtau);
+    knn_mfma_kernel<KCH, kCandRT, SPLIT, 1><<<dim3(groups, nsplit), 256, 0, s>>>(
+        h, l, arow, n, ntiles, ss, capg, gcnt, gcand, gtau);
   return hipGetLastError();
 }
 
-hipError_t launch_knn_cand(const void *fhi, const void *flo, const float *arow, uint32_t n,
-                           uint32_t kch, uint32_t ntiles, uint32_t ss, uint32_t capg,
-                           uint32_t *gcnt, uint32_t *gcand, float *gtau, bool split, hipStream_t s) {
+template <int KCH>
+static hipError_t cand_dispatch_split(bool split, int phase, const void *fhi, const void *flo,
+                                      const float *arow, uint32_t n, uint32_t ntiles, uint32_t ss,
+                                      uint32_t capg, uint32_t nsplit, uint32_t *gcnt, uint32_t *gcand,
+                                      float *gtau, hipStream_t s) {
+  return split ? cand_dispatch<KCH, true>(phase, fhi, flo, arow, n, ntiles, ss, capg, nsplit, gcnt,
+                                          gcand, gtau, s)
+               : cand_dispatch<KCH, false>(phase, fhi, flo, arow, n, ntiles, ss, capg, nsplit, gcnt,
+                                           gcand, gtau, s);
+}
+
+hipError_t launch_knn_cand(int phase, const void *fhi, const void *flo, const float *arow,
+                           uint32_t n, uint32_t kch, uint32_t ntiles, uint32_t ss, uint32_t capg,
+                           uint32_t nsplit, uint32_t *gcnt, uint32_t *gcand, float *gtau,
+                           bool split, hipStream_t s) {
+#define PM_CAND(K) \
+  case K: return cand_dispatch_split<K>(split, phase, fhi, flo, arow, n, ntiles, ss, capg, nsplit, gcnt, gcand, gtau, s);
   switch (kch) {
-    case 1: return cand_dispatch<1>(fhi, flo, arow, n, ntiles, ss, capg, gcnt, gcand, gtau, split, s);
-    case 2: return cand_dispatch<2>(fhi, flo, arow, n, ntiles, ss, capg, gcnt, gcand, gtau, split, s);
-    case 3: return cand_dispatch<3>(fhi, flo, arow, n, ntiles, ss, capg, gcnt, gcand, gtau, split, s);
-    case 4: return cand_dispatch<4>(fhi, flo, arow, n, ntiles, ss, capg, gcnt, gcand, gtau, split, s);
-    case 5: return cand_dispatch<5>(fhi, flo, arow, n, ntiles, ss, capg, gcnt, gcand, gtau, split, s);
-    case 6: return cand_dispatch<6>(fhi, flo, arow, n, ntiles, ss, capg, gcnt, gcand, gtau, split, s);
-    case 7: return cand_dispatch<7>(fhi, flo, arow, n, ntiles, ss, capg, gcnt, gcand, gtau, split, s);
-    case 8: return cand_dispatch<8>(fhi, flo, arow, n, ntiles, ss, capg, gcnt, gcand, gtau, split, s);
+    PM_CAND(1)
+    PM_CAND(2)
+    PM_CAND(3)
+    PM_CAND(4)
+    PM_CAND(5)
+    PM_CAND(6)
+    PM_CAND(7)
+    PM_CAND(8)
     default: return hipErrorInvalidValue;
   }
+#undef PM_CAND
 }
 
 hipError_t launch_knn_rerank(const float *xp, const KnnLayout &L, uint32_t n, uint32_t k,
-                             uint32_t capg, const uint32_t *gcnt, const uint32_t *gcand,
-                             const float *gtau, double cert_scale, uint32_t *idx, float *dist,
-                             uint32_t *fb_count, uint32_t *fb_list, hipStream_t s) {
+                             uint32_t capg, uint32_t nsplit, const uint32_t *gcnt,
+                             const uint32_t *gcand, const float *gtau, double cert_scale,
+                             uint32_t *idx, float *dist, uint32_t *fb_count, uint32_t *fb_list,
+                             hipStream_t s) {
   const uint32_t blocks = (n + 3) / 4;
   const size_t shm = static_cast<size_t>(4) * L.Dp * sizeof(float);
   if (capg == 512)
-    knn_rerank_kernel<8><<<blocks, 256, shm, s>>>(xp, L.Dp, L.Cp, n, k, capg, gcnt, gcand, gtau,
-                                                  cert_scale, idx, dist, fb_count, fb_list);
+    knn_rerank_kernel<8><<<blocks, 256, shm, s>>>(xp, L.Dp, L.Cp, n, k, capg, nsplit, gcnt, gcand,
+                                                  gtau, cert_scale, idx, dist, fb_count, fb_list);
   else if (capg == 1024)
-    knn_rerank_kernel<16><<<blocks, 256, shm, s>>>(xp, L.Dp, L.Cp, n, k, capg, gcnt, gcand, gtau,
-                                                   cert_scale, idx, dist, fb_count, fb_list);
+    knn_rerank_kernel<16><<<blocks, 256, shm, s>>>(xp, L.Dp, L.Cp, n, k, capg, nsplit, gcnt, gcand,
+                                                   gtau, cert_scale, idx, dist, fb_count, fb_list);
   else
     return hipErrorInvalidValue;
   return hipGetLastError();

@@ -76,7 +76,8 @@ class Timings(C.Structure):
                                         "optimize_ms")] + [
         (name, C.c_uint64) for name in ("iterations", "csr_entries", "knn_rescanned_rows",
                                         "knn_candidates")] + [
-        ("knn_rerank_ms", C.c_double), ("knn_fallback_ms", C.c_double), ("knn_path", C.c_uint64)]
+        ("knn_rerank_ms", C.c_double), ("knn_fallback_ms", C.c_double), ("knn_path", C.c_uint64),
+        ("knn_sample_ms", C.c_double)]
 
 
 INIT_PCA, INIT_VALUE, INIT_RANDOM = 0, 1, 2
