@@ -811,10 +811,19 @@ static uint32_t cov_splits(uint64_t n) {
   return static_cast<uint32_t>(s);
 }
 
+// row splits of the column sums: enough blocks to cover the chip
+static uint32_t colsum_splits(uint64_t n) {
+  uint64_t s = (n + 127) / 128;
+  if (s < 1) s = 1;
+  if (s > 1024) s = 1024;
+  return static_cast<uint32_t>(s);
+}
+
 uint64_t cov_scratch_doubles(uint64_t n, uint32_t d) {
   const uint64_t s = cov_splits(n);
-  const uint64_t colpart = s * d;
-  return s * static_cast<uint64_t>(d) * d + colpart;
+  const uint64_t cov = s * static_cast<uint64_t>(d) * d;
+  const uint64_t col = static_cast<uint64_t>(colsum_splits(n)) * d;
+  return cov > col ? cov : col;
 }
 
 __global__ void colsum_part_kernel(const float *__restrict__ x, uint64_t n, uint32_t d,
@@ -839,7 +848,7 @@ __global__ void colmean_fin_kernel(const double *__restrict__ part, uint32_t d, 
 
 hipError_t launch_colmean_f64(const float *x, uint64_t n, uint32_t d, double *mean,
                               double *scratch, hipStream_t s) {
-  const uint32_t splits = cov_splits(n);
+  const uint32_t splits = colsum_splits(n);
   dim3 g((d + 63) / 64, splits);
   colsum_part_kernel<<<g, 64, 0, s>>>(x, n, d, splits, scratch);
   colmean_fin_kernel<<<(d + 63) / 64, 64, 0, s>>>(scratch, d, splits, n, mean);
