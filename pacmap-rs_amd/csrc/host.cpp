@@ -352,142 +352,90 @@ void optimize_device(const DevCsr &csr, uint64_t n, uint64_t dims, std::vector<f
 }
 
 // ---------------------------------------------------------------------------
-// PCA: top-q principal axes of the rows of x (FP64 covariance on the GPU)
+// PCA: top-q principal axes of the rows of x, all on the GPU: FP64
+// covariance, then (d <= 128) a Jacobi eigensolve of it, or (d > 128) a block
+// subspace iteration with CholeskyQR2 followed by a Jacobi eigensolve of the
+// Rayleigh-Ritz matrix.  Axes are sorted by variance, each signed so its
+// largest-magnitude loading is positive.  Outputs stay on the device.
 // ---------------------------------------------------------------------------
-void cholesky_upper_inv(int b, const std::vector<double> &G, std::vector<double> &Rinv) {
-  // G = R^T R with R upper triangular; returns R^{-1} (upper triangular)
-  std::vector<double> R(static_cast<size_t>(b) * b, 0.0);
-  double shift = 0.0;
-  for (int attempt = 0; attempt < 4; ++attempt) {
-    bool ok = true;
-    std::fill(R.begin(), R.end(), 0.0);
-    for (int j = 0; j < b && ok; ++j) {
-      double s = G[static_cast<size_t>(j) * b + j] + shift;
-      for (int k = 0; k < j; ++k) s -= R[static_cast<size_t>(k) * b + j] * R[static_cast<size_t>(k) * b + j];
-      if (!(s > 0.0)) {
-        ok = false;
-        break;
-      }
-      const double rjj = std::sqrt(s);
-      R[static_cast<size_t>(j) * b + j] = rjj;
-      for (int i = j + 1; i < b; ++i) {
-        double t = G[static_cast<size_t>(j) * b + i];
-        for (int k = 0; k < j; ++k) t -= R[static_cast<size_t>(k) * b + j] * R[static_cast<size_t>(k) * b + i];
-        R[static_cast<size_t>(j) * b + i] = t / rjj;
-      }
-    }
-    if (ok) break;
-    double tr = 0.0;
-    for (int j = 0; j < b; ++j) tr += G[static_cast<size_t>(j) * b + j];
-    shift = shift == 0.0 ? 1e-12 * tr : shift * 100.0;
-    if (attempt == 3) fail(PACMAP_ERR_PCA, "subspace orthonormalisation failed");
-  }
-  Rinv.assign(static_cast<size_t>(b) * b, 0.0);
-  for (int j = b - 1; j >= 0; --j) {
-    Rinv[static_cast<size_t>(j) * b + j] = 1.0 / R[static_cast<size_t>(j) * b + j];
-    for (int i = j - 1; i >= 0; --i) {
-      double s = 0.0;
-      for (int k = i + 1; k <= j; ++k) s += R[static_cast<size_t>(i) * b + k] * Rinv[static_cast<size_t>(k) * b + j];
-      Rinv[static_cast<size_t>(i) * b + j] = -s / R[static_cast<size_t>(i) * b + i];
-    }
-  }
-}
+struct PcaAxes {
+  DevBuf<float> Vf;    // d x q (FP32 copy for the projection)
+  DevBuf<float> mean;  // d
+};
 
-// V: d x q (row-major, columns = axes by descending variance); mean: d.
-void pca_axes(const float *x_dev, uint64_t n, uint32_t d, uint32_t q, std::vector<double> &V,
-              std::vector<double> &mean, hipStream_t s) {
+void pca_axes_dev(const float *x_dev, uint64_t n, uint32_t d, uint32_t q, PcaAxes &out, hipStream_t s) {
+  constexpr uint32_t kIters = 6;
   DevBuf<double> d_mean(d), d_scr(cov_scratch_doubles(n, d)), d_cov(static_cast<size_t>(d) * d);
-  Timer tc;
+  Timer tc, te;
   tc.start(s);
   PM_HIP(launch_colmean_f64(x_dev, n, d, d_mean.get(), d_scr.get(), s));
   PM_HIP(launch_cov_f64(x_dev, n, d, d_mean.get(), d_scr.get(), d_cov.get(), s));
   tc.stop(s);
-  mean.resize(d);
-  download(mean.data(), d_mean.get(), d, s);
-  PM_HIP(hipStreamSynchronize(s));
-  g_tm.pca_cov_ms += tc.ms();
-  const double te = now_ms();
-  V.assign(static_cast<size_t>(d) * q, 0.0);
-  if (d <= 192) {
-    std::vector<double> C(static_cast<size_t>(d) * d), w, E;
-    download(C.data(), d_cov.get(), C.size(), s);
-    PM_HIP(hipStreamSynchronize(s));
-    jacobi_eigh(static_cast<int>(d), C, w, E);
-    for (uint32_t r = 0; r < d; ++r)
-      for (uint32_t c = 0; c < q; ++c) V[static_cast<size_t>(r) * q + c] = E[static_cast<size_t>(r) * d + c];
+  te.start(s);
+  DevBuf<double> V(static_cast<size_t>(d) * q);
+  DevBuf<uint32_t> status(1);
+  PM_HIP(launch_fill_zero(status.get(), status.bytes(), s));
+  if (d <= 128) {
+    DevBuf<double> work(static_cast<size_t>(d) * d);
+    PM_HIP(launch_jacobi(d_cov.get(), d, q, work.get(), V.get(), s));
   } else {
-    // block subspace iteration with CholeskyQR2 re-orthonormalisation, then a
-    // Rayleigh-Ritz step on the b x b projection (b = q + 16)
-    const uint32_t b = std::min<uint32_t>(d, q + 16);
-    std::vector<double> Q(static_cast<size_t>(d) * b);
+    const uint32_t b = std::min<uint32_t>(std::min<uint32_t>(d, q + 16), 128);
+    if (q > b) fail(PACMAP_ERR_PCA, "too many principal components requested");
+    std::vector<double> Q0(static_cast<size_t>(d) * b);
     SmallRng rng = SmallRng::seed_from_u64(0x5eedull);
-    for (auto &e : Q) e = rng.next_f64() - 0.5;
-    DevBuf<double> dQ(Q.size()), dZ(Q.size()), dG(static_cast<size_t>(b) * b), dR(static_cast<size_t>(b) * b);
-    upload(dQ, Q.data(), Q.size(), s);
-    std::vector<double> G(static_cast<size_t>(b) * b), Rinv;
+    for (auto &e : Q0) e = rng.next_f64() - 0.5;
+    DevBuf<double> Q(Q0.size()), Z(Q0.size()), G(static_cast<size_t>(b) * b), Rinv(static_cast<size_t>(b) * b),
+        U(static_cast<size_t>(b) * q), work(static_cast<size_t>(b) * b);
+    upload(Q, Q0.data(), Q0.size(), s);
+    // orthonormalise src into dst: two CholeskyQR passes (dst = src R^-1)
     auto orth = [&](DevBuf<double> &src, DevBuf<double> &dst) {
+      DevBuf<double> *in = &src;
       for (int pass = 0; pass < 2; ++pass) {
-        DevBuf<double> &in = pass == 0 ? src : dst;
-        PM_HIP(launch_gemm_f64(b, b, d, in.get(), b, true, in.get(), b, false, dG.get(), b, s));
-        download(G.data(), dG.get(), G.size(), s);
-        PM_HIP(hipStreamSynchronize(s));
-        cholesky_upper_inv(static_cast<int>(b), G, Rinv);
-        upload(dR, Rinv.data(), Rinv.size(), s);
-        DevBuf<double> tmp(static_cast<size_t>(d) * b);
-        PM_HIP(launch_gemm_f64(d, b, b, in.get(), b, false, dR.get(), b, false, tmp.get(), b, s));
-        PM_HIP(hipMemcpyAsync(dst.get(), tmp.get(), tmp.bytes(), hipMemcpyDeviceToDevice, s));
-        PM_HIP(hipStreamSynchronize(s));
+        PM_HIP(launch_gemm64(b, b, d, in->get(), b, true, in->get(), b, false, G.get(), b, s));
+        PM_HIP(launch_chol_inv(G.get(), b, Rinv.get(), status.get(), s));
+        DevBuf<double> &o = pass == 0 ? (in == &src ? Z : Q) : dst;
+        if (&o == in) {
+          DevBuf<double> tmp(Q0.size());
+          PM_HIP(launch_gemm64(d, b, b, in->get(), b, false, Rinv.get(), b, false, tmp.get(), b, s));
+          PM_HIP(hipMemcpyAsync(o.get(), tmp.get(), tmp.bytes(), hipMemcpyDeviceToDevice, s));
+        } else {
+          PM_HIP(launch_gemm64(d, b, b, in->get(), b, false, Rinv.get(), b, false, o.get(), b, s));
+        }
+        in = &o;
       }
     };
-    orth(dQ, dQ);
-    for (int it = 0; it < 10; ++it) {
-      PM_HIP(launch_gemm_f64(d, b, d, d_cov.get(), d, false, dQ.get(), b, false, dZ.get(), b, s));
-      orth(dZ, dQ);
+    orth(Q, Q);
+    for (uint32_t it = 0; it < kIters; ++it) {
+      PM_HIP(launch_gemm64(d, b, d, d_cov.get(), d, false, Q.get(), b, false, Z.get(), b, s));
+      orth(Z, Q);
     }
-    PM_HIP(launch_gemm_f64(d, b, d, d_cov.get(), d, false, dQ.get(), b, false, dZ.get(), b, s));
-    PM_HIP(launch_gemm_f64(b, b, d, dQ.get(), b, true, dZ.get(), b, false, dG.get(), b, s));
-    download(G.data(), dG.get(), G.size(), s);
-    download(Q.data(), dQ.get(), Q.size(), s);
-    PM_HIP(hipStreamSynchronize(s));
-    for (uint32_t i = 0; i < b; ++i)  // symmetrise the Ritz matrix
-      for (uint32_t j = i + 1; j < b; ++j) {
-        const double avg = 0.5 * (G[static_cast<size_t>(i) * b + j] + G[static_cast<size_t>(j) * b + i]);
-        G[static_cast<size_t>(i) * b + j] = avg;
-        G[static_cast<size_t>(j) * b + i] = avg;
-      }
-    std::vector<double> w, U;
-    jacobi_eigh(static_cast<int>(b), G, w, U);
-    for (uint32_t r = 0; r < d; ++r)
-      for (uint32_t c = 0; c < q; ++c) {
-        double acc = 0.0;
-        for (uint32_t k = 0; k < b; ++k) acc += Q[static_cast<size_t>(r) * b + k] * U[static_cast<size_t>(k) * b + c];
-        V[static_cast<size_t>(r) * q + c] = acc;
-      }
-    for (uint32_t c = 0; c < q; ++c) {  // sign: largest-magnitude loading positive
-      uint32_t best = 0;
-      double bv = -1.0;
-      for (uint32_t r = 0; r < d; ++r)
-        if (std::fabs(V[static_cast<size_t>(r) * q + c]) > bv) {
-          bv = std::fabs(V[static_cast<size_t>(r) * q + c]);
-          best = r;
-        }
-      if (V[static_cast<size_t>(best) * q + c] < 0)
-        for (uint32_t r = 0; r < d; ++r) V[static_cast<size_t>(r) * q + c] = -V[static_cast<size_t>(r) * q + c];
-    }
+    // Rayleigh-Ritz: T = Q^T C Q, eigenvectors U, axes V = Q U
+    PM_HIP(launch_gemm64(d, b, d, d_cov.get(), d, false, Q.get(), b, false, Z.get(), b, s));
+    PM_HIP(launch_gemm64(b, b, d, Q.get(), b, true, Z.get(), b, false, G.get(), b, s));
+    PM_HIP(launch_jacobi(G.get(), b, q, work.get(), U.get(), s));
+    PM_HIP(launch_gemm64(d, q, b, Q.get(), b, false, U.get(), q, false, V.get(), q, s));
   }
-  g_tm.pca_eig_ms += now_ms() - te;
+  out.Vf.alloc(static_cast<size_t>(d) * q);
+  PM_HIP(launch_sign_fix(V.get(), d, q, out.Vf.get(), s));
+  te.stop(s);
+  std::vector<double> hm(d);
+  uint32_t hstatus = 0;
+  download(hm.data(), d_mean.get(), d, s);
+  download(&hstatus, status.get(), 1, s);
+  PM_HIP(hipStreamSynchronize(s));
+  if (hstatus) fail(PACMAP_ERR_PCA, "subspace orthonormalisation failed");
+  std::vector<float> mf(hm.begin(), hm.end());
+  out.mean.alloc(d);
+  upload(out.mean, mf.data(), d, s);
+  g_tm.pca_cov_ms += tc.ms();
+  g_tm.pca_eig_ms += te.ms();
 }
 
 // out (n x q) = (x - mean) V on the device
-void pca_project(const float *x_dev, uint64_t n, uint32_t d, uint32_t q, const std::vector<double> &V,
-                 const std::vector<double> &mean, DevBuf<float> &out, hipStream_t s) {
-  std::vector<float> Vf(V.begin(), V.end()), mf(mean.begin(), mean.end());
-  DevBuf<float> dV(Vf.size()), dm(mf.size());
-  upload(dV, Vf.data(), Vf.size(), s);
-  upload(dm, mf.data(), mf.size(), s);
+void pca_project_dev(const float *x_dev, uint64_t n, uint32_t d, uint32_t q, const PcaAxes &ax,
+                     DevBuf<float> &out, hipStream_t s) {
   out.alloc(n * q);
-  PM_HIP(launch_project(x_dev, n, d, dm.get(), dV.get(), q, out.get(), s));
-  PM_HIP(hipStreamSynchronize(s));
+  PM_HIP(launch_project(x_dev, n, d, ax.mean.get(), ax.Vf.get(), q, out.get(), s));
 }
 
 // ---------------------------------------------------------------------------
@@ -608,9 +556,9 @@ void fit_impl(const float *x_host, const float *x_dev_in, uint64_t n, uint64_t d
   if (pca_path) {
     const uint32_t q = static_cast<uint32_t>(std::min<uint64_t>(100, n));  // lib.rs:457
     if (dims > q) fail(PACMAP_ERR_INVALID_ARGUMENT, "embedding_dimensions exceeds the PCA width");
-    std::vector<double> V, mean;
-    pca_axes(xd, n, static_cast<uint32_t>(d), q, V, mean, s);
-    pca_project(xd, n, static_cast<uint32_t>(d), q, V, mean, d_xk, s);
+    PcaAxes ax;
+    pca_axes_dev(xd, n, static_cast<uint32_t>(d), q, ax, s);
+    pca_project_dev(xd, n, static_cast<uint32_t>(d), q, ax, d_xk, s);
     dk = q;
   } else {
     d_xk.alloc(n * d);
@@ -623,10 +571,10 @@ void fit_impl(const float *x_host, const float *x_dev_in, uint64_t n, uint64_t d
     if (init == PACMAP_INIT_PCA) {  // lib.rs:513-519 then lib.rs:768
       const uint32_t nc = static_cast<uint32_t>(std::min<uint64_t>(n, dims));
       if (nc > d) fail(PACMAP_ERR_PCA, "PCA needs at least as many features as components");
-      std::vector<double> V, pm_mean;
-      pca_axes(d_xk.get(), n, static_cast<uint32_t>(d), nc, V, pm_mean, s);
+      PcaAxes ax;
+      pca_axes_dev(d_xk.get(), n, static_cast<uint32_t>(d), nc, ax, s);
       DevBuf<float> proj;
-      pca_project(d_xk.get(), n, static_cast<uint32_t>(d), nc, V, pm_mean, proj, s);
+      pca_project_dev(d_xk.get(), n, static_cast<uint32_t>(d), nc, ax, proj, s);
       y0.resize(n * dims);
       download(y0.data(), proj.get(), n * dims, s);
       PM_HIP(hipStreamSynchronize(s));

@@ -153,9 +153,4 @@ struct Csr {
 };
 void build_csr(uint64_t n, const PairList lists[3], Csr &out);
 
-// --- linear algebra (linalg.cpp) -------------------------------------------
-// Eigen-decomposition of a symmetric m x m matrix (row-major, destroyed):
-// eigenvalues descending in w, eigenvectors as columns of V (m x m).
-void jacobi_eigh(int m, std::vector<double> &A, std::vector<double> &w, std::vector<double> &V);
-
 }  // namespace pm

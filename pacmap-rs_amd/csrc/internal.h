@@ -131,8 +131,15 @@ hipError_t launch_fill_zero(void *p, uint64_t bytes, hipStream_t s);
 // dst (n x cols) = src[:, :cols] * scale, src with row stride ld
 hipError_t launch_take_cols(const float *src, uint64_t n, uint32_t ld, uint32_t cols, float scale,
                             float *dst, hipStream_t s);
-hipError_t launch_gemm_f64(uint32_t M, uint32_t N, uint32_t K, const double *A, uint32_t lda,
-                           bool tA, const double *B, uint32_t ldb, bool tB, double *C, uint32_t ldc,
-                           hipStream_t s);
+
+// Small dense FP64 linear algebra (linalg.hip)
+hipError_t launch_gemm64(uint32_t M, uint32_t N, uint32_t K, const double *A, uint32_t lda, bool tA,
+                         const double *B, uint32_t ldb, bool tB, double *C, uint32_t ldc, hipStream_t s);
+// G (b x b, b <= 128) = R^T R -> R^{-1}; status[0] = 1 when factoring fails
+hipError_t launch_chol_inv(const double *G, uint32_t b, double *Rinv, uint32_t *status, hipStream_t s);
+// q leading eigenvectors of symmetric A (m x m, m <= 128) as columns of Vout (m x q)
+hipError_t launch_jacobi(const double *A, uint32_t m, uint32_t q, double *Vwork, double *Vout, hipStream_t s);
+// column signs of V (d x q) and its FP32 copy Vf
+hipError_t launch_sign_fix(double *V, uint32_t d, uint32_t q, float *Vf, hipStream_t s);
 
 }  // namespace pm

@@ -1052,68 +1052,4 @@ hipError_t launch_fill_zero(void *p, uint64_t bytes, hipStream_t s) {
   return hipGetLastError();
 }
 
-// C (M x N) = op(A) (M x K) * op(B) (K x N), row-major, FP64.
-__global__ __launch_bounds__(256) void gemm_f64_kernel(uint32_t M, uint32_t N, uint32_t K,
-                                                       const double *__restrict__ A, uint32_t lda,
-                                                       int tA, const double *__restrict__ B,
-                                                       uint32_t ldb, int tB, double *__restrict__ C,
-                                                       uint32_t ldc) {
-  constexpr int TB = 64, KB = 16;
-  __shared__ double sA[KB][TB];
-  __shared__ double sB[KB][TB];
-  const int t = threadIdx.x, ty = t >> 4, tx = t & 15;
-  const uint32_t r0 = blockIdx.x * TB, c0 = blockIdx.y * TB;
-  double acc[4][4];
-#pragma unroll
-  for (int i = 0; i < 4; ++i)
-#pragma unroll
-    for (int j = 0; j < 4; ++j) acc[i][j] = 0.0;
-  for (uint32_t k0 = 0; k0 < K; k0 += KB) {
-    __syncthreads();
-    for (int li = t; li < TB * KB; li += 256) {
-      const int rr = li / KB, kk = li % KB;
-      const uint32_t r = r0 + rr, k = k0 + kk;
-      double v = 0.0;
-      if (r < M && k < K) v = tA ? A[static_cast<size_t>(k) * lda + r] : A[static_cast<size_t>(r) * lda + k];
-      sA[kk][rr] = v;
-    }
-    for (int li = t; li < TB * KB; li += 256) {
-      const int kk = li / TB, cc = li % TB;
-      const uint32_t k = k0 + kk, c = c0 + cc;
-      double v = 0.0;
-      if (k < K && c < N) v = tB ? B[static_cast<size_t>(c) * ldb + k] : B[static_cast<size_t>(k) * ldb + c];
-      sB[kk][cc] = v;
-    }
-    __syncthreads();
-#pragma unroll
-    for (int kk = 0; kk < KB; ++kk) {
-      double av[4], bv[4];
-#pragma unroll
-      for (int i = 0; i < 4; ++i) av[i] = sA[kk][ty * 4 + i];
-#pragma unroll
-      for (int j = 0; j < 4; ++j) bv[j] = sB[kk][tx * 4 + j];
-#pragma unroll
-      for (int i = 0; i < 4; ++i)
-#pragma unroll
-        for (int j = 0; j < 4; ++j) acc[i][j] = fma(av[i], bv[j], acc[i][j]);
-    }
-  }
-#pragma unroll
-  for (int i = 0; i < 4; ++i)
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const uint32_t r = r0 + ty * 4 + i, c = c0 + tx * 4 + j;
-      if (r < M && c < N) C[static_cast<size_t>(r) * ldc + c] = acc[i][j];
-    }
-}
-
-hipError_t launch_gemm_f64(uint32_t M, uint32_t N, uint32_t K, const double *A, uint32_t lda,
-                           bool tA, const double *B, uint32_t ldb, bool tB, double *C, uint32_t ldc,
-                           hipStream_t s) {
-  if (M == 0 || N == 0) return hipSuccess;
-  dim3 g((M + 63) / 64, (N + 63) / 64);
-  gemm_f64_kernel<<<g, 256, 0, s>>>(M, N, K, A, lda, tA ? 1 : 0, B, ldb, tB ? 1 : 0, C, ldc);
-  return hipGetLastError();
-}
-
 }  // namespace pm
