@@ -49,6 +49,29 @@ def config(pm):
                             approx_threshold=2 ** 63, num_iters=ITERS)
 
 
+# Large single-GPU run: 1,000,000 x 128 from a 100-component Gaussian mixture
+# with Zipf(1.1) cluster sizes (centres ~ N(0, 4I), within-cluster std 1);
+# n_neighbors 10 with the default ratios (10 NN, 5 MN, 20 FP pairs per point),
+# PCA init (128 -> 100 dims), bf16 kNN candidates with the exact FP32 rerank.
+LARGE = dict(n=1_000_000, d=128, k=10, n_mn=5, n_fp=20)
+
+
+def make_large(seed=7):
+    rng = np.random.default_rng(seed)
+    n, d = LARGE["n"], LARGE["d"]
+    w = 1.0 / np.arange(1, 101) ** 1.1
+    lab = rng.choice(100, size=n, p=w / w.sum())
+    centres = rng.normal(0.0, 2.0, size=(100, d)).astype(np.float32)
+    x = rng.standard_normal(size=(n, d), dtype=np.float32)
+    x += centres[lab]
+    return np.ascontiguousarray(x), lab
+
+
+def config_large(pm):
+    return pm.Configuration(initialization=pm.Initialization.Pca(), override_neighbors=LARGE["k"],
+                            seed=42, approx_threshold=2 ** 63, num_iters=ITERS, knn_precision="bf16")
+
+
 def grad_adam_bytes(n, pairs, dims):
     """Algorithmic HBM bytes of one fused iteration: CSR entries (2P x 4 B),
     row offsets, and y read/write, m and v read/write (6 x 4 B per coordinate)."""
@@ -97,6 +120,9 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--workload", choices=("70k", "1m"), default="70k",
+                    help="70k: the 70000 x 784 headline run; 1m: the 1,000,000 x 128 large run "
+                         "(no CPU baseline)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU design is "
@@ -104,10 +130,19 @@ def main():
         sys.exit(2)
     import pacmap_mi355x as pm
 
-    log("bench: generating 70000 x 784 Gaussian mixture")
-    x, _ = make_data()
+    large = args.workload == "1m"
+    if large:
+        n, d, k_nn, n_mn, n_fp = (LARGE[k] for k in ("n", "d", "k", "n_mn", "n_fp"))
+        log(f"bench: generating {n} x {d} Zipf Gaussian mixture")
+        x, _ = make_large()
+        cfg = config_large(pm)
+        args.no_cpu_baseline = True
+    else:
+        n, d, k_nn, n_mn, n_fp = N, D, K_NN, N_MN, N_FP
+        log("bench: generating 70000 x 784 Gaussian mixture")
+        x, _ = make_data()
+        cfg = config(pm)
     xd = pm.DeviceArray(x)
-    cfg = config(pm)
     for w in range(args.warmup):
         log(f"bench: warmup {w + 1}/{args.warmup}")
         pm.fit_transform(xd, cfg)
@@ -121,7 +156,7 @@ def main():
     assert np.isfinite(y).all()
     sec = float(np.mean(times))
     knn_ms = float(np.mean([t["knn_kernel_ms"] for t in tms]))
-    flops = 2.0 * N * N * DPRIME
+    flops = 2.0 * n * n * DPRIME
     achieved = flops / (knn_ms * 1e-3) / 1e12
     path = int(tms[-1]["knn_path"])
     if path == 0:
@@ -132,8 +167,8 @@ def main():
                  "knn_mfma_cand_kernel (bf16 MFMA candidates)")
         peak = BF16_PEAK_TFLOPS
     opt_ms = float(np.mean([t["optimize_ms"] for t in tms]))
-    pairs = N * (K_NN + N_MN + N_FP)
-    b_iter = grad_adam_bytes(N, pairs, DIMS)
+    pairs = n * (k_nn + n_mn + n_fp)
+    b_iter = grad_adam_bytes(n, pairs, DIMS)
     it_total = sum(ITERS)
     breakdown = {k: round(float(np.mean([t[k] for t in tms])), 3) for k in (
         "total_ms", "preprocess_ms", "pca_cov_ms", "pca_eig_ms", "knn_ms", "knn_kernel_ms",
@@ -143,10 +178,13 @@ def main():
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(sec * 1e3, 2),
         "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "fp32",
         "data": "synthetic",
-        "config": {"workload": "fit_transform 70000x784 GMM(10), PCA init -> 100 dims, exact kNN "
-                               "k=10 (60 candidates), 10 MN + 20 FP per point, 450 iterations",
-                   "n": N, "d": D, "knn_dims": DPRIME, "n_neighbors": K_NN, "n_mn": N_MN,
-                   "n_fp": N_FP, "iterations": it_total},
+        "config": {"workload": ("fit_transform 1000000x128 GMM(100, Zipf 1.1 sizes), PCA init -> "
+                                "100 dims, exact kNN from bf16 candidates k=10 (60 candidates), "
+                                "5 MN + 20 FP per point, 450 iterations" if large else
+                                "fit_transform 70000x784 GMM(10), PCA init -> 100 dims, exact kNN "
+                                "k=10 (60 candidates), 10 MN + 20 FP per point, 450 iterations"),
+                   "n": n, "d": d, "knn_dims": DPRIME, "n_neighbors": k_nn, "n_mn": n_mn,
+                   "n_fp": n_fp, "iterations": it_total},
         "roofline": {"kernel": kname, "bound": "mfma", "achieved": round(achieved, 3),
                      "peak": peak, "unit": "TFLOP/s", "frac": round(achieved / peak, 4),
                      "traffic": None,

@@ -353,7 +353,7 @@ void optimize_device(const DevCsr &csr, uint64_t n, uint64_t dims, std::vector<f
 
 // ---------------------------------------------------------------------------
 // PCA: top-q principal axes of the rows of x, all on the GPU: FP64
-// covariance, then (d <= 128) a Jacobi eigensolve of it, or (d > 128) a block
+// covariance, then (d <= 112) a Jacobi eigensolve of it, or (d > 112) a block
 // subspace iteration with CholeskyQR2 followed by a Jacobi eigensolve of the
 // Rayleigh-Ritz matrix.  Axes are sorted by variance, each signed so its
 // largest-magnitude loading is positive.  Outputs stay on the device.
@@ -365,6 +365,7 @@ struct PcaAxes {
 
 void pca_axes_dev(const float *x_dev, uint64_t n, uint32_t d, uint32_t q, PcaAxes &out, hipStream_t s) {
   constexpr uint32_t kIters = 6;
+  const uint32_t jmax = jacobi_max_size();
   DevBuf<double> d_mean(d), d_scr(cov_scratch_doubles(n, d)), d_cov(static_cast<size_t>(d) * d);
   Timer tc, te;
   tc.start(s);
@@ -375,45 +376,44 @@ void pca_axes_dev(const float *x_dev, uint64_t n, uint32_t d, uint32_t q, PcaAxe
   DevBuf<double> V(static_cast<size_t>(d) * q);
   DevBuf<uint32_t> status(1);
   PM_HIP(launch_fill_zero(status.get(), status.bytes(), s));
-  if (d <= 128) {
-    DevBuf<double> work(static_cast<size_t>(d) * d);
-    PM_HIP(launch_jacobi(d_cov.get(), d, q, work.get(), V.get(), s));
+  if (d <= jmax) {
+    PM_HIP(launch_jacobi(d_cov.get(), d, q, V.get(), s));
   } else {
-    const uint32_t b = std::min<uint32_t>(std::min<uint32_t>(d, q + 16), 128);
+    const uint32_t b = std::min<uint32_t>(std::min<uint32_t>(d, q + 12), jmax);
     if (q > b) fail(PACMAP_ERR_PCA, "too many principal components requested");
     std::vector<double> Q0(static_cast<size_t>(d) * b);
     SmallRng rng = SmallRng::seed_from_u64(0x5eedull);
     for (auto &e : Q0) e = rng.next_f64() - 0.5;
+    const uint64_t part_cap = 16ull * std::max<uint64_t>(static_cast<uint64_t>(b) * b, static_cast<uint64_t>(d) * b / 4);
     DevBuf<double> Q(Q0.size()), Z(Q0.size()), G(static_cast<size_t>(b) * b), Rinv(static_cast<size_t>(b) * b),
-        U(static_cast<size_t>(b) * q), work(static_cast<size_t>(b) * b);
+        U(static_cast<size_t>(b) * q), part(part_cap);
     upload(Q, Q0.data(), Q0.size(), s);
-    // orthonormalise src into dst: two CholeskyQR passes (dst = src R^-1)
-    auto orth = [&](DevBuf<double> &src, DevBuf<double> &dst) {
-      DevBuf<double> *in = &src;
-      for (int pass = 0; pass < 2; ++pass) {
-        PM_HIP(launch_gemm64(b, b, d, in->get(), b, true, in->get(), b, false, G.get(), b, s));
-        PM_HIP(launch_chol_inv(G.get(), b, Rinv.get(), status.get(), s));
-        DevBuf<double> &o = pass == 0 ? (in == &src ? Z : Q) : dst;
-        if (&o == in) {
-          DevBuf<double> tmp(Q0.size());
-          PM_HIP(launch_gemm64(d, b, b, in->get(), b, false, Rinv.get(), b, false, tmp.get(), b, s));
-          PM_HIP(hipMemcpyAsync(o.get(), tmp.get(), tmp.bytes(), hipMemcpyDeviceToDevice, s));
-        } else {
-          PM_HIP(launch_gemm64(d, b, b, in->get(), b, false, Rinv.get(), b, false, o.get(), b, s));
-        }
-        in = &o;
-      }
+    auto gemm = [&](uint32_t M, uint32_t N, uint32_t K, const double *A, uint32_t lda, bool tA, const double *B,
+                    uint32_t ldb, double *C, uint32_t ldc) {
+      PM_HIP(launch_gemm64(M, N, K, A, lda, tA, B, ldb, false, C, ldc, s, part.get(), part_cap));
     };
-    orth(Q, Q);
+    // Q <- orthonormal basis of the columns of Z: two CholeskyQR passes
+    // (Q = Z R^-1, then Q <- Q R'^-1); Z is overwritten.
+    auto orth = [&]() {
+      gemm(b, b, d, Z.get(), b, true, Z.get(), b, G.get(), b);
+      PM_HIP(launch_chol_inv(G.get(), b, Rinv.get(), status.get(), s));
+      gemm(d, b, b, Z.get(), b, false, Rinv.get(), b, Q.get(), b);
+      gemm(b, b, d, Q.get(), b, true, Q.get(), b, G.get(), b);
+      PM_HIP(launch_chol_inv(G.get(), b, Rinv.get(), status.get(), s));
+      gemm(d, b, b, Q.get(), b, false, Rinv.get(), b, Z.get(), b);
+      PM_HIP(hipMemcpyAsync(Q.get(), Z.get(), Z.bytes(), hipMemcpyDeviceToDevice, s));
+    };
+    PM_HIP(hipMemcpyAsync(Z.get(), Q.get(), Q.bytes(), hipMemcpyDeviceToDevice, s));
+    orth();
     for (uint32_t it = 0; it < kIters; ++it) {
-      PM_HIP(launch_gemm64(d, b, d, d_cov.get(), d, false, Q.get(), b, false, Z.get(), b, s));
-      orth(Z, Q);
+      gemm(d, b, d, d_cov.get(), d, false, Q.get(), b, Z.get(), b);
+      orth();
     }
     // Rayleigh-Ritz: T = Q^T C Q, eigenvectors U, axes V = Q U
-    PM_HIP(launch_gemm64(d, b, d, d_cov.get(), d, false, Q.get(), b, false, Z.get(), b, s));
-    PM_HIP(launch_gemm64(b, b, d, Q.get(), b, true, Z.get(), b, false, G.get(), b, s));
-    PM_HIP(launch_jacobi(G.get(), b, q, work.get(), U.get(), s));
-    PM_HIP(launch_gemm64(d, q, b, Q.get(), b, false, U.get(), q, false, V.get(), q, s));
+    gemm(d, b, d, d_cov.get(), d, false, Q.get(), b, Z.get(), b);
+    gemm(b, b, d, Q.get(), b, true, Z.get(), b, G.get(), b);
+    PM_HIP(launch_jacobi(G.get(), b, q, U.get(), s));
+    gemm(d, q, b, Q.get(), b, false, U.get(), q, V.get(), q);
   }
   out.Vf.alloc(static_cast<size_t>(d) * q);
   PM_HIP(launch_sign_fix(V.get(), d, q, out.Vf.get(), s));

@@ -1,8 +1,8 @@
 // linalg.hip -- small dense FP64 linear algebra for the PCA initialisation.
 //
-// The principal axes come from the FP64 covariance (d x d).  For d <= 128 a
+// The principal axes come from the FP64 covariance (d x d).  For d <= 112 a
 // Jacobi eigensolver runs on it directly; for larger d a block subspace
-// iteration (CholeskyQR2 re-orthonormalisation) narrows it to b <= 128
+// iteration (CholeskyQR2 re-orthonormalisation) narrows it to b <= 112
 // directions and the Jacobi solver diagonalises the b x b Rayleigh-Ritz
 // matrix.  Every step is a kernel on the caller's stream, so the whole
 // eigen-step runs without host round trips.
@@ -13,29 +13,34 @@
 namespace pm {
 
 // ---------------------------------------------------------------------------
-// C (M x N) = op(A) (M x K) * op(B) (K x N), row-major FP64, 32 x 32 tiles
+// C (M x N) = op(A) (M x K) * op(B) (K x N), row-major FP64, 32 x 32 tiles.
+// blockIdx.z takes the K range [z * kchunk, (z + 1) * kchunk) and writes the
+// z-th M x ldc slab of C (split-K partials, summed by sum_parts_kernel).
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void gemm64_kernel(uint32_t M, uint32_t N, uint32_t K,
                                                      const double *__restrict__ A, uint32_t lda, int tA,
                                                      const double *__restrict__ B, uint32_t ldb, int tB,
-                                                     double *__restrict__ C, uint32_t ldc) {
+                                                     double *__restrict__ C, uint32_t ldc, uint32_t kchunk) {
   constexpr int TB = 32, KB = 32;
   __shared__ double sA[KB][TB + 1];
   __shared__ double sB[KB][TB + 1];
   const int t = threadIdx.x, ty = t >> 4, tx = t & 15;
   const uint32_t r0 = blockIdx.x * TB, c0 = blockIdx.y * TB;
+  const uint32_t kbeg = blockIdx.z * kchunk;
+  const uint32_t kend = kbeg + kchunk < K ? kbeg + kchunk : K;
+  C += static_cast<size_t>(blockIdx.z) * M * ldc;
   double acc[2][2] = {{0.0, 0.0}, {0.0, 0.0}};
-  for (uint32_t k0 = 0; k0 < K; k0 += KB) {
+  for (uint32_t k0 = kbeg; k0 < kend; k0 += KB) {
     __syncthreads();
     for (int li = t; li < TB * KB; li += 256) {
       const int a = li / KB, b = li % KB;  // tile row / k
       const uint32_t r = r0 + a, k = k0 + b;
       double v = 0.0;
-      if (r < M && k < K) v = tA ? A[static_cast<size_t>(k) * lda + r] : A[static_cast<size_t>(r) * lda + k];
+      if (r < M && k < kend) v = tA ? A[static_cast<size_t>(k) * lda + r] : A[static_cast<size_t>(r) * lda + k];
       sA[b][a] = v;
       const uint32_t kk = k0 + a, c = c0 + b;
       double w = 0.0;
-      if (kk < K && c < N) w = tB ? B[static_cast<size_t>(c) * ldb + kk] : B[static_cast<size_t>(kk) * ldb + c];
+      if (kk < kend && c < N) w = tB ? B[static_cast<size_t>(c) * ldb + kk] : B[static_cast<size_t>(kk) * ldb + c];
       sB[a][b] = w;
     }
     __syncthreads();
@@ -58,123 +63,252 @@ __global__ __launch_bounds__(256) void gemm64_kernel(uint32_t M, uint32_t N, uin
     }
 }
 
+// C[r][c] = sum over z = 0..ks-1 (in order) of part[z][r][c]
+__global__ void sum_parts_kernel(const double *__restrict__ part, uint32_t ks, uint32_t M, uint32_t N,
+                                 double *__restrict__ C, uint32_t ldc) {
+  const uint64_t e = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const uint64_t mn = static_cast<uint64_t>(M) * N;
+  if (e >= mn) return;
+  double acc = part[e];
+  for (uint32_t z = 1; z < ks; ++z) acc += part[z * mn + e];
+  C[(e / N) * ldc + e % N] = acc;
+}
+
 hipError_t launch_gemm64(uint32_t M, uint32_t N, uint32_t K, const double *A, uint32_t lda, bool tA,
-                         const double *B, uint32_t ldb, bool tB, double *C, uint32_t ldc, hipStream_t s) {
+                         const double *B, uint32_t ldb, bool tB, double *C, uint32_t ldc, hipStream_t s,
+                         double *part, uint64_t part_cap) {
   if (M == 0 || N == 0) return hipSuccess;
-  gemm64_kernel<<<dim3((M + 31) / 32, (N + 31) / 32), 256, 0, s>>>(M, N, K, A, lda, tA ? 1 : 0, B, ldb,
-                                                                   tB ? 1 : 0, C, ldc);
+  const uint32_t tiles = ((M + 31) / 32) * ((N + 31) / 32);
+  uint64_t ks = (256 + tiles - 1) / tiles;
+  ks = ks < (K + 63) / 64 ? ks : (K + 63) / 64;
+  const uint64_t mn = static_cast<uint64_t>(M) * N;
+  if (part == nullptr || part_cap < 2 * mn) ks = 1;
+  else if (ks * mn > part_cap) ks = part_cap / mn;
+  if (ks < 1) ks = 1;
+  const uint32_t kchunk = static_cast<uint32_t>((((K + ks - 1) / ks) + 31) / 32 * 32);
+  ks = (K + kchunk - 1) / kchunk;  // no empty slabs
+  if (ks < 1) ks = 1;
+  const dim3 grid((M + 31) / 32, (N + 31) / 32, static_cast<uint32_t>(ks));
+  if (ks == 1) {
+    gemm64_kernel<<<grid, 256, 0, s>>>(M, N, K, A, lda, tA ? 1 : 0, B, ldb, tB ? 1 : 0, C, ldc,
+                                       K > 0 ? K : 1);
+    return hipGetLastError();
+  }
+  gemm64_kernel<<<grid, 256, 0, s>>>(M, N, K, A, lda, tA ? 1 : 0, B, ldb, tB ? 1 : 0, part, N, kchunk);
+  sum_parts_kernel<<<static_cast<uint32_t>((mn + 255) / 256), 256, 0, s>>>(part, static_cast<uint32_t>(ks), M, N,
+                                                                           C, ldc);
   return hipGetLastError();
 }
 
-constexpr int kSmallMax = 128;  // largest matrix the single-workgroup kernels hold in LDS
+// ---------------------------------------------------------------------------
+// G (b x b, SPD, b <= 128) = R^T R; writes R^{-1} (upper triangular, b x b).
+//
+// Elimination on the augmented matrix [G | I]: step j scales row j by
+// 1/sqrt(pivot) and subtracts f_r * (scaled row j) from every row r > j,
+// where f_r is entry r of the scaled row (the Schur complement is
+// symmetric).  The left half ends as R, the right half as E with E G = R,
+// i.e. E = R^{-T}; R^{-1} is its transpose.  The 128 x 256 array lives in
+// registers: thread (rg, cg) of a 16 x 32 grid holds rows rg + 16 a and
+// columns cg + 32 e (a, e < 8); the pivot row goes through a double-buffered
+// LDS row, one barrier per step.  A non-positive pivot restarts with a
+// diagonal shift (1e-12 trace, then x100); status[0] = 1 if six tries fail.
+// ---------------------------------------------------------------------------
+constexpr int kCholMax = 128;
 
-// ---------------------------------------------------------------------------
-// G (b x b, SPD) = R^T R; writes R^{-1} (upper triangular).  A failed pivot
-// retries with a growing diagonal shift; status[0] = 1 if every try fails.
-// One workgroup; G is held in LDS.
-// ---------------------------------------------------------------------------
-__global__ __launch_bounds__(1024) void chol_inv_kernel(const double *__restrict__ G, uint32_t b,
-                                                        double *__restrict__ Rinv,
-                                                        uint32_t *__restrict__ status) {
-  __shared__ double A[kSmallMax * kSmallMax];
-  __shared__ double shift;
-  __shared__ int fail;
-  const int t = threadIdx.x;
+__global__ __launch_bounds__(512) void chol_inv_kernel(const double *__restrict__ G, uint32_t b,
+                                                       double *__restrict__ Rinv,
+                                                       uint32_t *__restrict__ status) {
+  __shared__ double row[2][2 * kCholMax];
+  const int t = threadIdx.x, rg = t >> 5, cg = t & 31;
   double tr = 0.0;
   for (uint32_t j = 0; j < b; ++j) tr += G[static_cast<size_t>(j) * b + j];
-  if (t == 0) shift = 0.0;
-  for (int attempt = 0; attempt < 6; ++attempt) {
-    __syncthreads();
-    for (uint32_t e = t; e < b * b; e += blockDim.x) {
-      const uint32_t r = e / b, c = e % b;
-      A[r * kSmallMax + c] = G[e] + (r == c ? shift : 0.0);
-    }
-    if (t == 0) fail = 0;
-    __syncthreads();
-    for (uint32_t j = 0; j < b; ++j) {  // right-looking, upper factor in rows
-      if (t == 0) {
-        const double dj = A[j * kSmallMax + j];
-        if (!(dj > 0.0)) fail = 1;
-        A[j * kSmallMax + j] = dj > 0.0 ? sqrt(dj) : 1.0;
+  double shift = 0.0;
+  double M[8][8];
+  bool ok = false;
+  for (int attempt = 0; attempt < 6 && !ok; ++attempt) {
+    __syncthreads();  // every thread has left the previous attempt's reads
+#pragma unroll
+    for (int a = 0; a < 8; ++a) {
+      const uint32_t r = rg + 16 * a;
+#pragma unroll
+      for (int e = 0; e < 8; ++e) {
+        const uint32_t c = cg + 32 * e;
+        double v = 0.0;
+        if (r < b) {
+          if (c < kCholMax) {
+            if (c < b) v = G[static_cast<size_t>(r) * b + c] + (r == c ? shift : 0.0);
+          } else {
+            v = (c - kCholMax == r) ? 1.0 : 0.0;
+          }
+        }
+        M[a][e] = v;
       }
-      __syncthreads();
-      if (fail) break;
-      const double rjj = A[j * kSmallMax + j];
-      for (uint32_t i = j + 1 + t; i < b; i += blockDim.x) A[j * kSmallMax + i] /= rjj;
-      __syncthreads();
-      const uint32_t m = b - j - 1;
-      for (uint32_t e = t; e < m * m; e += blockDim.x) {
-        const uint32_t k = j + 1 + e / m, i = j + 1 + e % m;
-        if (i >= k) A[k * kSmallMax + i] -= A[j * kSmallMax + k] * A[j * kSmallMax + i];
-      }
-      __syncthreads();
     }
-    if (!fail) break;
-    if (t == 0) shift = shift == 0.0 ? 1e-12 * tr : shift * 100.0;
-    __syncthreads();
-    if (attempt == 5 && t == 0) status[0] = 1;
+    ok = true;
+#pragma unroll
+    for (int aj = 0; aj < 8; ++aj) {
+      for (uint32_t jj = 0; jj < 16; ++jj) {
+        const uint32_t j = 16 * aj + jj;
+        if (j >= b) break;
+        double *rb = row[j & 1];
+        if (static_cast<uint32_t>(rg) == jj) {
+#pragma unroll
+          for (int e = 0; e < 8; ++e) rb[cg + 32 * e] = M[aj][e];
+        }
+        __syncthreads();
+        const double p = rb[j];
+        if (!(p > 0.0)) {  // same value in every thread: a uniform exit
+          ok = false;
+          break;
+        }
+        const double inv = 1.0 / sqrt(p);
+        double rc[8], f[8];
+#pragma unroll
+        for (int e = 0; e < 8; ++e) rc[e] = rb[cg + 32 * e] * inv;
+#pragma unroll
+        for (int a = 0; a < 8; ++a) {
+          const uint32_t r = rg + 16 * a;
+          f[a] = r > j ? rb[r] * inv : 0.0;
+        }
+#pragma unroll
+        for (int a = 0; a < 8; ++a) {
+          const uint32_t r = rg + 16 * a;
+#pragma unroll
+          for (int e = 0; e < 8; ++e)
+            M[a][e] = (r == j) ? rc[e] : (r > j ? fma(-f[a], rc[e], M[a][e]) : M[a][e]);
+        }
+      }
+      if (!ok) break;
+    }
+    if (!ok) shift = shift == 0.0 ? 1e-12 * tr : shift * 100.0;
   }
-  __syncthreads();
-  // back substitution, one thread per column c of R^{-1}
-  for (uint32_t c = t; c < b; c += blockDim.x) {
-    for (uint32_t i = 0; i < b; ++i)
-      if (i > c) Rinv[static_cast<size_t>(i) * b + c] = 0.0;
-    Rinv[static_cast<size_t>(c) * b + c] = 1.0 / A[c * kSmallMax + c];
-    for (int i = static_cast<int>(c) - 1; i >= 0; --i) {
-      double acc = 0.0;
-      for (uint32_t k = i + 1; k <= c; ++k) acc += A[i * kSmallMax + k] * Rinv[static_cast<size_t>(k) * b + c];
-      Rinv[static_cast<size_t>(i) * b + c] = -acc / A[i * kSmallMax + i];
+  if (!ok) {
+    if (t == 0) status[0] = 1;
+    return;
+  }
+#pragma unroll
+  for (int a = 0; a < 8; ++a) {
+    const uint32_t r = rg + 16 * a;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      const uint32_t c = cg + 32 * e;
+      if (r < b && c >= kCholMax && c - kCholMax < b)
+        Rinv[static_cast<size_t>(c - kCholMax) * b + r] = M[a][e];
     }
   }
 }
 
 // ---------------------------------------------------------------------------
-// Jacobi eigensolver for a symmetric m x m matrix (m <= 128), one workgroup.
-// Parallel (round-robin) ordering: each round applies m/2 disjoint plane
-// rotations at once; m - 1 rounds form a sweep.  Output: the q eigenvectors
-// of largest eigenvalue as columns of Vout (m x q), sign chosen so the
-// largest-magnitude entry is positive.  Vwork: m x m scratch.
+// Jacobi eigensolver for a symmetric m x m matrix (m <= 112), one workgroup.
+//
+// The matrix is kept in LDS as its packed upper triangle and the
+// eigenvectors as V^T (row j = eigenvector j), also in LDS.  Parallel
+// (round-robin) ordering: each round applies the h = ceil(m/2) disjoint
+// rotations of that round at once, as J^T A J: for every two rotation pairs
+// i < j the 2 x 2 block of A between them is updated with both rotations,
+// each pair's own 2 x 2 diagonal block with the closed form (app - t apq,
+// aqq + t apq, 0), and V^T's rows p, q with the rotation.  The per-thread
+// work list (which block or V column each slot handles) is the same every
+// round, so it is decoded once.  Sweeps stop when the off-diagonal square
+// sum is below 1e-28 of the diagonal one.
+// Output: the q eigenvectors of largest eigenvalue (ties by index) as columns
+// of Vout (m x q), each with its largest-magnitude entry positive.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(1024) void jacobi_kernel(const double *__restrict__ Ain, uint32_t m,
-                                                      uint32_t q, double *__restrict__ Vwork,
-                                                      double *__restrict__ Vout) {
-  __shared__ double A[kSmallMax * kSmallMax];
-  __shared__ double cs[kSmallMax / 2], sn[kSmallMax / 2];
-  __shared__ int pp[kSmallMax / 2], qq[kSmallMax / 2];
-  __shared__ double red[2][1024];
-  __shared__ int rank[kSmallMax];
+constexpr int kJacMax = 112;
+constexpr int kJacPairs = kJacMax / 2;
+constexpr int kJacThreads = 1024;
+constexpr int kJacSlots = 8;  // work slots per thread per round
+static_assert(kJacPairs * (kJacPairs - 1) / 2 + kJacPairs + kJacPairs * kJacMax <= kJacThreads * kJacSlots,
+              "Jacobi work list does not fit the slots");
+
+__device__ __forceinline__ uint32_t packed_idx(uint32_t r, uint32_t c, uint32_t m) {
+  if (r > c) {
+    const uint32_t x = r;
+    r = c;
+    c = x;
+  }
+  return r * m - (r * (r - 1)) / 2 + (c - r);  // row r of the upper triangle starts at r*m - r(r-1)/2
+}
+
+__global__ __launch_bounds__(kJacThreads) void jacobi_kernel(const double *__restrict__ Ain, uint32_t m,
+                                                             uint32_t q, double *__restrict__ Vout) {
+  __shared__ double Ap[kJacMax * (kJacMax + 1) / 2];
+  __shared__ double Vt[kJacMax * kJacMax];
+  __shared__ double cs[kJacPairs], sn[kJacPairs], tn[kJacPairs];
+  __shared__ int pp[kJacPairs], qq[kJacPairs];
+  __shared__ double red[2][kJacThreads / 64];
+  __shared__ int rank[kJacMax];
   const int t = threadIdx.x;
   const uint32_t M = (m + 1) & ~1u;  // even count; index m is a dummy when m is odd
-  const uint32_t half = M / 2;
-  for (uint32_t e = t; e < m * m; e += blockDim.x) {
+  const uint32_t h = M / 2;
+  for (uint32_t e = t; e < m * m; e += kJacThreads) {
     const uint32_t r = e / m, c = e % m;
-    A[r * kSmallMax + c] = 0.5 * (Ain[e] + Ain[static_cast<size_t>(c) * m + r]);  // symmetrise
-    Vwork[e] = (r == c) ? 1.0 : 0.0;
+    if (r <= c) Ap[packed_idx(r, c, m)] = 0.5 * (Ain[e] + Ain[static_cast<size_t>(c) * m + r]);
+    Vt[e] = (r == c) ? 1.0 : 0.0;
+  }
+  // Work list: blocks (i < j) enumerated as (x, (x + s) mod h), then the h
+  // diagonal blocks, then the h x m V^T updates.  Code: type << 24 | a << 12 | b.
+  const uint32_t hs = h / 2;
+  const uint32_t full = (h & 1) ? hs : (hs > 0 ? hs - 1 : 0);
+  const uint32_t nb = h * (h - 1) / 2;
+  const uint32_t total = nb + h + h * m;
+  uint32_t slot[kJacSlots];
+#pragma unroll
+  for (int k = 0; k < kJacSlots; ++k) {
+    const uint32_t w = t + k * kJacThreads;
+    uint32_t code = 0xffffffffu;
+    if (w < nb) {
+      uint32_t s, x;
+      if (w < full * h) {
+        s = 1 + w / h;
+        x = w % h;
+      } else {
+        s = hs;
+        x = w - full * h;
+      }
+      uint32_t i = x, j = (x + s) % h;
+      if (i > j) {
+        const uint32_t y = i;
+        i = j;
+        j = y;
+      }
+      code = (0u << 24) | (i << 12) | j;
+    } else if (w < nb + h) {
+      code = (1u << 24) | ((w - nb) << 12);
+    } else if (w < total) {
+      const uint32_t u = w - nb - h;
+      code = (2u << 24) | ((u / m) << 12) | (u % m);
+    }
+    slot[k] = code;
   }
   __syncthreads();
-  for (int sweep = 0; sweep < 60; ++sweep) {
+  for (int sweep = 0; sweep < 40; ++sweep) {
     double off = 0.0, dg = 0.0;
-    for (uint32_t e = t; e < m * m; e += blockDim.x) {
-      const uint32_t r = e / m, c = e % m;
-      const double v = A[r * kSmallMax + c];
-      if (r == c) dg += v * v;
-      else if (r < c) off += v * v;
-    }
-    red[0][t] = off;
-    red[1][t] = dg;
-    __syncthreads();
-    for (int w = 512; w > 0; w >>= 1) {
-      if (t < w) {
-        red[0][t] += red[0][t + w];
-        red[1][t] += red[1][t + w];
+    for (uint32_t r = 0; r < m; ++r) {  // thread t takes columns c = t, t + 1024, ... of every row
+      const uint32_t base = r * m - (r * (r - 1)) / 2;
+      for (uint32_t c = r + t; c < m; c += kJacThreads) {
+        const double v = Ap[base + (c - r)];
+        if (c == r) dg += v * v;
+        else off += v * v;
       }
-      __syncthreads();
     }
-    const double offs = red[0][0], dgs = red[1][0];
+    for (int o = 32; o > 0; o >>= 1) {
+      off += __shfl_xor(off, o);
+      dg += __shfl_xor(dg, o);
+    }
+    if ((t & 63) == 0) {
+      red[0][t >> 6] = off;
+      red[1][t >> 6] = dg;
+    }
     __syncthreads();
-    if (offs <= 1e-30 * dgs || offs == 0.0) break;
+    double offs = 0.0, dgs = 0.0;
+    for (int w = 0; w < kJacThreads / 64; ++w) {
+      offs += red[0][w];
+      dgs += red[1][w];
+    }
+    if (offs == 0.0 || offs <= 1e-28 * dgs) break;  // uniform: every thread summed the same values
     for (uint32_t round = 0; round + 1 < M; ++round) {
-      if (static_cast<uint32_t>(t) < half) {
+      if (static_cast<uint32_t>(t) < h) {
         auto pos = [&](uint32_t j) -> uint32_t { return j == 0 ? 0 : 1 + (j - 1 + round) % (M - 1); };
         uint32_t p = pos(t), r2 = pos(M - 1 - t);
         if (p > r2) {
@@ -182,12 +316,12 @@ __global__ __launch_bounds__(1024) void jacobi_kernel(const double *__restrict__
           p = r2;
           r2 = tmp;
         }
-        double c = 1.0, s = 0.0;
+        double c = 1.0, s = 0.0, tt = 0.0;
         if (r2 < m) {
-          const double apq = A[p * kSmallMax + r2];
+          const double apq = Ap[packed_idx(p, r2, m)];
           if (apq != 0.0) {
-            const double theta = (A[r2 * kSmallMax + r2] - A[p * kSmallMax + p]) / (2.0 * apq);
-            const double tt = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
+            const double theta = (Ap[packed_idx(r2, r2, m)] - Ap[packed_idx(p, p, m)]) / (2.0 * apq);
+            tt = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
             c = 1.0 / sqrt(tt * tt + 1.0);
             s = tt * c;
           }
@@ -196,62 +330,77 @@ __global__ __launch_bounds__(1024) void jacobi_kernel(const double *__restrict__
         qq[t] = static_cast<int>(r2);
         cs[t] = c;
         sn[t] = s;
+        tn[t] = tt;
       }
       __syncthreads();
-      // columns: A <- A J
-      for (uint32_t e = t; e < m * half; e += blockDim.x) {
-        const uint32_t k = e / half, i = e % half;
-        const int p = pp[i], r2 = qq[i];
-        if (static_cast<uint32_t>(r2) >= m) continue;
-        const double akp = A[k * kSmallMax + p], akq = A[k * kSmallMax + r2];
-        A[k * kSmallMax + p] = cs[i] * akp - sn[i] * akq;
-        A[k * kSmallMax + r2] = sn[i] * akp + cs[i] * akq;
-      }
-      __syncthreads();
-      // rows: A <- J^T A; eigenvectors: V <- V J
-      for (uint32_t e = t; e < m * half; e += blockDim.x) {
-        const uint32_t k = e / half, i = e % half;
-        const int p = pp[i], r2 = qq[i];
-        if (static_cast<uint32_t>(r2) >= m) continue;
-        const double apk = A[p * kSmallMax + k], aqk = A[r2 * kSmallMax + k];
-        A[p * kSmallMax + k] = cs[i] * apk - sn[i] * aqk;
-        A[r2 * kSmallMax + k] = sn[i] * apk + cs[i] * aqk;
-        double &vkp = Vwork[static_cast<size_t>(k) * m + p], &vkq = Vwork[static_cast<size_t>(k) * m + r2];
-        const double x = vkp, y = vkq;
-        vkp = cs[i] * x - sn[i] * y;
-        vkq = sn[i] * x + cs[i] * y;
-      }
-      __syncthreads();
-      if (static_cast<uint32_t>(t) < half && static_cast<uint32_t>(qq[t]) < m) {
-        A[pp[t] * kSmallMax + qq[t]] = 0.0;
-        A[qq[t] * kSmallMax + pp[t]] = 0.0;
+#pragma unroll
+      for (int k = 0; k < kJacSlots; ++k) {
+        const uint32_t code = slot[k];
+        if (code == 0xffffffffu) continue;
+        const uint32_t type = code >> 24, a = (code >> 12) & 0xfff, bb = code & 0xfff;
+        if (type == 0) {
+          const uint32_t p1 = pp[a], q1 = qq[a], p2 = pp[bb], q2 = qq[bb];
+          const double c1 = cs[a], s1 = sn[a], c2 = cs[bb], s2 = sn[bb];
+          const bool h1 = q1 < m, h2 = q2 < m;  // false: the pair holds the dummy index
+          const uint32_t i00 = packed_idx(p1, p2, m);
+          const uint32_t i01 = h2 ? packed_idx(p1, q2, m) : 0;
+          const uint32_t i10 = h1 ? packed_idx(q1, p2, m) : 0;
+          const uint32_t i11 = (h1 && h2) ? packed_idx(q1, q2, m) : 0;
+          const double b00 = Ap[i00], b01 = h2 ? Ap[i01] : 0.0;
+          const double b10 = h1 ? Ap[i10] : 0.0, b11 = (h1 && h2) ? Ap[i11] : 0.0;
+          // rows (rotation a), then columns (rotation bb)
+          const double r00 = c1 * b00 - s1 * b10, r01 = c1 * b01 - s1 * b11;
+          const double r10 = s1 * b00 + c1 * b10, r11 = s1 * b01 + c1 * b11;
+          Ap[i00] = c2 * r00 - s2 * r01;
+          if (h2) Ap[i01] = s2 * r00 + c2 * r01;
+          if (h1) Ap[i10] = c2 * r10 - s2 * r11;
+          if (h1 && h2) Ap[i11] = s2 * r10 + c2 * r11;
+        } else if (type == 1) {
+          const uint32_t p = pp[a], r2 = qq[a];
+          const double tt = tn[a];
+          if (r2 < m && tt != 0.0) {
+            const uint32_t ipq = packed_idx(p, r2, m), ipp = packed_idx(p, p, m), iqq = packed_idx(r2, r2, m);
+            const double apq = Ap[ipq];
+            Ap[ipp] = Ap[ipp] - tt * apq;
+            Ap[iqq] = Ap[iqq] + tt * apq;
+            Ap[ipq] = 0.0;
+          }
+        } else {
+          const uint32_t p = pp[a], r2 = qq[a];
+          if (r2 < m) {
+            const double c = cs[a], s = sn[a];
+            const double x = Vt[p * m + bb], y = Vt[r2 * m + bb];
+            Vt[p * m + bb] = c * x - s * y;
+            Vt[r2 * m + bb] = s * x + c * y;
+          }
+        }
       }
       __syncthreads();
     }
   }
   // rank by eigenvalue (descending, ties by index)
-  for (uint32_t j = t; j < m; j += blockDim.x) {
-    const double wj = A[j * kSmallMax + j];
+  for (uint32_t j = t; j < m; j += kJacThreads) {
+    const double wj = Ap[packed_idx(j, j, m)];
     int r = 0;
     for (uint32_t i = 0; i < m; ++i) {
-      const double wi = A[i * kSmallMax + i];
+      const double wi = Ap[packed_idx(i, i, m)];
       r += (wi > wj || (wi == wj && i < j)) ? 1 : 0;
     }
     rank[j] = r;
   }
   __syncthreads();
-  for (uint32_t j = t; j < m; j += blockDim.x) {
+  for (uint32_t j = t; j < m; j += kJacThreads) {
     const uint32_t col = static_cast<uint32_t>(rank[j]);
     if (col >= q) continue;
     double best = -1.0, sg = 1.0;
     for (uint32_t k = 0; k < m; ++k) {
-      const double v = Vwork[static_cast<size_t>(k) * m + j];
+      const double v = Vt[j * m + k];
       if (fabs(v) > best) {
         best = fabs(v);
         sg = v < 0 ? -1.0 : 1.0;
       }
     }
-    for (uint32_t k = 0; k < m; ++k) Vout[static_cast<size_t>(k) * q + col] = sg * Vwork[static_cast<size_t>(k) * m + j];
+    for (uint32_t k = 0; k < m; ++k) Vout[static_cast<size_t>(k) * q + col] = sg * Vt[j * m + k];
   }
 }
 
@@ -275,15 +424,17 @@ __global__ void sign_fix_kernel(double *__restrict__ V, uint32_t d, uint32_t q, 
   }
 }
 
+uint32_t jacobi_max_size() { return kJacMax; }
+
 hipError_t launch_chol_inv(const double *G, uint32_t b, double *Rinv, uint32_t *status, hipStream_t s) {
-  if (b > kSmallMax) return hipErrorInvalidValue;
-  chol_inv_kernel<<<1, 1024, 0, s>>>(G, b, Rinv, status);
+  if (b == 0 || b > kCholMax) return hipErrorInvalidValue;
+  chol_inv_kernel<<<1, 512, 0, s>>>(G, b, Rinv, status);
   return hipGetLastError();
 }
 
-hipError_t launch_jacobi(const double *A, uint32_t m, uint32_t q, double *Vwork, double *Vout, hipStream_t s) {
-  if (m > kSmallMax || q > m) return hipErrorInvalidValue;
-  jacobi_kernel<<<1, 1024, 0, s>>>(A, m, q, Vwork, Vout);
+hipError_t launch_jacobi(const double *A, uint32_t m, uint32_t q, double *Vout, hipStream_t s) {
+  if (m == 0 || m > kJacMax || q > m) return hipErrorInvalidValue;
+  jacobi_kernel<<<1, kJacThreads, 0, s>>>(A, m, q, Vout);
   return hipGetLastError();
 }
 

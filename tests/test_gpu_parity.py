@@ -462,3 +462,57 @@ def test_fit_transform_pca_path_high_dimensional():
     assert pres > 5 * 10 / n
     # clusters separate: same-label neighbours dominate in the embedding
     assert (lab[from_out] == lab[:, None]).mean() > 0.9
+
+
+def _pca_init_expected(x, comps):
+    """FP64 numpy restatement of the PCA initialisation: Y0 = (x - mean) V[:, :2]
+    * 0.01 (lib.rs:767-768), V the leading covariance eigenvectors, each signed
+    so that its largest-magnitude loading is positive."""
+    xd = x.astype(np.float64)
+    xc = xd - xd.mean(axis=0)
+    w, v = np.linalg.eigh(xc.T @ xc)
+    order = np.argsort(-w, kind="stable")[:comps]
+    v = v[:, order]
+    v = v * np.sign(v[np.abs(v).argmax(axis=0), np.arange(comps)])
+    return xc @ v[:, :2] * 0.01, w[order]
+
+
+@pytest.mark.parametrize("n,d", [(4000, 60), (4000, 120), (3000, 300)])
+def test_pca_initialisation_matches_fp64_eigensolve(n, d):
+    """With 0 iterations fit_transform returns its PCA initialisation.  d <= 100
+    normalises and takes an exact 2-component PCA (lib.rs:489-519); d > 100
+    projects onto the leading 100 axes (lib.rs:456-484) and keeps the first two
+    columns.  Both must equal the FP64 eigenvectors of the covariance (parity
+    unpinned against petal-decomposition; this pins the axes themselves).
+    Spectrum 10 * 0.85^k keeps every eigenvalue gap wide."""
+    rng = np.random.default_rng(n + d)
+    basis, _ = np.linalg.qr(rng.standard_normal((d, d)))
+    x = (rng.standard_normal((n, d)) * (10.0 * 0.85 ** np.arange(d))) @ basis.T + 3.0
+    x = x.astype(np.float32)
+    cfg = pm.Configuration(seed=3, num_iters=(0, 0, 0))
+    y, _ = pm.fit_transform(x, cfg)
+    if d <= 100:
+        xn = x.astype(np.float32)
+        xn = (xn - xn.min()) / xn.max()  # lib.rs:500-503 (global min and max)
+        xn = xn - xn.astype(np.float64).mean(axis=0)
+        exp, _ = _pca_init_expected(xn, 2)
+    else:
+        exp, _ = _pca_init_expected(x, 100)
+    err = np.abs(y - exp).max() / np.abs(exp).max()
+    print(f"PCA init: max error {err:.2e} of max |Y0|", flush=True)
+    assert err < 1e-4
+
+
+def test_pca_initialisation_rank_deficient():
+    """n < d: the covariance has rank n - 1 < 100 requested axes; the
+    orthonormalisation takes its shifted retry and the leading axes are still
+    the FP64 ones."""
+    rng = np.random.default_rng(5)
+    n, d = 60, 300
+    x = (rng.standard_normal((n, d)) * np.linspace(3.0, 0.5, d)).astype(np.float32)
+    y, _ = pm.fit_transform(x, pm.Configuration(seed=3, num_iters=(0, 0, 0)))
+    exp, w = _pca_init_expected(x, min(100, n))
+    assert np.isfinite(y).all()
+    err = np.abs(y - exp).max() / np.abs(exp).max()
+    print(f"rank-deficient PCA init: max error {err:.2e}", flush=True)
+    assert err < 1e-3
