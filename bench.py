@@ -52,7 +52,9 @@ def config(pm):
 # Large single-GPU run: 1,000,000 x 128 from a 100-component Gaussian mixture
 # with Zipf(1.1) cluster sizes (centres ~ N(0, 4I), within-cluster std 1);
 # n_neighbors 10 with the default ratios (10 NN, 5 MN, 20 FP pairs per point),
-# PCA init (128 -> 100 dims), bf16 kNN candidates with the exact FP32 rerank.
+# PCA init (128 -> 100 dims), split-bf16 MFMA kNN candidates with the exact FP32
+# rerank (single-bf16 candidates cannot certify this data: nearly every row
+# falls back to the exact rescan).
 LARGE = dict(n=1_000_000, d=128, k=10, n_mn=5, n_fp=20)
 
 
@@ -69,7 +71,7 @@ def make_large(seed=7):
 
 def config_large(pm):
     return pm.Configuration(initialization=pm.Initialization.Pca(), override_neighbors=LARGE["k"],
-                            seed=42, approx_threshold=2 ** 63, num_iters=ITERS, knn_precision="bf16")
+                            seed=42, approx_threshold=2 ** 63, num_iters=ITERS)
 
 
 def grad_adam_bytes(n, pairs, dims):
@@ -114,15 +116,105 @@ def cpu_baseline(x, threads):
                        "PCA not timed")}
 
 
+# Learned encoder: a noisy Swiss roll (t ~ U[1.5 pi, 4.5 pi], h ~ U[0, 21],
+# 3-D noise std 0.3) mapped into 100 dims by a random orthogonal 100 x 3 map,
+# plus N(0, 0.01^2) noise; 150,000 training and 100,000 held-out points.
+ENC = dict(n_train=150_000, n_test=100_000, d=100, hidden=256, epochs=20, batch_pairs=8192)
+
+
+def make_swiss_roll(n, rng, basis):
+    t = rng.uniform(1.5 * np.pi, 4.5 * np.pi, n)
+    h = rng.uniform(0.0, 21.0, n)
+    p3 = np.stack([t * np.cos(t), h, t * np.sin(t)], 1) + rng.normal(0.0, 0.3, (n, 3))
+    return (p3 @ basis.T + rng.normal(0.0, 0.01, (n, basis.shape[0]))).astype(np.float32), t
+
+
+def knn_preservation(pm, x_in, y, k=10):
+    """Mean fraction of each point's k nearest input-space neighbours that are
+    also among its k nearest embedding neighbours."""
+    a, _ = pm.knn.find_k_nearest_neighbors(x_in, k)
+    b, _ = pm.knn.find_k_nearest_neighbors(y, k)
+    a.sort(axis=1)
+    b.sort(axis=1)
+    hits = sum(np.intersect1d(ra, rb, assume_unique=True).size for ra, rb in zip(a, b))
+    return hits / a.size
+
+
+def triplet_accuracy(x_in, y, rng, m=200_000):
+    """Fraction of random triplets (i, j, l) whose distance order d(i,j) < d(i,l)
+    is the same in the input and in the embedding."""
+    n = x_in.shape[0]
+    i, j, l = (rng.integers(0, n, m) for _ in range(3))
+    din = (np.linalg.norm(x_in[i] - x_in[j], axis=1) < np.linalg.norm(x_in[i] - x_in[l], axis=1))
+    dem = (np.linalg.norm(y[i] - y[j], axis=1) < np.linalg.norm(y[i] - y[l], axis=1))
+    return float((din == dem).mean())
+
+
+def encoder_run(pm):
+    """Trains the encoder on the method's own pairs of the training set, then
+    embeds the held-out set by the encoder and by the plain optimiser."""
+    rng = np.random.default_rng(42)
+    basis, _ = np.linalg.qr(rng.standard_normal((ENC["d"], 3)))
+    x_tr, _ = make_swiss_roll(ENC["n_train"], rng, basis)
+    x_te, _ = make_swiss_roll(ENC["n_test"], rng, basis)
+    cfg = pm.Configuration(seed=42)
+    t0 = time.perf_counter()
+    nn, mn, fp, xp_tr = pm.generate_pairs(x_tr, cfg)
+    t_pairs = time.perf_counter() - t0
+    # the training set's preprocessing (lib.rs:489-511) applied to held-out points
+    lo, hi = float(x_tr.min()), float(x_tr.max())
+    mean = ((x_tr - np.float32(lo)) / np.float32(hi)).mean(axis=0, dtype=np.float64).astype(np.float32)
+    xp_te = np.ascontiguousarray((x_te - np.float32(lo)) / np.float32(hi) - mean, np.float32)
+    p0 = pm.mlp.init(ENC["d"], ENC["hidden"], 2, seed=0)
+    log(f"bench: training the encoder on {len(nn) + len(mn) + len(fp)} pairs, {ENC['epochs']} epochs")
+    t0 = time.perf_counter()
+    params, eloss = pm.mlp.fit(xp_tr, nn, mn, fp, p0, hidden=ENC["hidden"], epochs=ENC["epochs"],
+                               batch_pairs=ENC["batch_pairs"], seed=1)
+    t_train = time.perf_counter() - t0
+    pm.mlp.transform(x_te[:1000] * 0, params, hidden=ENC["hidden"])  # warm the code path
+    t0 = time.perf_counter()
+    y_mlp = pm.mlp.transform(xp_te, params, hidden=ENC["hidden"])
+    t_mlp = time.perf_counter() - t0
+    xd = pm.DeviceArray(xp_te)
+    t0 = time.perf_counter()
+    pm.mlp.transform(xd, params, hidden=ENC["hidden"])
+    t_mlp_dev = time.perf_counter() - t0
+    log("bench: plain optimiser on the held-out set")
+    pm.fit_transform(x_te[:2000], cfg)  # warm
+    t0 = time.perf_counter()
+    y_plain, _ = pm.fit_transform(x_te, cfg)
+    t_plain = time.perf_counter() - t0
+    log("bench: scoring")
+    pres_mlp = knn_preservation(pm, xp_te, y_mlp)
+    pres_plain = knn_preservation(pm, xp_te, y_plain)
+    trip_mlp = triplet_accuracy(xp_te, y_mlp, np.random.default_rng(7))
+    trip_plain = triplet_accuracy(xp_te, y_plain, np.random.default_rng(7))
+    return {
+        "metric": METRIC, "value": round(t_plain / t_mlp, 2), "unit": "x (plain fit / encoder transform)",
+        "n_gpus": 1, "steps": 1, "warmup": 1, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "bf16 weights, fp32 accumulate", "data": "synthetic",
+        "config": {"workload": "encoder 100->256->256->2 (GELU) on a Swiss roll in 100 dims: 150000 "
+                               "training points (its own Generate pairs), 100000 held-out points",
+                   "epochs": ENC["epochs"], "batch_pairs": ENC["batch_pairs"]},
+        "encoder": {"train_s": round(t_train, 3), "pair_generation_s": round(t_pairs, 3),
+                    "epoch_loss": [round(float(v), 5) for v in eloss],
+                    "transform_ms": round(t_mlp * 1e3, 3), "transform_resident_ms": round(t_mlp_dev * 1e3, 3),
+                    "plain_fit_ms": round(t_plain * 1e3, 3),
+                    "knn10_preservation": {"encoder": round(pres_mlp, 4), "plain": round(pres_plain, 4)},
+                    "triplet_accuracy": {"encoder": round(trip_mlp, 4), "plain": round(trip_plain, 4)}},
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", choices=("70k", "1m"), default="70k",
+    ap.add_argument("--workload", choices=("70k", "1m", "encoder"), default="70k",
                     help="70k: the 70000 x 784 headline run; 1m: the 1,000,000 x 128 large run "
-                         "(no CPU baseline)")
+                         "(no CPU baseline); encoder: the learned encoder vs the plain optimiser "
+                         "on held-out Swiss-roll points")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU design is "
@@ -130,6 +222,9 @@ def main():
         sys.exit(2)
     import pacmap_mi355x as pm
 
+    if args.workload == "encoder":
+        print(json.dumps(encoder_run(pm)), flush=True)
+        return
     large = args.workload == "1m"
     if large:
         n, d, k_nn, n_mn, n_fp = (LARGE[k] for k in ("n", "d", "k", "n_mn", "n_fp"))
@@ -179,7 +274,7 @@ def main():
         "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "fp32",
         "data": "synthetic",
         "config": {"workload": ("fit_transform 1000000x128 GMM(100, Zipf 1.1 sizes), PCA init -> "
-                                "100 dims, exact kNN from bf16 candidates k=10 (60 candidates), "
+                                "100 dims, exact kNN from split-bf16 candidates k=10 (60 candidates), "
                                 "5 MN + 20 FP per point, 450 iterations" if large else
                                 "fit_transform 70000x784 GMM(10), PCA init -> 100 dims, exact kNN "
                                 "k=10 (60 candidates), 10 MN + 20 FP per point, 450 iterations"),

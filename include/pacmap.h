@@ -167,6 +167,54 @@ int pacmap_optimize(float *y, uint64_t n, uint64_t dims, const uint32_t *pair_nn
                     uint64_t num_snapshots, float *snapshots_out, float *loss_out,
                     pacmap_status *st);
 
+/* The pairs of PairConfiguration::Generate (lib.rs:629-650 with
+ * neighbors.rs:56-108): preprocess x as fit_transform does (lib.rs:442-532),
+ * then exact kNN, sigma/scale, NN pairs and seeded MN/FP sampling.  With
+ * pair_nn_out NULL only counts_out is filled: NN, MN and FP row counts and
+ * the preprocessed width d'.  Otherwise pair_*_out receive rows x 2 u32
+ * (i-major) and x_pre_out (optional) the preprocessed data n x d'. */
+int pacmap_generate_pairs(const float *x, uint64_t n, uint64_t d, const pacmap_config *cfg,
+                          uint32_t *pair_nn_out, uint32_t *pair_mn_out, uint32_t *pair_fp_out,
+                          float *x_pre_out, uint64_t counts_out[4], pacmap_status *st);
+
+/* ---- Parametric encoder (no counterpart in the reference) ----------------
+ * An MLP d -> hidden -> hidden -> dims with GELU, trained with the PaCMAP
+ * NN/MN/FP loss (gradient.rs:98-151 terms) on the method's own pairs, for
+ * out-of-sample transform and warm-start initialisation.  BF16 weights and
+ * activations on MFMA with FP32 accumulation; FP32 master weights and Adam.
+ * Parameters (FP32, row-major): W1 (hidden x d), b1, W2 (hidden x hidden), b2,
+ * W3 (dims x hidden), b3. */
+typedef struct {
+  uint64_t hidden;            /* 256 (multiple of 64, at most 1024) */
+  uint64_t epochs;            /* 20 */
+  uint64_t batch_pairs;       /* 8192 pairs per step (16384 encoder rows) */
+  float learning_rate;        /* Adam, 1e-3 */
+  uint64_t seed;              /* pair order of each epoch */
+  uint64_t schedule_iters[3]; /* the weight phases of weights.rs spread over training */
+} pacmap_mlp_config;
+void pacmap_mlp_config_default(pacmap_mlp_config *cfg);
+uint64_t pacmap_mlp_param_count(uint64_t d, uint64_t hidden, uint64_t dims);
+/* U(-1/sqrt(fan_in), 1/sqrt(fan_in)) initial parameters from a seed. */
+int pacmap_mlp_init(uint64_t d, uint64_t hidden, uint64_t dims, uint64_t seed, float *params,
+                    pacmap_status *st);
+/* Trains params (in: initial, out: trained) on x (n x d) and the three pair
+ * lists; epoch_loss (epochs floats, optional) gets the mean pair loss. */
+int pacmap_mlp_fit(const float *x, uint64_t n, uint64_t d, uint64_t dims, const uint32_t *pair_nn,
+                   uint64_t n_nn, const uint32_t *pair_mn, uint64_t n_mn, const uint32_t *pair_fp,
+                   uint64_t n_fp, const pacmap_mlp_config *cfg, float *params, float *epoch_loss,
+                   pacmap_status *st);
+/* y_out (n x dims) = encoder(x); x on the host or (``_dev``) in device memory. */
+int pacmap_mlp_transform(const float *x, uint64_t n, uint64_t d, uint64_t hidden, uint64_t dims,
+                         const float *params, float *y_out, pacmap_status *st);
+int pacmap_mlp_transform_dev(const float *x_dev, uint64_t n, uint64_t d, uint64_t hidden,
+                             uint64_t dims, const float *params, float *y_out, pacmap_status *st);
+/* Seam: mean loss and its gradient over one batch (all given pairs, at most
+ * 65536) with weights (w_nn, w_mn, w_fp); grad_out has the parameter layout. */
+int pacmap_mlp_grad(const float *x, uint64_t n, uint64_t d, uint64_t hidden, uint64_t dims,
+                    const float *params, const uint32_t *pair_nn, uint64_t n_nn,
+                    const uint32_t *pair_mn, uint64_t n_mn, const uint32_t *pair_fp, uint64_t n_fp,
+                    const float weights[3], float *grad_out, float *loss_out, pacmap_status *st);
+
 /* Device buffers for callers that keep inputs resident in HBM. */
 int pacmap_dev_alloc(uint64_t bytes, void **ptr_out, pacmap_status *st);
 int pacmap_dev_free(void *ptr);
@@ -180,13 +228,13 @@ typedef struct {
   double upload_ms;
   double preprocess_ms;       /* normalise or PCA (covariance, eigensolve, projection) */
   double pca_cov_ms;          /* covariance kernel alone */
-  double pca_eig_ms;          /* host eigensolve */
+  double pca_eig_ms;          /* eigen-step kernels (subspace iteration, Jacobi) */
   double knn_ms;              /* kNN kernels */
   double knn_kernel_ms;       /* the dominant kNN kernel alone */
   uint64_t knn_kernel_launches;
   double sigma_ms;            /* sigma/scale/NN-pair kernels */
-  double sampling_ms;         /* host MN/FP sampling */
-  double csr_ms;              /* host CSR build + upload */
+  double sampling_ms;         /* MN/FP sampling kernels */
+  double csr_ms;              /* CSR build (degrees, scan, fill, row sort) */
   double graph_build_ms;      /* capture + instantiate */
   double optimize_ms;         /* the graph of all iterations */
   uint64_t iterations;

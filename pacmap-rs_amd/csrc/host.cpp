@@ -15,49 +15,6 @@ namespace {
 
 thread_local pacmap_timings g_tm;
 
-void set_status(pacmap_status *st, int code, const char *msg) {
-  if (!st) return;
-  std::memset(st, 0, sizeof(*st));
-  st->code = code;
-  if (msg) {
-    std::strncpy(st->message, msg, sizeof(st->message) - 1);
-  }
-}
-
-template <class F>
-int guarded(pacmap_status *st, F &&body) {
-  try {
-    body();
-    set_status(st, PACMAP_OK, nullptr);
-    return PACMAP_OK;
-  } catch (const PmError &e) {
-    if (st) {
-      set_status(st, e.code, e.what());
-      st->gpu_error = e.gpu_error;
-      st->expected[0] = e.expected[0];
-      st->expected[1] = e.expected[1];
-      st->actual[0] = e.actual[0];
-      st->actual[1] = e.actual[1];
-    }
-    return e.code;
-  } catch (const std::bad_alloc &) {
-    set_status(st, PACMAP_ERR_GPU, "host allocation failed");
-    return PACMAP_ERR_GPU;
-  } catch (const std::exception &e) {
-    set_status(st, PACMAP_ERR_INVALID_ARGUMENT, e.what());
-    return PACMAP_ERR_INVALID_ARGUMENT;
-  }
-}
-
-template <class T>
-void upload(DevBuf<T> &dst, const T *src, size_t count, hipStream_t s) {
-  if (count) PM_HIP(hipMemcpyAsync(dst.get(), src, count * sizeof(T), hipMemcpyHostToDevice, s));
-}
-template <class T>
-void download(T *dst, const T *src, size_t count, hipStream_t s) {
-  if (count) PM_HIP(hipMemcpyAsync(dst, src, count * sizeof(T), hipMemcpyDeviceToHost, s));
-}
-
 // ---------------------------------------------------------------------------
 // exact kNN on a device matrix (knn.rs:137-194)
 // ---------------------------------------------------------------------------
@@ -485,12 +442,22 @@ void random_init(uint64_t n, uint64_t dims, bool has_seed, uint64_t seed, std::v
 // ---------------------------------------------------------------------------
 // fit_transform (lib.rs:294-367)
 // ---------------------------------------------------------------------------
+// Outputs of the pair-generation entry point: fit_impl stops after sampling
+// and returns the three pair lists (rows x 2) and the preprocessed features.
+struct PairsOut {
+  uint32_t *nn = nullptr, *mn = nullptr, *fp = nullptr;
+  float *x_pre = nullptr;  // n x dk, optional
+  uint64_t counts[4] = {0, 0, 0, 0};  // NN, MN, FP rows; dk
+  bool query = false;                 // only fill counts
+};
+
 void fit_impl(const float *x_host, const float *x_dev_in, uint64_t n, uint64_t d,
-              const pacmap_config *cfg, float *y_out, float *snaps_out, float *loss_out) {
+              const pacmap_config *cfg, float *y_out, float *snaps_out, float *loss_out,
+              PairsOut *po = nullptr) {
   const double t0 = now_ms();
   if (!cfg) fail(PACMAP_ERR_INVALID_ARGUMENT, "config is NULL");
   if (n <= 1) fail(PACMAP_ERR_SAMPLE_SIZE, "Sample size must be larger than one");  // lib.rs:300-302
-  if (!y_out) fail(PACMAP_ERR_INVALID_ARGUMENT, "y_out is NULL");
+  if (!y_out && !po) fail(PACMAP_ERR_INVALID_ARGUMENT, "y_out is NULL");
   if (!x_host && !x_dev_in && d) fail(PACMAP_ERR_INVALID_ARGUMENT, "x is NULL");
   const uint64_t dims = cfg->embedding_dimensions;
   if (dims == 0) fail(PACMAP_ERR_INVALID_ARGUMENT, "embedding_dimensions must be at least 1");
@@ -531,6 +498,15 @@ void fit_impl(const float *x_host, const float *x_dev_in, uint64_t n, uint64_t d
   }
   if (cfg->has_snapshots && cfg->num_snapshots && !cfg->snapshots)
     fail(PACMAP_ERR_INVALID_ARGUMENT, "snapshots is NULL");
+  if (po) {
+    if (pk != PACMAP_PAIRS_GENERATE) fail(PACMAP_ERR_INVALID_ARGUMENT, "pair generation needs PairConfiguration::Generate");
+    po->counts[0] = n * std::min<uint64_t>(k, n - 1);
+    po->counts[1] = n * n_mn;
+    po->counts[2] = n * n_fp;
+    po->counts[3] = pca_path ? std::min<uint64_t>(100, n) : d;
+    if (po->query) return;
+    if (!po->nn || (n_mn && !po->mn) || (n_fp && !po->fp)) fail(PACMAP_ERR_INVALID_ARGUMENT, "NULL pair output");
+  }
 
   Stream stream;
   const hipStream_t s = stream.get();
@@ -603,6 +579,14 @@ void fit_impl(const float *x_host, const float *x_dev_in, uint64_t n, uint64_t d
                                cfg->has_seed ? 1 : 0, d_mn.get(), d_fp.get(), s));
     tsmp.stop(s);
     g_tm.sampling_ms += tsmp.ms();
+    if (po) {
+      download(po->nn, d_nn, 2 * n * knn, s);
+      if (po->mn) download(po->mn, d_mn.get(), 2 * n * n_mn, s);
+      if (po->fp) download(po->fp, d_fp.get(), 2 * n * n_fp, s);
+      if (po->x_pre) download(po->x_pre, d_xk.get(), n * dk, s);
+      PM_HIP(hipStreamSynchronize(s));
+      return;
+    }
     const uint32_t *dp[3] = {d_nn, d_mn.get(), d_fp.get()};
     const uint64_t rows[3] = {n * knn, n * n_mn, n * n_fp};
     build_csr_dev(n, dp, rows, csr, s);
@@ -624,6 +608,7 @@ void fit_impl(const float *x_host, const float *x_dev_in, uint64_t n, uint64_t d
     tsg.stop(s);
     g_tm.sigma_ms += tsg.ms();
     sample_and_build(d_nn.get(), knn);
+    if (po) return;
   } else if (pk == PACMAP_PAIRS_NEIGHBORS_PROVIDED) {  // lib.rs:651-676
     for (uint64_t r = 0; r < 2 * n * k; ++r)
       if (cfg->pair_neighbors[r] >= n) fail(PACMAP_ERR_INVALID_PAIR_INDEX, "pair_neighbors index out of range");
@@ -817,6 +802,23 @@ int pacmap_sigma_scale(const float *knn_dist, const uint32_t *knn_idx, uint64_t 
     if (scaled_out) download(scaled_out, dsc.get(), n * kprime, s);
     if (pairs_out) download(pairs_out, dp.get(), n * n_neighbors * 2, s);
     PM_HIP(hipStreamSynchronize(s));
+  });
+}
+
+int pacmap_generate_pairs(const float *x, uint64_t n, uint64_t d, const pacmap_config *cfg,
+                          uint32_t *pair_nn_out, uint32_t *pair_mn_out, uint32_t *pair_fp_out,
+                          float *x_pre_out, uint64_t counts_out[4], pacmap_status *st) {
+  g_tm = pacmap_timings{};
+  return guarded(st, [&] {
+    PairsOut po;
+    po.query = pair_nn_out == nullptr;
+    po.nn = pair_nn_out;
+    po.mn = pair_mn_out;
+    po.fp = pair_fp_out;
+    po.x_pre = x_pre_out;
+    fit_impl(x, nullptr, n, d, cfg, nullptr, nullptr, nullptr, &po);
+    if (counts_out)
+      for (int i = 0; i < 4; ++i) counts_out[i] = po.counts[i];
   });
 }
 

@@ -6,6 +6,7 @@
 #include <cstdint>
 #include <cstring>
 #include <functional>
+#include <new>
 #include <stdexcept>
 #include <string>
 #include <thread>
@@ -107,6 +108,51 @@ class Timer {
  private:
   hipEvent_t a_ = nullptr, b_ = nullptr;
 };
+
+inline void set_status(pacmap_status *st, int code, const char *msg) {
+  if (!st) return;
+  std::memset(st, 0, sizeof(*st));
+  st->code = code;
+  if (msg) {
+    std::strncpy(st->message, msg, sizeof(st->message) - 1);
+  }
+}
+
+// Runs body(); converts a PmError (or another exception) into the status
+// code and message of the C boundary.
+template <class F>
+int guarded(pacmap_status *st, F &&body) {
+  try {
+    body();
+    set_status(st, PACMAP_OK, nullptr);
+    return PACMAP_OK;
+  } catch (const PmError &e) {
+    if (st) {
+      set_status(st, e.code, e.what());
+      st->gpu_error = e.gpu_error;
+      st->expected[0] = e.expected[0];
+      st->expected[1] = e.expected[1];
+      st->actual[0] = e.actual[0];
+      st->actual[1] = e.actual[1];
+    }
+    return e.code;
+  } catch (const std::bad_alloc &) {
+    set_status(st, PACMAP_ERR_GPU, "host allocation failed");
+    return PACMAP_ERR_GPU;
+  } catch (const std::exception &e) {
+    set_status(st, PACMAP_ERR_INVALID_ARGUMENT, e.what());
+    return PACMAP_ERR_INVALID_ARGUMENT;
+  }
+}
+
+template <class T>
+void upload(DevBuf<T> &dst, const T *src, size_t count, hipStream_t s) {
+  if (count) PM_HIP(hipMemcpyAsync(dst.get(), src, count * sizeof(T), hipMemcpyHostToDevice, s));
+}
+template <class T>
+void download(T *dst, const T *src, size_t count, hipStream_t s) {
+  if (count) PM_HIP(hipMemcpyAsync(dst, src, count * sizeof(T), hipMemcpyDeviceToHost, s));
+}
 
 int host_threads();
 // Runs fn(begin, end) over [0, total) split into contiguous ranges.

@@ -837,13 +837,24 @@ __global__ void colsum_part_kernel(const float *__restrict__ x, uint64_t n, uint
   part[static_cast<size_t>(sp) * d + c] = s;
 }
 
-__global__ void colmean_fin_kernel(const double *__restrict__ part, uint32_t d, uint32_t splits,
-                                   uint64_t n, double *__restrict__ mean) {
-  const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= d) return;
+// 16 columns per block; 16 thread groups stride the splits, then a fixed-order
+// LDS sum over the groups (deterministic).
+__global__ __launch_bounds__(256) void colmean_fin_kernel(const double *__restrict__ part, uint32_t d,
+                                                          uint32_t splits, uint64_t n,
+                                                          double *__restrict__ mean) {
+  __shared__ double red[16][17];
+  const int cl = threadIdx.x & 15, g = threadIdx.x >> 4;
+  const uint32_t c = blockIdx.x * 16 + cl;
   double s = 0.0;
-  for (uint32_t sp = 0; sp < splits; ++sp) s += part[static_cast<size_t>(sp) * d + c];
-  mean[c] = s / static_cast<double>(n);
+  if (c < d)
+    for (uint32_t sp = g; sp < splits; sp += 16) s += part[static_cast<size_t>(sp) * d + c];
+  red[g][cl] = s;
+  __syncthreads();
+  if (g == 0 && c < d) {
+    double tot = 0.0;
+    for (int k = 0; k < 16; ++k) tot += red[k][cl];
+    mean[c] = tot / static_cast<double>(n);
+  }
 }
 
 hipError_t launch_colmean_f64(const float *x, uint64_t n, uint32_t d, double *mean,
@@ -851,7 +862,7 @@ hipError_t launch_colmean_f64(const float *x, uint64_t n, uint32_t d, double *me
   const uint32_t splits = colsum_splits(n);
   dim3 g((d + 63) / 64, splits);
   colsum_part_kernel<<<g, 64, 0, s>>>(x, n, d, splits, scratch);
-  colmean_fin_kernel<<<(d + 63) / 64, 64, 0, s>>>(scratch, d, splits, n, mean);
+  colmean_fin_kernel<<<(d + 15) / 16, 256, 0, s>>>(scratch, d, splits, n, mean);
   return hipGetLastError();
 }
 
@@ -906,17 +917,18 @@ __global__ __launch_bounds__(256) void cov_part_kernel(const float *__restrict__
     }
 }
 
+// Sums the split partials of the upper triangle of tiles and writes both
+// (a, b) and (b, a); reads are coalesced along b.
 __global__ void cov_fin_kernel(const double *__restrict__ part, uint32_t d, uint32_t splits,
                                double *__restrict__ cov) {
   const uint64_t e = blockIdx.x * 256ull + threadIdx.x;
   if (e >= static_cast<uint64_t>(d) * d) return;
   const uint32_t a = static_cast<uint32_t>(e / d), b = static_cast<uint32_t>(e % d);
-  // only the upper triangle of tiles was written
-  const uint32_t ta = a / kCovTile, tb = b / kCovTile;
-  const uint64_t src = (tb >= ta) ? static_cast<uint64_t>(a) * d + b : static_cast<uint64_t>(b) * d + a;
+  if (b / kCovTile < a / kCovTile) return;  // lower tiles: written by their mirror
   double s = 0.0;
-  for (uint32_t sp = 0; sp < splits; ++sp) s += part[static_cast<size_t>(sp) * d * d + src];
+  for (uint32_t sp = 0; sp < splits; ++sp) s += part[static_cast<size_t>(sp) * d * d + e];
   cov[e] = s;
+  if (b / kCovTile > a / kCovTile) cov[static_cast<uint64_t>(b) * d + a] = s;
 }
 
 hipError_t launch_cov_f64(const float *x, uint64_t n, uint32_t d, const double *mean,

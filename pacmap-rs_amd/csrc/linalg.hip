@@ -404,20 +404,40 @@ __global__ __launch_bounds__(kJacThreads) void jacobi_kernel(const double *__res
   }
 }
 
-// Column signs of V (d x q): largest-magnitude entry positive; also writes
-// the FP32 copy used by the projection kernel.
-__global__ void sign_fix_kernel(double *__restrict__ V, uint32_t d, uint32_t q, float *__restrict__ Vf) {
-  const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= q) return;
-  double best = -1.0, sg = 1.0;
-  for (uint32_t r = 0; r < d; ++r) {
-    const double v = V[static_cast<size_t>(r) * q + c];
-    if (fabs(v) > best) {
-      best = fabs(v);
-      sg = v < 0 ? -1.0 : 1.0;
+// Column signs of V (d x q): largest-magnitude entry positive (first such
+// entry on ties); also writes the FP32 copy used by the projection kernel.
+// One 256-thread block per column.
+__global__ __launch_bounds__(256) void sign_fix_kernel(double *__restrict__ V, uint32_t d, uint32_t q,
+                                                       float *__restrict__ Vf) {
+  __shared__ double bv[256];
+  __shared__ uint32_t bi[256];
+  const uint32_t c = blockIdx.x, t = threadIdx.x;
+  double best = -1.0;
+  uint32_t bidx = 0xffffffffu;
+  for (uint32_t r = t; r < d; r += 256) {
+    const double v = fabs(V[static_cast<size_t>(r) * q + c]);
+    if (v > best) {  // rows visited in increasing order: keeps the first maximum
+      best = v;
+      bidx = r;
     }
   }
-  for (uint32_t r = 0; r < d; ++r) {
+  bv[t] = best;
+  bi[t] = bidx;
+  __syncthreads();
+  for (uint32_t w = 128; w > 0; w >>= 1) {
+    if (t < w) {
+      const double o = bv[t + w];
+      const uint32_t oi = bi[t + w];
+      if (o > bv[t] || (o == bv[t] && oi < bi[t])) {
+        bv[t] = o;
+        bi[t] = oi;
+      }
+    }
+    __syncthreads();
+  }
+  const uint32_t rmax = bi[0];
+  const double sg = (rmax < d && V[static_cast<size_t>(rmax) * q + c] < 0) ? -1.0 : 1.0;
+  for (uint32_t r = t; r < d; r += 256) {
     const double v = sg * V[static_cast<size_t>(r) * q + c];
     V[static_cast<size_t>(r) * q + c] = v;
     Vf[static_cast<size_t>(r) * q + c] = static_cast<float>(v);
@@ -439,7 +459,8 @@ hipError_t launch_jacobi(const double *A, uint32_t m, uint32_t q, double *Vout, 
 }
 
 hipError_t launch_sign_fix(double *V, uint32_t d, uint32_t q, float *Vf, hipStream_t s) {
-  sign_fix_kernel<<<(q + 63) / 64, 64, 0, s>>>(V, d, q, Vf);
+  if (q == 0) return hipSuccess;
+  sign_fix_kernel<<<q, 256, 0, s>>>(V, d, q, Vf);
   return hipGetLastError();
 }
 

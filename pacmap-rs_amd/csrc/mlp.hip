@@ -1,0 +1,441 @@
+// mlp.hip -- parametric encoder d -> H -> H -> dims (GELU), trained with the
+// PaCMAP pair loss.  BF16 weights and activations feed v_mfma_f32_16x16x32_bf16
+// with FP32 accumulation; FP32 master weights and Adam state.
+//
+// A batch of B pairs becomes M = 2B rows (row p: first endpoint of pair p,
+// row B + p: second endpoint).  Every reduction (split-K partials, bias sums,
+// loss) is summed in a fixed order, so training is deterministic.
+#include "mlp_internal.h"
+
+#include <hip/hip_bf16.h>
+
+namespace pm {
+
+typedef __bf16 bf16x8m __attribute__((ext_vector_type(8)));
+typedef float floatx4m __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ uint16_t f2bf(float f) {
+  return __bfloat16_as_ushort(__float2bfloat16(f));  // round to nearest even
+}
+__device__ __forceinline__ float bf2f(uint16_t b) { return __uint_as_float(static_cast<uint32_t>(b) << 16); }
+
+// GELU with the exact (erf) form and its derivative
+__device__ __forceinline__ float gelu(float z) { return 0.5f * z * (1.0f + erff(z * 0.70710678118654752f)); }
+__device__ __forceinline__ float gelu_grad(float z) {
+  return 0.5f * (1.0f + erff(z * 0.70710678118654752f)) + z * 0.39894228040143268f * expf(-0.5f * z * z);
+}
+
+// ---------------------------------------------------------------------------
+// Batch rows: pair p of the batch is pair index perm(start + p) of the typed
+// list (NN rows first, then MN, then FP); perm(pos) = (a pos + c) mod P.
+// ---------------------------------------------------------------------------
+__global__ void mlp_gather_kernel(const float *__restrict__ x, uint32_t d, uint32_t Kp,
+                                  const uint32_t *__restrict__ pi, const uint32_t *__restrict__ pj,
+                                  uint64_t P, uint64_t start, uint32_t count, uint64_t pa, uint64_t pc,
+                                  uint32_t B, uint32_t M, uint16_t *__restrict__ xb, uint16_t *__restrict__ xbT,
+                                  uint32_t *__restrict__ pidx) {
+  const uint32_t r = blockIdx.x;  // one block per batch row
+  uint32_t src = 0;
+  if (r < 2 * B) {
+    const uint32_t p = r < B ? r : r - B;
+    if (p < count) {
+      const uint64_t pos = start + p;
+      const uint64_t e = pa == 0 ? pos : (pa * pos + pc) % P;  // pa == 0: identity order
+      src = r < B ? pi[e] : pj[e];
+      if (r < B && threadIdx.x == 0) pidx[p] = static_cast<uint32_t>(e);
+    } else if (r < B && threadIdx.x == 0) {
+      pidx[p] = 0xffffffffu;
+    }
+  }
+  for (uint32_t k = threadIdx.x; k < Kp; k += blockDim.x) {
+    const uint16_t v = (r < 2 * B && k < d) ? f2bf(x[static_cast<size_t>(src) * d + k]) : 0;
+    xb[static_cast<size_t>(r) * Kp + k] = v;
+    xbT[static_cast<size_t>(k) * M + r] = v;
+  }
+}
+
+// Transform rows: x rows [row0, row0 + count) in order, padded with zeros.
+__global__ void mlp_rows_kernel(const float *__restrict__ x, uint32_t d, uint32_t Kp, uint64_t row0,
+                                uint32_t count, uint16_t *__restrict__ xb) {
+  const uint32_t r = blockIdx.x;
+  for (uint32_t k = threadIdx.x; k < Kp; k += blockDim.x) {
+    const uint16_t v = (r < count && k < d) ? f2bf(x[(row0 + r) * d + k]) : 0;
+    xb[static_cast<size_t>(r) * Kp + k] = v;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// C (M x N) = A (M x K) B^T with A, B row-major bf16 (K contiguous), FP32
+// accumulation.  64 x 64 block tile, 4 waves of 32 x 32; each k-step of 32
+// loads the MFMA operands straight from global memory (16 B per lane).
+// Requires M, N multiples of 64 and K (and kchunk) multiples of 32.
+// ---------------------------------------------------------------------------
+struct MlpEpi {
+  const float *bias = nullptr;   // EPI_FWD: bias[n]
+  float *z = nullptr;            // EPI_FWD: pre-activation out (M x N)
+  uint16_t *hb = nullptr;        // EPI_FWD: bf16 activation (M x N)
+  uint16_t *hT = nullptr;        // EPI_FWD: bf16 activation transposed (N x M)
+  float *hf = nullptr;           // EPI_FWD: FP32 activation (M x N), optional
+  const float *zin = nullptr;    // EPI_DH: pre-activation of the layer below (M x N)
+  uint16_t *dzT = nullptr;       // EPI_DH: bf16 dz transposed (N x M)
+  float *part = nullptr;         // EPI_PART: partial slab z at part + z * M * N
+};
+enum { EPI_FWD = 0, EPI_DH = 1, EPI_PART = 2 };
+
+template <int EPI>
+__global__ __launch_bounds__(256) void mlp_gemm_kernel(const uint16_t *__restrict__ A, uint32_t lda,
+                                                       const uint16_t *__restrict__ B, uint32_t ldb,
+                                                       uint32_t M, uint32_t N, uint32_t K, uint32_t kchunk,
+                                                       MlpEpi e) {
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const uint32_t m0 = blockIdx.x * 64 + (wave >> 1) * 32;
+  const uint32_t n0 = blockIdx.y * 64 + (wave & 1) * 32;
+  const uint32_t kbeg = blockIdx.z * kchunk;
+  const uint32_t kend = kbeg + kchunk < K ? kbeg + kchunk : K;
+  floatx4m acc[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) acc[i][j] = floatx4m{0.f, 0.f, 0.f, 0.f};
+  const uint32_t lr = lane & 15, lk = 8 * (lane >> 4);
+  const uint16_t *a0 = A + static_cast<size_t>(m0 + lr) * lda + lk;
+  const uint16_t *a1 = A + static_cast<size_t>(m0 + 16 + lr) * lda + lk;
+  const uint16_t *b0 = B + static_cast<size_t>(n0 + lr) * ldb + lk;
+  const uint16_t *b1 = B + static_cast<size_t>(n0 + 16 + lr) * ldb + lk;
+  for (uint32_t k0 = kbeg; k0 < kend; k0 += 32) {
+    const bf16x8m va0 = __builtin_bit_cast(bf16x8m, *reinterpret_cast<const uint4 *>(a0 + k0));
+    const bf16x8m va1 = __builtin_bit_cast(bf16x8m, *reinterpret_cast<const uint4 *>(a1 + k0));
+    const bf16x8m vb0 = __builtin_bit_cast(bf16x8m, *reinterpret_cast<const uint4 *>(b0 + k0));
+    const bf16x8m vb1 = __builtin_bit_cast(bf16x8m, *reinterpret_cast<const uint4 *>(b1 + k0));
+    acc[0][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(va0, vb0, acc[0][0], 0, 0, 0);
+    acc[0][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(va0, vb1, acc[0][1], 0, 0, 0);
+    acc[1][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(va1, vb0, acc[1][0], 0, 0, 0);
+    acc[1][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(va1, vb1, acc[1][1], 0, 0, 0);
+  }
+  // lane holds rows 4 (lane >> 4) + r (r < 4) of column lane & 15 of each 16 x 16 tile
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      const uint32_t mb = m0 + 16 * i + 4 * (lane >> 4);
+      const uint32_t n = n0 + 16 * j + lr;
+      if (EPI == EPI_PART) {
+        float *P = e.part + static_cast<size_t>(blockIdx.z) * M * N;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) P[static_cast<size_t>(mb + r) * N + n] = acc[i][j][r];
+      } else if (EPI == EPI_FWD) {
+        uint16_t hv[4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const float zv = acc[i][j][r] + e.bias[n];
+          const float h = gelu(zv);
+          const size_t o = static_cast<size_t>(mb + r) * N + n;
+          if (e.z) e.z[o] = zv;
+          hv[r] = f2bf(h);
+          if (e.hb) e.hb[o] = hv[r];
+          if (e.hf) e.hf[o] = h;
+        }
+        if (e.hT) {
+          const uint2 pk = make_uint2(hv[0] | (static_cast<uint32_t>(hv[1]) << 16),
+                                      hv[2] | (static_cast<uint32_t>(hv[3]) << 16));
+          *reinterpret_cast<uint2 *>(e.hT + static_cast<size_t>(n) * M + mb) = pk;
+        }
+      } else {  // EPI_DH: dz = (A B^T) * gelu'(z)
+        uint16_t dv[4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const size_t o = static_cast<size_t>(mb + r) * N + n;
+          dv[r] = f2bf(acc[i][j][r] * gelu_grad(e.zin[o]));
+        }
+        const uint2 pk = make_uint2(dv[0] | (static_cast<uint32_t>(dv[1]) << 16),
+                                    dv[2] | (static_cast<uint32_t>(dv[3]) << 16));
+        *reinterpret_cast<uint2 *>(e.dzT + static_cast<size_t>(n) * M + mb) = pk;
+      }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Output layer (FP32): Y[m][c] = sum_i h2[m][i] W3[c][i] + b3[c]
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void mlp_out_kernel(const float *__restrict__ h2, uint32_t H,
+                                                      const float *__restrict__ W3, const float *__restrict__ b3,
+                                                      uint32_t dims, uint32_t rows, float *__restrict__ Y) {
+  extern __shared__ float sW[];  // dims x H
+  for (uint32_t e = threadIdx.x; e < dims * H; e += blockDim.x) sW[e] = W3[e];
+  __syncthreads();
+  const uint32_t m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= rows) return;
+  const float *hr = h2 + static_cast<size_t>(m) * H;
+  for (uint32_t c = 0; c < dims; ++c) {
+    float s = 0.0f;
+    for (uint32_t i = 0; i < H; ++i) s = fmaf(hr[i], sW[c * H + i], s);
+    Y[static_cast<size_t>(m) * dims + c] = s + b3[c];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Pair loss (gradient.rs:98-151 terms on the encoder outputs): for pair p
+// with u = Y[p] - Y[B + p], D = 1 + |u|^2: NN w D/(10 + D), MN w D/(1e4 + D),
+// FP w/(1 + D); dY = +-w1 u / count as in process_pairs.  Self pairs and
+// padding rows contribute nothing.  Loss partials per block.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void mlp_pair_loss_kernel(const float *__restrict__ Y, uint32_t dims,
+                                                            uint32_t B, const uint32_t *__restrict__ pidx,
+                                                            const uint32_t *__restrict__ pi,
+                                                            const uint32_t *__restrict__ pj, uint64_t nn_end,
+                                                            uint64_t mn_end, float w_nn, float w_mn, float w_fp,
+                                                            float inv_count, float *__restrict__ dY,
+                                                            float *__restrict__ loss_part) {
+  __shared__ float red[256];
+  const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  float loss = 0.0f;
+  if (p < B) {
+    const uint32_t e = pidx[p];
+    float w1 = 0.0f;
+    float u[8];
+    float D = 1.0f;
+    for (uint32_t c = 0; c < dims; ++c) {
+      u[c] = Y[static_cast<size_t>(p) * dims + c] - Y[static_cast<size_t>(B + p) * dims + c];
+      D += u[c] * u[c];
+    }
+    if (e != 0xffffffffu && pi[e] != pj[e]) {
+      if (e < nn_end) {
+        loss = w_nn * (D / (10.0f + D));
+        w1 = w_nn * (20.0f / ((10.0f + D) * (10.0f + D)));
+      } else if (e < mn_end) {
+        loss = w_mn * (D / (10000.0f + D));
+        w1 = w_mn * (20000.0f / ((10000.0f + D) * (10000.0f + D)));
+      } else {
+        loss = w_fp / (1.0f + D);
+        w1 = -w_fp * (2.0f / ((1.0f + D) * (1.0f + D)));
+      }
+    }
+    for (uint32_t c = 0; c < dims; ++c) {
+      const float g = w1 * u[c] * inv_count;
+      dY[static_cast<size_t>(p) * dims + c] = g;
+      dY[static_cast<size_t>(B + p) * dims + c] = -g;
+    }
+  }
+  red[threadIdx.x] = loss;
+  __syncthreads();
+  for (uint32_t w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) loss_part[blockIdx.x] = red[0];
+}
+
+// ---------------------------------------------------------------------------
+// Output-layer backward: dh2 = dY W3, dz2 = dh2 gelu'(z2) (bf16, row-major
+// and transposed); per-block partials of dW3 = dY^T h2 and db3 = sum dY.
+// One block per 64 rows, thread per hidden unit.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void mlp_out_bwd_kernel(const float *__restrict__ dY, uint32_t dims,
+                                                          const float *__restrict__ W3, const float *__restrict__ h2,
+                                                          const float *__restrict__ z2, uint32_t H, uint32_t M,
+                                                          uint16_t *__restrict__ dz2b, uint16_t *__restrict__ dz2T,
+                                                          float *__restrict__ w3_part, float *__restrict__ b3_part) {
+  const uint32_t m0 = blockIdx.x * 64;
+  for (uint32_t i = threadIdx.x; i < H; i += blockDim.x) {
+    float w[8], gw[8];
+    for (uint32_t c = 0; c < dims; ++c) {
+      w[c] = W3[c * H + i];
+      gw[c] = 0.0f;
+    }
+    for (uint32_t r = 0; r < 64; r += 4) {
+      uint16_t dv[4];
+      for (uint32_t rr = 0; rr < 4; ++rr) {
+        const uint32_t m = m0 + r + rr;
+        float dh = 0.0f;
+        const float hv = h2[static_cast<size_t>(m) * H + i];
+        for (uint32_t c = 0; c < dims; ++c) {
+          const float g = dY[static_cast<size_t>(m) * dims + c];
+          dh = fmaf(g, w[c], dh);
+          gw[c] = fmaf(g, hv, gw[c]);
+        }
+        dv[rr] = f2bf(dh * gelu_grad(z2[static_cast<size_t>(m) * H + i]));
+        dz2b[static_cast<size_t>(m) * H + i] = dv[rr];
+      }
+      *reinterpret_cast<uint2 *>(dz2T + static_cast<size_t>(i) * M + m0 + r) =
+          make_uint2(dv[0] | (static_cast<uint32_t>(dv[1]) << 16), dv[2] | (static_cast<uint32_t>(dv[3]) << 16));
+    }
+    for (uint32_t c = 0; c < dims; ++c) w3_part[(static_cast<size_t>(blockIdx.x) * dims + c) * H + i] = gw[c];
+  }
+  if (threadIdx.x < dims) {
+    float s = 0.0f;
+    for (uint32_t r = 0; r < 64; ++r) s += dY[static_cast<size_t>(m0 + r) * dims + threadIdx.x];
+    b3_part[static_cast<size_t>(blockIdx.x) * dims + threadIdx.x] = s;
+  }
+}
+
+// bias gradients: out[i] = sum_m dzT[i][m] (bf16 values, FP32 sum, fixed order)
+__global__ __launch_bounds__(256) void mlp_rowsum_kernel(const uint16_t *__restrict__ dzT, uint32_t M,
+                                                         float *__restrict__ out) {
+  __shared__ float red[256];
+  const uint16_t *r = dzT + static_cast<size_t>(blockIdx.x) * M;
+  float s = 0.0f;
+  for (uint32_t m = threadIdx.x; m < M; m += 256) s += bf2f(r[m]);
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (uint32_t w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[blockIdx.x] = red[0];
+}
+
+// out[r][c] (r < rows, c < cols, row stride ldo) = sum over s < S of
+// part[s][r][c] (partial row stride ldp), in order of s
+__global__ void mlp_sum_parts_kernel(const float *__restrict__ part, uint32_t S, uint32_t rows, uint32_t cols,
+                                     uint32_t ldp, uint64_t slab, float *__restrict__ out, uint32_t ldo) {
+  const uint64_t e = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (e >= static_cast<uint64_t>(rows) * cols) return;
+  const uint32_t r = static_cast<uint32_t>(e / cols), c = static_cast<uint32_t>(e % cols);
+  float s = 0.0f;
+  for (uint32_t z = 0; z < S; ++z) s += part[z * slab + static_cast<size_t>(r) * ldp + c];
+  out[static_cast<size_t>(r) * ldo + c] = s;
+}
+
+// ---------------------------------------------------------------------------
+// Adam on the FP32 master parameters, then the bf16 copies used by the GEMMs:
+// W1b (H x Kp, zero padded), W2b (H x H) and W2tb (H x H, transposed).
+// ---------------------------------------------------------------------------
+__global__ void mlp_adam_kernel(float *__restrict__ p, const float *__restrict__ g, float *__restrict__ m,
+                                float *__restrict__ v, uint64_t count, float lr_t, float b1, float b2,
+                                float eps) {
+  const uint64_t e = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (e >= count) return;
+  const float ge = g[e];
+  const float me = b1 * m[e] + (1.0f - b1) * ge;
+  const float ve = b2 * v[e] + (1.0f - b2) * ge * ge;
+  m[e] = me;
+  v[e] = ve;
+  p[e] = p[e] - lr_t * me / (sqrtf(ve) + eps);
+}
+
+__global__ void mlp_pack_kernel(const float *__restrict__ p, uint32_t d, uint32_t Kp, uint32_t H,
+                                uint16_t *__restrict__ w1b, uint16_t *__restrict__ w2b,
+                                uint16_t *__restrict__ w2tb) {
+  const uint64_t e = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const uint64_t n1 = static_cast<uint64_t>(H) * Kp, n2 = static_cast<uint64_t>(H) * H;
+  if (e < n1) {
+    const uint32_t o = static_cast<uint32_t>(e / Kp), k = static_cast<uint32_t>(e % Kp);
+    w1b[e] = k < d ? f2bf(p[static_cast<size_t>(o) * d + k]) : 0;
+  } else if (e < n1 + n2) {
+    const uint64_t f = e - n1;
+    const uint32_t o = static_cast<uint32_t>(f / H), i = static_cast<uint32_t>(f % H);
+    const uint16_t b = f2bf(p[static_cast<size_t>(H) * d + H + f]);
+    w2b[f] = b;
+    w2tb[static_cast<size_t>(i) * H + o] = b;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+hipError_t launch_mlp_gather(const float *x, uint32_t d, uint32_t Kp, const uint32_t *pi, const uint32_t *pj,
+                             uint64_t P, uint64_t start, uint32_t count, uint64_t pa, uint64_t pc, uint32_t B,
+                             uint32_t M, uint16_t *xb, uint16_t *xbT, uint32_t *pidx, hipStream_t s) {
+  mlp_gather_kernel<<<M, 128, 0, s>>>(x, d, Kp, pi, pj, P, start, count, pa, pc, B, M, xb, xbT, pidx);
+  return hipGetLastError();
+}
+
+hipError_t launch_mlp_rows(const float *x, uint32_t d, uint32_t Kp, uint64_t row0, uint32_t count, uint32_t M,
+                           uint16_t *xb, hipStream_t s) {
+  mlp_rows_kernel<<<M, 128, 0, s>>>(x, d, Kp, row0, count, xb);
+  return hipGetLastError();
+}
+
+static hipError_t gemm_check(uint32_t M, uint32_t N, uint32_t K, uint32_t kchunk) {
+  if (M % 64 || N % 64 || K % 32 || kchunk % 32 || kchunk == 0) return hipErrorInvalidValue;
+  return hipSuccess;
+}
+
+hipError_t launch_mlp_fwd(const uint16_t *A, uint32_t lda, const uint16_t *W, uint32_t ldw, uint32_t M,
+                          uint32_t N, uint32_t K, const float *bias, float *z, uint16_t *hb, uint16_t *hT,
+                          float *hf, hipStream_t s) {
+  if (hipError_t e = gemm_check(M, N, K, K); e != hipSuccess) return e;
+  MlpEpi ep;
+  ep.bias = bias;
+  ep.z = z;
+  ep.hb = hb;
+  ep.hT = hT;
+  ep.hf = hf;
+  mlp_gemm_kernel<EPI_FWD><<<dim3(M / 64, N / 64, 1), 256, 0, s>>>(A, lda, W, ldw, M, N, K, K, ep);
+  return hipGetLastError();
+}
+
+hipError_t launch_mlp_dh(const uint16_t *dz, uint32_t ldd, const uint16_t *Wt, uint32_t ldw, uint32_t M,
+                         uint32_t N, uint32_t K, const float *zin, uint16_t *dzT, hipStream_t s) {
+  if (hipError_t e = gemm_check(M, N, K, K); e != hipSuccess) return e;
+  MlpEpi ep;
+  ep.zin = zin;
+  ep.dzT = dzT;
+  mlp_gemm_kernel<EPI_DH><<<dim3(M / 64, N / 64, 1), 256, 0, s>>>(dz, ldd, Wt, ldw, M, N, K, K, ep);
+  return hipGetLastError();
+}
+
+hipError_t launch_mlp_wgrad(const uint16_t *AT, const uint16_t *BT, uint32_t Mo, uint32_t No, uint32_t K,
+                            uint32_t S, float *part, float *out, uint32_t out_cols, uint32_t ldo, hipStream_t s) {
+  const uint32_t kchunk = K / S;
+  if (K % S) return hipErrorInvalidValue;
+  if (hipError_t e = gemm_check(Mo, No, K, kchunk); e != hipSuccess) return e;
+  MlpEpi ep;
+  ep.part = part;
+  mlp_gemm_kernel<EPI_PART><<<dim3(Mo / 64, No / 64, S), 256, 0, s>>>(AT, K, BT, K, Mo, No, K, kchunk, ep);
+  const uint64_t total = static_cast<uint64_t>(Mo) * out_cols;
+  mlp_sum_parts_kernel<<<static_cast<uint32_t>((total + 255) / 256), 256, 0, s>>>(
+      part, S, Mo, out_cols, No, static_cast<uint64_t>(Mo) * No, out, ldo);
+  return hipGetLastError();
+}
+
+hipError_t launch_mlp_out(const float *h2, uint32_t H, const float *W3, const float *b3, uint32_t dims,
+                          uint32_t rows, float *Y, hipStream_t s) {
+  mlp_out_kernel<<<(rows + 255) / 256, 256, dims * H * sizeof(float), s>>>(h2, H, W3, b3, dims, rows, Y);
+  return hipGetLastError();
+}
+
+hipError_t launch_mlp_pair_loss(const float *Y, uint32_t dims, uint32_t B, const uint32_t *pidx,
+                                const uint32_t *pi, const uint32_t *pj, uint64_t nn_end, uint64_t mn_end,
+                                const float w[3], float inv_count, float *dY, float *loss_part, hipStream_t s) {
+  mlp_pair_loss_kernel<<<(B + 255) / 256, 256, 0, s>>>(Y, dims, B, pidx, pi, pj, nn_end, mn_end, w[0], w[1],
+                                                       w[2], inv_count, dY, loss_part);
+  return hipGetLastError();
+}
+
+hipError_t launch_mlp_out_bwd(const float *dY, uint32_t dims, const float *W3, const float *h2, const float *z2,
+                              uint32_t H, uint32_t M, uint16_t *dz2b, uint16_t *dz2T, float *w3_part,
+                              float *b3_part, hipStream_t s) {
+  if (M % 64) return hipErrorInvalidValue;
+  mlp_out_bwd_kernel<<<M / 64, 256, 0, s>>>(dY, dims, W3, h2, z2, H, M, dz2b, dz2T, w3_part, b3_part);
+  return hipGetLastError();
+}
+
+hipError_t launch_mlp_rowsum(const uint16_t *dzT, uint32_t rows, uint32_t M, float *out, hipStream_t s) {
+  mlp_rowsum_kernel<<<rows, 256, 0, s>>>(dzT, M, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_mlp_sum_parts(const float *part, uint32_t S, uint32_t rows, uint32_t cols, uint32_t ldp,
+                                uint64_t slab, float *out, uint32_t ldo, hipStream_t s) {
+  const uint64_t total = static_cast<uint64_t>(rows) * cols;
+  if (total == 0) return hipSuccess;
+  mlp_sum_parts_kernel<<<static_cast<uint32_t>((total + 255) / 256), 256, 0, s>>>(part, S, rows, cols, ldp, slab,
+                                                                                 out, ldo);
+  return hipGetLastError();
+}
+
+hipError_t launch_mlp_adam(float *p, const float *g, float *m, float *v, uint64_t count, float lr_t, float b1,
+                           float b2, float eps, hipStream_t s) {
+  mlp_adam_kernel<<<static_cast<uint32_t>((count + 255) / 256), 256, 0, s>>>(p, g, m, v, count, lr_t, b1, b2, eps);
+  return hipGetLastError();
+}
+
+hipError_t launch_mlp_pack(const float *p, uint32_t d, uint32_t Kp, uint32_t H, uint16_t *w1b, uint16_t *w2b,
+                           uint16_t *w2tb, hipStream_t s) {
+  const uint64_t total = static_cast<uint64_t>(H) * Kp + static_cast<uint64_t>(H) * H;
+  mlp_pack_kernel<<<static_cast<uint32_t>((total + 255) / 256), 256, 0, s>>>(p, d, Kp, H, w1b, w2b, w2tb);
+  return hipGetLastError();
+}
+
+}  // namespace pm

@@ -1,0 +1,328 @@
+// mlp_host.cpp -- C ABI of the parametric encoder (no counterpart in the
+// reference): initialisation, training on the method's own pairs with the
+// PaCMAP loss, out-of-sample transform, and a one-batch gradient seam.
+//
+// Parameter vector (FP32, row-major): W1 (H x d), b1 (H), W2 (H x H), b2 (H),
+// W3 (dims x H), b3 (dims); y = W3 gelu(W2 gelu(W1 x + b1) + b2) + b3.
+#include <algorithm>
+#include <cmath>
+#include <numeric>
+
+#include "host_common.h"
+#include "internal.h"
+#include "mlp_internal.h"
+
+namespace pm {
+namespace {
+
+constexpr float kBeta1 = 0.9f, kBeta2 = 0.999f, kEps = 1e-8f;
+constexpr uint32_t kTransformRows = 16384;
+
+struct MlpShape {
+  uint32_t d = 0, Kp = 0, H = 0, dims = 0;
+  uint64_t oW1 = 0, ob1 = 0, oW2 = 0, ob2 = 0, oW3 = 0, ob3 = 0, total = 0;
+  MlpShape(uint64_t d_, uint64_t H_, uint64_t dims_) {
+    if (d_ == 0 || d_ > 4096) fail(PACMAP_ERR_UNSUPPORTED, "encoder input width must be in 1..4096");
+    if (H_ < 64 || H_ > 1024 || H_ % 64) fail(PACMAP_ERR_UNSUPPORTED, "hidden width must be a multiple of 64 in 64..1024");
+    if (dims_ == 0 || dims_ > 8) fail(PACMAP_ERR_UNSUPPORTED, "embedding dimensions must be in 1..8");
+    d = static_cast<uint32_t>(d_);
+    H = static_cast<uint32_t>(H_);
+    dims = static_cast<uint32_t>(dims_);
+    Kp = (d + 63) / 64 * 64;
+    oW1 = 0;
+    ob1 = oW1 + static_cast<uint64_t>(H) * d;
+    oW2 = ob1 + H;
+    ob2 = oW2 + static_cast<uint64_t>(H) * H;
+    oW3 = ob2 + H;
+    ob3 = oW3 + static_cast<uint64_t>(dims) * H;
+    total = ob3 + dims;
+  }
+};
+
+struct MlpWeights {
+  DevBuf<float> p;
+  DevBuf<uint16_t> w1b, w2b, w2tb;
+  MlpWeights(const MlpShape &sh, const float *host, hipStream_t s)
+      : p(sh.total), w1b(static_cast<size_t>(sh.H) * sh.Kp), w2b(static_cast<size_t>(sh.H) * sh.H),
+        w2tb(static_cast<size_t>(sh.H) * sh.H) {
+    upload(p, host, sh.total, s);
+    pack(sh, s);
+  }
+  void pack(const MlpShape &sh, hipStream_t s) {
+    PM_HIP(launch_mlp_pack(p.get(), sh.d, sh.Kp, sh.H, w1b.get(), w2b.get(), w2tb.get(), s));
+  }
+};
+
+// Device buffers for batches of up to B pairs (M = 2B rows, padded to 64).
+struct MlpWork {
+  uint32_t B = 0, M = 0, S = 1, nloss = 0;
+  DevBuf<uint16_t> xb, xbT, h1b, h1T, dz2b, dz2T, dz1T;
+  DevBuf<float> z1, z2, h2f, Y, dY, w3p, b3p, wpart;
+  DevBuf<uint32_t> pidx;
+  MlpWork(const MlpShape &sh, uint32_t B_) : B(B_) {
+    M = (2 * B + 63) / 64 * 64;
+    const uint32_t steps = M / 32;  // split-K slabs of whole 32-row steps
+    for (uint32_t c : {16u, 8u, 4u, 2u, 1u})
+      if (steps % c == 0) {
+        S = c;
+        break;
+      }
+    nloss = (B + 255) / 256;
+    const size_t MH = static_cast<size_t>(M) * sh.H;
+    xb.alloc(static_cast<size_t>(M) * sh.Kp);
+    xbT.alloc(static_cast<size_t>(M) * sh.Kp);
+    h1b.alloc(MH);
+    h1T.alloc(MH);
+    dz2b.alloc(MH);
+    dz2T.alloc(MH);
+    dz1T.alloc(MH);
+    z1.alloc(MH);
+    z2.alloc(MH);
+    h2f.alloc(MH);
+    Y.alloc(static_cast<size_t>(M) * sh.dims);
+    dY.alloc(static_cast<size_t>(M) * sh.dims);
+    w3p.alloc(static_cast<size_t>(M / 64) * sh.dims * sh.H);
+    b3p.alloc(static_cast<size_t>(M / 64) * sh.dims);
+    wpart.alloc(static_cast<size_t>(S) * sh.H * std::max(sh.H, sh.Kp));
+    pidx.alloc(B);
+  }
+};
+
+// Gradient (mean over `count` pairs) and loss partials of one batch.
+void batch_grad(const MlpShape &sh, MlpWork &w, const MlpWeights &W, const float *x, const uint32_t *pi,
+                const uint32_t *pj, uint64_t P, uint64_t nn_end, uint64_t mn_end, uint64_t start, uint32_t count,
+                uint64_t pa, uint64_t pc, const float wts[3], float *grad, float *loss_part, hipStream_t s) {
+  const uint32_t M = w.M, H = sh.H, Kp = sh.Kp;
+  PM_HIP(launch_mlp_gather(x, sh.d, Kp, pi, pj, P, start, count, pa, pc, w.B, M, w.xb.get(), w.xbT.get(),
+                           w.pidx.get(), s));
+  PM_HIP(launch_mlp_fwd(w.xb.get(), Kp, W.w1b.get(), Kp, M, H, Kp, W.p.get() + sh.ob1, w.z1.get(), w.h1b.get(),
+                        w.h1T.get(), nullptr, s));
+  PM_HIP(launch_mlp_fwd(w.h1b.get(), H, W.w2b.get(), H, M, H, H, W.p.get() + sh.ob2, w.z2.get(), nullptr, nullptr,
+                        w.h2f.get(), s));
+  PM_HIP(launch_mlp_out(w.h2f.get(), H, W.p.get() + sh.oW3, W.p.get() + sh.ob3, sh.dims, M, w.Y.get(), s));
+  PM_HIP(launch_mlp_pair_loss(w.Y.get(), sh.dims, w.B, w.pidx.get(), pi, pj, nn_end, mn_end, wts,
+                              1.0f / static_cast<float>(count), w.dY.get(), loss_part, s));
+  PM_HIP(launch_mlp_out_bwd(w.dY.get(), sh.dims, W.p.get() + sh.oW3, w.h2f.get(), w.z2.get(), H, M, w.dz2b.get(),
+                            w.dz2T.get(), w.w3p.get(), w.b3p.get(), s));
+  PM_HIP(launch_mlp_dh(w.dz2b.get(), H, W.w2tb.get(), H, M, H, H, w.z1.get(), w.dz1T.get(), s));
+  PM_HIP(launch_mlp_wgrad(w.dz2T.get(), w.h1T.get(), H, H, M, w.S, w.wpart.get(), grad + sh.oW2, H, H, s));
+  PM_HIP(launch_mlp_wgrad(w.dz1T.get(), w.xbT.get(), H, Kp, M, w.S, w.wpart.get(), grad + sh.oW1, sh.d, sh.d, s));
+  PM_HIP(launch_mlp_rowsum(w.dz2T.get(), H, M, grad + sh.ob2, s));
+  PM_HIP(launch_mlp_rowsum(w.dz1T.get(), H, M, grad + sh.ob1, s));
+  PM_HIP(launch_mlp_sum_parts(w.w3p.get(), M / 64, sh.dims, H, H, static_cast<uint64_t>(sh.dims) * H,
+                              grad + sh.oW3, H, s));
+  PM_HIP(launch_mlp_sum_parts(w.b3p.get(), M / 64, 1, sh.dims, sh.dims, sh.dims, grad + sh.ob3, sh.dims, s));
+}
+
+// Typed pair list on the device: NN rows, then MN rows, then FP rows.
+struct PairTable {
+  DevBuf<uint32_t> pi, pj;
+  uint64_t P = 0, nn_end = 0, mn_end = 0;
+  PairTable(uint64_t n, const uint32_t *nn, uint64_t n_nn, const uint32_t *mn, uint64_t n_mn, const uint32_t *fp,
+            uint64_t n_fp, hipStream_t s) {
+    P = n_nn + n_mn + n_fp;
+    nn_end = n_nn;
+    mn_end = n_nn + n_mn;
+    if ((n_nn && !nn) || (n_mn && !mn) || (n_fp && !fp)) fail(PACMAP_ERR_INVALID_ARGUMENT, "NULL pair array");
+    std::vector<uint32_t> a(P), b(P);
+    uint64_t o = 0;
+    for (auto [arr, rows] : {std::pair{nn, n_nn}, std::pair{mn, n_mn}, std::pair{fp, n_fp}})
+      for (uint64_t r = 0; r < rows; ++r, ++o) {
+        a[o] = arr[2 * r];
+        b[o] = arr[2 * r + 1];
+        if (a[o] >= n || b[o] >= n) fail(PACMAP_ERR_INVALID_PAIR_INDEX, "pair index out of range");
+      }
+    pi.alloc(std::max<uint64_t>(1, P));
+    pj.alloc(std::max<uint64_t>(1, P));
+    upload(pi, a.data(), P, s);
+    upload(pj, b.data(), P, s);
+    PM_HIP(hipStreamSynchronize(s));
+  }
+};
+
+void fill_init(const MlpShape &sh, uint64_t seed, float *p) {
+  SmallRng rng = SmallRng::seed_from_u64(seed);
+  auto uni = [&](uint64_t count, uint32_t fan_in, float *dst) {
+    const double a = 1.0 / std::sqrt(static_cast<double>(fan_in));
+    for (uint64_t i = 0; i < count; ++i) dst[i] = static_cast<float>((2.0 * rng.next_f64() - 1.0) * a);
+  };
+  uni(static_cast<uint64_t>(sh.H) * sh.d, sh.d, p + sh.oW1);
+  uni(sh.H, sh.d, p + sh.ob1);
+  uni(static_cast<uint64_t>(sh.H) * sh.H, sh.H, p + sh.oW2);
+  uni(sh.H, sh.H, p + sh.ob2);
+  uni(static_cast<uint64_t>(sh.dims) * sh.H, sh.H, p + sh.oW3);
+  uni(sh.dims, sh.H, p + sh.ob3);
+}
+
+}  // namespace
+}  // namespace pm
+
+using namespace pm;
+
+extern "C" {
+
+void pacmap_mlp_config_default(pacmap_mlp_config *cfg) {
+  if (!cfg) return;
+  std::memset(cfg, 0, sizeof(*cfg));
+  cfg->hidden = 256;
+  cfg->epochs = 20;
+  cfg->batch_pairs = 8192;
+  cfg->learning_rate = 1e-3f;
+  cfg->seed = 0;
+  cfg->schedule_iters[0] = 100;
+  cfg->schedule_iters[1] = 100;
+  cfg->schedule_iters[2] = 250;
+}
+
+uint64_t pacmap_mlp_param_count(uint64_t d, uint64_t hidden, uint64_t dims) {
+  return hidden * d + hidden + hidden * hidden + hidden + dims * hidden + dims;
+}
+
+int pacmap_mlp_init(uint64_t d, uint64_t hidden, uint64_t dims, uint64_t seed, float *params, pacmap_status *st) {
+  return guarded(st, [&] {
+    const MlpShape sh(d, hidden, dims);
+    if (!params) fail(PACMAP_ERR_INVALID_ARGUMENT, "params is NULL");
+    fill_init(sh, seed, params);
+  });
+}
+
+int pacmap_mlp_grad(const float *x, uint64_t n, uint64_t d, uint64_t hidden, uint64_t dims, const float *params,
+                    const uint32_t *pair_nn, uint64_t n_nn, const uint32_t *pair_mn, uint64_t n_mn,
+                    const uint32_t *pair_fp, uint64_t n_fp, const float weights[3], float *grad_out, float *loss_out,
+                    pacmap_status *st) {
+  return guarded(st, [&] {
+    const MlpShape sh(d, hidden, dims);
+    if (!x || !params || !weights || !grad_out) fail(PACMAP_ERR_INVALID_ARGUMENT, "NULL argument");
+    const uint64_t P = n_nn + n_mn + n_fp;
+    if (P == 0 || P > 65536) fail(PACMAP_ERR_UNSUPPORTED, "one gradient batch holds 1..65536 pairs");
+    Stream stream;
+    const hipStream_t s = stream.get();
+    PairTable pt(n, pair_nn, n_nn, pair_mn, n_mn, pair_fp, n_fp, s);
+    DevBuf<float> dx(n * d), grad(sh.total);
+    upload(dx, x, n * d, s);
+    MlpWeights W(sh, params, s);
+    MlpWork w(sh, static_cast<uint32_t>(P));
+    DevBuf<float> lp(w.nloss);
+    batch_grad(sh, w, W, dx.get(), pt.pi.get(), pt.pj.get(), P, pt.nn_end, pt.mn_end, 0, static_cast<uint32_t>(P), 0,
+               0, weights, grad.get(), lp.get(), s);
+    std::vector<float> hl(w.nloss);
+    download(grad_out, grad.get(), sh.total, s);
+    download(hl.data(), lp.get(), w.nloss, s);
+    PM_HIP(hipStreamSynchronize(s));
+    if (loss_out) {
+      double acc = 0.0;
+      for (float v : hl) acc += v;
+      *loss_out = static_cast<float>(acc / static_cast<double>(P));
+    }
+  });
+}
+
+int pacmap_mlp_fit(const float *x, uint64_t n, uint64_t d, uint64_t dims, const uint32_t *pair_nn, uint64_t n_nn,
+                   const uint32_t *pair_mn, uint64_t n_mn, const uint32_t *pair_fp, uint64_t n_fp,
+                   const pacmap_mlp_config *cfg, float *params, float *epoch_loss, pacmap_status *st) {
+  return guarded(st, [&] {
+    if (!cfg) fail(PACMAP_ERR_INVALID_ARGUMENT, "config is NULL");
+    const MlpShape sh(d, cfg->hidden, dims);
+    if (!x || !params) fail(PACMAP_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (cfg->batch_pairs == 0 || cfg->batch_pairs > 65536)
+      fail(PACMAP_ERR_UNSUPPORTED, "batch_pairs must be in 1..65536");
+    const uint64_t P = n_nn + n_mn + n_fp;
+    if (P == 0) fail(PACMAP_ERR_INVALID_ARGUMENT, "no pairs to train on");
+    if (P >= (1ull << 32)) fail(PACMAP_ERR_UNSUPPORTED, "more than 2^32 pairs");
+    Stream stream;
+    const hipStream_t s = stream.get();
+    PairTable pt(n, pair_nn, n_nn, pair_mn, n_mn, pair_fp, n_fp, s);
+    DevBuf<float> dx(n * d), grad(sh.total), m(sh.total), v(sh.total);
+    upload(dx, x, n * d, s);
+    PM_HIP(launch_fill_zero(m.get(), m.bytes(), s));
+    PM_HIP(launch_fill_zero(v.get(), v.bytes(), s));
+    MlpWeights W(sh, params, s);
+    const uint32_t B = static_cast<uint32_t>(std::min<uint64_t>(cfg->batch_pairs, P));
+    MlpWork w(sh, B);
+    const uint64_t nb = (P + B - 1) / B;
+    const uint64_t total_batches = nb * cfg->epochs;
+    const uint64_t T = cfg->schedule_iters[0] + cfg->schedule_iters[1] + cfg->schedule_iters[2];
+    DevBuf<float> lp(nb * w.nloss);
+    std::vector<float> hl(nb * w.nloss);
+    SmallRng rng = SmallRng::seed_from_u64(cfg->seed);
+    uint64_t step = 0;
+    for (uint64_t ep = 0; ep < cfg->epochs; ++ep) {
+      // pair order of this epoch: pos -> (a pos + c) mod P with gcd(a, P) = 1
+      uint64_t a = 1, c = 0;
+      if (P > 1) {
+        do a = rng.next_u64() % P; while (a == 0 || std::gcd(a, P) != 1);
+        c = rng.next_u64() % P;
+      }
+      for (uint64_t b = 0; b < nb; ++b, ++step) {
+        const uint64_t start = b * B;
+        const uint32_t count = static_cast<uint32_t>(std::min<uint64_t>(B, P - start));
+        float wts[3];
+        const uint64_t itr = T ? step * T / total_batches : 0;
+        find_weights(1000.0f, itr, cfg->schedule_iters[0], cfg->schedule_iters[1], wts);
+        batch_grad(sh, w, W, dx.get(), pt.pi.get(), pt.pj.get(), P, pt.nn_end, pt.mn_end, start, count, a, c, wts,
+                   grad.get(), lp.get() + b * w.nloss, s);
+        const double t = static_cast<double>(step + 1);
+        const float lr_t = static_cast<float>(cfg->learning_rate * std::sqrt(1.0 - std::pow(kBeta2, t)) /
+                                              (1.0 - std::pow(kBeta1, t)));
+        PM_HIP(launch_mlp_adam(W.p.get(), grad.get(), m.get(), v.get(), sh.total, lr_t, kBeta1, kBeta2, kEps, s));
+        W.pack(sh, s);
+      }
+      download(hl.data(), lp.get(), hl.size(), s);
+      PM_HIP(hipStreamSynchronize(s));
+      if (epoch_loss) {
+        double acc = 0.0;
+        for (float val : hl) acc += val;
+        epoch_loss[ep] = static_cast<float>(acc / static_cast<double>(P));
+      }
+    }
+    download(params, W.p.get(), sh.total, s);
+    PM_HIP(hipStreamSynchronize(s));
+  });
+}
+
+static int mlp_transform_impl(const float *x_host, const float *x_dev, uint64_t n, uint64_t d, uint64_t hidden,
+                              uint64_t dims, const float *params, float *y_out, pacmap_status *st) {
+  return guarded(st, [&] {
+    const MlpShape sh(d, hidden, dims);
+    if ((!x_host && !x_dev) || !params || !y_out) fail(PACMAP_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (n == 0) return;
+    Stream stream;
+    const hipStream_t s = stream.get();
+    DevBuf<float> dxb;
+    const float *xd = x_dev;
+    if (x_host) {
+      dxb.alloc(n * d);
+      upload(dxb, x_host, n * d, s);
+      xd = dxb.get();
+    }
+    MlpWeights W(sh, params, s);
+    const uint32_t Mt = static_cast<uint32_t>(std::min<uint64_t>(kTransformRows, (n + 63) / 64 * 64));
+    DevBuf<uint16_t> xb(static_cast<size_t>(Mt) * sh.Kp), h1b(static_cast<size_t>(Mt) * sh.H);
+    DevBuf<float> h2f(static_cast<size_t>(Mt) * sh.H), Y(n * dims);
+    for (uint64_t r0 = 0; r0 < n; r0 += Mt) {
+      const uint32_t count = static_cast<uint32_t>(std::min<uint64_t>(Mt, n - r0));
+      const uint32_t M = (count + 63) / 64 * 64;
+      PM_HIP(launch_mlp_rows(xd, sh.d, sh.Kp, r0, count, M, xb.get(), s));
+      PM_HIP(launch_mlp_fwd(xb.get(), sh.Kp, W.w1b.get(), sh.Kp, M, sh.H, sh.Kp, W.p.get() + sh.ob1, nullptr,
+                            h1b.get(), nullptr, nullptr, s));
+      PM_HIP(launch_mlp_fwd(h1b.get(), sh.H, W.w2b.get(), sh.H, M, sh.H, sh.H, W.p.get() + sh.ob2, nullptr, nullptr,
+                            nullptr, h2f.get(), s));
+      PM_HIP(launch_mlp_out(h2f.get(), sh.H, W.p.get() + sh.oW3, W.p.get() + sh.ob3, sh.dims, count,
+                            Y.get() + r0 * dims, s));
+    }
+    download(y_out, Y.get(), n * dims, s);
+    PM_HIP(hipStreamSynchronize(s));
+  });
+}
+
+int pacmap_mlp_transform(const float *x, uint64_t n, uint64_t d, uint64_t hidden, uint64_t dims,
+                         const float *params, float *y_out, pacmap_status *st) {
+  return mlp_transform_impl(x, nullptr, n, d, hidden, dims, params, y_out, st);
+}
+
+int pacmap_mlp_transform_dev(const float *x_dev, uint64_t n, uint64_t d, uint64_t hidden, uint64_t dims,
+                             const float *params, float *y_out, pacmap_status *st) {
+  return mlp_transform_impl(nullptr, x_dev, n, d, hidden, dims, params, y_out, st);
+}
+
+}  // extern "C"

@@ -89,6 +89,17 @@ _u64 = C.c_uint64
 _lib = None
 
 
+class MlpConfig(C.Structure):
+    _fields_ = [
+        ("hidden", C.c_uint64),
+        ("epochs", C.c_uint64),
+        ("batch_pairs", C.c_uint64),
+        ("learning_rate", C.c_float),
+        ("seed", C.c_uint64),
+        ("schedule_iters", C.c_uint64 * 3),
+    ]
+
+
 def lib():
     """Load the HIP library (raises when it has not been built)."""
     global _lib
@@ -114,6 +125,19 @@ def lib():
                                           C.POINTER(Status)]
         L.pacmap_optimize.argtypes = [_P, _u64, _u64, _P, _u64, _P, _u64, _P, _u64, C.c_float,
                                       _P, _P, _u64, _P, _P, C.POINTER(Status)]
+        L.pacmap_generate_pairs.argtypes = [_P, _u64, _u64, C.POINTER(_Config), _P, _P, _P, _P, _P,
+                                            C.POINTER(Status)]
+        L.pacmap_mlp_config_default.argtypes = [C.POINTER(MlpConfig)]
+        L.pacmap_mlp_config_default.restype = None
+        L.pacmap_mlp_param_count.argtypes = [_u64, _u64, _u64]
+        L.pacmap_mlp_param_count.restype = _u64
+        L.pacmap_mlp_init.argtypes = [_u64, _u64, _u64, _u64, _P, C.POINTER(Status)]
+        L.pacmap_mlp_fit.argtypes = [_P, _u64, _u64, _u64, _P, _u64, _P, _u64, _P, _u64,
+                                     C.POINTER(MlpConfig), _P, _P, C.POINTER(Status)]
+        L.pacmap_mlp_transform.argtypes = [_P, _u64, _u64, _u64, _u64, _P, _P, C.POINTER(Status)]
+        L.pacmap_mlp_transform_dev.argtypes = L.pacmap_mlp_transform.argtypes
+        L.pacmap_mlp_grad.argtypes = [_P, _u64, _u64, _u64, _u64, _P, _P, _u64, _P, _u64, _P, _u64,
+                                      _P, _P, _P, C.POINTER(Status)]
         L.pacmap_dev_alloc.argtypes = [_u64, C.POINTER(C.c_void_p), C.POINTER(Status)]
         L.pacmap_dev_free.argtypes = [_P]
         L.pacmap_dev_upload.argtypes = [_P, _P, _u64, C.POINTER(Status)]
@@ -464,6 +488,106 @@ def optimize(y0, pair_neighbors, pair_mn, pair_fp, lr=1.0, num_iters=(100, 100, 
                                snaps.ctypes.data if ns else None, loss.ctypes.data, C.byref(st))
     _check(rc, st)
     return y, snaps, loss[:total]
+
+
+def generate_pairs(x, config: Optional[Configuration] = None):
+    """PairConfiguration::Generate as fit_transform runs it (lib.rs:629-650,
+    neighbors.rs:56-108) after the same preprocessing (lib.rs:442-532).
+
+    Returns (pair_neighbors, pair_mn, pair_fp, x_preprocessed)."""
+    config = config or Configuration()
+    x = np.ascontiguousarray(x, np.float32)
+    if x.ndim != 2:
+        raise ValueError("x must be 2-D")
+    n, d = x.shape
+    keep = []
+    c = config._to_c(keep)
+    counts = np.zeros(4, np.uint64)
+    st = Status()
+    L = lib()
+    _check(L.pacmap_generate_pairs(x.ctypes.data, n, d, C.byref(c), None, None, None, None,
+                                   counts.ctypes.data, C.byref(st)), st)
+    nn = np.zeros((int(counts[0]), 2), np.uint32)
+    mn = np.zeros((int(counts[1]), 2), np.uint32)
+    fp = np.zeros((int(counts[2]), 2), np.uint32)
+    xp = np.zeros((n, int(counts[3])), np.float32)
+    _check(L.pacmap_generate_pairs(x.ctypes.data, n, d, C.byref(c), nn.ctypes.data,
+                                   mn.ctypes.data, fp.ctypes.data, xp.ctypes.data,
+                                   counts.ctypes.data, C.byref(st)), st)
+    return nn, mn, fp, xp
+
+
+class mlp:  # noqa: N801 -- namespace of the parametric encoder entry points
+    """Encoder d -> hidden -> hidden -> dims (GELU) trained with the PaCMAP pair
+    loss; flat FP32 parameters W1, b1, W2, b2, W3, b3 (row-major, out x in)."""
+
+    @staticmethod
+    def param_count(d, hidden=256, dims=2):
+        return int(lib().pacmap_mlp_param_count(d, hidden, dims))
+
+    @staticmethod
+    def init(d, hidden=256, dims=2, seed=0):
+        p = np.zeros(mlp.param_count(d, hidden, dims), np.float32)
+        st = Status()
+        _check(lib().pacmap_mlp_init(d, hidden, dims, seed, p.ctypes.data, C.byref(st)), st)
+        return p
+
+    @staticmethod
+    def fit(x, pair_neighbors, pair_mn, pair_fp, params, dims=2, hidden=256, epochs=20,
+            batch_pairs=8192, learning_rate=1e-3, seed=0, schedule=(100, 100, 250)):
+        """Trains a copy of params; returns (trained params, mean loss per epoch)."""
+        x = np.ascontiguousarray(x, np.float32)
+        n, d = x.shape
+        p = np.array(params, np.float32, copy=True)
+        if p.size != mlp.param_count(d, hidden, dims):
+            raise ValueError("params has the wrong size for (d, hidden, dims)")
+        a, na = _pairs(pair_neighbors)
+        b, nb = _pairs(pair_mn)
+        c, nc = _pairs(pair_fp)
+        cfg = MlpConfig()
+        lib().pacmap_mlp_config_default(C.byref(cfg))
+        cfg.hidden, cfg.epochs, cfg.batch_pairs = hidden, epochs, batch_pairs
+        cfg.learning_rate, cfg.seed = learning_rate, seed
+        for i in range(3):
+            cfg.schedule_iters[i] = schedule[i]
+        loss = np.zeros(max(epochs, 1), np.float32)
+        st = Status()
+        _check(lib().pacmap_mlp_fit(x.ctypes.data, n, d, dims, a.ctypes.data, na, b.ctypes.data, nb,
+                                    c.ctypes.data, nc, C.byref(cfg), p.ctypes.data, loss.ctypes.data,
+                                    C.byref(st)), st)
+        return p, loss[:epochs]
+
+    @staticmethod
+    def transform(x, params, hidden=256, dims=2):
+        dev = isinstance(x, DeviceArray)
+        if not dev:
+            x = np.ascontiguousarray(x, np.float32)
+        n, d = x.shape
+        p = np.ascontiguousarray(params, np.float32)
+        y = np.zeros((n, dims), np.float32)
+        st = Status()
+        fn = lib().pacmap_mlp_transform_dev if dev else lib().pacmap_mlp_transform
+        _check(fn(x.ptr if dev else x.ctypes.data, n, d, hidden, dims, p.ctypes.data, y.ctypes.data,
+                  C.byref(st)), st)
+        return y
+
+    @staticmethod
+    def grad(x, params, pair_neighbors, pair_mn, pair_fp, weights, hidden=256, dims=2):
+        """Mean pair loss and its parameter gradient over one batch of pairs."""
+        x = np.ascontiguousarray(x, np.float32)
+        n, d = x.shape
+        p = np.ascontiguousarray(params, np.float32)
+        a, na = _pairs(pair_neighbors)
+        b, nb = _pairs(pair_mn)
+        c, nc = _pairs(pair_fp)
+        w = np.ascontiguousarray(weights, np.float32)
+        g = np.zeros_like(p)
+        loss = np.zeros(1, np.float32)
+        st = Status()
+        _check(lib().pacmap_mlp_grad(x.ctypes.data, n, d, hidden, dims, p.ctypes.data,
+                                     a.ctypes.data, na, b.ctypes.data, nb, c.ctypes.data, nc,
+                                     w.ctypes.data, g.ctypes.data, loss.ctypes.data, C.byref(st)), st)
+        return g, float(loss[0])
 
 
 def last_timings() -> dict:
