@@ -205,16 +205,102 @@ def encoder_run(pm):
     }
 
 
+# Precision sweep: 200,000 x 256 from a 50-component Gaussian mixture, each
+# component a random rotation of a diagonal covariance with eigenvalues
+# log-uniform in [0.01, 1], centres ~ N(0, I), plus N(0, 0.05^2) noise.
+SWEEP = dict(n=200_000, d=256, k=60, sample=2000)
+
+
+def make_sweep(seed=3):
+    rng = np.random.default_rng(seed)
+    n, d = SWEEP["n"], SWEEP["d"]
+    lab = rng.integers(0, 50, n)
+    x = np.empty((n, d), np.float32)
+    centres = rng.normal(size=(50, d))
+    for c in range(50):
+        rows = np.nonzero(lab == c)[0]
+        q, _ = np.linalg.qr(rng.standard_normal((d, d)))
+        sd = np.sqrt(np.exp(rng.uniform(np.log(0.01), 0.0, d)))
+        x[rows] = (rng.standard_normal((rows.size, d)) * sd) @ q.T + centres[c]
+    x += rng.normal(0.0, 0.05, (n, d)).astype(np.float32)
+    return x
+
+
+def fp64_top(x, rows, k):
+    """FP64 k nearest neighbours (self excluded) of the sampled rows, by
+    exact FP64 differences on a Gram-matrix preselection of 4k candidates."""
+    x64 = x.astype(np.float64)
+    nrm = (x64 * x64).sum(1)
+    out_i = np.empty((len(rows), k), np.int64)
+    out_d = np.empty((len(rows), k))
+    for b in range(0, len(rows), 250):
+        r = rows[b:b + 250]
+        g = nrm[r, None] + nrm[None, :] - 2.0 * (x64[r] @ x64.T)
+        g[np.arange(len(r)), r] = np.inf
+        pre = np.argpartition(g, 4 * k, axis=1)[:, :4 * k]
+        for j, (q, cand) in enumerate(zip(r, pre)):
+            dd = np.sqrt(((x64[cand] - x64[q]) ** 2).sum(1))
+            o = np.lexsort((cand, dd))[:k]
+            out_i[b + j], out_d[b + j] = cand[o], dd[o]
+    return out_i, out_d
+
+
+def sweep_run(pm):
+    x = make_sweep()
+    n, d, k = SWEEP["n"], SWEEP["d"], SWEEP["k"]
+    rows = np.random.default_rng(5).choice(n, SWEEP["sample"], replace=False)
+    log("bench: FP64 neighbours of the sampled rows")
+    ti, td = fp64_top(x, rows, k)
+    res = {}
+    for name, prec, cert in (("fp32 split-bf16, certified", "fp32", True),
+                             ("fp32 split-bf16, no certificate", "fp32", False),
+                             ("bf16, certified", "bf16", True),
+                             ("bf16, no certificate", "bf16", False),
+                             ("fp8 e4m3 per-row scale, no certificate", "fp8", False),
+                             ("direct fp32", "direct", True)):
+        log(f"bench: kNN {name}")
+        pm.knn.find_k_nearest_neighbors(x[:20000], k, precision=prec, certify=cert)  # warm
+        t0 = time.perf_counter()
+        idx, dist = pm.knn.find_k_nearest_neighbors(x, k, precision=prec, certify=cert)
+        wall = time.perf_counter() - t0
+        t = pm.last_timings()
+        rec = np.mean([np.intersect1d(idx[q], ti[j]).size for j, q in enumerate(rows)]) / k
+        x64 = x.astype(np.float64)
+        dd = np.sqrt(((x64[idx[rows]] - x64[rows][:, None, :]) ** 2).sum(2))
+        res[name] = {"recall_at_60_vs_fp64": round(float(rec), 6),
+                     "max_abs_dist_err_vs_fp64": float(np.abs(dist[rows] - dd).max()),
+                     "knn_ms": round(t["knn_ms"], 3), "candidate_kernel_ms": round(t["knn_kernel_ms"], 3),
+                     "rerank_ms": round(t["knn_rerank_ms"], 3), "rescan_ms": round(t["knn_fallback_ms"], 3),
+                     "rescanned_rows": int(t["knn_rescanned_rows"]), "exact": bool(t["knn_certified"]),
+                     "call_s": round(wall, 3)}
+        log(f"bench:   {res[name]}")
+    flops = 2.0 * n * n * d
+    base = res["fp32 split-bf16, certified"]
+    return {
+        "metric": METRIC, "value": base["knn_ms"], "unit": "ms (certified exact kNN)", "n_gpus": 1,
+        "steps": 1, "warmup": 1, "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+        "dtype": "fp32 / bf16 / fp8 candidates, fp32 rerank", "data": "synthetic",
+        "config": {"workload": "kNN precision sweep: 200000x256 GMM(50) anisotropic, k=60, recall "
+                               "on 2000 sampled rows against FP64", "n": n, "d": d, "k": k},
+        "algorithmic_flop": flops,
+        "sweep": res,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", choices=("70k", "1m", "encoder"), default="70k",
+    ap.add_argument("--encoder-epochs", type=int, default=ENC["epochs"],
+                    help="training epochs of the encoder workload")
+    ap.add_argument("--workload", choices=("70k", "1m", "encoder", "sweep", "copy"), default="70k",
                     help="70k: the 70000 x 784 headline run; 1m: the 1,000,000 x 128 large run "
                          "(no CPU baseline); encoder: the learned encoder vs the plain optimiser "
-                         "on held-out Swiss-roll points")
+                         "on held-out Swiss-roll points; sweep: kNN candidate precisions on "
+                         "200000 x 256; copy: a 1 GiB streaming device copy (HBM reference "
+                         "for the traffic counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU design is "
@@ -223,7 +309,20 @@ def main():
     import pacmap_mi355x as pm
 
     if args.workload == "encoder":
+        ENC["epochs"] = args.encoder_epochs
         print(json.dumps(encoder_run(pm)), flush=True)
+        return
+    if args.workload == "copy":
+        nbytes = 1 << 30
+        ms = pm.hbm_copy(nbytes, repeats=max(1, args.steps))
+        print(json.dumps({"metric": "streaming copy", "value": round(2 * nbytes / (ms * 1e-3) / 1e9, 1),
+                          "unit": "GB/s (read + write)", "n_gpus": 1, "steps": args.steps,
+                          "ms_per_step": round(ms, 4), "bytes_read": nbytes, "bytes_written": nbytes,
+                          "higher_is_better": True, "data": "synthetic",
+                          "config": {"workload": "hbm_copy_kernel, 1 GiB"}}), flush=True)
+        return
+    if args.workload == "sweep":
+        print(json.dumps(sweep_run(pm)), flush=True)
         return
     large = args.workload == "1m"
     if large:

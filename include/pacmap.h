@@ -59,14 +59,19 @@ typedef struct {
 enum { PACMAP_INIT_PCA = 0, PACMAP_INIT_VALUE = 1, PACMAP_INIT_RANDOM = 2 };
 /* PairConfiguration (lib.rs:241-267) */
 enum { PACMAP_PAIRS_GENERATE = 0, PACMAP_PAIRS_NEIGHBORS_PROVIDED = 1, PACMAP_PAIRS_ALL_PROVIDED = 2 };
-/* kNN path (MI355X extension).  Every mode returns the exact FP32 neighbour
- * sets of knn.rs:137-194 (see pacmap_knn_exact):
+/* kNN path (MI355X extension).  FP32, BF16 and DIRECT return the exact FP32
+ * neighbour sets of knn.rs:137-194 (see pacmap_knn_exact):
  *   FP32   (default) split-bf16 MFMA candidates, exact FP32 rerank, per-row
  *          certificate, exact rescan of uncertified rows;
  *   BF16   single-bf16 MFMA candidates, same rerank/certificate/rescan;
- *   FP8    reserved (returns PACMAP_ERR_INVALID_ARGUMENT);
- *   DIRECT every distance in the reference's order on the vector ALUs. */
-enum { PACMAP_KNN_FP32 = 0, PACMAP_KNN_BF16 = 1, PACMAP_KNN_FP8 = 2, PACMAP_KNN_DIRECT = 3 };
+ *   DIRECT every distance in the reference's order on the vector ALUs;
+ *   FP8    FP8 E4M3 candidates with per-row scaling (16x16x128 f8f6f4 MFMA),
+ *          exact FP32 rerank of the candidates, no certificate: approximate
+ *          neighbours (the E4M3 error bound is too wide to certify).
+ * OR-ing PACMAP_KNN_APPROX into FP32 or BF16 also drops the certificate and
+ * the rescan (approximate neighbours, the exact top-k of the candidates). */
+enum { PACMAP_KNN_FP32 = 0, PACMAP_KNN_BF16 = 1, PACMAP_KNN_FP8 = 2, PACMAP_KNN_DIRECT = 3,
+       PACMAP_KNN_APPROX = 0x100 };
 
 /* Configuration (lib.rs:163-224).  Option<T> fields carry a has_ flag; enum
  * payloads are borrowed pointers that must outlive the call. */
@@ -121,7 +126,8 @@ int pacmap_fit_transform_dev(const float *x_dev, uint64_t n, uint64_t d,
  * idx: n x min(k, n-1) u32, dist: n x min(k, n-1) f32.  n == 0 is a no-op. */
 int pacmap_knn_exact(const float *x, uint64_t n, uint64_t d, uint64_t k, uint32_t *idx,
                      float *dist, pacmap_status *st);
-/* Same with a candidate precision (PACMAP_KNN_*); results equal pacmap_knn_exact.
+/* Same with a candidate precision (PACMAP_KNN_*); results equal pacmap_knn_exact
+ * except in the approximate modes (FP8, PACMAP_KNN_APPROX).
  * stats_out (may be NULL): [rows re-scanned exactly, candidates per row]. */
 int pacmap_knn(const float *x, uint64_t n, uint64_t d, uint64_t k, int32_t precision,
                uint32_t *idx, float *dist, uint64_t *stats_out, pacmap_status *st);
@@ -220,6 +226,11 @@ int pacmap_dev_alloc(uint64_t bytes, void **ptr_out, pacmap_status *st);
 int pacmap_dev_free(void *ptr);
 int pacmap_dev_upload(void *dst_dev, const void *src_host, uint64_t bytes, pacmap_status *st);
 int pacmap_dev_download(void *dst_host, const void *src_dev, uint64_t bytes, pacmap_status *st);
+/* HBM reference: a plain streaming device-to-device copy kernel run `repeats`
+ * times; ms_out gets the average time per copy (HIP events).  Its rocprof
+ * FETCH_SIZE / WRITE_SIZE readings calibrate the traffic counters. */
+int pacmap_dev_copy(void *dst_dev, const void *src_dev, uint64_t bytes, uint32_t repeats, double *ms_out,
+                    pacmap_status *st);
 
 /* Per-thread timings of the last call on this thread (HIP events on the call's
  * stream, milliseconds; host phases by steady clock). */
@@ -243,8 +254,10 @@ typedef struct {
   uint64_t knn_candidates;     /* candidate list capacity per row */
   double knn_rerank_ms;        /* exact rerank + certificate kernel */
   double knn_fallback_ms;      /* exact rescan of uncertified rows */
-  uint64_t knn_path;           /* 0 direct, 1 split-bf16 MFMA, 2 bf16 MFMA */
+  uint64_t knn_path;           /* 0 direct, 1 split-bf16 MFMA, 2 bf16 MFMA, 3 FP8 MFMA */
   double knn_sample_ms;        /* MFMA threshold-sampling kernel */
+  uint64_t knn_certified;      /* 1: exact neighbours (certificate + rescan), 0: approximate */
+  uint64_t grad_target_blocks; /* 0: one-pass gradient; 8: target-blocked two-pass gradient */
 } pacmap_timings;
 void pacmap_last_timings(pacmap_timings *out);
 

@@ -5,6 +5,7 @@
 #include <cmath>
 #include <memory>
 #include <new>
+#include <cstdlib>
 #include <random>
 
 #include "host_common.h"
@@ -39,34 +40,53 @@ double knn_cert_scale(uint32_t d) {
   return (1.0 - gammaA) * (1.0 - e24) * (1.0 - e24) * (1.0 - 1.0 / 1048576.0);
 }
 
+bool knn_precision_valid(int precision) {
+  const int base = precision & 0xff;
+  return (precision & ~(0xff | PACMAP_KNN_APPROX)) == 0 && base >= PACMAP_KNN_FP32 && base <= PACMAP_KNN_DIRECT;
+}
+
+// Candidate precision of the MFMA path: 0 single bf16, 1 split bf16, 2 FP8.
+int knn_mode(int precision) {
+  const int base = precision & 0xff;
+  return base == PACMAP_KNN_BF16 ? 0 : base == PACMAP_KNN_FP8 ? 2 : 1;
+}
+uint32_t knn_chunks(uint64_t d, int mode) { return static_cast<uint32_t>(mode == 2 ? (d + 127) / 128 : (d + 31) / 32); }
+
 bool knn_mfma_eligible(uint64_t n, uint64_t d, uint32_t k, int precision) {
-  if (precision == PACMAP_KNN_DIRECT) return false;
-  const uint64_t kch = (d + 31) / 32;
+  if ((precision & 0xff) == PACMAP_KNN_DIRECT) return false;
+  const uint64_t kch = knn_chunks(d, knn_mode(precision));
   const uint64_t ntiles = (n + 15) / 16;
   const uint32_t ss = std::max<uint32_t>(1, (3 * k + 15) / 16);
   return n >= 1024 && d >= 1 && kch <= 8 && k <= 128 && ntiles >= 8ull * ss;
 }
 
-// Certified MFMA candidates + exact rerank; returns false when the input is
-// not finite (the caller then uses the direct kernel).
+// MFMA candidates + exact rerank.  certify: every row is either proven exact
+// (certificate) or rescanned exactly; otherwise (approximate mode) the
+// listed candidates' exact top-k is returned as is.  Returns false when the
+// input is not finite (the caller then uses the direct kernel).
 bool knn_mfma_path(const float *x_dev, uint64_t n, uint64_t d, uint32_t k, const float *xp,
-                   const KnnLayout &L, bool split, DevBuf<uint32_t> &idx, DevBuf<float> &dist,
+                   const KnnLayout &L, int mode, bool certify, DevBuf<uint32_t> &idx, DevBuf<float> &dist,
                    hipStream_t s) {
-  const uint32_t kch = static_cast<uint32_t>((d + 31) / 32);
+  const bool split = mode == 1;
+  const uint32_t kch = knn_chunks(d, mode);
   const uint64_t ntiles = (n + 15) / 16;
   const uint64_t rows_padded = ntiles * 16;
   DevBuf<double> mean(d), scratch(cov_scratch_doubles(n, static_cast<uint32_t>(d)));
   PM_HIP(launch_colmean_f64(x_dev, n, static_cast<uint32_t>(d), mean.get(), scratch.get(), s));
   const size_t frag_words = ntiles * kch * 64 * 4;  // 16 B per lane fragment
-  DevBuf<uint32_t> fhi(frag_words), flo(split ? frag_words : 4);
-  PM_HIP(launch_knn_frag(x_dev, n, static_cast<uint32_t>(d), mean.get(), kch, ntiles, fhi.get(),
-                         split ? flo.get() : nullptr, s));
-  DevBuf<float> arow(rows_padded);
+  DevBuf<uint32_t> fhi(frag_words), flo(mode != 0 ? frag_words : 4);
+  DevBuf<float> arow(rows_padded), rinv(mode == 2 ? rows_padded : 1);
   DevBuf<uint32_t> flags(2);
   PM_HIP(launch_fill_zero(flags.get(), flags.bytes(), s));
-  const double kappa = knn_kappa(kch * 32, split);
-  PM_HIP(launch_knn_rownorm(x_dev, n, static_cast<uint32_t>(d), mean.get(),
-                            static_cast<float>(1.0 - kappa), rows_padded, arow.get(), flags.get(), s));
+  const double kappa = certify ? knn_kappa(kch * 32, split) : 0.0;
+  PM_HIP(launch_knn_rownorm(x_dev, n, static_cast<uint32_t>(d), mean.get(), static_cast<float>(1.0 - kappa),
+                            rows_padded, arow.get(), flags.get(), mode == 2 ? rinv.get() : nullptr, s));
+  if (mode == 2)
+    PM_HIP(launch_knn_frag8(x_dev, n, static_cast<uint32_t>(d), mean.get(), kch, ntiles, rinv.get(), fhi.get(),
+                            flo.get(), s));
+  else
+    PM_HIP(launch_knn_frag(x_dev, n, static_cast<uint32_t>(d), mean.get(), kch, ntiles, fhi.get(),
+                           split ? flo.get() : nullptr, s));
   uint32_t hflags[2] = {0, 0};
   download(hflags, flags.get(), 2, s);
   PM_HIP(hipStreamSynchronize(s));
@@ -79,22 +99,23 @@ bool knn_mfma_path(const float *x_dev, uint64_t n, uint64_t d, uint32_t k, const
   const uint32_t nsplit = std::min<uint32_t>(16, std::max<uint32_t>(1, (4096 + groups - 1) / groups));
   DevBuf<uint32_t> gcnt(n * nsplit), gcand(n * nsplit * capg), fb_list(n);
   DevBuf<float> gtau(n);
-  const void *flo_p = split ? flo.get() : fhi.get();
+  const void *flo_p = mode != 0 ? flo.get() : fhi.get();
+  const float *rinv_p = mode == 2 ? rinv.get() : nullptr;
   Timer tsamp, tk, tr;
   tsamp.start(s);
-  PM_HIP(launch_knn_cand(0, fhi.get(), flo_p, arow.get(), static_cast<uint32_t>(n), kch,
+  PM_HIP(launch_knn_cand(0, fhi.get(), flo_p, arow.get(), rinv_p, static_cast<uint32_t>(n), kch,
                          static_cast<uint32_t>(ntiles), ss, capg, nsplit, gcnt.get(), gcand.get(),
-                         gtau.get(), split, s));
+                         gtau.get(), mode, s));
   tsamp.stop(s);
   tk.start(s);
-  PM_HIP(launch_knn_cand(1, fhi.get(), flo_p, arow.get(), static_cast<uint32_t>(n), kch,
+  PM_HIP(launch_knn_cand(1, fhi.get(), flo_p, arow.get(), rinv_p, static_cast<uint32_t>(n), kch,
                          static_cast<uint32_t>(ntiles), ss, capg, nsplit, gcnt.get(), gcand.get(),
-                         gtau.get(), split, s));
+                         gtau.get(), mode, s));
   tk.stop(s);
   tr.start(s);
   PM_HIP(launch_knn_rerank(xp, L, static_cast<uint32_t>(n), k, capg, nsplit, gcnt.get(), gcand.get(),
-                           gtau.get(), knn_cert_scale(static_cast<uint32_t>(d)), idx.get(), dist.get(),
-                           flags.get() + 1, fb_list.get(), s));
+                           gtau.get(), certify ? knn_cert_scale(static_cast<uint32_t>(d)) : -1.0, idx.get(),
+                           dist.get(), flags.get() + 1, fb_list.get(), s));
   tr.stop(s);
   download(hflags, flags.get(), 2, s);
   PM_HIP(hipStreamSynchronize(s));
@@ -103,7 +124,8 @@ bool knn_mfma_path(const float *x_dev, uint64_t n, uint64_t d, uint32_t k, const
   g_tm.knn_sample_ms += tsamp.ms();
   g_tm.knn_rerank_ms += tr.ms();
   g_tm.knn_candidates = capg;
-  g_tm.knn_path = split ? 1 : 2;
+  g_tm.knn_path = mode == 1 ? 1 : mode == 0 ? 2 : 3;
+  g_tm.knn_certified = certify ? 1 : 0;
   const uint32_t nfb = hflags[1];
   g_tm.knn_rescanned_rows += nfb;
   if (nfb) {
@@ -135,8 +157,12 @@ void knn_device(const float *x_dev, uint64_t n, uint64_t d, uint32_t k, DevBuf<u
   all.start(s);
   PM_HIP(launch_knn_pack(x_dev, xp.get(), n, L, s));
   bool done = false;
-  if (knn_mfma_eligible(n, d, k, precision))
-    done = knn_mfma_path(x_dev, n, d, k, xp.get(), L, precision != PACMAP_KNN_BF16, idx, dist, s);
+  if (knn_mfma_eligible(n, d, k, precision)) {
+    const int mode = knn_mode(precision);
+    // FP8 candidates cannot be certified (the E4M3 error bound is too wide)
+    const bool certify = mode != 2 && !(precision & PACMAP_KNN_APPROX);
+    done = knn_mfma_path(x_dev, n, d, k, xp.get(), L, mode, certify, idx, dist, s);
+  }
   if (!done) {
     Timer ker;
     ker.start(s);
@@ -146,6 +172,7 @@ void knn_device(const float *x_dev, uint64_t n, uint64_t d, uint32_t k, DevBuf<u
     g_tm.knn_kernel_launches += 1;
     g_tm.knn_candidates = n - 1;
     g_tm.knn_path = 0;
+    g_tm.knn_certified = 1;
   }
   all.stop(s);
   g_tm.knn_ms += all.ms();
@@ -226,6 +253,16 @@ void build_csr_from_host(uint64_t n, const PairList lists[3], DevCsr &out, hipSt
   build_csr_dev(n, dp, rows, out, s);
 }
 
+// The target-blocked gradient pays off once Y no longer fits one XCD's L2
+// (n dims 4 B >= 4 MiB).  PACMAP_GRAD_TARGET_BLOCKS=0 / 1 forces it off / on.
+bool use_target_blocks(uint64_t n, uint64_t dims) {
+  if (const char *e = std::getenv("PACMAP_GRAD_TARGET_BLOCKS")) {
+    if (e[0] == '0') return false;
+    if (e[0] == '1') return n >= kTargetBlocks;
+  }
+  return n * dims * 4 >= (4ull << 20);
+}
+
 void optimize_device(const DevCsr &csr, uint64_t n, uint64_t dims, std::vector<float> &y, float lr,
                      const uint64_t iters[3], const uint64_t *snap_idx, uint64_t ns,
                      float *snaps_out, float *loss_out, bool record_loss, hipStream_t s) {
@@ -252,7 +289,26 @@ void optimize_device(const DevCsr &csr, uint64_t n, uint64_t dims, std::vector<f
   PM_HIP(launch_fill_zero(m.get(), m.bytes(), s));
   PM_HIP(launch_fill_zero(v.get(), v.bytes(), s));
   DevBuf<float> d_snap(ns && snaps_out ? ns * nd : 0);
-  const uint32_t nb = grad_blocks(static_cast<uint32_t>(n));
+  // Target-blocked gradient when Y outgrows an XCD's L2 (see kernels.hip)
+  bool tb = use_target_blocks(n, dims) && T > 0 && csr.nnz > 0;
+  DevBuf<uint32_t> tb_ent;
+  DevBuf<uint16_t> tb_off;
+  DevBuf<float> tb_part;
+  if (tb) {
+    tb_ent.alloc(csr.nnz);
+    tb_off.alloc(static_cast<size_t>(kTargetBlocks) * n);
+    DevBuf<uint32_t> bad(1);
+    PM_HIP(launch_fill_zero(bad.get(), bad.bytes(), s));
+    PM_HIP(launch_tb_partition(csr.row_ptr.get(), csr.ent.get(), static_cast<uint32_t>(n), tb_ent.get(),
+                               tb_off.get(), bad.get(), s));
+    uint32_t hb = 0;
+    download(&hb, bad.get(), 1, s);
+    PM_HIP(hipStreamSynchronize(s));
+    if (hb) tb = false;  // a row too long for the u16 range offsets: plain path
+    else tb_part.alloc(static_cast<size_t>(kTargetBlocks) * nd);
+  }
+  g_tm.grad_target_blocks = tb ? kTargetBlocks : 0;
+  const uint32_t nb = tb ? grad_tb_blocks(static_cast<uint32_t>(n)) : grad_blocks(static_cast<uint32_t>(n));
   DevBuf<float> part(record_loss ? T * nb : 0), d_loss(record_loss ? T : 0);
   PM_HIP(hipStreamSynchronize(s));
   g_tm.iterations = T;
@@ -278,7 +334,12 @@ void optimize_device(const DevCsr &csr, uint64_t n, uint64_t dims, std::vector<f
     a.omb2 = 1.0f - beta2;
     a.loss_part = record_loss ? part.get() + it * nb : nullptr;
     a.snap = (writer[it] >= 0 && d_snap.get()) ? d_snap.get() + writer[it] * nd : nullptr;
-    cap_err = launch_grad_adam(a, static_cast<uint32_t>(dims), true, s);
+    if (tb) {
+      a.ent = tb_ent.get();
+      cap_err = launch_grad_adam_tb(a, tb_off.get(), tb_part.get(), static_cast<uint32_t>(dims), true, s);
+    } else {
+      cap_err = launch_grad_adam(a, static_cast<uint32_t>(dims), true, s);
+    }
   }
   const hipError_t end_err = hipStreamEndCapture(s, &gg.g);
   if (cap_err != hipSuccess) fail_gpu(cap_err, "grad_adam launch during capture", __FILE__, __LINE__);
@@ -478,6 +539,7 @@ void fit_impl(const float *x_host, const float *x_dev_in, uint64_t n, uint64_t d
   const uint64_t k = counts[0], n_mn = counts[1], n_fp = counts[2];
   const int pk = cfg->pair_kind;
   if (pk < PACMAP_PAIRS_GENERATE || pk > PACMAP_PAIRS_ALL_PROVIDED) fail(PACMAP_ERR_INVALID_ARGUMENT, "unknown pair configuration");
+  if (!knn_precision_valid(cfg->knn_precision)) fail(PACMAP_ERR_INVALID_ARGUMENT, "unknown kNN precision");
   if (pk == PACMAP_PAIRS_NEIGHBORS_PROVIDED) {  // lib.rs:651-658
     if (cfg->pair_neighbors_rows != n * k || cfg->pair_neighbors_cols != 2) {
       PmError e(PACMAP_ERR_INVALID_NEAREST_NEIGHBOR_SHAPE, "Invalid shape for nearest neighbor pairs");
@@ -684,8 +746,7 @@ int pacmap_knn(const float *x, uint64_t n, uint64_t d, uint64_t k, int32_t preci
                float *dist, uint64_t *stats_out, pacmap_status *st) {
   g_tm = pacmap_timings{};
   return guarded(st, [&] {
-    if (precision < PACMAP_KNN_FP32 || precision > PACMAP_KNN_DIRECT || precision == PACMAP_KNN_FP8)
-      fail(PACMAP_ERR_INVALID_ARGUMENT, "unknown kNN precision");
+    if (!knn_precision_valid(precision)) fail(PACMAP_ERR_INVALID_ARGUMENT, "unknown kNN precision");
     if (n == 0) return;  // knn.rs:141-143
     if (!x && d) fail(PACMAP_ERR_INVALID_ARGUMENT, "x is NULL");
     const uint64_t kk = std::min<uint64_t>(k, n - 1);  // knn.rs:146
@@ -892,6 +953,22 @@ int pacmap_dev_upload(void *dst_dev, const void *src_host, uint64_t bytes, pacma
 
 int pacmap_dev_download(void *dst_host, const void *src_dev, uint64_t bytes, pacmap_status *st) {
   return guarded(st, [&] { PM_HIP(hipMemcpy(dst_host, src_dev, bytes, hipMemcpyDeviceToHost)); });
+}
+
+int pacmap_dev_copy(void *dst_dev, const void *src_dev, uint64_t bytes, uint32_t repeats, double *ms_out,
+                    pacmap_status *st) {
+  return guarded(st, [&] {
+    if (!dst_dev || !src_dev) fail(PACMAP_ERR_INVALID_ARGUMENT, "NULL pointer");
+    if (bytes % 16) fail(PACMAP_ERR_INVALID_ARGUMENT, "bytes must be a multiple of 16");
+    Stream stream;
+    const hipStream_t s = stream.get();
+    Timer t;
+    t.start(s);
+    for (uint32_t r = 0; r < std::max<uint32_t>(1, repeats); ++r) PM_HIP(launch_hbm_copy(dst_dev, src_dev, bytes, s));
+    t.stop(s);
+    const double ms = t.ms();
+    if (ms_out) *ms_out = ms / std::max<uint32_t>(1, repeats);
+  });
 }
 
 void pacmap_last_timings(pacmap_timings *out) {

@@ -59,17 +59,23 @@ hipError_t launch_knn_rescan(const float *xp, uint64_t n, const KnnLayout &L, ui
 // MFMA candidate path (knn_mfma.hip)
 hipError_t launch_knn_frag(const float *x, uint64_t n, uint32_t d, const double *mean, uint32_t kch,
                            uint64_t ntiles, void *fhi, void *flo, hipStream_t s);
+// FP8 E4M3 fragments of the centred rows scaled by 1 / rinv (128-feature chunks)
+hipError_t launch_knn_frag8(const float *x, uint64_t n, uint32_t d, const double *mean, uint32_t kch,
+                            uint64_t ntiles, const float *rinv, void *fa, void *fb, hipStream_t s);
+// arow = |c|^2 (1 - kappa); rinv (optional) = max |c| / 448 per row
 hipError_t launch_knn_rownorm(const float *x, uint64_t n, uint32_t d, const double *mean,
                               float one_minus_kappa, uint64_t rows_padded, float *arow,
-                              uint32_t *nonfinite, hipStream_t s);
+                              uint32_t *nonfinite, float *rinv, hipStream_t s);
 // phase 0: tau per row from a 1/ss tile sample; phase 1: append candidates
 // with lb <= tau to per-(row, split) lists (gcnt: n x nsplit, gcand:
 // n x nsplit x capg).  Query tiles are grouped knn_cand_groups(ntiles) ways.
+// mode 0: bf16, 1: split bf16, 2: FP8 E4M3 with per-row scales (rinv).
 uint32_t knn_cand_groups(uint32_t ntiles);
-hipError_t launch_knn_cand(int phase, const void *fhi, const void *flo, const float *arow,
+hipError_t launch_knn_cand(int phase, const void *fhi, const void *flo, const float *arow, const float *rinv,
                            uint32_t n, uint32_t kch, uint32_t ntiles, uint32_t ss, uint32_t capg,
                            uint32_t nsplit, uint32_t *gcnt, uint32_t *gcand, float *gtau,
-                           bool split, hipStream_t s);
+                           int mode, hipStream_t s);
+// cert_scale < 0: accept every row with at least k listed candidates (no certificate)
 hipError_t launch_knn_rerank(const float *xp, const KnnLayout &L, uint32_t n, uint32_t k,
                              uint32_t capg, uint32_t nsplit, const uint32_t *gcnt,
                              const uint32_t *gcand, const float *gtau, double cert_scale,
@@ -105,6 +111,17 @@ constexpr uint32_t kGradRowsPerBlock = 16;
 uint32_t grad_blocks(uint32_t n);
 // adam == false: write the gradient to grad_out instead of stepping.
 hipError_t launch_grad_adam(const GradArgs &a, uint32_t dims, bool adam, hipStream_t s);
+// Target-blocked gradient for large n (see kernels.hip): the CSR rows are
+// re-laid out by tb_partition (entries grouped by kTargetBlocks index ranges,
+// off: (kTargetBlocks) x n u16 range starts), then each iteration runs the
+// per-range pass (loss_part: grad_tb_blocks(n) floats) and the combine+Adam
+// pass (part: kTargetBlocks x n x dims floats).
+constexpr uint32_t kTargetBlocks = 8;
+uint32_t grad_tb_blocks(uint32_t n);
+hipError_t launch_tb_partition(const uint32_t *row_ptr, const uint32_t *ent, uint32_t n, uint32_t *out,
+                               uint16_t *off, uint32_t *bad, hipStream_t s);
+hipError_t launch_grad_adam_tb(const GradArgs &a, const uint16_t *off, float *part, uint32_t dims, bool adam,
+                               hipStream_t s);
 // loss[t] = sum of part[t * nb .. t * nb + nb) in a fixed order.
 hipError_t launch_loss_reduce(const float *part, uint32_t nb, uint32_t iters, float *loss,
                               hipStream_t s);
@@ -128,6 +145,8 @@ hipError_t launch_project(const float *x, uint64_t n, uint32_t d, const float *m
 hipError_t launch_adam(float *y, const float *g, float *m, float *v, uint64_t count, float lr_t,
                        float omb1, float omb2, hipStream_t s);
 hipError_t launch_fill_zero(void *p, uint64_t bytes, hipStream_t s);
+// dst = src, bytes a multiple of 16, both 16-byte aligned
+hipError_t launch_hbm_copy(void *dst, const void *src, uint64_t bytes, hipStream_t s);
 // dst (n x cols) = src[:, :cols] * scale, src with row stride ld
 hipError_t launch_take_cols(const float *src, uint64_t n, uint32_t ld, uint32_t cols, float scale,
                             float *dst, hipStream_t s);

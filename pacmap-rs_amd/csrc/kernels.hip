@@ -649,6 +649,219 @@ __global__ __launch_bounds__(256) void grad_adam_kernel(const GradArgs a) {
   }
 }
 
+// ---------------------------------------------------------------------------
+// Target-blocked variant for embeddings larger than one XCD's L2.  The other
+// points are split into kTargetBlocks contiguous index ranges; each CSR row
+// is stably partitioned by the range of its entries (tb_offsets below), and
+// pass 1 runs one workgroup per (row group, range) with blockIdx.x % 8 = the
+// range, so with the round-robin dispatch of workgroups over the 8 XCDs
+// every XCD gathers from one eighth of Y, which stays in its L2.  Pass 1
+// writes per-range partial sums; pass 2 adds them in range order and applies
+// Adam.  Deterministic: fixed lane order, butterfly and range order.
+// ---------------------------------------------------------------------------
+constexpr uint32_t kTbLanes = 4;                  // lanes per row in pass 1
+constexpr uint32_t kTbRows = 256 / kTbLanes;      // rows per pass-1 workgroup
+
+// Stable partition of each row by target range; off[b * n + row] (b = 1..7,
+// u16, relative to the row start) is where range b begins; bad[0] is set when
+// a row is longer than 65535 entries (the caller then keeps the plain path).
+__global__ void tb_partition_kernel(const uint32_t *__restrict__ row_ptr, const uint32_t *__restrict__ ent,
+                                    uint32_t n, uint32_t span, uint32_t *__restrict__ out,
+                                    uint16_t *__restrict__ off, uint32_t *__restrict__ bad) {
+  const uint32_t row = blockIdx.x * blockDim.x + threadIdx.x;
+  if (row >= n) return;
+  const uint32_t beg = row_ptr[row], end = row_ptr[row + 1];
+  if (end - beg > 65535u) {
+    atomicOr(bad, 1u);
+    return;
+  }
+  uint32_t cnt[kTargetBlocks];
+#pragma unroll
+  for (uint32_t b = 0; b < kTargetBlocks; ++b) cnt[b] = 0;
+  for (uint32_t e = beg; e < end; ++e) {
+    const uint32_t b = (ent[e] & kIdxMask) / span;
+#pragma unroll
+    for (uint32_t q = 0; q < kTargetBlocks; ++q) cnt[q] += (q == b) ? 1u : 0u;
+  }
+  uint32_t pos[kTargetBlocks];
+  uint32_t acc = 0;
+#pragma unroll
+  for (uint32_t b = 0; b < kTargetBlocks; ++b) {
+    pos[b] = acc;
+    if (b > 0) off[static_cast<size_t>(b) * n + row] = static_cast<uint16_t>(acc);
+    acc += cnt[b];
+  }
+  for (uint32_t e = beg; e < end; ++e) {
+    const uint32_t w = ent[e];
+    const uint32_t b = (w & kIdxMask) / span;
+    uint32_t p = 0;
+#pragma unroll
+    for (uint32_t q = 0; q < kTargetBlocks; ++q)
+      if (q == b) p = pos[q]++;
+    out[beg + p] = w;
+  }
+}
+
+template <int DIMS, bool LOSS>
+__global__ __launch_bounds__(256) void grad_tb_kernel(const GradArgs a, const uint16_t *__restrict__ off,
+                                                      float *__restrict__ part) {
+  const uint32_t b = blockIdx.x % kTargetBlocks, grp = blockIdx.x / kTargetBlocks;
+  const int t = threadIdx.x, gl = t & (kTbLanes - 1);
+  const uint32_t row = grp * kTbRows + (t / kTbLanes);
+  float g[DIMS], yr[DIMS];
+#pragma unroll
+  for (int d = 0; d < DIMS; ++d) {
+    g[d] = 0.0f;
+    yr[d] = 0.0f;
+  }
+  float loss = 0.0f;
+  if (row < a.n) {
+#pragma unroll
+    for (int d = 0; d < DIMS; ++d) yr[d] = a.y_in[static_cast<size_t>(row) * DIMS + d];
+    const uint32_t rb = a.row_ptr[row];
+    const uint32_t beg = rb + (b == 0 ? 0u : off[static_cast<size_t>(b) * a.n + row]);
+    const uint32_t end = b + 1 == kTargetBlocks ? a.row_ptr[row + 1] : rb + off[static_cast<size_t>(b + 1) * a.n + row];
+    constexpr int U = 4;
+    for (uint32_t e0 = beg + gl; e0 < end; e0 += kTbLanes * U) {
+      uint32_t wv[U];
+      float yo[U][DIMS];
+#pragma unroll
+      for (int uu = 0; uu < U; ++uu) {
+        const uint32_t e = e0 + kTbLanes * uu;
+        wv[uu] = e < end ? a.ent[e] : 0xffffffffu;
+      }
+#pragma unroll
+      for (int uu = 0; uu < U; ++uu) {
+        const uint32_t o = wv[uu] == 0xffffffffu ? row : (wv[uu] & kIdxMask);
+#pragma unroll
+        for (int d = 0; d < DIMS; ++d) yo[uu][d] = a.y_in[static_cast<size_t>(o) * DIMS + d];
+      }
+#pragma unroll
+      for (int uu = 0; uu < U; ++uu) {
+        const uint32_t w = wv[uu];
+        if (w == 0xffffffffu) break;
+        const uint32_t ty = w >> 30;
+        float u[DIMS];
+        float D = 1.0f;  // gradient.rs:121-125
+#pragma unroll
+        for (int d = 0; d < DIMS; ++d) {
+          u[d] = yr[d] - yo[uu][d];
+          const float sq = u[d] * u[d];
+          D = D + sq;
+        }
+        const bool fp = ty == 2u;
+        const float c = fp ? 1.0f : (ty == 0u ? 10.0f : 10000.0f);
+        const float c2 = fp ? 2.0f : (ty == 0u ? 20.0f : 20000.0f);
+        const float wt = fp ? a.w[2] : (ty == 0u ? a.w[0] : a.w[1]);
+        const float cd = c + D;
+        const float w1 = wt * (c2 / (cd * cd));  // gradient.rs:130 / 140
+#pragma unroll
+        for (int d = 0; d < DIMS; ++d) {
+          const float tt = w1 * u[d];
+          g[d] = fp ? g[d] - tt : g[d] + tt;
+        }
+        if (LOSS && (w & kOwnerBit)) {
+          const float num = fp ? 1.0f : D;  // gradient.rs:129 / 139
+          loss = loss + wt * (num / cd);
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int o = kTbLanes / 2; o > 0; o >>= 1) {
+#pragma unroll
+    for (int d = 0; d < DIMS; ++d) g[d] = g[d] + __shfl_xor(g[d], o, 64);
+    if (LOSS) loss = loss + __shfl_xor(loss, o, 64);
+  }
+  if (gl == 0 && row < a.n) {
+#pragma unroll
+    for (int d = 0; d < DIMS; ++d) part[(static_cast<size_t>(b) * a.n + row) * DIMS + d] = g[d];
+  }
+  if (LOSS) {
+    __shared__ float sLoss[kTbRows];
+    if (gl == 0) sLoss[t / kTbLanes] = loss;
+    __syncthreads();
+    if (t == 0) {
+      float s = 0.0f;
+      for (uint32_t i = 0; i < kTbRows; ++i) s = s + sLoss[i];
+      a.loss_part[blockIdx.x] = s;
+    }
+  }
+}
+
+template <int DIMS, bool ADAM>
+__global__ __launch_bounds__(256) void grad_tb_combine_kernel(const GradArgs a, const float *__restrict__ part) {
+  const uint32_t row = blockIdx.x * 256 + threadIdx.x;
+  if (row >= a.n) return;
+  const size_t base = static_cast<size_t>(row) * DIMS;
+  float g[DIMS];
+#pragma unroll
+  for (int d = 0; d < DIMS; ++d) g[d] = part[base + d];
+  for (uint32_t b = 1; b < kTargetBlocks; ++b)
+#pragma unroll
+    for (int d = 0; d < DIMS; ++d) g[d] = g[d] + part[(static_cast<size_t>(b) * a.n + row) * DIMS + d];
+  if (ADAM) {
+#pragma unroll
+    for (int d = 0; d < DIMS; ++d) {
+      float m = a.m[base + d], v = a.v[base + d];
+      m = m + a.omb1 * (g[d] - m);          // adam.rs:57
+      const float gg = g[d] * g[d];
+      v = v + a.omb2 * (gg - v);            // adam.rs:58
+      const float num = a.lr_t * m;
+      const float den = sqrtf(v) + 1e-7f;
+      const float y = a.y_in[base + d] - num / den;  // adam.rs:59
+      a.m[base + d] = m;
+      a.v[base + d] = v;
+      a.y_out[base + d] = y;
+      if (a.snap) a.snap[base + d] = y;
+    }
+  } else {
+#pragma unroll
+    for (int d = 0; d < DIMS; ++d) a.grad_out[base + d] = g[d];
+  }
+}
+
+uint32_t tb_span(uint32_t n) { return (n + kTargetBlocks - 1) / kTargetBlocks; }
+uint32_t grad_tb_blocks(uint32_t n) { return ((n + kTbRows - 1) / kTbRows) * kTargetBlocks; }
+
+hipError_t launch_tb_partition(const uint32_t *row_ptr, const uint32_t *ent, uint32_t n, uint32_t *out,
+                               uint16_t *off, uint32_t *bad, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  tb_partition_kernel<<<(n + 255) / 256, 256, 0, s>>>(row_ptr, ent, n, tb_span(n), out, off, bad);
+  return hipGetLastError();
+}
+
+template <int DIMS>
+static hipError_t launch_tb_dims(const GradArgs &a, const uint16_t *off, float *part, bool adam, hipStream_t s) {
+  const uint32_t g1 = grad_tb_blocks(a.n);
+  if (g1 == 0) return hipSuccess;
+  if (a.loss_part)
+    grad_tb_kernel<DIMS, true><<<g1, 256, 0, s>>>(a, off, part);
+  else
+    grad_tb_kernel<DIMS, false><<<g1, 256, 0, s>>>(a, off, part);
+  const uint32_t g2 = (a.n + 255) / 256;
+  if (adam)
+    grad_tb_combine_kernel<DIMS, true><<<g2, 256, 0, s>>>(a, part);
+  else
+    grad_tb_combine_kernel<DIMS, false><<<g2, 256, 0, s>>>(a, part);
+  return hipGetLastError();
+}
+
+hipError_t launch_grad_adam_tb(const GradArgs &a, const uint16_t *off, float *part, uint32_t dims, bool adam,
+                               hipStream_t s) {
+  switch (dims) {
+    case 1: return launch_tb_dims<1>(a, off, part, adam, s);
+    case 2: return launch_tb_dims<2>(a, off, part, adam, s);
+    case 3: return launch_tb_dims<3>(a, off, part, adam, s);
+    case 4: return launch_tb_dims<4>(a, off, part, adam, s);
+    case 5: return launch_tb_dims<5>(a, off, part, adam, s);
+    case 6: return launch_tb_dims<6>(a, off, part, adam, s);
+    case 7: return launch_tb_dims<7>(a, off, part, adam, s);
+    case 8: return launch_tb_dims<8>(a, off, part, adam, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
 uint32_t grad_blocks(uint32_t n) { return (n + kGradRowsPerBlock - 1) / kGradRowsPerBlock; }
 
 template <int DIMS>
@@ -1052,6 +1265,25 @@ hipError_t launch_take_cols(const float *src, uint64_t n, uint32_t ld, uint32_t 
   if (blocks > 65536) blocks = 65536;
   if (blocks == 0) return hipSuccess;
   take_cols_kernel<<<static_cast<uint32_t>(blocks), 256, 0, s>>>(src, n, ld, cols, scale, dst);
+  return hipGetLastError();
+}
+
+// Plain streaming copy (16-byte vectors, grid stride): the reference workload
+// for HBM bandwidth and for calibrating the FETCH_SIZE / WRITE_SIZE counters.
+__global__ __launch_bounds__(256) void hbm_copy_kernel(const uint4 *__restrict__ src, uint4 *__restrict__ dst,
+                                                       uint64_t n16) {
+  for (uint64_t e = blockIdx.x * 256ull + threadIdx.x; e < n16; e += gridDim.x * 256ull) dst[e] = src[e];
+}
+
+hipError_t launch_hbm_copy(void *dst, const void *src, uint64_t bytes, hipStream_t s) {
+  if (bytes % 16 || reinterpret_cast<uintptr_t>(dst) % 16 || reinterpret_cast<uintptr_t>(src) % 16)
+    return hipErrorInvalidValue;
+  const uint64_t n16 = bytes / 16;
+  uint64_t blocks = (n16 + 255) / 256;
+  if (blocks > 8192) blocks = 8192;
+  if (blocks == 0) return hipSuccess;
+  hbm_copy_kernel<<<static_cast<uint32_t>(blocks), 256, 0, s>>>(static_cast<const uint4 *>(src),
+                                                                static_cast<uint4 *>(dst), n16);
   return hipGetLastError();
 }
 

@@ -27,6 +27,7 @@ namespace pm {
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float floatx4 __attribute__((ext_vector_type(4)));
+typedef int intx8 __attribute__((ext_vector_type(8)));
 
 __device__ __forceinline__ uint32_t tkey_m(float f) {
   const uint32_t b = __float_as_uint(f);
@@ -155,25 +156,83 @@ __global__ void knn_frag_kernel(const float *__restrict__ x, uint64_t n, uint32_
   }
 }
 
+// f32 -> OCP FP8 E4M3 (bias 7, 3 mantissa bits, max 448, no infinities),
+// round to nearest even, saturating at +-448.  Finite inputs only.
+__device__ __forceinline__ uint32_t f32_to_e4m3(float v) {
+  const uint32_t sign = (__float_as_uint(v) >> 24) & 0x80u;
+  const float x = fabsf(v);
+  uint32_t code;
+  if (x < 0.015625f) {  // below 2^-6: subnormal steps of 2^-9 (8 -> smallest normal)
+    code = static_cast<uint32_t>(rintf(x * 512.0f));
+  } else {
+    const uint32_t b = __float_as_uint(x);
+    const uint32_t r = b + 0x7ffffu + ((b >> 20) & 1u);  // round the 20 dropped mantissa bits
+    const int e = static_cast<int>(r >> 23) - 127;
+    const uint32_t m = (r >> 20) & 7u;
+    code = (e > 8 || (e == 8 && m == 7u)) ? 0x7eu : (static_cast<uint32_t>(e + 7) << 3) | m;
+  }
+  return sign | code;
+}
+
+// FP8 fragments of the centred rows scaled by 1 / rinv[row] (|values| <= 448):
+// [row tile t][128-feature chunk kc][lane l][32 bytes], row = 16 t + (l & 15),
+// byte e = feature 128 kc + 32 (l >> 4) + e; bytes 0..15 go to fa, 16..31 to
+// fb (the two halves of a v_mfma_*_16x16x128_f8f6f4 operand).
+__global__ void knn_frag8_kernel(const float *__restrict__ x, uint64_t n, uint32_t d,
+                                 const double *__restrict__ mean, uint32_t kch, uint64_t ntiles,
+                                 const float *__restrict__ rinv, uint4 *__restrict__ fa,
+                                 uint4 *__restrict__ fb) {
+  const uint64_t total = ntiles * kch * 64;
+  for (uint64_t g = blockIdx.x * 256ull + threadIdx.x; g < total; g += gridDim.x * 256ull) {
+    const uint32_t l = static_cast<uint32_t>(g & 63);
+    const uint64_t tk = g >> 6;
+    const uint32_t kc = static_cast<uint32_t>(tk % kch);
+    const uint64_t t = tk / kch;
+    const uint64_t row = t * 16 + (l & 15);
+    const uint32_t f0 = kc * 128 + 32 * (l >> 4);
+    const float ri = row < n ? rinv[row] : 0.0f;
+    const float sc = ri > 0.0f ? 1.0f / ri : 0.0f;
+    uint32_t w[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      uint32_t word = 0;
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const uint32_t f = f0 + 4 * q + e;
+        float c = 0.0f;
+        if (row < n && f < d) c = static_cast<float>(static_cast<double>(x[row * d + f]) - mean[f]);
+        word |= f32_to_e4m3(c * sc) << (8 * e);
+      }
+      w[q] = word;
+    }
+    fa[g] = make_uint4(w[0], w[1], w[2], w[3]);
+    fb[g] = make_uint4(w[4], w[5], w[6], w[7]);
+  }
+}
+
 __global__ void knn_rownorm_kernel(const float *__restrict__ x, uint64_t n, uint32_t d,
                                    const double *__restrict__ mean, float one_minus_kappa,
                                    uint64_t rows_padded, float *__restrict__ arow,
-                                   uint32_t *__restrict__ nonfinite) {
+                                   uint32_t *__restrict__ nonfinite, float *__restrict__ rinv) {
   const uint64_t row = blockIdx.x * 256ull + threadIdx.x;
   if (row >= rows_padded) return;
   if (row >= n) {
     arow[row] = 0.0f;
+    if (rinv) rinv[row] = 0.0f;
     return;
   }
   double s = 0.0;
+  float mx = 0.0f;
   bool bad = false;
   for (uint32_t f = 0; f < d; ++f) {
     const float xv = x[row * d + f];
     bad = bad || !isfinite(xv);
     const float c = static_cast<float>(static_cast<double>(xv) - mean[f]);
     s += static_cast<double>(c) * c;
+    mx = fmaxf(mx, fabsf(c));
   }
   arow[row] = static_cast<float>(s * static_cast<double>(one_minus_kappa));
+  if (rinv) rinv[row] = mx / 448.0f;
   if (bad) atomicOr(nonfinite, 1u);
 }
 
@@ -190,17 +249,24 @@ __global__ void knn_rownorm_kernel(const float *__restrict__ x, uint64_t n, uint
 // PHASE 0 scans a 1/ss sample of candidate tiles and writes tau per row.
 // PHASE 1 scans the candidate tiles of split blockIdx.y (of gridDim.y) and
 // appends every candidate with lb <= tau to the row's list of that split.
-template <int KCH, int RT, bool SPLIT, int PHASE>
+//
+// MODE 0: one bf16 product per multiply-add; MODE 1: split bf16 (hi hi + hi lo
+// + lo hi); MODE 2: FP8 E4M3 rows scaled by 1 / rinv (one 16x16x128 f8f6f4
+// MFMA per 128-feature chunk), the product rescaled by rinv_q rinv_c.
+template <int KCH, int RT, int MODE, int PHASE>
 __global__ __launch_bounds__(256) void knn_mfma_kernel(
     const uint4 *__restrict__ fhi, const uint4 *__restrict__ flo, const float *__restrict__ arow,
-    uint32_t n, uint32_t ntiles, uint32_t ss, uint32_t capg, uint32_t *__restrict__ gcnt,
-    uint32_t *__restrict__ gcand, float *__restrict__ gtau) {
+    const float *__restrict__ rinv, uint32_t n, uint32_t ntiles, uint32_t ss, uint32_t capg,
+    uint32_t *__restrict__ gcnt, uint32_t *__restrict__ gcand, float *__restrict__ gtau) {
+  constexpr bool SPLIT = MODE == 1;
+  constexpr bool FP8 = MODE == 2;
   constexpr int CT = (8 / KCH) > 0 ? (8 / KCH) : 1;  // candidate tiles per LDS stage
-  constexpr int NH = SPLIT ? 2 : 1;
+  constexpr int NH = MODE != 0 ? 2 : 1;
   constexpr int PER = CT * KCH * NH * 64;            // uint4 per stage
   constexpr int PF = (PER + 255) / 256;              // prefetch registers per thread
   __shared__ uint4 sB[2][CT][KCH][NH][64];
   __shared__ float sA[2][CT][16];
+  __shared__ float sI[2][CT][16];  // FP8: candidate rinv
   __shared__ uint32_t sCnt[4][16 * RT];
 
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -209,7 +275,7 @@ __global__ __launch_bounds__(256) void knn_mfma_kernel(
   const uint32_t split = PHASE == 0 ? 0 : blockIdx.y, nsplit = PHASE == 0 ? 1 : gridDim.y;
 
   bf16x8 ahi[RT][KCH], alo[RT][KCH];
-  float aq[RT][4];
+  float aq[RT][4], iq[RT][4];
   uint32_t qidx[RT][4];
 #pragma unroll
   for (int rt = 0; rt < RT; ++rt) {
@@ -219,7 +285,7 @@ __global__ __launch_bounds__(256) void knn_mfma_kernel(
       uint4 h = make_uint4(0, 0, 0, 0), lo = make_uint4(0, 0, 0, 0);
       if (tile < ntiles) {
         h = fhi[(static_cast<size_t>(tile) * KCH + kc) * 64 + lane];
-        if (SPLIT) lo = flo[(static_cast<size_t>(tile) * KCH + kc) * 64 + lane];
+        if (NH == 2) lo = flo[(static_cast<size_t>(tile) * KCH + kc) * 64 + lane];
       }
       ahi[rt][kc] = __builtin_bit_cast(bf16x8, h);
       alo[rt][kc] = __builtin_bit_cast(bf16x8, lo);
@@ -229,6 +295,7 @@ __global__ __launch_bounds__(256) void knn_mfma_kernel(
       const uint32_t q = tile * 16 + 4 * rsub + r;
       qidx[rt][r] = q;
       aq[rt][r] = q < n ? arow[q] : 0.0f;
+      iq[rt][r] = (FP8 && q < n) ? rinv[q] : 1.0f;
     }
   }
   for (int i = threadIdx.x; i < 4 * 16 * RT; i += 256) (&sCnt[0][0])[i] = 0;
@@ -252,7 +319,7 @@ __global__ __launch_bounds__(256) void knn_mfma_kernel(
   const uint32_t nstage = (slot_hi - slot_lo + CT - 1) / CT;
 
   uint4 pf[PF];
-  float pa = 0.0f;
+  float pa = 0.0f, pi = 0.0f;
   auto fetch = [&](uint32_t st) {
     const uint32_t s0 = slot_lo + st * CT;
 #pragma unroll
@@ -274,6 +341,7 @@ __global__ __launch_bounds__(256) void knn_mfma_kernel(
       const uint32_t slot = s0 + threadIdx.x / 16;
       const uint32_t tile = slot_tile(slot);
       pa = (slot < slot_hi && tile < ntiles) ? arow[tile * 16 + (threadIdx.x & 15)] : 0.0f;
+      if (FP8) pi = (slot < slot_hi && tile < ntiles) ? rinv[tile * 16 + (threadIdx.x & 15)] : 0.0f;
     }
   };
   auto stash = [&](int buf) {
@@ -282,7 +350,10 @@ __global__ __launch_bounds__(256) void knn_mfma_kernel(
       const int i = static_cast<int>(threadIdx.x) + j * 256;
       if (i < PER) (&sB[buf][0][0][0][0])[i] = pf[j];
     }
-    if (threadIdx.x < CT * 16) (&sA[buf][0][0])[threadIdx.x] = pa;
+    if (threadIdx.x < CT * 16) {
+      (&sA[buf][0][0])[threadIdx.x] = pa;
+      if (FP8) (&sI[buf][0][0])[threadIdx.x] = pi;
+    }
   };
 
   if (nstage) {
@@ -304,6 +375,22 @@ __global__ __launch_bounds__(256) void knn_mfma_kernel(
       for (int rt = 0; rt < RT; ++rt) acc[rt] = floatx4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
       for (int kc = 0; kc < KCH; ++kc) {
+        if (FP8) {
+          const uint4 b0 = sB[buf][ct][kc][0][lane], b1 = sB[buf][ct][kc][NH - 1][lane];
+          const intx8 bv = {static_cast<int>(b0.x), static_cast<int>(b0.y), static_cast<int>(b0.z),
+                            static_cast<int>(b0.w), static_cast<int>(b1.x), static_cast<int>(b1.y),
+                            static_cast<int>(b1.z), static_cast<int>(b1.w)};
+#pragma unroll
+          for (int rt = 0; rt < RT; ++rt) {
+            const uint4 a0 = __builtin_bit_cast(uint4, ahi[rt][kc]), a1 = __builtin_bit_cast(uint4, alo[rt][kc]);
+            const intx8 av = {static_cast<int>(a0.x), static_cast<int>(a0.y), static_cast<int>(a0.z),
+                              static_cast<int>(a0.w), static_cast<int>(a1.x), static_cast<int>(a1.y),
+                              static_cast<int>(a1.z), static_cast<int>(a1.w)};
+            // format 0 (FP8 E4M3) for both operands, unit block scales (E8M0 127)
+            acc[rt] = __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(av, bv, acc[rt], 0, 0, 0, 127, 0, 127);
+          }
+          continue;
+        }
         const bf16x8 bh = __builtin_bit_cast(bf16x8, sB[buf][ct][kc][0][lane]);
 #pragma unroll
         for (int rt = 0; rt < RT; ++rt)
@@ -319,13 +406,15 @@ __global__ __launch_bounds__(256) void knn_mfma_kernel(
       }
       const uint32_t c = tile * 16 + col;
       const float ac = sA[buf][ct][col];
+      const float ic = FP8 ? sI[buf][ct][col] : 1.0f;
       const bool cvalid = c < n;
 #pragma unroll
       for (int rt = 0; rt < RT; ++rt)
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
           const uint32_t q = qidx[rt][r];
-          const float lb = fmaf(-2.0f, acc[rt][r], aq[rt][r] + ac);
+          const float dot = FP8 ? acc[rt][r] * (iq[rt][r] * ic) : acc[rt][r];
+          const float lb = fmaf(-2.0f, dot, aq[rt][r] + ac);
           const bool ok = cvalid && q < n && c != q;
           if (PHASE == 0) {
             if (ok && lb < best1[rt][r]) {
@@ -444,7 +533,8 @@ __global__ __launch_bounds__(256) void knn_rerank_kernel(
   const float t = gtau[q];
   // every unlisted candidate has a true squared distance > tau (up to the
   // rounding factor folded in cert_scale), hence a reference distance > dk
-  const bool cert = all_listed || (t > 0.0f && sqrt(static_cast<double>(t) * cert_scale) > static_cast<double>(dk));
+  const bool cert = all_listed || cert_scale < 0.0 ||
+                    (t > 0.0f && sqrt(static_cast<double>(t) * cert_scale) > static_cast<double>(dk));
   if (!cert) {
     if (lane == 0) fb_list[atomicAdd(fb_count, 1u)] = q;
     return;
@@ -474,11 +564,22 @@ hipError_t launch_knn_frag(const float *x, uint64_t n, uint32_t d, const double 
   return hipGetLastError();
 }
 
+hipError_t launch_knn_frag8(const float *x, uint64_t n, uint32_t d, const double *mean, uint32_t kch,
+                            uint64_t ntiles, const float *rinv, void *fa, void *fb, hipStream_t s) {
+  const uint64_t total = ntiles * kch * 64;
+  uint64_t blocks = (total + 255) / 256;
+  if (blocks > 65536) blocks = 65536;
+  if (blocks == 0) return hipSuccess;
+  knn_frag8_kernel<<<static_cast<uint32_t>(blocks), 256, 0, s>>>(x, n, d, mean, kch, ntiles, rinv,
+                                                                 static_cast<uint4 *>(fa), static_cast<uint4 *>(fb));
+  return hipGetLastError();
+}
+
 hipError_t launch_knn_rownorm(const float *x, uint64_t n, uint32_t d, const double *mean,
                               float one_minus_kappa, uint64_t rows_padded, float *arow,
-                              uint32_t *nonfinite, hipStream_t s) {
+                              uint32_t *nonfinite, float *rinv, hipStream_t s) {
   knn_rownorm_kernel<<<static_cast<uint32_t>((rows_padded + 255) / 256), 256, 0, s>>>(
-      x, n, d, mean, one_minus_kappa, rows_padded, arow, nonfinite);
+      x, n, d, mean, one_minus_kappa, rows_padded, arow, nonfinite, rinv);
   return hipGetLastError();
 }
 
@@ -486,39 +587,42 @@ constexpr int kCandRT = 2;  // query tiles per wave in the candidate kernels
 
 uint32_t knn_cand_groups(uint32_t ntiles) { return (ntiles + 4 * kCandRT - 1) / (4 * kCandRT); }
 
-template <int KCH, bool SPLIT>
+template <int KCH, int MODE>
 static hipError_t cand_dispatch(int phase, const void *fhi, const void *flo, const float *arow,
-                                uint32_t n, uint32_t ntiles, uint32_t ss, uint32_t capg,
+                                const float *rinv, uint32_t n, uint32_t ntiles, uint32_t ss, uint32_t capg,
                                 uint32_t nsplit, uint32_t *gcnt, uint32_t *gcand, float *gtau,
                                 hipStream_t s) {
   const uint32_t groups = knn_cand_groups(ntiles);
   const uint4 *h = static_cast<const uint4 *>(fhi), *l = static_cast<const uint4 *>(flo);
   if (phase == 0)
-    knn_mfma_kernel<KCH, kCandRT, SPLIT, 0><<<dim3(groups, 1), 256, 0, s>>>(h, l, arow, n, ntiles, ss,
-                                                                           capg, gcnt, gcand, gtau);
+    knn_mfma_kernel<KCH, kCandRT, MODE, 0><<<dim3(groups, 1), 256, 0, s>>>(h, l, arow, rinv, n, ntiles, ss,
+                                                                          capg, gcnt, gcand, gtau);
   else
-    knn_mfma_kernel<KCH, kCandRT, SPLIT, 1><<<dim3(groups, nsplit), 256, 0, s>>>(
-        h, l, arow, n, ntiles, ss, capg, gcnt, gcand, gtau);
+    knn_mfma_kernel<KCH, kCandRT, MODE, 1><<<dim3(groups, nsplit), 256, 0, s>>>(
+        h, l, arow, rinv, n, ntiles, ss, capg, gcnt, gcand, gtau);
   return hipGetLastError();
 }
 
 template <int KCH>
-static hipError_t cand_dispatch_split(bool split, int phase, const void *fhi, const void *flo,
-                                      const float *arow, uint32_t n, uint32_t ntiles, uint32_t ss,
-                                      uint32_t capg, uint32_t nsplit, uint32_t *gcnt, uint32_t *gcand,
-                                      float *gtau, hipStream_t s) {
-  return split ? cand_dispatch<KCH, true>(phase, fhi, flo, arow, n, ntiles, ss, capg, nsplit, gcnt,
-                                          gcand, gtau, s)
-               : cand_dispatch<KCH, false>(phase, fhi, flo, arow, n, ntiles, ss, capg, nsplit, gcnt,
-                                           gcand, gtau, s);
+static hipError_t cand_dispatch_mode(int mode, int phase, const void *fhi, const void *flo,
+                                     const float *arow, const float *rinv, uint32_t n, uint32_t ntiles,
+                                     uint32_t ss, uint32_t capg, uint32_t nsplit, uint32_t *gcnt,
+                                     uint32_t *gcand, float *gtau, hipStream_t s) {
+  switch (mode) {
+    case 0: return cand_dispatch<KCH, 0>(phase, fhi, flo, arow, rinv, n, ntiles, ss, capg, nsplit, gcnt, gcand, gtau, s);
+    case 1: return cand_dispatch<KCH, 1>(phase, fhi, flo, arow, rinv, n, ntiles, ss, capg, nsplit, gcnt, gcand, gtau, s);
+    case 2: return cand_dispatch<KCH, 2>(phase, fhi, flo, arow, rinv, n, ntiles, ss, capg, nsplit, gcnt, gcand, gtau, s);
+    default: return hipErrorInvalidValue;
+  }
 }
 
-hipError_t launch_knn_cand(int phase, const void *fhi, const void *flo, const float *arow,
+hipError_t launch_knn_cand(int phase, const void *fhi, const void *flo, const float *arow, const float *rinv,
                            uint32_t n, uint32_t kch, uint32_t ntiles, uint32_t ss, uint32_t capg,
                            uint32_t nsplit, uint32_t *gcnt, uint32_t *gcand, float *gtau,
-                           bool split, hipStream_t s) {
+                           int mode, hipStream_t s) {
+  if (mode == 2 && rinv == nullptr) return hipErrorInvalidValue;
 #define PM_CAND(K) \
-  case K: return cand_dispatch_split<K>(split, phase, fhi, flo, arow, n, ntiles, ss, capg, nsplit, gcnt, gcand, gtau, s);
+  case K: return cand_dispatch_mode<K>(mode, phase, fhi, flo, arow, rinv, n, ntiles, ss, capg, nsplit, gcnt, gcand, gtau, s);
   switch (kch) {
     PM_CAND(1)
     PM_CAND(2)

@@ -77,12 +77,14 @@ class Timings(C.Structure):
         (name, C.c_uint64) for name in ("iterations", "csr_entries", "knn_rescanned_rows",
                                         "knn_candidates")] + [
         ("knn_rerank_ms", C.c_double), ("knn_fallback_ms", C.c_double), ("knn_path", C.c_uint64),
-        ("knn_sample_ms", C.c_double)]
+        ("knn_sample_ms", C.c_double), ("knn_certified", C.c_uint64),
+        ("grad_target_blocks", C.c_uint64)]
 
 
 INIT_PCA, INIT_VALUE, INIT_RANDOM = 0, 1, 2
 PAIRS_GENERATE, PAIRS_NEIGHBORS_PROVIDED, PAIRS_ALL_PROVIDED = 0, 1, 2
 KNN_PRECISIONS = {"fp32": 0, "bf16": 1, "fp8": 2, "direct": 3}
+KNN_APPROX = 0x100  # drop the certificate and the rescan (approximate neighbours)
 
 _P = C.c_void_p
 _u64 = C.c_uint64
@@ -142,6 +144,7 @@ def lib():
         L.pacmap_dev_free.argtypes = [_P]
         L.pacmap_dev_upload.argtypes = [_P, _P, _u64, C.POINTER(Status)]
         L.pacmap_dev_download.argtypes = [_P, _P, _u64, C.POINTER(Status)]
+        L.pacmap_dev_copy.argtypes = [_P, _P, _u64, C.c_uint32, C.POINTER(C.c_double), C.POINTER(Status)]
         L.pacmap_last_timings.argtypes = [C.POINTER(Timings)]
         L.pacmap_last_timings.restype = None
         L.pacmap_device_info.argtypes = [C.c_char_p, _u64, C.POINTER(C.c_int32)]
@@ -372,8 +375,12 @@ def fit_transform(x, config: Optional[Configuration] = None, return_loss: bool =
 
 class knn:  # noqa: N801 -- mirrors the `pacmap::knn` module
     @staticmethod
-    def find_k_nearest_neighbors(data, k: int, precision: str = "fp32"):
-        """knn.rs:137-194: (n x min(k, n-1) u32 indices, f32 distances), ascending."""
+    def find_k_nearest_neighbors(data, k: int, precision: str = "fp32", certify: bool = True):
+        """knn.rs:137-194: (n x min(k, n-1) u32 indices, f32 distances), ascending.
+
+        precision "fp32" (split bf16), "bf16" and "direct" are exact; "fp8"
+        and certify=False return the exact top-k of the MFMA candidates
+        (approximate neighbours)."""
         data = np.ascontiguousarray(data, np.float32)
         if data.ndim != 2:
             raise ValueError("data must be 2-D")
@@ -385,7 +392,8 @@ class knn:  # noqa: N801 -- mirrors the `pacmap::knn` module
         dist = np.zeros((n, kk), np.float32)
         stats = np.zeros(2, np.uint64)
         st = Status()
-        rc = lib().pacmap_knn(data.ctypes.data, n, d, k, KNN_PRECISIONS[precision],
+        prec = KNN_PRECISIONS[precision] | (0 if certify else KNN_APPROX)
+        rc = lib().pacmap_knn(data.ctypes.data, n, d, k, prec,
                               idx.ctypes.data, dist.ctypes.data, stats.ctypes.data, C.byref(st))
         _check(rc, st)
         return idx, dist
@@ -588,6 +596,24 @@ class mlp:  # noqa: N801 -- namespace of the parametric encoder entry points
                                      a.ctypes.data, na, b.ctypes.data, nb, c.ctypes.data, nc,
                                      w.ctypes.data, g.ctypes.data, loss.ctypes.data, C.byref(st)), st)
         return g, float(loss[0])
+
+
+def hbm_copy(nbytes: int, repeats: int = 10) -> float:
+    """Streaming device copy of nbytes (HBM reference); returns ms per copy."""
+    st = Status()
+    a, b = C.c_void_p(), C.c_void_p()
+    L = lib()
+    _check(L.pacmap_dev_alloc(nbytes, C.byref(a), C.byref(st)), st)
+    try:
+        _check(L.pacmap_dev_alloc(nbytes, C.byref(b), C.byref(st)), st)
+        try:
+            ms = C.c_double()
+            _check(L.pacmap_dev_copy(b, a, nbytes, repeats, C.byref(ms), C.byref(st)), st)
+            return ms.value
+        finally:
+            L.pacmap_dev_free(b)
+    finally:
+        L.pacmap_dev_free(a)
 
 
 def last_timings() -> dict:
