@@ -25,6 +25,16 @@ METRIC = "fit_transform s @70k×784; grad+Adam GB/s; MLP vs plain at equal 10-NN
 N, D, DPRIME, K_NN, N_MN, N_FP, DIMS = 70000, 784, 100, 10, 10, 20, 2
 ITERS = (100, 100, 250)
 HBM_PEAK_GBS = 8000.0
+
+# HBM bytes per launch from rocprofv3 --pmc runs of this bench (FETCH_SIZE and
+# WRITE_SIZE in KiB per dispatch), scaled by the readings of a plain 1 GiB
+# copy kernel (bench.py --workload copy): FETCH_SIZE read 0.5 of the bytes
+# copied (factor 2.0), WRITE_SIZE all of them (factor 1.0).  Summaries:
+# profiles/pmc_traffic_70000x784.md.  Keys: (workload, kernel).
+MEASURED_TRAFFIC = {
+    ("70k", "grad"): None,
+    ("70k", "knn_mfma_kernel"): None,
+}
 FP32_PEAK_TFLOPS = 157.3
 BF16_PEAK_TFLOPS = 2500.0
 
@@ -354,16 +364,33 @@ def main():
     achieved = flops / (knn_ms * 1e-3) / 1e12
     path = int(tms[-1]["knn_path"])
     if path == 0:
-        kname, peak = "knn_exact_kernel (direct FP32 in the reference's distance order)", FP32_PEAK_TFLOPS
+        kname, kkey, peak = ("knn_exact_kernel (direct FP32 in the reference's distance order)",
+                             "knn_exact_kernel", FP32_PEAK_TFLOPS)
     else:
-        kname = ("knn_mfma_cand_kernel (split-bf16 MFMA candidates: 3 bf16 MFMAs per "
-                 "algorithmic multiply-add, plus a 1/ss sample pass)" if path == 1 else
-                 "knn_mfma_cand_kernel (bf16 MFMA candidates)")
-        peak = BF16_PEAK_TFLOPS
+        kname = ("knn_mfma_kernel phase 1 (split-bf16 MFMA candidates: 3 bf16 MFMAs per "
+                 "algorithmic multiply-add)" if path == 1 else "knn_mfma_kernel phase 1 (bf16 MFMA candidates)")
+        kkey, peak = "knn_mfma_kernel", BF16_PEAK_TFLOPS
     opt_ms = float(np.mean([t["optimize_ms"] for t in tms]))
     pairs = n * (k_nn + n_mn + n_fp)
     b_iter = grad_adam_bytes(n, pairs, DIMS)
     it_total = sum(ITERS)
+    us_iter = opt_ms * 1e3 / it_total
+    tb = int(tms[-1]["grad_target_blocks"])
+    gname = ("grad_tb_kernel + grad_tb_combine_kernel (target-blocked gradient, then Adam)" if tb else
+             "grad_adam_kernel (fused CSR gradient + Adam)")
+    traffic = MEASURED_TRAFFIC.get((args.workload, "grad" if not tb else "grad_tb"))
+    grad_roof = {"kernel": gname, "bound": "hbm", "achieved": round(b_iter / (us_iter * 1e-6) / 1e9, 1),
+                 "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                 "frac": round(b_iter / (us_iter * 1e-6) / 1e9 / HBM_PEAK_GBS, 4),
+                 "traffic": traffic,
+                 "algorithmic": f"B_iter = 8P + 4(n+1) + 24 n dims = {b_iter} B per launch, "
+                                f"{us_iter:.3f} us per launch ({it_total} launches in one hipGraph)"}
+    knn_roof = {"kernel": kname, "bound": "mfma", "achieved": round(achieved, 3),
+                "peak": peak, "unit": "TFLOP/s", "frac": round(achieved / peak, 4),
+                "traffic": MEASURED_TRAFFIC.get((args.workload, kkey)),
+                "algorithmic": f"2*n^2*d' = {flops:.3e} flop per launch, {knn_ms:.3f} ms per launch"}
+    # the roofline object describes the kernel with the larger share of the step
+    grad_dominant = opt_ms >= knn_ms
     breakdown = {k: round(float(np.mean([t[k] for t in tms])), 3) for k in (
         "total_ms", "preprocess_ms", "pca_cov_ms", "pca_eig_ms", "knn_ms", "knn_kernel_ms",
         "sigma_ms", "sampling_ms", "csr_ms", "graph_build_ms", "optimize_ms")}
@@ -379,18 +406,16 @@ def main():
                                 "k=10 (60 candidates), 10 MN + 20 FP per point, 450 iterations"),
                    "n": n, "d": d, "knn_dims": DPRIME, "n_neighbors": k_nn, "n_mn": n_mn,
                    "n_fp": n_fp, "iterations": it_total},
-        "roofline": {"kernel": kname, "bound": "mfma", "achieved": round(achieved, 3),
-                     "peak": peak, "unit": "TFLOP/s", "frac": round(achieved / peak, 4),
-                     "traffic": None,
-                     "algorithmic": f"2*n^2*d' = {flops:.3e} flop per launch, "
-                                    f"{knn_ms:.3f} ms per launch"},
-        "grad_adam": {"bytes_per_iter": b_iter, "us_per_iter": round(opt_ms * 1e3 / it_total, 3),
+        "roofline": grad_roof if grad_dominant else knn_roof,
+        "roofline_other": knn_roof if grad_dominant else grad_roof,
+        "grad_adam": {"bytes_per_iter": b_iter, "us_per_iter": round(us_iter, 3),
                       "GB_per_s": round(b_iter * it_total / (opt_ms * 1e-3) / 1e9, 1),
                       "frac_hbm_peak": round(b_iter * it_total / (opt_ms * 1e-3) / 1e9 /
-                                             HBM_PEAK_GBS, 4)},
+                                             HBM_PEAK_GBS, 4),
+                      "target_blocks": tb},
         "breakdown_ms": breakdown,
         "knn": {"path": ["direct", "split-bf16 MFMA + certified rerank",
-                         "bf16 MFMA + certified rerank"][path],
+                         "bf16 MFMA + certified rerank", "fp8 MFMA, uncertified"][path],
                 "rescanned_rows": int(tms[-1]["knn_rescanned_rows"]),
                 "candidates_capacity": int(tms[-1]["knn_candidates"]),
                 "rerank_ms": round(float(np.mean([t["knn_rerank_ms"] for t in tms])), 3),

@@ -549,21 +549,33 @@ __global__ __launch_bounds__(256) void grad_adam_kernel(const GradArgs a) {
   const int t = threadIdx.x, gl = t & 15, grp = t >> 4;
   const uint32_t row = blockIdx.x * kGradRowsPerBlock + grp;
   float g[DIMS];
-  float yr[DIMS];
+  float yr[DIMS], mr[DIMS], vr[DIMS];
 #pragma unroll
   for (int d = 0; d < DIMS; ++d) {
     g[d] = 0.0f;
     yr[d] = 0.0f;
+    mr[d] = 0.0f;
+    vr[d] = 0.0f;
   }
   float loss = 0.0f;
   if (row < a.n) {
+    const size_t base = static_cast<size_t>(row) * DIMS;
+    // the Adam state does not depend on the gradient: fetch it up front
+    if (ADAM && gl == 0) {
 #pragma unroll
-    for (int d = 0; d < DIMS; ++d) yr[d] = a.y_in[static_cast<size_t>(row) * DIMS + d];
+      for (int d = 0; d < DIMS; ++d) {
+        mr[d] = a.m[base + d];
+        vr[d] = a.v[base + d];
+      }
+    }
+#pragma unroll
+    for (int d = 0; d < DIMS; ++d) yr[d] = a.y_in[base + d];
     const uint32_t beg = a.row_ptr[row], end = a.row_ptr[row + 1];
-    // lane gl owns entries beg+gl, beg+gl+16, ...; four of them are loaded
-    // (entry word, then the other point's y) before any is used, so the
-    // gathers overlap; they are then consumed in the same order
-    constexpr int U = 4;
+    // lane gl owns entries beg+gl, beg+gl+16, ...; U of them are loaded
+    // (entry word, then the other point's y) before any is used, so one
+    // round of loads covers rows of up to 16 U entries; entries are consumed
+    // in the same order
+    constexpr int U = 8;
     for (uint32_t e0 = beg + gl; e0 < end; e0 += 16 * U) {
       uint32_t wv[U];
       float yo[U][DIMS];
@@ -620,7 +632,7 @@ __global__ __launch_bounds__(256) void grad_adam_kernel(const GradArgs a) {
     if (ADAM) {
 #pragma unroll
       for (int d = 0; d < DIMS; ++d) {
-        float m = a.m[base + d], v = a.v[base + d];
+        float m = mr[d], v = vr[d];
         m = m + a.omb1 * (g[d] - m);          // adam.rs:57
         const float gg = g[d] * g[d];
         v = v + a.omb2 * (gg - v);            // adam.rs:58
@@ -980,13 +992,24 @@ hipError_t launch_normalize(const float *x, float *out, uint64_t count, const ui
 }
 
 // mean_axis(Axis(0)) of a row-major array: rows added in order from zero,
-// then divided by n (lib.rs:506-508).  One thread per column keeps that order.
-__global__ void colmean_seq_kernel(const float *__restrict__ x, uint64_t n, uint32_t d,
-                                   float *__restrict__ mean) {
+// then divided by n (lib.rs:506-508).  One thread per column keeps that order;
+// the column is a serial chain, so each thread issues the loads of 128 rows
+// before adding them (in order) to keep enough memory requests in flight.
+__global__ __launch_bounds__(64) void colmean_seq_kernel(const float *__restrict__ x, uint64_t n, uint32_t d,
+                                                         float *__restrict__ mean) {
   const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= d) return;
+  constexpr int U = 128;
   float s = 0.0f;
-  for (uint64_t r = 0; r < n; ++r) s = s + x[r * d + c];
+  uint64_t r = 0;
+  for (; r + U <= n; r += U) {
+    float v[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) v[u] = x[(r + u) * d + c];
+#pragma unroll
+    for (int u = 0; u < U; ++u) s = s + v[u];
+  }
+  for (; r < n; ++r) s = s + x[r * d + c];
   mean[c] = s / static_cast<float>(n);
 }
 
@@ -1272,15 +1295,24 @@ hipError_t launch_take_cols(const float *src, uint64_t n, uint32_t ld, uint32_t 
 // for HBM bandwidth and for calibrating the FETCH_SIZE / WRITE_SIZE counters.
 __global__ __launch_bounds__(256) void hbm_copy_kernel(const uint4 *__restrict__ src, uint4 *__restrict__ dst,
                                                        uint64_t n16) {
-  for (uint64_t e = blockIdx.x * 256ull + threadIdx.x; e < n16; e += gridDim.x * 256ull) dst[e] = src[e];
+  const uint64_t stride = gridDim.x * 256ull;
+  uint64_t e = blockIdx.x * 256ull + threadIdx.x;
+  for (; e + 3 * stride < n16; e += 4 * stride) {  // four loads in flight per thread
+    const uint4 v0 = src[e], v1 = src[e + stride], v2 = src[e + 2 * stride], v3 = src[e + 3 * stride];
+    dst[e] = v0;
+    dst[e + stride] = v1;
+    dst[e + 2 * stride] = v2;
+    dst[e + 3 * stride] = v3;
+  }
+  for (; e < n16; e += stride) dst[e] = src[e];
 }
 
 hipError_t launch_hbm_copy(void *dst, const void *src, uint64_t bytes, hipStream_t s) {
   if (bytes % 16 || reinterpret_cast<uintptr_t>(dst) % 16 || reinterpret_cast<uintptr_t>(src) % 16)
     return hipErrorInvalidValue;
   const uint64_t n16 = bytes / 16;
-  uint64_t blocks = (n16 + 255) / 256;
-  if (blocks > 8192) blocks = 8192;
+  uint64_t blocks = (n16 + 1023) / 1024;
+  if (blocks > 16384) blocks = 16384;
   if (blocks == 0) return hipSuccess;
   hbm_copy_kernel<<<static_cast<uint32_t>(blocks), 256, 0, s>>>(static_cast<const uint4 *>(src),
                                                                 static_cast<uint4 *>(dst), n16);

@@ -29,28 +29,54 @@ __device__ __forceinline__ float gelu_grad(float z) {
 // Batch rows: pair p of the batch is pair index perm(start + p) of the typed
 // list (NN rows first, then MN, then FP); perm(pos) = (a pos + c) mod P.
 // ---------------------------------------------------------------------------
-__global__ void mlp_gather_kernel(const float *__restrict__ x, uint32_t d, uint32_t Kp,
-                                  const uint32_t *__restrict__ pi, const uint32_t *__restrict__ pj,
-                                  uint64_t P, uint64_t start, uint32_t count, uint64_t pa, uint64_t pc,
-                                  uint32_t B, uint32_t M, uint16_t *__restrict__ xb, uint16_t *__restrict__ xbT,
-                                  uint32_t *__restrict__ pidx) {
-  const uint32_t r = blockIdx.x;  // one block per batch row
-  uint32_t src = 0;
-  if (r < 2 * B) {
-    const uint32_t p = r < B ? r : r - B;
-    if (p < count) {
-      const uint64_t pos = start + p;
-      const uint64_t e = pa == 0 ? pos : (pa * pos + pc) % P;  // pa == 0: identity order
-      src = r < B ? pi[e] : pj[e];
-      if (r < B && threadIdx.x == 0) pidx[p] = static_cast<uint32_t>(e);
-    } else if (r < B && threadIdx.x == 0) {
-      pidx[p] = 0xffffffffu;
+// One block per 64 batch rows: the bf16 tile goes through LDS so that both
+// xb (rows) and xbT (features x rows) are written with contiguous stores.
+__global__ __launch_bounds__(256) void mlp_gather_kernel(const float *__restrict__ x, uint32_t d, uint32_t Kp,
+                                                         const uint32_t *__restrict__ pi,
+                                                         const uint32_t *__restrict__ pj, uint64_t P,
+                                                         uint64_t start, uint32_t count, uint64_t pa, uint64_t pc,
+                                                         uint32_t B, uint32_t M, uint16_t *__restrict__ xb,
+                                                         uint16_t *__restrict__ xbT, uint32_t *__restrict__ pidx) {
+  extern __shared__ uint16_t tile[];  // 64 x (Kp + 8)
+  __shared__ uint32_t src[64];
+  __shared__ bool valid[64];
+  const uint32_t ld = Kp + 8, r0 = blockIdx.x * 64, t = threadIdx.x;
+  if (t < 64) {
+    const uint32_t r = r0 + t;
+    uint32_t sidx = 0;
+    bool ok = false;
+    if (r < 2 * B) {
+      const uint32_t p = r < B ? r : r - B;
+      if (p < count) {
+        const uint64_t pos = start + p;
+        const uint64_t e = pa == 0 ? pos : (pa * pos + pc) % P;  // pa == 0: identity order
+        sidx = r < B ? pi[e] : pj[e];
+        ok = true;
+        if (r < B) pidx[p] = static_cast<uint32_t>(e);
+      } else if (r < B) {
+        pidx[p] = 0xffffffffu;
+      }
+      ok = ok || (r < 2 * B);  // rows of padding pairs read point 0
     }
+    src[t] = sidx;
+    valid[t] = ok;
   }
-  for (uint32_t k = threadIdx.x; k < Kp; k += blockDim.x) {
-    const uint16_t v = (r < 2 * B && k < d) ? f2bf(x[static_cast<size_t>(src) * d + k]) : 0;
-    xb[static_cast<size_t>(r) * Kp + k] = v;
-    xbT[static_cast<size_t>(k) * M + r] = v;
+  __syncthreads();
+  for (uint32_t e = t; e < 64 * Kp; e += 256) {
+    const uint32_t rr = e / Kp, k = e % Kp;
+    const uint16_t v = (valid[rr] && k < d) ? f2bf(x[static_cast<size_t>(src[rr]) * d + k]) : 0;
+    tile[rr * ld + k] = v;
+    xb[static_cast<size_t>(r0 + rr) * Kp + k] = v;
+  }
+  __syncthreads();
+  // xbT: for feature k, the 64 rows are 128 contiguous bytes (8 uint4 per k)
+  for (uint32_t e = t; e < Kp * 8; e += 256) {
+    const uint32_t k = e / 8, q = e % 8;
+    uint32_t w[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+      w[j] = tile[(8 * q + 2 * j) * ld + k] | (static_cast<uint32_t>(tile[(8 * q + 2 * j + 1) * ld + k]) << 16);
+    *reinterpret_cast<uint4 *>(xbT + static_cast<size_t>(k) * M + r0 + 8 * q) = make_uint4(w[0], w[1], w[2], w[3]);
   }
 }
 
@@ -102,15 +128,30 @@ __global__ __launch_bounds__(256) void mlp_gemm_kernel(const uint16_t *__restric
   const uint16_t *a1 = A + static_cast<size_t>(m0 + 16 + lr) * lda + lk;
   const uint16_t *b0 = B + static_cast<size_t>(n0 + lr) * ldb + lk;
   const uint16_t *b1 = B + static_cast<size_t>(n0 + 16 + lr) * ldb + lk;
-  for (uint32_t k0 = kbeg; k0 < kend; k0 += 32) {
-    const bf16x8m va0 = __builtin_bit_cast(bf16x8m, *reinterpret_cast<const uint4 *>(a0 + k0));
-    const bf16x8m va1 = __builtin_bit_cast(bf16x8m, *reinterpret_cast<const uint4 *>(a1 + k0));
-    const bf16x8m vb0 = __builtin_bit_cast(bf16x8m, *reinterpret_cast<const uint4 *>(b0 + k0));
-    const bf16x8m vb1 = __builtin_bit_cast(bf16x8m, *reinterpret_cast<const uint4 *>(b1 + k0));
-    acc[0][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(va0, vb0, acc[0][0], 0, 0, 0);
-    acc[0][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(va0, vb1, acc[0][1], 0, 0, 0);
-    acc[1][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(va1, vb0, acc[1][0], 0, 0, 0);
-    acc[1][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(va1, vb1, acc[1][1], 0, 0, 0);
+  // up to 8 k-steps (256 values of k) of operands are loaded before any MFMA
+  constexpr int KS = 8;
+  for (uint32_t k0 = kbeg; k0 < kend; k0 += 32 * KS) {
+    uint4 ra0[KS], ra1[KS], rb0[KS], rb1[KS];
+#pragma unroll
+    for (int q = 0; q < KS; ++q) {
+      const uint32_t k = k0 + 32 * q;
+      if (k < kend) {
+        ra0[q] = *reinterpret_cast<const uint4 *>(a0 + k);
+        ra1[q] = *reinterpret_cast<const uint4 *>(a1 + k);
+        rb0[q] = *reinterpret_cast<const uint4 *>(b0 + k);
+        rb1[q] = *reinterpret_cast<const uint4 *>(b1 + k);
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < KS; ++q) {
+      if (k0 + 32 * q >= kend) break;
+      const bf16x8m va0 = __builtin_bit_cast(bf16x8m, ra0[q]), va1 = __builtin_bit_cast(bf16x8m, ra1[q]);
+      const bf16x8m vb0 = __builtin_bit_cast(bf16x8m, rb0[q]), vb1 = __builtin_bit_cast(bf16x8m, rb1[q]);
+      acc[0][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(va0, vb0, acc[0][0], 0, 0, 0);
+      acc[0][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(va0, vb1, acc[0][1], 0, 0, 0);
+      acc[1][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(va1, vb0, acc[1][0], 0, 0, 0);
+      acc[1][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(va1, vb1, acc[1][1], 0, 0, 0);
+    }
   }
   // lane holds rows 4 (lane >> 4) + r (r < 4) of column lane & 15 of each 16 x 16 tile
 #pragma unroll
@@ -163,14 +204,30 @@ __global__ __launch_bounds__(256) void mlp_out_kernel(const float *__restrict__ 
   extern __shared__ float sW[];  // dims x H
   for (uint32_t e = threadIdx.x; e < dims * H; e += blockDim.x) sW[e] = W3[e];
   __syncthreads();
-  const uint32_t m = blockIdx.x * blockDim.x + threadIdx.x;
-  if (m >= rows) return;
-  const float *hr = h2 + static_cast<size_t>(m) * H;
-  for (uint32_t c = 0; c < dims; ++c) {
-    float s = 0.0f;
-    for (uint32_t i = 0; i < H; ++i) s = fmaf(hr[i], sW[c * H + i], s);
-    Y[static_cast<size_t>(m) * dims + c] = s + b3[c];
+  // 4 lanes per row, lane q takes the columns q*4 + 16 j (float4 loads)
+  const uint32_t m = blockIdx.x * 64 + (threadIdx.x >> 2), q = threadIdx.x & 3;
+  float s[8];
+#pragma unroll
+  for (int c = 0; c < 8; ++c) s[c] = 0.0f;
+  if (m < rows) {
+    const float *hr = h2 + static_cast<size_t>(m) * H;
+    for (uint32_t i = 4 * q; i < H; i += 16) {
+      const float4 h = *reinterpret_cast<const float4 *>(hr + i);
+#pragma unroll
+      for (int c = 0; c < 8; ++c)
+        if (static_cast<uint32_t>(c) < dims) {
+          const float *w = sW + c * H + i;
+          s[c] = fmaf(h.x, w[0], fmaf(h.y, w[1], fmaf(h.z, w[2], fmaf(h.w, w[3], s[c]))));
+        }
+    }
   }
+#pragma unroll
+  for (int c = 0; c < 8; ++c) {
+    s[c] += __shfl_xor(s[c], 1, 64);
+    s[c] += __shfl_xor(s[c], 2, 64);
+  }
+  if (m < rows && q == 0)
+    for (uint32_t c = 0; c < dims; ++c) Y[static_cast<size_t>(m) * dims + c] = s[c] + b3[c];
 }
 
 // ---------------------------------------------------------------------------
@@ -235,36 +292,42 @@ __global__ __launch_bounds__(256) void mlp_out_bwd_kernel(const float *__restric
                                                           const float *__restrict__ z2, uint32_t H, uint32_t M,
                                                           uint16_t *__restrict__ dz2b, uint16_t *__restrict__ dz2T,
                                                           float *__restrict__ w3_part, float *__restrict__ b3_part) {
-  const uint32_t m0 = blockIdx.x * 64;
+  constexpr uint32_t RB = 16;  // rows per block
+  const uint32_t m0 = blockIdx.x * RB;
+  __shared__ float sdY[RB * 8];
+  if (threadIdx.x < RB * dims) sdY[threadIdx.x] = dY[static_cast<size_t>(m0) * dims + threadIdx.x];
+  __syncthreads();
   for (uint32_t i = threadIdx.x; i < H; i += blockDim.x) {
     float w[8], gw[8];
     for (uint32_t c = 0; c < dims; ++c) {
       w[c] = W3[c * H + i];
       gw[c] = 0.0f;
     }
-    for (uint32_t r = 0; r < 64; r += 4) {
-      uint16_t dv[4];
-      for (uint32_t rr = 0; rr < 4; ++rr) {
-        const uint32_t m = m0 + r + rr;
-        float dh = 0.0f;
-        const float hv = h2[static_cast<size_t>(m) * H + i];
-        for (uint32_t c = 0; c < dims; ++c) {
-          const float g = dY[static_cast<size_t>(m) * dims + c];
-          dh = fmaf(g, w[c], dh);
-          gw[c] = fmaf(g, hv, gw[c]);
-        }
-        dv[rr] = f2bf(dh * gelu_grad(z2[static_cast<size_t>(m) * H + i]));
-        dz2b[static_cast<size_t>(m) * H + i] = dv[rr];
+    uint32_t pk[RB / 2];
+#pragma unroll
+    for (uint32_t r = 0; r < RB; ++r) {
+      const size_t o = static_cast<size_t>(m0 + r) * H + i;
+      const float hv = h2[o], zv = z2[o];
+      float dh = 0.0f;
+      for (uint32_t c = 0; c < dims; ++c) {
+        const float g = sdY[r * dims + c];
+        dh = fmaf(g, w[c], dh);
+        gw[c] = fmaf(g, hv, gw[c]);
       }
-      *reinterpret_cast<uint2 *>(dz2T + static_cast<size_t>(i) * M + m0 + r) =
-          make_uint2(dv[0] | (static_cast<uint32_t>(dv[1]) << 16), dv[2] | (static_cast<uint32_t>(dv[3]) << 16));
+      const uint16_t dv = f2bf(dh * gelu_grad(zv));
+      dz2b[o] = dv;
+      if (r & 1) pk[r / 2] |= static_cast<uint32_t>(dv) << 16;
+      else pk[r / 2] = dv;
     }
+    uint4 *dst = reinterpret_cast<uint4 *>(dz2T + static_cast<size_t>(i) * M + m0);
+    dst[0] = make_uint4(pk[0], pk[1], pk[2], pk[3]);
+    dst[1] = make_uint4(pk[4], pk[5], pk[6], pk[7]);
     for (uint32_t c = 0; c < dims; ++c) w3_part[(static_cast<size_t>(blockIdx.x) * dims + c) * H + i] = gw[c];
   }
   if (threadIdx.x < dims) {
-    float s = 0.0f;
-    for (uint32_t r = 0; r < 64; ++r) s += dY[static_cast<size_t>(m0 + r) * dims + threadIdx.x];
-    b3_part[static_cast<size_t>(blockIdx.x) * dims + threadIdx.x] = s;
+    float sb = 0.0f;
+    for (uint32_t r = 0; r < RB; ++r) sb += sdY[r * dims + threadIdx.x];
+    b3_part[static_cast<size_t>(blockIdx.x) * dims + threadIdx.x] = sb;
   }
 }
 
@@ -272,9 +335,14 @@ __global__ __launch_bounds__(256) void mlp_out_bwd_kernel(const float *__restric
 __global__ __launch_bounds__(256) void mlp_rowsum_kernel(const uint16_t *__restrict__ dzT, uint32_t M,
                                                          float *__restrict__ out) {
   __shared__ float red[256];
-  const uint16_t *r = dzT + static_cast<size_t>(blockIdx.x) * M;
+  const uint4 *r = reinterpret_cast<const uint4 *>(dzT + static_cast<size_t>(blockIdx.x) * M);
   float s = 0.0f;
-  for (uint32_t m = threadIdx.x; m < M; m += 256) s += bf2f(r[m]);
+  for (uint32_t m = threadIdx.x; m < M / 8; m += 256) {  // M is a multiple of 64
+    const uint4 v = r[m];
+    const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+    for (int j = 0; j < 4; ++j) s += bf2f(static_cast<uint16_t>(w[j] & 0xffffu)) + bf2f(static_cast<uint16_t>(w[j] >> 16));
+  }
   red[threadIdx.x] = s;
   __syncthreads();
   for (uint32_t w = 128; w > 0; w >>= 1) {
@@ -282,6 +350,23 @@ __global__ __launch_bounds__(256) void mlp_rowsum_kernel(const uint16_t *__restr
     __syncthreads();
   }
   if (threadIdx.x == 0) out[blockIdx.x] = red[0];
+}
+
+// out[o] = sum over z < S of part[z * slab + o] for o < count: one block per
+// output, 256 strided partial sums, then a fixed-order tree
+__global__ __launch_bounds__(256) void mlp_tree_sum_kernel(const float *__restrict__ part, uint32_t S, uint64_t slab,
+                                                           float *__restrict__ out) {
+  __shared__ float red[256];
+  const uint32_t o = blockIdx.x;
+  float s = 0.0f;
+  for (uint32_t z = threadIdx.x; z < S; z += 256) s += part[z * slab + o];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (uint32_t w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[o] = red[0];
 }
 
 // out[r][c] (r < rows, c < cols, row stride ldo) = sum over s < S of
@@ -336,7 +421,9 @@ __global__ void mlp_pack_kernel(const float *__restrict__ p, uint32_t d, uint32_
 hipError_t launch_mlp_gather(const float *x, uint32_t d, uint32_t Kp, const uint32_t *pi, const uint32_t *pj,
                              uint64_t P, uint64_t start, uint32_t count, uint64_t pa, uint64_t pc, uint32_t B,
                              uint32_t M, uint16_t *xb, uint16_t *xbT, uint32_t *pidx, hipStream_t s) {
-  mlp_gather_kernel<<<M, 128, 0, s>>>(x, d, Kp, pi, pj, P, start, count, pa, pc, B, M, xb, xbT, pidx);
+  if (M % 64 || Kp % 8) return hipErrorInvalidValue;
+  const size_t shm = static_cast<size_t>(64) * (Kp + 8) * sizeof(uint16_t);
+  mlp_gather_kernel<<<M / 64, 256, shm, s>>>(x, d, Kp, pi, pj, P, start, count, pa, pc, B, M, xb, xbT, pidx);
   return hipGetLastError();
 }
 
@@ -391,7 +478,8 @@ hipError_t launch_mlp_wgrad(const uint16_t *AT, const uint16_t *BT, uint32_t Mo,
 
 hipError_t launch_mlp_out(const float *h2, uint32_t H, const float *W3, const float *b3, uint32_t dims,
                           uint32_t rows, float *Y, hipStream_t s) {
-  mlp_out_kernel<<<(rows + 255) / 256, 256, dims * H * sizeof(float), s>>>(h2, H, W3, b3, dims, rows, Y);
+  if (H % 16 || dims > 8) return hipErrorInvalidValue;
+  mlp_out_kernel<<<(rows + 63) / 64, 256, dims * H * sizeof(float), s>>>(h2, H, W3, b3, dims, rows, Y);
   return hipGetLastError();
 }
 
@@ -406,8 +494,8 @@ hipError_t launch_mlp_pair_loss(const float *Y, uint32_t dims, uint32_t B, const
 hipError_t launch_mlp_out_bwd(const float *dY, uint32_t dims, const float *W3, const float *h2, const float *z2,
                               uint32_t H, uint32_t M, uint16_t *dz2b, uint16_t *dz2T, float *w3_part,
                               float *b3_part, hipStream_t s) {
-  if (M % 64) return hipErrorInvalidValue;
-  mlp_out_bwd_kernel<<<M / 64, 256, 0, s>>>(dY, dims, W3, h2, z2, H, M, dz2b, dz2T, w3_part, b3_part);
+  if (M % 64 || dims > 8) return hipErrorInvalidValue;
+  mlp_out_bwd_kernel<<<M / 16, 256, 0, s>>>(dY, dims, W3, h2, z2, H, M, dz2b, dz2T, w3_part, b3_part);
   return hipGetLastError();
 }
 
@@ -422,6 +510,13 @@ hipError_t launch_mlp_sum_parts(const float *part, uint32_t S, uint32_t rows, ui
   if (total == 0) return hipSuccess;
   mlp_sum_parts_kernel<<<static_cast<uint32_t>((total + 255) / 256), 256, 0, s>>>(part, S, rows, cols, ldp, slab,
                                                                                  out, ldo);
+  return hipGetLastError();
+}
+
+hipError_t launch_mlp_tree_sum(const float *part, uint32_t S, uint64_t slab, uint32_t count, float *out,
+                               hipStream_t s) {
+  if (count == 0) return hipSuccess;
+  mlp_tree_sum_kernel<<<count, 256, 0, s>>>(part, S, slab, out);
   return hipGetLastError();
 }
 

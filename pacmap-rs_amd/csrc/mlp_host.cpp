@@ -81,8 +81,8 @@ struct MlpWork {
     h2f.alloc(MH);
     Y.alloc(static_cast<size_t>(M) * sh.dims);
     dY.alloc(static_cast<size_t>(M) * sh.dims);
-    w3p.alloc(static_cast<size_t>(M / 64) * sh.dims * sh.H);
-    b3p.alloc(static_cast<size_t>(M / 64) * sh.dims);
+    w3p.alloc(static_cast<size_t>(M / 16) * sh.dims * sh.H);
+    b3p.alloc(static_cast<size_t>(M / 16) * sh.dims);
     wpart.alloc(static_cast<size_t>(S) * sh.H * std::max(sh.H, sh.Kp));
     pidx.alloc(B);
   }
@@ -109,9 +109,8 @@ void batch_grad(const MlpShape &sh, MlpWork &w, const MlpWeights &W, const float
   PM_HIP(launch_mlp_wgrad(w.dz1T.get(), w.xbT.get(), H, Kp, M, w.S, w.wpart.get(), grad + sh.oW1, sh.d, sh.d, s));
   PM_HIP(launch_mlp_rowsum(w.dz2T.get(), H, M, grad + sh.ob2, s));
   PM_HIP(launch_mlp_rowsum(w.dz1T.get(), H, M, grad + sh.ob1, s));
-  PM_HIP(launch_mlp_sum_parts(w.w3p.get(), M / 64, sh.dims, H, H, static_cast<uint64_t>(sh.dims) * H,
-                              grad + sh.oW3, H, s));
-  PM_HIP(launch_mlp_sum_parts(w.b3p.get(), M / 64, 1, sh.dims, sh.dims, sh.dims, grad + sh.ob3, sh.dims, s));
+  PM_HIP(launch_mlp_tree_sum(w.w3p.get(), M / 16, static_cast<uint64_t>(sh.dims) * H, sh.dims * H, grad + sh.oW3, s));
+  PM_HIP(launch_mlp_tree_sum(w.b3p.get(), M / 16, sh.dims, sh.dims, grad + sh.ob3, s));
 }
 
 // Typed pair list on the device: NN rows, then MN rows, then FP rows.

@@ -30,12 +30,17 @@ hipError_t launch_mlp_out(const float *h2, uint32_t H, const float *W3, const fl
 hipError_t launch_mlp_pair_loss(const float *Y, uint32_t dims, uint32_t B, const uint32_t *pidx,
                                 const uint32_t *pi, const uint32_t *pj, uint64_t nn_end, uint64_t mn_end,
                                 const float w[3], float inv_count, float *dY, float *loss_part, hipStream_t s);
+// per 16-row block: dz2 (bf16, row-major and transposed) and the partials of
+// dW3 (M/16 slabs of dims x H) and db3 (M/16 slabs of dims)
 hipError_t launch_mlp_out_bwd(const float *dY, uint32_t dims, const float *W3, const float *h2, const float *z2,
                               uint32_t H, uint32_t M, uint16_t *dz2b, uint16_t *dz2T, float *w3_part,
                               float *b3_part, hipStream_t s);
 hipError_t launch_mlp_rowsum(const uint16_t *dzT, uint32_t rows, uint32_t M, float *out, hipStream_t s);
 hipError_t launch_mlp_sum_parts(const float *part, uint32_t S, uint32_t rows, uint32_t cols, uint32_t ldp,
                                 uint64_t slab, float *out, uint32_t ldo, hipStream_t s);
+// out[o] = sum over z < S of part[z * slab + o], o < count (fixed-order tree)
+hipError_t launch_mlp_tree_sum(const float *part, uint32_t S, uint64_t slab, uint32_t count, float *out,
+                               hipStream_t s);
 hipError_t launch_mlp_adam(float *p, const float *g, float *m, float *v, uint64_t count, float lr_t, float b1,
                            float b2, float eps, hipStream_t s);
 hipError_t launch_mlp_pack(const float *p, uint32_t d, uint32_t Kp, uint32_t H, uint16_t *w1b, uint16_t *w2b,
