@@ -194,13 +194,18 @@ def encoder_run(pm):
     t0 = time.perf_counter()
     y_plain, _ = pm.fit_transform(x_te, cfg)
     t_plain = time.perf_counter() - t0
+    xte_dev = pm.DeviceArray(x_te)
+    t0 = time.perf_counter()
+    pm.fit_transform(xte_dev, cfg)
+    t_plain_dev = time.perf_counter() - t0
     log("bench: scoring")
     pres_mlp = knn_preservation(pm, xp_te, y_mlp)
     pres_plain = knn_preservation(pm, xp_te, y_plain)
     trip_mlp = triplet_accuracy(xp_te, y_mlp, np.random.default_rng(7))
     trip_plain = triplet_accuracy(xp_te, y_plain, np.random.default_rng(7))
     return {
-        "metric": METRIC, "value": round(t_plain / t_mlp, 2), "unit": "x (plain fit / encoder transform)",
+        "metric": METRIC, "value": round(t_plain_dev / t_mlp_dev, 2),
+        "unit": "x (plain fit_transform / encoder transform, held-out points resident in HBM)",
         "n_gpus": 1, "steps": 1, "warmup": 1, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "bf16 weights, fp32 accumulate", "data": "synthetic",
         "config": {"workload": "encoder 100->256->256->2 (GELU) on a Swiss roll in 100 dims: 150000 "
@@ -209,7 +214,8 @@ def encoder_run(pm):
         "encoder": {"train_s": round(t_train, 3), "pair_generation_s": round(t_pairs, 3),
                     "epoch_loss": [round(float(v), 5) for v in eloss],
                     "transform_ms": round(t_mlp * 1e3, 3), "transform_resident_ms": round(t_mlp_dev * 1e3, 3),
-                    "plain_fit_ms": round(t_plain * 1e3, 3),
+                    "plain_fit_ms": round(t_plain * 1e3, 3), "plain_fit_resident_ms": round(t_plain_dev * 1e3, 3),
+                    "speedup_host_inputs": round(t_plain / t_mlp, 2),
                     "knn10_preservation": {"encoder": round(pres_mlp, 4), "plain": round(pres_plain, 4)},
                     "triplet_accuracy": {"encoder": round(trip_mlp, 4), "plain": round(trip_plain, 4)}},
     }

@@ -323,6 +323,7 @@ void optimize_device(const DevCsr &csr, uint64_t n, uint64_t dims, std::vector<f
     GradArgs a;
     a.row_ptr = csr.row_ptr.get();
     a.ent = csr.ent.get();
+    a.mean_row_len = static_cast<float>(static_cast<double>(csr.nnz) / static_cast<double>(n));
     a.y_in = ybuf.get() + (it % 2) * nd;
     a.y_out = ybuf.get() + ((it + 1) % 2) * nd;
     a.m = m.get();
@@ -796,6 +797,7 @@ int pacmap_grad(const float *y, uint64_t n, uint64_t dims, const uint32_t *pair_
     GradArgs a;
     a.row_ptr = csr.row_ptr.get();
     a.ent = csr.ent.get();
+    a.mean_row_len = static_cast<float>(static_cast<double>(csr.nnz) / static_cast<double>(n));
     a.y_in = dy.get();
     a.grad_out = dg.get();
     a.loss_part = part.get();
@@ -962,6 +964,7 @@ int pacmap_dev_copy(void *dst_dev, const void *src_dev, uint64_t bytes, uint32_t
     if (bytes % 16) fail(PACMAP_ERR_INVALID_ARGUMENT, "bytes must be a multiple of 16");
     Stream stream;
     const hipStream_t s = stream.get();
+    PM_HIP(launch_hbm_copy(dst_dev, src_dev, bytes, s));  // first touch of the pages, untimed
     Timer t;
     t.start(s);
     for (uint32_t r = 0; r < std::max<uint32_t>(1, repeats); ++r) PM_HIP(launch_hbm_copy(dst_dev, src_dev, bytes, s));

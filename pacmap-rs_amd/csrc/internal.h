@@ -106,6 +106,7 @@ struct GradArgs {
   uint32_t n = 0;
   float w[3] = {0, 0, 0};             // w_nn, w_mn, w_fp
   float lr_t = 0, omb1 = 0, omb2 = 0; // Adam scalars
+  float mean_row_len = 0;             // row_ptr[n] / n (picks the load depth)
 };
 constexpr uint32_t kGradRowsPerBlock = 16;
 uint32_t grad_blocks(uint32_t n);

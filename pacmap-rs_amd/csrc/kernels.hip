@@ -544,7 +544,16 @@ hipError_t launch_scale_nn(const float *dist, const uint32_t *idx, const float *
 // and accumulates s * w1(D) * (y_row - y_other) exactly as the reference
 // computes each per-pair term; the 16 partials are combined by a fixed xor
 // butterfly, so the result is bit-identical from run to run (no atomics).
-template <int DIMS, bool ADAM, bool LOSS>
+// 1/x from v_rcp_f32 refined by one Newton step (x >= 1 here): within an ulp
+// of the correctly rounded quotient, at a quarter of the instructions of an
+// IEEE division.
+__device__ __forceinline__ float recip_nr(float x) {
+  const float r0 = __builtin_amdgcn_rcpf(x);
+  const float e = fmaf(-x, r0, 1.0f);
+  return fmaf(r0, e, r0);
+}
+
+template <int DIMS, bool ADAM, bool LOSS, int U>
 __global__ __launch_bounds__(256) void grad_adam_kernel(const GradArgs a) {
   const int t = threadIdx.x, gl = t & 15, grp = t >> 4;
   const uint32_t row = blockIdx.x * kGradRowsPerBlock + grp;
@@ -575,7 +584,6 @@ __global__ __launch_bounds__(256) void grad_adam_kernel(const GradArgs a) {
     // (entry word, then the other point's y) before any is used, so one
     // round of loads covers rows of up to 16 U entries; entries are consumed
     // in the same order
-    constexpr int U = 8;
     for (uint32_t e0 = beg + gl; e0 < end; e0 += 16 * U) {
       uint32_t wv[U];
       float yo[U][DIMS];
@@ -586,7 +594,8 @@ __global__ __launch_bounds__(256) void grad_adam_kernel(const GradArgs a) {
       }
 #pragma unroll
       for (int uu = 0; uu < U; ++uu) {
-        const uint32_t o = wv[uu] == 0xffffffffu ? row : (wv[uu] & kIdxMask);
+        if (wv[uu] == 0xffffffffu) break;  // slots past the row end: no gather
+        const uint32_t o = wv[uu] & kIdxMask;
 #pragma unroll
         for (int d = 0; d < DIMS; ++d) yo[uu][d] = a.y_in[static_cast<size_t>(o) * DIMS + d];
       }
@@ -608,7 +617,8 @@ __global__ __launch_bounds__(256) void grad_adam_kernel(const GradArgs a) {
         const float c2 = fp ? 2.0f : (ty == 0u ? 20.0f : 20000.0f);
         const float wt = fp ? a.w[2] : (ty == 0u ? a.w[0] : a.w[1]);
         const float cd = c + D;
-        const float w1 = wt * (c2 / (cd * cd));  // gradient.rs:130 / 140
+        const float rc = recip_nr(cd);
+        const float w1 = wt * (c2 * (rc * rc));  // gradient.rs:130 / 140
 #pragma unroll
         for (int d = 0; d < DIMS; ++d) {
           const float tt = w1 * u[d];
@@ -616,7 +626,7 @@ __global__ __launch_bounds__(256) void grad_adam_kernel(const GradArgs a) {
         }
         if (LOSS && (w & kOwnerBit)) {
           const float num = fp ? 1.0f : D;  // gradient.rs:129 / 139
-          loss = loss + wt * (num / cd);
+          loss = loss + wt * (num * rc);
         }
       }
     }
@@ -744,7 +754,8 @@ __global__ __launch_bounds__(256) void grad_tb_kernel(const GradArgs a, const ui
       }
 #pragma unroll
       for (int uu = 0; uu < U; ++uu) {
-        const uint32_t o = wv[uu] == 0xffffffffu ? row : (wv[uu] & kIdxMask);
+        if (wv[uu] == 0xffffffffu) break;  // slots past the row end: no gather
+        const uint32_t o = wv[uu] & kIdxMask;
 #pragma unroll
         for (int d = 0; d < DIMS; ++d) yo[uu][d] = a.y_in[static_cast<size_t>(o) * DIMS + d];
       }
@@ -766,7 +777,8 @@ __global__ __launch_bounds__(256) void grad_tb_kernel(const GradArgs a, const ui
         const float c2 = fp ? 2.0f : (ty == 0u ? 20.0f : 20000.0f);
         const float wt = fp ? a.w[2] : (ty == 0u ? a.w[0] : a.w[1]);
         const float cd = c + D;
-        const float w1 = wt * (c2 / (cd * cd));  // gradient.rs:130 / 140
+        const float rc = recip_nr(cd);
+        const float w1 = wt * (c2 * (rc * rc));  // gradient.rs:130 / 140
 #pragma unroll
         for (int d = 0; d < DIMS; ++d) {
           const float tt = w1 * u[d];
@@ -774,7 +786,7 @@ __global__ __launch_bounds__(256) void grad_tb_kernel(const GradArgs a, const ui
         }
         if (LOSS && (w & kOwnerBit)) {
           const float num = fp ? 1.0f : D;  // gradient.rs:129 / 139
-          loss = loss + wt * (num / cd);
+          loss = loss + wt * (num * rc);
         }
       }
     }
@@ -876,22 +888,32 @@ hipError_t launch_grad_adam_tb(const GradArgs &a, const uint16_t *off, float *pa
 
 uint32_t grad_blocks(uint32_t n) { return (n + kGradRowsPerBlock - 1) / kGradRowsPerBlock; }
 
+template <int DIMS, int U>
+static void launch_grad_u(const GradArgs &a, bool adam, uint32_t blocks, hipStream_t s) {
+  const bool loss = a.loss_part != nullptr;
+  if (adam) {
+    if (loss)
+      grad_adam_kernel<DIMS, true, true, U><<<blocks, 256, 0, s>>>(a);
+    else
+      grad_adam_kernel<DIMS, true, false, U><<<blocks, 256, 0, s>>>(a);
+  } else {
+    if (loss)
+      grad_adam_kernel<DIMS, false, true, U><<<blocks, 256, 0, s>>>(a);
+    else
+      grad_adam_kernel<DIMS, false, false, U><<<blocks, 256, 0, s>>>(a);
+  }
+}
+
 template <int DIMS>
 static hipError_t launch_grad_dims(const GradArgs &a, bool adam, hipStream_t s) {
   const uint32_t blocks = grad_blocks(a.n);
   if (blocks == 0) return hipSuccess;
-  const bool loss = a.loss_part != nullptr;
-  if (adam) {
-    if (loss)
-      grad_adam_kernel<DIMS, true, true><<<blocks, 256, 0, s>>>(a);
-    else
-      grad_adam_kernel<DIMS, true, false><<<blocks, 256, 0, s>>>(a);
-  } else {
-    if (loss)
-      grad_adam_kernel<DIMS, false, true><<<blocks, 256, 0, s>>>(a);
-    else
-      grad_adam_kernel<DIMS, false, false><<<blocks, 256, 0, s>>>(a);
-  }
+  // entries per lane loaded in one round: 4 when the mean row fits in 64
+  // entries, else 8 (fewer idle slots on short rows)
+  if (a.mean_row_len <= 64.0f)
+    launch_grad_u<DIMS, 4>(a, adam, blocks, s);
+  else
+    launch_grad_u<DIMS, 8>(a, adam, blocks, s);
   return hipGetLastError();
 }
 

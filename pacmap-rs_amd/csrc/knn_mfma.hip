@@ -216,8 +216,8 @@ __global__ void knn_rownorm_kernel(const float *__restrict__ x, uint64_t n, uint
                                    uint32_t *__restrict__ nonfinite, float *__restrict__ rinv) {
   const uint64_t row = blockIdx.x * 256ull + threadIdx.x;
   if (row >= rows_padded) return;
-  if (row >= n) {
-    arow[row] = 0.0f;
+  if (row >= n) {  // padding rows: NaN makes every lower bound against them fail
+    arow[row] = __builtin_nanf("");
     if (rinv) rinv[row] = 0.0f;
     return;
   }
@@ -294,7 +294,7 @@ __global__ __launch_bounds__(256) void knn_mfma_kernel(
     for (int r = 0; r < 4; ++r) {
       const uint32_t q = tile * 16 + 4 * rsub + r;
       qidx[rt][r] = q;
-      aq[rt][r] = q < n ? arow[q] : 0.0f;
+      aq[rt][r] = q < n ? arow[q] : __builtin_nanf("");
       iq[rt][r] = (FP8 && q < n) ? rinv[q] : 1.0f;
     }
   }
@@ -407,7 +407,8 @@ __global__ __launch_bounds__(256) void knn_mfma_kernel(
       const uint32_t c = tile * 16 + col;
       const float ac = sA[buf][ct][col];
       const float ic = FP8 ? sI[buf][ct][col] : 1.0f;
-      const bool cvalid = c < n;
+      // padded rows carry a = NaN (see knn_rownorm_kernel and aq), so their
+      // lb is NaN and fails every comparison: only self needs a test here
 #pragma unroll
       for (int rt = 0; rt < RT; ++rt)
 #pragma unroll
@@ -415,7 +416,7 @@ __global__ __launch_bounds__(256) void knn_mfma_kernel(
           const uint32_t q = qidx[rt][r];
           const float dot = FP8 ? acc[rt][r] * (iq[rt][r] * ic) : acc[rt][r];
           const float lb = fmaf(-2.0f, dot, aq[rt][r] + ac);
-          const bool ok = cvalid && q < n && c != q;
+          const bool ok = c != q;
           if (PHASE == 0) {
             if (ok && lb < best1[rt][r]) {
               const bool first = lb < best0[rt][r];

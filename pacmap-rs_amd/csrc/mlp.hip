@@ -416,6 +416,126 @@ __global__ void mlp_pack_kernel(const float *__restrict__ p, uint32_t d, uint32_
 }
 
 // ---------------------------------------------------------------------------
+// Fused transform (hidden width 256): one block per 64 rows keeps the bf16
+// input tile and the first activation in LDS, runs both hidden layers on MFMA
+// (wave w owns hidden units 64w .. 64w + 63, weights read from L2), and folds
+// the FP32 output layer into the second layer's epilogue: per-wave partial
+// outputs (lane butterfly over the 16 columns of a tile) are added over the
+// four waves in order.  No intermediate buffer leaves the CU.
+// ---------------------------------------------------------------------------
+constexpr uint32_t kFusedH = 256;
+
+template <int DIMS>
+__global__ __launch_bounds__(256) void mlp_transform_kernel(const float *__restrict__ x, uint64_t n, uint32_t d,
+                                                            uint32_t Kp, const uint16_t *__restrict__ W1b,
+                                                            const float *__restrict__ b1,
+                                                            const uint16_t *__restrict__ W2b,
+                                                            const float *__restrict__ b2,
+                                                            const float *__restrict__ W3,
+                                                            const float *__restrict__ b3, float *__restrict__ Y) {
+  constexpr uint32_t dims = DIMS;
+  constexpr uint32_t H = kFusedH, LH = H + 8;
+  extern __shared__ __align__(16) unsigned char smem[];
+  const uint32_t LX = Kp + 8;
+  uint16_t *sX = reinterpret_cast<uint16_t *>(smem);  // 64 x LX
+  uint16_t *sH = sX + 64 * LX;                         // 64 x LH
+  float *sY = reinterpret_cast<float *>(sH + 64 * LH); // 4 waves x 64 rows x dims
+  const uint32_t t = threadIdx.x, wave = t >> 6, lane = t & 63;
+  const uint64_t r0 = static_cast<uint64_t>(blockIdx.x) * 64;
+  for (uint32_t e = t; e < 64 * Kp; e += 256) {
+    const uint32_t rr = e / Kp, k = e % Kp;
+    sX[rr * LX + k] = (r0 + rr < n && k < d) ? f2bf(x[(r0 + rr) * d + k]) : 0;
+  }
+  __syncthreads();
+  const uint32_t lr = lane & 15, lk = 8 * (lane >> 4), c0 = wave * 64;
+  floatx4m acc[4][4];
+  auto layer = [&](const uint16_t *sA, uint32_t la, const uint16_t *W, uint32_t K) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) acc[i][j] = floatx4m{0.f, 0.f, 0.f, 0.f};
+    for (uint32_t k0 = 0; k0 < K; k0 += 32) {
+      bf16x8m a[4], b[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+        a[i] = __builtin_bit_cast(bf16x8m, *reinterpret_cast<const uint4 *>(sA + (16 * i + lr) * la + k0 + lk));
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+        b[j] = __builtin_bit_cast(bf16x8m, *reinterpret_cast<const uint4 *>(W + static_cast<size_t>(c0 + 16 * j + lr) * K + k0 + lk));
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+    }
+  };
+  // layer 1 -> sH (bf16)
+  layer(sX, LX, W1b, Kp);
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const uint32_t col = c0 + 16 * j + lr;
+      const float bias = b1[col];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) sH[(16 * i + 4 * (lane >> 4) + r) * LH + col] = f2bf(gelu(acc[i][j][r] + bias));
+    }
+  __syncthreads();
+  // layer 2 with the output layer folded in
+  layer(sH, LH, W2b, H);
+  float py[16][DIMS];
+#pragma unroll
+  for (int q = 0; q < 16; ++q)
+#pragma unroll
+    for (int c = 0; c < DIMS; ++c) py[q][c] = 0.0f;
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const uint32_t col = c0 + 16 * j + lr;
+    const float bias = b2[col];
+    float w3[DIMS];
+#pragma unroll
+    for (int c = 0; c < DIMS; ++c) w3[c] = W3[c * H + col];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const float h = gelu(acc[i][j][r] + bias);
+#pragma unroll
+        for (int c = 0; c < DIMS; ++c) py[4 * i + r][c] = fmaf(h, w3[c], py[4 * i + r][c]);
+      }
+  }
+#pragma unroll
+  for (int q = 0; q < 16; ++q)
+#pragma unroll
+    for (int c = 0; c < DIMS; ++c) {
+      float v = py[q][c];
+      v += __shfl_xor(v, 1, 64);
+      v += __shfl_xor(v, 2, 64);
+      v += __shfl_xor(v, 4, 64);
+      v += __shfl_xor(v, 8, 64);
+      py[q][c] = v;
+    }
+  if (lr == 0) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+#pragma unroll
+        for (int c = 0; c < DIMS; ++c)
+          sY[(wave * 64 + 16 * i + 4 * (lane >> 4) + r) * dims + c] = py[4 * i + r][c];
+  }
+  __syncthreads();
+  for (uint32_t e = t; e < 64 * dims; e += 256) {
+    const uint32_t rr = e / dims, c = e % dims;
+    if (r0 + rr >= n) continue;
+    float v = sY[(0 * 64 + rr) * dims + c];
+    v += sY[(1 * 64 + rr) * dims + c];
+    v += sY[(2 * 64 + rr) * dims + c];
+    v += sY[(3 * 64 + rr) * dims + c];
+    Y[(r0 + rr) * dims + c] = v + b3[c];
+  }
+}
+
+// ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
 hipError_t launch_mlp_gather(const float *x, uint32_t d, uint32_t Kp, const uint32_t *pi, const uint32_t *pj,
@@ -510,6 +630,25 @@ hipError_t launch_mlp_sum_parts(const float *part, uint32_t S, uint32_t rows, ui
   if (total == 0) return hipSuccess;
   mlp_sum_parts_kernel<<<static_cast<uint32_t>((total + 255) / 256), 256, 0, s>>>(part, S, rows, cols, ldp, slab,
                                                                                  out, ldo);
+  return hipGetLastError();
+}
+
+bool mlp_transform_fused_ok(uint32_t H, uint32_t dims) { return H == kFusedH && dims >= 1 && dims <= 8; }
+
+hipError_t launch_mlp_transform(const float *x, uint64_t n, uint32_t d, uint32_t Kp, const uint16_t *W1b,
+                                const float *b1, const uint16_t *W2b, const float *b2, const float *W3,
+                                const float *b3, uint32_t dims, float *Y, hipStream_t s) {
+  if (!mlp_transform_fused_ok(kFusedH, dims) || Kp % 32 || n == 0) return n == 0 ? hipSuccess : hipErrorInvalidValue;
+  const size_t shm = static_cast<size_t>(64) * (Kp + 8) * 2 + static_cast<size_t>(64) * (kFusedH + 8) * 2 +
+                     static_cast<size_t>(4) * 64 * dims * sizeof(float);
+  const uint32_t g = static_cast<uint32_t>((n + 63) / 64);
+  switch (dims) {
+#define PM_TR(D) \
+  case D: mlp_transform_kernel<D><<<g, 256, shm, s>>>(x, n, d, Kp, W1b, b1, W2b, b2, W3, b3, Y); break;
+    PM_TR(1) PM_TR(2) PM_TR(3) PM_TR(4) PM_TR(5) PM_TR(6) PM_TR(7) PM_TR(8)
+#undef PM_TR
+    default: return hipErrorInvalidValue;
+  }
   return hipGetLastError();
 }
 

@@ -295,6 +295,14 @@ static int mlp_transform_impl(const float *x_host, const float *x_dev, uint64_t 
       xd = dxb.get();
     }
     MlpWeights W(sh, params, s);
+    if (mlp_transform_fused_ok(sh.H, sh.dims)) {
+      DevBuf<float> Y(n * dims);
+      PM_HIP(launch_mlp_transform(xd, n, sh.d, sh.Kp, W.w1b.get(), W.p.get() + sh.ob1, W.w2b.get(), W.p.get() + sh.ob2,
+                                  W.p.get() + sh.oW3, W.p.get() + sh.ob3, sh.dims, Y.get(), s));
+      download(y_out, Y.get(), n * dims, s);
+      PM_HIP(hipStreamSynchronize(s));
+      return;
+    }
     const uint32_t Mt = static_cast<uint32_t>(std::min<uint64_t>(kTransformRows, (n + 63) / 64 * 64));
     DevBuf<uint16_t> xb(static_cast<size_t>(Mt) * sh.Kp), h1b(static_cast<size_t>(Mt) * sh.H);
     DevBuf<float> h2f(static_cast<size_t>(Mt) * sh.H), Y(n * dims);

@@ -38,6 +38,11 @@ hipError_t launch_mlp_out_bwd(const float *dY, uint32_t dims, const float *W3, c
 hipError_t launch_mlp_rowsum(const uint16_t *dzT, uint32_t rows, uint32_t M, float *out, hipStream_t s);
 hipError_t launch_mlp_sum_parts(const float *part, uint32_t S, uint32_t rows, uint32_t cols, uint32_t ldp,
                                 uint64_t slab, float *out, uint32_t ldo, hipStream_t s);
+// Fused transform Y = encoder(x) for hidden width 256 (mlp_transform_fused_ok)
+bool mlp_transform_fused_ok(uint32_t H, uint32_t dims);
+hipError_t launch_mlp_transform(const float *x, uint64_t n, uint32_t d, uint32_t Kp, const uint16_t *W1b,
+                                const float *b1, const uint16_t *W2b, const float *b2, const float *W3,
+                                const float *b3, uint32_t dims, float *Y, hipStream_t s);
 // out[o] = sum over z < S of part[z * slab + o], o < count (fixed-order tree)
 hipError_t launch_mlp_tree_sum(const float *part, uint32_t S, uint64_t slab, uint32_t count, float *out,
                                hipStream_t s);

@@ -101,28 +101,6 @@ __device__ __forceinline__ uint32_t tkey_p(float f) {
   return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
 }
 
-__device__ float euclid_ref_dev(const float *__restrict__ a, const float *__restrict__ b, uint32_t d) {
-  float acc[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-  const uint32_t chunks = d / 8;
-  for (uint32_t c = 0; c < chunks; ++c) {
-#pragma unroll
-    for (int l = 0; l < 8; ++l) {
-      const float df = a[8 * c + l] - b[8 * c + l];
-      const float sq = df * df;
-      acc[l] = acc[l] + sq;
-    }
-  }
-  float s = 0.0f;
-#pragma unroll
-  for (int l = 0; l < 8; ++l) s = s + acc[l];
-  for (uint32_t e = chunks * 8; e < d; ++e) {
-    const float df = a[e] - b[e];
-    const float sq = df * df;
-    s = s + sq;
-  }
-  return sqrtf(s);
-}
-
 // Seeds: seeded runs use seed + i (sampling.rs:97, 133); unseeded runs draw
 // one base seed per call and decorrelate points by an odd multiplier.
 __device__ __forceinline__ uint64_t point_seed(uint64_t base, int seeded, uint64_t i) {
@@ -145,21 +123,44 @@ __global__ void sample_fp_kernel(uint32_t n, const uint32_t *__restrict__ pair_n
 
 // sampling.rs:118-142 + 296-313: for each j, six distinct draws avoiding self
 // and the earlier picks, then the second closest by distance (ties keep draw
-// order; unfilled slots are (0.0, 0)).
-__global__ void sample_mn_kernel(const float *__restrict__ x, uint32_t n, uint32_t d, uint32_t n_mn,
-                                 uint64_t seed, int seeded, uint32_t *__restrict__ pair_mn) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+// order; unfilled slots are (0.0, 0)).  Eight lanes per point: every lane
+// runs the point's random stream (so all eight agree on the draws), lane l
+// accumulates the distance partial of features 8c + l, and the partials are
+// added in lane order 0..7, then the tail, exactly as distance.rs:50-75.
+// Every lane writes the chosen index (same value) and reads back its own
+// writes as the reject list of the next draws.
+__global__ __launch_bounds__(256) void sample_mn_kernel(const float *__restrict__ x, uint32_t n, uint32_t d,
+                                                        uint32_t n_mn, uint64_t seed, int seeded,
+                                                        uint32_t *__restrict__ pair_mn) {
+  const uint32_t i = (blockIdx.x * blockDim.x + threadIdx.x) >> 3;
+  const int lane = threadIdx.x & 63, l = lane & 7, g0 = lane & ~7;
+  if (i >= n) return;  // the eight lanes of a point leave together
   DevRng rng = rng_seed_from_u64(point_seed(seed, seeded, i));
   uint32_t *rows = pair_mn + static_cast<size_t>(2) * i * n_mn;
   const float *xi = x + static_cast<size_t>(i) * d;
+  const uint32_t chunks = d / 8;
   for (uint32_t j = 0; j < n_mn; ++j) {
     uint32_t sampled[6];
     const uint32_t got = dev_sample_fp(6, n, rows + 1, j, 2, i, rng, sampled, 1);
     float dist[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     uint32_t idx[6] = {0, 0, 0, 0, 0, 0};
     for (uint32_t q = 0; q < got; ++q) {
-      dist[q] = euclid_ref_dev(xi, x + static_cast<size_t>(sampled[q]) * d, d);
+      const float *xq = x + static_cast<size_t>(sampled[q]) * d;
+      float acc = 0.0f;
+      for (uint32_t c = 0; c < chunks; ++c) {
+        const float df = xi[8 * c + l] - xq[8 * c + l];
+        const float sq = df * df;
+        acc = acc + sq;
+      }
+      float sd = 0.0f;
+#pragma unroll
+      for (int k = 0; k < 8; ++k) sd = sd + __shfl(acc, g0 | k, 64);
+      for (uint32_t e = chunks * 8; e < d; ++e) {
+        const float df = xi[e] - xq[e];
+        const float sq = df * df;
+        sd = sd + sq;
+      }
+      dist[q] = sqrtf(sd);
       idx[q] = sampled[q];
     }
     for (int p = 1; p < 6; ++p) {  // insertion sort by total order: stable
@@ -185,7 +186,8 @@ hipError_t launch_sample_pairs(const float *x, uint32_t n, uint32_t d, const uin
   if (n == 0) return hipSuccess;
   const uint32_t blocks = (n + 63) / 64;
   if (n_fp) sample_fp_kernel<<<blocks, 64, 0, s>>>(n, pair_nn, knn, n_fp, seed, seeded, pair_fp);
-  if (n_mn) sample_mn_kernel<<<blocks, 64, 0, s>>>(x, n, d, n_mn, seed, seeded, pair_mn);
+  if (n_mn) sample_mn_kernel<<<static_cast<uint32_t>((8ull * n + 255) / 256), 256, 0, s>>>(x, n, d, n_mn, seed, seeded,
+                                                                                         pair_mn);
   return hipGetLastError();
 }
 
