@@ -411,22 +411,26 @@ void pca_axes_dev(const float *x_dev, uint64_t n, uint32_t d, uint32_t q, PcaAxe
                     uint32_t ldb, double *C, uint32_t ldc) {
       PM_HIP(launch_gemm64(M, N, K, A, lda, tA, B, ldb, false, C, ldc, s, part.get(), part_cap));
     };
-    // Q <- orthonormal basis of the columns of Z: two CholeskyQR passes
-    // (Q = Z R^-1, then Q <- Q R'^-1); Z is overwritten.
-    auto orth = [&]() {
+    // Q <- orthonormal basis of the columns of Z by CholeskyQR (Q = Z R^-1);
+    // `twice` repeats it on Q (CholeskyQR2) for full orthogonality.  Inside
+    // the iteration one pass suffices: its loss of orthogonality (~u cond^2)
+    // does not accumulate, and the first and last passes are doubled.  Z is
+    // overwritten.
+    auto orth = [&](bool twice) {
       gemm(b, b, d, Z.get(), b, true, Z.get(), b, G.get(), b);
       PM_HIP(launch_chol_inv(G.get(), b, Rinv.get(), status.get(), s));
       gemm(d, b, b, Z.get(), b, false, Rinv.get(), b, Q.get(), b);
+      if (!twice) return;
       gemm(b, b, d, Q.get(), b, true, Q.get(), b, G.get(), b);
       PM_HIP(launch_chol_inv(G.get(), b, Rinv.get(), status.get(), s));
       gemm(d, b, b, Q.get(), b, false, Rinv.get(), b, Z.get(), b);
       PM_HIP(hipMemcpyAsync(Q.get(), Z.get(), Z.bytes(), hipMemcpyDeviceToDevice, s));
     };
     PM_HIP(hipMemcpyAsync(Z.get(), Q.get(), Q.bytes(), hipMemcpyDeviceToDevice, s));
-    orth();
+    orth(true);
     for (uint32_t it = 0; it < kIters; ++it) {
       gemm(d, b, d, d_cov.get(), d, false, Q.get(), b, Z.get(), b);
-      orth();
+      orth(it + 1 == kIters);
     }
     // Rayleigh-Ritz: T = Q^T C Q, eigenvectors U, axes V = Q U
     gemm(d, b, d, d_cov.get(), d, false, Q.get(), b, Z.get(), b);

@@ -468,10 +468,12 @@ __global__ __launch_bounds__(256) void knn_rerank_kernel(
     uint32_t nsplit, const uint32_t *__restrict__ gcnt, const uint32_t *__restrict__ gcand,
     const float *__restrict__ gtau, double cert_scale, uint32_t *__restrict__ oidx,
     float *__restrict__ odist, uint32_t *__restrict__ fb_count, uint32_t *__restrict__ fb_list) {
-  extern __shared__ float sQrow[];  // 4 x Dp
+  // per wave: the query row (Dp floats), then 64 R keys
+  extern __shared__ __align__(16) float sQrow[];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const uint32_t q = blockIdx.x * 4 + wave;
-  float *qv = sQrow + static_cast<size_t>(wave) * Dp;
+  float *qv = sQrow + static_cast<size_t>(wave) * (Dp + 128 * R);
+  uint64_t *keys = reinterpret_cast<uint64_t *>(qv + Dp);
   if (q < n)
     for (uint32_t f = lane; f < Dp; f += 64) qv[f] = xp[static_cast<size_t>(q) * Dp + f];
   __syncthreads();
@@ -489,45 +491,48 @@ __global__ __launch_bounds__(256) void knn_rerank_kernel(
     if (lane == 0) fb_list[atomicAdd(fb_count, 1u)] = q;
     return;
   }
+  // Eight lanes per candidate: lane l sums segment l (features 8c + l) in
+  // order of c, the eight partials are added in lane order, then the tail, as
+  // distance.rs:50-75 does; a group reads one packed row contiguously.
+  const int grp = lane >> 3, l = lane & 7, g0 = lane & ~7;
+  const float *qseg = qv + static_cast<size_t>(l) * Cp;
   const uint32_t segEnd = 8u * Cp;
+  for (uint32_t e = grp; e < cnt; e += 8) {
+    uint32_t sp = 0, off = e;
+    while (off >= gcnt[static_cast<size_t>(q) * nsplit + sp]) off -= gcnt[static_cast<size_t>(q) * nsplit + sp++];
+    const uint32_t c = gcand[(static_cast<size_t>(q) * nsplit + sp) * capg + off];
+    const float *cr = xp + static_cast<size_t>(c) * Dp;
+    const float *cseg = cr + static_cast<size_t>(l) * Cp;
+    float acc = 0.0f;
+    for (uint32_t f = 0; f < Cp; f += 4) {
+      const float4 cv = *reinterpret_cast<const float4 *>(cseg + f);
+      const float4 qq = *reinterpret_cast<const float4 *>(qseg + f);
+      const float cvs[4] = {cv.x, cv.y, cv.z, cv.w}, qvs[4] = {qq.x, qq.y, qq.z, qq.w};
+#pragma unroll
+      for (int e2 = 0; e2 < 4; ++e2) {
+        const float df = qvs[e2] - cvs[e2];
+        const float sq = df * df;
+        acc = acc + sq;
+      }
+    }
+    float sd = 0.0f;
+#pragma unroll
+    for (int w = 0; w < 8; ++w) sd = sd + __shfl(acc, g0 | w, 64);
+    for (uint32_t f = segEnd; f < Dp; ++f) {
+      const float df = qv[f] - cr[f];
+      const float sq = df * df;
+      sd = sd + sq;
+    }
+    if (l == 0) keys[e] = (static_cast<uint64_t>(tkey_m(sqrtf(sd))) << 32) | c;
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
   uint64_t v[R];
 #pragma unroll
   for (int r = 0; r < R; ++r) {
     const uint32_t e = r * 64 + lane;
-    v[r] = ~0ull;
-    if (e < cnt) {
-      uint32_t sp = 0, off = e;
-      while (off >= gcnt[static_cast<size_t>(q) * nsplit + sp]) off -= gcnt[static_cast<size_t>(q) * nsplit + sp++];
-      const uint32_t c = gcand[(static_cast<size_t>(q) * nsplit + sp) * capg + off];
-      const float *cr = xp + static_cast<size_t>(c) * Dp;
-      float s = 0.0f, acc = 0.0f;
-      for (uint32_t f = 0; f < Dp; f += 4) {
-        const float4 cv = *reinterpret_cast<const float4 *>(cr + f);
-        const float4 qq = *reinterpret_cast<const float4 *>(qv + f);
-        const float cvs[4] = {cv.x, cv.y, cv.z, cv.w}, qvs[4] = {qq.x, qq.y, qq.z, qq.w};
-        if (f >= segEnd) {
-#pragma unroll
-          for (int e2 = 0; e2 < 4; ++e2) {
-            const float df = qvs[e2] - cvs[e2];
-            const float sq = df * df;
-            s = s + sq;
-          }
-        } else {
-#pragma unroll
-          for (int e2 = 0; e2 < 4; ++e2) {
-            const float df = qvs[e2] - cvs[e2];
-            const float sq = df * df;
-            acc = acc + sq;
-          }
-          if ((f + 4u) % Cp == 0u) {
-            s = s + acc;
-            acc = 0.0f;
-          }
-        }
-      }
-      const float dd = sqrtf(s);
-      v[r] = (static_cast<uint64_t>(tkey_m(dd)) << 32) | c;
-    }
+    v[r] = e < cnt ? keys[e] : ~0ull;
   }
   wave_bitonic_m<R>(v, lane);
   const float dk = from_tkey_m(static_cast<uint32_t>(wave_pick_m<R>(v, k - 1) >> 32));
@@ -644,7 +649,8 @@ hipError_t launch_knn_rerank(const float *xp, const KnnLayout &L, uint32_t n, ui
                              uint32_t *idx, float *dist, uint32_t *fb_count, uint32_t *fb_list,
                              hipStream_t s) {
   const uint32_t blocks = (n + 3) / 4;
-  const size_t shm = static_cast<size_t>(4) * L.Dp * sizeof(float);
+  const uint32_t R = capg / 64;
+  const size_t shm = static_cast<size_t>(4) * (L.Dp + 128 * R) * sizeof(float);
   if (capg == 512)
     knn_rerank_kernel<8><<<blocks, 256, shm, s>>>(xp, L.Dp, L.Cp, n, k, capg, nsplit, gcnt, gcand,
                                                   gtau, cert_scale, idx, dist, fb_count, fb_list);

@@ -170,12 +170,19 @@ __global__ __launch_bounds__(512) void chol_inv_kernel(const double *__restrict_
           const uint32_t r = rg + 16 * a;
           f[a] = r > j ? rb[r] * inv : 0.0;
         }
+        // Blocks that cannot change are skipped (the conditions are the same
+        // in every thread): rows below j are final; left-half columns below j
+        // are no longer read; right-half columns above j are still zero.
 #pragma unroll
         for (int a = 0; a < 8; ++a) {
+          if (16u * a + 15u < j) continue;
           const uint32_t r = rg + 16 * a;
 #pragma unroll
-          for (int e = 0; e < 8; ++e)
+          for (int e = 0; e < 8; ++e) {
+            if (e < 4 && 32u * e + 31u < j) continue;
+            if (e >= 4 && 32u * (e - 4) > j) continue;
             M[a][e] = (r == j) ? rc[e] : (r > j ? fma(-f[a], rc[e], M[a][e]) : M[a][e]);
+          }
         }
       }
       if (!ok) break;
@@ -210,7 +217,7 @@ __global__ __launch_bounds__(512) void chol_inv_kernel(const double *__restrict_
 // aqq + t apq, 0), and V^T's rows p, q with the rotation.  The per-thread
 // work list (which block or V column each slot handles) is the same every
 // round, so it is decoded once.  Sweeps stop when the off-diagonal square
-// sum is below 1e-28 of the diagonal one.
+// sum is below 1e-24 of the diagonal one (off-diagonal norm 1e-12 relative).
 // Output: the q eigenvectors of largest eigenvalue (ties by index) as columns
 // of Vout (m x q), each with its largest-magnitude entry positive.
 // ---------------------------------------------------------------------------
@@ -306,7 +313,7 @@ __global__ __launch_bounds__(kJacThreads) void jacobi_kernel(const double *__res
       offs += red[0][w];
       dgs += red[1][w];
     }
-    if (offs == 0.0 || offs <= 1e-28 * dgs) break;  // uniform: every thread summed the same values
+    if (offs == 0.0 || offs <= 1e-24 * dgs) break;  // uniform: every thread summed the same values
     for (uint32_t round = 0; round + 1 < M; ++round) {
       if (static_cast<uint32_t>(t) < h) {
         auto pos = [&](uint32_t j) -> uint32_t { return j == 0 ? 0 : 1 + (j - 1 + round) % (M - 1); };
