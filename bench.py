@@ -32,8 +32,8 @@ HBM_PEAK_GBS = 8000.0
 # copied (factor 2.0), WRITE_SIZE all of them (factor 1.0).  Summaries:
 # profiles/pmc_traffic_70000x784.md.  Keys: (workload, kernel).
 MEASURED_TRAFFIC = {
-    ("70k", "grad"): None,
-    ("70k", "knn_mfma_kernel"): None,
+    ("70k", "grad"): 31_290_000,               # FETCH 14,387.8 KiB x 2.0 + WRITE 1,777.4 KiB
+    ("70k", "knn_mfma_kernel"): 1_333_170_000,  # FETCH 447,653 KiB x 2.0 + WRITE 406,613 KiB
 }
 FP32_PEAK_TFLOPS = 157.3
 BF16_PEAK_TFLOPS = 2500.0

@@ -226,6 +226,10 @@ int pacmap_dev_alloc(uint64_t bytes, void **ptr_out, pacmap_status *st);
 int pacmap_dev_free(void *ptr);
 int pacmap_dev_upload(void *dst_dev, const void *src_host, uint64_t bytes, pacmap_status *st);
 int pacmap_dev_download(void *dst_host, const void *src_dev, uint64_t bytes, pacmap_status *st);
+/* Device blocks the library allocated are cached per calling thread and
+ * reused by its later calls (no hipMalloc / hipFree per call).  This frees
+ * the calling thread's cache. */
+void pacmap_release_cached_memory(void);
 /* HBM reference: a plain streaming device-to-device copy kernel run `repeats`
  * times; ms_out gets the average time per copy (HIP events).  Its rocprof
  * FETCH_SIZE / WRITE_SIZE readings calibrate the traffic counters. */

@@ -41,6 +41,17 @@ struct PmError : std::runtime_error {
     if (pm_e_ != hipSuccess) ::pm::fail_gpu(pm_e_, #expr, __FILE__, __LINE__); \
   } while (0)
 
+// Per-thread cache of device blocks (pool.cpp).  A released block is kept
+// under its size class and handed out again by a later allocation of the
+// same thread instead of a hipMalloc / hipFree pair (hipFree synchronises the
+// device).  Reuse is safe because all work of an entry point is ordered on
+// its one stream and every entry point synchronises that stream before it
+// returns; blocks never move between threads.
+size_t dev_size_class(size_t bytes);
+void *dev_acquire(size_t cls);        // cls from dev_size_class
+void dev_recycle(void *p, size_t cls);
+void dev_trim();                      // frees this thread's cached blocks
+
 template <class T>
 class DevBuf {
  public:
@@ -52,10 +63,13 @@ class DevBuf {
   void alloc(size_t count) {
     release();
     n_ = count;
-    if (count) PM_HIP(hipMalloc(reinterpret_cast<void **>(&p_), count * sizeof(T)));
+    if (count) {
+      cls_ = dev_size_class(count * sizeof(T));
+      p_ = static_cast<T *>(dev_acquire(cls_));
+    }
   }
   void release() {
-    if (p_) (void)hipFree(p_);
+    if (p_) dev_recycle(p_, cls_);
     p_ = nullptr;
     n_ = 0;
   }
@@ -66,6 +80,7 @@ class DevBuf {
  private:
   T *p_ = nullptr;
   size_t n_ = 0;
+  size_t cls_ = 0;
 };
 
 class Stream {

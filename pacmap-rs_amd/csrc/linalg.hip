@@ -170,19 +170,12 @@ __global__ __launch_bounds__(512) void chol_inv_kernel(const double *__restrict_
           const uint32_t r = rg + 16 * a;
           f[a] = r > j ? rb[r] * inv : 0.0;
         }
-        // Blocks that cannot change are skipped (the conditions are the same
-        // in every thread): rows below j are final; left-half columns below j
-        // are no longer read; right-half columns above j are still zero.
 #pragma unroll
         for (int a = 0; a < 8; ++a) {
-          if (16u * a + 15u < j) continue;
           const uint32_t r = rg + 16 * a;
 #pragma unroll
-          for (int e = 0; e < 8; ++e) {
-            if (e < 4 && 32u * e + 31u < j) continue;
-            if (e >= 4 && 32u * (e - 4) > j) continue;
+          for (int e = 0; e < 8; ++e)
             M[a][e] = (r == j) ? rc[e] : (r > j ? fma(-f[a], rc[e], M[a][e]) : M[a][e]);
-          }
         }
       }
       if (!ok) break;

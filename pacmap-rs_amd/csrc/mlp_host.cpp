@@ -102,6 +102,9 @@ void batch_grad(const MlpShape &sh, MlpWork &w, const MlpWeights &W, const float
   PM_HIP(launch_mlp_out(w.h2f.get(), H, W.p.get() + sh.oW3, W.p.get() + sh.ob3, sh.dims, M, w.Y.get(), s));
   PM_HIP(launch_mlp_pair_loss(w.Y.get(), sh.dims, w.B, w.pidx.get(), pi, pj, nn_end, mn_end, wts,
                               1.0f / static_cast<float>(count), w.dY.get(), loss_part, s));
+  if (M > 2 * w.B)  // padding rows (M rounded up to 64) carry no gradient
+    PM_HIP(launch_fill_zero(w.dY.get() + static_cast<size_t>(2) * w.B * sh.dims,
+                            static_cast<uint64_t>(M - 2 * w.B) * sh.dims * sizeof(float), s));
   PM_HIP(launch_mlp_out_bwd(w.dY.get(), sh.dims, W.p.get() + sh.oW3, w.h2f.get(), w.z2.get(), H, M, w.dz2b.get(),
                             w.dz2T.get(), w.w3p.get(), w.b3p.get(), s));
   PM_HIP(launch_mlp_dh(w.dz2b.get(), H, W.w2tb.get(), H, M, H, H, w.z1.get(), w.dz1T.get(), s));

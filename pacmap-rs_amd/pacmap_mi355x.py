@@ -144,6 +144,8 @@ def lib():
         L.pacmap_dev_free.argtypes = [_P]
         L.pacmap_dev_upload.argtypes = [_P, _P, _u64, C.POINTER(Status)]
         L.pacmap_dev_download.argtypes = [_P, _P, _u64, C.POINTER(Status)]
+        L.pacmap_release_cached_memory.argtypes = []
+        L.pacmap_release_cached_memory.restype = None
         L.pacmap_dev_copy.argtypes = [_P, _P, _u64, C.c_uint32, C.POINTER(C.c_double), C.POINTER(Status)]
         L.pacmap_last_timings.argtypes = [C.POINTER(Timings)]
         L.pacmap_last_timings.restype = None
@@ -596,6 +598,11 @@ class mlp:  # noqa: N801 -- namespace of the parametric encoder entry points
                                      a.ctypes.data, na, b.ctypes.data, nb, c.ctypes.data, nc,
                                      w.ctypes.data, g.ctypes.data, loss.ctypes.data, C.byref(st)), st)
         return g, float(loss[0])
+
+
+def release_cached_memory() -> None:
+    """Free the device blocks the library cached for the calling thread."""
+    lib().pacmap_release_cached_memory()
 
 
 def hbm_copy(nbytes: int, repeats: int = 10) -> float:
