@@ -364,13 +364,9 @@ __global__ __launch_bounds__(256) void knn_mfma_kernel(
   for (uint32_t st = 0; st < nstage; ++st) {
     const int buf = st & 1;
     if (st + 1 < nstage) fetch(st + 1);  // in flight while this stage computes
-#pragma unroll 1
-    for (int ct = 0; ct < CT; ++ct) {
-      const uint32_t slot = slot_lo + st * CT + ct;
-      if (slot >= slot_hi) break;
-      const uint32_t tile = slot_tile(slot);
-      if (tile >= ntiles) break;
-      floatx4 acc[RT];
+    // products of candidate tile ct into acc (3 bf16 MFMAs per chunk when
+    // split, one FP8 or bf16 MFMA otherwise)
+    auto products = [&](int ct, floatx4 (&acc)[RT]) {
 #pragma unroll
       for (int rt = 0; rt < RT; ++rt) acc[rt] = floatx4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
@@ -404,6 +400,8 @@ __global__ __launch_bounds__(256) void knn_mfma_kernel(
           }
         }
       }
+    };
+    auto epilogue = [&](int ct, uint32_t tile, const floatx4 (&acc)[RT]) {
       const uint32_t c = tile * 16 + col;
       const float ac = sA[buf][ct][col];
       const float ic = FP8 ? sI[buf][ct][col] : 1.0f;
@@ -428,6 +426,27 @@ __global__ __launch_bounds__(256) void knn_mfma_kernel(
             if (pos < capg) gcand[(static_cast<size_t>(q) * nsplit + split) * capg + pos] = c;
           }
         }
+    };
+    // two candidate tiles at a time: their independent MFMA chains interleave
+    // (slots past the range hold zero fragments; their epilogue is skipped)
+#pragma unroll 1
+    for (int ct = 0; ct < CT; ct += 2) {
+      const uint32_t slot0 = slot_lo + st * CT + ct;
+      const uint32_t tile0 = slot_tile(slot0);
+      if (slot0 >= slot_hi || tile0 >= ntiles) break;
+      floatx4 acc0[RT];
+      if (ct + 1 < CT) {
+        const uint32_t slot1 = slot0 + 1;
+        const uint32_t tile1 = slot_tile(slot1);
+        floatx4 acc1[RT];
+        products(ct, acc0);
+        products(ct + 1, acc1);
+        epilogue(ct, tile0, acc0);
+        if (slot1 < slot_hi && tile1 < ntiles) epilogue(ct + 1, tile1, acc1);
+      } else {
+        products(ct, acc0);
+        epilogue(ct, tile0, acc0);
+      }
     }
     if (st + 1 < nstage) stash(buf ^ 1);
     __syncthreads();
