@@ -448,9 +448,15 @@ def main():
     it_total = sum(ITERS)
     us_iter = opt_ms * 1e3 / it_total
     tb = int(tms[-1]["grad_target_blocks"])
-    gname = ("grad_tb_kernel + grad_tb_combine_kernel (target-blocked gradient, then Adam)" if tb else
-             "grad_adam_kernel (fused CSR gradient + Adam)")
-    traffic = MEASURED_TRAFFIC.get((args.workload, "grad" if not tb else "grad_tb"))
+    sell = int(tms[-1]["grad_sell"])
+    if sell:
+        gname, gkey = ("grad_sell_kernel + grad_tb_combine_kernel (LDS-staged sliced-ELL gradient, "
+                       "then Adam)"), "grad_sell"
+    elif tb:
+        gname, gkey = "grad_tb_kernel + grad_tb_combine_kernel (target-blocked gradient, then Adam)", "grad_tb"
+    else:
+        gname, gkey = "grad_adam_kernel (fused CSR gradient + Adam)", "grad"
+    traffic = MEASURED_TRAFFIC.get((args.workload, gkey))
     grad_roof = {"kernel": gname, "bound": "hbm", "achieved": round(b_iter / (us_iter * 1e-6) / 1e9, 1),
                  "peak": HBM_PEAK_GBS, "unit": "GB/s",
                  "frac": round(b_iter / (us_iter * 1e-6) / 1e9 / HBM_PEAK_GBS, 4),
@@ -484,7 +490,8 @@ def main():
                       "GB_per_s": round(b_iter * it_total / (opt_ms * 1e-3) / 1e9, 1),
                       "frac_hbm_peak": round(b_iter * it_total / (opt_ms * 1e-3) / 1e9 /
                                              HBM_PEAK_GBS, 4),
-                      "target_blocks": tb},
+                      "target_blocks": tb, "sliced_ell": sell,
+                      "sliced_ell_words": int(tms[-1]["grad_sell_entries"])},
         "breakdown_ms": breakdown,
         "knn": {"path": ["direct", "split-bf16 MFMA + certified rerank",
                          "bf16 MFMA + certified rerank", "fp8 MFMA, uncertified"][path],

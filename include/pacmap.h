@@ -103,7 +103,7 @@ typedef struct {
   uint64_t approx_threshold;          /* default 8000 */
   /* MI355X extensions */
   int32_t knn_precision;              /* PACMAP_KNN_*, default FP32 */
-  int32_t record_loss;                /* 1: keep the per-iteration loss (default 1) */
+  int32_t record_loss;                /* 1: fill loss_out when it is non-NULL (default 1) */
 } pacmap_config;
 
 /* Fills `cfg` with Configuration::default() (lib.rs:208-224). */
@@ -111,7 +111,8 @@ void pacmap_config_default(pacmap_config *cfg);
 
 /* fit_transform (lib.rs:294-367).  y_out: n x dims.  snapshots_out: when the
  * config has snapshots, num_snapshots x n x dims (else may be NULL).
- * loss_out (may be NULL): total-iteration float array of losses. */
+ * loss_out (may be NULL): total-iteration float array of losses.  With NULL (or
+ * record_loss 0) the optimiser skips the loss terms; the embedding is the same. */
 int pacmap_fit_transform(const float *x, uint64_t n, uint64_t d, const pacmap_config *cfg,
                          float *y_out, float *snapshots_out, float *loss_out,
                          pacmap_status *st);
@@ -262,6 +263,8 @@ typedef struct {
   double knn_sample_ms;        /* MFMA threshold-sampling kernel */
   uint64_t knn_certified;      /* 1: exact neighbours (certificate + rescan), 0: approximate */
   uint64_t grad_target_blocks; /* 0: one-pass gradient; 8: target-blocked two-pass gradient */
+  uint64_t grad_sell;          /* 1: LDS-staged target-blocked gradient (sliced ELL) */
+  uint64_t grad_sell_entries;  /* sliced-ELL words including padding */
 } pacmap_timings;
 void pacmap_last_timings(pacmap_timings *out);
 

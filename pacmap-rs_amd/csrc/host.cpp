@@ -263,6 +263,18 @@ bool use_target_blocks(uint64_t n, uint64_t dims) {
   return n * dims * 4 >= (4ull << 20);
 }
 
+// The LDS-staged (sliced ELL) gradient, for embeddings small enough that a
+// target range of Y plus a row chunk fits in LDS and large enough to fill
+// the GPU.  PACMAP_GRAD_SELL=0 / 1 forces it off / on (when it fits).
+bool use_sell(uint64_t n, uint64_t dims, SellPlan &plan) {
+  if (n >= kMaxPoints || !sell_plan(static_cast<uint32_t>(n), static_cast<uint32_t>(dims), plan)) return false;
+  if (const char *e = std::getenv("PACMAP_GRAD_SELL")) {
+    if (e[0] == '0') return false;
+    if (e[0] == '1') return true;
+  }
+  return n >= 16384;
+}
+
 void optimize_device(const DevCsr &csr, uint64_t n, uint64_t dims, std::vector<float> &y, float lr,
                      const uint64_t iters[3], const uint64_t *snap_idx, uint64_t ns,
                      float *snaps_out, float *loss_out, bool record_loss, hipStream_t s) {
@@ -289,12 +301,15 @@ void optimize_device(const DevCsr &csr, uint64_t n, uint64_t dims, std::vector<f
   PM_HIP(launch_fill_zero(m.get(), m.bytes(), s));
   PM_HIP(launch_fill_zero(v.get(), v.bytes(), s));
   DevBuf<float> d_snap(ns && snaps_out ? ns * nd : 0);
-  // Target-blocked gradient when Y outgrows an XCD's L2 (see kernels.hip)
+  // Target-blocked gradient when Y outgrows an XCD's L2, its LDS-staged
+  // form when a target range fits in LDS (see kernels.hip)
+  SellPlan sp;
   bool tb = use_target_blocks(n, dims) && T > 0 && csr.nnz > 0;
-  DevBuf<uint32_t> tb_ent;
+  bool sell = !tb && T > 0 && csr.nnz > 0 && use_sell(n, dims, sp);
+  DevBuf<uint32_t> tb_ent, sell_perm, sell_off, sell_ent;
   DevBuf<uint16_t> tb_off;
   DevBuf<float> tb_part;
-  if (tb) {
+  if (tb || sell) {
     tb_ent.alloc(csr.nnz);
     tb_off.alloc(static_cast<size_t>(kTargetBlocks) * n);
     DevBuf<uint32_t> bad(1);
@@ -304,11 +319,33 @@ void optimize_device(const DevCsr &csr, uint64_t n, uint64_t dims, std::vector<f
     uint32_t hb = 0;
     download(&hb, bad.get(), 1, s);
     PM_HIP(hipStreamSynchronize(s));
-    if (hb) tb = false;  // a row too long for the u16 range offsets: plain path
-    else tb_part.alloc(static_cast<size_t>(kTargetBlocks) * nd);
+    if (hb) {
+      tb = sell = false;  // a row too long for the u16 range offsets: plain path
+    } else {
+      tb_part.alloc(static_cast<size_t>(kTargetBlocks) * nd);
+    }
+  }
+  if (sell) {
+    const uint32_t n32 = static_cast<uint32_t>(n);
+    const uint64_t ns_ = sell_slices(sp);
+    sell_perm.alloc(static_cast<size_t>(kTargetBlocks) * n);
+    sell_off.alloc(ns_ + 1);
+    DevBuf<uint32_t> ssz(ns_), cur(ns_), scr((ns_ + 1023) / 1024 + 1);
+    PM_HIP(launch_sell_sort(csr.row_ptr.get(), tb_off.get(), n32, sp, sell_perm.get(), ssz.get(), s));
+    PM_HIP(launch_csr_scan(ssz.get(), static_cast<uint32_t>(ns_), sell_off.get(), cur.get(), scr.get(), s));
+    uint32_t total = 0;
+    download(&total, sell_off.get() + ns_, 1, s);
+    PM_HIP(hipStreamSynchronize(s));
+    sell_ent.alloc(total ? total : 1);
+    sp.words = total ? total : 1;
+    PM_HIP(launch_sell_fill(csr.row_ptr.get(), tb_ent.get(), tb_off.get(), n32, sp, sell_perm.get(),
+                            sell_off.get(), sell_ent.get(), s));
+    g_tm.grad_sell_entries = total;
   }
   g_tm.grad_target_blocks = tb ? kTargetBlocks : 0;
-  const uint32_t nb = tb ? grad_tb_blocks(static_cast<uint32_t>(n)) : grad_blocks(static_cast<uint32_t>(n));
+  g_tm.grad_sell = sell ? 1 : 0;
+  const uint32_t nb = sell ? sell_blocks(sp)
+                           : (tb ? grad_tb_blocks(static_cast<uint32_t>(n)) : grad_blocks(static_cast<uint32_t>(n)));
   DevBuf<float> part(record_loss ? T * nb : 0), d_loss(record_loss ? T : 0);
   PM_HIP(hipStreamSynchronize(s));
   g_tm.iterations = T;
@@ -335,7 +372,10 @@ void optimize_device(const DevCsr &csr, uint64_t n, uint64_t dims, std::vector<f
     a.omb2 = 1.0f - beta2;
     a.loss_part = record_loss ? part.get() + it * nb : nullptr;
     a.snap = (writer[it] >= 0 && d_snap.get()) ? d_snap.get() + writer[it] * nd : nullptr;
-    if (tb) {
+    if (sell) {
+      cap_err = launch_grad_adam_sell(a, sp, sell_perm.get(), sell_off.get(), sell_ent.get(), tb_part.get(),
+                                      static_cast<uint32_t>(dims), true, s);
+    } else if (tb) {
       a.ent = tb_ent.get();
       cap_err = launch_grad_adam_tb(a, tb_off.get(), tb_part.get(), static_cast<uint32_t>(dims), true, s);
     } else {
@@ -706,7 +746,7 @@ void fit_impl(const float *x_host, const float *x_dev_in, uint64_t n, uint64_t d
   optimize_device(csr, n, dims, y0, cfg->learning_rate, cfg->num_iters,
                   cfg->has_snapshots ? cfg->snapshots : nullptr,
                   cfg->has_snapshots ? cfg->num_snapshots : 0, snaps_out, loss_out,
-                  cfg->record_loss != 0, s);
+                  cfg->record_loss != 0 && loss_out != nullptr, s);
   std::memcpy(y_out, y0.data(), sizeof(float) * n * dims);
   g_tm.total_ms = now_ms() - t0;
 }

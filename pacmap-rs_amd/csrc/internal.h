@@ -123,6 +123,30 @@ hipError_t launch_tb_partition(const uint32_t *row_ptr, const uint32_t *ent, uin
                                uint16_t *off, uint32_t *bad, hipStream_t s);
 hipError_t launch_grad_adam_tb(const GradArgs &a, const uint16_t *off, float *part, uint32_t dims, bool adam,
                                hipStream_t s);
+// LDS-staged variant of the target-blocked gradient (sliced ELL, see
+// kernels.hip), used while a range of Y plus a row chunk fits in LDS.  Built
+// once per fit from the tb_partition output: sort (perm: kTargetBlocks x n,
+// ssz: sell_slices() slice sizes), exclusive scan of ssz into soff (+1
+// total), fill (sent: soff[last] words).  Each iteration runs grad_sell
+// (loss_part: sell_blocks() floats) and the same combine+Adam pass.
+struct SellPlan {
+  uint32_t span = 0;       // points per target range (tb ranges)
+  uint32_t rc = 0;         // rows per chunk (multiple of 16)
+  uint32_t chunks = 0;
+  uint32_t smax = 0;       // slices per chunk
+  uint32_t lds_bytes = 0;  // LDS per workgroup
+  uint32_t words = 0;      // slice words (set once the layout is built)
+};
+bool sell_plan(uint32_t n, uint32_t dims, SellPlan &p);
+uint64_t sell_slices(const SellPlan &p);
+uint32_t sell_blocks(const SellPlan &p);
+hipError_t launch_sell_sort(const uint32_t *row_ptr, const uint16_t *off, uint32_t n, const SellPlan &p,
+                            uint32_t *perm, uint32_t *ssz, hipStream_t s);
+hipError_t launch_sell_fill(const uint32_t *row_ptr, const uint32_t *tb_ent, const uint16_t *off, uint32_t n,
+                            const SellPlan &p, const uint32_t *perm, const uint32_t *soff, uint32_t *sent,
+                            hipStream_t s);
+hipError_t launch_grad_adam_sell(const GradArgs &a, const SellPlan &p, const uint32_t *perm, const uint32_t *soff,
+                                 const uint32_t *sent, float *part, uint32_t dims, bool adam, hipStream_t s);
 // loss[t] = sum of part[t * nb .. t * nb + nb) in a fixed order.
 hipError_t launch_loss_reduce(const float *part, uint32_t nb, uint32_t iters, float *loss,
                               hipStream_t s);

@@ -553,10 +553,63 @@ __device__ __forceinline__ float recip_nr(float x) {
   return fmaf(r0, e, r0);
 }
 
+// Per-type constants of the pair terms (gradient.rs:127-147), one set per
+// iteration: k = w * numerator constant with the far-pair sign folded in (so
+// every entry adds k rc^2 u), wt = w for the loss.  Kept as separate scalars:
+// selecting between members of an aggregate can turn into an indexed memory
+// load per entry.
+struct PairConsts {
+  float k_nn, k_mn, k_fp;
+  float w_nn, w_mn, w_fp;
+};
+__device__ __forceinline__ PairConsts pair_consts(const float (&w)[3]) {
+  PairConsts q;
+  q.k_nn = w[0] * 20.0f;
+  q.k_mn = w[1] * 20000.0f;
+  q.k_fp = -(w[2] * 2.0f);
+  q.w_nn = w[0];
+  q.w_mn = w[1];
+  q.w_fp = w[2];
+  return q;
+}
+
+// One CSR entry's contribution (gradient.rs:118-148): w = entry word, yr =
+// the row's point, yo = the other point.  Every gradient kernel below calls
+// this, so all of them round each term identically.  D = 1 + |yr - yo|^2 is
+// summed in the reference's order; the coefficient is k / (c + D)^2 with the
+// reciprocal from recip_nr.
+template <int DIMS, bool LOSS>
+__device__ __forceinline__ void pair_term(uint32_t w, const float (&yr)[DIMS], const float *yo, float k_nn,
+                                          float k_mn, float k_fp, float w_nn, float w_mn, float w_fp,
+                                          float (&g)[DIMS], float &loss) {
+  float u[DIMS];
+  float D = 1.0f;  // gradient.rs:121-125
+#pragma unroll
+  for (int d = 0; d < DIMS; ++d) {
+    u[d] = yr[d] - yo[d];
+    const float sq = u[d] * u[d];
+    D = D + sq;
+  }
+  const bool nn = w < (1u << 30), fp = w >= (2u << 30);
+  const float c = nn ? 10.0f : (fp ? 1.0f : 10000.0f);
+  const float k = nn ? k_nn : (fp ? k_fp : k_mn);
+  const float rc = recip_nr(c + D);
+  const float w1 = k * (rc * rc);  // gradient.rs:130 / 140 (sign folded into k)
+#pragma unroll
+  for (int d = 0; d < DIMS; ++d) g[d] = g[d] + w1 * u[d];
+  if (LOSS && (w & kOwnerBit)) {
+    const float wt = nn ? w_nn : (fp ? w_fp : w_mn);
+    const float num = fp ? 1.0f : D;  // gradient.rs:129 / 139
+    loss = loss + wt * (num * rc);
+  }
+}
+
 template <int DIMS, bool ADAM, bool LOSS, int U>
 __global__ __launch_bounds__(256) void grad_adam_kernel(const GradArgs a) {
   const int t = threadIdx.x, gl = t & 15, grp = t >> 4;
   const uint32_t row = blockIdx.x * kGradRowsPerBlock + grp;
+  const PairConsts pc = pair_consts(a.w);
+  const float kn = pc.k_nn, km = pc.k_mn, kf = pc.k_fp, wn = pc.w_nn, wm = pc.w_mn, wf = pc.w_fp;
   float g[DIMS];
   float yr[DIMS], mr[DIMS], vr[DIMS];
 #pragma unroll
@@ -603,31 +656,7 @@ __global__ __launch_bounds__(256) void grad_adam_kernel(const GradArgs a) {
       for (int uu = 0; uu < U; ++uu) {
         const uint32_t w = wv[uu];
         if (w == 0xffffffffu) break;
-        const uint32_t ty = w >> 30;
-        float u[DIMS];
-        float D = 1.0f;  // gradient.rs:121-125
-#pragma unroll
-        for (int d = 0; d < DIMS; ++d) {
-          u[d] = yr[d] - yo[uu][d];
-          const float sq = u[d] * u[d];
-          D = D + sq;
-        }
-        const bool fp = ty == 2u;
-        const float c = fp ? 1.0f : (ty == 0u ? 10.0f : 10000.0f);
-        const float c2 = fp ? 2.0f : (ty == 0u ? 20.0f : 20000.0f);
-        const float wt = fp ? a.w[2] : (ty == 0u ? a.w[0] : a.w[1]);
-        const float cd = c + D;
-        const float rc = recip_nr(cd);
-        const float w1 = wt * (c2 * (rc * rc));  // gradient.rs:130 / 140
-#pragma unroll
-        for (int d = 0; d < DIMS; ++d) {
-          const float tt = w1 * u[d];
-          g[d] = fp ? g[d] - tt : g[d] + tt;
-        }
-        if (LOSS && (w & kOwnerBit)) {
-          const float num = fp ? 1.0f : D;  // gradient.rs:129 / 139
-          loss = loss + wt * (num * rc);
-        }
+        pair_term<DIMS, LOSS>(w, yr, yo[uu], kn, km, kf, wn, wm, wf, g, loss);
       }
     }
   }
@@ -730,6 +759,8 @@ __global__ __launch_bounds__(256) void grad_tb_kernel(const GradArgs a, const ui
   const uint32_t b = blockIdx.x % kTargetBlocks, grp = blockIdx.x / kTargetBlocks;
   const int t = threadIdx.x, gl = t & (kTbLanes - 1);
   const uint32_t row = grp * kTbRows + (t / kTbLanes);
+  const PairConsts pc = pair_consts(a.w);
+  const float kn = pc.k_nn, km = pc.k_mn, kf = pc.k_fp, wn = pc.w_nn, wm = pc.w_mn, wf = pc.w_fp;
   float g[DIMS], yr[DIMS];
 #pragma unroll
   for (int d = 0; d < DIMS; ++d) {
@@ -763,31 +794,7 @@ __global__ __launch_bounds__(256) void grad_tb_kernel(const GradArgs a, const ui
       for (int uu = 0; uu < U; ++uu) {
         const uint32_t w = wv[uu];
         if (w == 0xffffffffu) break;
-        const uint32_t ty = w >> 30;
-        float u[DIMS];
-        float D = 1.0f;  // gradient.rs:121-125
-#pragma unroll
-        for (int d = 0; d < DIMS; ++d) {
-          u[d] = yr[d] - yo[uu][d];
-          const float sq = u[d] * u[d];
-          D = D + sq;
-        }
-        const bool fp = ty == 2u;
-        const float c = fp ? 1.0f : (ty == 0u ? 10.0f : 10000.0f);
-        const float c2 = fp ? 2.0f : (ty == 0u ? 20.0f : 20000.0f);
-        const float wt = fp ? a.w[2] : (ty == 0u ? a.w[0] : a.w[1]);
-        const float cd = c + D;
-        const float rc = recip_nr(cd);
-        const float w1 = wt * (c2 * (rc * rc));  // gradient.rs:130 / 140
-#pragma unroll
-        for (int d = 0; d < DIMS; ++d) {
-          const float tt = w1 * u[d];
-          g[d] = fp ? g[d] - tt : g[d] + tt;
-        }
-        if (LOSS && (w & kOwnerBit)) {
-          const float num = fp ? 1.0f : D;  // gradient.rs:129 / 139
-          loss = loss + wt * (num * rc);
-        }
+        pair_term<DIMS, LOSS>(w, yr, yo[uu], kn, km, kf, wn, wm, wf, g, loss);
       }
     }
   }
@@ -813,10 +820,10 @@ __global__ __launch_bounds__(256) void grad_tb_kernel(const GradArgs a, const ui
   }
 }
 
+// Sum of a row's kTargetBlocks range partials in range order, then Adam
+// (adam.rs:51-59) or the plain gradient.
 template <int DIMS, bool ADAM>
-__global__ __launch_bounds__(256) void grad_tb_combine_kernel(const GradArgs a, const float *__restrict__ part) {
-  const uint32_t row = blockIdx.x * 256 + threadIdx.x;
-  if (row >= a.n) return;
+__device__ __forceinline__ void combine_row(const GradArgs &a, const float *__restrict__ part, uint32_t row) {
   const size_t base = static_cast<size_t>(row) * DIMS;
   float g[DIMS];
 #pragma unroll
@@ -843,6 +850,12 @@ __global__ __launch_bounds__(256) void grad_tb_combine_kernel(const GradArgs a, 
 #pragma unroll
     for (int d = 0; d < DIMS; ++d) a.grad_out[base + d] = g[d];
   }
+}
+
+template <int DIMS, bool ADAM>
+__global__ __launch_bounds__(256) void grad_tb_combine_kernel(const GradArgs a, const float *__restrict__ part) {
+  const uint32_t row = blockIdx.x * 256 + threadIdx.x;
+  if (row < a.n) combine_row<DIMS, ADAM>(a, part, row);
 }
 
 uint32_t tb_span(uint32_t n) { return (n + kTargetBlocks - 1) / kTargetBlocks; }
@@ -882,6 +895,314 @@ hipError_t launch_grad_adam_tb(const GradArgs &a, const uint16_t *off, float *pa
     case 6: return launch_tb_dims<6>(a, off, part, adam, s);
     case 7: return launch_tb_dims<7>(a, off, part, adam, s);
     case 8: return launch_tb_dims<8>(a, off, part, adam, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// LDS-staged target-blocked gradient (sliced ELL).  Same ranges, per-lane
+// entry order, butterfly and combine as grad_tb_kernel, so the result is
+// bit-identical to it; what changes is where the other point comes from and
+// how lanes are fed:
+//  * a workgroup (range b = blockIdx.x % 8, row chunk c = blockIdx.x / 8)
+//    first copies range b of Y and its chunk's own rows of Y into LDS, so
+//    every gather is an LDS read instead of an L1/L2 request;
+//  * the chunk's rows are sorted by their range-b entry count (descending,
+//    ties by row) and cut into slices of 16 rows x 4 lanes; a slice stores its
+//    entries step-major (step k holds entries 4k + lane of its 16 rows, 64
+//    words, padded with kSellPad to the slice's longest row), so each step is
+//    one coalesced 256-byte load and lanes of a wave do nearly equal work.
+// Slice entries carry the target as an index relative to the range start.
+// ---------------------------------------------------------------------------
+constexpr uint32_t kSellRows = 16;         // rows per slice (4 lanes each)
+constexpr uint32_t kSellMaxChunk = 2048;   // rows per chunk (LDS bitonic sort)
+constexpr uint32_t kSellThreads = 1024;
+constexpr uint32_t kSellPad = 0xffffffffu;
+constexpr uint32_t kLdsPerCu = 163840;
+
+__device__ __forceinline__ uint32_t tb_seg_beg(const uint16_t *__restrict__ off, uint32_t n, uint32_t b,
+                                               uint32_t row) {
+  return b == 0 ? 0u : off[static_cast<size_t>(b) * n + row];
+}
+__device__ __forceinline__ uint32_t tb_seg_end(const uint32_t *__restrict__ row_ptr,
+                                               const uint16_t *__restrict__ off, uint32_t n, uint32_t b,
+                                               uint32_t row) {
+  return b + 1 == kTargetBlocks ? row_ptr[row + 1] - row_ptr[row] : off[static_cast<size_t>(b + 1) * n + row];
+}
+
+bool sell_plan(uint32_t n, uint32_t dims, SellPlan &p) {
+  p = SellPlan{};
+  if (n < kTargetBlocks || dims == 0 || dims > 8) return false;
+  p.span = tb_span(n);
+  // two workgroups per CU when both fit in LDS, else one
+  for (uint32_t per_cu = 2; per_cu >= 1; --per_cu) {
+    const uint32_t want = 256u * per_cu / kTargetBlocks;  // chunks
+    uint32_t rc = (n + want - 1) / want;
+    rc = (rc + kSellRows - 1) / kSellRows * kSellRows;
+    if (rc > kSellMaxChunk) continue;
+    const uint64_t lds = (static_cast<uint64_t>(p.span) + rc) * dims * 4u;
+    if (lds > kLdsPerCu / per_cu) continue;
+    p.rc = rc;
+    p.chunks = (n + rc - 1) / rc;
+    p.smax = rc / kSellRows;
+    p.lds_bytes = static_cast<uint32_t>(lds);
+    return true;
+  }
+  return false;
+}
+uint64_t sell_slices(const SellPlan &p) { return static_cast<uint64_t>(kTargetBlocks) * p.chunks * p.smax; }
+uint32_t sell_blocks(const SellPlan &p) { return kTargetBlocks * p.chunks; }
+
+// One workgroup per (range, chunk): bitonic sort of (~len << 32 | slot) in
+// LDS; writes the chunk's row order and each slice's entry count (16 x the
+// longest row rounded up to 4).
+__global__ __launch_bounds__(256) void sell_sort_kernel(const uint32_t *__restrict__ row_ptr,
+                                                        const uint16_t *__restrict__ off, uint32_t n,
+                                                        SellPlan p, uint32_t *__restrict__ perm,
+                                                        uint32_t *__restrict__ ssz) {
+  __shared__ uint64_t key[kSellMaxChunk];
+  const uint32_t b = blockIdx.x % kTargetBlocks, c = blockIdx.x / kTargetBlocks;
+  const uint32_t r0 = c * p.rc, rows = min(p.rc, n - r0);
+  uint32_t P = kSellRows;
+  while (P < rows) P <<= 1;
+  for (uint32_t i = threadIdx.x; i < P; i += 256) {
+    uint64_t k = ~0ull;
+    if (i < rows) {
+      const uint32_t row = r0 + i;
+      const uint32_t len = tb_seg_end(row_ptr, off, n, b, row) - tb_seg_beg(off, n, b, row);
+      k = (static_cast<uint64_t>(~len) << 32) | i;
+    }
+    key[i] = k;
+  }
+  __syncthreads();
+  for (uint32_t k = 2; k <= P; k <<= 1) {
+    for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+      for (uint32_t i = threadIdx.x; i < P; i += 256) {
+        const uint32_t ij = i ^ j;
+        if (ij > i) {
+          const uint64_t x = key[i], y = key[ij];
+          const bool up = (i & k) == 0;
+          if ((x > y) == up) {
+            key[i] = y;
+            key[ij] = x;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+  uint32_t *pm = perm + static_cast<size_t>(b) * n + r0;
+  for (uint32_t i = threadIdx.x; i < rows; i += 256) pm[i] = r0 + static_cast<uint32_t>(key[i]);
+  uint32_t *sz = ssz + (static_cast<size_t>(b) * p.chunks + c) * p.smax;
+  for (uint32_t sl = threadIdx.x; sl < p.smax; sl += 256) {
+    uint32_t w = 0;
+    if (sl * kSellRows < rows) {
+      const uint32_t len = ~static_cast<uint32_t>(key[sl * kSellRows] >> 32);
+      w = (len + 3u) & ~3u;
+    }
+    sz[sl] = kSellRows * w;
+  }
+}
+
+// One thread per (range, row slot): copies the row's range-b segment of the
+// partitioned CSR into its slice, step-major, and pads it to the slice width.
+__global__ __launch_bounds__(256) void sell_fill_kernel(const uint32_t *__restrict__ row_ptr,
+                                                        const uint32_t *__restrict__ tb_ent,
+                                                        const uint16_t *__restrict__ off, uint32_t n,
+                                                        SellPlan p, const uint32_t *__restrict__ perm,
+                                                        const uint32_t *__restrict__ soff,
+                                                        uint32_t *__restrict__ sent) {
+  const uint64_t gid = static_cast<uint64_t>(blockIdx.x) * 256 + threadIdx.x;
+  if (gid >= static_cast<uint64_t>(kTargetBlocks) * n) return;
+  const uint32_t b = static_cast<uint32_t>(gid / n), slot = static_cast<uint32_t>(gid % n);
+  const uint32_t c = slot / p.rc, i = slot % p.rc;
+  const uint32_t row = perm[static_cast<size_t>(b) * n + slot];
+  const uint32_t sl = i / kSellRows, j = i % kSellRows;
+  const size_t gs = (static_cast<size_t>(b) * p.chunks + c) * p.smax + sl;
+  const uint32_t base = soff[gs], width = (soff[gs + 1] - base) / kSellRows;
+  const uint32_t sb = tb_seg_beg(off, n, b, row);
+  const uint32_t len = tb_seg_end(row_ptr, off, n, b, row) - sb;
+  const uint32_t *src = tb_ent + row_ptr[row] + sb;
+  const uint32_t lo = b * p.span;
+  for (uint32_t q = 0; q < width; ++q) {
+    uint32_t v = kSellPad;
+    if (q < len) {
+      const uint32_t w = src[q];
+      v = (w & ~kIdxMask) | ((w & kIdxMask) - lo);
+    }
+    sent[base + (q >> 2) * 64u + j * 4u + (q & 3u)] = v;
+  }
+}
+
+hipError_t launch_sell_sort(const uint32_t *row_ptr, const uint16_t *off, uint32_t n, const SellPlan &p,
+                            uint32_t *perm, uint32_t *ssz, hipStream_t s) {
+  sell_sort_kernel<<<sell_blocks(p), 256, 0, s>>>(row_ptr, off, n, p, perm, ssz);
+  return hipGetLastError();
+}
+
+hipError_t launch_sell_fill(const uint32_t *row_ptr, const uint32_t *tb_ent, const uint16_t *off, uint32_t n,
+                            const SellPlan &p, const uint32_t *perm, const uint32_t *soff, uint32_t *sent,
+                            hipStream_t s) {
+  const uint64_t total = static_cast<uint64_t>(kTargetBlocks) * n;
+  sell_fill_kernel<<<static_cast<uint32_t>((total + 255) / 256), 256, 0, s>>>(row_ptr, tb_ent, off, n, p, perm,
+                                                                              soff, sent);
+  return hipGetLastError();
+}
+
+template <int DIMS, bool LOSS>
+__global__ __launch_bounds__(kSellThreads) void grad_sell_kernel(const GradArgs a, const SellPlan p,
+                                                                 const uint32_t *__restrict__ perm,
+                                                                 const uint32_t *__restrict__ soff,
+                                                                 const uint32_t *__restrict__ sent,
+                                                                 float *__restrict__ part) {
+  extern __shared__ __align__(16) float lds[];
+  const uint32_t b = blockIdx.x % kTargetBlocks, c = blockIdx.x / kTargetBlocks;
+  const uint32_t n = a.n;
+  const uint32_t lo = b * p.span;
+  const uint32_t nlo = lo < n ? min(p.span, n - lo) : 0u;
+  const uint32_t r0 = c * p.rc, rows = min(p.rc, n - r0);
+  float *sY = lds;                                       // range b of Y
+  float *sR = lds + static_cast<size_t>(p.span) * DIMS;  // the chunk's rows of Y
+  const int t = threadIdx.x;
+  {
+    // full tiles with 8 loads in flight per thread, then the remainder
+    constexpr uint32_t F = 8, T = kSellThreads;
+    const float *src = a.y_in + static_cast<size_t>(lo) * DIMS;
+    const uint32_t ny = nlo * DIMS, nr = rows * DIMS;
+    uint32_t i0 = t;
+    for (; i0 + (F - 1) * T < ny; i0 += F * T) {
+      float v[F];
+#pragma unroll
+      for (uint32_t f = 0; f < F; ++f) v[f] = src[i0 + f * T];
+#pragma unroll
+      for (uint32_t f = 0; f < F; ++f) sY[i0 + f * T] = v[f];
+    }
+    for (; i0 < ny; i0 += T) sY[i0] = src[i0];
+    const float *srr = a.y_in + static_cast<size_t>(r0) * DIMS;
+    for (uint32_t i = t; i < nr; i += T) sR[i] = srr[i];
+  }
+  const PairConsts pc = pair_consts(a.w);
+  const float kn = pc.k_nn, km = pc.k_mn, kf = pc.k_fp, wn = pc.w_nn, wm = pc.w_mn, wf = pc.w_fp;
+  __syncthreads();
+  const uint32_t wave = __builtin_amdgcn_readfirstlane(t >> 6);
+  const int lane = t & 63, j = lane >> 2;
+  const uint32_t *pm = perm + static_cast<size_t>(b) * n + r0;
+  const uint32_t *so = soff + (static_cast<size_t>(b) * p.chunks + c) * p.smax;
+  constexpr uint32_t kStride = kSellThreads / 64;  // slices between a wave's turns
+  constexpr int U = 4;
+  float loss = 0.0f;
+  // software pipeline: the next slice's row, extent and first U steps of
+  // entries are loaded before the current slice is computed.  The loads are
+  // unconditional (clamped addresses, results masked afterwards) so the
+  // compiler can count them and wait only for the older ones.
+  uint32_t sl = wave;
+  uint32_t row = r0, beg = 0, cnt = 0;
+  uint32_t wv[U];
+  bool valid = false;
+  const uint32_t last_sl = (rows - 1) / kSellRows;
+  auto fetch = [&](uint32_t s2, uint32_t &row2, uint32_t &beg2, uint32_t &cnt2, uint32_t (&w2)[U], bool &v2) {
+    const uint32_t s2c = min(s2, last_sl);
+    const uint32_t i = s2c * kSellRows + j;
+    v2 = s2 <= last_sl && i < rows;  // lanes past the chunk's last row hold no entries
+    row2 = pm[min(i, rows - 1)];
+    beg2 = so[s2c];
+    cnt2 = so[s2c + 1] - beg2;
+#pragma unroll
+    for (int uu = 0; uu < U; ++uu) w2[uu] = sent[min(beg2 + 64u * uu + lane, p.words - 1)];
+  };
+  if (sl * kSellRows < rows) fetch(sl, row, beg, cnt, wv, valid);
+  while (sl * kSellRows < rows) {
+    const uint32_t nsl = sl + kStride;
+    uint32_t nrow, nbeg, ncnt, nwv[U];
+    bool nvalid;
+    fetch(nsl, nrow, nbeg, ncnt, nwv, nvalid);
+    float yr[DIMS], g[DIMS];
+#pragma unroll
+    for (int d = 0; d < DIMS; ++d) {
+      yr[d] = sR[(row - r0) * DIMS + d];
+      g[d] = 0.0f;
+    }
+    // lim: steps of a round that belong to the lane's row (the other loaded
+    // words are clamped reads of other data)
+    auto round = [&](const uint32_t (&ww)[U], uint32_t lim) {
+#pragma unroll
+      for (int uu = 0; uu < U; ++uu) {
+        const uint32_t w = ww[uu];
+        if (static_cast<uint32_t>(uu) >= lim || w == kSellPad) continue;
+        const float *yo = sY + (w & kIdxMask) * DIMS;
+        float yv[DIMS];
+#pragma unroll
+        for (int d = 0; d < DIMS; ++d) yv[d] = yo[d];
+        pair_term<DIMS, LOSS>(w, yr, yv, kn, km, kf, wn, wm, wf, g, loss);
+      }
+    };
+    round(wv, valid ? min(static_cast<uint32_t>(U), cnt >> 6) : 0u);
+    for (uint32_t k0 = 64u * U; k0 < cnt; k0 += 64u * U) {  // rows longer than 4 U entries
+      uint32_t xv[U];
+#pragma unroll
+      for (int uu = 0; uu < U; ++uu) xv[uu] = sent[min(beg + k0 + 64u * uu + lane, p.words - 1)];
+      round(xv, valid ? min(static_cast<uint32_t>(U), (cnt - k0) >> 6) : 0u);
+    }
+#pragma unroll
+    for (int o = 2; o > 0; o >>= 1)
+#pragma unroll
+      for (int d = 0; d < DIMS; ++d) g[d] = g[d] + __shfl_xor(g[d], o, 64);
+    if ((lane & 3) == 0 && valid) {
+#pragma unroll
+      for (int d = 0; d < DIMS; ++d) part[(static_cast<size_t>(b) * n + row) * DIMS + d] = g[d];
+    }
+    sl = nsl;
+    row = nrow;
+    beg = nbeg;
+    cnt = ncnt;
+    valid = nvalid;
+#pragma unroll
+    for (int uu = 0; uu < U; ++uu) wv[uu] = nwv[uu];
+  }
+  if (LOSS) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) loss = loss + __shfl_xor(loss, o, 64);
+    __syncthreads();  // sY is no longer read: reuse it for the per-wave sums
+    if (lane == 0) lds[wave] = loss;
+    __syncthreads();
+    if (t == 0) {
+      float s = 0.0f;
+      for (uint32_t w = 0; w < kSellThreads / 64; ++w) s = s + lds[w];
+      a.loss_part[blockIdx.x] = s;
+    }
+  }
+}
+
+template <int DIMS>
+static hipError_t launch_sell_dims(const GradArgs &a, const SellPlan &p, const uint32_t *perm,
+                                   const uint32_t *soff, const uint32_t *sent, float *part, bool adam,
+                                   hipStream_t s) {
+  const uint32_t g1 = sell_blocks(p);
+  if (g1 == 0) return hipSuccess;
+  const size_t shm = (static_cast<size_t>(p.span) + p.rc) * DIMS * sizeof(float);
+  if (a.loss_part)
+    grad_sell_kernel<DIMS, true><<<g1, kSellThreads, shm, s>>>(a, p, perm, soff, sent, part);
+  else
+    grad_sell_kernel<DIMS, false><<<g1, kSellThreads, shm, s>>>(a, p, perm, soff, sent, part);
+  const uint32_t g2 = (a.n + 255) / 256;
+  if (adam)
+    grad_tb_combine_kernel<DIMS, true><<<g2, 256, 0, s>>>(a, part);
+  else
+    grad_tb_combine_kernel<DIMS, false><<<g2, 256, 0, s>>>(a, part);
+  return hipGetLastError();
+}
+
+hipError_t launch_grad_adam_sell(const GradArgs &a, const SellPlan &p, const uint32_t *perm, const uint32_t *soff,
+                                 const uint32_t *sent, float *part, uint32_t dims, bool adam, hipStream_t s) {
+  switch (dims) {
+    case 1: return launch_sell_dims<1>(a, p, perm, soff, sent, part, adam, s);
+    case 2: return launch_sell_dims<2>(a, p, perm, soff, sent, part, adam, s);
+    case 3: return launch_sell_dims<3>(a, p, perm, soff, sent, part, adam, s);
+    case 4: return launch_sell_dims<4>(a, p, perm, soff, sent, part, adam, s);
+    case 5: return launch_sell_dims<5>(a, p, perm, soff, sent, part, adam, s);
+    case 6: return launch_sell_dims<6>(a, p, perm, soff, sent, part, adam, s);
+    case 7: return launch_sell_dims<7>(a, p, perm, soff, sent, part, adam, s);
+    case 8: return launch_sell_dims<8>(a, p, perm, soff, sent, part, adam, s);
     default: return hipErrorInvalidValue;
   }
 }

@@ -78,7 +78,8 @@ class Timings(C.Structure):
                                         "knn_candidates")] + [
         ("knn_rerank_ms", C.c_double), ("knn_fallback_ms", C.c_double), ("knn_path", C.c_uint64),
         ("knn_sample_ms", C.c_double), ("knn_certified", C.c_uint64),
-        ("grad_target_blocks", C.c_uint64)]
+        ("grad_target_blocks", C.c_uint64), ("grad_sell", C.c_uint64),
+        ("grad_sell_entries", C.c_uint64)]
 
 
 INIT_PCA, INIT_VALUE, INIT_RANDOM = 0, 1, 2
@@ -363,12 +364,14 @@ def fit_transform(x, config: Optional[Configuration] = None, return_loss: bool =
     if config.snapshots is not None:
         snaps = np.zeros((len(config.snapshots), n, dims), np.float32)
     total = int(sum(config.num_iters))
-    loss = np.zeros(max(total, 1), np.float32)
+    # The reference computes the loss inside every gradient call but never
+    # returns it (lib.rs:806); it is only produced here when asked for.
+    loss = np.zeros(max(total, 1), np.float32) if return_loss else None
     st = Status()
     fn = lib().pacmap_fit_transform_dev if dev else lib().pacmap_fit_transform
     rc = fn(x.ptr if dev else x.ctypes.data, n, d, C.byref(c), y.ctypes.data,
-            snaps.ctypes.data if snaps is not None and snaps.size else None, loss.ctypes.data,
-            C.byref(st))
+            snaps.ctypes.data if snaps is not None and snaps.size else None,
+            loss.ctypes.data if loss is not None else None, C.byref(st))
     _check(rc, st)
     if return_loss:
         return y, snaps, loss[:total]
@@ -473,10 +476,11 @@ def sample_pairs(x, pair_neighbors, n_neighbors, n_mn, n_fp, seed):
 
 
 def optimize(y0, pair_neighbors, pair_mn, pair_fp, lr=1.0, num_iters=(100, 100, 250),
-             snapshots=None):
+             snapshots=None, with_loss=True):
     """Seam for the optimiser loop of pacmap() (lib.rs:781-837) from a given Y.
 
-    Returns (y, snapshots or None, per-iteration loss)."""
+    Returns (y, snapshots or None, per-iteration loss or None when with_loss
+    is False, which runs the loss-free kernels)."""
     y = np.array(y0, np.float32, copy=True, order="C")
     n, dims = y.shape
     a, na = _pairs(pair_neighbors)
@@ -484,7 +488,7 @@ def optimize(y0, pair_neighbors, pair_mn, pair_fp, lr=1.0, num_iters=(100, 100, 
     c, nc = _pairs(pair_fp)
     it = np.ascontiguousarray(num_iters, np.uint64)
     total = int(it.sum())
-    loss = np.zeros(max(total, 1), np.float32)
+    loss = np.zeros(max(total, 1), np.float32) if with_loss else None
     snaps = sidx = None
     ns = 0
     if snapshots is not None:
@@ -495,9 +499,10 @@ def optimize(y0, pair_neighbors, pair_mn, pair_fp, lr=1.0, num_iters=(100, 100, 
     rc = lib().pacmap_optimize(y.ctypes.data, n, dims, a.ctypes.data, na, b.ctypes.data, nb,
                                c.ctypes.data, nc, lr, it.ctypes.data,
                                sidx.ctypes.data if ns else None, ns,
-                               snaps.ctypes.data if ns else None, loss.ctypes.data, C.byref(st))
+                               snaps.ctypes.data if ns else None,
+                               loss.ctypes.data if loss is not None else None, C.byref(st))
     _check(rc, st)
-    return y, snaps, loss[:total]
+    return y, snaps, loss[:total] if loss is not None else None
 
 
 def generate_pairs(x, config: Optional[Configuration] = None):

@@ -372,6 +372,40 @@ def test_optimize_target_blocked_matches_oracle(monkeypatch, dims):
     assert pm.last_timings()["grad_target_blocks"] == 0
 
 
+@pytest.mark.parametrize("n,dims", [(3000, 2), (3000, 3), (777, 1), (20000, 2)])
+def test_optimize_lds_sliced_matches_target_blocked(monkeypatch, n, dims):
+    """The LDS-staged sliced-ELL gradient restages the target-blocked kernel
+    (same ranges, per-lane entry order, butterfly and combine): bit-identical
+    embeddings, deterministic, and the same bar against the oracle.  n = 3000
+    leaves a partial last slice (48-row chunks); 20000 takes the default path."""
+    y0, nn, mn, fp = _problem(64, n=n)
+    if dims != 2:
+        y0 = np.random.default_rng(65).normal(size=(n, dims)).astype(np.float32)
+    it = (4, 3, 3)
+    monkeypatch.setenv("PACMAP_GRAD_TARGET_BLOCKS", "1")
+    yt, _, _ = pm.optimize(y0, nn, mn, fp, 1.0, it)
+    assert pm.last_timings()["grad_target_blocks"] == 8
+    monkeypatch.setenv("PACMAP_GRAD_TARGET_BLOCKS", "0")
+    if n < 16384:
+        monkeypatch.setenv("PACMAP_GRAD_SELL", "1")
+    y, _, loss = pm.optimize(y0, nn, mn, fp, 1.0, it)
+    t = pm.last_timings()
+    assert t["grad_sell"] == 1 and t["grad_target_blocks"] == 0
+    assert t["grad_sell_entries"] >= 2 * (len(nn) + len(mn) + len(fp)) - 2 * n
+    assert np.array_equal(y, yt)
+    y2, _, loss2 = pm.optimize(y0, nn, mn, fp, 1.0, it)
+    assert np.array_equal(y, y2) and np.array_equal(loss, loss2)
+    assert_loss(loss, oracle_loss64(y0, nn, mn, fp, 1.0, it))
+    if n <= 3000:
+        ry, _, _ = O.optimize(y0, nn, mn, fp, 1.0, it)
+        assert_parity(y, ry, order_spread(lambda a, b, c: O.optimize(y0, a, b, c, 1.0, it)[0], nn, mn, fp))
+    monkeypatch.setenv("PACMAP_GRAD_SELL", "0")
+    y3, _, _ = pm.optimize(y0, nn, mn, fp, 1.0, it)
+    assert pm.last_timings()["grad_sell"] == 0
+    if n < 16384:
+        assert_parity(y3, y, order_spread(lambda a, b, c: O.optimize(y0, a, b, c, 1.0, it)[0], nn, mn, fp))
+
+
 def test_optimize_full_schedule_deterministic():
     y0, nn, mn, fp = _problem(61, n=2000)
     a = pm.optimize(y0, nn, mn, fp, 1.0, (100, 100, 250))
@@ -417,6 +451,9 @@ def test_fit_transform_value_allprovided_matches_oracle():
     assert snaps is None
     assert_parity(y, ry, order_spread(lambda a, b, c: O.optimize(y0, a, b, c, 1.0, it)[0], nn, mn, fp))
     assert_loss(loss, oracle_loss64(y0, nn, mn, fp, 1.0, it))
+    # Without a loss buffer the optimiser runs its loss-free kernels: same embedding.
+    y2, _ = pm.fit_transform(x, cfg)
+    assert np.array_equal(y2, y)
 
 
 def _oracle_pairs_generate(x, k, n_mn, n_fp, seed):
