@@ -265,6 +265,7 @@ typedef struct {
   uint64_t grad_target_blocks; /* 0: one-pass gradient; 8: target-blocked two-pass gradient */
   uint64_t grad_sell;          /* 1: LDS-staged target-blocked gradient (sliced ELL) */
   uint64_t grad_sell_entries;  /* sliced-ELL words including padding */
+  uint64_t grad_sell_ranges;   /* target ranges of the sliced-ELL gradient (4 or 8) */
 } pacmap_timings;
 void pacmap_last_timings(pacmap_timings *out);
 

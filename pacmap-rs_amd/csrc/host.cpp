@@ -265,9 +265,20 @@ bool use_target_blocks(uint64_t n, uint64_t dims) {
 
 // The LDS-staged (sliced ELL) gradient, for embeddings small enough that a
 // target range of Y plus a row chunk fits in LDS and large enough to fill
-// the GPU.  PACMAP_GRAD_SELL=0 / 1 forces it off / on (when it fits).
+// the GPU.  PACMAP_GRAD_SELL=0 / 1 forces it off / on (when it fits); 4
+// target ranges when they fit (one workgroup per CU), else 8.
 bool use_sell(uint64_t n, uint64_t dims, SellPlan &plan) {
-  if (n >= kMaxPoints || !sell_plan(static_cast<uint32_t>(n), static_cast<uint32_t>(dims), plan)) return false;
+  if (n >= kMaxPoints) return false;
+  // PACMAP_GRAD_SELL_RANGES=4 / 8 picks the number of target ranges
+  uint32_t order[2] = {4, 8};
+  if (const char *e = std::getenv("PACMAP_GRAD_SELL_RANGES")) {
+    if (e[0] == '8') order[0] = 8, order[1] = 0;
+    if (e[0] == '4') order[1] = 0;
+  }
+  bool ok = false;
+  for (uint32_t r : order)
+    if (r && !ok) ok = sell_plan(static_cast<uint32_t>(n), static_cast<uint32_t>(dims), r, plan);
+  if (!ok) return false;
   if (const char *e = std::getenv("PACMAP_GRAD_SELL")) {
     if (e[0] == '0') return false;
     if (e[0] == '1') return true;
@@ -328,7 +339,7 @@ void optimize_device(const DevCsr &csr, uint64_t n, uint64_t dims, std::vector<f
   if (sell) {
     const uint32_t n32 = static_cast<uint32_t>(n);
     const uint64_t ns_ = sell_slices(sp);
-    sell_perm.alloc(static_cast<size_t>(kTargetBlocks) * n);
+    sell_perm.alloc(static_cast<size_t>(sp.ranges) * n);
     sell_off.alloc(ns_ + 1);
     DevBuf<uint32_t> ssz(ns_), cur(ns_), scr((ns_ + 1023) / 1024 + 1);
     PM_HIP(launch_sell_sort(csr.row_ptr.get(), tb_off.get(), n32, sp, sell_perm.get(), ssz.get(), s));
@@ -341,6 +352,7 @@ void optimize_device(const DevCsr &csr, uint64_t n, uint64_t dims, std::vector<f
     PM_HIP(launch_sell_fill(csr.row_ptr.get(), tb_ent.get(), tb_off.get(), n32, sp, sell_perm.get(),
                             sell_off.get(), sell_ent.get(), s));
     g_tm.grad_sell_entries = total;
+    g_tm.grad_sell_ranges = sp.ranges;
   }
   g_tm.grad_target_blocks = tb ? kTargetBlocks : 0;
   g_tm.grad_sell = sell ? 1 : 0;

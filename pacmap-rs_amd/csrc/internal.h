@@ -128,16 +128,19 @@ hipError_t launch_grad_adam_tb(const GradArgs &a, const uint16_t *off, float *pa
 // once per fit from the tb_partition output: sort (perm: kTargetBlocks x n,
 // ssz: sell_slices() slice sizes), exclusive scan of ssz into soff (+1
 // total), fill (sent: soff[last] words).  Each iteration runs grad_sell
-// (loss_part: sell_blocks() floats) and the same combine+Adam pass.
+// (loss_part: sell_blocks() floats) and a combine+Adam pass over the
+// ranges' partials.
 struct SellPlan {
-  uint32_t span = 0;       // points per target range (tb ranges)
+  uint32_t ranges = 0;     // target ranges: 8 (the tb ranges) or 4 (tb ranges merged in pairs)
+  uint32_t group = 0;      // tb ranges per SELL range
+  uint32_t span = 0;       // points per target range
   uint32_t rc = 0;         // rows per chunk (multiple of 16)
   uint32_t chunks = 0;
   uint32_t smax = 0;       // slices per chunk
   uint32_t lds_bytes = 0;  // LDS per workgroup
   uint32_t words = 0;      // slice words (set once the layout is built)
 };
-bool sell_plan(uint32_t n, uint32_t dims, SellPlan &p);
+bool sell_plan(uint32_t n, uint32_t dims, uint32_t ranges, SellPlan &p);
 uint64_t sell_slices(const SellPlan &p);
 uint32_t sell_blocks(const SellPlan &p);
 hipError_t launch_sell_sort(const uint32_t *row_ptr, const uint16_t *off, uint32_t n, const SellPlan &p,

@@ -5,6 +5,7 @@
 #include "internal.h"
 
 #include <cstdint>
+#include <type_traits>
 
 namespace pm {
 
@@ -858,6 +859,36 @@ __global__ __launch_bounds__(256) void grad_tb_combine_kernel(const GradArgs a, 
   if (row < a.n) combine_row<DIMS, ADAM>(a, part, row);
 }
 
+// The same combine with one thread per coordinate (twice the waves for
+// dims = 2, half the dependent work per thread).
+template <bool ADAM, int RANGES>
+__global__ __launch_bounds__(256) void grad_combine_elem_kernel(const GradArgs a, const float *__restrict__ part,
+                                                                uint64_t count) {
+  const uint64_t e = blockIdx.x * 256ull + threadIdx.x;
+  if (e >= count) return;
+  float pv[RANGES];  // all loads in flight before the ordered sum
+#pragma unroll
+  for (int b = 0; b < RANGES; ++b) pv[b] = part[b * count + e];
+  float g = pv[0];
+#pragma unroll
+  for (int b = 1; b < RANGES; ++b) g = g + pv[b];
+  if (ADAM) {
+    float m = a.m[e], v = a.v[e];
+    m = m + a.omb1 * (g - m);          // adam.rs:57
+    const float gg = g * g;
+    v = v + a.omb2 * (gg - v);         // adam.rs:58
+    const float num = a.lr_t * m;
+    const float den = sqrtf(v) + 1e-7f;
+    const float y = a.y_in[e] - num / den;  // adam.rs:59
+    a.m[e] = m;
+    a.v[e] = v;
+    a.y_out[e] = y;
+    if (a.snap) a.snap[e] = y;
+  } else {
+    a.grad_out[e] = g;
+  }
+}
+
 uint32_t tb_span(uint32_t n) { return (n + kTargetBlocks - 1) / kTargetBlocks; }
 uint32_t grad_tb_blocks(uint32_t n) { return ((n + kTbRows - 1) / kTbRows) * kTargetBlocks; }
 
@@ -904,15 +935,18 @@ hipError_t launch_grad_adam_tb(const GradArgs &a, const uint16_t *off, float *pa
 // entry order, butterfly and combine as grad_tb_kernel, so the result is
 // bit-identical to it; what changes is where the other point comes from and
 // how lanes are fed:
-//  * a workgroup (range b = blockIdx.x % 8, row chunk c = blockIdx.x / 8)
-//    first copies range b of Y and its chunk's own rows of Y into LDS, so
-//    every gather is an LDS read instead of an L1/L2 request;
+//  * a workgroup (range b = blockIdx.x % R, row chunk c = blockIdx.x / R,
+//    R = 8 target ranges or 4 = the 8 merged in pairs) first copies range b
+//    of Y and its chunk's own rows of Y into LDS, so every gather is an LDS
+//    read instead of an L1/L2 request;
 //  * the chunk's rows are sorted by their range-b entry count (descending,
 //    ties by row) and cut into slices of 16 rows x 4 lanes; a slice stores its
 //    entries step-major (step k holds entries 4k + lane of its 16 rows, 64
 //    words, padded with kSellPad to the slice's longest row), so each step is
 //    one coalesced 256-byte load and lanes of a wave do nearly equal work.
 // Slice entries carry the target as an index relative to the range start.
+// With R = 8 the result is bit-identical to grad_tb_kernel; with R = 4 the
+// range partials are sums of two of its partials.
 // ---------------------------------------------------------------------------
 constexpr uint32_t kSellRows = 16;         // rows per slice (4 lanes each)
 constexpr uint32_t kSellMaxChunk = 2048;   // rows per chunk (LDS bitonic sort)
@@ -930,13 +964,27 @@ __device__ __forceinline__ uint32_t tb_seg_end(const uint32_t *__restrict__ row_
   return b + 1 == kTargetBlocks ? row_ptr[row + 1] - row_ptr[row] : off[static_cast<size_t>(b + 1) * n + row];
 }
 
-bool sell_plan(uint32_t n, uint32_t dims, SellPlan &p) {
+// Entries of SELL range b: tb ranges b g .. b g + g - 1 (g = 8 / R), which
+// tb_partition stored contiguously.
+__device__ __forceinline__ uint32_t sell_seg_beg(const uint16_t *__restrict__ off, uint32_t n, const SellPlan &p,
+                                                 uint32_t b, uint32_t row) {
+  return tb_seg_beg(off, n, b * p.group, row);
+}
+__device__ __forceinline__ uint32_t sell_seg_end(const uint32_t *__restrict__ row_ptr,
+                                                 const uint16_t *__restrict__ off, uint32_t n, const SellPlan &p,
+                                                 uint32_t b, uint32_t row) {
+  return tb_seg_end(row_ptr, off, n, b * p.group + p.group - 1, row);
+}
+
+bool sell_plan(uint32_t n, uint32_t dims, uint32_t ranges, SellPlan &p) {
   p = SellPlan{};
-  if (n < kTargetBlocks || dims == 0 || dims > 8) return false;
-  p.span = tb_span(n);
+  if (n < kTargetBlocks || dims == 0 || dims > 8 || (ranges != 4 && ranges != 8)) return false;
+  p.ranges = ranges;
+  p.group = kTargetBlocks / ranges;
+  p.span = tb_span(n) * p.group;
   // two workgroups per CU when both fit in LDS, else one
   for (uint32_t per_cu = 2; per_cu >= 1; --per_cu) {
-    const uint32_t want = 256u * per_cu / kTargetBlocks;  // chunks
+    const uint32_t want = 256u * per_cu / ranges;  // chunks
     uint32_t rc = (n + want - 1) / want;
     rc = (rc + kSellRows - 1) / kSellRows * kSellRows;
     if (rc > kSellMaxChunk) continue;
@@ -950,8 +998,8 @@ bool sell_plan(uint32_t n, uint32_t dims, SellPlan &p) {
   }
   return false;
 }
-uint64_t sell_slices(const SellPlan &p) { return static_cast<uint64_t>(kTargetBlocks) * p.chunks * p.smax; }
-uint32_t sell_blocks(const SellPlan &p) { return kTargetBlocks * p.chunks; }
+uint64_t sell_slices(const SellPlan &p) { return static_cast<uint64_t>(p.ranges) * p.chunks * p.smax; }
+uint32_t sell_blocks(const SellPlan &p) { return p.ranges * p.chunks; }
 
 // One workgroup per (range, chunk): bitonic sort of (~len << 32 | slot) in
 // LDS; writes the chunk's row order and each slice's entry count (16 x the
@@ -961,7 +1009,7 @@ __global__ __launch_bounds__(256) void sell_sort_kernel(const uint32_t *__restri
                                                         SellPlan p, uint32_t *__restrict__ perm,
                                                         uint32_t *__restrict__ ssz) {
   __shared__ uint64_t key[kSellMaxChunk];
-  const uint32_t b = blockIdx.x % kTargetBlocks, c = blockIdx.x / kTargetBlocks;
+  const uint32_t b = blockIdx.x % p.ranges, c = blockIdx.x / p.ranges;
   const uint32_t r0 = c * p.rc, rows = min(p.rc, n - r0);
   uint32_t P = kSellRows;
   while (P < rows) P <<= 1;
@@ -969,7 +1017,7 @@ __global__ __launch_bounds__(256) void sell_sort_kernel(const uint32_t *__restri
     uint64_t k = ~0ull;
     if (i < rows) {
       const uint32_t row = r0 + i;
-      const uint32_t len = tb_seg_end(row_ptr, off, n, b, row) - tb_seg_beg(off, n, b, row);
+      const uint32_t len = sell_seg_end(row_ptr, off, n, p, b, row) - sell_seg_beg(off, n, p, b, row);
       k = (static_cast<uint64_t>(~len) << 32) | i;
     }
     key[i] = k;
@@ -1013,15 +1061,15 @@ __global__ __launch_bounds__(256) void sell_fill_kernel(const uint32_t *__restri
                                                         const uint32_t *__restrict__ soff,
                                                         uint32_t *__restrict__ sent) {
   const uint64_t gid = static_cast<uint64_t>(blockIdx.x) * 256 + threadIdx.x;
-  if (gid >= static_cast<uint64_t>(kTargetBlocks) * n) return;
+  if (gid >= static_cast<uint64_t>(p.ranges) * n) return;
   const uint32_t b = static_cast<uint32_t>(gid / n), slot = static_cast<uint32_t>(gid % n);
   const uint32_t c = slot / p.rc, i = slot % p.rc;
   const uint32_t row = perm[static_cast<size_t>(b) * n + slot];
   const uint32_t sl = i / kSellRows, j = i % kSellRows;
   const size_t gs = (static_cast<size_t>(b) * p.chunks + c) * p.smax + sl;
   const uint32_t base = soff[gs], width = (soff[gs + 1] - base) / kSellRows;
-  const uint32_t sb = tb_seg_beg(off, n, b, row);
-  const uint32_t len = tb_seg_end(row_ptr, off, n, b, row) - sb;
+  const uint32_t sb = sell_seg_beg(off, n, p, b, row);
+  const uint32_t len = sell_seg_end(row_ptr, off, n, p, b, row) - sb;
   const uint32_t *src = tb_ent + row_ptr[row] + sb;
   const uint32_t lo = b * p.span;
   for (uint32_t q = 0; q < width; ++q) {
@@ -1040,23 +1088,41 @@ hipError_t launch_sell_sort(const uint32_t *row_ptr, const uint16_t *off, uint32
   return hipGetLastError();
 }
 
+__global__ void fill_u32_kernel(uint32_t *__restrict__ p, uint64_t count, uint32_t v) {
+  for (uint64_t e = blockIdx.x * 256ull + threadIdx.x; e < count; e += gridDim.x * 256ull) p[e] = v;
+}
+
 hipError_t launch_sell_fill(const uint32_t *row_ptr, const uint32_t *tb_ent, const uint16_t *off, uint32_t n,
                             const SellPlan &p, const uint32_t *perm, const uint32_t *soff, uint32_t *sent,
                             hipStream_t s) {
-  const uint64_t total = static_cast<uint64_t>(kTargetBlocks) * n;
+  // every word starts as padding: rows missing from a chunk's last slice keep it
+  if (p.words) {
+    const uint64_t nb = (p.words + 255) / 256;
+    fill_u32_kernel<<<static_cast<uint32_t>(nb < 16384 ? nb : 16384), 256, 0, s>>>(sent, p.words, kSellPad);
+  }
+  const uint64_t total = static_cast<uint64_t>(p.ranges) * n;
   sell_fill_kernel<<<static_cast<uint32_t>((total + 255) / 256), 256, 0, s>>>(row_ptr, tb_ent, off, n, p, perm,
                                                                               soff, sent);
   return hipGetLastError();
 }
 
-template <int DIMS, bool LOSS>
+// Value of another lane of the same quad (DPP quad_perm, e.g. 0xB1 = xor 1,
+// 0x4E = xor 2); every lane of the wave must be active.
+template <int PERM>
+__device__ __forceinline__ float quad_swap(float v) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), PERM, 0xF, 0xF, false));
+}
+
+// U: steps of a slice loaded ahead (4 with 8 ranges, 8 with 4 ranges: about
+// the longest rows of most slices)
+template <int DIMS, bool LOSS, int U>
 __global__ __launch_bounds__(kSellThreads) void grad_sell_kernel(const GradArgs a, const SellPlan p,
                                                                  const uint32_t *__restrict__ perm,
                                                                  const uint32_t *__restrict__ soff,
                                                                  const uint32_t *__restrict__ sent,
                                                                  float *__restrict__ part) {
   extern __shared__ __align__(16) float lds[];
-  const uint32_t b = blockIdx.x % kTargetBlocks, c = blockIdx.x / kTargetBlocks;
+  const uint32_t b = blockIdx.x % p.ranges, c = blockIdx.x / p.ranges;
   const uint32_t n = a.n;
   const uint32_t lo = b * p.span;
   const uint32_t nlo = lo < n ? min(p.span, n - lo) : 0u;
@@ -1089,46 +1155,48 @@ __global__ __launch_bounds__(kSellThreads) void grad_sell_kernel(const GradArgs 
   const uint32_t *pm = perm + static_cast<size_t>(b) * n + r0;
   const uint32_t *so = soff + (static_cast<size_t>(b) * p.chunks + c) * p.smax;
   constexpr uint32_t kStride = kSellThreads / 64;  // slices between a wave's turns
-  constexpr int U = 4;
   float loss = 0.0f;
   // software pipeline: the next slice's row, extent and first U steps of
-  // entries are loaded before the current slice is computed.  The loads are
-  // unconditional (clamped addresses, results masked afterwards) so the
-  // compiler can count them and wait only for the older ones.
-  uint32_t sl = wave;
-  uint32_t row = r0, beg = 0, cnt = 0;
-  uint32_t wv[U];
-  bool valid = false;
+  // entries are loaded before the current slice is computed.  Entry words
+  // come through a buffer resource: the offsets are per-wave scalars plus
+  // the lane, and reads past the array return 0 (steps past the slice are
+  // never used: lim below is uniform over the wave).  Rows missing from a
+  // chunk's last slice hold kSellPad words.
+  const __amdgpu_buffer_rsrc_t rs =
+      __builtin_amdgcn_make_buffer_rsrc(const_cast<uint32_t *>(sent), 0, static_cast<int>(p.words * 4u), 0x00020000);
+  struct Slice {
+    uint32_t row, beg, cnt;
+    uint32_t w[U];
+    bool valid;
+  };
   const uint32_t last_sl = (rows - 1) / kSellRows;
-  auto fetch = [&](uint32_t s2, uint32_t &row2, uint32_t &beg2, uint32_t &cnt2, uint32_t (&w2)[U], bool &v2) {
+  auto fetch = [&](uint32_t s2, Slice &q) {
     const uint32_t s2c = min(s2, last_sl);
     const uint32_t i = s2c * kSellRows + j;
-    v2 = s2 <= last_sl && i < rows;  // lanes past the chunk's last row hold no entries
-    row2 = pm[min(i, rows - 1)];
-    beg2 = so[s2c];
-    cnt2 = so[s2c + 1] - beg2;
+    q.valid = i < rows;
+    q.row = pm[min(i, rows - 1)];
+    q.beg = so[s2c];
+    q.cnt = s2 <= last_sl ? so[s2c + 1] - q.beg : 0u;
 #pragma unroll
-    for (int uu = 0; uu < U; ++uu) w2[uu] = sent[min(beg2 + 64u * uu + lane, p.words - 1)];
+    for (int uu = 0; uu < U; ++uu)
+      q.w[uu] = __builtin_amdgcn_raw_buffer_load_b32(rs, lane * 4, static_cast<int>((q.beg + 64u * uu) * 4u), 0);
   };
-  if (sl * kSellRows < rows) fetch(sl, row, beg, cnt, wv, valid);
-  while (sl * kSellRows < rows) {
-    const uint32_t nsl = sl + kStride;
-    uint32_t nrow, nbeg, ncnt, nwv[U];
-    bool nvalid;
-    fetch(nsl, nrow, nbeg, ncnt, nwv, nvalid);
+  auto compute = [&](const Slice &q) {
     float yr[DIMS], g[DIMS];
 #pragma unroll
     for (int d = 0; d < DIMS; ++d) {
-      yr[d] = sR[(row - r0) * DIMS + d];
+      yr[d] = sR[(q.row - r0) * DIMS + d];
       g[d] = 0.0f;
     }
-    // lim: steps of a round that belong to the lane's row (the other loaded
-    // words are clamped reads of other data)
-    auto round = [&](const uint32_t (&ww)[U], uint32_t lim) {
+    // CUT: stop at the first step past lim (prefetched round); otherwise
+    // every loaded word is inspected, so no load of the round is left in
+    // flight when the slice ends
+    auto round = [&](const uint32_t (&ww)[U], uint32_t lim, auto cut) {
 #pragma unroll
       for (int uu = 0; uu < U; ++uu) {
-        const uint32_t w = ww[uu];
-        if (static_cast<uint32_t>(uu) >= lim || w == kSellPad) continue;
+        if (decltype(cut)::value && static_cast<uint32_t>(uu) >= lim) break;
+        const uint32_t w = static_cast<uint32_t>(uu) < lim ? ww[uu] : kSellPad;
+        if (w == kSellPad) continue;
         const float *yo = sY + (w & kIdxMask) * DIMS;
         float yv[DIMS];
 #pragma unroll
@@ -1136,28 +1204,38 @@ __global__ __launch_bounds__(kSellThreads) void grad_sell_kernel(const GradArgs 
         pair_term<DIMS, LOSS>(w, yr, yv, kn, km, kf, wn, wm, wf, g, loss);
       }
     };
-    round(wv, valid ? min(static_cast<uint32_t>(U), cnt >> 6) : 0u);
-    for (uint32_t k0 = 64u * U; k0 < cnt; k0 += 64u * U) {  // rows longer than 4 U entries
+    round(q.w, min(static_cast<uint32_t>(U), q.cnt >> 6), std::true_type{});
+    for (uint32_t k0 = 64u * U; k0 < q.cnt; k0 += 64u * U) {  // rows longer than 4 U entries
       uint32_t xv[U];
 #pragma unroll
-      for (int uu = 0; uu < U; ++uu) xv[uu] = sent[min(beg + k0 + 64u * uu + lane, p.words - 1)];
-      round(xv, valid ? min(static_cast<uint32_t>(U), (cnt - k0) >> 6) : 0u);
+      for (int uu = 0; uu < U; ++uu)
+        xv[uu] = __builtin_amdgcn_raw_buffer_load_b32(rs, lane * 4, static_cast<int>((q.beg + k0 + 64u * uu) * 4u), 0);
+      round(xv, min(static_cast<uint32_t>(U), (q.cnt - k0) >> 6), std::false_type{});
     }
+    // butterfly over the row's 4 lanes (a quad): xor 2, then xor 1
 #pragma unroll
-    for (int o = 2; o > 0; o >>= 1)
-#pragma unroll
-      for (int d = 0; d < DIMS; ++d) g[d] = g[d] + __shfl_xor(g[d], o, 64);
-    if ((lane & 3) == 0 && valid) {
-#pragma unroll
-      for (int d = 0; d < DIMS; ++d) part[(static_cast<size_t>(b) * n + row) * DIMS + d] = g[d];
+    for (int d = 0; d < DIMS; ++d) {
+      g[d] = g[d] + quad_swap<0x4E>(g[d]);
+      g[d] = g[d] + quad_swap<0xB1>(g[d]);
     }
-    sl = nsl;
-    row = nrow;
-    beg = nbeg;
-    cnt = ncnt;
-    valid = nvalid;
+    if ((lane & 3) == 0 && q.valid) {
 #pragma unroll
-    for (int uu = 0; uu < U; ++uu) wv[uu] = nwv[uu];
+      for (int d = 0; d < DIMS; ++d) part[(static_cast<size_t>(b) * n + q.row) * DIMS + d] = g[d];
+    }
+  };
+  // two register sets in turn, so the prefetch needs no copies (and no wait
+  // for the slice just requested)
+  Slice qa, qb;
+  uint32_t sl = wave;
+  if (sl <= last_sl) fetch(sl, qa);
+  while (sl <= last_sl) {
+    fetch(sl + kStride, qb);
+    compute(qa);
+    sl += kStride;
+    if (sl > last_sl) break;
+    fetch(sl + kStride, qa);
+    compute(qb);
+    sl += kStride;
   }
   if (LOSS) {
 #pragma unroll
@@ -1180,15 +1258,31 @@ static hipError_t launch_sell_dims(const GradArgs &a, const SellPlan &p, const u
   const uint32_t g1 = sell_blocks(p);
   if (g1 == 0) return hipSuccess;
   const size_t shm = (static_cast<size_t>(p.span) + p.rc) * DIMS * sizeof(float);
-  if (a.loss_part)
-    grad_sell_kernel<DIMS, true><<<g1, kSellThreads, shm, s>>>(a, p, perm, soff, sent, part);
-  else
-    grad_sell_kernel<DIMS, false><<<g1, kSellThreads, shm, s>>>(a, p, perm, soff, sent, part);
-  const uint32_t g2 = (a.n + 255) / 256;
-  if (adam)
-    grad_tb_combine_kernel<DIMS, true><<<g2, 256, 0, s>>>(a, part);
-  else
-    grad_tb_combine_kernel<DIMS, false><<<g2, 256, 0, s>>>(a, part);
+  const bool loss = a.loss_part != nullptr;
+  if (p.ranges == 4) {
+    if (loss)
+      grad_sell_kernel<DIMS, true, 8><<<g1, kSellThreads, shm, s>>>(a, p, perm, soff, sent, part);
+    else
+      grad_sell_kernel<DIMS, false, 8><<<g1, kSellThreads, shm, s>>>(a, p, perm, soff, sent, part);
+  } else {
+    if (loss)
+      grad_sell_kernel<DIMS, true, 4><<<g1, kSellThreads, shm, s>>>(a, p, perm, soff, sent, part);
+    else
+      grad_sell_kernel<DIMS, false, 4><<<g1, kSellThreads, shm, s>>>(a, p, perm, soff, sent, part);
+  }
+  const uint64_t count = static_cast<uint64_t>(a.n) * DIMS;
+  const uint32_t g2 = static_cast<uint32_t>((count + 255) / 256);
+  if (p.ranges == 4) {
+    if (adam)
+      grad_combine_elem_kernel<true, 4><<<g2, 256, 0, s>>>(a, part, count);
+    else
+      grad_combine_elem_kernel<false, 4><<<g2, 256, 0, s>>>(a, part, count);
+  } else {
+    if (adam)
+      grad_combine_elem_kernel<true, 8><<<g2, 256, 0, s>>>(a, part, count);
+    else
+      grad_combine_elem_kernel<false, 8><<<g2, 256, 0, s>>>(a, part, count);
+  }
   return hipGetLastError();
 }
 

@@ -374,10 +374,11 @@ def test_optimize_target_blocked_matches_oracle(monkeypatch, dims):
 
 @pytest.mark.parametrize("n,dims", [(3000, 2), (3000, 3), (777, 1), (20000, 2)])
 def test_optimize_lds_sliced_matches_target_blocked(monkeypatch, n, dims):
-    """The LDS-staged sliced-ELL gradient restages the target-blocked kernel
-    (same ranges, per-lane entry order, butterfly and combine): bit-identical
-    embeddings, deterministic, and the same bar against the oracle.  n = 3000
-    leaves a partial last slice (48-row chunks); 20000 takes the default path."""
+    """The LDS-staged sliced-ELL gradient restages the target-blocked kernel:
+    with 8 target ranges (the same ranges, per-lane entry order, butterfly
+    and range-order combine) the embeddings are bit-identical; with 4 merged
+    ranges the same bar against the oracle holds.  Both are deterministic.
+    n = 3000 leaves a partial last slice; 20000 takes the default path."""
     y0, nn, mn, fp = _problem(64, n=n)
     if dims != 2:
         y0 = np.random.default_rng(65).normal(size=(n, dims)).astype(np.float32)
@@ -388,22 +389,27 @@ def test_optimize_lds_sliced_matches_target_blocked(monkeypatch, n, dims):
     monkeypatch.setenv("PACMAP_GRAD_TARGET_BLOCKS", "0")
     if n < 16384:
         monkeypatch.setenv("PACMAP_GRAD_SELL", "1")
-    y, _, loss = pm.optimize(y0, nn, mn, fp, 1.0, it)
-    t = pm.last_timings()
-    assert t["grad_sell"] == 1 and t["grad_target_blocks"] == 0
-    assert t["grad_sell_entries"] >= 2 * (len(nn) + len(mn) + len(fp)) - 2 * n
-    assert np.array_equal(y, yt)
-    y2, _, loss2 = pm.optimize(y0, nn, mn, fp, 1.0, it)
-    assert np.array_equal(y, y2) and np.array_equal(loss, loss2)
-    assert_loss(loss, oracle_loss64(y0, nn, mn, fp, 1.0, it))
-    if n <= 3000:
-        ry, _, _ = O.optimize(y0, nn, mn, fp, 1.0, it)
-        assert_parity(y, ry, order_spread(lambda a, b, c: O.optimize(y0, a, b, c, 1.0, it)[0], nn, mn, fp))
+    spread = None
+    for ranges in (8, 4):
+        monkeypatch.setenv("PACMAP_GRAD_SELL_RANGES", str(ranges))
+        y, _, loss = pm.optimize(y0, nn, mn, fp, 1.0, it)
+        t = pm.last_timings()
+        assert t["grad_sell"] == 1 and t["grad_target_blocks"] == 0 and t["grad_sell_ranges"] == ranges
+        assert t["grad_sell_entries"] >= 2 * (len(nn) + len(mn) + len(fp)) - 2 * n
+        if ranges == 8:
+            assert np.array_equal(y, yt)
+        y2, _, loss2 = pm.optimize(y0, nn, mn, fp, 1.0, it)
+        assert np.array_equal(y, y2) and np.array_equal(loss, loss2)
+        assert_loss(loss, oracle_loss64(y0, nn, mn, fp, 1.0, it))
+        if n <= 3000:
+            ry, _, _ = O.optimize(y0, nn, mn, fp, 1.0, it)
+            if spread is None:
+                spread = order_spread(lambda a, b, c: O.optimize(y0, a, b, c, 1.0, it)[0], nn, mn, fp)
+            assert_parity(y, ry, spread)
+    monkeypatch.delenv("PACMAP_GRAD_SELL_RANGES")
     monkeypatch.setenv("PACMAP_GRAD_SELL", "0")
-    y3, _, _ = pm.optimize(y0, nn, mn, fp, 1.0, it)
+    pm.optimize(y0, nn, mn, fp, 1.0, it)
     assert pm.last_timings()["grad_sell"] == 0
-    if n < 16384:
-        assert_parity(y3, y, order_spread(lambda a, b, c: O.optimize(y0, a, b, c, 1.0, it)[0], nn, mn, fp))
 
 
 def test_optimize_full_schedule_deterministic():
