@@ -228,8 +228,9 @@ int pacmap_dev_free(void *ptr);
 int pacmap_dev_upload(void *dst_dev, const void *src_host, uint64_t bytes, pacmap_status *st);
 int pacmap_dev_download(void *dst_host, const void *src_dev, uint64_t bytes, pacmap_status *st);
 /* Device blocks the library allocated are cached per calling thread and
- * reused by its later calls (no hipMalloc / hipFree per call).  This frees
- * the calling thread's cache. */
+ * reused by its later calls (no hipMalloc / hipFree per call), and so is the
+ * thread's last optimiser graph (replayed when a call's launches are
+ * identical).  This frees the calling thread's caches. */
 void pacmap_release_cached_memory(void);
 /* HBM reference: a plain streaming device-to-device copy kernel run `repeats`
  * times; ms_out gets the average time per copy (HIP events).  Its rocprof
@@ -266,6 +267,7 @@ typedef struct {
   uint64_t grad_sell;          /* 1: LDS-staged target-blocked gradient (sliced ELL) */
   uint64_t grad_sell_entries;  /* sliced-ELL words including padding */
   uint64_t grad_sell_ranges;   /* target ranges of the sliced-ELL gradient (4 or 8) */
+  uint64_t graph_reused;       /* 1: the previous call's optimiser graph was replayed */
 } pacmap_timings;
 void pacmap_last_timings(pacmap_timings *out);
 

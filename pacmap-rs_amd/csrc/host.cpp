@@ -190,6 +190,45 @@ struct GraphGuard {
   }
 };
 
+// The optimiser graph of this thread's previous call, kept with a key made
+// of every launch argument it captured.  A call whose launches would be
+// identical (same sizes and schedule and, through the caching allocator, the
+// same buffers) replays it instead of capturing and instantiating 450+
+// nodes again.  Each call still writes all of the graph's inputs first.
+struct GraphCache {
+  std::vector<uint64_t> key;
+  hipGraph_t g = nullptr;
+  hipGraphExec_t e = nullptr;
+  void reset() {
+    if (e) (void)hipGraphExecDestroy(e);
+    if (g) (void)hipGraphDestroy(g);
+    e = nullptr;
+    g = nullptr;
+    key.clear();
+  }
+};
+thread_local GraphCache g_graph;  // not destroyed at thread exit: the runtime may be gone by then
+
+}  // namespace
+void graph_cache_reset() { g_graph.reset(); }
+namespace {
+
+struct KeyWriter {
+  std::vector<uint64_t> &k;
+  void add(uint64_t v) { k.push_back(v); }
+  void add(const void *p) { k.push_back(reinterpret_cast<uint64_t>(p)); }
+  void add(float f) {
+    uint32_t u;
+    std::memcpy(&u, &f, 4);
+    k.push_back(u);
+  }
+  void add(const GradArgs &a) {
+    add(a.row_ptr), add(a.ent), add(a.y_in), add(static_cast<const void *>(a.y_out)), add(a.m), add(a.v);
+    add(a.grad_out), add(a.loss_part), add(a.snap), add(static_cast<uint64_t>(a.n));
+    add(a.w[0]), add(a.w[1]), add(a.w[2]), add(a.lr_t), add(a.omb1), add(a.omb2), add(a.mean_row_len);
+  }
+};
+
 // Symmetrised CSR in device memory (see internal.h for the entry format).
 struct DevCsr {
   DevBuf<uint32_t> row_ptr, ent;
@@ -207,7 +246,7 @@ void build_csr_dev(uint64_t n, const uint32_t *const dp[3], const uint64_t rows[
   PM_HIP(launch_fill_zero(deg.get(), deg.bytes(), s));
   PM_HIP(launch_fill_zero(flags.get(), flags.bytes(), s));
   PM_HIP(launch_csr_count(dp, rows, static_cast<uint32_t>(n), deg.get(), flags.get(), s));
-  out.row_ptr.alloc(n + 1);
+  out.row_ptr.alloc(n + 1, kSlotCsrRow);
   PM_HIP(launch_csr_scan(deg.get(), static_cast<uint32_t>(n), out.row_ptr.get(), cursor.get(), scratch.get(), s));
   uint32_t hf[2] = {0, 0}, nnz = 0;
   download(hf, flags.get(), 2, s);
@@ -215,7 +254,7 @@ void build_csr_dev(uint64_t n, const uint32_t *const dp[3], const uint64_t rows[
   PM_HIP(hipStreamSynchronize(s));
   if (hf[0]) fail(PACMAP_ERR_INVALID_PAIR_INDEX, "pair index out of range");
   out.nnz = nnz;
-  out.ent.alloc(std::max<uint64_t>(1, nnz));
+  out.ent.alloc(std::max<uint64_t>(1, nnz), kSlotCsrEnt);
   PM_HIP(launch_csr_fill_sort(dp, rows, static_cast<uint32_t>(n), out.row_ptr.get(), cursor.get(),
                               out.ent.get(), flags.get() + 1, s));
   download(hf, flags.get(), 2, s);
@@ -307,11 +346,11 @@ void optimize_device(const DevCsr &csr, uint64_t n, uint64_t dims, std::vector<f
     std::memset(snaps_out, 0, sizeof(float) * ns * nd);
     std::memcpy(snaps_out, y.data(), sizeof(float) * nd);
   }
-  DevBuf<float> ybuf(2 * nd), m(nd), v(nd);
+  DevBuf<float> ybuf(2 * nd, kSlotY), m(nd, kSlotM), v(nd, kSlotV);
   upload(ybuf, y.data(), nd, s);
   PM_HIP(launch_fill_zero(m.get(), m.bytes(), s));
   PM_HIP(launch_fill_zero(v.get(), v.bytes(), s));
-  DevBuf<float> d_snap(ns && snaps_out ? ns * nd : 0);
+  DevBuf<float> d_snap(ns && snaps_out ? ns * nd : 0, kSlotSnap);
   // Target-blocked gradient when Y outgrows an XCD's L2, its LDS-staged
   // form when a target range fits in LDS (see kernels.hip)
   SellPlan sp;
@@ -321,8 +360,8 @@ void optimize_device(const DevCsr &csr, uint64_t n, uint64_t dims, std::vector<f
   DevBuf<uint16_t> tb_off;
   DevBuf<float> tb_part;
   if (tb || sell) {
-    tb_ent.alloc(csr.nnz);
-    tb_off.alloc(static_cast<size_t>(kTargetBlocks) * n);
+    tb_ent.alloc(csr.nnz, kSlotTbEnt);
+    tb_off.alloc(static_cast<size_t>(kTargetBlocks) * n, kSlotTbOff);
     DevBuf<uint32_t> bad(1);
     PM_HIP(launch_fill_zero(bad.get(), bad.bytes(), s));
     PM_HIP(launch_tb_partition(csr.row_ptr.get(), csr.ent.get(), static_cast<uint32_t>(n), tb_ent.get(),
@@ -333,21 +372,21 @@ void optimize_device(const DevCsr &csr, uint64_t n, uint64_t dims, std::vector<f
     if (hb) {
       tb = sell = false;  // a row too long for the u16 range offsets: plain path
     } else {
-      tb_part.alloc(static_cast<size_t>(kTargetBlocks) * nd);
+      tb_part.alloc(static_cast<size_t>(kTargetBlocks) * nd, kSlotTbPart);
     }
   }
   if (sell) {
     const uint32_t n32 = static_cast<uint32_t>(n);
     const uint64_t ns_ = sell_slices(sp);
-    sell_perm.alloc(static_cast<size_t>(sp.ranges) * n);
-    sell_off.alloc(ns_ + 1);
+    sell_perm.alloc(static_cast<size_t>(sp.ranges) * n, kSlotSellPerm);
+    sell_off.alloc(ns_ + 1, kSlotSellOff);
     DevBuf<uint32_t> ssz(ns_), cur(ns_), scr((ns_ + 1023) / 1024 + 1);
     PM_HIP(launch_sell_sort(csr.row_ptr.get(), tb_off.get(), n32, sp, sell_perm.get(), ssz.get(), s));
     PM_HIP(launch_csr_scan(ssz.get(), static_cast<uint32_t>(ns_), sell_off.get(), cur.get(), scr.get(), s));
     uint32_t total = 0;
     download(&total, sell_off.get() + ns_, 1, s);
     PM_HIP(hipStreamSynchronize(s));
-    sell_ent.alloc(total ? total : 1);
+    sell_ent.alloc(total ? total : 1, kSlotSellEnt);
     sp.words = total ? total : 1;
     PM_HIP(launch_sell_fill(csr.row_ptr.get(), tb_ent.get(), tb_off.get(), n32, sp, sell_perm.get(),
                             sell_off.get(), sell_ent.get(), s));
@@ -358,20 +397,18 @@ void optimize_device(const DevCsr &csr, uint64_t n, uint64_t dims, std::vector<f
   g_tm.grad_sell = sell ? 1 : 0;
   const uint32_t nb = sell ? sell_blocks(sp)
                            : (tb ? grad_tb_blocks(static_cast<uint32_t>(n)) : grad_blocks(static_cast<uint32_t>(n)));
-  DevBuf<float> part(record_loss ? T * nb : 0), d_loss(record_loss ? T : 0);
+  DevBuf<float> part(record_loss ? T * nb : 0, kSlotLossPart), d_loss(record_loss ? T : 0);
   PM_HIP(hipStreamSynchronize(s));
   g_tm.iterations = T;
   if (T == 0) return;
 
   const float beta1 = 0.9f, beta2 = 0.999f, w_mn_init = 1000.0f;  // lib.rs:782-784
-  GraphGuard gg;
   const double tg = now_ms();
-  PM_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-  hipError_t cap_err = hipSuccess;
-  for (uint64_t it = 0; it < T && cap_err == hipSuccess; ++it) {
-    GradArgs a;
+  std::vector<GradArgs> plan(T);
+  for (uint64_t it = 0; it < T; ++it) {
+    GradArgs &a = plan[it];
     a.row_ptr = csr.row_ptr.get();
-    a.ent = csr.ent.get();
+    a.ent = tb && !sell ? tb_ent.get() : csr.ent.get();
     a.mean_row_len = static_cast<float>(static_cast<double>(csr.nnz) / static_cast<double>(n));
     a.y_in = ybuf.get() + (it % 2) * nd;
     a.y_out = ybuf.get() + ((it + 1) % 2) * nd;
@@ -384,24 +421,47 @@ void optimize_device(const DevCsr &csr, uint64_t n, uint64_t dims, std::vector<f
     a.omb2 = 1.0f - beta2;
     a.loss_part = record_loss ? part.get() + it * nb : nullptr;
     a.snap = (writer[it] >= 0 && d_snap.get()) ? d_snap.get() + writer[it] * nd : nullptr;
-    if (sell) {
-      cap_err = launch_grad_adam_sell(a, sp, sell_perm.get(), sell_off.get(), sell_ent.get(), tb_part.get(),
-                                      static_cast<uint32_t>(dims), true, s);
-    } else if (tb) {
-      a.ent = tb_ent.get();
-      cap_err = launch_grad_adam_tb(a, tb_off.get(), tb_part.get(), static_cast<uint32_t>(dims), true, s);
-    } else {
-      cap_err = launch_grad_adam(a, static_cast<uint32_t>(dims), true, s);
-    }
   }
-  const hipError_t end_err = hipStreamEndCapture(s, &gg.g);
-  if (cap_err != hipSuccess) fail_gpu(cap_err, "grad_adam launch during capture", __FILE__, __LINE__);
-  if (end_err != hipSuccess) fail_gpu(end_err, "hipStreamEndCapture", __FILE__, __LINE__);
-  PM_HIP(hipGraphInstantiate(&gg.e, gg.g, nullptr, nullptr, 0));
+  std::vector<uint64_t> key;
+  {
+    KeyWriter kw{key};
+    kw.add(static_cast<uint64_t>(sell ? 2 : (tb ? 1 : 0))), kw.add(dims), kw.add(T);
+    kw.add(static_cast<uint64_t>(sp.ranges)), kw.add(static_cast<uint64_t>(sp.group)),
+        kw.add(static_cast<uint64_t>(sp.span)), kw.add(static_cast<uint64_t>(sp.rc)),
+        kw.add(static_cast<uint64_t>(sp.chunks)), kw.add(static_cast<uint64_t>(sp.smax)),
+        kw.add(static_cast<uint64_t>(sp.words));
+    kw.add(sell_perm.get()), kw.add(sell_off.get()), kw.add(sell_ent.get()), kw.add(tb_off.get()),
+        kw.add(tb_part.get());
+    for (const GradArgs &a : plan) kw.add(a);
+  }
+  if (!(g_graph.e && g_graph.key == key)) {
+    g_graph.reset();
+    PM_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    hipError_t cap_err = hipSuccess;
+    for (uint64_t it = 0; it < T && cap_err == hipSuccess; ++it) {
+      const GradArgs &a = plan[it];
+      if (sell) {
+        cap_err = launch_grad_adam_sell(a, sp, sell_perm.get(), sell_off.get(), sell_ent.get(), tb_part.get(),
+                                        static_cast<uint32_t>(dims), true, s);
+      } else if (tb) {
+        cap_err = launch_grad_adam_tb(a, tb_off.get(), tb_part.get(), static_cast<uint32_t>(dims), true, s);
+      } else {
+        cap_err = launch_grad_adam(a, static_cast<uint32_t>(dims), true, s);
+      }
+    }
+    const hipError_t end_err = hipStreamEndCapture(s, &g_graph.g);
+    if (cap_err != hipSuccess) fail_gpu(cap_err, "grad_adam launch during capture", __FILE__, __LINE__);
+    if (end_err != hipSuccess) fail_gpu(end_err, "hipStreamEndCapture", __FILE__, __LINE__);
+    PM_HIP(hipGraphInstantiate(&g_graph.e, g_graph.g, nullptr, nullptr, 0));
+    g_graph.key = std::move(key);
+    g_tm.graph_reused = 0;
+  } else {
+    g_tm.graph_reused = 1;
+  }
   g_tm.graph_build_ms += now_ms() - tg;
   Timer opt;
   opt.start(s);
-  PM_HIP(hipGraphLaunch(gg.e, s));
+  PM_HIP(hipGraphLaunch(g_graph.e, s));
   opt.stop(s);
   if (record_loss) PM_HIP(launch_loss_reduce(part.get(), nb, static_cast<uint32_t>(T), d_loss.get(), s));
   download(y.data(), ybuf.get() + (T % 2) * nd, nd, s);

@@ -51,25 +51,56 @@ size_t dev_size_class(size_t bytes);
 void *dev_acquire(size_t cls);        // cls from dev_size_class
 void dev_recycle(void *p, size_t cls);
 void dev_trim();                      // frees this thread's cached blocks
+void graph_cache_reset();             // drops this thread's cached optimiser graph
+
+// Buffers of the optimiser graph keep their own block from call to call
+// (a slot per role), so identical calls launch identical kernel arguments
+// and the graph can be replayed.  A slot block of the wrong size class goes
+// back to the general cache.
+enum Slot : int {
+  kSlotNone = -1,
+  kSlotCsrRow,
+  kSlotCsrEnt,
+  kSlotY,
+  kSlotM,
+  kSlotV,
+  kSlotLossPart,
+  kSlotSnap,
+  kSlotTbEnt,
+  kSlotTbOff,
+  kSlotTbPart,
+  kSlotSellPerm,
+  kSlotSellOff,
+  kSlotSellEnt,
+  kSlotCount
+};
+void *dev_acquire_slot(int slot, size_t cls);
+void dev_release_slot(int slot, void *p, size_t cls);
 
 template <class T>
 class DevBuf {
  public:
   DevBuf() = default;
-  explicit DevBuf(size_t count) { alloc(count); }
+  explicit DevBuf(size_t count, int slot = kSlotNone) { alloc(count, slot); }
   DevBuf(const DevBuf &) = delete;
   DevBuf &operator=(const DevBuf &) = delete;
   ~DevBuf() { release(); }
-  void alloc(size_t count) {
+  void alloc(size_t count, int slot = kSlotNone) {
     release();
     n_ = count;
+    slot_ = slot;
     if (count) {
       cls_ = dev_size_class(count * sizeof(T));
-      p_ = static_cast<T *>(dev_acquire(cls_));
+      p_ = static_cast<T *>(slot >= 0 ? dev_acquire_slot(slot, cls_) : dev_acquire(cls_));
     }
   }
   void release() {
-    if (p_) dev_recycle(p_, cls_);
+    if (p_) {
+      if (slot_ >= 0)
+        dev_release_slot(slot_, p_, cls_);
+      else
+        dev_recycle(p_, cls_);
+    }
     p_ = nullptr;
     n_ = 0;
   }
@@ -81,6 +112,7 @@ class DevBuf {
   T *p_ = nullptr;
   size_t n_ = 0;
   size_t cls_ = 0;
+  int slot_ = kSlotNone;
 };
 
 class Stream {

@@ -1,4 +1,5 @@
 // pool.cpp -- per-thread cache of device memory blocks (see host_common.h).
+#include <iterator>
 #include <map>
 
 #include "host_common.h"
@@ -8,8 +9,14 @@ namespace {
 
 struct Pool {
   std::multimap<size_t, void *> free_blocks;
+  struct Kept {
+    size_t cls = 0;
+    void *p = nullptr;
+  } slots[kSlotCount];
   ~Pool() {
     for (auto &kv : free_blocks) (void)hipFree(kv.second);
+    for (auto &k : slots)
+      if (k.p) (void)hipFree(k.p);
   }
 };
 
@@ -30,10 +37,14 @@ size_t dev_size_class(size_t bytes) {
   return (bytes + step - 1) / step * step;
 }
 
+// Last in, first out within a class: a call that allocates and frees in the
+// same order as the previous one gets the same blocks back (which also lets
+// the optimiser graph of that call be replayed).
 void *dev_acquire(size_t cls) {
   auto &fb = pool().free_blocks;
-  auto it = fb.find(cls);
-  if (it != fb.end()) {
+  const auto range = fb.equal_range(cls);
+  if (range.first != range.second) {
+    const auto it = std::prev(range.second);
     void *p = it->second;
     fb.erase(it);
     return p;
@@ -51,8 +62,36 @@ void *dev_acquire(size_t cls) {
 
 void dev_recycle(void *p, size_t cls) { pool().free_blocks.emplace(cls, p); }
 
+void *dev_acquire_slot(int slot, size_t cls) {
+  auto &k = pool().slots[slot];
+  if (k.p && k.cls == cls) {
+    void *p = k.p;
+    k.p = nullptr;
+    return p;
+  }
+  if (k.p) dev_recycle(k.p, k.cls);
+  k.p = nullptr;
+  return dev_acquire(cls);
+}
+
+void dev_release_slot(int slot, void *p, size_t cls) {
+  auto &k = pool().slots[slot];
+  if (k.p) {  // the slot already holds a block: this one joins the general cache
+    dev_recycle(p, cls);
+    return;
+  }
+  k.p = p;
+  k.cls = cls;
+}
+
 void dev_trim() {
-  auto &fb = pool().free_blocks;
+  auto &pl = pool();
+  for (auto &k : pl.slots)
+    if (k.p) {
+      pl.free_blocks.emplace(k.cls, k.p);
+      k.p = nullptr;
+    }
+  auto &fb = pl.free_blocks;
   if (fb.empty()) return;
   (void)hipDeviceSynchronize();
   for (auto &kv : fb) (void)hipFree(kv.second);
@@ -61,4 +100,7 @@ void dev_trim() {
 
 }  // namespace pm
 
-extern "C" void pacmap_release_cached_memory(void) { pm::dev_trim(); }
+extern "C" void pacmap_release_cached_memory(void) {
+  pm::graph_cache_reset();
+  pm::dev_trim();
+}

@@ -412,6 +412,29 @@ def test_optimize_lds_sliced_matches_target_blocked(monkeypatch, n, dims):
     assert pm.last_timings()["grad_sell"] == 0
 
 
+def test_optimizer_graph_replayed_only_for_identical_launches():
+    """A call whose launches match the previous call's replays its graph; the
+    results equal a freshly captured run, and any change recaptures."""
+    y0, nn, mn, fp = _problem(66, n=2500)
+    it = (5, 5, 5)
+    pm.release_cached_memory()
+    a = pm.optimize(y0, nn, mn, fp, 1.0, it)
+    assert pm.last_timings()["graph_reused"] == 0
+    b = pm.optimize(y0, nn, mn, fp, 1.0, it)
+    assert pm.last_timings()["graph_reused"] == 1
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[2], b[2])
+    c = pm.optimize(y0, nn, mn, fp, 0.5, it)  # another learning rate: other Adam scalars
+    assert pm.last_timings()["graph_reused"] == 0
+    assert not np.array_equal(c[0], a[0])
+    y1 = (y0 * 2.0).astype(np.float32)  # same launches, other inputs
+    d = pm.optimize(y1, nn, mn, fp, 0.5, it)
+    assert pm.last_timings()["graph_reused"] == 1
+    pm.release_cached_memory()
+    e = pm.optimize(y1, nn, mn, fp, 0.5, it)
+    assert pm.last_timings()["graph_reused"] == 0
+    assert np.array_equal(d[0], e[0])
+
+
 def test_optimize_full_schedule_deterministic():
     y0, nn, mn, fp = _problem(61, n=2000)
     a = pm.optimize(y0, nn, mn, fp, 1.0, (100, 100, 250))
