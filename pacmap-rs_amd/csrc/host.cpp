@@ -507,8 +507,9 @@ void pca_axes_dev(const float *x_dev, uint64_t n, uint32_t d, uint32_t q, PcaAxe
   DevBuf<double> V(static_cast<size_t>(d) * q);
   DevBuf<uint32_t> status(1);
   PM_HIP(launch_fill_zero(status.get(), status.bytes(), s));
+  DevBuf<double> jwork(jacobi_work_doubles(std::min(d, jmax)));
   if (d <= jmax) {
-    PM_HIP(launch_jacobi(d_cov.get(), d, q, V.get(), s));
+    PM_HIP(launch_jacobi(d_cov.get(), d, q, V.get(), jwork.get(), s));
   } else {
     const uint32_t b = std::min<uint32_t>(std::min<uint32_t>(d, q + 12), jmax);
     if (q > b) fail(PACMAP_ERR_PCA, "too many principal components requested");
@@ -547,7 +548,7 @@ void pca_axes_dev(const float *x_dev, uint64_t n, uint32_t d, uint32_t q, PcaAxe
     // Rayleigh-Ritz: T = Q^T C Q, eigenvectors U, axes V = Q U
     gemm(d, b, d, d_cov.get(), d, false, Q.get(), b, Z.get(), b);
     gemm(b, b, d, Q.get(), b, true, Z.get(), b, G.get(), b);
-    PM_HIP(launch_jacobi(G.get(), b, q, U.get(), s));
+    PM_HIP(launch_jacobi(G.get(), b, q, U.get(), jwork.get(), s));
     gemm(d, q, b, Q.get(), b, false, U.get(), q, V.get(), q);
   }
   out.Vf.alloc(static_cast<size_t>(d) * q);

@@ -189,8 +189,10 @@ hipError_t launch_gemm64(uint32_t M, uint32_t N, uint32_t K, const double *A, ui
 hipError_t launch_chol_inv(const double *G, uint32_t b, double *Rinv, uint32_t *status, hipStream_t s);
 // q leading eigenvectors of symmetric A (m x m, m <= jacobi_max_size()) as
 // columns of Vout (m x q)
+// (work: jacobi_work_doubles(m) doubles of scratch)
 uint32_t jacobi_max_size();
-hipError_t launch_jacobi(const double *A, uint32_t m, uint32_t q, double *Vout, hipStream_t s);
+uint64_t jacobi_work_doubles(uint32_t m);
+hipError_t launch_jacobi(const double *A, uint32_t m, uint32_t q, double *Vout, double *work, hipStream_t s);
 // column signs of V (d x q) and its FP32 copy Vf
 hipError_t launch_sign_fix(double *V, uint32_t d, uint32_t q, float *Vf, hipStream_t s);
 

@@ -199,26 +199,33 @@ __global__ __launch_bounds__(512) void chol_inv_kernel(const double *__restrict_
 }
 
 // ---------------------------------------------------------------------------
-// Jacobi eigensolver for a symmetric m x m matrix (m <= 112), one workgroup.
+// Jacobi eigensolver for a symmetric m x m matrix (m <= 112).
 //
-// The matrix is kept in LDS as its packed upper triangle and the
-// eigenvectors as V^T (row j = eigenvector j), also in LDS.  Parallel
-// (round-robin) ordering: each round applies the h = ceil(m/2) disjoint
-// rotations of that round at once, as J^T A J: for every two rotation pairs
-// i < j the 2 x 2 block of A between them is updated with both rotations,
-// each pair's own 2 x 2 diagonal block with the closed form (app - t apq,
-// aqq + t apq, 0), and V^T's rows p, q with the rotation.  The per-thread
-// work list (which block or V column each slot handles) is the same every
-// round, so it is decoded once.  Sweeps stop when the off-diagonal square
-// sum is below 1e-24 of the diagonal one (off-diagonal norm 1e-12 relative).
+// Rotations (one workgroup): the matrix is kept in LDS as its packed upper
+// triangle.  Parallel (round-robin) ordering: each round applies the
+// h = ceil(m/2) disjoint rotations of that round at once, as J^T A J: for
+// every two rotation pairs i < j the 2 x 2 block of A between them is
+// updated with both rotations, and each pair's own 2 x 2 diagonal block with
+// the closed form (app - t apq, aqq + t apq, 0).  The per-thread work list is
+// the same every round, so it is decoded once.  Sweeps stop when the
+// off-diagonal square sum is below 1e-24 of the diagonal one (off-diagonal
+// norm 1e-12 relative).  Every round's rotations (p, q, c, s) are recorded.
+//
+// Eigenvectors (one wave per component): V^T starts as the identity and every
+// recorded rotation turns its rows p and q.  Column k of V^T only ever mixes
+// with itself, so each wave replays all rounds on one column held in LDS,
+// with the rounds' records loaded ahead.  The rotation workgroup is left
+// with A alone, which takes 1/5 of the LDS traffic of updating both.
+//
 // Output: the q eigenvectors of largest eigenvalue (ties by index) as columns
 // of Vout (m x q), each with its largest-magnitude entry positive.
 // ---------------------------------------------------------------------------
 constexpr int kJacMax = 112;
 constexpr int kJacPairs = kJacMax / 2;
 constexpr int kJacThreads = 1024;
-constexpr int kJacSlots = 8;  // work slots per thread per round
-static_assert(kJacPairs * (kJacPairs - 1) / 2 + kJacPairs + kJacPairs * kJacMax <= kJacThreads * kJacSlots,
+constexpr int kJacSlots = 2;  // work slots per thread per round
+constexpr int kJacSweeps = 40;
+static_assert(kJacPairs * (kJacPairs - 1) / 2 + kJacPairs <= kJacThreads * kJacSlots,
               "Jacobi work list does not fit the slots");
 
 __device__ __forceinline__ uint32_t packed_idx(uint32_t r, uint32_t c, uint32_t m) {
@@ -230,59 +237,62 @@ __device__ __forceinline__ uint32_t packed_idx(uint32_t r, uint32_t c, uint32_t 
   return r * m - (r * (r - 1)) / 2 + (c - r);  // row r of the upper triangle starts at r*m - r(r-1)/2
 }
 
-__global__ __launch_bounds__(kJacThreads) void jacobi_kernel(const double *__restrict__ Ain, uint32_t m,
-                                                             uint32_t q, double *__restrict__ Vout) {
+struct JacobiWork {
+  double2 *cs;        // rounds x h rotations (c, s)
+  uint32_t *pq;       // rounds x h: p | q << 16 (q >= m: the dummy index, no rotation)
+  double *evals;      // m
+  double *vt;         // m x m, V^T
+  uint32_t *rounds;   // 1
+};
+
+__global__ __launch_bounds__(kJacThreads) void jacobi_rot_kernel(const double *__restrict__ Ain, uint32_t m,
+                                                                 JacobiWork w) {
   __shared__ double Ap[kJacMax * (kJacMax + 1) / 2];
-  __shared__ double Vt[kJacMax * kJacMax];
   __shared__ double cs[kJacPairs], sn[kJacPairs], tn[kJacPairs];
   __shared__ int pp[kJacPairs], qq[kJacPairs];
   __shared__ double red[2][kJacThreads / 64];
-  __shared__ int rank[kJacMax];
   const int t = threadIdx.x;
   const uint32_t M = (m + 1) & ~1u;  // even count; index m is a dummy when m is odd
   const uint32_t h = M / 2;
   for (uint32_t e = t; e < m * m; e += kJacThreads) {
     const uint32_t r = e / m, c = e % m;
     if (r <= c) Ap[packed_idx(r, c, m)] = 0.5 * (Ain[e] + Ain[static_cast<size_t>(c) * m + r]);
-    Vt[e] = (r == c) ? 1.0 : 0.0;
   }
   // Work list: blocks (i < j) enumerated as (x, (x + s) mod h), then the h
-  // diagonal blocks, then the h x m V^T updates.  Code: type << 24 | a << 12 | b.
+  // diagonal blocks.  Code: type << 24 | a << 12 | b.
   const uint32_t hs = h / 2;
   const uint32_t full = (h & 1) ? hs : (hs > 0 ? hs - 1 : 0);
   const uint32_t nb = h * (h - 1) / 2;
-  const uint32_t total = nb + h + h * m;
+  const uint32_t total = nb + h;
   uint32_t slot[kJacSlots];
 #pragma unroll
   for (int k = 0; k < kJacSlots; ++k) {
-    const uint32_t w = t + k * kJacThreads;
+    const uint32_t wi = t + k * kJacThreads;
     uint32_t code = 0xffffffffu;
-    if (w < nb) {
-      uint32_t s, x;
-      if (w < full * h) {
-        s = 1 + w / h;
-        x = w % h;
+    if (wi < nb) {
+      uint32_t sft, x;
+      if (wi < full * h) {
+        sft = 1 + wi / h;
+        x = wi % h;
       } else {
-        s = hs;
-        x = w - full * h;
+        sft = hs;
+        x = wi - full * h;
       }
-      uint32_t i = x, j = (x + s) % h;
+      uint32_t i = x, j = (x + sft) % h;
       if (i > j) {
         const uint32_t y = i;
         i = j;
         j = y;
       }
       code = (0u << 24) | (i << 12) | j;
-    } else if (w < nb + h) {
-      code = (1u << 24) | ((w - nb) << 12);
-    } else if (w < total) {
-      const uint32_t u = w - nb - h;
-      code = (2u << 24) | ((u / m) << 12) | (u % m);
+    } else if (wi < total) {
+      code = (1u << 24) | ((wi - nb) << 12);
     }
     slot[k] = code;
   }
   __syncthreads();
-  for (int sweep = 0; sweep < 40; ++sweep) {
+  uint32_t grnd = 0;  // rounds recorded
+  for (int sweep = 0; sweep < kJacSweeps; ++sweep) {
     double off = 0.0, dg = 0.0;
     for (uint32_t r = 0; r < m; ++r) {  // thread t takes columns c = t, t + 1024, ... of every row
       const uint32_t base = r * m - (r * (r - 1)) / 2;
@@ -302,12 +312,12 @@ __global__ __launch_bounds__(kJacThreads) void jacobi_kernel(const double *__res
     }
     __syncthreads();
     double offs = 0.0, dgs = 0.0;
-    for (int w = 0; w < kJacThreads / 64; ++w) {
-      offs += red[0][w];
-      dgs += red[1][w];
+    for (int wv = 0; wv < kJacThreads / 64; ++wv) {
+      offs += red[0][wv];
+      dgs += red[1][wv];
     }
     if (offs == 0.0 || offs <= 1e-24 * dgs) break;  // uniform: every thread summed the same values
-    for (uint32_t round = 0; round + 1 < M; ++round) {
+    for (uint32_t round = 0; round + 1 < M; ++round, ++grnd) {
       if (static_cast<uint32_t>(t) < h) {
         auto pos = [&](uint32_t j) -> uint32_t { return j == 0 ? 0 : 1 + (j - 1 + round) % (M - 1); };
         uint32_t p = pos(t), r2 = pos(M - 1 - t);
@@ -331,6 +341,9 @@ __global__ __launch_bounds__(kJacThreads) void jacobi_kernel(const double *__res
         cs[t] = c;
         sn[t] = s;
         tn[t] = tt;
+        const size_t rec = static_cast<size_t>(grnd) * h + t;
+        w.cs[rec] = make_double2(c, s);
+        w.pq[rec] = p | (r2 << 16);
       }
       __syncthreads();
 #pragma unroll
@@ -355,7 +368,7 @@ __global__ __launch_bounds__(kJacThreads) void jacobi_kernel(const double *__res
           if (h2) Ap[i01] = s2 * r00 + c2 * r01;
           if (h1) Ap[i10] = c2 * r10 - s2 * r11;
           if (h1 && h2) Ap[i11] = s2 * r10 + c2 * r11;
-        } else if (type == 1) {
+        } else {
           const uint32_t p = pp[a], r2 = qq[a];
           const double tt = tn[a];
           if (r2 < m && tt != 0.0) {
@@ -365,43 +378,94 @@ __global__ __launch_bounds__(kJacThreads) void jacobi_kernel(const double *__res
             Ap[iqq] = Ap[iqq] + tt * apq;
             Ap[ipq] = 0.0;
           }
-        } else {
-          const uint32_t p = pp[a], r2 = qq[a];
-          if (r2 < m) {
-            const double c = cs[a], s = sn[a];
-            const double x = Vt[p * m + bb], y = Vt[r2 * m + bb];
-            Vt[p * m + bb] = c * x - s * y;
-            Vt[r2 * m + bb] = s * x + c * y;
-          }
         }
       }
       __syncthreads();
     }
   }
-  // rank by eigenvalue (descending, ties by index)
-  for (uint32_t j = t; j < m; j += kJacThreads) {
-    const double wj = Ap[packed_idx(j, j, m)];
-    int r = 0;
-    for (uint32_t i = 0; i < m; ++i) {
-      const double wi = Ap[packed_idx(i, i, m)];
-      r += (wi > wj || (wi == wj && i < j)) ? 1 : 0;
+  for (uint32_t j = t; j < m; j += kJacThreads) w.evals[j] = Ap[packed_idx(j, j, m)];
+  if (t == 0) w.rounds[0] = grnd;
+}
+
+// One wave per column k of V^T: replays every recorded round on it.
+constexpr int kJacChunk = 8;  // rounds whose records are loaded ahead
+__global__ __launch_bounds__(64) void jacobi_vec_kernel(uint32_t m, JacobiWork w) {
+  __shared__ double x[kJacMax + 1];
+  const uint32_t k = blockIdx.x, lane = threadIdx.x;
+  const uint32_t M = (m + 1) & ~1u, h = M / 2;
+  for (uint32_t j = lane; j <= m; j += 64) x[j] = (j == k) ? 1.0 : 0.0;
+  const uint32_t R = w.rounds[0];
+  const uint32_t last = R * h;  // records
+  const bool act = lane < h;
+  struct Chunk {
+    double2 cs[kJacChunk];
+    uint32_t pq[kJacChunk];
+  };
+  auto load = [&](uint32_t r0, Chunk &ch) {
+#pragma unroll
+    for (int i = 0; i < kJacChunk; ++i) {
+      const uint32_t rec = min((r0 + i) * h + lane, last ? last - 1 : 0u);  // clamped; unused past R
+      ch.cs[i] = w.cs[rec];
+      ch.pq[i] = w.pq[rec];
     }
-    rank[j] = r;
-  }
-  __syncthreads();
-  for (uint32_t j = t; j < m; j += kJacThreads) {
-    const uint32_t col = static_cast<uint32_t>(rank[j]);
-    if (col >= q) continue;
-    double best = -1.0, sg = 1.0;
-    for (uint32_t k = 0; k < m; ++k) {
-      const double v = Vt[j * m + k];
-      if (fabs(v) > best) {
-        best = fabs(v);
-        sg = v < 0 ? -1.0 : 1.0;
+  };
+  auto run = [&](uint32_t r0, const Chunk &ch) {
+#pragma unroll
+    for (int i = 0; i < kJacChunk; ++i) {
+      if (r0 + i >= R) break;
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      if (act) {
+        const uint32_t p = ch.pq[i] & 0xffffu, q = ch.pq[i] >> 16;
+        if (q < m) {
+          const double c = ch.cs[i].x, s = ch.cs[i].y;
+          const double xp = x[p], xq = x[q];
+          x[p] = c * xp - s * xq;
+          x[q] = s * xp + c * xq;
+        }
       }
     }
-    for (uint32_t k = 0; k < m; ++k) Vout[static_cast<size_t>(k) * q + col] = sg * Vt[j * m + k];
+  };
+  Chunk ca, cb;
+  if (R) load(0, ca);
+  for (uint32_t r0 = 0; r0 < R;) {
+    load(r0 + kJacChunk, cb);
+    run(r0, ca);
+    r0 += kJacChunk;
+    if (r0 >= R) break;
+    load(r0 + kJacChunk, ca);
+    run(r0, cb);
+    r0 += kJacChunk;
   }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  for (uint32_t j = lane; j < m; j += 64) w.vt[static_cast<size_t>(j) * m + k] = x[j];
+}
+
+// rank by eigenvalue (descending, ties by index), sign, and write Vout
+__global__ __launch_bounds__(128) void jacobi_out_kernel(uint32_t m, uint32_t q, JacobiWork w,
+                                                         double *__restrict__ Vout) {
+  const uint32_t j = threadIdx.x;
+  if (j >= m) return;
+  const double wj = w.evals[j];
+  uint32_t col = 0;
+  for (uint32_t i = 0; i < m; ++i) {
+    const double wi = w.evals[i];
+    col += (wi > wj || (wi == wj && i < j)) ? 1u : 0u;
+  }
+  if (col >= q) return;
+  const double *v = w.vt + static_cast<size_t>(j) * m;
+  double best = -1.0, sg = 1.0;
+  for (uint32_t k = 0; k < m; ++k) {
+    const double e = v[k];
+    if (fabs(e) > best) {
+      best = fabs(e);
+      sg = e < 0 ? -1.0 : 1.0;
+    }
+  }
+  for (uint32_t k = 0; k < m; ++k) Vout[static_cast<size_t>(k) * q + col] = sg * v[k];
 }
 
 // Column signs of V (d x q): largest-magnitude entry positive (first such
@@ -452,9 +516,26 @@ hipError_t launch_chol_inv(const double *G, uint32_t b, double *Rinv, uint32_t *
   return hipGetLastError();
 }
 
-hipError_t launch_jacobi(const double *A, uint32_t m, uint32_t q, double *Vout, hipStream_t s) {
+static uint64_t jacobi_records(uint32_t m) {
+  const uint64_t M = (m + 1) & ~1u;
+  return static_cast<uint64_t>(kJacSweeps) * (M - 1) * (M / 2);
+}
+uint64_t jacobi_work_doubles(uint32_t m) {
+  return 2 * jacobi_records(m) + (jacobi_records(m) + 1) / 2 + static_cast<uint64_t>(m) * m + m + 1;
+}
+
+hipError_t launch_jacobi(const double *A, uint32_t m, uint32_t q, double *Vout, double *work, hipStream_t s) {
   if (m == 0 || m > kJacMax || q > m) return hipErrorInvalidValue;
-  jacobi_kernel<<<1, kJacThreads, 0, s>>>(A, m, q, Vout);
+  const uint64_t nr = jacobi_records(m);
+  JacobiWork w;
+  w.cs = reinterpret_cast<double2 *>(work);
+  w.pq = reinterpret_cast<uint32_t *>(work + 2 * nr);
+  w.vt = work + 2 * nr + (nr + 1) / 2;
+  w.evals = w.vt + static_cast<size_t>(m) * m;
+  w.rounds = reinterpret_cast<uint32_t *>(w.evals + m);
+  jacobi_rot_kernel<<<1, kJacThreads, 0, s>>>(A, m, w);
+  jacobi_vec_kernel<<<m, 64, 0, s>>>(m, w);
+  jacobi_out_kernel<<<1, 128, 0, s>>>(m, q, w, Vout);
   return hipGetLastError();
 }
 
