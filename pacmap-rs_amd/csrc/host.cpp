@@ -395,7 +395,7 @@ void optimize_device(const DevCsr &csr, uint64_t n, uint64_t dims, std::vector<f
   }
   g_tm.grad_target_blocks = tb ? kTargetBlocks : 0;
   g_tm.grad_sell = sell ? 1 : 0;
-  const uint32_t nb = sell ? sell_blocks(sp)
+  const uint32_t nb = sell ? sell_grad_blocks(sp)
                            : (tb ? grad_tb_blocks(static_cast<uint32_t>(n)) : grad_blocks(static_cast<uint32_t>(n)));
   DevBuf<float> part(record_loss ? T * nb : 0, kSlotLossPart), d_loss(record_loss ? T : 0);
   PM_HIP(hipStreamSynchronize(s));

@@ -128,7 +128,7 @@ hipError_t launch_grad_adam_tb(const GradArgs &a, const uint16_t *off, float *pa
 // once per fit from the tb_partition output: sort (perm: kTargetBlocks x n,
 // ssz: sell_slices() slice sizes), exclusive scan of ssz into soff (+1
 // total), fill (sent: soff[last] words).  Each iteration runs grad_sell
-// (loss_part: sell_blocks() floats) and a combine+Adam pass over the
+// (loss_part: sell_grad_blocks() floats) and a combine+Adam pass over the
 // ranges' partials.
 struct SellPlan {
   uint32_t ranges = 0;     // target ranges: 8 (the tb ranges) or 4 (tb ranges merged in pairs)
@@ -142,7 +142,8 @@ struct SellPlan {
 };
 bool sell_plan(uint32_t n, uint32_t dims, uint32_t ranges, SellPlan &p);
 uint64_t sell_slices(const SellPlan &p);
-uint32_t sell_blocks(const SellPlan &p);
+uint32_t sell_blocks(const SellPlan &p);       // sort pass workgroups
+uint32_t sell_grad_blocks(const SellPlan &p);  // gradient pass workgroups (loss_part entries)
 hipError_t launch_sell_sort(const uint32_t *row_ptr, const uint16_t *off, uint32_t n, const SellPlan &p,
                             uint32_t *perm, uint32_t *ssz, hipStream_t s);
 hipError_t launch_sell_fill(const uint32_t *row_ptr, const uint32_t *tb_ent, const uint16_t *off, uint32_t n,

@@ -863,8 +863,15 @@ __global__ __launch_bounds__(256) void grad_tb_combine_kernel(const GradArgs a, 
 // dims = 2, half the dependent work per thread).
 template <bool ADAM, int RANGES>
 __global__ __launch_bounds__(256) void grad_combine_elem_kernel(const GradArgs a, const float *__restrict__ part,
-                                                                uint64_t count) {
-  const uint64_t e = blockIdx.x * 256ull + threadIdx.x;
+                                                                uint64_t count, uint32_t chunk_elems,
+                                                                uint32_t per_chunk) {
+  // workgroup -> (chunk c, piece j) with c % 8 = blockIdx.x % 8 (the XCD
+  // that wrote the chunk's partials)
+  const uint32_t x = blockIdx.x % 8, k = blockIdx.x / 8;
+  const uint32_t j = k % per_chunk, c = (k / per_chunk) * 8 + x;
+  const uint32_t off = j * 256 + threadIdx.x;
+  if (off >= chunk_elems) return;
+  const uint64_t e = static_cast<uint64_t>(c) * chunk_elems + off;
   if (e >= count) return;
   float pv[RANGES];  // all loads in flight before the ordered sum
 #pragma unroll
@@ -1000,6 +1007,16 @@ bool sell_plan(uint32_t n, uint32_t dims, uint32_t ranges, SellPlan &p) {
 }
 uint64_t sell_slices(const SellPlan &p) { return static_cast<uint64_t>(p.ranges) * p.chunks * p.smax; }
 uint32_t sell_blocks(const SellPlan &p) { return p.ranges * p.chunks; }
+// The gradient pass runs every range of chunk c on XCD c % 8 (workgroups
+// are dealt to the XCDs round-robin by index), so a chunk's range partials
+// are all written to one L2, where the combine pass of the chunk (placed the
+// same way) reads them.  Chunks are padded to a multiple of 8.
+uint32_t sell_grad_blocks(const SellPlan &p) { return p.ranges * ((p.chunks + 7) / 8 * 8); }
+__device__ __forceinline__ void sell_grad_block(const SellPlan &p, uint32_t blk, uint32_t &b, uint32_t &c) {
+  const uint32_t x = blk % 8, k = blk / 8;
+  b = k % p.ranges;
+  c = (k / p.ranges) * 8 + x;
+}
 
 // One workgroup per (range, chunk): bitonic sort of (~len << 32 | slot) in
 // LDS; writes the chunk's row order and each slice's entry count (16 x the
@@ -1122,7 +1139,12 @@ __global__ __launch_bounds__(kSellThreads) void grad_sell_kernel(const GradArgs 
                                                                  const uint32_t *__restrict__ sent,
                                                                  float *__restrict__ part) {
   extern __shared__ __align__(16) float lds[];
-  const uint32_t b = blockIdx.x % p.ranges, c = blockIdx.x / p.ranges;
+  uint32_t b, c;
+  sell_grad_block(p, blockIdx.x, b, c);
+  if (c >= p.chunks) {  // padding workgroup
+    if (LOSS && threadIdx.x == 0) a.loss_part[blockIdx.x] = 0.0f;
+    return;
+  }
   const uint32_t n = a.n;
   const uint32_t lo = b * p.span;
   const uint32_t nlo = lo < n ? min(p.span, n - lo) : 0u;
@@ -1255,7 +1277,7 @@ template <int DIMS>
 static hipError_t launch_sell_dims(const GradArgs &a, const SellPlan &p, const uint32_t *perm,
                                    const uint32_t *soff, const uint32_t *sent, float *part, bool adam,
                                    hipStream_t s) {
-  const uint32_t g1 = sell_blocks(p);
+  const uint32_t g1 = sell_grad_blocks(p);
   if (g1 == 0) return hipSuccess;
   const size_t shm = (static_cast<size_t>(p.span) + p.rc) * DIMS * sizeof(float);
   const bool loss = a.loss_part != nullptr;
@@ -1271,17 +1293,18 @@ static hipError_t launch_sell_dims(const GradArgs &a, const SellPlan &p, const u
       grad_sell_kernel<DIMS, false, 4><<<g1, kSellThreads, shm, s>>>(a, p, perm, soff, sent, part);
   }
   const uint64_t count = static_cast<uint64_t>(a.n) * DIMS;
-  const uint32_t g2 = static_cast<uint32_t>((count + 255) / 256);
+  const uint32_t chunk_elems = p.rc * DIMS, per_chunk = (chunk_elems + 255) / 256;
+  const uint32_t g2 = per_chunk * ((p.chunks + 7) / 8 * 8);
   if (p.ranges == 4) {
     if (adam)
-      grad_combine_elem_kernel<true, 4><<<g2, 256, 0, s>>>(a, part, count);
+      grad_combine_elem_kernel<true, 4><<<g2, 256, 0, s>>>(a, part, count, chunk_elems, per_chunk);
     else
-      grad_combine_elem_kernel<false, 4><<<g2, 256, 0, s>>>(a, part, count);
+      grad_combine_elem_kernel<false, 4><<<g2, 256, 0, s>>>(a, part, count, chunk_elems, per_chunk);
   } else {
     if (adam)
-      grad_combine_elem_kernel<true, 8><<<g2, 256, 0, s>>>(a, part, count);
+      grad_combine_elem_kernel<true, 8><<<g2, 256, 0, s>>>(a, part, count, chunk_elems, per_chunk);
     else
-      grad_combine_elem_kernel<false, 8><<<g2, 256, 0, s>>>(a, part, count);
+      grad_combine_elem_kernel<false, 8><<<g2, 256, 0, s>>>(a, part, count, chunk_elems, per_chunk);
   }
   return hipGetLastError();
 }
