@@ -487,28 +487,57 @@ __global__ __launch_bounds__(256) void knn_rerank_kernel(
     uint32_t nsplit, const uint32_t *__restrict__ gcnt, const uint32_t *__restrict__ gcand,
     const float *__restrict__ gtau, double cert_scale, uint32_t *__restrict__ oidx,
     float *__restrict__ odist, uint32_t *__restrict__ fb_count, uint32_t *__restrict__ fb_list) {
-  // per wave: the query row (Dp floats), then 64 R keys
+  // per wave: the query row (Dp floats), 64 R keys, then 64 R candidate ids
   extern __shared__ __align__(16) float sQrow[];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const uint32_t q = blockIdx.x * 4 + wave;
-  float *qv = sQrow + static_cast<size_t>(wave) * (Dp + 128 * R);
+  float *qv = sQrow + static_cast<size_t>(wave) * (Dp + 192 * R);
   uint64_t *keys = reinterpret_cast<uint64_t *>(qv + Dp);
+  uint32_t *list = reinterpret_cast<uint32_t *>(qv + Dp + 128 * R);
   if (q < n)
     for (uint32_t f = lane; f < Dp; f += 64) qv[f] = xp[static_cast<size_t>(q) * Dp + f];
   __syncthreads();
   if (q >= n) return;
-  // the row's list is the concatenation of its per-split lists
-  uint32_t cnt = 0;
-  bool overflow = false;
-  for (uint32_t sp = 0; sp < nsplit; ++sp) {
-    const uint32_t c = gcnt[static_cast<size_t>(q) * nsplit + sp];
-    overflow = overflow || c > capg;
-    cnt += c;
+  // the row's list is the concatenation of its per-split lists: split sizes
+  // in lanes 0..nsplit-1, their exclusive prefix by a wave scan
+  const uint32_t myc = static_cast<uint32_t>(lane) < nsplit ? gcnt[static_cast<size_t>(q) * nsplit + lane] : 0u;
+  uint32_t incl = myc;
+#pragma unroll
+  for (int o = 1; o < 16; o <<= 1) {
+    const uint32_t v = __shfl_up(incl, o, 64);
+    if (lane >= o) incl += v;
   }
+  const uint32_t cnt = __shfl(incl, 15, 64);
+  const bool overflow = __any(myc > capg);
   const bool all_listed = cnt == n - 1;
   if (overflow || cnt > capg || cnt < k) {
     if (lane == 0) fb_list[atomicAdd(fb_count, 1u)] = q;
     return;
+  }
+  {
+    // gather the split lists into LDS: all loads of a 64-entry round first
+    uint32_t maxc = myc;
+#pragma unroll
+    for (int o = 1; o < 16; o <<= 1) maxc = max(maxc, __shfl_xor(maxc, o, 64));
+    maxc = __shfl(maxc, 0, 64);  // lanes 0..15 hold the split sizes: the wave's bound
+    for (uint32_t j0 = 0; j0 < maxc; j0 += 64) {
+      uint32_t v[16];
+#pragma unroll
+      for (int sp = 0; sp < 16; ++sp) {
+        const uint32_t csp = __shfl(myc, sp, 64);
+        v[sp] = static_cast<uint32_t>(sp) < nsplit && j0 + lane < csp
+                    ? gcand[(static_cast<size_t>(q) * nsplit + sp) * capg + j0 + lane]
+                    : 0u;
+      }
+#pragma unroll
+      for (int sp = 0; sp < 16; ++sp) {
+        const uint32_t csp = __shfl(myc, sp, 64), start = __shfl(incl, sp, 64) - csp;
+        if (static_cast<uint32_t>(sp) < nsplit && j0 + lane < csp) list[start + j0 + lane] = v[sp];
+      }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
   }
   // Eight lanes per candidate: lane l sums segment l (features 8c + l) in
   // order of c, the eight partials are added in lane order, then the tail, as
@@ -517,9 +546,7 @@ __global__ __launch_bounds__(256) void knn_rerank_kernel(
   const float *qseg = qv + static_cast<size_t>(l) * Cp;
   const uint32_t segEnd = 8u * Cp;
   for (uint32_t e = grp; e < cnt; e += 8) {
-    uint32_t sp = 0, off = e;
-    while (off >= gcnt[static_cast<size_t>(q) * nsplit + sp]) off -= gcnt[static_cast<size_t>(q) * nsplit + sp++];
-    const uint32_t c = gcand[(static_cast<size_t>(q) * nsplit + sp) * capg + off];
+    const uint32_t c = list[e];
     const float *cr = xp + static_cast<size_t>(c) * Dp;
     const float *cseg = cr + static_cast<size_t>(l) * Cp;
     float acc = 0.0f;
@@ -669,7 +696,8 @@ hipError_t launch_knn_rerank(const float *xp, const KnnLayout &L, uint32_t n, ui
                              hipStream_t s) {
   const uint32_t blocks = (n + 3) / 4;
   const uint32_t R = capg / 64;
-  const size_t shm = static_cast<size_t>(4) * (L.Dp + 128 * R) * sizeof(float);
+  const size_t shm = static_cast<size_t>(4) * (L.Dp + 192 * R) * sizeof(float);
+  if (nsplit > 16) return hipErrorInvalidValue;
   if (capg == 512)
     knn_rerank_kernel<8><<<blocks, 256, shm, s>>>(xp, L.Dp, L.Cp, n, k, capg, nsplit, gcnt, gcand,
                                                   gtau, cert_scale, idx, dist, fb_count, fb_list);
