@@ -129,8 +129,11 @@ bool knn_mfma_path(const float *x_dev, uint64_t n, uint64_t d, uint32_t k, const
   const uint32_t nfb = hflags[1];
   g_tm.knn_rescanned_rows += nfb;
   if (nfb) {
-    // split each listed row's candidates over S blocks, then merge
-    const uint32_t S = static_cast<uint32_t>(std::min<uint64_t>(64, std::max<uint64_t>(1, n / 2048)));
+    // split each listed row's candidates over S blocks (about 512 workgroups
+    // in all, at least 256 candidates each), then merge
+    const uint64_t qblocks = (nfb + 15) / 16;
+    const uint32_t S = static_cast<uint32_t>(
+        std::max<uint64_t>(1, std::min<uint64_t>({(512 + qblocks - 1) / qblocks, 128, n / 256})));
     DevBuf<uint64_t> part(static_cast<size_t>(nfb) * S * k);
     Timer tf;
     tf.start(s);

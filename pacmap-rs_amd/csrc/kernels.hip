@@ -374,12 +374,20 @@ __global__ __launch_bounds__(256) void knn_merge_kernel(const uint64_t *__restri
   uint64_t run[R];
 #pragma unroll
   for (int r = 0; r < R; ++r) run[r] = ~0ull;
+  // the next chunk's keys are loaded while the current one is merged
+  uint64_t nxt[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const uint32_t e = r * 64 + lane;
+    nxt[r] = e < total ? src[e] : ~0ull;
+  }
   for (uint32_t c0 = 0; c0 < total; c0 += N) {
     uint64_t b[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) {
-      const uint32_t e = c0 + r * 64 + lane;
-      b[r] = e < total ? src[e] : ~0ull;
+      b[r] = nxt[r];
+      const uint32_t e = c0 + N + r * 64 + lane;
+      nxt[r] = e < total ? src[e] : ~0ull;
     }
     wave_bitonic<R>(b, lane);
 #pragma unroll
