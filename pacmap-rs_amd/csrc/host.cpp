@@ -757,9 +757,12 @@ void fit_impl(const float *x_host, const float *x_dev_in, uint64_t n, uint64_t d
     tsmp.start(s);
     DevBuf<uint32_t> d_mn(std::max<uint64_t>(2, 2 * n * n_mn)), d_fp(std::max<uint64_t>(2, 2 * n * n_fp));
     PM_HIP(launch_fill_zero(d_fp.get(), d_fp.bytes(), s));
-    PM_HIP(launch_sample_pairs(d_xk.get(), static_cast<uint32_t>(n), static_cast<uint32_t>(dk), d_nn, knn,
-                               static_cast<uint32_t>(n_mn), static_cast<uint32_t>(n_fp), seed_base,
-                               cfg->has_seed ? 1 : 0, d_mn.get(), d_fp.get(), s));
+    const KnnLayout Ls = knn_layout(static_cast<uint32_t>(dk));
+    DevBuf<float> d_xs(n_mn ? n * Ls.Dp : 0);
+    if (n_mn) PM_HIP(launch_knn_pack(d_xk.get(), d_xs.get(), n, Ls, s));
+    PM_HIP(launch_sample_pairs(d_xs.get(), Ls, static_cast<uint32_t>(n), d_nn, knn, static_cast<uint32_t>(n_mn),
+                               static_cast<uint32_t>(n_fp), seed_base, cfg->has_seed ? 1 : 0, d_mn.get(),
+                               d_fp.get(), s));
     tsmp.stop(s);
     g_tm.sampling_ms += tsmp.ms();
     if (po) {
@@ -1024,7 +1027,10 @@ int pacmap_sample_pairs(const float *x, uint64_t n, uint64_t d, const uint32_t *
     upload(dx, x, n * d, s);
     upload(dnn, pair_nn, 2 * n * n_neighbors, s);
     PM_HIP(launch_fill_zero(dfp.get(), dfp.bytes(), s));
-    PM_HIP(launch_sample_pairs(dx.get(), static_cast<uint32_t>(n), static_cast<uint32_t>(d), dnn.get(),
+    const KnnLayout L = knn_layout(static_cast<uint32_t>(d));
+    DevBuf<float> dxs(n_mn ? n * L.Dp : 0);
+    if (n_mn) PM_HIP(launch_knn_pack(dx.get(), dxs.get(), n, L, s));
+    PM_HIP(launch_sample_pairs(dxs.get(), L, static_cast<uint32_t>(n), dnn.get(),
                                static_cast<uint32_t>(n_neighbors), static_cast<uint32_t>(n_mn),
                                static_cast<uint32_t>(n_fp), seed, 1, dmn.get(), dfp.get(), s));
     download(pair_mn_out, dmn.get(), 2 * n * n_mn, s);

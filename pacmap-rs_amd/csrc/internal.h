@@ -34,9 +34,10 @@ hipError_t launch_knn_exact(const float *xp, uint64_t n, const KnnLayout &L, uin
                             uint32_t *idx, float *dist, hipStream_t s,
                             const uint32_t *qlist = nullptr, uint32_t nq = 0);
 
-// Pair sampling on the GPU (pairs.hip); pair arrays are rows x 2 u32 and
-// pair_fp must be zeroed by the caller (unfilled far pairs stay (0, 0)).
-hipError_t launch_sample_pairs(const float *x, uint32_t n, uint32_t d, const uint32_t *pair_nn,
+// Pair sampling on the GPU (pairs.hip) from the points in the packed kNN
+// layout (launch_knn_pack); pair arrays are rows x 2 u32 and pair_fp must be
+// zeroed by the caller (unfilled far pairs stay (0, 0)).
+hipError_t launch_sample_pairs(const float *xp, const KnnLayout &L, uint32_t n, const uint32_t *pair_nn,
                                uint32_t knn, uint32_t n_mn, uint32_t n_fp, uint64_t seed, int seeded,
                                uint32_t *pair_mn, uint32_t *pair_fp, hipStream_t s);
 // Symmetrised CSR on the GPU: count degrees (flags out-of-range indices in

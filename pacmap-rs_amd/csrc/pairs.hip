@@ -124,44 +124,73 @@ __global__ void sample_fp_kernel(uint32_t n, const uint32_t *__restrict__ pair_n
 // sampling.rs:118-142 + 296-313: for each j, six distinct draws avoiding self
 // and the earlier picks, then the second closest by distance (ties keep draw
 // order; unfilled slots are (0.0, 0)).  Eight lanes per point: every lane
-// runs the point's random stream (so all eight agree on the draws), lane l
-// accumulates the distance partial of features 8c + l, and the partials are
-// added in lane order 0..7, then the tail, exactly as distance.rs:50-75.
-// Every lane writes the chosen index (same value) and reads back its own
-// writes as the reject list of the next draws.
-__global__ __launch_bounds__(256) void sample_mn_kernel(const float *__restrict__ x, uint32_t n, uint32_t d,
-                                                        uint32_t n_mn, uint64_t seed, int seeded,
-                                                        uint32_t *__restrict__ pair_mn) {
-  const uint32_t i = (blockIdx.x * blockDim.x + threadIdx.x) >> 3;
+// runs the point's random stream (so all eight agree on the draws).  The
+// rows come in the packed kNN layout (feature 8c + l at segment l, then the
+// d mod 8 tail): lane l sums its segment in order of c with float4 loads of
+// all six drawn rows in flight together, the partials are added in lane
+// order 0..7, then the tail, exactly as distance.rs:50-75.  The picks so far
+// (the reject list of the next draws) stay in LDS when n_mn <= 32.
+constexpr uint32_t kMnLdsPicks = 32;
+
+__global__ __launch_bounds__(256) void sample_mn_kernel(const float *__restrict__ xp, uint32_t n, uint32_t Cp,
+                                                        uint32_t R, uint32_t Dp, uint32_t n_mn, uint64_t seed,
+                                                        int seeded, uint32_t *__restrict__ pair_mn) {
+  __shared__ uint32_t sPick[32][kMnLdsPicks];
+  const uint32_t pt = threadIdx.x >> 3;
+  const uint32_t i = blockIdx.x * 32 + pt;
   const int lane = threadIdx.x & 63, l = lane & 7, g0 = lane & ~7;
   if (i >= n) return;  // the eight lanes of a point leave together
   DevRng rng = rng_seed_from_u64(point_seed(seed, seeded, i));
   uint32_t *rows = pair_mn + static_cast<size_t>(2) * i * n_mn;
-  const float *xi = x + static_cast<size_t>(i) * d;
-  const uint32_t chunks = d / 8;
+  const bool in_lds = n_mn <= kMnLdsPicks;
+  uint32_t *picks = in_lds ? sPick[pt] : rows + 1;
+  const uint32_t pstride = in_lds ? 1u : 2u;
+  const float *xi = xp + static_cast<size_t>(i) * Dp;
+  const float4 *own = reinterpret_cast<const float4 *>(xi + static_cast<size_t>(l) * Cp);
+  const uint32_t c4n = Cp / 4, tail = 8u * Cp;
   for (uint32_t j = 0; j < n_mn; ++j) {
     uint32_t sampled[6];
-    const uint32_t got = dev_sample_fp(6, n, rows + 1, j, 2, i, rng, sampled, 1);
+    const uint32_t got = dev_sample_fp(6, n, picks, j, pstride, i, rng, sampled, 1);
+    const float4 *seg[6];
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {  // unfilled slots read the point's own row (unused)
+      const uint32_t r = static_cast<uint32_t>(q) < got ? sampled[q] : i;
+      seg[q] = reinterpret_cast<const float4 *>(xp + static_cast<size_t>(r) * Dp + static_cast<size_t>(l) * Cp);
+    }
+    float acc[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    for (uint32_t c4 = 0; c4 < c4n; ++c4) {
+      const float4 av = own[c4];
+      float4 bv[6];
+#pragma unroll
+      for (int q = 0; q < 6; ++q) bv[q] = seg[q][c4];
+#pragma unroll
+      for (int q = 0; q < 6; ++q) {
+        const float as[4] = {av.x, av.y, av.z, av.w}, bs[4] = {bv[q].x, bv[q].y, bv[q].z, bv[q].w};
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          const float df = as[e] - bs[e];
+          const float sq = df * df;
+          acc[q] = acc[q] + sq;
+        }
+      }
+    }
     float dist[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     uint32_t idx[6] = {0, 0, 0, 0, 0, 0};
-    for (uint32_t q = 0; q < got; ++q) {
-      const float *xq = x + static_cast<size_t>(sampled[q]) * d;
-      float acc = 0.0f;
-      for (uint32_t c = 0; c < chunks; ++c) {
-        const float df = xi[8 * c + l] - xq[8 * c + l];
-        const float sq = df * df;
-        acc = acc + sq;
-      }
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
       float sd = 0.0f;
 #pragma unroll
-      for (int k = 0; k < 8; ++k) sd = sd + __shfl(acc, g0 | k, 64);
-      for (uint32_t e = chunks * 8; e < d; ++e) {
-        const float df = xi[e] - xq[e];
-        const float sq = df * df;
-        sd = sd + sq;
+      for (int k = 0; k < 8; ++k) sd = sd + __shfl(acc[q], g0 | k, 64);
+      if (static_cast<uint32_t>(q) < got) {
+        const float *xq = xp + static_cast<size_t>(sampled[q]) * Dp;
+        for (uint32_t e = 0; e < R; ++e) {
+          const float df = xi[tail + e] - xq[tail + e];
+          const float sq = df * df;
+          sd = sd + sq;
+        }
+        dist[q] = sqrtf(sd);
+        idx[q] = sampled[q];
       }
-      dist[q] = sqrtf(sd);
-      idx[q] = sampled[q];
     }
     for (int p = 1; p < 6; ++p) {  // insertion sort by total order: stable
       const float td = dist[p];
@@ -177,17 +206,18 @@ __global__ __launch_bounds__(256) void sample_mn_kernel(const float *__restrict_
     }
     rows[2 * j] = i;
     rows[2 * j + 1] = idx[1];
+    if (in_lds) picks[j] = idx[1];  // every lane of the point writes the same value
   }
 }
 
-hipError_t launch_sample_pairs(const float *x, uint32_t n, uint32_t d, const uint32_t *pair_nn,
+hipError_t launch_sample_pairs(const float *xp, const KnnLayout &L, uint32_t n, const uint32_t *pair_nn,
                                uint32_t knn, uint32_t n_mn, uint32_t n_fp, uint64_t seed, int seeded,
                                uint32_t *pair_mn, uint32_t *pair_fp, hipStream_t s) {
   if (n == 0) return hipSuccess;
   const uint32_t blocks = (n + 63) / 64;
   if (n_fp) sample_fp_kernel<<<blocks, 64, 0, s>>>(n, pair_nn, knn, n_fp, seed, seeded, pair_fp);
-  if (n_mn) sample_mn_kernel<<<static_cast<uint32_t>((8ull * n + 255) / 256), 256, 0, s>>>(x, n, d, n_mn, seed, seeded,
-                                                                                         pair_mn);
+  if (n_mn)
+    sample_mn_kernel<<<(n + 31) / 32, 256, 0, s>>>(xp, n, L.Cp, L.R, L.Dp, n_mn, seed, seeded, pair_mn);
   return hipGetLastError();
 }
 
