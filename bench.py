@@ -33,7 +33,10 @@ HBM_PEAK_GBS = 8000.0
 # profiles/pmc_traffic_70000x784.md.  Keys: (workload, kernel).
 MEASURED_TRAFFIC = {
     ("70k", "grad"): 31_290_000,               # FETCH 14,387.8 KiB x 2.0 + WRITE 1,777.4 KiB
-    ("70k", "knn_mfma_kernel"): 1_333_170_000,  # FETCH 447,653 KiB x 2.0 + WRITE 406,613 KiB
+    # grad_sell_kernel FETCH 14,960.1 KiB x 2.0 + WRITE 2,204.0 KiB, plus
+    # grad_combine_elem_kernel FETCH 1,928.6 KiB x 2.0 + WRITE 1,640.6 KiB
+    ("70k", "grad_sell"): 38_525_000,
+    ("70k", "knn_mfma_kernel"): 1_321_000_000,  # FETCH 440,848 KiB x 2.0 + WRITE 408,345 KiB
 }
 FP32_PEAK_TFLOPS = 157.3
 BF16_PEAK_TFLOPS = 2500.0
@@ -450,7 +453,7 @@ def main():
     tb = int(tms[-1]["grad_target_blocks"])
     sell = int(tms[-1]["grad_sell"])
     if sell:
-        gname, gkey = ("grad_sell_kernel + grad_tb_combine_kernel (LDS-staged sliced-ELL gradient, "
+        gname, gkey = ("grad_sell_kernel + grad_combine_elem_kernel (LDS-staged sliced-ELL gradient, "
                        "then Adam)"), "grad_sell"
     elif tb:
         gname, gkey = "grad_tb_kernel + grad_tb_combine_kernel (target-blocked gradient, then Adam)", "grad_tb"
