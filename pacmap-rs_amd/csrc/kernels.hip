@@ -356,19 +356,23 @@ __device__ __forceinline__ void wave_bitonic_last(uint64_t (&v)[R], int lane) {
 }
 
 // Merges the S sorted partial lists (k keys each) of one listed query row
-// into its first k (one wavefront per row): the running list A and each
+// into its first k (one workgroup per row): the running list A and each
 // sorted chunk B give min(A[i], B[N-1-i]), a bitonic sequence holding the N
-// smallest of both, which the last bitonic stage sorts.
+// smallest of both, which the last bitonic stage sorts.  The four waves take
+// every fourth chunk, then wave 0 merges their four running lists the same
+// way.  Keys are unique (distance bits, index), so the result does not
+// depend on the merge order.
 template <int R>
 __global__ __launch_bounds__(256) void knn_merge_kernel(const uint64_t *__restrict__ part,
                                                         uint32_t nq, uint32_t S, uint32_t k,
                                                         const uint32_t *__restrict__ qlist,
                                                         uint32_t *__restrict__ oidx,
                                                         float *__restrict__ odist) {
-  const int lane = threadIdx.x & 63;
-  const uint32_t slot = blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (slot >= nq) return;
   constexpr uint32_t N = 64 * R;
+  __shared__ uint64_t sRun[3][N];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const uint32_t slot = blockIdx.x;
+  if (slot >= nq) return;
   const uint64_t *src = part + static_cast<size_t>(slot) * S * k;
   const uint32_t total = S * k;
   uint64_t run[R];
@@ -376,23 +380,38 @@ __global__ __launch_bounds__(256) void knn_merge_kernel(const uint64_t *__restri
   for (int r = 0; r < R; ++r) run[r] = ~0ull;
   // the next chunk's keys are loaded while the current one is merged
   uint64_t nxt[R];
+  const uint32_t first = static_cast<uint32_t>(wave) * N;
 #pragma unroll
   for (int r = 0; r < R; ++r) {
-    const uint32_t e = r * 64 + lane;
+    const uint32_t e = first + r * 64 + lane;
     nxt[r] = e < total ? src[e] : ~0ull;
   }
-  for (uint32_t c0 = 0; c0 < total; c0 += N) {
+  for (uint32_t c0 = first; c0 < total; c0 += 4 * N) {
     uint64_t b[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) {
       b[r] = nxt[r];
-      const uint32_t e = c0 + N + r * 64 + lane;
+      const uint32_t e = c0 + 4 * N + r * 64 + lane;
       nxt[r] = e < total ? src[e] : ~0ull;
     }
     wave_bitonic<R>(b, lane);
 #pragma unroll
     for (int r = 0; r < R; ++r) {
       const uint64_t rev = shfl_xor_u64(b[R - 1 - r], 63);  // element N-1-i
+      run[r] = run[r] < rev ? run[r] : rev;
+    }
+    wave_bitonic_last<R>(run, lane);
+  }
+  if (wave > 0) {
+#pragma unroll
+    for (int r = 0; r < R; ++r) sRun[wave - 1][r * 64 + lane] = run[r];
+  }
+  __syncthreads();
+  if (wave != 0) return;
+  for (int w = 0; w < 3; ++w) {
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const uint64_t rev = sRun[w][N - 1 - (r * 64 + lane)];  // element N-1-i of a sorted list
       run[r] = run[r] < rev ? run[r] : rev;
     }
     wave_bitonic_last<R>(run, lane);
@@ -427,7 +446,7 @@ hipError_t launch_knn_rescan(const float *xp, uint64_t n, const KnnLayout &L, ui
     return hipErrorInvalidValue;
   const hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  const uint32_t mb = (nq + 3) / 4;
+  const uint32_t mb = nq;  // one workgroup per listed row
   if (k <= 64)
     knn_merge_kernel<1><<<mb, 256, 0, s>>>(part, nq, S, k, qlist, idx, dist);
   else if (k <= 128)

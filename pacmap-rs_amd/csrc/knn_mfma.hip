@@ -22,6 +22,7 @@
 #include "internal.h"
 
 #include <cstdint>
+#include <type_traits>
 
 namespace pm {
 
@@ -574,31 +575,41 @@ __global__ __launch_bounds__(256) void knn_rerank_kernel(
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-  uint64_t v[R];
+  // sort only as many keys as the row listed (k <= cnt <= 64 R2)
+  auto finish = [&](auto rr) {
+    constexpr int R2 = decltype(rr)::value;
+    uint64_t v[R2];
 #pragma unroll
-  for (int r = 0; r < R; ++r) {
-    const uint32_t e = r * 64 + lane;
-    v[r] = e < cnt ? keys[e] : ~0ull;
-  }
-  wave_bitonic_m<R>(v, lane);
-  const float dk = from_tkey_m(static_cast<uint32_t>(wave_pick_m<R>(v, k - 1) >> 32));
-  const float t = gtau[q];
-  // every unlisted candidate has a true squared distance > tau (up to the
-  // rounding factor folded in cert_scale), hence a reference distance > dk
-  const bool cert = all_listed || cert_scale < 0.0 ||
-                    (t > 0.0f && sqrt(static_cast<double>(t) * cert_scale) > static_cast<double>(dk));
-  if (!cert) {
-    if (lane == 0) fb_list[atomicAdd(fb_count, 1u)] = q;
-    return;
-  }
-#pragma unroll
-  for (int r = 0; r < R; ++r) {
-    const uint32_t e = r * 64 + lane;
-    if (e < k) {
-      oidx[static_cast<size_t>(q) * k + e] = static_cast<uint32_t>(v[r]);
-      odist[static_cast<size_t>(q) * k + e] = from_tkey_m(static_cast<uint32_t>(v[r] >> 32));
+    for (int r = 0; r < R2; ++r) {
+      const uint32_t e = r * 64 + lane;
+      v[r] = e < cnt ? keys[e] : ~0ull;
     }
-  }
+    wave_bitonic_m<R2>(v, lane);
+    const float dk = from_tkey_m(static_cast<uint32_t>(wave_pick_m<R2>(v, k - 1) >> 32));
+    const float t = gtau[q];
+    // every unlisted candidate has a true squared distance > tau (up to the
+    // rounding factor folded in cert_scale), hence a reference distance > dk
+    const bool cert = all_listed || cert_scale < 0.0 ||
+                      (t > 0.0f && sqrt(static_cast<double>(t) * cert_scale) > static_cast<double>(dk));
+    if (!cert) {
+      if (lane == 0) fb_list[atomicAdd(fb_count, 1u)] = q;
+      return;
+    }
+#pragma unroll
+    for (int r = 0; r < R2; ++r) {
+      const uint32_t e = r * 64 + lane;
+      if (e < k) {
+        oidx[static_cast<size_t>(q) * k + e] = static_cast<uint32_t>(v[r]);
+        odist[static_cast<size_t>(q) * k + e] = from_tkey_m(static_cast<uint32_t>(v[r] >> 32));
+      }
+    }
+  };
+  if (R >= 2 && cnt <= 128)
+    finish(std::integral_constant<int, (R >= 2 ? 2 : R)>{});
+  else if (R >= 4 && cnt <= 256)
+    finish(std::integral_constant<int, (R >= 4 ? 4 : R)>{});
+  else
+    finish(std::integral_constant<int, R>{});
 }
 
 // ---------------------------------------------------------------------------
