@@ -530,8 +530,7 @@ void pca_axes_dev(const float *x_dev, uint64_t n, uint32_t d, uint32_t q, PcaAxe
     // Q <- orthonormal basis of the columns of Z by CholeskyQR (Q = Z R^-1);
     // `twice` repeats it on Q (CholeskyQR2) for full orthogonality.  Inside
     // the iteration one pass suffices: its loss of orthogonality (~u cond^2)
-    // does not accumulate, and the first and last passes are doubled.  Z is
-    // overwritten.
+    // does not accumulate, and the last pass is doubled.  Z is overwritten.
     auto orth = [&](bool twice) {
       gemm(b, b, d, Z.get(), b, true, Z.get(), b, G.get(), b);
       PM_HIP(launch_chol_inv(G.get(), b, Rinv.get(), status.get(), s));
@@ -543,7 +542,7 @@ void pca_axes_dev(const float *x_dev, uint64_t n, uint32_t d, uint32_t q, PcaAxe
       PM_HIP(hipMemcpyAsync(Q.get(), Z.get(), Z.bytes(), hipMemcpyDeviceToDevice, s));
     };
     PM_HIP(hipMemcpyAsync(Z.get(), Q.get(), Q.bytes(), hipMemcpyDeviceToDevice, s));
-    orth(true);
+    orth(false);  // the random start is well conditioned: one pass is orthonormal to ~u
     for (uint32_t it = 0; it < kIters; ++it) {
       gemm(d, b, d, d_cov.get(), d, false, Q.get(), b, Z.get(), b);
       orth(it + 1 == kIters);

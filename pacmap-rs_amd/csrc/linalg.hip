@@ -110,8 +110,9 @@ hipError_t launch_gemm64(uint32_t M, uint32_t N, uint32_t K, const double *A, ui
 // i.e. E = R^{-T}; R^{-1} is its transpose.  The 128 x 256 array lives in
 // registers: thread (rg, cg) of a 16 x 32 grid holds rows rg + 16 a and
 // columns cg + 32 e (a, e < 8); the pivot row goes through a double-buffered
-// LDS row, one barrier per step.  A non-positive pivot restarts with a
-// diagonal shift (1e-12 trace, then x100); status[0] = 1 if six tries fail.
+// LDS row, one barrier per step.  Row j is final after step j, so its right
+// half is written out then.  A non-positive pivot restarts with a diagonal
+// shift (1e-12 trace, then x100); status[0] = 1 if six tries fail.
 // ---------------------------------------------------------------------------
 constexpr int kCholMax = 128;
 
@@ -165,37 +166,31 @@ __global__ __launch_bounds__(512) void chol_inv_kernel(const double *__restrict_
         double rc[8], f[8];
 #pragma unroll
         for (int e = 0; e < 8; ++e) rc[e] = rb[cg + 32 * e] * inv;
+        // row j is final once scaled (later steps only change rows below
+        // it): its right half goes out now, and rows <= j take f = 0 (the
+        // fma then leaves them as they are)
+        if (static_cast<uint32_t>(rg) == jj) {
+#pragma unroll
+          for (int e = 4; e < 8; ++e) {
+            const uint32_t c = cg + 32 * e - kCholMax;
+            if (c < b) Rinv[static_cast<size_t>(c) * b + j] = rc[e];
+          }
+        }
 #pragma unroll
         for (int a = 0; a < 8; ++a) {
           const uint32_t r = rg + 16 * a;
           f[a] = r > j ? rb[r] * inv : 0.0;
         }
 #pragma unroll
-        for (int a = 0; a < 8; ++a) {
-          const uint32_t r = rg + 16 * a;
+        for (int a = 0; a < 8; ++a)
 #pragma unroll
-          for (int e = 0; e < 8; ++e)
-            M[a][e] = (r == j) ? rc[e] : (r > j ? fma(-f[a], rc[e], M[a][e]) : M[a][e]);
-        }
+          for (int e = 0; e < 8; ++e) M[a][e] = fma(-f[a], rc[e], M[a][e]);
       }
       if (!ok) break;
     }
     if (!ok) shift = shift == 0.0 ? 1e-12 * tr : shift * 100.0;
   }
-  if (!ok) {
-    if (t == 0) status[0] = 1;
-    return;
-  }
-#pragma unroll
-  for (int a = 0; a < 8; ++a) {
-    const uint32_t r = rg + 16 * a;
-#pragma unroll
-    for (int e = 0; e < 8; ++e) {
-      const uint32_t c = cg + 32 * e;
-      if (r < b && c >= kCholMax && c - kCholMax < b)
-        Rinv[static_cast<size_t>(c - kCholMax) * b + r] = M[a][e];
-    }
-  }
+  if (!ok && t == 0) status[0] = 1;
 }
 
 // ---------------------------------------------------------------------------
