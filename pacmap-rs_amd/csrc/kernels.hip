@@ -744,40 +744,52 @@ constexpr uint32_t kTbRows = 256 / kTbLanes;      // rows per pass-1 workgroup
 // Stable partition of each row by target range; off[b * n + row] (b = 1..7,
 // u16, relative to the row start) is where range b begins; bad[0] is set when
 // a row is longer than 65535 entries (the caller then keeps the plain path).
-__global__ void tb_partition_kernel(const uint32_t *__restrict__ row_ptr, const uint32_t *__restrict__ ent,
-                                    uint32_t n, uint32_t span, uint32_t *__restrict__ out,
-                                    uint16_t *__restrict__ off, uint32_t *__restrict__ bad) {
-  const uint32_t row = blockIdx.x * blockDim.x + threadIdx.x;
+// One wave per row, 64 entries at a time: per-range counts from ballots,
+// then each entry's place = its range's start + earlier entries of the same
+// range (ballot below the lane, plus earlier chunks).
+__global__ __launch_bounds__(256) void tb_partition_kernel(const uint32_t *__restrict__ row_ptr,
+                                                           const uint32_t *__restrict__ ent, uint32_t n,
+                                                           uint32_t span, uint32_t *__restrict__ out,
+                                                           uint16_t *__restrict__ off,
+                                                           uint32_t *__restrict__ bad) {
+  const uint32_t row = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
   if (row >= n) return;
   const uint32_t beg = row_ptr[row], end = row_ptr[row + 1];
   if (end - beg > 65535u) {
-    atomicOr(bad, 1u);
+    if (lane == 0) atomicOr(bad, 1u);
     return;
   }
-  uint32_t cnt[kTargetBlocks];
-#pragma unroll
-  for (uint32_t b = 0; b < kTargetBlocks; ++b) cnt[b] = 0;
-  for (uint32_t e = beg; e < end; ++e) {
-    const uint32_t b = (ent[e] & kIdxMask) / span;
-#pragma unroll
-    for (uint32_t q = 0; q < kTargetBlocks; ++q) cnt[q] += (q == b) ? 1u : 0u;
-  }
+  const uint64_t below = (1ull << lane) - 1ull;
   uint32_t pos[kTargetBlocks];
+#pragma unroll
+  for (uint32_t q = 0; q < kTargetBlocks; ++q) pos[q] = 0;
+  for (uint32_t e0 = beg; e0 < end; e0 += 64) {
+    const uint32_t e = e0 + lane;
+    const uint32_t b = e < end ? (ent[e] & kIdxMask) / span : kTargetBlocks;
+#pragma unroll
+    for (uint32_t q = 0; q < kTargetBlocks; ++q) pos[q] += __popcll(__ballot(b == q));
+  }
   uint32_t acc = 0;
 #pragma unroll
-  for (uint32_t b = 0; b < kTargetBlocks; ++b) {
-    pos[b] = acc;
-    if (b > 0) off[static_cast<size_t>(b) * n + row] = static_cast<uint16_t>(acc);
-    acc += cnt[b];
+  for (uint32_t q = 0; q < kTargetBlocks; ++q) {
+    const uint32_t c = pos[q];
+    pos[q] = acc;
+    if (q > 0 && lane == 0) off[static_cast<size_t>(q) * n + row] = static_cast<uint16_t>(acc);
+    acc += c;
   }
-  for (uint32_t e = beg; e < end; ++e) {
-    const uint32_t w = ent[e];
-    const uint32_t b = (w & kIdxMask) / span;
+  for (uint32_t e0 = beg; e0 < end; e0 += 64) {
+    const uint32_t e = e0 + lane;
+    const uint32_t w = e < end ? ent[e] : 0u;
+    const uint32_t b = e < end ? (w & kIdxMask) / span : kTargetBlocks;
     uint32_t p = 0;
 #pragma unroll
-    for (uint32_t q = 0; q < kTargetBlocks; ++q)
-      if (q == b) p = pos[q]++;
-    out[beg + p] = w;
+    for (uint32_t q = 0; q < kTargetBlocks; ++q) {
+      const uint64_t m = __ballot(b == q);
+      if (b == q) p = pos[q] + __popcll(m & below);
+      pos[q] += __popcll(m);
+    }
+    if (e < end) out[beg + p] = w;
   }
 }
 
@@ -929,7 +941,7 @@ uint32_t grad_tb_blocks(uint32_t n) { return ((n + kTbRows - 1) / kTbRows) * kTa
 hipError_t launch_tb_partition(const uint32_t *row_ptr, const uint32_t *ent, uint32_t n, uint32_t *out,
                                uint16_t *off, uint32_t *bad, hipStream_t s) {
   if (n == 0) return hipSuccess;
-  tb_partition_kernel<<<(n + 255) / 256, 256, 0, s>>>(row_ptr, ent, n, tb_span(n), out, off, bad);
+  tb_partition_kernel<<<(n + 3) / 4, 256, 0, s>>>(row_ptr, ent, n, tb_span(n), out, off, bad);
   return hipGetLastError();
 }
 

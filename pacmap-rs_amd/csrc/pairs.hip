@@ -244,31 +244,68 @@ __device__ __forceinline__ bool pair_at(const PairArrays &pa, uint64_t g, uint32
   return false;
 }
 
+// Pair arrays are row-major by their first index (i), so the lanes of a
+// wave mostly hold runs of equal i.  The i side is counted (and given its
+// slots) once per run by the run's last lane; the j side, random, by one
+// atomic per pair.  run_info: head = first lane of the lane's run, tail =
+// last lane (valid lanes only; key 0xffffffff marks an invalid lane).
+__device__ __forceinline__ void lane_run(uint32_t key, int lane, int &head, int &tail) {
+  const uint32_t up = __shfl_up(key, 1, 64), dn = __shfl_down(key, 1, 64);
+  const bool is_head = lane == 0 || up != key;
+  const bool is_tail = lane == 63 || dn != key;
+  const uint64_t H = __ballot(is_head), T = __ballot(is_tail);
+  const uint64_t le = (lane == 63) ? ~0ull : ((2ull << lane) - 1ull);
+  head = 63 - __clzll(H & le);
+  tail = __ffsll(static_cast<long long>(T & ~((1ull << lane) - 1ull))) - 1;
+}
+
 // degree count and index validation (bad pair -> flag, pair index kept)
-__global__ void csr_degree_kernel(PairArrays pa, uint64_t total, uint32_t n, uint32_t *__restrict__ deg,
-                                  uint32_t *__restrict__ bad) {
-  for (uint64_t g = blockIdx.x * 256ull + threadIdx.x; g < total; g += gridDim.x * 256ull) {
-    uint32_t i, j, t;
-    pair_at(pa, g, i, j, t);
-    if (i >= n || j >= n) {
-      atomicOr(bad, 1u);
-      continue;
+__global__ __launch_bounds__(256) void csr_degree_kernel(PairArrays pa, uint64_t total, uint32_t n,
+                                                         uint32_t *__restrict__ deg, uint32_t *__restrict__ bad) {
+  const int lane = threadIdx.x & 63;
+  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * 256;
+  for (uint64_t g0 = blockIdx.x * 256ull + (threadIdx.x & ~63u); g0 < total; g0 += stride) {
+    const uint64_t g = g0 + lane;
+    uint32_t i = 0, j = 0, t = 0;
+    bool valid = g < total;
+    if (valid) {
+      pair_at(pa, g, i, j, t);
+      if (i >= n || j >= n) {
+        atomicOr(bad, 1u);
+        valid = false;
+      } else if (i == j) {
+        valid = false;  // gradient.rs:116-118
+      }
     }
-    if (i == j) continue;  // gradient.rs:116-118
-    atomicAdd(&deg[i], 1u);
-    atomicAdd(&deg[j], 1u);
+    int head, tail;
+    lane_run(valid ? i : 0xffffffffu, lane, head, tail);
+    if (valid && tail == lane) atomicAdd(&deg[i], static_cast<uint32_t>(lane - head + 1));
+    if (valid) atomicAdd(&deg[j], 1u);
   }
 }
 
-__global__ void csr_fill_kernel(PairArrays pa, uint64_t total, uint32_t *__restrict__ cursor,
-                                uint32_t *__restrict__ ent) {
-  for (uint64_t g = blockIdx.x * 256ull + threadIdx.x; g < total; g += gridDim.x * 256ull) {
-    uint32_t i, j, t;
-    pair_at(pa, g, i, j, t);
-    if (i == j) continue;
+__global__ __launch_bounds__(256) void csr_fill_kernel(PairArrays pa, uint64_t total, uint32_t n,
+                                                       uint32_t *__restrict__ cursor, uint32_t *__restrict__ ent) {
+  const int lane = threadIdx.x & 63;
+  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * 256;
+  for (uint64_t g0 = blockIdx.x * 256ull + (threadIdx.x & ~63u); g0 < total; g0 += stride) {
+    const uint64_t g = g0 + lane;
+    uint32_t i = 0, j = 0, t = 0;
+    bool valid = g < total;
+    if (valid) {
+      pair_at(pa, g, i, j, t);
+      valid = i < n && j < n && i != j;
+    }
+    int head, tail;
+    lane_run(valid ? i : 0xffffffffu, lane, head, tail);
+    uint32_t base = 0;
+    if (valid && tail == lane) base = atomicAdd(&cursor[i], static_cast<uint32_t>(lane - head + 1));
+    base = __shfl(base, tail < 0 ? lane : tail, 64);
     const uint32_t tb = t << 30;
-    ent[atomicAdd(&cursor[i], 1u)] = j | tb | kOwnerBit;
-    ent[atomicAdd(&cursor[j], 1u)] = i | tb;
+    if (valid) {
+      ent[base + static_cast<uint32_t>(lane - head)] = j | tb | kOwnerBit;
+      ent[atomicAdd(&cursor[j], 1u)] = i | tb;
+    }
   }
 }
 
@@ -425,7 +462,7 @@ hipError_t launch_csr_fill_sort(const uint32_t *const p[3], const uint64_t rows[
     pa.rows[t] = rows[t];
     total += rows[t];
   }
-  if (total) csr_fill_kernel<<<grid_for(total), 256, 0, s>>>(pa, total, cursor, ent);
+  if (total) csr_fill_kernel<<<grid_for(total), 256, 0, s>>>(pa, total, n, cursor, ent);
   const uint32_t blocks = (n + 3) / 4;
   csr_sort_rows_kernel<1><<<blocks, 256, 0, s>>>(row_ptr, n, 0, ent, too_long);
   csr_sort_rows_kernel<4><<<blocks, 256, 0, s>>>(row_ptr, n, 64, ent, too_long);
