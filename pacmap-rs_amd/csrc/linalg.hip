@@ -325,9 +325,14 @@ __global__ __launch_bounds__(kJacThreads) void jacobi_rot_kernel(const double *_
         if (r2 < m) {
           const double apq = Ap[packed_idx(p, r2, m)];
           if (apq != 0.0) {
-            const double theta = (Ap[packed_idx(r2, r2, m)] - Ap[packed_idx(p, p, m)]) / (2.0 * apq);
-            tt = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
-            c = 1.0 / sqrt(tt * tt + 1.0);
+            // tan of the rotation angle, the smaller root of t^2 + 2 theta t - 1 = 0
+            // with theta = (aqq - app) / (2 apq), written as 2 apq sgn(dd) / (|dd| +
+            // sqrt(dd^2 + 4 apq^2)) to save a division
+            const double dd = Ap[packed_idx(r2, r2, m)] - Ap[packed_idx(p, p, m)];
+            const double a2 = 2.0 * apq;
+            const double den = fabs(dd) + sqrt(fma(dd, dd, a2 * a2));
+            tt = (dd >= 0.0 ? a2 : -a2) / den;
+            c = rsqrt(fma(tt, tt, 1.0));
             s = tt * c;
           }
         }
