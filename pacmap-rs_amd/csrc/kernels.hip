@@ -995,7 +995,7 @@ hipError_t launch_grad_adam_tb(const GradArgs &a, const uint16_t *off, float *pa
 // range partials are sums of two of its partials.
 // ---------------------------------------------------------------------------
 constexpr uint32_t kSellRows = 16;         // rows per slice (4 lanes each)
-constexpr uint32_t kSellMaxChunk = 2048;   // rows per chunk (LDS bitonic sort)
+constexpr uint32_t kSellMaxChunk = 4096;   // rows per chunk (LDS bitonic sort)
 constexpr uint32_t kSellThreads = 1024;
 constexpr uint32_t kSellPad = 0xffffffffu;
 constexpr uint32_t kLdsPerCu = 163840;
@@ -1028,12 +1028,13 @@ bool sell_plan(uint32_t n, uint32_t dims, uint32_t ranges, SellPlan &p) {
   p.ranges = ranges;
   p.group = kTargetBlocks / ranges;
   p.span = tb_span(n) * p.group;
-  // two workgroups per CU when both fit in LDS, else one
+  // two workgroups per CU when both fit in LDS, else one; chunks of at most
+  // kSellMaxChunk rows (then more workgroups than CUs)
   for (uint32_t per_cu = 2; per_cu >= 1; --per_cu) {
     const uint32_t want = 256u * per_cu / ranges;  // chunks
     uint32_t rc = (n + want - 1) / want;
     rc = (rc + kSellRows - 1) / kSellRows * kSellRows;
-    if (rc > kSellMaxChunk) continue;
+    if (rc > kSellMaxChunk) rc = kSellMaxChunk;
     const uint64_t lds = (static_cast<uint64_t>(p.span) + rc) * dims * 4u;
     if (lds > kLdsPerCu / per_cu) continue;
     p.rc = rc;

@@ -372,13 +372,14 @@ def test_optimize_target_blocked_matches_oracle(monkeypatch, dims):
     assert pm.last_timings()["grad_target_blocks"] == 0
 
 
-@pytest.mark.parametrize("n,dims", [(3000, 2), (3000, 3), (777, 1), (20000, 2)])
+@pytest.mark.parametrize("n,dims", [(3000, 2), (3000, 3), (777, 1), (20000, 2), (120000, 2)])
 def test_optimize_lds_sliced_matches_target_blocked(monkeypatch, n, dims):
     """The LDS-staged sliced-ELL gradient restages the target-blocked kernel:
     with 8 target ranges (the same ranges, per-lane entry order, butterfly
     and range-order combine) the embeddings are bit-identical; with 4 merged
     ranges the same bar against the oracle holds.  Both are deterministic.
-    n = 3000 leaves a partial last slice; 20000 takes the default path."""
+    n = 3000 leaves a partial last slice; 20000 takes the default path;
+    120000 needs 8 ranges and chunks capped at 4096 rows."""
     y0, nn, mn, fp = _problem(64, n=n)
     if dims != 2:
         y0 = np.random.default_rng(65).normal(size=(n, dims)).astype(np.float32)
@@ -390,7 +391,7 @@ def test_optimize_lds_sliced_matches_target_blocked(monkeypatch, n, dims):
     if n < 16384:
         monkeypatch.setenv("PACMAP_GRAD_SELL", "1")
     spread = None
-    for ranges in (8, 4):
+    for ranges in ((8,) if n > 80000 else (8, 4)):
         monkeypatch.setenv("PACMAP_GRAD_SELL_RANGES", str(ranges))
         y, _, loss = pm.optimize(y0, nn, mn, fp, 1.0, it)
         t = pm.last_timings()
