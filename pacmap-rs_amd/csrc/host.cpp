@@ -349,7 +349,10 @@ void optimize_device(const DevCsr &csr, uint64_t n, uint64_t dims, std::vector<f
     std::memset(snaps_out, 0, sizeof(float) * ns * nd);
     std::memcpy(snaps_out, y.data(), sizeof(float) * nd);
   }
-  DevBuf<float> ybuf(2 * nd, kSlotY), m(nd, kSlotM), v(nd, kSlotV);
+  // the two Y buffers start 256-byte aligned (the gradient kernels stage
+  // Y with 16-byte loads)
+  const uint64_t ystride = (nd + 63) / 64 * 64;
+  DevBuf<float> ybuf(2 * ystride, kSlotY), m(nd, kSlotM), v(nd, kSlotV);
   upload(ybuf, y.data(), nd, s);
   PM_HIP(launch_fill_zero(m.get(), m.bytes(), s));
   PM_HIP(launch_fill_zero(v.get(), v.bytes(), s));
@@ -413,8 +416,8 @@ void optimize_device(const DevCsr &csr, uint64_t n, uint64_t dims, std::vector<f
     a.row_ptr = csr.row_ptr.get();
     a.ent = tb && !sell ? tb_ent.get() : csr.ent.get();
     a.mean_row_len = static_cast<float>(static_cast<double>(csr.nnz) / static_cast<double>(n));
-    a.y_in = ybuf.get() + (it % 2) * nd;
-    a.y_out = ybuf.get() + ((it + 1) % 2) * nd;
+    a.y_in = ybuf.get() + (it % 2) * ystride;
+    a.y_out = ybuf.get() + ((it + 1) % 2) * ystride;
     a.m = m.get();
     a.v = v.get();
     a.n = static_cast<uint32_t>(n);
@@ -467,7 +470,7 @@ void optimize_device(const DevCsr &csr, uint64_t n, uint64_t dims, std::vector<f
   PM_HIP(hipGraphLaunch(g_graph.e, s));
   opt.stop(s);
   if (record_loss) PM_HIP(launch_loss_reduce(part.get(), nb, static_cast<uint32_t>(T), d_loss.get(), s));
-  download(y.data(), ybuf.get() + (T % 2) * nd, nd, s);
+  download(y.data(), ybuf.get() + (T % 2) * ystride, nd, s);
   std::vector<float> loss(record_loss ? T : 0);
   if (record_loss) download(loss.data(), d_loss.get(), T, s);
   PM_HIP(hipStreamSynchronize(s));

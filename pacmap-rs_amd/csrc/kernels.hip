@@ -601,15 +601,16 @@ __device__ __forceinline__ PairConsts pair_consts(const float (&w)[3]) {
   return q;
 }
 
-// One CSR entry's contribution (gradient.rs:118-148): w = entry word, yr =
-// the row's point, yo = the other point.  Every gradient kernel below calls
-// this, so all of them round each term identically.  D = 1 + |yr - yo|^2 is
-// summed in the reference's order; the coefficient is k / (c + D)^2 with the
-// reciprocal from recip_nr.
+// One CSR entry's contribution (gradient.rs:118-148) with the pair type's
+// constants already selected: w = entry word, c = 10 / 10000 / 1 and
+// k = its numerator constant (NN / MN / FP), yr = the row's point, yo = the
+// other point.  Every gradient kernel below ends here, so all of them round
+// each term identically.  D = 1 + |yr - yo|^2 is summed in the reference's
+// order; the coefficient is k / (c + D)^2 with the reciprocal from recip_nr.
 template <int DIMS, bool LOSS>
-__device__ __forceinline__ void pair_term(uint32_t w, const float (&yr)[DIMS], const float *yo, float k_nn,
-                                          float k_mn, float k_fp, float w_nn, float w_mn, float w_fp,
-                                          float (&g)[DIMS], float &loss) {
+__device__ __forceinline__ void pair_term_ck(uint32_t w, float c, float k, const float (&yr)[DIMS],
+                                             const float *yo, float w_nn, float w_mn, float w_fp,
+                                             float (&g)[DIMS], float &loss) {
   float u[DIMS];
   float D = 1.0f;  // gradient.rs:121-125
 #pragma unroll
@@ -618,18 +619,28 @@ __device__ __forceinline__ void pair_term(uint32_t w, const float (&yr)[DIMS], c
     const float sq = u[d] * u[d];
     D = D + sq;
   }
-  const bool nn = w < (1u << 30), fp = w >= (2u << 30);
-  const float c = nn ? 10.0f : (fp ? 1.0f : 10000.0f);
-  const float k = nn ? k_nn : (fp ? k_fp : k_mn);
   const float rc = recip_nr(c + D);
   const float w1 = k * (rc * rc);  // gradient.rs:130 / 140 (sign folded into k)
 #pragma unroll
   for (int d = 0; d < DIMS; ++d) g[d] = g[d] + w1 * u[d];
   if (LOSS && (w & kOwnerBit)) {
+    const bool nn = w < (1u << 30), fp = w >= (2u << 30);
     const float wt = nn ? w_nn : (fp ? w_fp : w_mn);
     const float num = fp ? 1.0f : D;  // gradient.rs:129 / 139
     loss = loss + wt * (num * rc);
   }
+}
+
+// pair_term_ck with the type read from the word's top bits (0 NN, 1 MN,
+// 2 and 3 FP).
+template <int DIMS, bool LOSS>
+__device__ __forceinline__ void pair_term(uint32_t w, const float (&yr)[DIMS], const float *yo, float k_nn,
+                                          float k_mn, float k_fp, float w_nn, float w_mn, float w_fp,
+                                          float (&g)[DIMS], float &loss) {
+  const bool nn = w < (1u << 30), fp = w >= (2u << 30);
+  const float c = nn ? 10.0f : (fp ? 1.0f : 10000.0f);
+  const float k = nn ? k_nn : (fp ? k_fp : k_mn);
+  pair_term_ck<DIMS, LOSS>(w, c, k, yr, yo, w_nn, w_mn, w_fp, g, loss);
 }
 
 template <int DIMS, bool ADAM, bool LOSS, int U>
@@ -935,7 +946,9 @@ __global__ __launch_bounds__(256) void grad_combine_elem_kernel(const GradArgs a
   }
 }
 
-uint32_t tb_span(uint32_t n) { return (n + kTargetBlocks - 1) / kTargetBlocks; }
+// Points per target range, a multiple of 4 so that every range starts on a
+// 16-byte boundary of Y for any dims.
+uint32_t tb_span(uint32_t n) { return ((n + kTargetBlocks - 1) / kTargetBlocks + 3u) & ~3u; }
 uint32_t grad_tb_blocks(uint32_t n) { return ((n + kTbRows - 1) / kTbRows) * kTargetBlocks; }
 
 hipError_t launch_tb_partition(const uint32_t *row_ptr, const uint32_t *ent, uint32_t n, uint32_t *out,
@@ -988,17 +1001,33 @@ hipError_t launch_grad_adam_tb(const GradArgs &a, const uint16_t *off, float *pa
 //  * the chunk's rows are sorted by their range-b entry count (descending,
 //    ties by row) and cut into slices of 16 rows x 4 lanes; a slice stores its
 //    entries step-major (step k holds entries 4k + lane of its 16 rows, 64
-//    words, padded with kSellPad to the slice's longest row), so each step is
+//    words, padded to the slice's longest row), so each step is
 //    one coalesced 256-byte load and lanes of a wave do nearly equal work.
 // Slice entries carry the target as an index relative to the range start.
-// With R = 8 the result is bit-identical to grad_tb_kernel; with R = 4 the
-// range partials are sums of two of its partials.
+// Padding words are far-pair entries without the owner bit whose target is
+// a dummy point staged after the range at kSellFar in every coordinate: its
+// coefficient k rc^2 underflows to zero, so a pad adds a signed zero to a
+// gradient sum that is never -0 (it starts at +0, and round-to-nearest sums
+// give -0 only from two -0 terms) and no loss.  Lanes therefore never
+// branch on padding, and a wave's LDS reads of several steps are in flight
+// together.  With R = 8 the result is bit-identical to grad_tb_kernel; with
+// R = 4 the range partials are sums of two of its partials.
 // ---------------------------------------------------------------------------
 constexpr uint32_t kSellRows = 16;         // rows per slice (4 lanes each)
 constexpr uint32_t kSellMaxChunk = 4096;   // rows per chunk (LDS bitonic sort)
 constexpr uint32_t kSellThreads = 1024;
-constexpr uint32_t kSellPad = 0xffffffffu;
 constexpr uint32_t kLdsPerCu = 163840;
+// |yr - far|^2 <= 8 (1e18)^2 stays finite; 1 / (c + D) >= 1.2e-37 is a
+// normal number; its square is far below the smallest denormal.
+constexpr float kSellFar = 1.0e18f;
+// LDS table of (c, k) by pair type (static LDS, at a fixed address): a
+// shift, a mask and one broadcast LDS read replace the type's compares and
+// selects.
+constexpr uint32_t kSellTabBytes = 32;
+// The far point and 3 spare slots after the range, so that the chunk's rows
+// start on a 16-byte boundary of LDS.
+constexpr uint32_t kSellFarSlots = 4;
+__host__ __device__ __forceinline__ uint32_t sell_pad(const SellPlan &p) { return (2u << 30) | p.span; }
 
 __device__ __forceinline__ uint32_t tb_seg_beg(const uint16_t *__restrict__ off, uint32_t n, uint32_t b,
                                                uint32_t row) {
@@ -1035,7 +1064,7 @@ bool sell_plan(uint32_t n, uint32_t dims, uint32_t ranges, SellPlan &p) {
     uint32_t rc = (n + want - 1) / want;
     rc = (rc + kSellRows - 1) / kSellRows * kSellRows;
     if (rc > kSellMaxChunk) rc = kSellMaxChunk;
-    const uint64_t lds = (static_cast<uint64_t>(p.span) + rc) * dims * 4u;
+    const uint64_t lds = (static_cast<uint64_t>(p.span) + kSellFarSlots + rc) * dims * 4u + kSellTabBytes;
     if (lds > kLdsPerCu / per_cu) continue;
     p.rc = rc;
     p.chunks = (n + rc - 1) / rc;
@@ -1130,7 +1159,7 @@ __global__ __launch_bounds__(256) void sell_fill_kernel(const uint32_t *__restri
   const uint32_t *src = tb_ent + row_ptr[row] + sb;
   const uint32_t lo = b * p.span;
   for (uint32_t q = 0; q < width; ++q) {
-    uint32_t v = kSellPad;
+    uint32_t v = sell_pad(p);
     if (q < len) {
       const uint32_t w = src[q];
       v = (w & ~kIdxMask) | ((w & kIdxMask) - lo);
@@ -1155,7 +1184,7 @@ hipError_t launch_sell_fill(const uint32_t *row_ptr, const uint32_t *tb_ent, con
   // every word starts as padding: rows missing from a chunk's last slice keep it
   if (p.words) {
     const uint64_t nb = (p.words + 255) / 256;
-    fill_u32_kernel<<<static_cast<uint32_t>(nb < 16384 ? nb : 16384), 256, 0, s>>>(sent, p.words, kSellPad);
+    fill_u32_kernel<<<static_cast<uint32_t>(nb < 16384 ? nb : 16384), 256, 0, s>>>(sent, p.words, sell_pad(p));
   }
   const uint64_t total = static_cast<uint64_t>(p.ranges) * n;
   sell_fill_kernel<<<static_cast<uint32_t>((total + 255) / 256), 256, 0, s>>>(row_ptr, tb_ent, off, n, p, perm,
@@ -1168,6 +1197,32 @@ hipError_t launch_sell_fill(const uint32_t *row_ptr, const uint32_t *tb_ent, con
 template <int PERM>
 __device__ __forceinline__ float quad_swap(float v) {
   return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), PERM, 0xF, 0xF, false));
+}
+
+// Copies count floats from global memory to LDS with a kSellThreads
+// workgroup: 16-byte copies (both addresses 16-byte aligned), 8 in flight
+// per thread, then the remainder and the last < 4 floats.
+__device__ __forceinline__ void stage_copy(float *__restrict__ dst, const float *__restrict__ src, uint32_t count,
+                                           uint32_t t) {
+  constexpr uint32_t T = kSellThreads;
+  const uint32_t n4 = count / 4;
+  const float4 *s4 = reinterpret_cast<const float4 *>(src);
+  float4 *d4 = reinterpret_cast<float4 *>(dst);
+  uint32_t i0 = t;
+  for (; i0 + 7 * T < n4; i0 += 8 * T) {
+    const float4 v0 = s4[i0], v1 = s4[i0 + T], v2 = s4[i0 + 2 * T], v3 = s4[i0 + 3 * T];
+    const float4 v4 = s4[i0 + 4 * T], v5 = s4[i0 + 5 * T], v6 = s4[i0 + 6 * T], v7 = s4[i0 + 7 * T];
+    d4[i0] = v0;
+    d4[i0 + T] = v1;
+    d4[i0 + 2 * T] = v2;
+    d4[i0 + 3 * T] = v3;
+    d4[i0 + 4 * T] = v4;
+    d4[i0 + 5 * T] = v5;
+    d4[i0 + 6 * T] = v6;
+    d4[i0 + 7 * T] = v7;
+  }
+  for (; i0 < n4; i0 += T) d4[i0] = s4[i0];
+  if (t < count % 4) dst[n4 * 4 + t] = src[n4 * 4 + t];
 }
 
 // U: steps of a slice loaded ahead (4 with 8 ranges, 8 with 4 ranges: about
@@ -1189,28 +1244,20 @@ __global__ __launch_bounds__(kSellThreads) void grad_sell_kernel(const GradArgs 
   const uint32_t lo = b * p.span;
   const uint32_t nlo = lo < n ? min(p.span, n - lo) : 0u;
   const uint32_t r0 = c * p.rc, rows = min(p.rc, n - r0);
-  float *sY = lds;                                       // range b of Y
-  float *sR = lds + static_cast<size_t>(p.span) * DIMS;  // the chunk's rows of Y
+  __shared__ float2 sTab[kSellTabBytes / 8];                   // (c, k) by type
+  float *sY = lds;                                                         // range b of Y, then the far point
+  float *sR = lds + (static_cast<size_t>(p.span) + kSellFarSlots) * DIMS;  // the chunk's rows of Y
   const int t = threadIdx.x;
   {
-    // full tiles with 8 loads in flight per thread, then the remainder
-    constexpr uint32_t F = 8, T = kSellThreads;
-    const float *src = a.y_in + static_cast<size_t>(lo) * DIMS;
-    const uint32_t ny = nlo * DIMS, nr = rows * DIMS;
-    uint32_t i0 = t;
-    for (; i0 + (F - 1) * T < ny; i0 += F * T) {
-      float v[F];
-#pragma unroll
-      for (uint32_t f = 0; f < F; ++f) v[f] = src[i0 + f * T];
-#pragma unroll
-      for (uint32_t f = 0; f < F; ++f) sY[i0 + f * T] = v[f];
-    }
-    for (; i0 < ny; i0 += T) sY[i0] = src[i0];
-    const float *srr = a.y_in + static_cast<size_t>(r0) * DIMS;
-    for (uint32_t i = t; i < nr; i += T) sR[i] = srr[i];
+    // lo and r0 are multiples of 4 points and the LDS areas start on
+    // 16-byte boundaries, as stage_copy needs
+    stage_copy(sY, a.y_in + static_cast<size_t>(lo) * DIMS, nlo * DIMS, t);
+    stage_copy(sR, a.y_in + static_cast<size_t>(r0) * DIMS, rows * DIMS, t);
+    if (t < DIMS) sY[static_cast<size_t>(p.span) * DIMS + t] = kSellFar;
   }
   const PairConsts pc = pair_consts(a.w);
   const float kn = pc.k_nn, km = pc.k_mn, kf = pc.k_fp, wn = pc.w_nn, wm = pc.w_mn, wf = pc.w_fp;
+  if (t < 4) sTab[t] = t == 0 ? make_float2(10.0f, kn) : t == 1 ? make_float2(10000.0f, km) : make_float2(1.0f, kf);
   __syncthreads();
   const uint32_t wave = __builtin_amdgcn_readfirstlane(t >> 6);
   const int lane = t & 63, j = lane >> 2;
@@ -1223,7 +1270,7 @@ __global__ __launch_bounds__(kSellThreads) void grad_sell_kernel(const GradArgs 
   // come through a buffer resource: the offsets are per-wave scalars plus
   // the lane, and reads past the array return 0 (steps past the slice are
   // never used: lim below is uniform over the wave).  Rows missing from a
-  // chunk's last slice hold kSellPad words.
+  // chunk's last slice hold padding words.
   const __amdgpu_buffer_rsrc_t rs =
       __builtin_amdgcn_make_buffer_rsrc(const_cast<uint32_t *>(sent), 0, static_cast<int>(p.words * 4u), 0x00020000);
   struct Slice {
@@ -1250,29 +1297,59 @@ __global__ __launch_bounds__(kSellThreads) void grad_sell_kernel(const GradArgs 
       yr[d] = sR[(q.row - r0) * DIMS + d];
       g[d] = 0.0f;
     }
-    // CUT: stop at the first step past lim (prefetched round); otherwise
-    // every loaded word is inspected, so no load of the round is left in
-    // flight when the slice ends
-    auto round = [&](const uint32_t (&ww)[U], uint32_t lim, auto cut) {
+    // steps G0 .. G0 + G - 1 of a round: their LDS reads are issued
+    // together, then the terms computed in step order
+    auto group = [&](const uint32_t (&ww)[U], auto g0c, auto gsize) __attribute__((always_inline)) {
+      constexpr int G0 = decltype(g0c)::value, G = decltype(gsize)::value;
+      float yv[G][DIMS];
+      float2 ck[G];
 #pragma unroll
-      for (int uu = 0; uu < U; ++uu) {
-        if (decltype(cut)::value && static_cast<uint32_t>(uu) >= lim) break;
-        const uint32_t w = static_cast<uint32_t>(uu) < lim ? ww[uu] : kSellPad;
-        if (w == kSellPad) continue;
+      for (int uu = 0; uu < G; ++uu) {
+        const uint32_t w = ww[G0 + uu];
         const float *yo = sY + (w & kIdxMask) * DIMS;
-        float yv[DIMS];
 #pragma unroll
-        for (int d = 0; d < DIMS; ++d) yv[d] = yo[d];
-        pair_term<DIMS, LOSS>(w, yr, yv, kn, km, kf, wn, wm, wf, g, loss);
+        for (int d = 0; d < DIMS; ++d) yv[uu][d] = yo[d];
+        ck[uu] = sTab[__builtin_amdgcn_ubfe(w, 30, 2)];
+      }
+#pragma unroll
+      for (int uu = 0; uu < G; ++uu)
+        pair_term_ck<DIMS, LOSS>(ww[G0 + uu], ck[uu].x, ck[uu].y, yr, yv[uu], wn, wm, wf, g, loss);
+    };
+    using I0 = std::integral_constant<int, 0>;
+    using I1 = std::integral_constant<int, 1>;
+    using I2 = std::integral_constant<int, 2>;
+    using I4 = std::integral_constant<int, 4>;
+    // steps G0 .. G0 + rest - 1 (rest < 4, uniform over the wave)
+    auto tail = [&](const uint32_t (&ww)[U], auto g0c, uint32_t rest) __attribute__((always_inline)) {
+      constexpr int G0 = decltype(g0c)::value;
+      if (rest >= 2) {
+        group(ww, g0c, I2{});
+        if (rest == 3) group(ww, std::integral_constant<int, G0 + 2>{}, I1{});
+      } else if (rest == 1) {
+        group(ww, g0c, I1{});
       }
     };
-    round(q.w, min(static_cast<uint32_t>(U), q.cnt >> 6), std::true_type{});
+    // the first lim steps (lim <= U): groups of 4, then 2, then 1
+    auto round = [&](const uint32_t (&ww)[U], uint32_t lim) __attribute__((always_inline)) {
+      if (lim < 4) {
+        tail(ww, I0{}, lim);
+        return;
+      }
+      group(ww, I0{}, I4{});
+      if constexpr (U == 8) {
+        if (lim == 8)
+          group(ww, I4{}, I4{});
+        else
+          tail(ww, I4{}, lim - 4);
+      }
+    };
+    round(q.w, min(static_cast<uint32_t>(U), q.cnt >> 6));
     for (uint32_t k0 = 64u * U; k0 < q.cnt; k0 += 64u * U) {  // rows longer than 4 U entries
       uint32_t xv[U];
 #pragma unroll
       for (int uu = 0; uu < U; ++uu)
         xv[uu] = __builtin_amdgcn_raw_buffer_load_b32(rs, lane * 4, static_cast<int>((q.beg + k0 + 64u * uu) * 4u), 0);
-      round(xv, min(static_cast<uint32_t>(U), (q.cnt - k0) >> 6), std::false_type{});
+      round(xv, min(static_cast<uint32_t>(U), (q.cnt - k0) >> 6));
     }
     // butterfly over the row's 4 lanes (a quad): xor 2, then xor 1
 #pragma unroll
@@ -1319,7 +1396,7 @@ static hipError_t launch_sell_dims(const GradArgs &a, const SellPlan &p, const u
                                    hipStream_t s) {
   const uint32_t g1 = sell_grad_blocks(p);
   if (g1 == 0) return hipSuccess;
-  const size_t shm = (static_cast<size_t>(p.span) + p.rc) * DIMS * sizeof(float);
+  const size_t shm = (static_cast<size_t>(p.span) + kSellFarSlots + p.rc) * DIMS * sizeof(float);
   const bool loss = a.loss_part != nullptr;
   if (p.ranges == 4) {
     if (loss)
