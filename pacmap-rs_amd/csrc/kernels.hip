@@ -605,8 +605,10 @@ __device__ __forceinline__ PairConsts pair_consts(const float (&w)[3]) {
 // constants already selected: w = entry word, c = 10 / 10000 / 1 and
 // k = its numerator constant (NN / MN / FP), yr = the row's point, yo = the
 // other point.  Every gradient kernel below ends here, so all of them round
-// each term identically.  D = 1 + |yr - yo|^2 is summed in the reference's
-// order; the coefficient is k / (c + D)^2 with the reciprocal from recip_nr.
+// each term identically.  D = 1 + |yr - yo|^2 is accumulated in the
+// reference's order with fused multiply-adds, as is the gradient (one
+// rounding per step instead of the reference's two); the coefficient is
+// k / (c + D)^2 with the reciprocal from recip_nr.
 template <int DIMS, bool LOSS>
 __device__ __forceinline__ void pair_term_ck(uint32_t w, float c, float k, const float (&yr)[DIMS],
                                              const float *yo, float w_nn, float w_mn, float w_fp,
@@ -616,13 +618,12 @@ __device__ __forceinline__ void pair_term_ck(uint32_t w, float c, float k, const
 #pragma unroll
   for (int d = 0; d < DIMS; ++d) {
     u[d] = yr[d] - yo[d];
-    const float sq = u[d] * u[d];
-    D = D + sq;
+    D = fmaf(u[d], u[d], D);
   }
   const float rc = recip_nr(c + D);
   const float w1 = k * (rc * rc);  // gradient.rs:130 / 140 (sign folded into k)
 #pragma unroll
-  for (int d = 0; d < DIMS; ++d) g[d] = g[d] + w1 * u[d];
+  for (int d = 0; d < DIMS; ++d) g[d] = fmaf(w1, u[d], g[d]);
   if (LOSS && (w & kOwnerBit)) {
     const bool nn = w < (1u << 30), fp = w >= (2u << 30);
     const float wt = nn ? w_nn : (fp ? w_fp : w_mn);
