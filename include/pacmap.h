@@ -268,6 +268,8 @@ typedef struct {
   uint64_t grad_sell_entries;  /* sliced-ELL words including padding */
   uint64_t grad_sell_ranges;   /* target ranges of the sliced-ELL gradient (4 or 8) */
   uint64_t graph_reused;       /* 1: the previous call's optimiser graph was replayed */
+  uint64_t knn_listed;         /* candidates listed, summed over the reranked rows */
+  uint64_t knn_reranked;       /* of those, candidates given an exact distance */
 } pacmap_timings;
 void pacmap_last_timings(pacmap_timings *out);
 

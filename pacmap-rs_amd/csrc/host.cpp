@@ -40,6 +40,17 @@ double knn_cert_scale(uint32_t d) {
   return (1.0 - gammaA) * (1.0 - e24) * (1.0 - e24) * (1.0 - 1.0 / 1048576.0);
 }
 
+// Upper bounds of the rerank: d^2 <= (aq + ac) ub_scale1 - 2 dot covers
+// the product and norm errors that kappa covers for the lower bound (and
+// the rounding of a to f32); ub_scale2 the reference's FP32 summation
+// (1 + gammaA) and sqrt rounding (1 + 2^-24)^2, with 2^-20 margin.
+double knn_ub_scale1(double kappa) { return (1.0 + kappa) / ((1.0 - kappa) * (1.0 - 1.0 / 8388608.0)); }
+double knn_ub_scale2(uint32_t d) {
+  const double e24 = 1.0 / 16777216.0;
+  const double gammaA = (d / 8.0 + 16.0) * e24 * 1.1;
+  return (1.0 + gammaA) * (1.0 + e24) * (1.0 + e24) * (1.0 + 1.0 / 1048576.0);
+}
+
 bool knn_precision_valid(int precision) {
   const int base = precision & 0xff;
   return (precision & ~(0xff | PACMAP_KNN_APPROX)) == 0 && base >= PACMAP_KNN_FP32 && base <= PACMAP_KNN_DIRECT;
@@ -97,8 +108,10 @@ bool knn_mfma_path(const float *x_dev, uint64_t n, uint64_t d, uint32_t k, const
   // to keep every CU busy with a short tail
   const uint32_t groups = knn_cand_groups(static_cast<uint32_t>(ntiles));
   const uint32_t nsplit = std::min<uint32_t>(16, std::max<uint32_t>(1, (4096 + groups - 1) / groups));
-  DevBuf<uint32_t> gcnt(n * nsplit), gcand(n * nsplit * capg), fb_list(n);
+  DevBuf<uint32_t> gcnt(n * nsplit), gcand(n * nsplit * capg * (split ? 2 : 1)), fb_list(n);
   DevBuf<float> gtau(n);
+  DevBuf<unsigned long long> kstats(512);
+  PM_HIP(launch_fill_zero(kstats.get(), kstats.bytes(), s));
   const void *flo_p = mode != 0 ? flo.get() : fhi.get();
   const float *rinv_p = mode == 2 ? rinv.get() : nullptr;
   Timer tsamp, tk, tr;
@@ -113,17 +126,26 @@ bool knn_mfma_path(const float *x_dev, uint64_t n, uint64_t d, uint32_t k, const
                          gtau.get(), mode, s));
   tk.stop(s);
   tr.start(s);
+  // upper-bound pruning (certified split lists): see knn_rerank_kernel
+  const double ub_s1 = split && certify ? knn_ub_scale1(kappa) : 0.0;
   PM_HIP(launch_knn_rerank(xp, L, static_cast<uint32_t>(n), k, capg, nsplit, gcnt.get(), gcand.get(),
-                           gtau.get(), certify ? knn_cert_scale(static_cast<uint32_t>(d)) : -1.0, idx.get(),
-                           dist.get(), flags.get() + 1, fb_list.get(), s));
+                           gtau.get(), certify ? knn_cert_scale(static_cast<uint32_t>(d)) : -1.0, arow.get(),
+                           split, ub_s1, knn_ub_scale2(static_cast<uint32_t>(d)), idx.get(), dist.get(),
+                           flags.get() + 1, fb_list.get(), kstats.get(), s));
   tr.stop(s);
   download(hflags, flags.get(), 2, s);
+  std::vector<unsigned long long> hstats(512);
+  download(hstats.data(), kstats.get(), 512, s);
   PM_HIP(hipStreamSynchronize(s));
   g_tm.knn_kernel_ms += tk.ms();
   g_tm.knn_kernel_launches += 1;
   g_tm.knn_sample_ms += tsamp.ms();
   g_tm.knn_rerank_ms += tr.ms();
   g_tm.knn_candidates = capg;
+  for (int i = 0; i < 256; ++i) {
+    g_tm.knn_listed += hstats[2 * i];
+    g_tm.knn_reranked += hstats[2 * i + 1];
+  }
   g_tm.knn_path = mode == 1 ? 1 : mode == 0 ? 2 : 3;
   g_tm.knn_certified = certify ? 1 : 0;
   const uint32_t nfb = hflags[1];

@@ -69,19 +69,27 @@ hipError_t launch_knn_rownorm(const float *x, uint64_t n, uint32_t d, const doub
                               uint32_t *nonfinite, float *rinv, hipStream_t s);
 // phase 0: tau per row from a 1/ss tile sample; phase 1: append candidates
 // with lb <= tau to per-(row, split) lists (gcnt: n x nsplit, gcand:
-// n x nsplit x capg).  Query tiles are grouped knn_cand_groups(ntiles) ways.
+// n x nsplit x capg words, twice that in mode 1, whose entries are
+// (candidate, dot) pairs).  Query tiles are grouped knn_cand_groups(ntiles)
+// ways.
 // mode 0: bf16, 1: split bf16, 2: FP8 E4M3 with per-row scales (rinv).
 uint32_t knn_cand_groups(uint32_t ntiles);
 hipError_t launch_knn_cand(int phase, const void *fhi, const void *flo, const float *arow, const float *rinv,
                            uint32_t n, uint32_t kch, uint32_t ntiles, uint32_t ss, uint32_t capg,
                            uint32_t nsplit, uint32_t *gcnt, uint32_t *gcand, float *gtau,
                            int mode, hipStream_t s);
-// cert_scale < 0: accept every row with at least k listed candidates (no certificate)
+// cert_scale < 0: accept every row with at least k listed candidates (no
+// certificate).  dots: the lists hold (candidate, dot) pairs (mode 1); with
+// ub_s1 > 0 as well, candidates whose lower bound exceeds the k-th smallest
+// upper bound are dropped before the exact distances (arow: the rows' a).
+// stats[2 s] / [2 s + 1], s = row mod 256, accumulate the listed and the
+// exactly reranked candidates of the rows reranked (512 counters).
 hipError_t launch_knn_rerank(const float *xp, const KnnLayout &L, uint32_t n, uint32_t k,
                              uint32_t capg, uint32_t nsplit, const uint32_t *gcnt,
                              const uint32_t *gcand, const float *gtau, double cert_scale,
+                             const float *arow, bool dots, double ub_s1, double ub_s2,
                              uint32_t *idx, float *dist, uint32_t *fb_count, uint32_t *fb_list,
-                             hipStream_t s);
+                             unsigned long long *stats, hipStream_t s);
 
 hipError_t launch_sigma(const float *dist, uint64_t n, uint32_t kp, float *sig, hipStream_t s);
 hipError_t launch_scale_nn(const float *dist, const uint32_t *idx, const float *sig, uint64_t n,

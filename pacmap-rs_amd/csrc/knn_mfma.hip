@@ -249,13 +249,17 @@ __global__ void knn_rownorm_kernel(const float *__restrict__ x, uint64_t n, uint
 //
 // PHASE 0 scans a 1/ss sample of candidate tiles and writes tau per row.
 // PHASE 1 scans the candidate tiles of split blockIdx.y (of gridDim.y) and
-// appends every candidate with lb <= tau to the row's list of that split.
+// appends every candidate with lb <= tau to the row's list of that split
+// (MODE 1: entries are (candidate, dot) pairs).
 //
 // MODE 0: one bf16 product per multiply-add; MODE 1: split bf16 (hi hi + hi lo
 // + lo hi); MODE 2: FP8 E4M3 rows scaled by 1 / rinv (one 16x16x128 f8f6f4
 // MFMA per 128-feature chunk), the product rescaled by rinv_q rinv_c.
+// (split phase 1 with 4 chunks: held to 3 waves per SIMD, 168 VGPRs, which
+// it fits without spilling)
 template <int KCH, int RT, int MODE, int PHASE>
-__global__ __launch_bounds__(256) void knn_mfma_kernel(
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MODE == 1 && PHASE == 1 && KCH == 4 ? 3 : 1)))
+void knn_mfma_kernel(
     const uint4 *__restrict__ fhi, const uint4 *__restrict__ flo, const float *__restrict__ arow,
     const float *__restrict__ rinv, uint32_t n, uint32_t ntiles, uint32_t ss, uint32_t capg,
     uint32_t *__restrict__ gcnt, uint32_t *__restrict__ gcand, float *__restrict__ gtau) {
@@ -424,7 +428,14 @@ __global__ __launch_bounds__(256) void knn_mfma_kernel(
             }
           } else if (ok && lb <= tau[rt][r]) {
             const uint32_t pos = atomicAdd(&sCnt[wave][rt * 16 + 4 * rsub + r], 1u);
-            if (pos < capg) gcand[(static_cast<size_t>(q) * nsplit + split) * capg + pos] = c;
+            if (pos < capg) {
+              const size_t slot = (static_cast<size_t>(q) * nsplit + split) * capg + pos;
+              // split lists carry the pair's dot, for the rerank's upper bounds
+              if (SPLIT)
+                reinterpret_cast<uint2 *>(gcand)[slot] = make_uint2(c, __float_as_uint(dot));
+              else
+                gcand[slot] = c;
+            }
           }
         }
     };
@@ -486,8 +497,9 @@ template <int R>
 __global__ __launch_bounds__(256) void knn_rerank_kernel(
     const float *__restrict__ xp, uint32_t Dp, uint32_t Cp, uint32_t n, uint32_t k, uint32_t capg,
     uint32_t nsplit, const uint32_t *__restrict__ gcnt, const uint32_t *__restrict__ gcand,
-    const float *__restrict__ gtau, double cert_scale, uint32_t *__restrict__ oidx,
-    float *__restrict__ odist, uint32_t *__restrict__ fb_count, uint32_t *__restrict__ fb_list) {
+    const float *__restrict__ gtau, double cert_scale, const float *__restrict__ arow, int dots,
+    double ub_s1, double ub_s2, uint32_t *__restrict__ oidx, float *__restrict__ odist,
+    uint32_t *__restrict__ fb_count, uint32_t *__restrict__ fb_list, unsigned long long *__restrict__ stats) {
   // per wave: the query row (Dp floats), 64 R keys, then 64 R candidate ids
   extern __shared__ __align__(16) float sQrow[];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -515,30 +527,113 @@ __global__ __launch_bounds__(256) void knn_rerank_kernel(
     if (lane == 0) fb_list[atomicAdd(fb_count, 1u)] = q;
     return;
   }
+  // split lists (dots != 0) carry each pair's dot: it goes to the key area
+  float *dotl = reinterpret_cast<float *>(keys);
   {
     // gather the split lists into LDS: all loads of a 64-entry round first
     uint32_t maxc = myc;
 #pragma unroll
     for (int o = 1; o < 16; o <<= 1) maxc = max(maxc, __shfl_xor(maxc, o, 64));
     maxc = __shfl(maxc, 0, 64);  // lanes 0..15 hold the split sizes: the wave's bound
+    const uint2 *gpair = reinterpret_cast<const uint2 *>(gcand);
     for (uint32_t j0 = 0; j0 < maxc; j0 += 64) {
-      uint32_t v[16];
+      uint32_t v[16], dv[16];
 #pragma unroll
       for (int sp = 0; sp < 16; ++sp) {
         const uint32_t csp = __shfl(myc, sp, 64);
-        v[sp] = static_cast<uint32_t>(sp) < nsplit && j0 + lane < csp
-                    ? gcand[(static_cast<size_t>(q) * nsplit + sp) * capg + j0 + lane]
-                    : 0u;
+        const size_t slot = (static_cast<size_t>(q) * nsplit + sp) * capg + j0 + lane;
+        v[sp] = 0u;
+        dv[sp] = 0u;
+        if (static_cast<uint32_t>(sp) < nsplit && j0 + lane < csp) {
+          if (dots) {
+            const uint2 pr = gpair[slot];
+            v[sp] = pr.x;
+            dv[sp] = pr.y;
+          } else {
+            v[sp] = gcand[slot];
+          }
+        }
       }
 #pragma unroll
       for (int sp = 0; sp < 16; ++sp) {
         const uint32_t csp = __shfl(myc, sp, 64), start = __shfl(incl, sp, 64) - csp;
-        if (static_cast<uint32_t>(sp) < nsplit && j0 + lane < csp) list[start + j0 + lane] = v[sp];
+        if (static_cast<uint32_t>(sp) < nsplit && j0 + lane < csp) {
+          list[start + j0 + lane] = v[sp];
+          if (dots) dotl[start + j0 + lane] = __uint_as_float(dv[sp]);
+        }
       }
     }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+  }
+  // Pruning by upper bounds (certified split lists: dots != 0, ub_s1 > 0).
+  // With a = |c|^2 (1 - kappa) and the split-bf16 dot, every listed pair has
+  //   lb = fma(-2, dot, aq + ac) (the candidate kernel's value), and
+  //   d_ref^2 >= lb cert_scale (the certificate's own bound), and
+  //   d_ref^2 <= U = ((aq + ac) ub_s1 - 2 dot) ub_s2,
+  // ub_s1 = (1 + kappa) / ((1 - kappa)(1 - 2^-23)) bounding the product and
+  // norm errors as kappa does for lb, ub_s2 the reference's summation and
+  // sqrt rounding.  If theta is the k-th smallest U, at least k listed
+  // candidates have d_ref^2 <= theta, so a candidate with lb cert_scale >
+  // theta is strictly farther than the k-th and needs no exact distance.
+  uint32_t cnt2 = cnt;
+  if (dots && ub_s1 > 0.0) {
+    const float aq = arow[q];
+    auto theta_of = [&](auto rr) -> double {
+      constexpr int R2 = decltype(rr)::value;
+      uint64_t v[R2];
+#pragma unroll
+      for (int r = 0; r < R2; ++r) {
+        const uint32_t e = r * 64 + lane;
+        v[r] = ~0ull;
+        if (e < cnt) {
+          const float dot = dotl[e], ac = arow[list[e]];
+          const float lb = fmaf(-2.0f, dot, aq + ac);
+          const double u =
+              fmax((static_cast<double>(aq) + static_cast<double>(ac)) * ub_s1 - 2.0 * static_cast<double>(dot), 0.0) *
+              ub_s2;
+          dotl[e] = lb;  // the dot is not needed again
+          v[r] = (static_cast<uint64_t>(tkey_m(__double2float_ru(u))) << 32) | e;
+        }
+      }
+      wave_bitonic_m<R2>(v, lane);
+      return static_cast<double>(from_tkey_m(static_cast<uint32_t>(wave_pick_m<R2>(v, k - 1) >> 32)));
+    };
+    double theta;
+    if (R >= 2 && cnt <= 128)
+      theta = theta_of(std::integral_constant<int, (R >= 2 ? 2 : R)>{});
+    else if (R >= 4 && cnt <= 256)
+      theta = theta_of(std::integral_constant<int, (R >= 4 ? 4 : R)>{});
+    else
+      theta = theta_of(std::integral_constant<int, R>{});
+    // keep the candidates with lb cert_scale <= theta, compacted in place
+    // (a chunk is read before any of it is overwritten)
+    uint32_t total = 0;
+    for (uint32_t e0 = 0; e0 < cnt; e0 += 64) {
+      const uint32_t e = e0 + lane;
+      uint32_t c = 0;
+      bool keep = false;
+      if (e < cnt) {
+        c = list[e];
+        keep = static_cast<double>(dotl[e]) * cert_scale <= theta;
+      }
+      const uint64_t m = __ballot(keep);
+      if (keep) list[total + static_cast<uint32_t>(__popcll(m & ((1ull << lane) - 1ull)))] = c;
+      total += static_cast<uint32_t>(__popcll(m));
+    }
+    cnt2 = total;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    if (cnt2 < k) {  // cannot happen with sound bounds: rescan rather than guess
+      if (lane == 0) fb_list[atomicAdd(fb_count, 1u)] = q;
+      return;
+    }
+  }
+  if (lane == 0) {  // listed and exactly reranked candidates, for the timings (256 slots)
+    atomicAdd(&stats[2 * (q & 255u)], static_cast<unsigned long long>(cnt));
+    atomicAdd(&stats[2 * (q & 255u) + 1], static_cast<unsigned long long>(cnt2));
   }
   // Eight lanes per candidate: lane l sums segment l (features 8c + l) in
   // order of c, the eight partials are added in lane order, then the tail, as
@@ -546,7 +641,7 @@ __global__ __launch_bounds__(256) void knn_rerank_kernel(
   const int grp = lane >> 3, l = lane & 7, g0 = lane & ~7;
   const float *qseg = qv + static_cast<size_t>(l) * Cp;
   const uint32_t segEnd = 8u * Cp;
-  for (uint32_t e = grp; e < cnt; e += 8) {
+  for (uint32_t e = grp; e < cnt2; e += 8) {
     const uint32_t c = list[e];
     const float *cr = xp + static_cast<size_t>(c) * Dp;
     const float *cseg = cr + static_cast<size_t>(l) * Cp;
@@ -575,14 +670,14 @@ __global__ __launch_bounds__(256) void knn_rerank_kernel(
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-  // sort only as many keys as the row listed (k <= cnt <= 64 R2)
+  // sort only as many keys as remain (k <= cnt2 <= 64 R2)
   auto finish = [&](auto rr) {
     constexpr int R2 = decltype(rr)::value;
     uint64_t v[R2];
 #pragma unroll
     for (int r = 0; r < R2; ++r) {
       const uint32_t e = r * 64 + lane;
-      v[r] = e < cnt ? keys[e] : ~0ull;
+      v[r] = e < cnt2 ? keys[e] : ~0ull;
     }
     wave_bitonic_m<R2>(v, lane);
     const float dk = from_tkey_m(static_cast<uint32_t>(wave_pick_m<R2>(v, k - 1) >> 32));
@@ -604,9 +699,9 @@ __global__ __launch_bounds__(256) void knn_rerank_kernel(
       }
     }
   };
-  if (R >= 2 && cnt <= 128)
+  if (R >= 2 && cnt2 <= 128)
     finish(std::integral_constant<int, (R >= 2 ? 2 : R)>{});
-  else if (R >= 4 && cnt <= 256)
+  else if (R >= 4 && cnt2 <= 256)
     finish(std::integral_constant<int, (R >= 4 ? 4 : R)>{});
   else
     finish(std::integral_constant<int, R>{});
@@ -703,18 +798,21 @@ hipError_t launch_knn_cand(int phase, const void *fhi, const void *flo, const fl
 hipError_t launch_knn_rerank(const float *xp, const KnnLayout &L, uint32_t n, uint32_t k,
                              uint32_t capg, uint32_t nsplit, const uint32_t *gcnt,
                              const uint32_t *gcand, const float *gtau, double cert_scale,
+                             const float *arow, bool dots, double ub_s1, double ub_s2,
                              uint32_t *idx, float *dist, uint32_t *fb_count, uint32_t *fb_list,
-                             hipStream_t s) {
+                             unsigned long long *stats, hipStream_t s) {
   const uint32_t blocks = (n + 3) / 4;
   const uint32_t R = capg / 64;
   const size_t shm = static_cast<size_t>(4) * (L.Dp + 192 * R) * sizeof(float);
   if (nsplit > 16) return hipErrorInvalidValue;
   if (capg == 512)
-    knn_rerank_kernel<8><<<blocks, 256, shm, s>>>(xp, L.Dp, L.Cp, n, k, capg, nsplit, gcnt, gcand,
-                                                  gtau, cert_scale, idx, dist, fb_count, fb_list);
+    knn_rerank_kernel<8><<<blocks, 256, shm, s>>>(xp, L.Dp, L.Cp, n, k, capg, nsplit, gcnt, gcand, gtau,
+                                                  cert_scale, arow, dots ? 1 : 0, ub_s1, ub_s2, idx, dist,
+                                                  fb_count, fb_list, stats);
   else if (capg == 1024)
-    knn_rerank_kernel<16><<<blocks, 256, shm, s>>>(xp, L.Dp, L.Cp, n, k, capg, nsplit, gcnt, gcand,
-                                                   gtau, cert_scale, idx, dist, fb_count, fb_list);
+    knn_rerank_kernel<16><<<blocks, 256, shm, s>>>(xp, L.Dp, L.Cp, n, k, capg, nsplit, gcnt, gcand, gtau,
+                                                   cert_scale, arow, dots ? 1 : 0, ub_s1, ub_s2, idx, dist,
+                                                   fb_count, fb_list, stats);
   else
     return hipErrorInvalidValue;
   return hipGetLastError();

@@ -202,6 +202,33 @@ def test_knn_certificate_covers_most_rows():
     assert t["knn_rescanned_rows"] <= 0.01 * n
 
 
+@pytest.mark.parametrize("kind", ["clustered", "grid"])
+def test_knn_rerank_prunes_by_upper_bounds(kind):
+    """The certified split-bf16 rerank gives exact distances only to listed
+    candidates whose lower bound is within the k-th smallest upper bound:
+    far fewer than are listed, with the neighbour sets still equal to the
+    oracle's bit for bit (the grid has many exactly tied distances)."""
+    rng = np.random.default_rng(108)
+    n, d, k = 8000, 100, 60
+    if kind == "clustered":
+        c = rng.normal(scale=2.0, size=(10, d))
+        x = (c[rng.integers(0, 10, n)] + rng.normal(size=(n, d))).astype(np.float32)
+    else:
+        x = _data("grid", n, 24, 109)
+    idx, dist = pm.knn.find_k_nearest_neighbors(x, k)
+    t = pm.last_timings()
+    ref_i, ref_d = O.knn_rowwise(x, k, nthreads=16)
+    rows = n - t["knn_rescanned_rows"]
+    print(f"knn {kind}: listed {t['knn_listed'] / max(rows, 1):.1f} per row, reranked "
+          f"{t['knn_reranked'] / max(rows, 1):.1f}, rescanned rows {t['knn_rescanned_rows']}", flush=True)
+    assert t["knn_path"] == 1 and t["knn_certified"] == 1
+    assert np.array_equal(dist.view(np.uint32), ref_d.view(np.uint32))
+    assert np.array_equal(idx, ref_i)
+    assert t["knn_reranked"] >= k * rows
+    if kind == "clustered":
+        assert t["knn_reranked"] <= 0.6 * t["knn_listed"]
+
+
 def test_knn_matches_triple_sort_reference_algorithm():
     x = _data("grid", 300, 5, 21)
     idx, dist = pm.knn.find_k_nearest_neighbors(x, 60)
