@@ -1680,55 +1680,89 @@ hipError_t launch_colmean_f64(const float *x, uint64_t n, uint32_t d, double *me
   return hipGetLastError();
 }
 
+typedef double doublex4 __attribute__((ext_vector_type(4)));
+
 // Partial covariance of one row split for one 64x64 tile (upper triangle of
-// tiles only).  256 threads each own a 4x4 block of the tile in FP64.
+// tiles only) on the FP64 matrix cores: each of the 4 waves owns a 32x32
+// quarter as 2x2 v_mfma_f64_16x16x4 tiles (A = the tile's columns a, B = its
+// columns b, K = rows).  Stages of 32 centred rows are staged in LDS as
+// doubles; the next stage's floats are loaded into registers while the
+// current one is multiplied.  Operand lane l holds element (l % 16, l / 16)
+// of A (16 x 4) and (l / 16, l % 16) of B (4 x 16); the result lane l holds
+// rows l / 16 + 4 v, v = 0..3, of column l % 16 (unlike the FP32 16x16
+// products, whose lane l holds rows 4 (l / 16) + v).
 __global__ __launch_bounds__(256) void cov_part_kernel(const float *__restrict__ x, uint64_t n,
                                                        uint32_t d, const double *__restrict__ mean,
                                                        uint32_t splits, double *__restrict__ part) {
   const uint32_t ti = blockIdx.x, tj = blockIdx.y, sp = blockIdx.z;
   if (tj < ti) return;
-  constexpr int RB = 16;
+  constexpr int RB = 32;
   __shared__ double sA[RB][kCovTile];
   __shared__ double sB[RB][kCovTile];
-  const int t = threadIdx.x, ty = t >> 4, tx = t & 15;
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const int wr = wave >> 1, wc = wave & 1;
   const uint32_t a0 = ti * kCovTile, b0 = tj * kCovTile;
   const uint64_t r0 = n * sp / splits, r1 = n * (sp + 1) / splits;
-  double acc[4][4];
+  // staging: thread t moves column cc of rows rq + 4 f (f < 8) of a stage
+  const int cc = t & 63, rq = t >> 6;
+  const uint32_t ca = a0 + cc, cb = b0 + cc;
+  const bool va = ca < d, vb = cb < d;
+  const double ma = va ? mean[ca] : 0.0, mb = vb ? mean[cb] : 0.0;
+  float pa[8], pb[8];
+  auto fetch = [&](uint64_t rb) {
 #pragma unroll
-  for (int i = 0; i < 4; ++i)
-#pragma unroll
-    for (int j = 0; j < 4; ++j) acc[i][j] = 0.0;
-  for (uint64_t rb = r0; rb < r1; rb += RB) {
-    __syncthreads();
-    for (int li = t; li < RB * static_cast<int>(kCovTile); li += 256) {
-      const int rr = li / kCovTile, cc = li % kCovTile;
-      const uint64_t r = rb + rr;
-      const uint32_t ca = a0 + cc, cb = b0 + cc;
-      sA[rr][cc] = (r < r1 && ca < d) ? static_cast<double>(x[r * d + ca]) - mean[ca] : 0.0;
-      sB[rr][cc] = (r < r1 && cb < d) ? static_cast<double>(x[r * d + cb]) - mean[cb] : 0.0;
+    for (int f = 0; f < 8; ++f) {
+      const uint64_t r = rb + rq + 4 * f;
+      pa[f] = (r < r1 && va) ? x[r * d + ca] : 0.0f;
+      pb[f] = (r < r1 && vb) ? x[r * d + cb] : 0.0f;
     }
+  };
+  auto stash = [&](uint64_t rb) {
+#pragma unroll
+    for (int f = 0; f < 8; ++f) {
+      const uint64_t r = rb + rq + 4 * f;
+      sA[rq + 4 * f][cc] = (r < r1 && va) ? static_cast<double>(pa[f]) - ma : 0.0;
+      sB[rq + 4 * f][cc] = (r < r1 && vb) ? static_cast<double>(pb[f]) - mb : 0.0;
+    }
+  };
+  doublex4 acc[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) acc[i][j] = doublex4{0.0, 0.0, 0.0, 0.0};
+  const uint64_t nst = (r1 - r0 + RB - 1) / RB;
+  if (nst) fetch(r0);
+  const int kq = lane >> 4, m = lane & 15;
+  for (uint64_t st = 0; st < nst; ++st) {
+    const uint64_t rb = r0 + st * RB;
+    __syncthreads();  // the previous stage has been read
+    stash(rb);
     __syncthreads();
-#pragma unroll 4
-    for (int rr = 0; rr < RB; ++rr) {
-      double av[4], bv[4];
+    if (st + 1 < nst) fetch(rb + RB);
 #pragma unroll
-      for (int i = 0; i < 4; ++i) av[i] = sA[rr][ty * 4 + i];
+    for (int kk = 0; kk < RB / 4; ++kk) {
+      const int kr = kk * 4 + kq;
+      double av[2], bv[2];
 #pragma unroll
-      for (int j = 0; j < 4; ++j) bv[j] = sB[rr][tx * 4 + j];
+      for (int i = 0; i < 2; ++i) av[i] = sA[kr][32 * wr + 16 * i + m];
 #pragma unroll
-      for (int i = 0; i < 4; ++i)
+      for (int j = 0; j < 2; ++j) bv[j] = sB[kr][32 * wc + 16 * j + m];
 #pragma unroll
-        for (int j = 0; j < 4; ++j) acc[i][j] = fma(av[i], bv[j], acc[i][j]);
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[i], bv[j], acc[i][j], 0, 0, 0);
     }
   }
   double *P = part + static_cast<size_t>(sp) * d * d;
 #pragma unroll
-  for (int i = 0; i < 4; ++i)
+  for (int i = 0; i < 2; ++i)
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const uint32_t a = a0 + ty * 4 + i, b = b0 + tx * 4 + j;
-      if (a < d && b < d) P[static_cast<size_t>(a) * d + b] = acc[i][j];
-    }
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const uint32_t a = a0 + 32 * wr + 16 * i + kq + 4 * v, b = b0 + 32 * wc + 16 * j + m;
+        if (a < d && b < d) P[static_cast<size_t>(a) * d + b] = acc[i][j][v];
+      }
 }
 
 // Sums the split partials of the upper triangle of tiles and writes both
