@@ -566,8 +566,9 @@ void pca_axes_dev(const float *x_dev, uint64_t n, uint32_t d, uint32_t q, PcaAxe
       gemm(d, b, b, Q.get(), b, false, Rinv.get(), b, Z.get(), b);
       PM_HIP(hipMemcpyAsync(Q.get(), Z.get(), Z.bytes(), hipMemcpyDeviceToDevice, s));
     };
-    PM_HIP(hipMemcpyAsync(Z.get(), Q.get(), Q.bytes(), hipMemcpyDeviceToDevice, s));
-    orth(false);  // the random start is well conditioned: one pass is orthonormal to ~u
+    // the random start is not orthonormalised: the iteration depends only on
+    // its span, and a d x b uniform random block is well conditioned
+    // (singular values within ~(sqrt d +- sqrt b))
     for (uint32_t it = 0; it < kIters; ++it) {
       gemm(d, b, d, d_cov.get(), d, false, Q.get(), b, Z.get(), b);
       orth(it + 1 == kIters);
