@@ -305,6 +305,10 @@ void knn_mfma_kernel(
   }
   for (int i = threadIdx.x; i < 4 * 16 * RT; i += 256) (&sCnt[0][0])[i] = 0;
 
+  // Phase 1 tests lb = aq + ac - 2 dot <= tau as dot >= hq + ac / 2 with
+  // hq = (aq - tau) / 2 per row (held in tau[]): two operations per pair.
+  // Its roundings (a few 2^-24 of aq + ac) are far inside kappa's margin, so
+  // an unlisted pair still has a squared distance above tau.
   float best0[RT][4], best1[RT][4], tau[RT][4];
 #pragma unroll
   for (int rt = 0; rt < RT; ++rt)
@@ -313,7 +317,7 @@ void knn_mfma_kernel(
       best0[rt][r] = __builtin_huge_valf();
       best1[rt][r] = __builtin_huge_valf();
       const uint32_t q = qidx[rt][r];
-      tau[rt][r] = (PHASE == 1 && q < n) ? gtau[q] : __builtin_huge_valf();
+      tau[rt][r] = (PHASE == 1 && q < n) ? (aq[rt][r] - gtau[q]) * 0.5f : __builtin_huge_valf();
     }
 
   // candidate slots: sample tiles (phase 0) or this split's range (phase 1)
@@ -411,22 +415,24 @@ void knn_mfma_kernel(
       const float ac = sA[buf][ct][col];
       const float ic = FP8 ? sI[buf][ct][col] : 1.0f;
       // padded rows carry a = NaN (see knn_rownorm_kernel and aq), so their
-      // lb is NaN and fails every comparison: only self needs a test here
+      // tests fail every comparison: only self needs a test, and only on
+      // the wave's own query tiles (a uniform condition)
+      const float gc = ac * 0.5f;
+      const bool diag = tile - qtile0 < static_cast<uint32_t>(RT);
 #pragma unroll
       for (int rt = 0; rt < RT; ++rt)
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
           const uint32_t q = qidx[rt][r];
           const float dot = FP8 ? acc[rt][r] * (iq[rt][r] * ic) : acc[rt][r];
-          const float lb = fmaf(-2.0f, dot, aq[rt][r] + ac);
-          const bool ok = c != q;
           if (PHASE == 0) {
-            if (ok && lb < best1[rt][r]) {
+            const float lb = fmaf(-2.0f, dot, aq[rt][r] + ac);
+            if (c != q && lb < best1[rt][r]) {
               const bool first = lb < best0[rt][r];
               best1[rt][r] = first ? best0[rt][r] : lb;
               best0[rt][r] = first ? lb : best0[rt][r];
             }
-          } else if (ok && lb <= tau[rt][r]) {
+          } else if (dot >= tau[rt][r] + gc && (!diag || c != q)) {
             const uint32_t pos = atomicAdd(&sCnt[wave][rt * 16 + 4 * rsub + r], 1u);
             if (pos < capg) {
               const size_t slot = (static_cast<size_t>(q) * nsplit + split) * capg + pos;
