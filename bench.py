@@ -500,8 +500,11 @@ def main():
                          "bf16 MFMA + certified rerank", "fp8 MFMA, uncertified"][path],
                 "rescanned_rows": int(tms[-1]["knn_rescanned_rows"]),
                 "candidates_capacity": int(tms[-1]["knn_candidates"]),
+                "sample_ms": round(float(np.mean([t["knn_sample_ms"] for t in tms])), 3),
                 "rerank_ms": round(float(np.mean([t["knn_rerank_ms"] for t in tms])), 3),
-                "fallback_ms": round(float(np.mean([t["knn_fallback_ms"] for t in tms])), 3)},
+                "fallback_ms": round(float(np.mean([t["knn_fallback_ms"] for t in tms])), 3),
+                "listed_per_row": round(float(tms[-1]["knn_listed"]) / max(1, n - int(tms[-1]["knn_rescanned_rows"])), 1),
+                "reranked_per_row": round(float(tms[-1]["knn_reranked"]) / max(1, n - int(tms[-1]["knn_rescanned_rows"])), 1)},
     }
     if not args.no_cpu_baseline:
         log("bench: timing the CPU oracle on a bounded sample")

@@ -258,7 +258,7 @@ __global__ void knn_rownorm_kernel(const float *__restrict__ x, uint64_t n, uint
 // (split phase 1 with 4 chunks: held to 3 waves per SIMD, 168 VGPRs, which
 // it fits without spilling)
 template <int KCH, int RT, int MODE, int PHASE>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MODE == 1 && PHASE == 1 && KCH == 4 ? 3 : 1)))
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MODE == 1 && KCH == 4 ? 3 : 1)))
 void knn_mfma_kernel(
     const uint4 *__restrict__ fhi, const uint4 *__restrict__ flo, const float *__restrict__ arow,
     const float *__restrict__ rinv, uint32_t n, uint32_t ntiles, uint32_t ss, uint32_t capg,
