@@ -599,7 +599,8 @@ void pca_axes_dev(const float *x_dev, uint64_t n, uint32_t d, uint32_t q, PcaAxe
 void pca_project_dev(const float *x_dev, uint64_t n, uint32_t d, uint32_t q, const PcaAxes &ax,
                      DevBuf<float> &out, hipStream_t s) {
   out.alloc(n * q);
-  PM_HIP(launch_project(x_dev, n, d, ax.mean.get(), ax.Vf.get(), q, out.get(), s));
+  DevBuf<float> work(std::max<uint64_t>(project_workspace_floats(d, q), 4));
+  PM_HIP(launch_project(x_dev, n, d, ax.mean.get(), ax.Vf.get(), q, work.get(), out.get(), s));
 }
 
 // ---------------------------------------------------------------------------

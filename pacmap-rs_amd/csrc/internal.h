@@ -177,8 +177,12 @@ hipError_t launch_colmean_f64(const float *x, uint64_t n, uint32_t d, double *me
 uint64_t cov_scratch_doubles(uint64_t n, uint32_t d);
 hipError_t launch_cov_f64(const float *x, uint64_t n, uint32_t d, const double *mean,
                           double *scratch, double *cov, hipStream_t s);
+// out (n x q) = (x - mean) V; the workspace (project_workspace_floats) holds
+// V and the mean packed in the order the FP32-MFMA kernel stages them
+uint64_t project_workspace_floats(uint32_t d, uint32_t q);
 hipError_t launch_project(const float *x, uint64_t n, uint32_t d, const float *mean,
-                          const float *V, uint32_t q, float *out, hipStream_t s);
+                          const float *V, uint32_t q, float *workspace, float *out,
+                          hipStream_t s);
 
 hipError_t launch_adam(float *y, const float *g, float *m, float *v, uint64_t count, float lr_t,
                        float omb1, float omb2, hipStream_t s);

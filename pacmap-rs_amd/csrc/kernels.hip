@@ -1843,11 +1843,177 @@ __global__ __launch_bounds__(256) void project_kernel(const float *__restrict__ 
     }
 }
 
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+
+// The same product on the FP32 matrix cores for q <= 16 CN: a block owns 64
+// rows and every column; wave w owns rows 16 w .. 16 w + 15 as CN
+// v_mfma_f32_16x16x4f32 tiles.  Operand lane l holds element (l % 16, l / 16)
+// of A (16 x 4) and (l / 16, l % 16) of B (4 x 16); the result lane l holds
+// rows 4 (l / 16) + v, v = 0..3, of column l % 16.  A 16-feature block is
+// consumed in 4 steps; step s takes feature f + 4 (l / 16) + s, so each lane's
+// A values for a block are one 16-byte load of its row and each B value set
+// one 16-byte LDS read.  V and the mean are first packed, stage by stage, into
+// exactly the LDS image a stage needs (project_pack_kernel), so staging is a
+// straight copy of kProjStride float4 per KC = 32 features, loaded into
+// registers one stage ahead like the row values.
+constexpr int kProjKC = 32;
+template <int CN>
+constexpr int proj_stride() {
+  return (kProjKC / 16) * CN * 64 + kProjKC / 4;  // V slots, then the mean
+}
+
+uint64_t project_workspace_floats(uint32_t d, uint32_t q) {
+  const uint32_t cn = (q + 15) / 16;
+  if (cn == 0 || cn > 8) return 0;
+  const uint64_t stages = (d + kProjKC - 1) / kProjKC;
+  return stages * ((kProjKC / 16) * cn * 64 + kProjKC / 4) * 4;
+}
+
+// Packs stage st's slot e: slots below KBLK CN 64 hold V[f + s][c] for s =
+// 0..3 with (block kb, column tile j, lane l) = e's digits, f = 32 st + 16 kb
+// + 4 (l / 16), c = 16 j + l % 16; the last KC / 4 slots hold the mean.
+// Features >= d and columns >= q are zero.
+__global__ void project_pack_kernel(const float *__restrict__ V, const float *__restrict__ mean,
+                                    uint32_t d, uint32_t q, uint32_t cn, uint64_t total,
+                                    float4 *__restrict__ packed) {
+  const uint64_t e = blockIdx.x * 256ull + threadIdx.x;
+  if (e >= total) return;
+  const uint32_t vslots = (kProjKC / 16) * cn * 64, stride = vslots + kProjKC / 4;
+  const uint32_t st = static_cast<uint32_t>(e / stride), slot = static_cast<uint32_t>(e % stride);
+  float v[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+  if (slot < vslots) {
+    const uint32_t kb = slot / (cn * 64), rem = slot % (cn * 64);
+    const uint32_t c = 16 * (rem / 64) + (rem & 15);
+    const uint32_t fb = st * kProjKC + kb * 16 + 4 * ((rem & 63) >> 4);
+    for (int s2 = 0; s2 < 4; ++s2)
+      if (c < q && fb + s2 < d) v[s2] = V[static_cast<size_t>(fb + s2) * q + c];
+  } else {
+    const uint32_t fm = st * kProjKC + 4 * (slot - vslots);
+    for (int s2 = 0; s2 < 4; ++s2)
+      if (fm + s2 < d) v[s2] = mean[fm + s2];
+  }
+  packed[e] = make_float4(v[0], v[1], v[2], v[3]);
+}
+
+template <int CN, bool V4>
+__global__ __launch_bounds__(256) void project_mfma_kernel(const float *__restrict__ x, uint64_t n,
+                                                           uint32_t d,
+                                                           const float4 *__restrict__ packed,
+                                                           uint32_t q, float *__restrict__ out) {
+  constexpr int KBLK = kProjKC / 16, SLOTS = KBLK * CN * 64, STRIDE = proj_stride<CN>();
+  constexpr int NS = (STRIDE + 255) / 256;
+  __shared__ float4 sV[STRIDE];
+  const float4 *sM = sV + SLOTS;
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const int m = lane & 15, g = lane >> 4;
+  const uint64_t rbase = static_cast<uint64_t>(blockIdx.x) * 64 + wave * 16;
+  const uint64_t row = rbase + m;
+  const bool rv = row < n;
+  const float *xr = x + (rv ? row : 0) * d;
+  float4 pv[NS], px[KBLK];
+  auto fetch = [&](uint32_t st) __attribute__((always_inline)) {
+    const float4 *src = packed + static_cast<size_t>(st) * STRIDE;
+#pragma unroll
+    for (int i = 0; i < NS; ++i)  // only the last copy can fall outside the stage
+      pv[i] = (256 * (i + 1) <= STRIDE || t + 256 * i < STRIDE)
+                  ? src[t + 256 * i]
+                  : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    const uint32_t f0 = st * kProjKC;
+#pragma unroll
+    for (int kb = 0; kb < KBLK; ++kb) {
+      const uint32_t fb = f0 + kb * 16 + 4 * g;
+      if (V4) {
+        px[kb] = (rv && fb < d) ? *reinterpret_cast<const float4 *>(xr + fb)
+                                : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      } else {
+        float e[4];
+#pragma unroll
+        for (int s2 = 0; s2 < 4; ++s2) e[s2] = (rv && fb + s2 < d) ? xr[fb + s2] : 0.0f;
+        px[kb] = make_float4(e[0], e[1], e[2], e[3]);
+      }
+    }
+  };
+  floatx4 acc[CN];
+#pragma unroll
+  for (int j = 0; j < CN; ++j) acc[j] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
+  float4 xa[KBLK];
+  const uint32_t stages = (d + kProjKC - 1) / kProjKC;
+  fetch(0);
+  for (uint32_t st = 0; st < stages; ++st) {
+    __syncthreads();  // the previous stage has been read
+#pragma unroll
+    for (int i = 0; i < NS; ++i)
+      if (256 * (i + 1) <= STRIDE || t + 256 * i < STRIDE) sV[t + 256 * i] = pv[i];
+#pragma unroll
+    for (int kb = 0; kb < KBLK; ++kb) xa[kb] = px[kb];
+    __syncthreads();
+    if (st + 1 < stages) fetch(st + 1);
+#pragma unroll
+    for (int kb = 0; kb < KBLK; ++kb) {
+      if (st * kProjKC + kb * 16 >= d) break;  // uniform: the tail stage's empty block
+      float4 b[CN];
+#pragma unroll
+      for (int j = 0; j < CN; ++j) b[j] = sV[(kb * CN + j) * 64 + lane];
+      const float4 mv = sM[kb * 4 + g];
+      const float a0 = xa[kb].x - mv.x, a1 = xa[kb].y - mv.y;
+      const float a2 = xa[kb].z - mv.z, a3 = xa[kb].w - mv.w;
+      // step outer, column tile inner: consecutive products use different
+      // accumulators
+#pragma unroll
+      for (int j = 0; j < CN; ++j) acc[j] = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, b[j].x, acc[j], 0, 0, 0);
+#pragma unroll
+      for (int j = 0; j < CN; ++j) acc[j] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, b[j].y, acc[j], 0, 0, 0);
+#pragma unroll
+      for (int j = 0; j < CN; ++j) acc[j] = __builtin_amdgcn_mfma_f32_16x16x4f32(a2, b[j].z, acc[j], 0, 0, 0);
+#pragma unroll
+      for (int j = 0; j < CN; ++j) acc[j] = __builtin_amdgcn_mfma_f32_16x16x4f32(a3, b[j].w, acc[j], 0, 0, 0);
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < CN; ++j)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const uint64_t r = rbase + 4 * g + v;
+      const uint32_t c = 16 * j + m;
+      if (r < n && c < q) out[r * q + c] = acc[j][v];
+    }
+}
+
+template <int CN>
+static void launch_project_mfma(const float *x, uint64_t n, uint32_t d, const float4 *packed,
+                                uint32_t q, float *out, hipStream_t s) {
+  const dim3 g(static_cast<uint32_t>((n + 63) / 64));
+  const bool v4 = d % 4 == 0 && reinterpret_cast<uintptr_t>(x) % 16 == 0;
+  if (v4)
+    project_mfma_kernel<CN, true><<<g, 256, 0, s>>>(x, n, d, packed, q, out);
+  else
+    project_mfma_kernel<CN, false><<<g, 256, 0, s>>>(x, n, d, packed, q, out);
+}
+
 hipError_t launch_project(const float *x, uint64_t n, uint32_t d, const float *mean,
-                          const float *V, uint32_t q, float *out, hipStream_t s) {
+                          const float *V, uint32_t q, float *workspace, float *out,
+                          hipStream_t s) {
   if (n == 0 || q == 0) return hipSuccess;
-  dim3 g(static_cast<uint32_t>((n + 63) / 64), (q + 63) / 64);
-  project_kernel<<<g, 256, 0, s>>>(x, n, d, mean, V, q, out);
+  const uint32_t cn = (q + 15) / 16;
+  if (cn > 8) {
+    dim3 g(static_cast<uint32_t>((n + 63) / 64), (q + 63) / 64);
+    project_kernel<<<g, 256, 0, s>>>(x, n, d, mean, V, q, out);
+    return hipGetLastError();
+  }
+  float4 *packed = reinterpret_cast<float4 *>(workspace);
+  const uint64_t total = project_workspace_floats(d, q) / 4;
+  project_pack_kernel<<<static_cast<uint32_t>((total + 255) / 256), 256, 0, s>>>(V, mean, d, q, cn,
+                                                                                 total, packed);
+  switch (cn) {
+    case 1: launch_project_mfma<1>(x, n, d, packed, q, out, s); break;
+    case 2: launch_project_mfma<2>(x, n, d, packed, q, out, s); break;
+    case 3: launch_project_mfma<3>(x, n, d, packed, q, out, s); break;
+    case 4: launch_project_mfma<4>(x, n, d, packed, q, out, s); break;
+    case 5: launch_project_mfma<5>(x, n, d, packed, q, out, s); break;
+    case 6: launch_project_mfma<6>(x, n, d, packed, q, out, s); break;
+    case 7: launch_project_mfma<7>(x, n, d, packed, q, out, s); break;
+    default: launch_project_mfma<8>(x, n, d, packed, q, out, s); break;
+  }
   return hipGetLastError();
 }
 
