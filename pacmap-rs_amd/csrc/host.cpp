@@ -137,10 +137,10 @@ bool knn_mfma_path(const float *x_dev, uint64_t n, uint64_t d, uint32_t k, const
   std::vector<unsigned long long> hstats(512);
   download(hstats.data(), kstats.get(), 512, s);
   PM_HIP(hipStreamSynchronize(s));
-  g_tm.knn_kernel_ms += tk.ms();
+  tk.add_to(g_tm.knn_kernel_ms);
   g_tm.knn_kernel_launches += 1;
-  g_tm.knn_sample_ms += tsamp.ms();
-  g_tm.knn_rerank_ms += tr.ms();
+  tsamp.add_to(g_tm.knn_sample_ms);
+  tr.add_to(g_tm.knn_rerank_ms);
   g_tm.knn_candidates = capg;
   for (int i = 0; i < 256; ++i) {
     g_tm.knn_listed += hstats[2 * i];
@@ -161,7 +161,7 @@ bool knn_mfma_path(const float *x_dev, uint64_t n, uint64_t d, uint32_t k, const
     tf.start(s);
     PM_HIP(launch_knn_rescan(xp, n, L, k, fb_list.get(), nfb, part.get(), S, idx.get(), dist.get(), s));
     tf.stop(s);
-    g_tm.knn_fallback_ms += tf.ms();
+    tf.add_to(g_tm.knn_fallback_ms);
   }
   return true;
 }
@@ -193,14 +193,14 @@ void knn_device(const float *x_dev, uint64_t n, uint64_t d, uint32_t k, DevBuf<u
     ker.start(s);
     PM_HIP(launch_knn_exact(xp.get(), n, L, k, idx.get(), dist.get(), s));
     ker.stop(s);
-    g_tm.knn_kernel_ms += ker.ms();
+    ker.add_to(g_tm.knn_kernel_ms);
     g_tm.knn_kernel_launches += 1;
     g_tm.knn_candidates = n - 1;
     g_tm.knn_path = 0;
     g_tm.knn_certified = 1;
   }
   all.stop(s);
-  g_tm.knn_ms += all.ms();
+  all.add_to(g_tm.knn_ms);
 }
 
 // ---------------------------------------------------------------------------
@@ -350,8 +350,10 @@ bool use_sell(uint64_t n, uint64_t dims, SellPlan &plan) {
   return n >= 16384;
 }
 
-void optimize_device(const DevCsr &csr, uint64_t n, uint64_t dims, std::vector<float> &y, float lr,
-                     const uint64_t iters[3], const uint64_t *snap_idx, uint64_t ns,
+// The initial Y is y (host), or y_dev (device, n x dims) when that is not
+// null; y ends as the final Y (y.size() >= n dims).
+void optimize_device(const DevCsr &csr, uint64_t n, uint64_t dims, std::vector<float> &y, const float *y_dev,
+                     float lr, const uint64_t iters[3], const uint64_t *snap_idx, uint64_t ns,
                      float *snaps_out, float *loss_out, bool record_loss, hipStream_t s) {
   const uint64_t T = iters[0] + iters[1] + iters[2];
   const uint64_t nd = n * dims;
@@ -369,13 +371,22 @@ void optimize_device(const DevCsr &csr, uint64_t n, uint64_t dims, std::vector<f
   }
   if (snaps_out && ns) {
     std::memset(snaps_out, 0, sizeof(float) * ns * nd);
-    std::memcpy(snaps_out, y.data(), sizeof(float) * nd);
+    if (y_dev) {
+      download(snaps_out, y_dev, nd, s);
+      PM_HIP(hipStreamSynchronize(s));
+    } else {
+      std::memcpy(snaps_out, y.data(), sizeof(float) * nd);
+    }
   }
   // the two Y buffers start 256-byte aligned (the gradient kernels stage
   // Y with 16-byte loads)
   const uint64_t ystride = (nd + 63) / 64 * 64;
   DevBuf<float> ybuf(2 * ystride, kSlotY), m(nd, kSlotM), v(nd, kSlotV);
-  upload(ybuf, y.data(), nd, s);
+  if (y_dev) {
+    if (nd) PM_HIP(hipMemcpyAsync(ybuf.get(), y_dev, nd * sizeof(float), hipMemcpyDeviceToDevice, s));
+  } else {
+    upload(ybuf, y.data(), nd, s);
+  }
   PM_HIP(launch_fill_zero(m.get(), m.bytes(), s));
   PM_HIP(launch_fill_zero(v.get(), v.bytes(), s));
   DevBuf<float> d_snap(ns && snaps_out ? ns * nd : 0, kSlotSnap);
@@ -426,6 +437,7 @@ void optimize_device(const DevCsr &csr, uint64_t n, uint64_t dims, std::vector<f
   const uint32_t nb = sell ? sell_grad_blocks(sp)
                            : (tb ? grad_tb_blocks(static_cast<uint32_t>(n)) : grad_blocks(static_cast<uint32_t>(n)));
   DevBuf<float> part(record_loss ? T * nb : 0, kSlotLossPart), d_loss(record_loss ? T : 0);
+  if (T == 0 && y_dev) download(y.data(), y_dev, nd, s);
   PM_HIP(hipStreamSynchronize(s));
   g_tm.iterations = T;
   if (T == 0) return;
@@ -496,7 +508,7 @@ void optimize_device(const DevCsr &csr, uint64_t n, uint64_t dims, std::vector<f
   std::vector<float> loss(record_loss ? T : 0);
   if (record_loss) download(loss.data(), d_loss.get(), T, s);
   PM_HIP(hipStreamSynchronize(s));
-  g_tm.optimize_ms += opt.ms();
+  opt.add_to(g_tm.optimize_ms);
   if (loss_out) {
     if (record_loss)
       std::memcpy(loss_out, loss.data(), sizeof(float) * T);
@@ -580,19 +592,14 @@ void pca_axes_dev(const float *x_dev, uint64_t n, uint32_t d, uint32_t q, PcaAxe
     gemm(d, q, b, Q.get(), b, false, U.get(), q, V.get(), q);
   }
   out.Vf.alloc(static_cast<size_t>(d) * q);
-  PM_HIP(launch_sign_fix(V.get(), d, q, out.Vf.get(), s));
-  te.stop(s);
-  std::vector<double> hm(d);
-  uint32_t hstatus = 0;
-  download(hm.data(), d_mean.get(), d, s);
-  download(&hstatus, status.get(), 1, s);
-  PM_HIP(hipStreamSynchronize(s));
-  if (hstatus) fail(PACMAP_ERR_PCA, "subspace orthonormalisation failed");
-  std::vector<float> mf(hm.begin(), hm.end());
   out.mean.alloc(d);
-  upload(out.mean, mf.data(), d, s);
-  g_tm.pca_cov_ms += tc.ms();
-  g_tm.pca_eig_ms += te.ms();
+  PM_HIP(launch_sign_fix(V.get(), d, q, out.Vf.get(), d_mean.get(), out.mean.get(), s));
+  te.stop(s);
+  // checked when the entry point returns (the status stays 0 unless the
+  // orthonormalisation failed even with a shift)
+  defer_flag_check(status.get(), PACMAP_ERR_PCA, "subspace orthonormalisation failed", s);
+  tc.add_to(g_tm.pca_cov_ms);
+  te.add_to(g_tm.pca_eig_ms);
 }
 
 // out (n x q) = (x - mean) V on the device
@@ -727,7 +734,7 @@ void fit_impl(const float *x_host, const float *x_dev_in, uint64_t n, uint64_t d
     d_xin.alloc(n * d);
     upload(d_xin, x_host, n * d, s);
     tu.stop(s);
-    g_tm.upload_ms += tu.ms();
+    tu.add_to(g_tm.upload_ms);
     xd = d_xin.get();
   }
 
@@ -768,7 +775,7 @@ void fit_impl(const float *x_host, const float *x_dev_in, uint64_t n, uint64_t d
     }
   }
   tp.stop(s);
-  g_tm.preprocess_ms += tp.ms();
+  tp.add_to(g_tm.preprocess_ms);
 
   // --- sample_pairs (lib.rs:629-692) ---
   DevCsr csr;
@@ -790,7 +797,7 @@ void fit_impl(const float *x_host, const float *x_dev_in, uint64_t n, uint64_t d
                                static_cast<uint32_t>(n_fp), seed_base, cfg->has_seed ? 1 : 0, d_mn.get(),
                                d_fp.get(), s));
     tsmp.stop(s);
-    g_tm.sampling_ms += tsmp.ms();
+    tsmp.add_to(g_tm.sampling_ms);
     if (po) {
       download(po->nn, d_nn, 2 * n * knn, s);
       if (po->mn) download(po->mn, d_mn.get(), 2 * n * n_mn, s);
@@ -818,7 +825,7 @@ void fit_impl(const float *x_host, const float *x_dev_in, uint64_t n, uint64_t d
     PM_HIP(launch_sigma(dist.get(), n, kp, sig.get(), s));
     PM_HIP(launch_scale_nn(dist.get(), idx.get(), sig.get(), n, kp, knn, scaled.get(), d_nn.get(), s));
     tsg.stop(s);
-    g_tm.sigma_ms += tsg.ms();
+    tsg.add_to(g_tm.sigma_ms);
     sample_and_build(d_nn.get(), knn);
     if (po) return;
   } else if (pk == PACMAP_PAIRS_NEIGHBORS_PROVIDED) {  // lib.rs:651-676
@@ -835,20 +842,19 @@ void fit_impl(const float *x_host, const float *x_dev_in, uint64_t n, uint64_t d
   }
 
   // --- YInit (lib.rs:752-779) ---
+  DevBuf<float> y_init_dev;  // the initial Y when it is computed on the device
   if (pca_path) {  // YInit::Preprocessed: x[:, :dims] * 0.01 (lib.rs:767)
     y0.resize(n * dims);
-    DevBuf<float> cols(n * dims);
+    y_init_dev.alloc(n * dims);
     PM_HIP(launch_take_cols(d_xk.get(), n, static_cast<uint32_t>(dk), static_cast<uint32_t>(dims), 0.01f,
-                            cols.get(), s));
-    download(y0.data(), cols.get(), n * dims, s);
-    PM_HIP(hipStreamSynchronize(s));
+                            y_init_dev.get(), s));
   } else if (!have_y) {
     if (init == PACMAP_INIT_VALUE)
       value_init(cfg->init_value, n, dims, y0);
     else
       random_init(n, dims, cfg->init_random_has_seed != 0, cfg->init_random_seed, y0);
   }
-  optimize_device(csr, n, dims, y0, cfg->learning_rate, cfg->num_iters,
+  optimize_device(csr, n, dims, y0, y_init_dev.get(), cfg->learning_rate, cfg->num_iters,
                   cfg->has_snapshots ? cfg->snapshots : nullptr,
                   cfg->has_snapshots ? cfg->num_snapshots : 0, snaps_out, loss_out,
                   cfg->record_loss != 0 && loss_out != nullptr, s);
@@ -1084,7 +1090,7 @@ int pacmap_optimize(float *y, uint64_t n, uint64_t dims, const uint32_t *pair_nn
     DevCsr csr;
     build_csr_from_host(n, lists, csr, stream.get());
     std::vector<float> yv(y, y + n * dims);
-    optimize_device(csr, n, dims, yv, lr, num_iters, snapshots, num_snapshots, snapshots_out, loss_out,
+    optimize_device(csr, n, dims, yv, nullptr, lr, num_iters, snapshots, num_snapshots, snapshots_out, loss_out,
                     loss_out != nullptr, stream.get());
     std::memcpy(y, yv.data(), sizeof(float) * n * dims);
     g_tm.total_ms = now_ms() - t0;

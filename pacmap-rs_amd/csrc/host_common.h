@@ -131,6 +131,80 @@ class Stream {
 };
 
 // Event pair timing a region of one stream.
+// Work settled when an entry point returns (settle_pending): phase timings,
+// and device flags copied into pinned host words.  Reading either as soon as
+// its work is queued would make the host wait for the stream, and the GPU
+// would then idle until the next launch reaches it.
+struct PendingTime {
+  hipEvent_t a, b;
+  double *dst;
+};
+struct PendingFlag {
+  hipEvent_t done;  // recorded after the copy into *word
+  uint32_t *word;   // pinned host memory
+  int code;
+  const char *msg;
+};
+inline std::vector<PendingTime> &pending_times() {
+  static thread_local std::vector<PendingTime> v;
+  return v;
+}
+inline std::vector<PendingFlag> &pending_flags() {
+  static thread_local std::vector<PendingFlag> v;
+  return v;
+}
+constexpr int kPinnedWords = 64;
+// kPinnedWords words of pinned host memory per thread, kept for the process
+inline uint32_t *pinned_words() {
+  static thread_local uint32_t *p = nullptr;
+  if (!p) PM_HIP(hipHostMalloc(reinterpret_cast<void **>(&p), kPinnedWords * sizeof(uint32_t), 0));
+  return p;
+}
+// Copies the device word `flag` into pinned memory on stream s; when the
+// entry point returns, a nonzero value fails it with (code, msg).
+inline void defer_flag_check(const uint32_t *flag, int code, const char *msg, hipStream_t s) {
+  auto &v = pending_flags();
+  if (v.size() >= static_cast<size_t>(kPinnedWords)) {  // no free word: check now
+    uint32_t h = 0;
+    PM_HIP(hipMemcpyAsync(&h, flag, sizeof(h), hipMemcpyDeviceToHost, s));
+    PM_HIP(hipStreamSynchronize(s));
+    if (h) fail(code, msg);
+    return;
+  }
+  uint32_t *word = pinned_words() + v.size();
+  *word = 0;
+  hipEvent_t ev = nullptr;
+  PM_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+  v.push_back({ev, word, code, msg});
+  PM_HIP(hipMemcpyAsync(word, flag, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  PM_HIP(hipEventRecord(ev, s));
+}
+// Adds the pending timings to their destinations and checks the pending
+// flags (read = true), or drops both; either way the lists end empty.
+inline void settle_pending(bool read) {
+  auto &t = pending_times();
+  auto &f = pending_flags();
+  int code = 0;
+  const char *msg = nullptr;
+  for (PendingTime &p : t) {
+    float v = 0.f;
+    if (read && hipEventSynchronize(p.b) == hipSuccess && hipEventElapsedTime(&v, p.a, p.b) == hipSuccess)
+      *p.dst += v;
+    (void)hipEventDestroy(p.a);
+    (void)hipEventDestroy(p.b);
+  }
+  for (PendingFlag &p : f) {
+    if (read && !code && hipEventSynchronize(p.done) == hipSuccess && *p.word) {
+      code = p.code;
+      msg = p.msg;
+    }
+    (void)hipEventDestroy(p.done);
+  }
+  t.clear();
+  f.clear();
+  if (code) fail(code, msg);
+}
+
 class Timer {
  public:
   Timer() {
@@ -138,18 +212,25 @@ class Timer {
     PM_HIP(hipEventCreate(&b_));
   }
   ~Timer() {
-    (void)hipEventDestroy(a_);
-    (void)hipEventDestroy(b_);
+    if (a_) (void)hipEventDestroy(a_);
+    if (b_) (void)hipEventDestroy(b_);
   }
   Timer(const Timer &) = delete;
   Timer &operator=(const Timer &) = delete;
   void start(hipStream_t s) { PM_HIP(hipEventRecord(a_, s)); }
   void stop(hipStream_t s) { PM_HIP(hipEventRecord(b_, s)); }
+  // waits for the stop event
   double ms() {
     PM_HIP(hipEventSynchronize(b_));
     float v = 0.f;
     PM_HIP(hipEventElapsedTime(&v, a_, b_));
     return v;
+  }
+  // adds the elapsed time to dst when the entry point returns; the timer
+  // gives its events away and can no longer be used
+  void add_to(double &dst) {
+    pending_times().push_back({a_, b_, &dst});
+    a_ = b_ = nullptr;
   }
 
  private:
@@ -171,9 +252,11 @@ template <class F>
 int guarded(pacmap_status *st, F &&body) {
   try {
     body();
+    settle_pending(true);
     set_status(st, PACMAP_OK, nullptr);
     return PACMAP_OK;
   } catch (const PmError &e) {
+    settle_pending(false);
     if (st) {
       set_status(st, e.code, e.what());
       st->gpu_error = e.gpu_error;
@@ -184,9 +267,11 @@ int guarded(pacmap_status *st, F &&body) {
     }
     return e.code;
   } catch (const std::bad_alloc &) {
+    settle_pending(false);
     set_status(st, PACMAP_ERR_GPU, "host allocation failed");
     return PACMAP_ERR_GPU;
   } catch (const std::exception &e) {
+    settle_pending(false);
     set_status(st, PACMAP_ERR_INVALID_ARGUMENT, e.what());
     return PACMAP_ERR_INVALID_ARGUMENT;
   }

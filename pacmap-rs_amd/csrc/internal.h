@@ -207,7 +207,8 @@ hipError_t launch_chol_inv(const double *G, uint32_t b, double *Rinv, uint32_t *
 uint32_t jacobi_max_size();
 uint64_t jacobi_work_doubles(uint32_t m);
 hipError_t launch_jacobi(const double *A, uint32_t m, uint32_t q, double *Vout, double *work, hipStream_t s);
-// column signs of V (d x q) and its FP32 copy Vf
-hipError_t launch_sign_fix(double *V, uint32_t d, uint32_t q, float *Vf, hipStream_t s);
+// column signs of V (d x q), its FP32 copy Vf, and the FP32 copy of the mean
+hipError_t launch_sign_fix(double *V, uint32_t d, uint32_t q, float *Vf, const double *mean, float *meanf,
+                           hipStream_t s);
 
 }  // namespace pm

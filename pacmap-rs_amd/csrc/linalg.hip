@@ -469,10 +469,15 @@ __global__ __launch_bounds__(128) void jacobi_out_kernel(uint32_t m, uint32_t q,
 }
 
 // Column signs of V (d x q): largest-magnitude entry positive (first such
-// entry on ties); also writes the FP32 copy used by the projection kernel.
-// One 256-thread block per column.
+// entry on ties); also writes the FP32 copy used by the projection kernel,
+// and the FP32 copy of the mean (rounded to nearest).  One 256-thread block
+// per column (at least one block).
 __global__ __launch_bounds__(256) void sign_fix_kernel(double *__restrict__ V, uint32_t d, uint32_t q,
-                                                       float *__restrict__ Vf) {
+                                                       float *__restrict__ Vf, const double *__restrict__ mean,
+                                                       float *__restrict__ meanf) {
+  for (uint32_t r = blockIdx.x * 256 + threadIdx.x; r < d; r += gridDim.x * 256)
+    meanf[r] = static_cast<float>(mean[r]);
+  if (blockIdx.x >= q) return;
   __shared__ double bv[256];
   __shared__ uint32_t bi[256];
   const uint32_t c = blockIdx.x, t = threadIdx.x;
@@ -539,9 +544,10 @@ hipError_t launch_jacobi(const double *A, uint32_t m, uint32_t q, double *Vout, 
   return hipGetLastError();
 }
 
-hipError_t launch_sign_fix(double *V, uint32_t d, uint32_t q, float *Vf, hipStream_t s) {
-  if (q == 0) return hipSuccess;
-  sign_fix_kernel<<<q, 256, 0, s>>>(V, d, q, Vf);
+hipError_t launch_sign_fix(double *V, uint32_t d, uint32_t q, float *Vf, const double *mean, float *meanf,
+                           hipStream_t s) {
+  if (d == 0) return hipSuccess;
+  sign_fix_kernel<<<q > 0 ? q : 1, 256, 0, s>>>(V, d, q, Vf, mean, meanf);
   return hipGetLastError();
 }
 
