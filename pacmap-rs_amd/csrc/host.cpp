@@ -793,9 +793,10 @@ void fit_impl(const float *x_host, const float *x_dev_in, uint64_t n, uint64_t d
     const KnnLayout Ls = knn_layout(static_cast<uint32_t>(dk));
     DevBuf<float> d_xs(n_mn ? n * Ls.Dp : 0);
     if (n_mn) PM_HIP(launch_knn_pack(d_xk.get(), d_xs.get(), n, Ls, s));
+    DevBuf<uint32_t> d_ws(mn_stream_words(static_cast<uint32_t>(n), static_cast<uint32_t>(n_mn)));
     PM_HIP(launch_sample_pairs(d_xs.get(), Ls, static_cast<uint32_t>(n), d_nn, knn, static_cast<uint32_t>(n_mn),
                                static_cast<uint32_t>(n_fp), seed_base, cfg->has_seed ? 1 : 0, d_mn.get(),
-                               d_fp.get(), s));
+                               d_fp.get(), d_ws.get(), s));
     tsmp.stop(s);
     tsmp.add_to(g_tm.sampling_ms);
     if (po) {
@@ -1062,9 +1063,10 @@ int pacmap_sample_pairs(const float *x, uint64_t n, uint64_t d, const uint32_t *
     const KnnLayout L = knn_layout(static_cast<uint32_t>(d));
     DevBuf<float> dxs(n_mn ? n * L.Dp : 0);
     if (n_mn) PM_HIP(launch_knn_pack(dx.get(), dxs.get(), n, L, s));
+    DevBuf<uint32_t> dws(mn_stream_words(static_cast<uint32_t>(n), static_cast<uint32_t>(n_mn)));
     PM_HIP(launch_sample_pairs(dxs.get(), L, static_cast<uint32_t>(n), dnn.get(),
                                static_cast<uint32_t>(n_neighbors), static_cast<uint32_t>(n_mn),
-                               static_cast<uint32_t>(n_fp), seed, 1, dmn.get(), dfp.get(), s));
+                               static_cast<uint32_t>(n_fp), seed, 1, dmn.get(), dfp.get(), dws.get(), s));
     download(pair_mn_out, dmn.get(), 2 * n * n_mn, s);
     download(pair_fp_out, dfp.get(), 2 * n * n_fp, s);
     PM_HIP(hipStreamSynchronize(s));

@@ -37,9 +37,12 @@ hipError_t launch_knn_exact(const float *xp, uint64_t n, const KnnLayout &L, uin
 // Pair sampling on the GPU (pairs.hip) from the points in the packed kNN
 // layout (launch_knn_pack); pair arrays are rows x 2 u32 and pair_fp must be
 // zeroed by the caller (unfilled far pairs stay (0, 0)).
+// stream_ws holds mn_stream_words(n, n_mn) words (the MN sampler's
+// precomputed random streams).
+uint64_t mn_stream_words(uint32_t n, uint32_t n_mn);
 hipError_t launch_sample_pairs(const float *xp, const KnnLayout &L, uint32_t n, const uint32_t *pair_nn,
                                uint32_t knn, uint32_t n_mn, uint32_t n_fp, uint64_t seed, int seeded,
-                               uint32_t *pair_mn, uint32_t *pair_fp, hipStream_t s);
+                               uint32_t *pair_mn, uint32_t *pair_fp, uint32_t *stream_ws, hipStream_t s);
 // Symmetrised CSR on the GPU: count degrees (flags out-of-range indices in
 // bad[0]), scan to row offsets, fill, and sort each row by entry value (rows
 // longer than 2048 entries set too_long[0] and stay unsorted).

@@ -121,36 +121,133 @@ __global__ void sample_fp_kernel(uint32_t n, const uint32_t *__restrict__ pair_n
   for (uint32_t q = 0; q < got; ++q) rows[2 * q] = i;
 }
 
+// The MN sampler's random streams, precomputed: point i's stream (seeded as
+// in sampling.rs:133) is only ever used through rng_gen_range, which takes
+// the upper 32 bits of each output, so the first kMnStream(n_mn) of those are
+// written to vals[i][.] and the state after them to states[i] (a point that
+// needs more continues from there).  One thread per point.
+__host__ __device__ constexpr uint32_t mn_stream_len(uint32_t n_mn) {
+  return (6 * n_mn + 16 + 7) / 8 * 8;  // six draws a step, a margin for rejections
+}
+
+__global__ void mn_stream_kernel(uint32_t n, uint32_t nv, uint64_t seed, int seeded,
+                                 uint32_t *__restrict__ vals, uint64_t *__restrict__ states) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  DevRng rng = rng_seed_from_u64(point_seed(seed, seeded, i));
+  uint4 *dst = reinterpret_cast<uint4 *>(vals + static_cast<size_t>(i) * nv);
+  for (uint32_t k = 0; k < nv / 4; ++k) {
+    uint4 w;
+    w.x = static_cast<uint32_t>(rng_next_u64(rng) >> 32);
+    w.y = static_cast<uint32_t>(rng_next_u64(rng) >> 32);
+    w.z = static_cast<uint32_t>(rng_next_u64(rng) >> 32);
+    w.w = static_cast<uint32_t>(rng_next_u64(rng) >> 32);
+    dst[k] = w;
+  }
+#pragma unroll
+  for (int q = 0; q < 4; ++q) states[static_cast<size_t>(i) * 4 + q] = rng.s[q];
+}
+
 // sampling.rs:118-142 + 296-313: for each j, six distinct draws avoiding self
 // and the earlier picks, then the second closest by distance (ties keep draw
-// order; unfilled slots are (0.0, 0)).  Eight lanes per point: every lane
-// runs the point's random stream (so all eight agree on the draws).  The
-// rows come in the packed kNN layout (feature 8c + l at segment l, then the
-// d mod 8 tail): lane l sums its segment in order of c with float4 loads of
-// all six drawn rows in flight together, the partials are added in lane
-// order 0..7, then the tail, exactly as distance.rs:50-75.  The picks so far
-// (the reject list of the next draws) stay in LDS when n_mn <= 32.
-constexpr uint32_t kMnLdsPicks = 32;
+// order; unfilled slots are (0.0, 0)).  Eight lanes per point.
+//
+// Draws: the group reads its stream eight values at a time (lane l holds
+// value base + l) and takes them in order by shuffle, so the eight lanes see
+// the sequence sampling.rs:37-61 consumes.  A candidate is tested against
+// the draws of this step (lane q holds draw q) and the earlier picks (lane l
+// holds picks l, l + 8, l + 16, l + 24) in parallel and the group's lanes
+// are combined by ballot.  n_mn > 32 keeps the picks in the pair rows and
+// tests them eight at a time.
+//
+// Distances: the rows come in the packed kNN layout (feature 8c + l at
+// segment l, then the d mod 8 tail): lane l sums its segment in order of c
+// with float4 loads of all six drawn rows in flight together, the partials
+// are added in lane order 0..7, then the tail, exactly as distance.rs:50-75.
+constexpr uint32_t kMnRegPicks = 32;
 
 __global__ __launch_bounds__(256) void sample_mn_kernel(const float *__restrict__ xp, uint32_t n, uint32_t Cp,
-                                                        uint32_t R, uint32_t Dp, uint32_t n_mn, uint64_t seed,
-                                                        int seeded, uint32_t *__restrict__ pair_mn) {
-  __shared__ uint32_t sPick[32][kMnLdsPicks];
+                                                        uint32_t R, uint32_t Dp, uint32_t n_mn,
+                                                        const uint32_t *__restrict__ vals,
+                                                        uint64_t *__restrict__ states,
+                                                        uint32_t *__restrict__ pair_mn) {
   const uint32_t pt = threadIdx.x >> 3;
   const uint32_t i = blockIdx.x * 32 + pt;
   const int lane = threadIdx.x & 63, l = lane & 7, g0 = lane & ~7;
   if (i >= n) return;  // the eight lanes of a point leave together
-  DevRng rng = rng_seed_from_u64(point_seed(seed, seeded, i));
+  const uint32_t nv = mn_stream_len(n_mn);
+  const uint32_t *sv = vals + static_cast<size_t>(i) * nv;
+  uint32_t buf = sv[l], base = 0, pos = 0;
+  auto next_value = [&]() -> uint32_t {
+    if (pos >= nv) {  // past the precomputed values (rare): continue the stream kept in states[i]
+      uint64_t *st = states + static_cast<size_t>(i) * 4;
+      DevRng rng;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) rng.s[q] = st[q];
+      const uint32_t v = static_cast<uint32_t>(rng_next_u64(rng) >> 32);
+#pragma unroll
+      for (int q = 0; q < 4; ++q) st[q] = rng.s[q];  // the group's lanes store the same state
+      ++pos;
+      return v;
+    }
+    if (pos - base >= 8) {
+      base += 8;
+      buf = sv[base + l];  // nv is a multiple of 8
+    }
+    const uint32_t v = static_cast<uint32_t>(__shfl(static_cast<int>(buf), g0 | static_cast<int>(pos - base), 64));
+    ++pos;
+    return v;
+  };
+  const uint32_t zone = (n << __clz(n)) - 1u;  // rng_gen_range(n)
+  auto group_any = [&](bool p) -> bool { return ((__ballot(p) >> g0) & 0xffull) != 0; };
   uint32_t *rows = pair_mn + static_cast<size_t>(2) * i * n_mn;
-  const bool in_lds = n_mn <= kMnLdsPicks;
-  uint32_t *picks = in_lds ? sPick[pt] : rows + 1;
-  const uint32_t pstride = in_lds ? 1u : 2u;
+  const bool in_reg = n_mn <= kMnRegPicks;
+  uint32_t pk0 = 0, pk1 = 0, pk2 = 0, pk3 = 0;  // picks l, l + 8, l + 16, l + 24
+  // is `c` among the first `cnt` picks?
+  auto picked = [&](uint32_t c, uint32_t cnt) -> bool {
+    if (in_reg) {
+      const uint32_t ul = static_cast<uint32_t>(l);
+      return group_any((ul < cnt && pk0 == c) || (ul + 8 < cnt && pk1 == c) || (ul + 16 < cnt && pk2 == c) ||
+                       (ul + 24 < cnt && pk3 == c));
+    }
+    bool hit = false;
+    for (uint32_t q0 = 0; q0 < cnt; q0 += 8) {
+      const uint32_t q = q0 + static_cast<uint32_t>(l);
+      hit = hit || (q < cnt && rows[2 * q + 1] == c);
+    }
+    return group_any(hit);
+  };
   const float *xi = xp + static_cast<size_t>(i) * Dp;
   const float4 *own = reinterpret_cast<const float4 *>(xi + static_cast<size_t>(l) * Cp);
   const uint32_t c4n = Cp / 4, tail = 8u * Cp;
   for (uint32_t j = 0; j < n_mn; ++j) {
+    // sampling.rs:37-61 with n_samples = 6, the reject list = the j picks
+    const bool self_absent = !picked(i, j);
+    uint64_t avail = n;
+    avail = avail > j ? avail - j : 0;
+    const uint32_t sa = self_absent ? 1u : 0u;
+    avail = avail > sa ? avail - sa : 0;
+    const uint32_t want = avail < 6 ? static_cast<uint32_t>(avail) : 6u;
+    uint32_t got = 0, mine = 0;  // lane q < got holds draw q
+    while (got < want) {
+      uint32_t c;
+      for (;;) {
+        const uint64_t m = static_cast<uint64_t>(next_value()) * n;
+        if (static_cast<uint32_t>(m) <= zone) {
+          c = static_cast<uint32_t>(m >> 32);
+          break;
+        }
+      }
+      if (c == i) continue;
+      const bool dup = group_any(static_cast<uint32_t>(l) < got && mine == c) || picked(c, j);
+      if (!dup) {
+        if (static_cast<uint32_t>(l) == got) mine = c;
+        ++got;
+      }
+    }
     uint32_t sampled[6];
-    const uint32_t got = dev_sample_fp(6, n, picks, j, pstride, i, rng, sampled, 1);
+#pragma unroll
+    for (int q = 0; q < 6; ++q) sampled[q] = static_cast<uint32_t>(__shfl(static_cast<int>(mine), g0 | q, 64));
     const float4 *seg[6];
 #pragma unroll
     for (int q = 0; q < 6; ++q) {  // unfilled slots read the point's own row (unused)
@@ -174,6 +271,16 @@ __global__ __launch_bounds__(256) void sample_mn_kernel(const float *__restrict_
         }
       }
     }
+    // the tail: lane e < R holds the squared differences of feature 8 Cp + e
+    float tq[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    if (static_cast<uint32_t>(l) < R) {
+      const float xe = xi[tail + l];
+#pragma unroll
+      for (int q = 0; q < 6; ++q) {
+        const float df = xe - xp[static_cast<size_t>(q < static_cast<int>(got) ? sampled[q] : i) * Dp + tail + l];
+        tq[q] = df * df;
+      }
+    }
     float dist[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     uint32_t idx[6] = {0, 0, 0, 0, 0, 0};
 #pragma unroll
@@ -181,13 +288,8 @@ __global__ __launch_bounds__(256) void sample_mn_kernel(const float *__restrict_
       float sd = 0.0f;
 #pragma unroll
       for (int k = 0; k < 8; ++k) sd = sd + __shfl(acc[q], g0 | k, 64);
+      for (uint32_t e = 0; e < R; ++e) sd = sd + __shfl(tq[q], g0 | static_cast<int>(e), 64);
       if (static_cast<uint32_t>(q) < got) {
-        const float *xq = xp + static_cast<size_t>(sampled[q]) * Dp;
-        for (uint32_t e = 0; e < R; ++e) {
-          const float df = xi[tail + e] - xq[tail + e];
-          const float sq = df * df;
-          sd = sd + sq;
-        }
         dist[q] = sqrtf(sd);
         idx[q] = sampled[q];
       }
@@ -204,20 +306,37 @@ __global__ __launch_bounds__(256) void sample_mn_kernel(const float *__restrict_
       dist[c + 1] = td;
       idx[c + 1] = ti;
     }
-    rows[2 * j] = i;
-    rows[2 * j + 1] = idx[1];
-    if (in_lds) picks[j] = idx[1];  // every lane of the point writes the same value
+    if (l == 0) {
+      rows[2 * j] = i;
+      rows[2 * j + 1] = idx[1];
+    }
+    if (in_reg && static_cast<uint32_t>(l) == (j & 7)) {
+      const uint32_t slot = j >> 3;
+      pk0 = slot == 0 ? idx[1] : pk0;
+      pk1 = slot == 1 ? idx[1] : pk1;
+      pk2 = slot == 2 ? idx[1] : pk2;
+      pk3 = slot == 3 ? idx[1] : pk3;
+    }
+    if (!in_reg) __threadfence_block();  // the row written above is read back by picked()
   }
+}
+
+uint64_t mn_stream_words(uint32_t n, uint32_t n_mn) {
+  return n_mn ? static_cast<uint64_t>(n) * (mn_stream_len(n_mn) + 8) : 0;
 }
 
 hipError_t launch_sample_pairs(const float *xp, const KnnLayout &L, uint32_t n, const uint32_t *pair_nn,
                                uint32_t knn, uint32_t n_mn, uint32_t n_fp, uint64_t seed, int seeded,
-                               uint32_t *pair_mn, uint32_t *pair_fp, hipStream_t s) {
+                               uint32_t *pair_mn, uint32_t *pair_fp, uint32_t *stream_ws, hipStream_t s) {
   if (n == 0) return hipSuccess;
   const uint32_t blocks = (n + 63) / 64;
   if (n_fp) sample_fp_kernel<<<blocks, 64, 0, s>>>(n, pair_nn, knn, n_fp, seed, seeded, pair_fp);
-  if (n_mn)
-    sample_mn_kernel<<<(n + 31) / 32, 256, 0, s>>>(xp, n, L.Cp, L.R, L.Dp, n_mn, seed, seeded, pair_mn);
+  if (n_mn) {
+    const uint32_t nv = mn_stream_len(n_mn);
+    uint64_t *states = reinterpret_cast<uint64_t *>(stream_ws + static_cast<size_t>(n) * nv);
+    mn_stream_kernel<<<(n + 255) / 256, 256, 0, s>>>(n, nv, seed, seeded, stream_ws, states);
+    sample_mn_kernel<<<(n + 31) / 32, 256, 0, s>>>(xp, n, L.Cp, L.R, L.Dp, n_mn, stream_ws, states, pair_mn);
+  }
   return hipGetLastError();
 }
 

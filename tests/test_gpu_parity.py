@@ -291,6 +291,8 @@ def test_scale_dist_golden_on_gpu(golden):
     (777, 13, 15, 7, 30, 7),
     (9, 4, 3, 4, 10, 3),       # fewer points than draws: short samples, (0, 0) rows
     (5000, 33, 10, 5, 20, 2024),
+    (300, 20, 10, 40, 5, 11),  # more than 32 MN pairs a point: picks read back from the pair rows
+    (12, 5, 3, 12, 4, 5),      # many rejected draws: the MN streams run past their precomputed values
 ])
 def test_sample_pairs_match_oracle(n, d, knn, n_mn, n_fp, seed):
     """The GPU samplers restate the same RNG streams as the oracle: identical pairs."""
