@@ -556,13 +556,12 @@ void pca_axes_dev(const float *x_dev, uint64_t n, uint32_t d, uint32_t q, PcaAxe
     std::vector<double> Q0(static_cast<size_t>(d) * b);
     SmallRng rng = SmallRng::seed_from_u64(0x5eedull);
     for (auto &e : Q0) e = rng.next_f64() - 0.5;
-    const uint64_t part_cap = 16ull * std::max<uint64_t>(static_cast<uint64_t>(b) * b, static_cast<uint64_t>(d) * b / 4);
     DevBuf<double> Q(Q0.size()), Z(Q0.size()), G(static_cast<size_t>(b) * b), Rinv(static_cast<size_t>(b) * b),
-        U(static_cast<size_t>(b) * q), part(part_cap);
+        U(static_cast<size_t>(b) * q);
     upload(Q, Q0.data(), Q0.size(), s);
     auto gemm = [&](uint32_t M, uint32_t N, uint32_t K, const double *A, uint32_t lda, bool tA, const double *B,
                     uint32_t ldb, double *C, uint32_t ldc) {
-      PM_HIP(launch_gemm64(M, N, K, A, lda, tA, B, ldb, false, C, ldc, s, part.get(), part_cap));
+      PM_HIP(launch_gemm64(M, N, K, A, lda, tA, B, ldb, C, ldc, s));
     };
     // Q <- orthonormal basis of the columns of Z by CholeskyQR (Q = Z R^-1);
     // `twice` repeats it on Q (CholeskyQR2) for full orthogonality.  Inside
@@ -581,12 +580,14 @@ void pca_axes_dev(const float *x_dev, uint64_t n, uint32_t d, uint32_t q, PcaAxe
     // the random start is not orthonormalised: the iteration depends only on
     // its span, and a d x b uniform random block is well conditioned
     // (singular values within ~(sqrt d +- sqrt b))
+    // the covariance is exactly symmetric (cov_fin_kernel writes both halves
+    // from one sum), so it is read as transposed: 128-byte rows per load
     for (uint32_t it = 0; it < kIters; ++it) {
-      gemm(d, b, d, d_cov.get(), d, false, Q.get(), b, Z.get(), b);
+      gemm(d, b, d, d_cov.get(), d, true, Q.get(), b, Z.get(), b);
       orth(it + 1 == kIters);
     }
     // Rayleigh-Ritz: T = Q^T C Q, eigenvectors U, axes V = Q U
-    gemm(d, b, d, d_cov.get(), d, false, Q.get(), b, Z.get(), b);
+    gemm(d, b, d, d_cov.get(), d, true, Q.get(), b, Z.get(), b);
     gemm(b, b, d, Q.get(), b, true, Z.get(), b, G.get(), b);
     PM_HIP(launch_jacobi(G.get(), b, q, U.get(), jwork.get(), s));
     gemm(d, q, b, Q.get(), b, false, U.get(), q, V.get(), q);

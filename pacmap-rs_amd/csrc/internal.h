@@ -197,11 +197,10 @@ hipError_t launch_take_cols(const float *src, uint64_t n, uint32_t ld, uint32_t 
                             float *dst, hipStream_t s);
 
 // Small dense FP64 linear algebra (linalg.hip)
-// C = op(A) op(B); with a scratch `part` of part_cap doubles the K range is
-// split over several workgroups and the partials summed in a fixed order.
+// C = op(A) B, row-major FP64 (the K range is split inside each tile's
+// workgroup and the partials summed in a fixed order)
 hipError_t launch_gemm64(uint32_t M, uint32_t N, uint32_t K, const double *A, uint32_t lda, bool tA,
-                         const double *B, uint32_t ldb, bool tB, double *C, uint32_t ldc, hipStream_t s,
-                         double *part = nullptr, uint64_t part_cap = 0);
+                         const double *B, uint32_t ldb, double *C, uint32_t ldc, hipStream_t s);
 // G (b x b, b <= 128) = R^T R -> R^{-1}; status[0] = 1 when factoring fails
 hipError_t launch_chol_inv(const double *G, uint32_t b, double *Rinv, uint32_t *status, hipStream_t s);
 // q leading eigenvectors of symmetric A (m x m, m <= jacobi_max_size()) as

@@ -13,90 +13,88 @@
 namespace pm {
 
 // ---------------------------------------------------------------------------
-// C (M x N) = op(A) (M x K) * op(B) (K x N), row-major FP64, 32 x 32 tiles.
-// blockIdx.z takes the K range [z * kchunk, (z + 1) * kchunk) and writes the
-// z-th M x ldc slab of C (split-K partials, summed by sum_parts_kernel).
+// C (M x N) = op(A) (M x K) * B (K x N), row-major FP64, on the FP64 matrix
+// cores: one 512-thread block per 16 x 32 tile of C (small tiles: these
+// products have few tiles, and more blocks reach more CUs).  The block's 8
+// waves split K into contiguous ranges of 16-deep blocks; each accumulates
+// the whole tile as two v_mfma_f64_16x16x4 products, and the 8 partial tiles
+// are summed in LDS in wave order (a fixed order: the result does not depend
+// on timing).  Within a 16-deep block, step s takes k = 4 (l / 16) + s at
+// lane l, so a lane's four A values of a row are contiguous when A is not
+// transposed.  Operand lane l holds element (l % 16, l / 16) of A (16 x 4)
+// and (l / 16, l % 16) of B (4 x 16); the result lane l holds rows
+// l / 16 + 4 v of column l % 16.  The next block's operands are loaded into
+// registers while the current one is multiplied.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void gemm64_kernel(uint32_t M, uint32_t N, uint32_t K,
-                                                     const double *__restrict__ A, uint32_t lda, int tA,
-                                                     const double *__restrict__ B, uint32_t ldb, int tB,
-                                                     double *__restrict__ C, uint32_t ldc, uint32_t kchunk) {
-  constexpr int TB = 32, KB = 32;
-  __shared__ double sA[KB][TB + 1];
-  __shared__ double sB[KB][TB + 1];
-  const int t = threadIdx.x, ty = t >> 4, tx = t & 15;
-  const uint32_t r0 = blockIdx.x * TB, c0 = blockIdx.y * TB;
-  const uint32_t kbeg = blockIdx.z * kchunk;
-  const uint32_t kend = kbeg + kchunk < K ? kbeg + kchunk : K;
-  C += static_cast<size_t>(blockIdx.z) * M * ldc;
-  double acc[2][2] = {{0.0, 0.0}, {0.0, 0.0}};
-  for (uint32_t k0 = kbeg; k0 < kend; k0 += KB) {
-    __syncthreads();
-    for (int li = t; li < TB * KB; li += 256) {
-      const int a = li / KB, b = li % KB;  // tile row / k
-      const uint32_t r = r0 + a, k = k0 + b;
-      double v = 0.0;
-      if (r < M && k < kend) v = tA ? A[static_cast<size_t>(k) * lda + r] : A[static_cast<size_t>(r) * lda + k];
-      sA[b][a] = v;
-      const uint32_t kk = k0 + a, c = c0 + b;
-      double w = 0.0;
-      if (kk < kend && c < N) w = tB ? B[static_cast<size_t>(c) * ldb + kk] : B[static_cast<size_t>(kk) * ldb + c];
-      sB[a][b] = w;
-    }
-    __syncthreads();
-#pragma unroll 8
-    for (int kk = 0; kk < KB; ++kk) {
-      const double a0 = sA[kk][ty * 2], a1 = sA[kk][ty * 2 + 1];
-      const double b0 = sB[kk][tx * 2], b1 = sB[kk][tx * 2 + 1];
-      acc[0][0] = fma(a0, b0, acc[0][0]);
-      acc[0][1] = fma(a0, b1, acc[0][1]);
-      acc[1][0] = fma(a1, b0, acc[1][0]);
-      acc[1][1] = fma(a1, b1, acc[1][1]);
-    }
-  }
+constexpr int kG64Waves = 8;
+typedef double doublex4 __attribute__((ext_vector_type(4)));
+
+__global__ __launch_bounds__(kG64Waves * 64) void gemm64_mfma_kernel(uint32_t M, uint32_t N, uint32_t K,
+                                                                     const double *__restrict__ A, uint32_t lda,
+                                                                     int tA, const double *__restrict__ B,
+                                                                     uint32_t ldb, double *__restrict__ C,
+                                                                     uint32_t ldc) {
+  __shared__ double sP[kG64Waves][16 * 32];
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const int m = lane & 15, g = lane >> 4;
+  const uint32_t m0 = blockIdx.x * 16, n0 = blockIdx.y * 32;
+  const uint32_t nkb = (K + 15) / 16;
+  const uint32_t kb0 = nkb * wave / kG64Waves, kb1 = nkb * (wave + 1) / kG64Waves;
+  double pa[4], pb[2][4];
+  auto fetch = [&](uint32_t kb) __attribute__((always_inline)) {
+    const uint32_t kbase = kb * 16 + 4 * g;
+    const uint32_t r = m0 + m;
 #pragma unroll
-  for (int i = 0; i < 2; ++i)
+    for (int s2 = 0; s2 < 4; ++s2) {
+      const uint32_t k = kbase + s2;
+      const size_t off = tA ? static_cast<size_t>(k) * lda + r : static_cast<size_t>(r) * lda + k;
+      pa[s2] = (r < M && k < K) ? A[off] : 0.0;
+    }
 #pragma unroll
     for (int j = 0; j < 2; ++j) {
-      const uint32_t r = r0 + ty * 2 + i, c = c0 + tx * 2 + j;
-      if (r < M && c < N) C[static_cast<size_t>(r) * ldc + c] = acc[i][j];
+      const uint32_t c = n0 + 16 * j + m;
+#pragma unroll
+      for (int s2 = 0; s2 < 4; ++s2) {
+        const uint32_t k = kbase + s2;
+        pb[j][s2] = (c < N && k < K) ? B[static_cast<size_t>(k) * ldb + c] : 0.0;
+      }
     }
-}
-
-// C[r][c] = sum over z = 0..ks-1 (in order) of part[z][r][c]
-__global__ void sum_parts_kernel(const double *__restrict__ part, uint32_t ks, uint32_t M, uint32_t N,
-                                 double *__restrict__ C, uint32_t ldc) {
-  const uint64_t e = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  const uint64_t mn = static_cast<uint64_t>(M) * N;
-  if (e >= mn) return;
-  double acc = part[e];
-  for (uint32_t z = 1; z < ks; ++z) acc += part[z * mn + e];
-  C[(e / N) * ldc + e % N] = acc;
+  };
+  doublex4 acc[2] = {doublex4{0.0, 0.0, 0.0, 0.0}, doublex4{0.0, 0.0, 0.0, 0.0}};
+  if (kb0 < kb1) fetch(kb0);
+  for (uint32_t kb = kb0; kb < kb1; ++kb) {
+    double a[4], b[2][4];
+#pragma unroll
+    for (int s2 = 0; s2 < 4; ++s2) {
+      a[s2] = pa[s2];
+      b[0][s2] = pb[0][s2];
+      b[1][s2] = pb[1][s2];
+    }
+    if (kb + 1 < kb1) fetch(kb + 1);
+#pragma unroll
+    for (int s2 = 0; s2 < 4; ++s2)
+#pragma unroll
+      for (int j = 0; j < 2; ++j) acc[j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[s2], b[j][s2], acc[j], 0, 0, 0);
+  }
+#pragma unroll
+  for (int j = 0; j < 2; ++j)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) sP[wave][(g + 4 * v) * 32 + 16 * j + m] = acc[j][v];
+  __syncthreads();
+  for (int e = t; e < 16 * 32; e += kG64Waves * 64) {
+    double sum = sP[0][e];
+#pragma unroll
+    for (int w = 1; w < kG64Waves; ++w) sum += sP[w][e];
+    const uint32_t r = m0 + e / 32, c = n0 + e % 32;
+    if (r < M && c < N) C[static_cast<size_t>(r) * ldc + c] = sum;
+  }
 }
 
 hipError_t launch_gemm64(uint32_t M, uint32_t N, uint32_t K, const double *A, uint32_t lda, bool tA,
-                         const double *B, uint32_t ldb, bool tB, double *C, uint32_t ldc, hipStream_t s,
-                         double *part, uint64_t part_cap) {
+                         const double *B, uint32_t ldb, double *C, uint32_t ldc, hipStream_t s) {
   if (M == 0 || N == 0) return hipSuccess;
-  const uint32_t tiles = ((M + 31) / 32) * ((N + 31) / 32);
-  uint64_t ks = (256 + tiles - 1) / tiles;
-  ks = ks < (K + 63) / 64 ? ks : (K + 63) / 64;
-  const uint64_t mn = static_cast<uint64_t>(M) * N;
-  if (part == nullptr || part_cap < 2 * mn) ks = 1;
-  else if (ks * mn > part_cap) ks = part_cap / mn;
-  if (ks < 1) ks = 1;
-  const uint32_t kchunk = static_cast<uint32_t>((((K + ks - 1) / ks) + 31) / 32 * 32);
-  ks = (K + kchunk - 1) / kchunk;  // no empty slabs
-  if (ks < 1) ks = 1;
-  const dim3 grid((M + 31) / 32, (N + 31) / 32, static_cast<uint32_t>(ks));
-  if (ks == 1) {
-    gemm64_kernel<<<grid, 256, 0, s>>>(M, N, K, A, lda, tA ? 1 : 0, B, ldb, tB ? 1 : 0, C, ldc,
-                                       K > 0 ? K : 1);
-    return hipGetLastError();
-  }
-  gemm64_kernel<<<grid, 256, 0, s>>>(M, N, K, A, lda, tA ? 1 : 0, B, ldb, tB ? 1 : 0, part, N, kchunk);
-  sum_parts_kernel<<<static_cast<uint32_t>((mn + 255) / 256), 256, 0, s>>>(part, static_cast<uint32_t>(ks), M, N,
-                                                                           C, ldc);
+  const dim3 grid((M + 15) / 16, (N + 31) / 32);
+  gemm64_mfma_kernel<<<grid, kG64Waves * 64, 0, s>>>(M, N, K, A, lda, tA ? 1 : 0, B, ldb, C, ldc);
   return hipGetLastError();
 }
 
