@@ -179,10 +179,19 @@ __global__ __launch_bounds__(512) void chol_inv_kernel(const double *__restrict_
           const uint32_t r = rg + 16 * a;
           f[a] = r > j ? rb[r] * inv : 0.0;
         }
+        // Only rows below j change, and only 32-column blocks that can hold
+        // live entries: left columns > j (columns <= j are never read
+        // again) and right columns <= j (row j of E = R^-T is zero beyond
+        // its diagonal).  Those form the block range [elo, ehi].
+        const uint32_t elo = (j + 1) / 32, ehi = 4 + j / 32;
 #pragma unroll
-        for (int a = 0; a < 8; ++a)
+        for (int a = 0; a < 8; ++a) {
+          if (static_cast<uint32_t>(rg + 16 * a) <= j) continue;
 #pragma unroll
-          for (int e = 0; e < 8; ++e) M[a][e] = fma(-f[a], rc[e], M[a][e]);
+          for (int e = 0; e < 8; ++e)
+            if (static_cast<uint32_t>(e) >= elo && static_cast<uint32_t>(e) <= ehi)
+              M[a][e] = fma(-f[a], rc[e], M[a][e]);
+        }
       }
       if (!ok) break;
     }
