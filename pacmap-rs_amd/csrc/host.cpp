@@ -266,7 +266,8 @@ struct DevCsr {
 void build_csr_dev(uint64_t n, const uint32_t *const dp[3], const uint64_t rows[3], DevCsr &out,
                    hipStream_t s) {
   if (n >= kMaxPoints) fail(PACMAP_ERR_UNSUPPORTED, "too many points for the CSR entry format");
-  const double t0 = now_ms();
+  Timer tcsr;  // stream time from the first CSR launch to the last
+  tcsr.start(s);
   DevBuf<uint32_t> deg(n), flags(2), scratch((n + 1023) / 1024 + 2), cursor(n);
   PM_HIP(launch_fill_zero(deg.get(), deg.bytes(), s));
   PM_HIP(launch_fill_zero(flags.get(), flags.bytes(), s));
@@ -299,7 +300,8 @@ void build_csr_dev(uint64_t n, const uint32_t *const dp[3], const uint64_t rows[
     upload(out.ent, h.ent.data(), h.ent.size(), s);
     PM_HIP(hipStreamSynchronize(s));
   }
-  g_tm.csr_ms += now_ms() - t0;
+  tcsr.stop(s);
+  tcsr.add_to(g_tm.csr_ms);
   g_tm.csr_entries = nnz;
 }
 
