@@ -250,8 +250,11 @@ struct JacobiWork {
 __global__ __launch_bounds__(kJacThreads) void jacobi_rot_kernel(const double *__restrict__ Ain, uint32_t m,
                                                                  JacobiWork w) {
   __shared__ double Ap[kJacMax * (kJacMax + 1) / 2];
-  __shared__ double cs[kJacPairs], sn[kJacPairs], tn[kJacPairs];
-  __shared__ int pp[kJacPairs], qq[kJacPairs];
+  // round rotations: (p, q, rowbase(p), rowbase(q)) and (c, s); rowbase(x)
+  // = packed_idx(x, x, m) - x, so packed_idx(x, y, m) = rowbase(x) + y for x <= y
+  __shared__ int4 rq[kJacPairs];
+  __shared__ double2 rcs[kJacPairs];
+  __shared__ double tn[kJacPairs];
   __shared__ double red[2][kJacThreads / 64];
   const int t = threadIdx.x;
   const uint32_t M = (m + 1) & ~1u;  // even count; index m is a dummy when m is odd
@@ -343,10 +346,9 @@ __global__ __launch_bounds__(kJacThreads) void jacobi_rot_kernel(const double *_
             s = tt * c;
           }
         }
-        pp[t] = static_cast<int>(p);
-        qq[t] = static_cast<int>(r2);
-        cs[t] = c;
-        sn[t] = s;
+        auto rowbase = [&](uint32_t x) { return static_cast<int>(x * m - (x * (x - 1)) / 2 - x); };
+        rq[t] = make_int4(static_cast<int>(p), static_cast<int>(r2), rowbase(p), r2 < m ? rowbase(r2) : 0);
+        rcs[t] = make_double2(c, s);
         tn[t] = tt;
         const size_t rec = static_cast<size_t>(grnd) * h + t;
         w.cs[rec] = make_double2(c, s);
@@ -359,13 +361,18 @@ __global__ __launch_bounds__(kJacThreads) void jacobi_rot_kernel(const double *_
         if (code == 0xffffffffu) continue;
         const uint32_t type = code >> 24, a = (code >> 12) & 0xfff, bb = code & 0xfff;
         if (type == 0) {
-          const uint32_t p1 = pp[a], q1 = qq[a], p2 = pp[bb], q2 = qq[bb];
-          const double c1 = cs[a], s1 = sn[a], c2 = cs[bb], s2 = sn[bb];
+          const int4 ra = rq[a], rb4 = rq[bb];
+          const double2 ca = rcs[a], cb = rcs[bb];
+          const uint32_t p1 = ra.x, q1 = ra.y, p2 = rb4.x, q2 = rb4.y;
+          const double c1 = ca.x, s1 = ca.y, c2 = cb.x, s2 = cb.y;
           const bool h1 = q1 < m, h2 = q2 < m;  // false: the pair holds the dummy index
-          const uint32_t i00 = packed_idx(p1, p2, m);
-          const uint32_t i01 = h2 ? packed_idx(p1, q2, m) : 0;
-          const uint32_t i10 = h1 ? packed_idx(q1, p2, m) : 0;
-          const uint32_t i11 = (h1 && h2) ? packed_idx(q1, q2, m) : 0;
+          auto idx = [](uint32_t x, int bx, uint32_t y, int by) -> uint32_t {
+            return x <= y ? static_cast<uint32_t>(bx) + y : static_cast<uint32_t>(by) + x;
+          };
+          const uint32_t i00 = idx(p1, ra.z, p2, rb4.z);
+          const uint32_t i01 = h2 ? idx(p1, ra.z, q2, rb4.w) : 0;
+          const uint32_t i10 = h1 ? idx(q1, ra.w, p2, rb4.z) : 0;
+          const uint32_t i11 = (h1 && h2) ? idx(q1, ra.w, q2, rb4.w) : 0;
           const double b00 = Ap[i00], b01 = h2 ? Ap[i01] : 0.0;
           const double b10 = h1 ? Ap[i10] : 0.0, b11 = (h1 && h2) ? Ap[i11] : 0.0;
           // rows (rotation a), then columns (rotation bb)
@@ -376,10 +383,11 @@ __global__ __launch_bounds__(kJacThreads) void jacobi_rot_kernel(const double *_
           if (h1) Ap[i10] = c2 * r10 - s2 * r11;
           if (h1 && h2) Ap[i11] = s2 * r10 + c2 * r11;
         } else {
-          const uint32_t p = pp[a], r2 = qq[a];
+          const int4 ra = rq[a];
+          const uint32_t p = ra.x, r2 = ra.y;  // p < r2
           const double tt = tn[a];
           if (r2 < m && tt != 0.0) {
-            const uint32_t ipq = packed_idx(p, r2, m), ipp = packed_idx(p, p, m), iqq = packed_idx(r2, r2, m);
+            const uint32_t ipq = ra.z + r2, ipp = ra.z + p, iqq = ra.w + r2;
             const double apq = Ap[ipq];
             Ap[ipp] = Ap[ipp] - tt * apq;
             Ap[iqq] = Ap[iqq] + tt * apq;
