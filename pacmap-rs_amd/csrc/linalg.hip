@@ -247,6 +247,9 @@ struct JacobiWork {
   uint32_t *rounds;   // 1
 };
 
+// ODD: m is odd, so every round pairs one index with the dummy index m
+constexpr int kJacNormSlots = (kJacMax * (kJacMax + 1) / 2 + kJacThreads - 1) / kJacThreads;
+template <bool ODD>
 __global__ __launch_bounds__(kJacThreads) void jacobi_rot_kernel(const double *__restrict__ Ain, uint32_t m,
                                                                  JacobiWork w) {
   __shared__ double Ap[kJacMax * (kJacMax + 1) / 2];
@@ -295,15 +298,29 @@ __global__ __launch_bounds__(kJacThreads) void jacobi_rot_kernel(const double *_
     }
     slot[k] = code;
   }
+  // the packed entries this thread adds into the sweep's square sums:
+  // e = t + 1024 k, flagged when e is a diagonal entry
+  uint32_t nidx[kJacNormSlots], nvalid = 0, ndiag = 0;
+#pragma unroll
+  for (int k = 0; k < kJacNormSlots; ++k) {
+    const uint32_t e = t + k * kJacThreads;
+    nidx[k] = 0;
+    if (e < m * (m + 1) / 2) {
+      nidx[k] = e;
+      nvalid |= 1u << k;
+      for (uint32_t r = 0; r < m; ++r)
+        if (packed_idx(r, r, m) == e) ndiag |= 1u << k;
+    }
+  }
   __syncthreads();
   uint32_t grnd = 0;  // rounds recorded
   for (int sweep = 0; sweep < kJacSweeps; ++sweep) {
     double off = 0.0, dg = 0.0;
-    for (uint32_t r = 0; r < m; ++r) {  // thread t takes columns c = t, t + 1024, ... of every row
-      const uint32_t base = r * m - (r * (r - 1)) / 2;
-      for (uint32_t c = r + t; c < m; c += kJacThreads) {
-        const double v = Ap[base + (c - r)];
-        if (c == r) dg += v * v;
+#pragma unroll
+    for (int k = 0; k < kJacNormSlots; ++k) {
+      const double v = Ap[nidx[k]];
+      if (nvalid >> k & 1u) {
+        if (ndiag >> k & 1u) dg += v * v;
         else off += v * v;
       }
     }
@@ -332,7 +349,7 @@ __global__ __launch_bounds__(kJacThreads) void jacobi_rot_kernel(const double *_
           r2 = tmp;
         }
         double c = 1.0, s = 0.0, tt = 0.0;
-        if (r2 < m) {
+        if (!ODD || r2 < m) {
           const double apq = Ap[packed_idx(p, r2, m)];
           if (apq != 0.0) {
             // tan of the rotation angle, the smaller root of t^2 + 2 theta t - 1 = 0
@@ -347,7 +364,8 @@ __global__ __launch_bounds__(kJacThreads) void jacobi_rot_kernel(const double *_
           }
         }
         auto rowbase = [&](uint32_t x) { return static_cast<int>(x * m - (x * (x - 1)) / 2 - x); };
-        rq[t] = make_int4(static_cast<int>(p), static_cast<int>(r2), rowbase(p), r2 < m ? rowbase(r2) : 0);
+        rq[t] = make_int4(static_cast<int>(p), static_cast<int>(r2), rowbase(p),
+                          (!ODD || r2 < m) ? rowbase(r2) : 0);
         rcs[t] = make_double2(c, s);
         tn[t] = tt;
         const size_t rec = static_cast<size_t>(grnd) * h + t;
@@ -365,7 +383,7 @@ __global__ __launch_bounds__(kJacThreads) void jacobi_rot_kernel(const double *_
           const double2 ca = rcs[a], cb = rcs[bb];
           const uint32_t p1 = ra.x, q1 = ra.y, p2 = rb4.x, q2 = rb4.y;
           const double c1 = ca.x, s1 = ca.y, c2 = cb.x, s2 = cb.y;
-          const bool h1 = q1 < m, h2 = q2 < m;  // false: the pair holds the dummy index
+          const bool h1 = !ODD || q1 < m, h2 = !ODD || q2 < m;  // false: the pair holds the dummy index
           auto idx = [](uint32_t x, int bx, uint32_t y, int by) -> uint32_t {
             return x <= y ? static_cast<uint32_t>(bx) + y : static_cast<uint32_t>(by) + x;
           };
@@ -386,7 +404,7 @@ __global__ __launch_bounds__(kJacThreads) void jacobi_rot_kernel(const double *_
           const int4 ra = rq[a];
           const uint32_t p = ra.x, r2 = ra.y;  // p < r2
           const double tt = tn[a];
-          if (r2 < m && tt != 0.0) {
+          if ((!ODD || r2 < m) && tt != 0.0) {
             const uint32_t ipq = ra.z + r2, ipp = ra.z + p, iqq = ra.w + r2;
             const double apq = Ap[ipq];
             Ap[ipp] = Ap[ipp] - tt * apq;
@@ -553,7 +571,10 @@ hipError_t launch_jacobi(const double *A, uint32_t m, uint32_t q, double *Vout, 
   w.vt = work + 2 * nr + (nr + 1) / 2;
   w.evals = w.vt + static_cast<size_t>(m) * m;
   w.rounds = reinterpret_cast<uint32_t *>(w.evals + m);
-  jacobi_rot_kernel<<<1, kJacThreads, 0, s>>>(A, m, w);
+  if (m & 1)
+    jacobi_rot_kernel<true><<<1, kJacThreads, 0, s>>>(A, m, w);
+  else
+    jacobi_rot_kernel<false><<<1, kJacThreads, 0, s>>>(A, m, w);
   jacobi_vec_kernel<<<m, 64, 0, s>>>(m, w);
   jacobi_out_kernel<<<1, 128, 0, s>>>(m, q, w, Vout);
   return hipGetLastError();

@@ -619,7 +619,8 @@ def _pca_init_expected(x, comps):
     return xc @ v[:, :2] * 0.01, w[order]
 
 
-@pytest.mark.parametrize("n,d", [(4000, 60), (4000, 120), (3000, 300)])
+@pytest.mark.parametrize("n,d", [(4000, 60), (4000, 120), (3000, 300),
+                                 (4000, 61), (3000, 105)])  # odd sizes: a dummy index per Jacobi round
 def test_pca_initialisation_matches_fp64_eigensolve(n, d):
     """With 0 iterations fit_transform returns its PCA initialisation.  d <= 100
     normalises and takes an exact 2-component PCA (lib.rs:489-519); d > 100
