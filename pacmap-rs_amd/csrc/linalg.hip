@@ -341,31 +341,40 @@ __global__ __launch_bounds__(kJacThreads) void jacobi_rot_kernel(const double *_
     if (offs == 0.0 || offs <= 1e-24 * dgs) break;  // uniform: every thread summed the same values
     for (uint32_t round = 0; round + 1 < M; ++round, ++grnd) {
       if (static_cast<uint32_t>(t) < h) {
-        auto pos = [&](uint32_t j) -> uint32_t { return j == 0 ? 0 : 1 + (j - 1 + round) % (M - 1); };
+        // round-robin position of slot j: 1 + (j - 1 + round) mod (M - 1),
+        // where j - 1 + round < 2 (M - 1)
+        auto pos = [&](uint32_t j) -> uint32_t {
+          if (j == 0) return 0;
+          uint32_t x = j - 1 + round;
+          if (x >= M - 1) x -= M - 1;
+          return 1 + x;
+        };
         uint32_t p = pos(t), r2 = pos(M - 1 - t);
         if (p > r2) {
           const uint32_t tmp = p;
           p = r2;
           r2 = tmp;
         }
-        double c = 1.0, s = 0.0, tt = 0.0;
-        if (!ODD || r2 < m) {
-          const double apq = Ap[packed_idx(p, r2, m)];
-          if (apq != 0.0) {
-            // tan of the rotation angle, the smaller root of t^2 + 2 theta t - 1 = 0
-            // with theta = (aqq - app) / (2 apq), written as 2 apq sgn(dd) / (|dd| +
-            // sqrt(dd^2 + 4 apq^2)) to save a division
-            const double dd = Ap[packed_idx(r2, r2, m)] - Ap[packed_idx(p, p, m)];
-            const double a2 = 2.0 * apq;
-            const double den = fabs(dd) + sqrt(fma(dd, dd, a2 * a2));
-            tt = (dd >= 0.0 ? a2 : -a2) / den;
-            c = rsqrt(fma(tt, tt, 1.0));
-            s = tt * c;
-          }
-        }
+        const bool live = !ODD || r2 < m;  // false: r2 is the dummy index
         auto rowbase = [&](uint32_t x) { return static_cast<int>(x * m - (x * (x - 1)) / 2 - x); };
-        rq[t] = make_int4(static_cast<int>(p), static_cast<int>(r2), rowbase(p),
-                          (!ODD || r2 < m) ? rowbase(r2) : 0);
+        const int bp = rowbase(p), bq = live ? rowbase(r2) : 0;
+        // the three entries are read together (packed_idx(x, y) = rowbase(x) + y, x <= y)
+        const double apq = live ? Ap[bp + r2] : 0.0;
+        const double app = Ap[bp + p];
+        const double aqq = live ? Ap[bq + r2] : 0.0;
+        double c = 1.0, s = 0.0, tt = 0.0;
+        if (apq != 0.0) {
+          // tan of the rotation angle, the smaller root of t^2 + 2 theta t - 1 = 0
+          // with theta = (aqq - app) / (2 apq), written as 2 apq sgn(dd) / (|dd| +
+          // sqrt(dd^2 + 4 apq^2)) to save a division
+          const double dd = aqq - app;
+          const double a2 = 2.0 * apq;
+          const double den = fabs(dd) + sqrt(fma(dd, dd, a2 * a2));
+          tt = (dd >= 0.0 ? a2 : -a2) / den;
+          c = rsqrt(fma(tt, tt, 1.0));
+          s = tt * c;
+        }
+        rq[t] = make_int4(static_cast<int>(p), static_cast<int>(r2), bp, bq);
         rcs[t] = make_double2(c, s);
         tn[t] = tt;
         const size_t rec = static_cast<size_t>(grnd) * h + t;
