@@ -1618,8 +1618,16 @@ hipError_t launch_sub_mean(float *x, uint64_t n, uint32_t d, const float *mean, 
 constexpr uint32_t kCovTile = 64;
 constexpr uint32_t kCovRowsPerSplit = 2048;
 
-static uint32_t cov_splits(uint64_t n) {
+// Row splits of the covariance: about kCovRowsPerSplit rows each, rounded
+// so the (upper tile, split) blocks come closest to whole rounds of the
+// resident slots (256 CUs x 4 blocks of cov_part_kernel on MI355X); a
+// partly filled last round idles most of the chip for a whole block time.
+constexpr uint64_t kCovSlots = 1024;
+static uint32_t cov_splits(uint64_t n, uint32_t d) {
+  const uint64_t tiles = (d + kCovTile - 1) / kCovTile, upper = tiles * (tiles + 1) / 2;
   uint64_t s = (n + kCovRowsPerSplit - 1) / kCovRowsPerSplit;
+  const uint64_t rounds = (s * upper + kCovSlots / 2) / kCovSlots;
+  if (rounds >= 1 && rounds * kCovSlots >= upper) s = rounds * kCovSlots / upper;
   if (s < 1) s = 1;
   if (s > 64) s = 64;
   return static_cast<uint32_t>(s);
@@ -1634,7 +1642,7 @@ static uint32_t colsum_splits(uint64_t n) {
 }
 
 uint64_t cov_scratch_doubles(uint64_t n, uint32_t d) {
-  const uint64_t s = cov_splits(n);
+  const uint64_t s = cov_splits(n, d);
   const uint64_t cov = s * static_cast<uint64_t>(d) * d;
   const uint64_t col = static_cast<uint64_t>(colsum_splits(n)) * d;
   return cov > col ? cov : col;
@@ -1694,8 +1702,14 @@ typedef double doublex4 __attribute__((ext_vector_type(4)));
 __global__ __launch_bounds__(256) void cov_part_kernel(const float *__restrict__ x, uint64_t n,
                                                        uint32_t d, const double *__restrict__ mean,
                                                        uint32_t splits, double *__restrict__ part) {
-  const uint32_t ti = blockIdx.x, tj = blockIdx.y, sp = blockIdx.z;
-  if (tj < ti) return;
+  // blockIdx.x enumerates the upper tiles (ti <= tj) row by row
+  const uint32_t tiles = (d + kCovTile - 1) / kCovTile, sp = blockIdx.y;
+  uint32_t ti = 0, u = blockIdx.x;
+  while (u >= tiles - ti) {
+    u -= tiles - ti;
+    ++ti;
+  }
+  const uint32_t tj = ti + u;
   constexpr int RB = 32;
   __shared__ double sA[RB][kCovTile];
   __shared__ double sB[RB][kCovTile];
@@ -1781,9 +1795,9 @@ __global__ void cov_fin_kernel(const double *__restrict__ part, uint32_t d, uint
 
 hipError_t launch_cov_f64(const float *x, uint64_t n, uint32_t d, const double *mean,
                           double *scratch, double *cov, hipStream_t s) {
-  const uint32_t splits = cov_splits(n);
+  const uint32_t splits = cov_splits(n, d);
   const uint32_t tiles = (d + kCovTile - 1) / kCovTile;
-  dim3 g(tiles, tiles, splits);
+  dim3 g(tiles * (tiles + 1) / 2, splits);
   cov_part_kernel<<<g, 256, 0, s>>>(x, n, d, mean, splits, scratch);
   const uint64_t total = static_cast<uint64_t>(d) * d;
   cov_fin_kernel<<<static_cast<uint32_t>((total + 255) / 256), 256, 0, s>>>(scratch, d, splits, cov);
