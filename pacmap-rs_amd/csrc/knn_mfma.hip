@@ -501,7 +501,7 @@ void knn_mfma_kernel(
 // ---------------------------------------------------------------------------
 template <int R>
 __global__ __launch_bounds__(256) void knn_rerank_kernel(
-    const float *__restrict__ xp, uint32_t Dp, uint32_t Cp, uint32_t n, uint32_t k, uint32_t capg,
+    const float *__restrict__ xp, uint32_t Dp, uint32_t Cp, uint32_t tail, uint32_t n, uint32_t k, uint32_t capg,
     uint32_t nsplit, const uint32_t *__restrict__ gcnt, const uint32_t *__restrict__ gcand,
     const float *__restrict__ gtau, double cert_scale, const float *__restrict__ arow, int dots,
     double ub_s1, double ub_s2, uint32_t *__restrict__ oidx, float *__restrict__ odist,
@@ -642,19 +642,25 @@ __global__ __launch_bounds__(256) void knn_rerank_kernel(
     atomicAdd(&stats[2 * (q & 255u) + 1], static_cast<unsigned long long>(cnt2));
   }
   // Eight lanes per candidate: lane l sums segment l (features 8c + l) in
-  // order of c, the eight partials are added in lane order, then the tail, as
-  // distance.rs:50-75 does; a group reads one packed row contiguously.
+  // order of c, the eight partials are added in lane order, then the tail
+  // (d mod 8 features; the row's padding beyond it is zero in both rows and
+  // adds nothing), as distance.rs:50-75 does; a group reads one packed row
+  // contiguously.  A candidate's loads are issued together, up to four
+  // 16-byte segment loads and its tail feature (lane l < tail holds feature
+  // 8 Cp + l), before any is used.
   const int grp = lane >> 3, l = lane & 7, g0 = lane & ~7;
   const float *qseg = qv + static_cast<size_t>(l) * Cp;
-  const uint32_t segEnd = 8u * Cp;
+  const uint32_t segEnd = 8u * Cp, nf4 = Cp / 4;
+  const bool tail_lane = static_cast<uint32_t>(l) < tail;
+  const float qtail = tail_lane ? qv[segEnd + l] : 0.0f;
   for (uint32_t e = grp; e < cnt2; e += 8) {
     const uint32_t c = list[e];
     const float *cr = xp + static_cast<size_t>(c) * Dp;
-    const float *cseg = cr + static_cast<size_t>(l) * Cp;
+    const float4 *cseg = reinterpret_cast<const float4 *>(cr + static_cast<size_t>(l) * Cp);
+    const float4 *qs4 = reinterpret_cast<const float4 *>(qseg);
+    const float ctail = tail_lane ? cr[segEnd + l] : 0.0f;
     float acc = 0.0f;
-    for (uint32_t f = 0; f < Cp; f += 4) {
-      const float4 cv = *reinterpret_cast<const float4 *>(cseg + f);
-      const float4 qq = *reinterpret_cast<const float4 *>(qseg + f);
+    auto add4 = [&](const float4 &cv, const float4 &qq) __attribute__((always_inline)) {
       const float cvs[4] = {cv.x, cv.y, cv.z, cv.w}, qvs[4] = {qq.x, qq.y, qq.z, qq.w};
 #pragma unroll
       for (int e2 = 0; e2 < 4; ++e2) {
@@ -662,15 +668,22 @@ __global__ __launch_bounds__(256) void knn_rerank_kernel(
         const float sq = df * df;
         acc = acc + sq;
       }
+    };
+    for (uint32_t f4 = 0; f4 < nf4; f4 += 4) {
+      float4 cv[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+        cv[u] = f4 + u < nf4 ? cseg[f4 + u] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+        if (f4 + u < nf4) add4(cv[u], qs4[f4 + u]);
     }
     float sd = 0.0f;
 #pragma unroll
     for (int w = 0; w < 8; ++w) sd = sd + __shfl(acc, g0 | w, 64);
-    for (uint32_t f = segEnd; f < Dp; ++f) {
-      const float df = qv[f] - cr[f];
-      const float sq = df * df;
-      sd = sd + sq;
-    }
+    const float dft = qtail - ctail;
+    const float sqt = dft * dft;
+    for (uint32_t f = 0; f < tail; ++f) sd = sd + __shfl(sqt, g0 | static_cast<int>(f), 64);
     if (l == 0) keys[e] = (static_cast<uint64_t>(tkey_m(sqrtf(sd))) << 32) | c;
   }
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
@@ -812,11 +825,11 @@ hipError_t launch_knn_rerank(const float *xp, const KnnLayout &L, uint32_t n, ui
   const size_t shm = static_cast<size_t>(4) * (L.Dp + 192 * R) * sizeof(float);
   if (nsplit > 16) return hipErrorInvalidValue;
   if (capg == 512)
-    knn_rerank_kernel<8><<<blocks, 256, shm, s>>>(xp, L.Dp, L.Cp, n, k, capg, nsplit, gcnt, gcand, gtau,
+    knn_rerank_kernel<8><<<blocks, 256, shm, s>>>(xp, L.Dp, L.Cp, L.R, n, k, capg, nsplit, gcnt, gcand, gtau,
                                                   cert_scale, arow, dots ? 1 : 0, ub_s1, ub_s2, idx, dist,
                                                   fb_count, fb_list, stats);
   else if (capg == 1024)
-    knn_rerank_kernel<16><<<blocks, 256, shm, s>>>(xp, L.Dp, L.Cp, n, k, capg, nsplit, gcnt, gcand, gtau,
+    knn_rerank_kernel<16><<<blocks, 256, shm, s>>>(xp, L.Dp, L.Cp, L.R, n, k, capg, nsplit, gcnt, gcand, gtau,
                                                    cert_scale, arow, dots ? 1 : 0, ub_s1, ub_s2, idx, dist,
                                                    fb_count, fb_list, stats);
   else
