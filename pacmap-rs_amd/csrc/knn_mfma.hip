@@ -653,38 +653,56 @@ __global__ __launch_bounds__(256) void knn_rerank_kernel(
   const uint32_t segEnd = 8u * Cp, nf4 = Cp / 4;
   const bool tail_lane = static_cast<uint32_t>(l) < tail;
   const float qtail = tail_lane ? qv[segEnd + l] : 0.0f;
-  for (uint32_t e = grp; e < cnt2; e += 8) {
-    const uint32_t c = list[e];
-    const float *cr = xp + static_cast<size_t>(c) * Dp;
-    const float4 *cseg = reinterpret_cast<const float4 *>(cr + static_cast<size_t>(l) * Cp);
+  // two candidates per group at a time (e and e + 8): twice the gathers in flight
+  for (uint32_t e = grp; e < cnt2; e += 16) {
+    const bool two = e + 8 < cnt2;  // the same in the group's eight lanes
+    uint32_t c[2];
+    c[0] = list[e];
+    c[1] = two ? list[e + 8] : c[0];
+    const float4 *cseg[2];
+    float ctail[2];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const float *cr = xp + static_cast<size_t>(c[h]) * Dp;
+      cseg[h] = reinterpret_cast<const float4 *>(cr + static_cast<size_t>(l) * Cp);
+      ctail[h] = tail_lane ? cr[segEnd + l] : 0.0f;
+    }
     const float4 *qs4 = reinterpret_cast<const float4 *>(qseg);
-    const float ctail = tail_lane ? cr[segEnd + l] : 0.0f;
-    float acc = 0.0f;
-    auto add4 = [&](const float4 &cv, const float4 &qq) __attribute__((always_inline)) {
+    float acc[2] = {0.0f, 0.0f};
+    auto add4 = [&](int h, const float4 &cv, const float4 &qq) __attribute__((always_inline)) {
       const float cvs[4] = {cv.x, cv.y, cv.z, cv.w}, qvs[4] = {qq.x, qq.y, qq.z, qq.w};
 #pragma unroll
       for (int e2 = 0; e2 < 4; ++e2) {
         const float df = qvs[e2] - cvs[e2];
         const float sq = df * df;
-        acc = acc + sq;
+        acc[h] = acc[h] + sq;
       }
     };
     for (uint32_t f4 = 0; f4 < nf4; f4 += 4) {
-      float4 cv[4];
+      float4 cv[2][4];
+#pragma unroll
+      for (int h = 0; h < 2; ++h)
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+          cv[h][u] = f4 + u < nf4 ? cseg[h][f4 + u] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 #pragma unroll
       for (int u = 0; u < 4; ++u)
-        cv[u] = f4 + u < nf4 ? cseg[f4 + u] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-#pragma unroll
-      for (int u = 0; u < 4; ++u)
-        if (f4 + u < nf4) add4(cv[u], qs4[f4 + u]);
+        if (f4 + u < nf4) {
+          const float4 qq = qs4[f4 + u];
+          add4(0, cv[0][u], qq);
+          add4(1, cv[1][u], qq);
+        }
     }
-    float sd = 0.0f;
 #pragma unroll
-    for (int w = 0; w < 8; ++w) sd = sd + __shfl(acc, g0 | w, 64);
-    const float dft = qtail - ctail;
-    const float sqt = dft * dft;
-    for (uint32_t f = 0; f < tail; ++f) sd = sd + __shfl(sqt, g0 | static_cast<int>(f), 64);
-    if (l == 0) keys[e] = (static_cast<uint64_t>(tkey_m(sqrtf(sd))) << 32) | c;
+    for (int h = 0; h < 2; ++h) {
+      float sd = 0.0f;
+#pragma unroll
+      for (int w = 0; w < 8; ++w) sd = sd + __shfl(acc[h], g0 | w, 64);
+      const float dft = qtail - ctail[h];
+      const float sqt = dft * dft;
+      for (uint32_t f = 0; f < tail; ++f) sd = sd + __shfl(sqt, g0 | static_cast<int>(f), 64);
+      if (l == 0 && (h == 0 || two)) keys[e + 8 * h] = (static_cast<uint64_t>(tkey_m(sqrtf(sd))) << 32) | c[h];
+    }
   }
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
   __builtin_amdgcn_wave_barrier();
