@@ -424,8 +424,11 @@ __global__ void mlp_pack_kernel(const float *__restrict__ p, uint32_t d, uint32_
 // four waves in order.  No intermediate buffer leaves the CU.
 // ---------------------------------------------------------------------------
 constexpr uint32_t kFusedH = 256;
+// 16-row tiles per transform block: 2 (32 rows) keeps the accumulators at 32
+// registers, so two waves share a SIMD
+constexpr int kTransformTiles = 2;
 
-template <int DIMS>
+template <int DIMS, int IT>
 __global__ __launch_bounds__(256) void mlp_transform_kernel(const float *__restrict__ x, uint64_t n, uint32_t d,
                                                             uint32_t Kp, const uint16_t *__restrict__ W1b,
                                                             const float *__restrict__ b1,
@@ -435,35 +438,36 @@ __global__ __launch_bounds__(256) void mlp_transform_kernel(const float *__restr
                                                             const float *__restrict__ b3, float *__restrict__ Y) {
   constexpr uint32_t dims = DIMS;
   constexpr uint32_t H = kFusedH, LH = H + 8;
+  constexpr uint32_t ROWS = 16 * IT;  // rows per block
   extern __shared__ __align__(16) unsigned char smem[];
   const uint32_t LX = Kp + 8;
-  uint16_t *sX = reinterpret_cast<uint16_t *>(smem);  // 64 x LX
-  uint16_t *sH = sX + 64 * LX;                         // 64 x LH
-  float *sY = reinterpret_cast<float *>(sH + 64 * LH); // 4 waves x 64 rows x dims
+  uint16_t *sX = reinterpret_cast<uint16_t *>(smem);  // ROWS x LX
+  uint16_t *sH = sX + ROWS * LX;  // ROWS x LH
+  float *sY = reinterpret_cast<float *>(sH + ROWS * LH);  // 4 waves x ROWS rows x dims
   const uint32_t t = threadIdx.x, wave = t >> 6, lane = t & 63;
-  const uint64_t r0 = static_cast<uint64_t>(blockIdx.x) * 64;
-  for (uint32_t e = t; e < 64 * Kp; e += 256) {
+  const uint64_t r0 = static_cast<uint64_t>(blockIdx.x) * ROWS;
+  for (uint32_t e = t; e < ROWS * Kp; e += 256) {
     const uint32_t rr = e / Kp, k = e % Kp;
     sX[rr * LX + k] = (r0 + rr < n && k < d) ? f2bf(x[(r0 + rr) * d + k]) : 0;
   }
   __syncthreads();
   const uint32_t lr = lane & 15, lk = 8 * (lane >> 4), c0 = wave * 64;
-  floatx4m acc[4][4];
+  floatx4m acc[IT][4];
   auto layer = [&](const uint16_t *sA, uint32_t la, const uint16_t *W, uint32_t K) {
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+    for (int i = 0; i < IT; ++i)
 #pragma unroll
       for (int j = 0; j < 4; ++j) acc[i][j] = floatx4m{0.f, 0.f, 0.f, 0.f};
     for (uint32_t k0 = 0; k0 < K; k0 += 32) {
-      bf16x8m a[4], b[4];
+      bf16x8m a[IT], b[4];
 #pragma unroll
-      for (int i = 0; i < 4; ++i)
+      for (int i = 0; i < IT; ++i)
         a[i] = __builtin_bit_cast(bf16x8m, *reinterpret_cast<const uint4 *>(sA + (16 * i + lr) * la + k0 + lk));
 #pragma unroll
       for (int j = 0; j < 4; ++j)
         b[j] = __builtin_bit_cast(bf16x8m, *reinterpret_cast<const uint4 *>(W + static_cast<size_t>(c0 + 16 * j + lr) * K + k0 + lk));
 #pragma unroll
-      for (int i = 0; i < 4; ++i)
+      for (int i = 0; i < IT; ++i)
 #pragma unroll
         for (int j = 0; j < 4; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
     }
@@ -471,7 +475,7 @@ __global__ __launch_bounds__(256) void mlp_transform_kernel(const float *__restr
   // layer 1 -> sH (bf16)
   layer(sX, LX, W1b, Kp);
 #pragma unroll
-  for (int i = 0; i < 4; ++i)
+  for (int i = 0; i < IT; ++i)
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
       const uint32_t col = c0 + 16 * j + lr;
@@ -482,9 +486,9 @@ __global__ __launch_bounds__(256) void mlp_transform_kernel(const float *__restr
   __syncthreads();
   // layer 2 with the output layer folded in
   layer(sH, LH, W2b, H);
-  float py[16][DIMS];
+  float py[4 * IT][DIMS];
 #pragma unroll
-  for (int q = 0; q < 16; ++q)
+  for (int q = 0; q < 4 * IT; ++q)
 #pragma unroll
     for (int c = 0; c < DIMS; ++c) py[q][c] = 0.0f;
 #pragma unroll
@@ -495,7 +499,7 @@ __global__ __launch_bounds__(256) void mlp_transform_kernel(const float *__restr
 #pragma unroll
     for (int c = 0; c < DIMS; ++c) w3[c] = W3[c * H + col];
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+    for (int i = 0; i < IT; ++i)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const float h = gelu(acc[i][j][r] + bias);
@@ -504,7 +508,7 @@ __global__ __launch_bounds__(256) void mlp_transform_kernel(const float *__restr
       }
   }
 #pragma unroll
-  for (int q = 0; q < 16; ++q)
+  for (int q = 0; q < 4 * IT; ++q)
 #pragma unroll
     for (int c = 0; c < DIMS; ++c) {
       float v = py[q][c];
@@ -516,21 +520,21 @@ __global__ __launch_bounds__(256) void mlp_transform_kernel(const float *__restr
     }
   if (lr == 0) {
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+    for (int i = 0; i < IT; ++i)
 #pragma unroll
       for (int r = 0; r < 4; ++r)
 #pragma unroll
         for (int c = 0; c < DIMS; ++c)
-          sY[(wave * 64 + 16 * i + 4 * (lane >> 4) + r) * dims + c] = py[4 * i + r][c];
+          sY[(wave * ROWS + 16 * i + 4 * (lane >> 4) + r) * dims + c] = py[4 * i + r][c];
   }
   __syncthreads();
-  for (uint32_t e = t; e < 64 * dims; e += 256) {
+  for (uint32_t e = t; e < ROWS * dims; e += 256) {
     const uint32_t rr = e / dims, c = e % dims;
     if (r0 + rr >= n) continue;
-    float v = sY[(0 * 64 + rr) * dims + c];
-    v += sY[(1 * 64 + rr) * dims + c];
-    v += sY[(2 * 64 + rr) * dims + c];
-    v += sY[(3 * 64 + rr) * dims + c];
+    float v = sY[(0 * ROWS + rr) * dims + c];
+    v += sY[(1 * ROWS + rr) * dims + c];
+    v += sY[(2 * ROWS + rr) * dims + c];
+    v += sY[(3 * ROWS + rr) * dims + c];
     Y[(r0 + rr) * dims + c] = v + b3[c];
   }
 }
@@ -639,12 +643,13 @@ hipError_t launch_mlp_transform(const float *x, uint64_t n, uint32_t d, uint32_t
                                 const float *b1, const uint16_t *W2b, const float *b2, const float *W3,
                                 const float *b3, uint32_t dims, float *Y, hipStream_t s) {
   if (!mlp_transform_fused_ok(kFusedH, dims) || Kp % 32 || n == 0) return n == 0 ? hipSuccess : hipErrorInvalidValue;
-  const size_t shm = static_cast<size_t>(64) * (Kp + 8) * 2 + static_cast<size_t>(64) * (kFusedH + 8) * 2 +
-                     static_cast<size_t>(4) * 64 * dims * sizeof(float);
-  const uint32_t g = static_cast<uint32_t>((n + 63) / 64);
+  constexpr uint32_t kRows = 16 * kTransformTiles;
+  const size_t shm = static_cast<size_t>(kRows) * (Kp + 8) * 2 + static_cast<size_t>(kRows) * (kFusedH + 8) * 2 +
+                     static_cast<size_t>(4) * kRows * dims * sizeof(float);
+  const uint32_t g = static_cast<uint32_t>((n + kRows - 1) / kRows);
   switch (dims) {
 #define PM_TR(D) \
-  case D: mlp_transform_kernel<D><<<g, 256, shm, s>>>(x, n, d, Kp, W1b, b1, W2b, b2, W3, b3, Y); break;
+  case D: mlp_transform_kernel<D, kTransformTiles><<<g, 256, shm, s>>>(x, n, d, Kp, W1b, b1, W2b, b2, W3, b3, Y); break;
     PM_TR(1) PM_TR(2) PM_TR(3) PM_TR(4) PM_TR(5) PM_TR(6) PM_TR(7) PM_TR(8)
 #undef PM_TR
     default: return hipErrorInvalidValue;
