@@ -33,10 +33,10 @@ HBM_PEAK_GBS = 8000.0
 # profiles/pmc_traffic_70000x784.md.  Keys: (workload, kernel).
 MEASURED_TRAFFIC = {
     ("70k", "grad"): 31_290_000,               # FETCH 14,387.8 KiB x 2.0 + WRITE 1,777.4 KiB
-    # grad_sell_kernel FETCH 14,981.3 KiB x 2.0 + WRITE 2,216.6 KiB, plus
+    # grad_sell_kernel FETCH 14,983.1 KiB x 2.0 + WRITE 2,206.0 KiB, plus
     # grad_combine_elem_kernel FETCH 1,928.6 KiB x 2.0 + WRITE 1,640.6 KiB
-    ("70k", "grad_sell"): 38_580_000,
-    ("70k", "knn_mfma_kernel"): 1_412_950_000,  # FETCH 456,594.0 KiB x 2.0 + WRITE 466,650.0 KiB
+    ("70k", "grad_sell"): 38_570_000,
+    ("70k", "knn_mfma_kernel"): 1_418_460_000,  # FETCH 459,788.2 KiB x 2.0 + WRITE 465,641.2 KiB
 }
 FP32_PEAK_TFLOPS = 157.3
 BF16_PEAK_TFLOPS = 2500.0
