@@ -411,7 +411,7 @@ __global__ void mlp_pack_kernel(const float *__restrict__ p, uint32_t d, uint32_
     const uint32_t o = static_cast<uint32_t>(f / H), i = static_cast<uint32_t>(f % H);
     const uint16_t b = f2bf(p[static_cast<size_t>(H) * d + H + f]);
     w2b[f] = b;
-    w2tb[static_cast<size_t>(i) * H + o] = b;
+    if (w2tb) w2tb[static_cast<size_t>(i) * H + o] = b;
   }
 }
 
@@ -637,15 +637,25 @@ hipError_t launch_mlp_sum_parts(const float *part, uint32_t S, uint32_t rows, ui
   return hipGetLastError();
 }
 
-bool mlp_transform_fused_ok(uint32_t H, uint32_t dims) { return H == kFusedH && dims >= 1 && dims <= 8; }
+// LDS of one fused transform block: the bf16 input tile, the bf16 first
+// activation and the four waves' partial outputs.
+static size_t mlp_transform_lds_bytes(uint32_t Kp, uint32_t dims) {
+  constexpr uint32_t kRows = 16 * kTransformTiles;
+  return static_cast<size_t>(kRows) * (Kp + 8) * 2 + static_cast<size_t>(kRows) * (kFusedH + 8) * 2 +
+         static_cast<size_t>(4) * kRows * dims * sizeof(float);
+}
+
+bool mlp_transform_fused_ok(uint32_t H, uint32_t dims, uint32_t Kp) {
+  return H == kFusedH && dims >= 1 && dims <= 8 && Kp % 32 == 0 && mlp_transform_lds_bytes(Kp, dims) <= kLdsBytes;
+}
 
 hipError_t launch_mlp_transform(const float *x, uint64_t n, uint32_t d, uint32_t Kp, const uint16_t *W1b,
                                 const float *b1, const uint16_t *W2b, const float *b2, const float *W3,
                                 const float *b3, uint32_t dims, float *Y, hipStream_t s) {
-  if (!mlp_transform_fused_ok(kFusedH, dims) || Kp % 32 || n == 0) return n == 0 ? hipSuccess : hipErrorInvalidValue;
+  if (n == 0) return hipSuccess;
+  if (!mlp_transform_fused_ok(kFusedH, dims, Kp)) return hipErrorInvalidValue;
   constexpr uint32_t kRows = 16 * kTransformTiles;
-  const size_t shm = static_cast<size_t>(kRows) * (Kp + 8) * 2 + static_cast<size_t>(kRows) * (kFusedH + 8) * 2 +
-                     static_cast<size_t>(4) * kRows * dims * sizeof(float);
+  const size_t shm = mlp_transform_lds_bytes(Kp, dims);
   const uint32_t g = static_cast<uint32_t>((n + kRows - 1) / kRows);
   switch (dims) {
 #define PM_TR(D) \
@@ -676,5 +686,6 @@ hipError_t launch_mlp_pack(const float *p, uint32_t d, uint32_t Kp, uint32_t H, 
   mlp_pack_kernel<<<static_cast<uint32_t>((total + 255) / 256), 256, 0, s>>>(p, d, Kp, H, w1b, w2b, w2tb);
   return hipGetLastError();
 }
+
 
 }  // namespace pm

@@ -298,7 +298,7 @@ static int mlp_transform_impl(const float *x_host, const float *x_dev, uint64_t 
       xd = dxb.get();
     }
     MlpWeights W(sh, params, s);
-    if (mlp_transform_fused_ok(sh.H, sh.dims)) {
+    if (mlp_transform_fused_ok(sh.H, sh.dims, sh.Kp)) {
       DevBuf<float> Y(n * dims);
       PM_HIP(launch_mlp_transform(xd, n, sh.d, sh.Kp, W.w1b.get(), W.p.get() + sh.ob1, W.w2b.get(), W.p.get() + sh.ob2,
                                   W.p.get() + sh.oW3, W.p.get() + sh.ob3, sh.dims, Y.get(), s));

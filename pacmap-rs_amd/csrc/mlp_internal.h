@@ -38,8 +38,11 @@ hipError_t launch_mlp_out_bwd(const float *dY, uint32_t dims, const float *W3, c
 hipError_t launch_mlp_rowsum(const uint16_t *dzT, uint32_t rows, uint32_t M, float *out, hipStream_t s);
 hipError_t launch_mlp_sum_parts(const float *part, uint32_t S, uint32_t rows, uint32_t cols, uint32_t ldp,
                                 uint64_t slab, float *out, uint32_t ldo, hipStream_t s);
-// Fused transform Y = encoder(x) for hidden width 256 (mlp_transform_fused_ok)
-bool mlp_transform_fused_ok(uint32_t H, uint32_t dims);
+// LDS per workgroup on gfx950
+constexpr size_t kLdsBytes = 160 * 1024;
+// Fused transform Y = encoder(x) for hidden width 256 when its block's LDS
+// (input tile of Kp bf16 columns, first activation, partial outputs) fits
+bool mlp_transform_fused_ok(uint32_t H, uint32_t dims, uint32_t Kp);
 hipError_t launch_mlp_transform(const float *x, uint64_t n, uint32_t d, uint32_t Kp, const uint16_t *W1b,
                                 const float *b1, const uint16_t *W2b, const float *b2, const float *W3,
                                 const float *b3, uint32_t dims, float *Y, hipStream_t s);
@@ -48,6 +51,7 @@ hipError_t launch_mlp_tree_sum(const float *part, uint32_t S, uint64_t slab, uin
                                hipStream_t s);
 hipError_t launch_mlp_adam(float *p, const float *g, float *m, float *v, uint64_t count, float lr_t, float b1,
                            float b2, float eps, hipStream_t s);
+// bf16 copies of W1 (H x Kp, zero padded), W2 and (w2tb non-null) W2^T
 hipError_t launch_mlp_pack(const float *p, uint32_t d, uint32_t Kp, uint32_t H, uint16_t *w1b, uint16_t *w2b,
                            uint16_t *w2tb, hipStream_t s);
 

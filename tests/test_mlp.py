@@ -116,6 +116,33 @@ def test_mlp_transform_matches_oracle():
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("n,d,dims", [
+    (1, 100, 2), (33, 100, 1), (1000, 100, 2), (1000, 37, 3), (33, 101, 8),
+    (1000, 2176, 8),   # widest input whose block still fits in LDS with 8 outputs
+    (300, 3000, 2),    # block LDS above 160 KiB: the unfused per-layer kernels
+])
+def test_mlp_transform_hidden256_matches_oracle(n, d, dims):
+    """The default hidden width (256) runs the fused transform kernel while its
+    LDS fits, the per-layer kernels otherwise; both against the bf16-emulating
+    oracle (2e-3 of the largest output), and host and device inputs agree
+    bit for bit."""
+    import pacmap_mi355x as pm
+    rng = np.random.default_rng(n + d + dims)
+    x = rng.normal(size=(n, d)).astype(np.float32)
+    p = pm.mlp.init(d, 256, dims, seed=d)
+    p[-dims:] = rng.normal(size=dims).astype(np.float32)  # a nonzero output bias
+    print(f"transform n={n} d={d} dims={dims}", flush=True)
+    y = pm.mlp.transform(x, p, hidden=256, dims=dims)
+    xd = pm.DeviceArray(x)
+    yd = pm.mlp.transform(xd, p, hidden=256, dims=dims)
+    ref = MO.transform(p, x, d, 256, dims, emulate=True)
+    err = np.abs(y - ref).max() / np.abs(ref).max()
+    print(f"  max err {err:.2e}", flush=True)
+    assert err < 2e-3
+    assert np.array_equal(y, yd)
+
+
+@pytest.mark.gpu
 def test_mlp_fit_deterministic_and_loss_falls():
     import pacmap_mi355x as pm
     rng = np.random.default_rng(4)
