@@ -215,6 +215,20 @@ int pacmap_mlp_transform(const float *x, uint64_t n, uint64_t d, uint64_t hidden
                          const float *params, float *y_out, pacmap_status *st);
 int pacmap_mlp_transform_dev(const float *x_dev, uint64_t n, uint64_t d, uint64_t hidden,
                              uint64_t dims, const float *params, float *y_out, pacmap_status *st);
+/* Allocation-free pacmap_mlp_transform_dev.  No counterpart in the reference:
+ * it is for callers that cannot allocate device memory (serving loops that
+ * embed new points with a trained encoder, harnesses that forbid allocation).
+ * x_dev (n x d), params_dev (pacmap_mlp_param_count floats) and y_dev
+ * (n x dims) are all device pointers; the call makes no hipMalloc/hipFree.
+ * The bf16 weight copies go to a static per-device workspace of 1,179,648
+ * bytes (held by each GPU the library runs on until the process ends), so
+ * hidden must be 256, d at most 2048 and dims 1..8 (else
+ * PACMAP_ERR_UNSUPPORTED).  Runs on the calling thread's default stream and
+ * returns when done.  Calls that overlap on one GPU (other host threads)
+ * share the workspace and race: serialise them. */
+int pacmap_mlp_transform_noalloc(const float *x_dev, uint64_t n, uint64_t d, uint64_t hidden,
+                                 uint64_t dims, const float *params_dev, float *y_dev,
+                                 pacmap_status *st);
 /* Seam: mean loss and its gradient over one batch (all given pairs, at most
  * 65536) with weights (w_nn, w_mn, w_fp); grad_out has the parameter layout. */
 int pacmap_mlp_grad(const float *x, uint64_t n, uint64_t d, uint64_t hidden, uint64_t dims,

@@ -9,6 +9,8 @@
 
 #include <hip/hip_bf16.h>
 
+#include <mutex>
+
 namespace pm {
 
 typedef __bf16 bf16x8m __attribute__((ext_vector_type(8)));
@@ -687,5 +689,37 @@ hipError_t launch_mlp_pack(const float *p, uint32_t d, uint32_t Kp, uint32_t H, 
   return hipGetLastError();
 }
 
+// ---------------------------------------------------------------------------
+// Workspace of the allocation-free transform: the bf16 copies of W1 (H x Kp)
+// and W2 (H x H) for H = 256 and Kp <= kNoAllocMaxKp, in a file-scope device
+// array.  Each device holds its own copy of the array, at its own address,
+// for the life of the process; the host looks that address up once per device
+// and passes it to the kernels (the array's name is not a device address in
+// host code).
+// ---------------------------------------------------------------------------
+constexpr size_t kNoAllocHalfs = static_cast<size_t>(kFusedH) * kNoAllocMaxKp + static_cast<size_t>(kFusedH) * kFusedH;
+static_assert(kNoAllocHalfs * sizeof(uint16_t) == kNoAllocScratchBytes, "scratch size");
+static __device__ __align__(256) uint16_t scratch[kNoAllocHalfs];
+
+hipError_t mlp_noalloc_scratch(uint16_t **w1b, uint16_t **w2b) {
+  constexpr int kMaxDevices = 64;
+  static std::mutex mu;
+  static void *addr[kMaxDevices] = {};
+  int dev = -1;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lock(mu);
+  if (!addr[dev]) {
+    void *p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    if (!p) return hipErrorInvalidSymbol;
+    addr[dev] = p;
+  }
+  *w1b = static_cast<uint16_t *>(addr[dev]);
+  *w2b = *w1b + static_cast<size_t>(kFusedH) * kNoAllocMaxKp;
+  return hipSuccess;
+}
 
 }  // namespace pm

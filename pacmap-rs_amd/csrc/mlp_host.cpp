@@ -335,4 +335,31 @@ int pacmap_mlp_transform_dev(const float *x_dev, uint64_t n, uint64_t d, uint64_
   return mlp_transform_impl(nullptr, x_dev, n, d, hidden, dims, params, y_out, st);
 }
 
+// The transform with every buffer in device memory and no allocation: the
+// bf16 weight copies go to the per-device static workspace, the fused kernel
+// writes y_dev directly.  Work runs on the per-thread default stream (no
+// stream is created) and the call returns when it is done.
+int pacmap_mlp_transform_noalloc(const float *x_dev, uint64_t n, uint64_t d, uint64_t hidden, uint64_t dims,
+                                 const float *params_dev, float *y_dev, pacmap_status *st) {
+  return guarded(st, [&] {
+    if (hidden != 256) fail(PACMAP_ERR_UNSUPPORTED, "the allocation-free transform needs hidden width 256");
+    if (d == 0 || d > kNoAllocMaxKp)
+      fail(PACMAP_ERR_UNSUPPORTED, "the allocation-free transform takes input widths 1.." +
+                                       std::to_string(kNoAllocMaxKp) + " (its workspace holds W1 for at most that)");
+    if (dims == 0 || dims > 8) fail(PACMAP_ERR_UNSUPPORTED, "embedding dimensions must be in 1..8");
+    const MlpShape sh(d, hidden, dims);
+    if (!mlp_transform_fused_ok(sh.H, sh.dims, sh.Kp))
+      fail(PACMAP_ERR_UNSUPPORTED, "input width too large for the fused transform kernel");
+    if (n == 0) return;
+    if (!x_dev || !params_dev || !y_dev) fail(PACMAP_ERR_INVALID_ARGUMENT, "NULL argument");
+    uint16_t *w1b = nullptr, *w2b = nullptr;
+    PM_HIP(mlp_noalloc_scratch(&w1b, &w2b));
+    const hipStream_t s = hipStreamPerThread;
+    PM_HIP(launch_mlp_pack(params_dev, sh.d, sh.Kp, sh.H, w1b, w2b, nullptr, s));
+    PM_HIP(launch_mlp_transform(x_dev, n, sh.d, sh.Kp, w1b, params_dev + sh.ob1, w2b, params_dev + sh.ob2,
+                                params_dev + sh.oW3, params_dev + sh.ob3, sh.dims, y_dev, s));
+    PM_HIP(hipStreamSynchronize(s));
+  });
+}
+
 }  // extern "C"

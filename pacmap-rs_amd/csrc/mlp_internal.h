@@ -54,5 +54,11 @@ hipError_t launch_mlp_adam(float *p, const float *g, float *m, float *v, uint64_
 // bf16 copies of W1 (H x Kp, zero padded), W2 and (w2tb non-null) W2^T
 hipError_t launch_mlp_pack(const float *p, uint32_t d, uint32_t Kp, uint32_t H, uint16_t *w1b, uint16_t *w2b,
                            uint16_t *w2tb, hipStream_t s);
+// Allocation-free transform: the bf16 W1 (256 x Kp, Kp <= kNoAllocMaxKp) and
+// W2 (256 x 256) live in a static device array of kNoAllocScratchBytes per
+// device; this returns the current device's addresses of both parts.
+constexpr uint32_t kNoAllocMaxKp = 2048;
+constexpr size_t kNoAllocScratchBytes = (static_cast<size_t>(256) * kNoAllocMaxKp + 256 * 256) * 2;
+hipError_t mlp_noalloc_scratch(uint16_t **w1b, uint16_t **w2b);
 
 }  // namespace pm

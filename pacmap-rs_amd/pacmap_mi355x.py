@@ -140,6 +140,7 @@ def lib():
                                      C.POINTER(MlpConfig), _P, _P, C.POINTER(Status)]
         L.pacmap_mlp_transform.argtypes = [_P, _u64, _u64, _u64, _u64, _P, _P, C.POINTER(Status)]
         L.pacmap_mlp_transform_dev.argtypes = L.pacmap_mlp_transform.argtypes
+        L.pacmap_mlp_transform_noalloc.argtypes = L.pacmap_mlp_transform.argtypes
         L.pacmap_mlp_grad.argtypes = [_P, _u64, _u64, _u64, _u64, _P, _P, _u64, _P, _u64, _P, _u64,
                                       _P, _P, _P, C.POINTER(Status)]
         L.pacmap_dev_alloc.argtypes = [_u64, C.POINTER(C.c_void_p), C.POINTER(Status)]
@@ -335,6 +336,13 @@ class DeviceArray:
         st = Status()
         _check(lib().pacmap_dev_alloc(x.nbytes, C.byref(self.ptr), C.byref(st)), st)
         _check(lib().pacmap_dev_upload(self.ptr, x.ctypes.data, x.nbytes, C.byref(st)), st)
+
+    def numpy(self):
+        """Copy of the device matrix on the host."""
+        out = np.empty(self.shape, np.float32)
+        st = Status()
+        _check(lib().pacmap_dev_download(out.ctypes.data, self.ptr, out.nbytes, C.byref(st)), st)
+        return out
 
     def free(self):
         if self.ptr:
@@ -586,6 +594,17 @@ class mlp:  # noqa: N801 -- namespace of the parametric encoder entry points
         _check(fn(x.ptr if dev else x.ctypes.data, n, d, hidden, dims, p.ctypes.data, y.ctypes.data,
                   C.byref(st)), st)
         return y
+
+    @staticmethod
+    def transform_noalloc(x_dev, params_dev, y_dev, hidden=256, dims=2):
+        """Allocation-free transform: x_dev (n x d), params_dev and y_dev
+        (n x dims) are DeviceArrays; y_dev receives encoder(x)."""
+        n, d = x_dev.shape
+        if tuple(y_dev.shape) != (n, dims):
+            raise ValueError("y_dev must be n x dims")
+        st = Status()
+        _check(lib().pacmap_mlp_transform_noalloc(x_dev.ptr, n, d, hidden, dims, params_dev.ptr, y_dev.ptr,
+                                                  C.byref(st)), st)
 
     @staticmethod
     def grad(x, params, pair_neighbors, pair_mn, pair_fp, weights, hidden=256, dims=2):
