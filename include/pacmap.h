@@ -156,6 +156,12 @@ int pacmap_sigma_scale(const float *knn_dist, const uint32_t *knn_idx, uint64_t 
                        uint64_t kprime, uint64_t n_neighbors, float *sigma_out,
                        float *scaled_out, uint32_t *pairs_out, pacmap_status *st);
 
+/* Seam: scale_dist (distance.rs:27-37) with sigma given: scaled[i][c] =
+ * knn_dist[i][c]^2 / (sig[i] * sig[nbrs[i][c]]).  knn_dist, nbrs, scaled_out:
+ * n x k; sig: n. */
+int pacmap_scale_dist(const float *knn_dist, const float *sig, const uint32_t *nbrs, uint64_t n,
+                      uint64_t k, float *scaled_out, pacmap_status *st);
+
 /* Seam: mid-near and far pair sampling of generate_pairs with a seed
  * (sampling.rs:77-142, 296-313; neighbors.rs:92-96) on the GPU.  x: n x d
  * (the preprocessed data), pair_nn: (n*n_neighbors) x 2 (i-major NN pairs,
@@ -286,6 +292,11 @@ typedef struct {
   uint64_t knn_reranked;       /* of those, candidates given an exact distance */
 } pacmap_timings;
 void pacmap_last_timings(pacmap_timings *out);
+/* Seam: the rows (ascending) that the last kNN on this thread (pacmap_knn or
+ * the kNN inside fit_transform / generate_pairs) recomputed by the exact
+ * rescan because the certificate did not cover them.  Copies at most cap
+ * rows; returns their count. */
+uint64_t pacmap_last_rescanned_rows(uint32_t *rows_out, uint64_t cap);
 
 /* Library and device information; returns 0 when a device is usable. */
 int pacmap_device_info(char *name_out, uint64_t name_len, int32_t *cu_count_out);

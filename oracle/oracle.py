@@ -70,6 +70,14 @@ def lib():
         L.orc_sample_neighbors_pair.argtypes = [_f32p, _u32p, _u64, _u64, _u64, _u32p]
         L.orc_value_init.argtypes = [_f32p, _u64, _u64, _f32p]
         L.orc_normalize.argtypes = [_f32p, _u64, _u64, _f32p]
+        L.orc_knn_rows.argtypes = [_f32p, _u64, _u64, _u64, _u32p, _u64, _u32p, _f32p, C.c_int]
+        L.orc_sample_mn_rows.argtypes = [_f32p, _u64, _u64, _u64, _u64, _u32p, _u64, _u32p]
+        L.orc_sample_fp_rows.argtypes = [_u64, _u32p, _u64, _u64, _u64, _u32p, _u64, _u32p]
+        L.orc_rng_seed_from_bytes.argtypes = [C.c_void_p, C.c_char_p]
+        L.orc_zig_tables.argtypes = [_f64p, _f64p]
+        L.orc_standard_normal.argtypes = [C.c_void_p]
+        L.orc_standard_normal.restype = C.c_double
+        L.orc_random_init.argtypes = [_u64, _u64, _u64, _f32p]
         L.orc_optimize.argtypes = [_f32p, _u64, _u64, _u32p, _u64, _u32p, _u64, _u32p, _u64,
                                    C.c_float, _u64p, C.c_void_p, _u64, C.c_void_p, C.c_void_p,
                                    C.c_int]
@@ -185,6 +193,19 @@ def knn_rowwise(x, k, nthreads=1):
     return idx, dist
 
 
+def knn_rows(x, k, rows, nthreads=1):
+    """knn_rowwise for the listed query rows only: (len(rows) x min(k, n-1))."""
+    x = np.ascontiguousarray(x, np.float32)
+    n, d = x.shape
+    rows = np.ascontiguousarray(rows, np.uint32)
+    kk = min(k, n - 1)
+    idx = np.zeros((rows.size, kk), np.uint32)
+    dist = np.zeros((rows.size, kk), np.float32)
+    if rows.size and lib().orc_knn_rows(x, n, d, k, rows, rows.size, idx, dist, nthreads):
+        raise ValueError("row index out of range")
+    return idx, dist
+
+
 def decide_num_pairs(n, override=None, mn_ratio=0.5, fp_ratio=2.0):
     out = np.zeros(3, np.uint64)
     rc = lib().orc_decide_num_pairs(n, override is not None, override or 0, mn_ratio, fp_ratio,
@@ -202,6 +223,16 @@ class Rng:
     @property
     def ptr(self):
         return C.addressof(self._buf)
+
+    @classmethod
+    def from_seed(cls, seed_bytes):
+        """SeedableRng::from_seed with 32 bytes."""
+        r = cls(0)
+        lib().orc_rng_seed_from_bytes(r.ptr, bytes(seed_bytes))
+        return r
+
+    def standard_normal(self):
+        return lib().orc_standard_normal(self.ptr)
 
     def next_u64(self):
         return lib().orc_rng_next_u64(self.ptr)
@@ -230,6 +261,39 @@ def sample_mn_pair_deterministic(x, n_mn, seed):
     n, d = x.shape
     out = np.zeros((n * n_mn, 2), np.uint32)
     lib().orc_sample_mn_pair_det(x, n, d, n_mn, seed, out)
+    return out
+
+
+def sample_mn_rows(x, n_mn, seed, rows):
+    """sample_mn_pair_deterministic's rows for the listed points only."""
+    x = np.ascontiguousarray(x, np.float32)
+    n, d = x.shape
+    rows = np.ascontiguousarray(rows, np.uint32)
+    out = np.zeros((rows.size * n_mn, 2), np.uint32)
+    lib().orc_sample_mn_rows(x, n, d, n_mn, seed, rows, rows.size, out)
+    return out
+
+
+def sample_fp_rows(n, pair_neighbors, n_neighbors, n_fp, seed, rows):
+    """sample_fp_pair_deterministic's rows for the listed points only."""
+    pn = np.ascontiguousarray(pair_neighbors, np.uint32)
+    rows = np.ascontiguousarray(rows, np.uint32)
+    out = np.zeros((rows.size * n_fp, 2), np.uint32)
+    lib().orc_sample_fp_rows(n, pn, n_neighbors, n_fp, seed, rows, rows.size, out)
+    return out
+
+
+def zig_tables():
+    x = np.zeros(257)
+    f = np.zeros(257)
+    lib().orc_zig_tables(x, f)
+    return x, f
+
+
+def random_init(n, dims, seed):
+    """lib.rs:769-778 Random(Some(seed)): N(0, 1) f32 draws, row-major."""
+    out = np.zeros((n, dims), np.float32)
+    lib().orc_random_init(n, dims, seed, out)
     return out
 
 

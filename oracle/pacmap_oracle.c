@@ -488,6 +488,7 @@ typedef struct {
   uint64_t n, d, k;
   uint32_t *idx;
   float *dist;
+  const uint32_t *rows; /* query rows (NULL: every row); output row t is query rows[t] */
 } orc_row_ctx;
 
 static void orc_heap_sift_down(uint64_t *h, uint64_t sz, uint64_t p) {
@@ -513,7 +514,8 @@ static void orc_row_task(void *vc, uint64_t b, uint64_t e) {
   uint64_t k = c->k;
   uint64_t *heap = (uint64_t *)malloc(sizeof(uint64_t) * (k ? k : 1));
   float *dj = (float *)malloc(sizeof(float) * c->n);
-  for (uint64_t i = b; i < e; i++) {
+  for (uint64_t t = b; t < e; t++) {
+    const uint64_t i = c->rows ? c->rows[t] : t;
     uint64_t sz = 0;
     for (uint64_t j = 0; j < c->n; j++) {
       if (j == i) continue;
@@ -538,8 +540,8 @@ static void orc_row_task(void *vc, uint64_t b, uint64_t e) {
     qsort(heap, sz, sizeof(uint64_t), orc_u64_cmp);
     for (uint64_t c2 = 0; c2 < k; c2++) {
       uint32_t j = (uint32_t)(heap[c2] & 0xffffffffu);
-      c->idx[i * k + c2] = j;
-      c->dist[i * k + c2] = dj[j];
+      c->idx[t * k + c2] = j;
+      c->dist[t * k + c2] = dj[j];
     }
   }
   free(heap);
@@ -550,8 +552,19 @@ int orc_knn_rowwise(const float *x, uint64_t n, uint64_t d, uint64_t k, uint32_t
                     int nthreads) {
   if (n == 0) return 0;
   if (k > n - 1) k = n - 1;
-  orc_row_ctx c = {x, n, d, k, idx, dist};
+  orc_row_ctx c = {x, n, d, k, idx, dist, NULL};
   orc_parallel_for(n, nthreads, orc_row_task, &c);
+  return 0;
+}
+
+int orc_knn_rows(const float *x, uint64_t n, uint64_t d, uint64_t k, const uint32_t *rows,
+                 uint64_t nrows, uint32_t *idx, float *dist, int nthreads) {
+  if (n == 0 || nrows == 0) return 0;
+  if (k > n - 1) k = n - 1;
+  for (uint64_t t = 0; t < nrows; t++)
+    if (rows[t] >= n) return -1;
+  orc_row_ctx c = {x, n, d, k, idx, dist, rows};
+  orc_parallel_for(nrows, nthreads, orc_row_task, &c);
   return 0;
 }
 
@@ -619,6 +632,8 @@ static void orc_rng_from_seed(orc_rng *r, const uint8_t seed[32]) {
   }
 }
 
+void orc_rng_seed_from_bytes(orc_rng *r, const uint8_t seed[32]) { orc_rng_from_seed(r, seed); }
+
 void orc_rng_seed_from_u64(orc_rng *r, uint64_t state) {
   uint8_t seed[32];
   for (int c = 0; c < 8; c++) {
@@ -662,6 +677,78 @@ uint32_t orc_rng_gen_range_u32(orc_rng *r, uint32_t low, uint32_t high) {
   }
 }
 
+/* rand 0.8.5 Standard for f64 (53 high bits scaled by 2^-53) and Open01
+ * (52 high bits as a [1, 2) mantissa, minus 1 - 2^-53). */
+static double orc_rng_f64(orc_rng *r) {
+  return (double)(orc_rng_next_u64(r) >> 11) * (1.0 / 9007199254740992.0);
+}
+static double orc_f64_with_exponent(uint64_t mant52, int e) {
+  uint64_t b = mant52 | ((uint64_t)(1023 + e) << 52);
+  double v;
+  memcpy(&v, &b, sizeof(v));
+  return v;
+}
+static double orc_rng_open01(orc_rng *r) {
+  return orc_f64_with_exponent(orc_rng_next_u64(r) >> 12, 0) - (1.0 - 2.220446049250313e-16 / 2.0);
+}
+
+/* rand_distr 0.4.3 ziggurat tables for N(0, 1): the ZIGNOR construction
+ * (Doornik 2005) with 256 layers, R = 3.6541528853610088, V = 0.00492867323399:
+ * x[0] = V / f(R), x[1] = R, x[i] = f^-1(V / x[i-1] + f(x[i-1])), x[256] = 0,
+ * F[i] = f(x[i]) with f(x) = exp(-x^2 / 2). */
+#define ORC_ZIG_R 3.6541528853610088
+#define ORC_ZIG_V 0.00492867323399
+static double orc_zig_x[257], orc_zig_f[257];
+static pthread_once_t orc_zig_once = PTHREAD_ONCE_INIT;
+static double orc_norm_pdf(double x) { return exp(-x * x / 2.0); }
+static void orc_zig_init(void) {
+  orc_zig_x[0] = ORC_ZIG_V / orc_norm_pdf(ORC_ZIG_R);
+  orc_zig_x[1] = ORC_ZIG_R;
+  for (int i = 2; i < 256; i++) {
+    double last = orc_zig_x[i - 1];
+    orc_zig_x[i] = sqrt(-2.0 * log(ORC_ZIG_V / last + orc_norm_pdf(last)));
+  }
+  orc_zig_x[256] = 0.0;
+  for (int i = 0; i < 257; i++) orc_zig_f[i] = orc_norm_pdf(orc_zig_x[i]);
+}
+
+void orc_zig_tables(double x[257], double f[257]) {
+  pthread_once(&orc_zig_once, orc_zig_init);
+  memcpy(x, orc_zig_x, sizeof(orc_zig_x));
+  memcpy(f, orc_zig_f, sizeof(orc_zig_f));
+}
+
+/* rand_distr 0.4.3 StandardNormal for f64: utils.rs ziggurat(symmetric) with
+ * the normal pdf and the tail sampler of normal.rs (zero_case). */
+double orc_standard_normal(orc_rng *r) {
+  pthread_once(&orc_zig_once, orc_zig_init);
+  for (;;) {
+    uint64_t bits = orc_rng_next_u64(r);
+    unsigned i = (unsigned)(bits & 0xff);
+    double u = orc_f64_with_exponent(bits >> 12, 1) - 3.0; /* [2, 4) - 3 = [-1, 1) */
+    double x = u * orc_zig_x[i];
+    if (fabs(x) < orc_zig_x[i + 1]) return x;
+    if (i == 0) { /* the tail beyond R */
+      double xt = 1.0, yt = 0.0;
+      while (-2.0 * yt < xt * xt) {
+        double x_ = orc_rng_open01(r), y_ = orc_rng_open01(r);
+        xt = log(x_) / ORC_ZIG_R;
+        yt = log(y_);
+      }
+      return u < 0.0 ? xt - ORC_ZIG_R : ORC_ZIG_R - xt;
+    }
+    if (orc_zig_f[i + 1] + (orc_zig_f[i] - orc_zig_f[i + 1]) * orc_rng_f64(r) < orc_norm_pdf(x)) return x;
+  }
+}
+
+/* lib.rs:769-778: Normal<f32>(0, 1) draws in row-major order from
+ * SmallRng::seed_from_u64(seed); each is mean + std_dev * (z as f32). */
+void orc_random_init(uint64_t n, uint64_t dims, uint64_t seed, float *y) {
+  orc_rng r;
+  orc_rng_seed_from_u64(&r, seed);
+  for (uint64_t e = 0; e < n * dims; e++) y[e] = 0.0f + 1.0f * (float)orc_standard_normal(&r);
+}
+
 /* ------------------------------------------------------------------------ */
 /* sampling.rs                                                               */
 /* ------------------------------------------------------------------------ */
@@ -688,21 +775,37 @@ uint64_t orc_sample_fp(uint64_t n_samples, uint32_t maximum, const uint32_t *rej
   return len;
 }
 
+/* The n_fp far-pair rows of point i (sampling.rs:90-101): rows (out) start
+ * zeroed (sampling.rs:84); buf holds n_fp words. */
+static void orc_fp_point(uint64_t n, const uint32_t *pair_neighbors, uint64_t n_neighbors,
+                         uint64_t n_fp, uint64_t seed, uint64_t i, uint32_t *buf, uint32_t *out) {
+  orc_rng rng;
+  orc_rng_seed_from_u64(&rng, seed + i); /* sampling.rs:97 */
+  /* reject = pair_neighbors[i*k .. (i+1)*k, 1] (sampling.rs:94-95) */
+  uint64_t got = orc_sample_fp(n_fp, (uint32_t)n, pair_neighbors + 2 * i * n_neighbors + 1,
+                               n_neighbors, 2, (uint32_t)i, &rng, buf);
+  for (uint64_t q = 0; q < got; q++) { /* sampling.rs:124-132 */
+    out[2 * q] = (uint32_t)i;
+    out[2 * q + 1] = buf[q];
+  }
+}
+
 void orc_sample_fp_pair_det(uint64_t n, const uint32_t *pair_neighbors, uint64_t n_neighbors,
                             uint64_t n_fp, uint64_t seed, uint32_t *pair_fp) {
   memset(pair_fp, 0, sizeof(uint32_t) * 2 * n * n_fp); /* sampling.rs:84 zeros */
   uint32_t *buf = (uint32_t *)malloc(sizeof(uint32_t) * (n_fp ? n_fp : 1));
-  for (uint64_t i = 0; i < n; i++) {
-    orc_rng rng;
-    orc_rng_seed_from_u64(&rng, seed + i); /* sampling.rs:97 */
-    /* reject = pair_neighbors[i*k .. (i+1)*k, 1] (sampling.rs:94-95) */
-    uint64_t got = orc_sample_fp(n_fp, (uint32_t)n, pair_neighbors + 2 * i * n_neighbors + 1,
-                                 n_neighbors, 2, (uint32_t)i, &rng, buf);
-    for (uint64_t q = 0; q < got; q++) { /* sampling.rs:124-132 */
-      pair_fp[2 * (i * n_fp + q)] = (uint32_t)i;
-      pair_fp[2 * (i * n_fp + q) + 1] = buf[q];
-    }
-  }
+  for (uint64_t i = 0; i < n; i++)
+    orc_fp_point(n, pair_neighbors, n_neighbors, n_fp, seed, i, buf, pair_fp + 2 * i * n_fp);
+  free(buf);
+}
+
+void orc_sample_fp_rows(uint64_t n, const uint32_t *pair_neighbors, uint64_t n_neighbors,
+                        uint64_t n_fp, uint64_t seed, const uint32_t *rows, uint64_t nrows,
+                        uint32_t *out) {
+  memset(out, 0, sizeof(uint32_t) * 2 * nrows * n_fp);
+  uint32_t *buf = (uint32_t *)malloc(sizeof(uint32_t) * (n_fp ? n_fp : 1));
+  for (uint64_t t = 0; t < nrows; t++)
+    orc_fp_point(n, pair_neighbors, n_neighbors, n_fp, seed, rows[t], buf, out + 2 * t * n_fp);
   free(buf);
 }
 
@@ -711,41 +814,50 @@ typedef struct {
   uint32_t idx;
 } orc_di;
 
+/* The n_mn mid-near rows of point i (sampling.rs:128-139), rows zeroed. */
+static void orc_mn_point(const float *x, uint64_t n, uint64_t d, uint64_t n_mn, uint64_t seed,
+                         uint64_t i, uint32_t *rows) {
+  orc_rng rng;
+  orc_rng_seed_from_u64(&rng, seed + i); /* sampling.rs:133 */
+  for (uint64_t j = 0; j < n_mn; j++) {
+    uint32_t sampled[6];
+    /* reject = rows[0..j, 1] (sampling.rs:135) */
+    uint64_t got = orc_sample_fp(6, (uint32_t)n, rows + 1, j, 2, (uint32_t)i, &rng, sampled);
+    /* sampling.rs:302-309: six (dist, idx) slots initialised to (0.0, 0) */
+    orc_di e[6];
+    for (int q = 0; q < 6; q++) {
+      e[q].d = 0.0f;
+      e[q].idx = 0;
+    }
+    for (uint64_t q = 0; q < got; q++) {
+      e[q].d = orc_euclid(x + i * d, x + (uint64_t)sampled[q] * d, d);
+      e[q].idx = sampled[q];
+    }
+    /* sort_unstable_by(total_cmp) on six entries: insertion order kept for ties */
+    for (int a = 1; a < 6; a++) {
+      orc_di t = e[a];
+      int b = a - 1;
+      while (b >= 0 && orc_total_key(e[b].d) > orc_total_key(t.d)) {
+        e[b + 1] = e[b];
+        b--;
+      }
+      e[b + 1] = t;
+    }
+    rows[2 * j] = (uint32_t)i; /* sampling.rs:310-312 */
+    rows[2 * j + 1] = e[1].idx;
+  }
+}
+
 void orc_sample_mn_pair_det(const float *x, uint64_t n, uint64_t d, uint64_t n_mn, uint64_t seed,
                             uint32_t *pair_mn) {
   memset(pair_mn, 0, sizeof(uint32_t) * 2 * n * n_mn);
-  for (uint64_t i = 0; i < n; i++) {
-    orc_rng rng;
-    orc_rng_seed_from_u64(&rng, seed + i); /* sampling.rs:133 */
-    uint32_t *rows = pair_mn + 2 * i * n_mn;
-    for (uint64_t j = 0; j < n_mn; j++) {
-      uint32_t sampled[6];
-      /* reject = rows[0..j, 1] (sampling.rs:135) */
-      uint64_t got = orc_sample_fp(6, (uint32_t)n, rows + 1, j, 2, (uint32_t)i, &rng, sampled);
-      /* sampling.rs:302-309: six (dist, idx) slots initialised to (0.0, 0) */
-      orc_di e[6];
-      for (int q = 0; q < 6; q++) {
-        e[q].d = 0.0f;
-        e[q].idx = 0;
-      }
-      for (uint64_t q = 0; q < got; q++) {
-        e[q].d = orc_euclid(x + i * d, x + (uint64_t)sampled[q] * d, d);
-        e[q].idx = sampled[q];
-      }
-      /* sort_unstable_by(total_cmp) on six entries: insertion order kept for ties */
-      for (int a = 1; a < 6; a++) {
-        orc_di t = e[a];
-        int b = a - 1;
-        while (b >= 0 && orc_total_key(e[b].d) > orc_total_key(t.d)) {
-          e[b + 1] = e[b];
-          b--;
-        }
-        e[b + 1] = t;
-      }
-      rows[2 * j] = (uint32_t)i; /* sampling.rs:310-312 */
-      rows[2 * j + 1] = e[1].idx;
-    }
-  }
+  for (uint64_t i = 0; i < n; i++) orc_mn_point(x, n, d, n_mn, seed, i, pair_mn + 2 * i * n_mn);
+}
+
+void orc_sample_mn_rows(const float *x, uint64_t n, uint64_t d, uint64_t n_mn, uint64_t seed,
+                        const uint32_t *rows, uint64_t nrows, uint32_t *out) {
+  memset(out, 0, sizeof(uint32_t) * 2 * nrows * n_mn);
+  for (uint64_t t = 0; t < nrows; t++) orc_mn_point(x, n, d, n_mn, seed, rows[t], out + 2 * t * n_mn);
 }
 
 void orc_sample_neighbors_pair(const float *scaled, const uint32_t *nbrs, uint64_t n,

@@ -21,8 +21,13 @@
  *   orc_knn_triple_sort     knn.rs:215-324 KATs + FP64 brute force on tie-free data
  *   orc_knn_rowwise         equal to orc_knn_triple_sort on random inputs
  *   orc_decide_num_pairs    closed form (lib.rs:565-597) at hand-computed n
- *   orc_rng_*, orc_sample_* invariants of sampling.rs:323-538 only:
- *                           rand 0.8.5 stream values are PARITY UNPINNED
+ *   orc_rng_next_u64        the xoshiro256++ reference-implementation vector for
+ *                           the state {1, 2, 3, 4} (pins the generator and from_seed)
+ *   orc_rng_seed_from_u64,  invariants of sampling.rs:323-538 only: the seed
+ *   orc_sample_*            expansion and the stream values are PARITY UNPINNED
+ *   orc_standard_normal,    closed forms of the ziggurat tables (layer areas, the
+ *   orc_random_init         top layer meeting f(0) = 1) and the N(0, 1) law of a
+ *                           large sample: PARITY UNPINNED (no reference vectors)
  *   orc_value_init          closed form (mean 0, std 1e-4) only
  *   orc_optimize            composition of the above
  */
@@ -80,6 +85,10 @@ int orc_knn_triple_sort(const float *x, uint64_t n, uint64_t d, uint64_t k,
 /* Same result, per-row selection, O(n*k) memory. */
 int orc_knn_rowwise(const float *x, uint64_t n, uint64_t d, uint64_t k,
                     uint32_t *idx, float *dist, int nthreads);
+/* The rows of orc_knn_rowwise for the nrows query rows listed in `rows`
+ * (output row t is query rows[t]); -1 on an index >= n. */
+int orc_knn_rows(const float *x, uint64_t n, uint64_t d, uint64_t k, const uint32_t *rows,
+                 uint64_t nrows, uint32_t *idx, float *dist, int nthreads);
 
 /* lib.rs:565-597.  Returns 0, 5 (InvalidNeighborCount), 6 (InvalidFarPointCount). */
 int orc_decide_num_pairs(uint64_t n, int has_override, uint64_t override_k,
@@ -88,9 +97,17 @@ int orc_decide_num_pairs(uint64_t n, int has_override, uint64_t override_k,
 /* rand 0.8.5 SmallRng (Xoshiro256++ on 64-bit) restatement -- PARITY UNPINNED. */
 typedef struct { uint64_t s[4]; } orc_rng;
 void orc_rng_seed_from_u64(orc_rng *r, uint64_t state);
+/* SeedableRng::from_seed: four little-endian u64 words (all zero: SplitMix64(0)) */
+void orc_rng_seed_from_bytes(orc_rng *r, const uint8_t seed[32]);
 uint64_t orc_rng_next_u64(orc_rng *r);
 uint32_t orc_rng_next_u32(orc_rng *r);
 uint32_t orc_rng_gen_range_u32(orc_rng *r, uint32_t low, uint32_t high);
+
+/* rand_distr 0.4.3 ziggurat tables (x, f: 257 each) and StandardNormal for
+ * f64; lib.rs:769-778 Random(seed) initial Y (n x dims, row-major). */
+void orc_zig_tables(double x[257], double f[257]);
+double orc_standard_normal(orc_rng *r);
+void orc_random_init(uint64_t n, uint64_t dims, uint64_t seed, float *y);
 
 /* sampling.rs:37-61 */
 uint64_t orc_sample_fp(uint64_t n_samples, uint32_t maximum, const uint32_t *reject,
@@ -102,6 +119,13 @@ void orc_sample_fp_pair_det(uint64_t n, const uint32_t *pair_neighbors, uint64_t
 /* sampling.rs:118-142 + 296-313 */
 void orc_sample_mn_pair_det(const float *x, uint64_t n, uint64_t d, uint64_t n_mn,
                             uint64_t seed, uint32_t *pair_mn /* (n*n_mn) x 2 */);
+/* The same samplers for chosen points only: out holds nrows x n_mn (or
+ * n_fp) rows x 2, the rows of point rows[t] at block t. */
+void orc_sample_mn_rows(const float *x, uint64_t n, uint64_t d, uint64_t n_mn, uint64_t seed,
+                        const uint32_t *rows, uint64_t nrows, uint32_t *out);
+void orc_sample_fp_rows(uint64_t n, const uint32_t *pair_neighbors, uint64_t n_neighbors,
+                        uint64_t n_fp, uint64_t seed, const uint32_t *rows, uint64_t nrows,
+                        uint32_t *out);
 /* sampling.rs:251-286 (ties by column: stable) */
 void orc_sample_neighbors_pair(const float *scaled, const uint32_t *nbrs, uint64_t n,
                                uint64_t kprime, uint64_t n_neighbors, uint32_t *pairs);

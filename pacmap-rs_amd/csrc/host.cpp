@@ -15,6 +15,9 @@ namespace pm {
 namespace {
 
 thread_local pacmap_timings g_tm;
+// rows of the last kNN on this thread that the certificate sent to the exact
+// rescan (filled when the stream reaches the copy; read after the call returns)
+thread_local std::vector<uint32_t> g_rescanned;
 
 // ---------------------------------------------------------------------------
 // exact kNN on a device matrix (knn.rs:137-194)
@@ -162,6 +165,8 @@ bool knn_mfma_path(const float *x_dev, uint64_t n, uint64_t d, uint32_t k, const
     PM_HIP(launch_knn_rescan(xp, n, L, k, fb_list.get(), nfb, part.get(), S, idx.get(), dist.get(), s));
     tf.stop(s);
     tf.add_to(g_tm.knn_fallback_ms);
+    g_rescanned.resize(nfb);
+    download(g_rescanned.data(), fb_list.get(), nfb, s);
   }
   return true;
 }
@@ -175,6 +180,7 @@ void knn_device(const float *x_dev, uint64_t n, uint64_t d, uint32_t k, DevBuf<u
   if (d >= (1ull << 28)) fail(PACMAP_ERR_UNSUPPORTED, "too many features for the kNN kernel");
   idx.alloc(n * k);
   dist.alloc(n * k);
+  g_rescanned.clear();
   if (k == 0) return;
   const KnnLayout L = knn_layout(static_cast<uint32_t>(d));
   DevBuf<float> xp(n * L.Dp);
@@ -260,9 +266,8 @@ struct DevCsr {
   uint64_t nnz = 0;
 };
 
-// Builds the CSR of three device pair arrays on the GPU.  Rows longer than
-// the per-row sort handles are rare (hubs); the whole CSR is then built on
-// the host from the same pairs, which keeps the layout deterministic.
+// Builds the CSR of three device pair arrays on the GPU, every row sorted by
+// entry value (hub rows too, by a workgroup each).
 void build_csr_dev(uint64_t n, const uint32_t *const dp[3], const uint64_t rows[3], DevCsr &out,
                    hipStream_t s) {
   if (n >= kMaxPoints) fail(PACMAP_ERR_UNSUPPORTED, "too many points for the CSR entry format");
@@ -281,25 +286,10 @@ void build_csr_dev(uint64_t n, const uint32_t *const dp[3], const uint64_t rows[
   if (hf[0]) fail(PACMAP_ERR_INVALID_PAIR_INDEX, "pair index out of range");
   out.nnz = nnz;
   out.ent.alloc(std::max<uint64_t>(1, nnz), kSlotCsrEnt);
+  const uint32_t long_cap = nnz / 2049 + 1;  // rows above 2048 entries
+  DevBuf<uint32_t> long_list(long_cap);
   PM_HIP(launch_csr_fill_sort(dp, rows, static_cast<uint32_t>(n), out.row_ptr.get(), cursor.get(),
-                              out.ent.get(), flags.get() + 1, s));
-  download(hf, flags.get(), 2, s);
-  PM_HIP(hipStreamSynchronize(s));
-  if (hf[1]) {
-    std::vector<uint32_t> hp[3];
-    PairList lists[3];
-    for (int t = 0; t < 3; ++t) {
-      hp[t].resize(2 * rows[t]);
-      download(hp[t].data(), dp[t], 2 * rows[t], s);
-      lists[t] = {hp[t].data(), rows[t]};
-    }
-    PM_HIP(hipStreamSynchronize(s));
-    Csr h;
-    build_csr(n, lists, h);
-    upload(out.row_ptr, h.row_ptr.data(), n + 1, s);
-    upload(out.ent, h.ent.data(), h.ent.size(), s);
-    PM_HIP(hipStreamSynchronize(s));
-  }
+                              out.ent.get(), flags.get() + 1, long_list.get(), long_cap, s));
   tcsr.stop(s);
   tcsr.add_to(g_tm.csr_ms);
   g_tm.csr_entries = nnz;
@@ -352,6 +342,74 @@ bool use_sell(uint64_t n, uint64_t dims, SellPlan &plan) {
   return n >= 16384;
 }
 
+// The gradient kernel for a CSR and its per-fit layout: the target-blocked
+// kernel when Y outgrows an XCD's L2, its LDS-staged (sliced ELL) form when a
+// target range fits in LDS, else the one-pass kernel (see kernels.hip).  The
+// optimiser and the gradient seam both build it, so they run the same kernel.
+struct GradLayout {
+  bool tb = false, sell = false;
+  SellPlan sp;
+  DevBuf<uint32_t> tb_ent, sell_perm, sell_off, sell_ent;
+  DevBuf<uint16_t> tb_off;
+  DevBuf<float> tb_part;
+  uint32_t loss_blocks(uint32_t n) const {
+    return sell ? sell_grad_blocks(sp) : (tb ? grad_tb_blocks(n) : grad_blocks(n));
+  }
+  const uint32_t *entries(const DevCsr &csr) const { return tb && !sell ? tb_ent.get() : csr.ent.get(); }
+  hipError_t launch(const GradArgs &a, uint32_t dims, bool adam, hipStream_t s) const {
+    if (sell)
+      return launch_grad_adam_sell(a, sp, sell_perm.get(), sell_off.get(), sell_ent.get(), tb_part.get(), dims,
+                                   adam, s);
+    if (tb) return launch_grad_adam_tb(a, tb_off.get(), tb_part.get(), dims, adam, s);
+    return launch_grad_adam(a, dims, adam, s);
+  }
+};
+
+void build_grad_layout(const DevCsr &csr, uint64_t n, uint64_t dims, bool want, GradLayout &L, hipStream_t s) {
+  const uint64_t nd = n * dims;
+  SellPlan &sp = L.sp;
+  bool tb = want && use_target_blocks(n, dims) && csr.nnz > 0;
+  bool sell = want && !tb && csr.nnz > 0 && use_sell(n, dims, sp);
+  if (tb || sell) {
+    L.tb_ent.alloc(csr.nnz, kSlotTbEnt);
+    L.tb_off.alloc(static_cast<size_t>(kTargetBlocks) * n, kSlotTbOff);
+    DevBuf<uint32_t> bad(1);
+    PM_HIP(launch_fill_zero(bad.get(), bad.bytes(), s));
+    PM_HIP(launch_tb_partition(csr.row_ptr.get(), csr.ent.get(), static_cast<uint32_t>(n), L.tb_ent.get(),
+                               L.tb_off.get(), bad.get(), s));
+    uint32_t hb = 0;
+    download(&hb, bad.get(), 1, s);
+    PM_HIP(hipStreamSynchronize(s));
+    if (hb) {
+      tb = sell = false;  // a row too long for the u16 range offsets: one-pass kernel
+    } else {
+      L.tb_part.alloc(static_cast<size_t>(kTargetBlocks) * nd, kSlotTbPart);
+    }
+  }
+  if (sell) {
+    const uint32_t n32 = static_cast<uint32_t>(n);
+    const uint64_t ns_ = sell_slices(sp);
+    L.sell_perm.alloc(static_cast<size_t>(sp.ranges) * n, kSlotSellPerm);
+    L.sell_off.alloc(ns_ + 1, kSlotSellOff);
+    DevBuf<uint32_t> ssz(ns_), cur(ns_), scr((ns_ + 1023) / 1024 + 1);
+    PM_HIP(launch_sell_sort(csr.row_ptr.get(), L.tb_off.get(), n32, sp, L.sell_perm.get(), ssz.get(), s));
+    PM_HIP(launch_csr_scan(ssz.get(), static_cast<uint32_t>(ns_), L.sell_off.get(), cur.get(), scr.get(), s));
+    uint32_t total = 0;
+    download(&total, L.sell_off.get() + ns_, 1, s);
+    PM_HIP(hipStreamSynchronize(s));
+    L.sell_ent.alloc(total ? total : 1, kSlotSellEnt);
+    sp.words = total ? total : 1;
+    PM_HIP(launch_sell_fill(csr.row_ptr.get(), L.tb_ent.get(), L.tb_off.get(), n32, sp, L.sell_perm.get(),
+                            L.sell_off.get(), L.sell_ent.get(), s));
+    g_tm.grad_sell_entries = total;
+    g_tm.grad_sell_ranges = sp.ranges;
+  }
+  L.tb = tb;
+  L.sell = sell;
+  g_tm.grad_target_blocks = tb ? kTargetBlocks : 0;
+  g_tm.grad_sell = sell ? 1 : 0;
+}
+
 // The initial Y is y (host), or y_dev (device, n x dims) when that is not
 // null; y ends as the final Y (y.size() >= n dims).
 void optimize_device(const DevCsr &csr, uint64_t n, uint64_t dims, std::vector<float> &y, const float *y_dev,
@@ -392,52 +450,11 @@ void optimize_device(const DevCsr &csr, uint64_t n, uint64_t dims, std::vector<f
   PM_HIP(launch_fill_zero(m.get(), m.bytes(), s));
   PM_HIP(launch_fill_zero(v.get(), v.bytes(), s));
   DevBuf<float> d_snap(ns && snaps_out ? ns * nd : 0, kSlotSnap);
-  // Target-blocked gradient when Y outgrows an XCD's L2, its LDS-staged
-  // form when a target range fits in LDS (see kernels.hip)
-  SellPlan sp;
-  bool tb = use_target_blocks(n, dims) && T > 0 && csr.nnz > 0;
-  bool sell = !tb && T > 0 && csr.nnz > 0 && use_sell(n, dims, sp);
-  DevBuf<uint32_t> tb_ent, sell_perm, sell_off, sell_ent;
-  DevBuf<uint16_t> tb_off;
-  DevBuf<float> tb_part;
-  if (tb || sell) {
-    tb_ent.alloc(csr.nnz, kSlotTbEnt);
-    tb_off.alloc(static_cast<size_t>(kTargetBlocks) * n, kSlotTbOff);
-    DevBuf<uint32_t> bad(1);
-    PM_HIP(launch_fill_zero(bad.get(), bad.bytes(), s));
-    PM_HIP(launch_tb_partition(csr.row_ptr.get(), csr.ent.get(), static_cast<uint32_t>(n), tb_ent.get(),
-                               tb_off.get(), bad.get(), s));
-    uint32_t hb = 0;
-    download(&hb, bad.get(), 1, s);
-    PM_HIP(hipStreamSynchronize(s));
-    if (hb) {
-      tb = sell = false;  // a row too long for the u16 range offsets: plain path
-    } else {
-      tb_part.alloc(static_cast<size_t>(kTargetBlocks) * nd, kSlotTbPart);
-    }
-  }
-  if (sell) {
-    const uint32_t n32 = static_cast<uint32_t>(n);
-    const uint64_t ns_ = sell_slices(sp);
-    sell_perm.alloc(static_cast<size_t>(sp.ranges) * n, kSlotSellPerm);
-    sell_off.alloc(ns_ + 1, kSlotSellOff);
-    DevBuf<uint32_t> ssz(ns_), cur(ns_), scr((ns_ + 1023) / 1024 + 1);
-    PM_HIP(launch_sell_sort(csr.row_ptr.get(), tb_off.get(), n32, sp, sell_perm.get(), ssz.get(), s));
-    PM_HIP(launch_csr_scan(ssz.get(), static_cast<uint32_t>(ns_), sell_off.get(), cur.get(), scr.get(), s));
-    uint32_t total = 0;
-    download(&total, sell_off.get() + ns_, 1, s);
-    PM_HIP(hipStreamSynchronize(s));
-    sell_ent.alloc(total ? total : 1, kSlotSellEnt);
-    sp.words = total ? total : 1;
-    PM_HIP(launch_sell_fill(csr.row_ptr.get(), tb_ent.get(), tb_off.get(), n32, sp, sell_perm.get(),
-                            sell_off.get(), sell_ent.get(), s));
-    g_tm.grad_sell_entries = total;
-    g_tm.grad_sell_ranges = sp.ranges;
-  }
-  g_tm.grad_target_blocks = tb ? kTargetBlocks : 0;
-  g_tm.grad_sell = sell ? 1 : 0;
-  const uint32_t nb = sell ? sell_grad_blocks(sp)
-                           : (tb ? grad_tb_blocks(static_cast<uint32_t>(n)) : grad_blocks(static_cast<uint32_t>(n)));
+  GradLayout lay;
+  build_grad_layout(csr, n, dims, T > 0, lay, s);
+  const bool tb = lay.tb, sell = lay.sell;
+  SellPlan &sp = lay.sp;
+  const uint32_t nb = lay.loss_blocks(static_cast<uint32_t>(n));
   DevBuf<float> part(record_loss ? T * nb : 0, kSlotLossPart), d_loss(record_loss ? T : 0);
   if (T == 0 && y_dev) download(y.data(), y_dev, nd, s);
   PM_HIP(hipStreamSynchronize(s));
@@ -450,7 +467,7 @@ void optimize_device(const DevCsr &csr, uint64_t n, uint64_t dims, std::vector<f
   for (uint64_t it = 0; it < T; ++it) {
     GradArgs &a = plan[it];
     a.row_ptr = csr.row_ptr.get();
-    a.ent = tb && !sell ? tb_ent.get() : csr.ent.get();
+    a.ent = lay.entries(csr);
     a.mean_row_len = static_cast<float>(static_cast<double>(csr.nnz) / static_cast<double>(n));
     a.y_in = ybuf.get() + (it % 2) * ystride;
     a.y_out = ybuf.get() + ((it + 1) % 2) * ystride;
@@ -472,8 +489,8 @@ void optimize_device(const DevCsr &csr, uint64_t n, uint64_t dims, std::vector<f
         kw.add(static_cast<uint64_t>(sp.span)), kw.add(static_cast<uint64_t>(sp.rc)),
         kw.add(static_cast<uint64_t>(sp.chunks)), kw.add(static_cast<uint64_t>(sp.smax)),
         kw.add(static_cast<uint64_t>(sp.words));
-    kw.add(sell_perm.get()), kw.add(sell_off.get()), kw.add(sell_ent.get()), kw.add(tb_off.get()),
-        kw.add(tb_part.get());
+    kw.add(lay.sell_perm.get()), kw.add(lay.sell_off.get()), kw.add(lay.sell_ent.get()), kw.add(lay.tb_off.get()),
+        kw.add(lay.tb_part.get());
     for (const GradArgs &a : plan) kw.add(a);
   }
   if (!(g_graph.e && g_graph.key == key)) {
@@ -481,15 +498,7 @@ void optimize_device(const DevCsr &csr, uint64_t n, uint64_t dims, std::vector<f
     PM_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
     hipError_t cap_err = hipSuccess;
     for (uint64_t it = 0; it < T && cap_err == hipSuccess; ++it) {
-      const GradArgs &a = plan[it];
-      if (sell) {
-        cap_err = launch_grad_adam_sell(a, sp, sell_perm.get(), sell_off.get(), sell_ent.get(), tb_part.get(),
-                                        static_cast<uint32_t>(dims), true, s);
-      } else if (tb) {
-        cap_err = launch_grad_adam_tb(a, tb_off.get(), tb_part.get(), static_cast<uint32_t>(dims), true, s);
-      } else {
-        cap_err = launch_grad_adam(a, static_cast<uint32_t>(dims), true, s);
-      }
+      cap_err = lay.launch(plan[it], static_cast<uint32_t>(dims), true, s);
     }
     const hipError_t end_err = hipStreamEndCapture(s, &g_graph.g);
     if (cap_err != hipSuccess) fail_gpu(cap_err, "grad_adam launch during capture", __FILE__, __LINE__);
@@ -640,7 +649,9 @@ void value_init(const float *yin, uint64_t n, uint64_t dims, std::vector<float> 
     }
 }
 
-// Random(seed) initialisation: N(0, 1) draws (Box-Muller on the SmallRng stream).
+// Random(seed) initialisation (lib.rs:769-778): Normal<f32>(0, 1) draws in
+// row-major order from SmallRng::seed_from_u64(seed), each the f64
+// StandardNormal sample rounded to f32, times 1, plus 0.
 void random_init(uint64_t n, uint64_t dims, bool has_seed, uint64_t seed, std::vector<float> &y) {
   if (!has_seed) {
     std::random_device rd;
@@ -648,13 +659,7 @@ void random_init(uint64_t n, uint64_t dims, bool has_seed, uint64_t seed, std::v
   }
   SmallRng rng = SmallRng::seed_from_u64(seed);
   y.resize(n * dims);
-  const double two_pi = 6.283185307179586;
-  for (uint64_t e = 0; e < n * dims; e += 2) {
-    const double u1 = 1.0 - rng.next_f64(), u2 = rng.next_f64();
-    const double r = std::sqrt(-2.0 * std::log(u1));
-    y[e] = static_cast<float>(r * std::cos(two_pi * u2));
-    if (e + 1 < n * dims) y[e + 1] = static_cast<float>(r * std::sin(two_pi * u2));
-  }
+  for (uint64_t e = 0; e < n * dims; ++e) y[e] = 0.0f + 1.0f * static_cast<float>(rng.standard_normal());
 }
 
 // ---------------------------------------------------------------------------
@@ -949,13 +954,17 @@ int pacmap_grad(const float *y, uint64_t n, uint64_t dims, const uint32_t *pair_
     const hipStream_t s = stream.get();
     DevCsr csr;
     build_csr_from_host(n, lists, csr, s);
-    DevBuf<float> dy(n * dims), dg(n * dims);
+    // the kernel the optimiser runs at this size, in gradient mode; Y starts
+    // 256-byte aligned as the optimiser's buffers do
+    GradLayout lay;
+    build_grad_layout(csr, n, dims, true, lay, s);
+    DevBuf<float> dy(n * dims, kSlotY), dg(n * dims);
     upload(dy, y, n * dims, s);
-    const uint32_t nb = grad_blocks(static_cast<uint32_t>(n));
+    const uint32_t nb = lay.loss_blocks(static_cast<uint32_t>(n));
     DevBuf<float> part(nb), loss(1);
     GradArgs a;
     a.row_ptr = csr.row_ptr.get();
-    a.ent = csr.ent.get();
+    a.ent = lay.entries(csr);
     a.mean_row_len = static_cast<float>(static_cast<double>(csr.nnz) / static_cast<double>(n));
     a.y_in = dy.get();
     a.grad_out = dg.get();
@@ -964,7 +973,7 @@ int pacmap_grad(const float *y, uint64_t n, uint64_t dims, const uint32_t *pair_
     a.w[0] = w_nn;
     a.w[1] = w_mn;
     a.w[2] = w_fp;
-    PM_HIP(launch_grad_adam(a, static_cast<uint32_t>(dims), false, s));
+    PM_HIP(lay.launch(a, static_cast<uint32_t>(dims), false, s));
     PM_HIP(launch_loss_reduce(part.get(), nb, 1, loss.get(), s));
     download(grad_out, dg.get(), n * dims, s);
     float lv = 0.f;
@@ -1023,6 +1032,27 @@ int pacmap_sigma_scale(const float *knn_dist, const uint32_t *knn_idx, uint64_t 
     if (sigma_out) download(sigma_out, dsig.get(), n, s);
     if (scaled_out) download(scaled_out, dsc.get(), n * kprime, s);
     if (pairs_out) download(pairs_out, dp.get(), n * n_neighbors * 2, s);
+    PM_HIP(hipStreamSynchronize(s));
+  });
+}
+
+int pacmap_scale_dist(const float *knn_dist, const float *sig, const uint32_t *nbrs, uint64_t n,
+                      uint64_t k, float *scaled_out, pacmap_status *st) {
+  return guarded(st, [&] {
+    if (n == 0 || k == 0) return;
+    if (k > 512) fail(PACMAP_ERR_UNSUPPORTED, "k above 512");
+    if (!knn_dist || !sig || !nbrs || !scaled_out) fail(PACMAP_ERR_INVALID_ARGUMENT, "NULL array");
+    for (uint64_t e = 0; e < n * k; ++e)
+      if (nbrs[e] >= n) fail(PACMAP_ERR_INVALID_PAIR_INDEX, "neighbour index out of range");
+    Stream stream;
+    const hipStream_t s = stream.get();
+    DevBuf<float> dd(n * k), dsig(n), dsc(n * k);
+    DevBuf<uint32_t> di(n * k);
+    upload(dd, knn_dist, n * k, s);
+    upload(dsig, sig, n, s);
+    upload(di, nbrs, n * k, s);
+    PM_HIP(launch_scale_nn(dd.get(), di.get(), dsig.get(), n, static_cast<uint32_t>(k), 0, dsc.get(), nullptr, s));
+    download(scaled_out, dsc.get(), n * k, s);
     PM_HIP(hipStreamSynchronize(s));
   });
 }
@@ -1139,6 +1169,14 @@ int pacmap_dev_copy(void *dst_dev, const void *src_dev, uint64_t bytes, uint32_t
 
 void pacmap_last_timings(pacmap_timings *out) {
   if (out) *out = g_tm;
+}
+
+uint64_t pacmap_last_rescanned_rows(uint32_t *rows_out, uint64_t cap) {
+  std::vector<uint32_t> v = g_rescanned;
+  std::sort(v.begin(), v.end());
+  if (rows_out)
+    for (uint64_t i = 0; i < std::min<uint64_t>(cap, v.size()); ++i) rows_out[i] = v[i];
+  return v.size();
 }
 
 int pacmap_device_info(char *name_out, uint64_t name_len, int32_t *cu_count_out) {

@@ -303,8 +303,6 @@ double now_ms();
 void find_weights(float w_mn_init, uint64_t itr, uint64_t p1, uint64_t p2, float out[3]);
 // adam.rs:46-47 (powi lowered to binary square-and-multiply)
 float adam_lr_t(float lr, float beta1, float beta2, uint64_t itr);
-// distance.rs:50-75
-float euclid_ref(const float *a, const float *b, uint64_t d);
 // lib.rs:565-597 (throws on invalid counts)
 void decide_num_pairs(uint64_t n, bool has_override, uint64_t override_k, float mn_ratio,
                       float fp_ratio, uint64_t out[3]);
@@ -317,24 +315,13 @@ struct SmallRng {
   uint32_t next_u32() { return static_cast<uint32_t>(next_u64() >> 32); }
   uint32_t gen_range(uint32_t low, uint32_t high);
   double next_f64();  // [0, 1) with 53 bits
+  double standard_normal();  // rand_distr 0.4.3 StandardNormal (ziggurat)
 };
 
-// sampling.rs:77-103 / 157-195 (seed == nullptr: fresh entropy per point)
-void sample_fp_pairs(uint64_t n, const uint32_t *pair_neighbors, uint64_t n_neighbors,
-                     uint64_t n_fp, const uint64_t *seed, uint32_t *pair_fp);
-// sampling.rs:118-142 / 216-235 + 296-313
-void sample_mn_pairs(const float *x, uint64_t n, uint64_t d, uint64_t n_mn, const uint64_t *seed,
-                     uint32_t *pair_mn);
-
-// Symmetrised CSR of the three pair lists (see internal.h for the entry bits).
+// A pair array (rows x 2 u32) in host memory.
 struct PairList {
   const uint32_t *p = nullptr;
   uint64_t rows = 0;
 };
-struct Csr {
-  std::vector<uint32_t> row_ptr;
-  std::vector<uint32_t> ent;
-};
-void build_csr(uint64_t n, const PairList lists[3], Csr &out);
 
 }  // namespace pm

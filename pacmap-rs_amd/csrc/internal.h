@@ -44,15 +44,17 @@ hipError_t launch_sample_pairs(const float *xp, const KnnLayout &L, uint32_t n, 
                                uint32_t knn, uint32_t n_mn, uint32_t n_fp, uint64_t seed, int seeded,
                                uint32_t *pair_mn, uint32_t *pair_fp, uint32_t *stream_ws, hipStream_t s);
 // Symmetrised CSR on the GPU: count degrees (flags out-of-range indices in
-// bad[0]), scan to row offsets, fill, and sort each row by entry value (rows
-// longer than 2048 entries set too_long[0] and stay unsorted).
+// bad[0]), scan to row offsets, fill, and sort each row by entry value.  Rows
+// of up to 2048 entries are sorted by a wavefront each; longer rows are
+// listed (long_count zeroed by the caller, long_list of long_cap >=
+// entries / 2049 + 1 words) and sorted by a workgroup each.
 hipError_t launch_csr_count(const uint32_t *const p[3], const uint64_t rows[3], uint32_t n,
                             uint32_t *deg, uint32_t *bad, hipStream_t s);
 hipError_t launch_csr_scan(const uint32_t *deg, uint32_t n, uint32_t *row_ptr, uint32_t *cursor,
                            uint32_t *scratch /* (n + 1023) / 1024 + 1 */, hipStream_t s);
 hipError_t launch_csr_fill_sort(const uint32_t *const p[3], const uint64_t rows[3], uint32_t n,
                                 const uint32_t *row_ptr, uint32_t *cursor, uint32_t *ent,
-                                uint32_t *too_long, hipStream_t s);
+                                uint32_t *long_count, uint32_t *long_list, uint32_t long_cap, hipStream_t s);
 
 // Exact rescan of the nq listed rows with the candidate range split over S
 // blocks per row; part holds nq x S x k partial keys.

@@ -12,6 +12,8 @@
 // atomic order.
 #include "internal.h"
 
+#include <algorithm>
+
 #include <cstdint>
 
 namespace pm {
@@ -515,18 +517,23 @@ __device__ __forceinline__ void wave_sort_u32(uint32_t (&v)[R], int lane) {
 }
 
 // one wavefront per row of length <= 64 * R (longer rows are left for the
-// next size class; rows above the largest class are reported)
+// next size class; rows above the largest class are appended to long_list
+// for csr_sort_long_kernel)
 template <int R>
 __global__ __launch_bounds__(256) void csr_sort_rows_kernel(const uint32_t *__restrict__ row_ptr, uint32_t n,
                                                             uint32_t min_len, uint32_t *__restrict__ ent,
-                                                            uint32_t *__restrict__ too_long) {
+                                                            uint32_t *__restrict__ long_count,
+                                                            uint32_t *__restrict__ long_list, uint32_t long_cap) {
   const int lane = threadIdx.x & 63;
   const uint32_t row = blockIdx.x * 4 + (threadIdx.x >> 6);
   if (row >= n) return;
   const uint32_t b = row_ptr[row], len = row_ptr[row + 1] - b;
   if (len <= min_len || len < 2) return;
   if (len > 64u * R) {
-    if (R == 32 && lane == 0) atomicOr(too_long, 1u);
+    if (R == 32 && lane == 0) {
+      const uint32_t slot = atomicAdd(long_count, 1u);
+      if (slot < long_cap) long_list[slot] = row;
+    }
     return;
   }
   uint32_t v[R];
@@ -540,6 +547,74 @@ __global__ __launch_bounds__(256) void csr_sort_rows_kernel(const uint32_t *__re
   for (int r = 0; r < R; ++r) {
     const uint32_t e = r * 64 + lane;
     if (e < len) ent[b + e] = v[r];
+  }
+}
+
+// Ascending bitonic sort of a[0, len) by one workgroup, len padded to a
+// power of two N with virtual +infinity entries.  Each stage of size S first
+// compares i with its mirror in the S-block (S-1-offset), then runs the
+// half-cleaners of stride S/4 .. 1; every comparison puts the smaller value
+// at the lower index, so a partner at or beyond len (+infinity) never moves
+// and is skipped.  a is in LDS or global memory; the barriers order both.
+template <class P>
+__device__ void block_bitonic_sort(P a, uint32_t len) {
+  uint32_t N = 1;
+  while (N < len) N <<= 1;
+  const uint32_t pairs = N >> 1;
+  for (uint32_t size = 2; size <= N; size <<= 1) {
+    const uint32_t hs = __builtin_ctz(size) - 1;  // log2(size / 2)
+    for (uint32_t p = threadIdx.x; p < pairs; p += blockDim.x) {
+      const uint32_t blk = p >> hs, off = p & ((1u << hs) - 1u);
+      const uint32_t i = (blk << (hs + 1)) + off, j = (blk << (hs + 1)) + size - 1u - off;
+      if (j < len) {
+        const uint32_t x = a[i], y = a[j];
+        if (x > y) {
+          a[i] = y;
+          a[j] = x;
+        }
+      }
+    }
+    __syncthreads();
+    for (uint32_t st = size >> 2; st > 0; st >>= 1) {
+      const uint32_t ls = __builtin_ctz(st);
+      for (uint32_t p = threadIdx.x; p < pairs; p += blockDim.x) {
+        const uint32_t blk = p >> ls, off = p & (st - 1u);
+        const uint32_t i = (blk << (ls + 1)) + off, j = i + st;
+        if (j < len) {
+          const uint32_t x = a[i], y = a[j];
+          if (x > y) {
+            a[i] = y;
+            a[j] = x;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+}
+
+// Rows longer than the wave sorts take: one workgroup per listed row (the
+// list count is read on the device, the grid strides over it).  A row of up
+// to kLongRowLds entries is sorted in LDS, a longer one in place.
+constexpr uint32_t kLongRowLds = 32768;
+__global__ __launch_bounds__(1024) void csr_sort_long_kernel(const uint32_t *__restrict__ row_ptr,
+                                                             const uint32_t *__restrict__ long_count,
+                                                             const uint32_t *__restrict__ long_list, uint32_t long_cap,
+                                                             uint32_t *__restrict__ ent) {
+  extern __shared__ uint32_t srow[];
+  const uint32_t cnt = min(*long_count, long_cap);
+  for (uint32_t q = blockIdx.x; q < cnt; q += gridDim.x) {
+    const uint32_t row = long_list[q];
+    const uint32_t b = row_ptr[row], len = row_ptr[row + 1] - b;
+    if (len <= kLongRowLds) {
+      for (uint32_t e = threadIdx.x; e < len; e += blockDim.x) srow[e] = ent[b + e];
+      __syncthreads();
+      block_bitonic_sort(srow, len);
+      for (uint32_t e = threadIdx.x; e < len; e += blockDim.x) ent[b + e] = srow[e];
+    } else {
+      block_bitonic_sort(ent + b, len);
+    }
+    __syncthreads();
   }
 }
 
@@ -573,7 +648,7 @@ hipError_t launch_csr_scan(const uint32_t *deg, uint32_t n, uint32_t *row_ptr, u
 
 hipError_t launch_csr_fill_sort(const uint32_t *const p[3], const uint64_t rows[3], uint32_t n,
                                 const uint32_t *row_ptr, uint32_t *cursor, uint32_t *ent,
-                                uint32_t *too_long, hipStream_t s) {
+                                uint32_t *long_count, uint32_t *long_list, uint32_t long_cap, hipStream_t s) {
   PairArrays pa;
   uint64_t total = 0;
   for (int t = 0; t < 3; ++t) {
@@ -583,9 +658,11 @@ hipError_t launch_csr_fill_sort(const uint32_t *const p[3], const uint64_t rows[
   }
   if (total) csr_fill_kernel<<<grid_for(total), 256, 0, s>>>(pa, total, n, cursor, ent);
   const uint32_t blocks = (n + 3) / 4;
-  csr_sort_rows_kernel<1><<<blocks, 256, 0, s>>>(row_ptr, n, 0, ent, too_long);
-  csr_sort_rows_kernel<4><<<blocks, 256, 0, s>>>(row_ptr, n, 64, ent, too_long);
-  csr_sort_rows_kernel<32><<<blocks, 256, 0, s>>>(row_ptr, n, 256, ent, too_long);
+  csr_sort_rows_kernel<1><<<blocks, 256, 0, s>>>(row_ptr, n, 0, ent, long_count, long_list, long_cap);
+  csr_sort_rows_kernel<4><<<blocks, 256, 0, s>>>(row_ptr, n, 64, ent, long_count, long_list, long_cap);
+  csr_sort_rows_kernel<32><<<blocks, 256, 0, s>>>(row_ptr, n, 256, ent, long_count, long_list, long_cap);
+  const uint32_t g = std::min<uint32_t>(256u, std::max<uint32_t>(1u, long_cap));
+  csr_sort_long_kernel<<<g, 1024, kLongRowLds * sizeof(uint32_t), s>>>(row_ptr, long_count, long_list, long_cap, ent);
   return hipGetLastError();
 }
 

@@ -1,12 +1,10 @@
-// sampling.cpp -- host pair sampling, the reference arithmetic the host needs,
-// and the symmetrised CSR build.
-//
-// Pair sampling stays on the host with the reference's RNG so pair lists follow
-// the reference's streams: rand 0.8.5 SmallRng (Xoshiro256++ on 64-bit
-// targets) seeded by rand_core 0.6's default seed_from_u64 (PCG32 expansion),
-// and UniformInt<u32>::sample_single (widening multiply, zone rejection).
+// sampling.cpp -- the reference arithmetic the host needs: the pair-count
+// policy, the weight schedule, Adam's learning rate, and rand 0.8.5's SmallRng
+// (Xoshiro256++ on 64-bit targets, seeded by rand_core 0.6's default
+// seed_from_u64: PCG32 expansion) with UniformInt<u32>::sample_single and
+// rand_distr's StandardNormal.  The pair samplers built on these streams run
+// on the GPU (pairs.hip).
 #include <algorithm>
-#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <random>
@@ -82,24 +80,6 @@ float adam_lr_t(float lr, float beta1, float beta2, uint64_t itr) {
   return lr_num / (1.0f - powi_f32(beta1, t));
 }
 
-float euclid_ref(const float *a, const float *b, uint64_t d) {
-  float acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-  const uint64_t chunks = d / 8;
-  for (uint64_t c = 0; c < chunks; ++c)
-    for (int l = 0; l < 8; ++l) {
-      const float df = a[8 * c + l] - b[8 * c + l];
-      const float sq = df * df;
-      acc[l] = acc[l] + sq;
-    }
-  float s = 0.0f;
-  for (int l = 0; l < 8; ++l) s = s + acc[l];
-  for (uint64_t e = chunks * 8; e < d; ++e) {
-    const float df = a[e] - b[e];
-    const float sq = df * df;
-    s = s + sq;
-  }
-  return std::sqrt(s);
-}
 
 void decide_num_pairs(uint64_t n, bool has_override, uint64_t override_k, float mn_ratio,
                       float fp_ratio, uint64_t out[3]) {
@@ -190,182 +170,55 @@ uint32_t SmallRng::gen_range(uint32_t low, uint32_t high) {
 
 double SmallRng::next_f64() { return static_cast<double>(next_u64() >> 11) * (1.0 / 9007199254740992.0); }
 
-// sampling.rs:37-61
-static uint64_t sample_fp(uint64_t n_samples, uint32_t maximum, const uint32_t *reject,
-                          uint64_t n_reject, uint64_t stride, uint32_t self_ind, SmallRng &rng,
-                          uint32_t *out) {
-  bool self_absent = true;
-  for (uint64_t q = 0; q < n_reject; ++q)
-    if (reject[q * stride] == self_ind) self_absent = false;
-  uint64_t avail = maximum;
-  avail = avail > n_reject ? avail - n_reject : 0;
-  avail = avail > (self_absent ? 1u : 0u) ? avail - (self_absent ? 1u : 0u) : 0;
-  n_samples = std::min(n_samples, avail);
-  uint64_t len = 0;
-  while (len < n_samples) {
-    const uint32_t j = rng.gen_range(0, maximum);
-    if (j == self_ind) continue;
-    bool dup = false;
-    for (uint64_t q = 0; q < len && !dup; ++q) dup = out[q] == j;
-    for (uint64_t q = 0; q < n_reject && !dup; ++q) dup = reject[q * stride] == j;
-    if (!dup) out[len++] = j;
+// A value of `mant52` (52 mantissa bits) scaled into [2^e, 2^(e+1)).
+static double f64_with_exponent(uint64_t mant52, int e) {
+  const uint64_t b = mant52 | (static_cast<uint64_t>(1023 + e) << 52);
+  double v;
+  std::memcpy(&v, &b, sizeof(v));
+  return v;
+}
+
+// Ziggurat layers of N(0, 1) as rand_distr 0.4.3 tabulates them: Doornik's
+// ZIGNOR construction with 256 layers of area V under exp(-x^2 / 2), the
+// base layer at R including the tail.
+namespace {
+constexpr double kZigR = 3.6541528853610088, kZigV = 0.00492867323399;
+struct ZigTables {
+  double x[257], f[257];
+  ZigTables() {
+    auto pdf = [](double v) { return std::exp(-v * v / 2.0); };
+    x[0] = kZigV / pdf(kZigR);
+    x[1] = kZigR;
+    for (int i = 2; i < 256; ++i) x[i] = std::sqrt(-2.0 * std::log(kZigV / x[i - 1] + pdf(x[i - 1])));
+    x[256] = 0.0;
+    for (int i = 0; i < 257; ++i) f[i] = pdf(x[i]);
   }
-  return len;
-}
+};
+}  // namespace
 
-static uint64_t entropy_seed() {
-  std::random_device rd;
-  return (static_cast<uint64_t>(rd()) << 32) ^ rd();
-}
-
-void sample_fp_pairs(uint64_t n, const uint32_t *pair_neighbors, uint64_t n_neighbors,
-                     uint64_t n_fp, const uint64_t *seed, uint32_t *pair_fp) {
-  std::memset(pair_fp, 0, sizeof(uint32_t) * 2 * n * n_fp);
-  const uint64_t base = seed ? *seed : entropy_seed();
-  parallel_for(n, [&](uint64_t b, uint64_t e) {
-    std::vector<uint32_t> buf(n_fp ? n_fp : 1);
-    for (uint64_t i = b; i < e; ++i) {
-      // seeded: SmallRng::seed_from_u64(seed + i) (sampling.rs:97); unseeded
-      // (thread_rng, sampling.rs:176) draws an independent stream per point
-      SmallRng rng = SmallRng::seed_from_u64(seed ? base + i : base ^ (0x9e3779b97f4a7c15ull * (i + 1)));
-      const uint64_t got =
-          sample_fp(n_fp, static_cast<uint32_t>(n), pair_neighbors + 2 * i * n_neighbors + 1,
-                    n_neighbors, 2, static_cast<uint32_t>(i), rng, buf.data());
-      for (uint64_t q = 0; q < got; ++q) {
-        pair_fp[2 * (i * n_fp + q)] = static_cast<uint32_t>(i);
-        pair_fp[2 * (i * n_fp + q) + 1] = buf[q];
+// rand_distr 0.4.3 StandardNormal (f64): one u64 gives the layer (low 8
+// bits) and a uniform in [-1, 1) (high 52 bits); the wedge test draws a
+// Standard f64, the base layer's tail draws Open01 pairs.
+double SmallRng::standard_normal() {
+  static const ZigTables z;
+  for (;;) {
+    const uint64_t bits = next_u64();
+    const unsigned i = static_cast<unsigned>(bits & 0xff);
+    const double u = f64_with_exponent(bits >> 12, 1) - 3.0;
+    const double x = u * z.x[i];
+    if (std::fabs(x) < z.x[i + 1]) return x;
+    if (i == 0) {
+      double xt = 1.0, yt = 0.0;
+      while (-2.0 * yt < xt * xt) {
+        const double a = f64_with_exponent(next_u64() >> 12, 0) - (1.0 - 1.1102230246251565e-16);
+        const double b = f64_with_exponent(next_u64() >> 12, 0) - (1.0 - 1.1102230246251565e-16);
+        xt = std::log(a) / kZigR;
+        yt = std::log(b);
       }
+      return u < 0.0 ? xt - kZigR : kZigR - xt;
     }
-  });
-}
-
-static inline uint32_t total_key(float f) {
-  uint32_t b;
-  std::memcpy(&b, &f, 4);
-  return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
-}
-
-void sample_mn_pairs(const float *x, uint64_t n, uint64_t d, uint64_t n_mn, const uint64_t *seed,
-                     uint32_t *pair_mn) {
-  std::memset(pair_mn, 0, sizeof(uint32_t) * 2 * n * n_mn);
-  const uint64_t base = seed ? *seed : entropy_seed();
-  parallel_for(n, [&](uint64_t b, uint64_t e) {
-    for (uint64_t i = b; i < e; ++i) {
-      SmallRng rng = SmallRng::seed_from_u64(seed ? base + i : base ^ (0x9e3779b97f4a7c15ull * (i + 1)));
-      uint32_t *rows = pair_mn + 2 * i * n_mn;
-      for (uint64_t j = 0; j < n_mn; ++j) {
-        uint32_t sampled[6];
-        const uint64_t got = sample_fp(6, static_cast<uint32_t>(n), rows + 1, j, 2,
-                                       static_cast<uint32_t>(i), rng, sampled);
-        float dist[6] = {0, 0, 0, 0, 0, 0};
-        uint32_t idx[6] = {0, 0, 0, 0, 0, 0};
-        for (uint64_t q = 0; q < got; ++q) {
-          dist[q] = euclid_ref(x + i * d, x + static_cast<uint64_t>(sampled[q]) * d, d);
-          idx[q] = sampled[q];
-        }
-        // sampling.rs:308: order by total_cmp; equal distances keep slot order
-        for (int a = 1; a < 6; ++a) {
-          const float td = dist[a];
-          const uint32_t ti = idx[a];
-          int c = a - 1;
-          while (c >= 0 && total_key(dist[c]) > total_key(td)) {
-            dist[c + 1] = dist[c];
-            idx[c + 1] = idx[c];
-            --c;
-          }
-          dist[c + 1] = td;
-          idx[c + 1] = ti;
-        }
-        rows[2 * j] = static_cast<uint32_t>(i);
-        rows[2 * j + 1] = idx[1];
-      }
-    }
-  });
-}
-
-// ---------------------------------------------------------------------------
-// symmetrised CSR: every non-self pair (i, j) of type t appears in row i as
-// (j, t, owner) and in row j as (i, t).  Row order follows the global pair
-// order (NN, then MN, then FP, each in array order), which fixes the
-// summation order of every row.
-// ---------------------------------------------------------------------------
-void build_csr(uint64_t n, const PairList lists[3], Csr &out) {
-  if (n >= kMaxPoints) fail(PACMAP_ERR_UNSUPPORTED, "too many points for the CSR entry format");
-  uint64_t total_pairs = 0;
-  for (int t = 0; t < 3; ++t) total_pairs += lists[t].rows;
-  const int T = host_threads();
-  const uint64_t nt = std::max<uint64_t>(1, std::min<uint64_t>(T, total_pairs / 65536 + 1));
-  auto pair_at = [&](uint64_t p, int &type) -> const uint32_t * {
-    for (int t = 0; t < 3; ++t) {
-      if (p < lists[t].rows) {
-        type = t;
-        return lists[t].p + 2 * p;
-      }
-      p -= lists[t].rows;
-    }
-    type = -1;
-    return nullptr;
-  };
-  std::vector<std::vector<uint32_t>> cnt(nt, std::vector<uint32_t>(n, 0));
-  std::atomic<bool> bad{false};
-  std::atomic<uint64_t> bad_pair{0};
-  parallel_for(nt, [&](uint64_t tb, uint64_t te) {
-    for (uint64_t t = tb; t < te; ++t) {
-      const uint64_t p0 = total_pairs * t / nt, p1 = total_pairs * (t + 1) / nt;
-      auto &c = cnt[t];
-      for (uint64_t p = p0; p < p1; ++p) {
-        int type;
-        const uint32_t *pr = pair_at(p, type);
-        const uint32_t i = pr[0], j = pr[1];
-        if (i >= n || j >= n) {
-          bad = true;
-          bad_pair = p;
-          return;
-        }
-        if (i == j) continue;  // gradient.rs:116-118
-        ++c[i];
-        ++c[j];
-      }
-    }
-  }, static_cast<int>(nt));
-  if (bad) fail(PACMAP_ERR_INVALID_PAIR_INDEX, "pair index out of range at pair " + std::to_string(bad_pair.load()));
-  out.row_ptr.assign(n + 1, 0);
-  uint64_t acc = 0;
-  for (uint64_t r = 0; r < n; ++r) {
-    out.row_ptr[r] = static_cast<uint32_t>(acc);
-    uint64_t deg = 0;
-    for (uint64_t t = 0; t < nt; ++t) deg += cnt[t][r];
-    acc += deg;
-    if (acc >= 0xffffffffull) fail(PACMAP_ERR_UNSUPPORTED, "too many pairs for 32-bit CSR offsets");
+    if (z.f[i + 1] + (z.f[i] - z.f[i + 1]) * next_f64() < std::exp(-x * x / 2.0)) return x;
   }
-  out.row_ptr[n] = static_cast<uint32_t>(acc);
-  // per-thread start offsets inside each row (turn counts into cursors)
-  parallel_for(n, [&](uint64_t rb, uint64_t re) {
-    for (uint64_t r = rb; r < re; ++r) {
-      uint32_t cur = out.row_ptr[r];
-      for (uint64_t t = 0; t < nt; ++t) {
-        const uint32_t c = cnt[t][r];
-        cnt[t][r] = cur;
-        cur += c;
-      }
-    }
-  });
-  out.ent.assign(acc, 0);
-  parallel_for(nt, [&](uint64_t tb, uint64_t te) {
-    for (uint64_t t = tb; t < te; ++t) {
-      const uint64_t p0 = total_pairs * t / nt, p1 = total_pairs * (t + 1) / nt;
-      auto &c = cnt[t];
-      for (uint64_t p = p0; p < p1; ++p) {
-        int type;
-        const uint32_t *pr = pair_at(p, type);
-        const uint32_t i = pr[0], j = pr[1];
-        if (i == j) continue;
-        const uint32_t tb30 = static_cast<uint32_t>(type) << 30;
-        out.ent[c[i]++] = j | tb30 | kOwnerBit;
-        out.ent[c[j]++] = i | tb30;
-      }
-    }
-  }, static_cast<int>(nt));
 }
 
 }  // namespace pm

@@ -125,12 +125,15 @@ def lib():
         L.pacmap_weights.argtypes = [C.c_float, _u64, _u64, _u64, _P]
         L.pacmap_weights.restype = None
         L.pacmap_sigma_scale.argtypes = [_P, _P, _u64, _u64, _u64, _P, _P, _P, C.POINTER(Status)]
+        L.pacmap_scale_dist.argtypes = [_P, _P, _P, _u64, _u64, _P, C.POINTER(Status)]
         L.pacmap_sample_pairs.argtypes = [_P, _u64, _u64, _P, _u64, _u64, _u64, _u64, _P, _P,
                                           C.POINTER(Status)]
         L.pacmap_optimize.argtypes = [_P, _u64, _u64, _P, _u64, _P, _u64, _P, _u64, C.c_float,
                                       _P, _P, _u64, _P, _P, C.POINTER(Status)]
         L.pacmap_generate_pairs.argtypes = [_P, _u64, _u64, C.POINTER(_Config), _P, _P, _P, _P, _P,
                                             C.POINTER(Status)]
+        L.pacmap_last_rescanned_rows.argtypes = [_P, _u64]
+        L.pacmap_last_rescanned_rows.restype = _u64
         L.pacmap_mlp_config_default.argtypes = [C.POINTER(MlpConfig)]
         L.pacmap_mlp_config_default.restype = None
         L.pacmap_mlp_param_count.argtypes = [_u64, _u64, _u64]
@@ -470,6 +473,19 @@ def sigma_scale(knn_dist, knn_idx, n_neighbors=0):
     return sig, sc, pairs
 
 
+def scale_dist(knn_distances, sig, neighbors):
+    """Seam for distance.rs:27-37: knn_distances^2 / (sig_i * sig_neighbor)."""
+    dist = np.ascontiguousarray(knn_distances, np.float32)
+    sig = np.ascontiguousarray(sig, np.float32)
+    nbrs = np.ascontiguousarray(neighbors, np.uint32)
+    n, k = dist.shape
+    out = np.zeros((n, k), np.float32)
+    st = Status()
+    _check(lib().pacmap_scale_dist(dist.ctypes.data, sig.ctypes.data, nbrs.ctypes.data, n, k,
+                                   out.ctypes.data, C.byref(st)), st)
+    return out
+
+
 def sample_pairs(x, pair_neighbors, n_neighbors, n_mn, n_fp, seed):
     """Seam for the seeded MN/FP samplers (sampling.rs:77-142): (pair_mn, pair_fp)."""
     x = np.ascontiguousarray(x, np.float32)
@@ -652,6 +668,16 @@ def last_timings() -> dict:
     t = Timings()
     lib().pacmap_last_timings(C.byref(t))
     return {name: getattr(t, name) for name, _ in Timings._fields_}
+
+
+def last_rescanned_rows():
+    """Rows the last kNN on this thread sent to the exact rescan (ascending)."""
+    L = lib()
+    count = int(L.pacmap_last_rescanned_rows(None, 0))
+    out = np.zeros(count, np.uint32)
+    if count:
+        L.pacmap_last_rescanned_rows(out.ctypes.data, count)
+    return out
 
 
 def device_info():

@@ -103,3 +103,22 @@ def test_missing_library_fails_loudly(tmp_path, monkeypatch):
     monkeypatch.setattr(pm, "_lib", None)
     with pytest.raises(ImportError):
         pm.lib()
+
+
+@pytest.mark.parametrize("n,counts", [(70_000, (23, 12, 46)), (150_000, (28, 14, 56)),
+                                      (1_000_000, (40, 20, 80)), (10_000, (10, 5, 20))])
+def test_generate_pairs_counts_without_override(n, counts):
+    """lib.rs:565-597 as the library applies it: above 10,000 points k is
+    round(10 + 15 (log10 n - 4)), MN and FP counts round(k * 0.5) and
+    round(k * 2.0).  The count query (NULL outputs) runs no device work."""
+    import ctypes as C
+    x = np.zeros((n, 3), np.float32)
+    c = pm.Configuration()._to_c([])
+    out = np.zeros(4, np.uint64)
+    st = pm.Status()
+    rc = pm.lib().pacmap_generate_pairs(x.ctypes.data, n, 3, C.byref(c), None, None, None, None,
+                                        out.ctypes.data, C.byref(st))
+    assert rc == 0, st.message
+    k, mn, fp = counts
+    assert tuple(int(v) for v in out) == (n * k, n * mn, n * fp, 3)
+    assert O.decide_num_pairs(n) == (0, counts)

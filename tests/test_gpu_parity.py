@@ -46,6 +46,39 @@ def assert_parity(got, ref, spread, rel=REL):
     assert worst <= 1.0, f"element error {worst:.2f}x the bound"
 
 
+def other_orders(y0, nn, mn, fp, it, nthreads=1):
+    """The oracle's 10-iteration result under two other valid summation
+    orders of the same pairs: every list reversed, and the far pairs
+    permuted (the reference's rayon reduce fixes no order, gradient.rs:53-70)."""
+    perm = np.random.default_rng(9).permutation(len(fp))
+    return [O.optimize(y0, np.ascontiguousarray(nn[::-1]), np.ascontiguousarray(mn[::-1]),
+                       np.ascontiguousarray(fp[::-1]), 1.0, it, nthreads=nthreads)[0],
+            O.optimize(y0, nn, mn, np.ascontiguousarray(fp[perm]), 1.0, it, nthreads=nthreads)[0]]
+
+
+def assert_parity_large(got, ref, alts, rel=REL):
+    """10-iteration parity at sizes where valid summation orders of the
+    reference itself disagree near the tolerance.  From a Value init (std
+    1e-4) Adam's first steps are about lr * sign(g), so a gradient component
+    within rounding noise of zero takes either sign depending on the order;
+    `alts` are the oracle under other valid orders.  Bars: norm-wise error
+    at most max(rel, 3x the largest norm-wise difference between oracle
+    orders), and no more elements off by more than rel * max(|ref|, rms)
+    than 3x the most such elements of another oracle order, plus 10."""
+    got, ref = np.asarray(got, np.float64), np.asarray(ref, np.float64)
+    nrm = np.linalg.norm(ref)
+    rms = np.sqrt(np.mean(ref ** 2))
+    tol = rel * np.maximum(np.abs(ref), rms)
+    a_err = max(np.linalg.norm(np.asarray(a, np.float64) - ref) / nrm for a in alts)
+    a_out = max(int((np.abs(np.asarray(a, np.float64) - ref) > tol).sum()) for a in alts)
+    g_err = np.linalg.norm(got - ref) / nrm
+    g_out = int((np.abs(got - ref) > tol).sum())
+    print(f"parity: norm-wise rel err {g_err:.2e} (oracle orders {a_err:.2e}), elements beyond "
+          f"{rel:g} relative {g_out} (oracle orders {a_out}) of {ref.size}", flush=True)
+    assert g_err <= max(rel, 3.0 * a_err)
+    assert g_out <= 3 * a_out + 10
+
+
 def oracle_loss64(y0, nn, mn, fp, lr, iters):
     """FP64 loss of every iteration along the oracle's FP32 trajectory
     (lib.rs:803-823).  The reference's own f32 loss sums 131072-term chunks
@@ -274,15 +307,22 @@ def test_sigma_scale_and_nn_pairs_bitwise():
     assert np.array_equal(pairs, ref_pairs)
 
 
-def test_scale_dist_golden_on_gpu(golden):
-    c = golden("scale_dist.json")["cases"][1]
+@pytest.mark.parametrize("which", [0, 1])
+def test_scale_dist_golden_on_gpu(golden, which):
+    """distance.rs:120-190: the reference's expected scaled distances for its
+    given sigma (exact for the 2x2 case, 1e-6 for the 10x5 one), and the
+    oracle's bits."""
+    c = golden("scale_dist.json")["cases"][which]
     dist = np.array(c["knn_distances"], np.float32)
+    sig = np.array(c["sig"], np.float32)
     idx = np.array(c["neighbors"], np.uint32)
-    # the golden provides sigma directly: recover scaled values through the seam
-    # by checking sigma-independent pieces against the oracle restatement
-    sig, sc, _ = pm.sigma_scale(dist, idx, 0)
-    assert np.array_equal(sig, O.sigma(dist))
-    assert np.array_equal(sc, O.scale_dist(dist, O.sigma(dist), idx))
+    got = pm.scale_dist(dist, sig, idx)
+    np.testing.assert_allclose(got, np.array(c["expected"], np.float32), rtol=0, atol=c["tolerance"])
+    assert np.array_equal(got.view(np.uint32), O.scale_dist(dist, sig, idx).view(np.uint32))
+    # sigma from the distances themselves (neighbors.rs:79-84) through the other seam
+    sig2, sc2, _ = pm.sigma_scale(dist, idx, 0)
+    assert np.array_equal(sig2, O.sigma(dist))
+    assert np.array_equal(sc2, O.scale_dist(dist, O.sigma(dist), idx))
 
 
 # --------------------------------------------------------------- sampling --
@@ -329,6 +369,35 @@ def test_grad_matches_oracle(dims):
     e_gpu = np.linalg.norm(got[:-1] - ref64[:-1])
     e_ref = np.linalg.norm(ref[:-1] - ref64[:-1])
     assert e_gpu <= 2.0 * e_ref + 1e-7 * np.linalg.norm(ref64[:-1])
+
+
+@pytest.mark.parametrize("n", [3000, 20000])
+def test_grad_hub_rows_sorted_on_the_gpu(n):
+    """Hub rows longer than a wavefront sorts (2048 entries): one point in
+    6,000 NN pairs (sorted in LDS) and another in 40,000 FP pairs (sorted in
+    place in global memory).  Every CSR row is sorted by entry value, so the
+    gradient is bit-identical for any order of the input pair lists; it
+    matches the oracle at 1e-5, and 10 iterations match the oracle's.
+    n = 20000 runs the LDS-staged gradient, n = 3000 the one-pass kernel."""
+    rng = np.random.default_rng(n)
+    y = rng.normal(size=(n, 2)).astype(np.float32)
+    hub_a = np.stack([np.zeros(6000, np.int64), rng.integers(1, n, 6000)], 1)
+    hub_b = np.stack([rng.integers(0, n, 40000), np.ones(40000, np.int64)], 1)
+    nn = np.concatenate([_pairs(rng, n, n * 5), hub_a]).astype(np.uint32)
+    mn = _pairs(rng, n, n * 3)
+    fp = np.concatenate([_pairs(rng, n, n * 10), hub_b]).astype(np.uint32)
+    w = (2.0, 120.0, 1.0)
+    g1 = pm.pacmap_grad(y, nn, mn, fp, *w)
+    perm = [rng.permutation(len(a)) for a in (nn, mn, fp)]
+    g2 = pm.pacmap_grad(y, nn[perm[0]], mn[perm[1]], fp[perm[2]], *w)
+    assert np.array_equal(g1[:-1].view(np.uint32), g2[:-1].view(np.uint32))
+    assert_rel(g1[:-1], O.pacmap_grad(y, nn, mn, fp, *w)[:-1])
+    it = (4, 3, 3)
+    y0 = O.value_init(y)
+    got, _, _ = pm.optimize(y0, nn, mn, fp, 1.0, it)
+    assert pm.last_timings()["grad_sell"] == (1 if n >= 16384 else 0)
+    ry, _, _ = O.optimize(y0, nn, mn, fp, 1.0, it)
+    assert_parity_large(got, ry, other_orders(y0, nn, mn, fp, it))
 
 
 def test_grad_rejects_out_of_range_pairs():
@@ -545,6 +614,42 @@ def test_fit_transform_generate_matches_oracle_pipeline():
     ry, _, rloss = O.optimize(y0, nn, mn, fp, 1.0, it)
     assert_parity(y, ry, order_spread(lambda a, b, c: O.optimize(y0, a, b, c, 1.0, it)[0], nn, mn, fp))
     assert_loss(loss, oracle_loss64(y0, nn, mn, fp, 1.0, it))
+
+
+def test_fit_transform_neighbors_provided_matches_oracle():
+    """PairConfiguration::NeighborsProvided (lib.rs:651-676): the given NN
+    pairs are kept, MN and FP pairs come from the seeded samplers on the
+    preprocessed data with the given NN pairs as the far-pair reject lists
+    (neighbors.rs:127-145); 10 iterations from a Value init at 1e-5."""
+    rng = np.random.default_rng(72)
+    n, d, k = 1500, 30, 10
+    x = rng.normal(size=(n, d)).astype(np.float32)
+    v = rng.normal(size=(n, 2)).astype(np.float32)
+    nn = np.stack([np.repeat(np.arange(n), k), rng.integers(0, n, n * k)], 1).astype(np.uint32)
+    it = (4, 3, 3)
+    cfg = pm.Configuration(initialization=pm.Initialization.Value(v), seed=7, num_iters=it,
+                           pair_configuration=pm.PairConfiguration.NeighborsProvided(nn))
+    y, _, loss = pm.fit_transform(x, cfg, return_loss=True)
+    xn = O.normalize(x)
+    mn = O.sample_mn_pair_deterministic(xn, 5, 7)
+    fp = O.sample_fp_pair_deterministic(n, nn, k, 20, 7)
+    y0 = O.value_init(v)
+    ry, _, _ = O.optimize(y0, nn, mn, fp, 1.0, it)
+    assert_parity(y, ry, order_spread(lambda a, b, c: O.optimize(y0, a, b, c, 1.0, it)[0], nn, mn, fp))
+    assert_loss(loss, oracle_loss64(y0, nn, mn, fp, 1.0, it))
+
+
+def test_random_init_matches_oracle_draw():
+    """Initialization::Random(Some(seed)) (lib.rs:769-778): with no iterations
+    fit_transform returns the initial Y, which is the oracle's rand_distr
+    StandardNormal draw (ziggurat) from SmallRng::seed_from_u64(seed), bit for
+    bit, in row-major order (parity unpinned against rand_distr itself)."""
+    x = np.random.default_rng(73).normal(size=(500, 12)).astype(np.float32)
+    for dims, seed in ((2, 42), (3, 7)):
+        cfg = pm.Configuration(initialization=pm.Initialization.Random(seed), embedding_dimensions=dims,
+                               num_iters=(0, 0, 0), seed=1)
+        y, _ = pm.fit_transform(x, cfg)
+        assert np.array_equal(y.view(np.uint32), O.random_init(500, dims, seed).view(np.uint32))
 
 
 def _reference_test_data():

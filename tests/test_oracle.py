@@ -350,3 +350,78 @@ def test_optimize_loop_matches_stepwise_oracle():
     assert np.array_equal(snaps[0], states[2]) and np.array_equal(snaps[1], states[5])
     assert not snaps[2].any() and not snaps[4].any()  # duplicate and unreachable stay zero
     assert not snaps[3].any()
+
+
+def test_xoshiro256pp_reference_vector():
+    """The xoshiro256++ reference implementation (Blackman and Vigna's
+    xoshiro256plusplus.c) from the state {1, 2, 3, 4}, seeded through
+    SeedableRng::from_seed with those words little-endian: the first ten
+    outputs.  A wrong rotation, shift, word order or byte order fails."""
+    seed = b"".join(int(w).to_bytes(8, "little") for w in (1, 2, 3, 4))
+    r = O.Rng.from_seed(seed)
+    expected = [41943041, 58720359, 3588806011781223, 3591011842654386, 9228616714210784205,
+                9973669472204895162, 14011001112246962877, 12406186145184390807,
+                15849039046786891736, 10450023813501588000]
+    assert [r.next_u64() for _ in range(10)] == expected
+
+
+def test_ziggurat_tables_closed_forms():
+    """rand_distr's N(0, 1) ziggurat: x[1] = R, x[256] = 0, every layer
+    between x[i] and x[i+1] has area V (x[i] (f[i+1] - f[i]) = V), the base
+    layer R f(R) plus the tail integral is V as well, and the top layer
+    closes on f(0) = 1 (V / x[255] + f(x[255]) = 1), which pins R and V
+    together; f[i] = exp(-x[i]^2 / 2)."""
+    from scipy.stats import norm
+    x, f = O.zig_tables()
+    R, V = 3.6541528853610088, 0.00492867323399
+    assert x[1] == R and x[256] == 0.0 and f[256] == 1.0
+    assert abs(x[0] - V / np.exp(-R * R / 2)) <= 1e-15 * x[0]
+    areas = np.array([x[i] * (f[i + 1] - f[i]) for i in range(1, 255)])
+    assert np.abs(areas - V).max() <= 1e-14
+    base = R * np.exp(-R * R / 2) + norm.sf(R) * np.sqrt(2 * np.pi)
+    assert abs(base - V) <= 1e-12
+    assert abs(V / x[255] + f[255] - 1.0) <= 1e-9
+    np.testing.assert_allclose(f, np.exp(-x * x / 2.0), rtol=4.5e-16, atol=0)
+    assert (np.diff(x[1:]) < 0).all()
+
+
+def test_random_init_is_standard_normal():
+    """Random(seed) initial Y (lib.rs:769-778): a large sample follows N(0, 1)
+    (mean, variance, Kolmogorov-Smirnov distance, and the share of draws from
+    the tail beyond R, which only the base layer's tail sampler produces),
+    deterministic per seed, different across seeds.  Stream values against
+    rand_distr are parity unpinned."""
+    from scipy import stats
+    y = O.random_init(500_000, 2, 42)
+    z = y.ravel().astype(np.float64)
+    assert abs(z.mean()) < 5 / np.sqrt(z.size)
+    assert abs(z.var() - 1.0) < 5 * np.sqrt(2.0 / z.size)
+    assert stats.kstest(z, "norm").statistic < 1.63 / np.sqrt(z.size) * 1.5
+    tail = int((np.abs(z) > 3.6541528853610088).sum())
+    expect = 2 * stats.norm.sf(3.6541528853610088) * z.size
+    assert abs(tail - expect) < 5 * np.sqrt(expect)
+    assert np.array_equal(y, O.random_init(500_000, 2, 42))
+    assert not np.array_equal(y[:10], O.random_init(10, 2, 43))
+    # row-major fill: the first rows of a longer draw are the draw of fewer rows
+    assert np.array_equal(O.random_init(7, 2, 42), y[:7])
+
+
+def test_row_subset_helpers_equal_full_functions():
+    """knn_rows and the per-point samplers return exactly the listed rows of
+    the full-array oracle functions."""
+    rng = np.random.default_rng(12)
+    n, d = 700, 19
+    x = rng.normal(size=(n, d)).astype(np.float32)
+    rows = np.array([0, 5, 699, 5, 123, 300], np.uint32)
+    idx, dist = O.knn_rowwise(x, 30, nthreads=4)
+    ri, rd = O.knn_rows(x, 30, rows, nthreads=3)
+    assert np.array_equal(ri, idx[rows]) and np.array_equal(rd.view(np.uint32), dist[rows].view(np.uint32))
+    nn = O.sample_neighbors_pair(O.scale_dist(dist[:, :30], O.sigma(dist), idx), idx, 10)
+    mn = O.sample_mn_pair_deterministic(x, 7, 99)
+    fp = O.sample_fp_pair_deterministic(n, nn, 10, 20, 99)
+    mr = O.sample_mn_rows(x, 7, 99, rows).reshape(len(rows), 7, 2)
+    fr = O.sample_fp_rows(n, nn, 10, 20, 99, rows).reshape(len(rows), 20, 2)
+    assert np.array_equal(mr, mn.reshape(n, 7, 2)[rows])
+    assert np.array_equal(fr, fp.reshape(n, 20, 2)[rows])
+    with pytest.raises(ValueError):
+        O.knn_rows(x, 5, np.array([n], np.uint32))
