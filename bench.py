@@ -5,8 +5,10 @@ X is 70,000 x 784 f32 from a 10-component isotropic Gaussian mixture (centres
 ~ N(0, 4I), within-cluster std 1, equal weights), min-max scaled to [0, 1];
 PCA initialisation (X -> 100 principal components, lib.rs:456-484), exact kNN
 with n_neighbors = 10 (60 candidates), 10 mid-near and 20 far pairs per point,
-seed 42, 450 FP32 iterations.  One step = one full fit_transform with X
-already resident in HBM; the embedding is returned to the host.
+seed 42, 450 FP32 iterations.  One step = one full fit_transform with X in
+host memory, as a drop-in caller makes it (the upload is inside the step);
+the embedding is returned to the host.  The same call with X already
+resident in HBM and the first call of the process are reported beside it.
 
 Prints ONE JSON line on stdout (progress goes to stderr).
 """
@@ -93,40 +95,78 @@ def grad_adam_bytes(n, pairs, dims):
     return 8 * pairs + 4 * (n + 1) + 24 * n * dims
 
 
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo", encoding="utf-8") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def randomized_pca_cpu(x, q=100, oversample=10, power_iters=2, seed=0):
+    """Randomized PCA to q components (Halko, Martinsson and Tropp 2011, the
+    method of petal-decomposition's RandomizedPca that lib.rs:464-473 calls):
+    centre, range finder with power iterations, QR, small SVD, projection.
+    numpy's LAPACK/BLAS stands in for the reference's; only its time is used."""
+    rng = np.random.default_rng(seed)
+    xc = x - x.mean(axis=0)
+    q_ = np.linalg.qr(xc @ rng.standard_normal((x.shape[1], q + oversample)).astype(np.float32))[0]
+    for _ in range(power_iters):
+        q_ = np.linalg.qr(xc.T @ q_)[0]
+        q_ = np.linalg.qr(xc @ q_)[0]
+    _, _, vt = np.linalg.svd(q_.T @ xc, full_matrices=False)
+    return xc @ vt[:q].T
+
+
 def cpu_baseline(x, threads):
     """The oracle (C restatement of the reference algorithms) on a bounded sample.
 
-    kNN: the reference's triple-sort algorithm (knn.rs:149-191) on n_s rows of
-    the 100-column data, scaled by (70000 / n_s)^2.  Optimiser: 10 iterations
-    of the chunked gradient + Adam (gradient.rs:53-70, adam.rs:51-59) on the
-    full 70,000-point pair set, scaled to 450.  Sampling: the seeded MN/FP
-    samplers on all 70,000 points.  The randomized PCA is not timed."""
+    kNN: the reference's triple-sort algorithm (knn.rs:149-191: all n(n-1)/2
+    distances, one global sort, the greedy fill) on the first n_s rows of the
+    100-column PCA projection, scaled to 70,000 rows by the algorithm's
+    n^2 log(n^2) cost.  PCA: a randomized PCA to 100 components on all
+    70,000 x 784 rows with numpy's LAPACK/BLAS (the reference calls
+    petal-decomposition on OpenBLAS).  Optimiser: 10 iterations of the chunked
+    gradient + Adam (gradient.rs:53-70, adam.rs:51-59) on the full
+    70,000-point pair set, scaled to 450.  Sampling: the seeded MN/FP samplers
+    on all 70,000 points."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
 
-    x100 = np.ascontiguousarray(x[:, :DPRIME])
-    n_s = 8000
+    t = time.perf_counter()
+    x100 = np.ascontiguousarray(randomized_pca_cpu(x), np.float32)
+    t_pca = time.perf_counter() - t
+    n_s = 16000
     t = time.perf_counter()
     O.knn_triple_sort(x100[:n_s], K_NN + 50, nthreads=threads)
-    t_knn = time.perf_counter() - t
+    t_knn_s = time.perf_counter() - t
+    scale = (N / n_s) ** 2 * np.log(N * (N - 1) / 2) / np.log(n_s * (n_s - 1) / 2)
+    t_knn = t_knn_s * scale
     rng = np.random.default_rng(1)
     n = x.shape[0]
     nn = np.stack([np.repeat(np.arange(n), K_NN), rng.integers(0, n, n * K_NN)], 1).astype(np.uint32)
     t = time.perf_counter()
-    mn = O.sample_mn_pair_deterministic(x100, N_MN, 42)
+    mn = O.sample_mn_pair_deterministic(x100, N_MN, 42)  # all 70,000 points
     fp = O.sample_fp_pair_deterministic(n, nn, K_NN, N_FP, 42)
     t_samp = time.perf_counter() - t
     y0 = rng.normal(size=(n, DIMS)).astype(np.float32) * np.float32(1e-4)
     t = time.perf_counter()
     O.optimize(y0, nn, mn, fp, 1.0, (10, 0, 0), nthreads=threads)
     t_it = (time.perf_counter() - t) / 10
-    est = t_knn * (N / n_s) ** 2 + t_samp + t_it * sum(ITERS)
+    est = t_pca + t_knn + t_samp + t_it * sum(ITERS)
     return {"value": round(est, 3), "unit": "s", "cores": threads, "kind": "port",
-            "sample": (f"oracle on host cores: triple-sort exact kNN on {n_s} x {DPRIME} "
-                       f"({t_knn:.2f} s, scaled by (70000/{n_s})^2), seeded MN/FP sampling on "
-                       f"70000 points ({t_samp:.2f} s), 10 chunked grad+Adam iterations on the "
-                       f"full pair set ({t_it * 1e3:.1f} ms/iter, scaled to 450); randomized "
-                       "PCA not timed")}
+            "cpu_model": cpu_model(),
+            "parts_s": {"pca": round(t_pca, 3), "knn": round(t_knn, 3), "sampling": round(t_samp, 3),
+                        "optimizer": round(t_it * sum(ITERS), 3)},
+            "sample": (f"oracle on {threads} host threads ({cpu_model()}): randomized PCA of all "
+                       f"70000 x 784 rows to 100 ({t_pca:.2f} s, numpy LAPACK/BLAS); triple-sort "
+                       f"exact kNN on the first {n_s} rows of the projection ({t_knn_s:.2f} s, "
+                       f"scaled by the n^2 log n^2 cost to 70000 rows: {t_knn:.1f} s); seeded MN/FP "
+                       f"sampling on 70000 points ({t_samp:.2f} s); 10 chunked grad+Adam "
+                       f"iterations on the full pair set ({t_it * 1e3:.1f} ms/iter, scaled to 450)")}
 
 
 # Learned encoder: a noisy Swiss roll (t ~ U[1.5 pi, 4.5 pi], h ~ U[0, 21],
@@ -368,6 +408,38 @@ def warmstart_run(pm, epochs=2):
     }
 
 
+def grad_run(pm, steps, warmup):
+    """The 1,000,000 x 128 workload's optimiser alone: its Generate pairs
+    (10M NN, 5M MN, 20M FP) computed once, then fit_transform's optimiser
+    loop (450 fused gradient + Adam iterations in one hipGraph, no loss
+    terms) from a fixed initial Y, timed by HIP events on its stream."""
+    x, _ = make_large()
+    n = x.shape[0]
+    log("bench: generating the 1M pair set")
+    nn, mn, fp, xp = pm.generate_pairs(x, config_large(pm))
+    y0 = np.ascontiguousarray(xp[:, :DIMS] * np.float32(0.01))
+    del x, xp
+    for w in range(warmup):
+        pm.optimize(y0, nn, mn, fp, 1.0, ITERS, with_loss=False)
+    us = []
+    for s in range(steps):
+        pm.optimize(y0, nn, mn, fp, 1.0, ITERS, with_loss=False)
+        t = pm.last_timings()
+        us.append(t["optimize_ms"] * 1e3 / sum(ITERS))
+        log(f"bench: step {s + 1}/{steps} {us[-1]:.2f} us per iteration")
+    pairs = len(nn) + len(mn) + len(fp)
+    b_iter = grad_adam_bytes(n, pairs, DIMS)
+    u = float(np.mean(us))
+    gbs = b_iter / (u * 1e-6) / 1e9
+    return {"metric": "grad+Adam GB/s (1M points)", "value": round(gbs, 1), "unit": "GB/s",
+            "n_gpus": 1, "steps": steps, "warmup": warmup, "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "fp32", "data": "synthetic",
+            "config": {"workload": "optimiser of the 1000000x128 Zipf GMM fit: 35M generated pairs, 450 "
+                                   "iterations in one hipGraph", "n": n, "pairs": pairs},
+            "us_per_iter": round(u, 3), "bytes_per_iter": b_iter, "frac_hbm_peak": round(gbs / HBM_PEAK_GBS, 4),
+            "target_blocks": int(t["grad_target_blocks"]), "sliced_ell": int(t["grad_sell"])}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -376,13 +448,15 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--encoder-epochs", type=int, default=ENC["epochs"],
                     help="training epochs of the encoder workload")
-    ap.add_argument("--workload", choices=("70k", "1m", "encoder", "sweep", "copy", "warmstart"), default="70k",
+    ap.add_argument("--workload", choices=("70k", "1m", "grad1m", "encoder", "sweep", "copy", "warmstart"),
+                    default="70k",
                     help="70k: the 70000 x 784 headline run; 1m: the 1,000,000 x 128 large run "
                          "(no CPU baseline); encoder: the learned encoder vs the plain optimiser "
                          "on held-out Swiss-roll points; sweep: kNN candidate precisions on "
                          "200000 x 256; copy: a 1 GiB streaming device copy (HBM reference "
                          "for the traffic counters); warmstart: 1M points from the encoder's "
-                         "embedding vs the PCA initialisation")
+                         "embedding vs the PCA initialisation; grad1m: the 1M fit's optimiser "
+                         "alone on its generated pairs")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU design is "
@@ -393,6 +467,9 @@ def main():
     if args.workload == "encoder":
         ENC["epochs"] = args.encoder_epochs
         print(json.dumps(encoder_run(pm)), flush=True)
+        return
+    if args.workload == "grad1m":
+        print(json.dumps(grad_run(pm, args.steps, args.warmup)), flush=True)
         return
     if args.workload == "warmstart":
         print(json.dumps(warmstart_run(pm)), flush=True)
@@ -421,19 +498,34 @@ def main():
         log("bench: generating 70000 x 784 Gaussian mixture")
         x, _ = make_data()
         cfg = config(pm)
+    # The first call of the process pays the one-off costs (code object load,
+    # graph capture and instantiation, the caching allocator's first blocks).
+    t0 = time.perf_counter()
+    pm.fit_transform(x, cfg)
+    first_call = time.perf_counter() - t0
+    log(f"bench: first call {first_call * 1e3:.1f} ms")
     xd = pm.DeviceArray(x)
     for w in range(args.warmup):
         log(f"bench: warmup {w + 1}/{args.warmup}")
+        pm.fit_transform(x, cfg)
         pm.fit_transform(xd, cfg)
-    times, tms = [], []
+    # value: fit_transform as a drop-in caller makes it, X in host memory
+    # (its upload to HBM inside the timed call); then the same call with X
+    # already resident in HBM
+    times, tms, times_res, tms_res = [], [], [], []
     for s in range(args.steps):
         t0 = time.perf_counter()
-        y, _ = pm.fit_transform(xd, cfg)
+        y, _ = pm.fit_transform(x, cfg)
         times.append(time.perf_counter() - t0)
         tms.append(pm.last_timings())
-        log(f"bench: step {s + 1}/{args.steps} {times[-1] * 1e3:.1f} ms")
-    assert np.isfinite(y).all()
+        t0 = time.perf_counter()
+        y_res, _ = pm.fit_transform(xd, cfg)
+        times_res.append(time.perf_counter() - t0)
+        tms_res.append(pm.last_timings())
+        log(f"bench: step {s + 1}/{args.steps} {times[-1] * 1e3:.1f} ms (X in HBM: {times_res[-1] * 1e3:.1f} ms)")
+    assert np.isfinite(y).all() and np.array_equal(y, y_res)
     sec = float(np.mean(times))
+    sec_res = float(np.mean(times_res))
     knn_ms = float(np.mean([t["knn_kernel_ms"] for t in tms]))
     flops = 2.0 * n * n * DPRIME
     achieved = flops / (knn_ms * 1e-3) / 1e12
@@ -480,6 +572,8 @@ def main():
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(sec * 1e3, 2),
         "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "fp32",
         "data": "synthetic",
+        "value_x_in_hbm_s": round(sec_res, 4), "upload_ms": round(float(np.mean([t["upload_ms"] for t in tms])), 3),
+        "first_call_s": round(first_call, 4),
         "config": {"workload": ("fit_transform 1000000x128 GMM(100, Zipf 1.1 sizes), PCA init -> "
                                 "100 dims, exact kNN from split-bf16 candidates k=10 (60 candidates), "
                                 "5 MN + 20 FP per point, 450 iterations" if large else

@@ -14,7 +14,7 @@ sizes in seconds, so the checks are split by stage:
   GPU's preprocessed data, bit for bit (all points at 70,000, a sample of
   points at 1,000,000);
 * gradient: one evaluation at full size through the optimiser's own kernel
-  (sliced ELL at 70,000, target-blocked at 1,000,000) against the oracle at
+  (LDS-staged sliced ELL at 70,000, target-blocked at 1,000,000) against the oracle at
   1e-5 relative, the loss against FP64;
 * optimiser: 10 iterations from a Value initialisation on the full generated
   pair set against the oracle, with a bar set by how far other valid
