@@ -722,6 +722,17 @@ void fit_impl(const float *x_host, const float *x_dev_in, uint64_t n, uint64_t d
   }
   if (cfg->has_snapshots && cfg->num_snapshots && !cfg->snapshots)
     fail(PACMAP_ERR_INVALID_ARGUMENT, "snapshots is NULL");
+  if (x_host) require_host(x_host, "x");
+  if (x_dev_in) require_device(x_dev_in, "x_dev");
+  if (y_out) require_host(y_out, "y_out");
+  if (snaps_out) require_host(snaps_out, "snapshots_out");
+  if (loss_out) require_host(loss_out, "loss_out");
+  if (init == PACMAP_INIT_VALUE) require_host(cfg->init_value, "init_value");
+  if (pk != PACMAP_PAIRS_GENERATE) {
+    require_host(cfg->pair_neighbors, "pair_neighbors");
+    require_host(cfg->pair_mn, "pair_mn");
+    require_host(cfg->pair_fp, "pair_fp");
+  }
   if (po) {
     if (pk != PACMAP_PAIRS_GENERATE) fail(PACMAP_ERR_INVALID_ARGUMENT, "pair generation needs PairConfiguration::Generate");
     po->counts[0] = n * std::min<uint64_t>(k, n - 1);
@@ -730,6 +741,10 @@ void fit_impl(const float *x_host, const float *x_dev_in, uint64_t n, uint64_t d
     po->counts[3] = pca_path ? std::min<uint64_t>(100, n) : d;
     if (po->query) return;
     if (!po->nn || (n_mn && !po->mn) || (n_fp && !po->fp)) fail(PACMAP_ERR_INVALID_ARGUMENT, "NULL pair output");
+    require_host(po->nn, "pair_nn_out");
+    require_host(po->mn, "pair_mn_out");
+    require_host(po->fp, "pair_fp_out");
+    require_host(po->x_pre, "x_pre_out");
   }
 
   Stream stream;
@@ -917,6 +932,9 @@ int pacmap_knn(const float *x, uint64_t n, uint64_t d, uint64_t k, int32_t preci
     const uint64_t kk = std::min<uint64_t>(k, n - 1);  // knn.rs:146
     if (kk && (!idx || !dist)) fail(PACMAP_ERR_INVALID_ARGUMENT, "output is NULL");
     if (kk == 0) return;
+    require_host(x, "x");
+    require_host(idx, "idx");
+    require_host(dist, "dist");
     Stream stream;
     const hipStream_t s = stream.get();
     DevBuf<float> dx(n * d);
@@ -949,6 +967,10 @@ int pacmap_grad(const float *y, uint64_t n, uint64_t dims, const uint32_t *pair_
     if (!y || !grad_out) fail(PACMAP_ERR_INVALID_ARGUMENT, "NULL array");
     if ((n_nn && !pair_nn) || (n_mn && !pair_mn) || (n_fp && !pair_fp))
       fail(PACMAP_ERR_INVALID_ARGUMENT, "NULL pair array");
+    for (const void *hp : {static_cast<const void *>(y), static_cast<const void *>(pair_nn),
+                           static_cast<const void *>(pair_mn), static_cast<const void *>(pair_fp),
+                           static_cast<const void *>(grad_out)})
+      require_host(hp, "an array argument");
     PairList lists[3] = {{pair_nn, n_nn}, {pair_mn, n_mn}, {pair_fp, n_fp}};
     Stream stream;
     const hipStream_t s = stream.get();
@@ -1119,6 +1141,10 @@ int pacmap_optimize(float *y, uint64_t n, uint64_t dims, const uint32_t *pair_nn
     if ((n_nn && !pair_nn) || (n_mn && !pair_mn) || (n_fp && !pair_fp))
       fail(PACMAP_ERR_INVALID_ARGUMENT, "NULL pair array");
     if (num_snapshots && !snapshots) fail(PACMAP_ERR_INVALID_ARGUMENT, "snapshots is NULL");
+    for (const void *hp : {static_cast<const void *>(y), static_cast<const void *>(pair_nn),
+                           static_cast<const void *>(pair_mn), static_cast<const void *>(pair_fp),
+                           static_cast<const void *>(snapshots_out), static_cast<const void *>(loss_out)})
+      require_host(hp, "an array argument");
     const double t0 = now_ms();
     PairList lists[3] = {{pair_nn, n_nn}, {pair_mn, n_mn}, {pair_fp, n_fp}};
     Stream stream;

@@ -283,6 +283,28 @@ int guarded(pacmap_status *st, F &&body) {
   }
 }
 
+// Pointer kinds at the C boundary: a device allocation passed where host
+// memory is expected (or host memory where device memory is) is reported as
+// an invalid argument instead of faulting in a copy or a kernel.  Memory the
+// runtime does not know (plain pageable host memory) counts as host memory;
+// the error the query leaves for it is cleared, so it cannot surface in a
+// later launch check.
+inline bool device_pointer(const void *p) {
+  if (!p) return false;
+  hipPointerAttribute_t a{};
+  if (hipPointerGetAttributes(&a, p) != hipSuccess) {
+    (void)hipGetLastError();
+    return false;
+  }
+  return a.type == hipMemoryTypeDevice;
+}
+inline void require_host(const void *p, const char *what) {
+  if (device_pointer(p)) fail(PACMAP_ERR_INVALID_ARGUMENT, std::string(what) + " is device memory; host memory expected");
+}
+inline void require_device(const void *p, const char *what) {
+  if (p && !device_pointer(p)) fail(PACMAP_ERR_INVALID_ARGUMENT, std::string(what) + " is not device memory");
+}
+
 template <class T>
 void upload(DevBuf<T> &dst, const T *src, size_t count, hipStream_t s) {
   if (count) PM_HIP(hipMemcpyAsync(dst.get(), src, count * sizeof(T), hipMemcpyHostToDevice, s));

@@ -226,6 +226,10 @@ int pacmap_mlp_fit(const float *x, uint64_t n, uint64_t d, uint64_t dims, const 
     if (!cfg) fail(PACMAP_ERR_INVALID_ARGUMENT, "config is NULL");
     const MlpShape sh(d, cfg->hidden, dims);
     if (!x || !params) fail(PACMAP_ERR_INVALID_ARGUMENT, "NULL argument");
+    for (const void *hp : {static_cast<const void *>(x), static_cast<const void *>(params),
+                           static_cast<const void *>(pair_nn), static_cast<const void *>(pair_mn),
+                           static_cast<const void *>(pair_fp), static_cast<const void *>(epoch_loss)})
+      require_host(hp, "an array argument");
     if (cfg->batch_pairs == 0 || cfg->batch_pairs > 65536)
       fail(PACMAP_ERR_UNSUPPORTED, "batch_pairs must be in 1..65536");
     const uint64_t P = n_nn + n_mn + n_fp;
@@ -288,6 +292,10 @@ static int mlp_transform_impl(const float *x_host, const float *x_dev, uint64_t 
     const MlpShape sh(d, hidden, dims);
     if ((!x_host && !x_dev) || !params || !y_out) fail(PACMAP_ERR_INVALID_ARGUMENT, "NULL argument");
     if (n == 0) return;
+    if (x_host) require_host(x_host, "x");
+    if (x_dev) require_device(x_dev, "x_dev");
+    require_host(params, "params");
+    require_host(y_out, "y_out");
     Stream stream;
     const hipStream_t s = stream.get();
     DevBuf<float> dxb;
@@ -352,6 +360,9 @@ int pacmap_mlp_transform_noalloc(const float *x_dev, uint64_t n, uint64_t d, uin
       fail(PACMAP_ERR_UNSUPPORTED, "input width too large for the fused transform kernel");
     if (n == 0) return;
     if (!x_dev || !params_dev || !y_dev) fail(PACMAP_ERR_INVALID_ARGUMENT, "NULL argument");
+    require_device(x_dev, "x_dev");
+    require_device(params_dev, "params_dev");
+    require_device(y_dev, "y_dev");
     uint16_t *w1b = nullptr, *w2b = nullptr;
     PM_HIP(mlp_noalloc_scratch(&w1b, &w2b));
     const hipStream_t s = hipStreamPerThread;

@@ -764,3 +764,25 @@ def test_pca_initialisation_rank_deficient():
     err = np.abs(y - exp).max() / np.abs(exp).max()
     print(f"rank-deficient PCA init: max error {err:.2e}", flush=True)
     assert err < 1e-3
+
+
+def test_pointer_kinds_checked_at_the_boundary():
+    """A device allocation where host memory is expected, or host memory where
+    device memory is, returns InvalidArgument instead of faulting."""
+    import ctypes as C
+    x = np.random.default_rng(74).normal(size=(300, 8)).astype(np.float32)
+    xd = pm.DeviceArray(x)
+    c = pm.Configuration(num_iters=(1, 1, 1))._to_c([])
+    y = np.zeros((300, 2), np.float32)
+    st = pm.Status()
+    rc = pm.lib().pacmap_fit_transform(xd.ptr, 300, 8, C.byref(c), y.ctypes.data, None, None, C.byref(st))
+    assert rc == pm.InvalidArgument.code and b"device memory" in st.message
+    rc = pm.lib().pacmap_fit_transform_dev(x.ctypes.data, 300, 8, C.byref(c), y.ctypes.data, None, None,
+                                           C.byref(st))
+    assert rc == pm.InvalidArgument.code and b"not device memory" in st.message
+    yd = pm.DeviceArray(y)
+    rc = pm.lib().pacmap_fit_transform_dev(xd.ptr, 300, 8, C.byref(c), yd.ptr, None, None, C.byref(st))
+    assert rc == pm.InvalidArgument.code
+    # the rejected calls leave no error behind: a valid call still succeeds
+    out, _ = pm.fit_transform(xd, pm.Configuration(num_iters=(1, 1, 1)))
+    assert np.isfinite(out).all()
