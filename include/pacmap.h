@@ -250,7 +250,9 @@ int pacmap_dev_download(void *dst_host, const void *src_dev, uint64_t bytes, pac
 /* Device blocks the library allocated are cached per calling thread and
  * reused by its later calls (no hipMalloc / hipFree per call), and so is the
  * thread's last optimiser graph (replayed when a call's launches are
- * identical).  This frees the calling thread's caches. */
+ * identical).  Uploads of 8 MiB or more go through two 32 MiB pinned host
+ * buffers per calling thread, also kept between calls.  This frees the
+ * calling thread's caches. */
 void pacmap_release_cached_memory(void);
 /* HBM reference: a plain streaming device-to-device copy kernel run `repeats`
  * times; ms_out gets the average time per copy (HIP events).  Its rocprof

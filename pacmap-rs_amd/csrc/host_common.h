@@ -305,9 +305,14 @@ inline void require_device(const void *p, const char *what) {
   if (p && !device_pointer(p)) fail(PACMAP_ERR_INVALID_ARGUMENT, std::string(what) + " is not device memory");
 }
 
+// Host-to-device copy on stream s: large copies go through the calling
+// thread's pinned staging buffers (pool.cpp); src may be reused when this
+// returns.  staging_release frees the thread's buffers.
+void upload_bytes(void *dst, const void *src, size_t bytes, hipStream_t s);
+void staging_release();
 template <class T>
 void upload(DevBuf<T> &dst, const T *src, size_t count, hipStream_t s) {
-  if (count) PM_HIP(hipMemcpyAsync(dst.get(), src, count * sizeof(T), hipMemcpyHostToDevice, s));
+  if (count) upload_bytes(dst.get(), src, count * sizeof(T), s);
 }
 template <class T>
 void download(T *dst, const T *src, size_t count, hipStream_t s) {

@@ -1,6 +1,11 @@
-// pool.cpp -- per-thread cache of device memory blocks (see host_common.h).
+// pool.cpp -- per-thread cache of device memory blocks and the staged
+// host-to-device upload (see host_common.h).
+#include <algorithm>
+#include <condition_variable>
 #include <iterator>
 #include <map>
+#include <mutex>
+#include <thread>
 
 #include "host_common.h"
 
@@ -98,9 +103,136 @@ void dev_trim() {
   fb.clear();
 }
 
+// ---------------------------------------------------------------------------
+// Staged uploads.  A copy from pageable host memory runs at about half the
+// DMA rate of pinned memory (the runtime stages it through one thread), so
+// large uploads go through two pinned buffers: host threads copy the next
+// chunk into one buffer while the DMA engine moves the previous one from the
+// other.  Each buffer is rewritten only after the event recorded behind its
+// last DMA has completed.
+// ---------------------------------------------------------------------------
+namespace {
+
+// Persistent host threads that split a memcpy between them (created on first
+// use; one copy at a time, callers are serialised).
+class CopyThreads {
+ public:
+  static CopyThreads &get() {
+    static CopyThreads c;
+    return c;
+  }
+  void copy(void *dst, const void *src, size_t len) {
+    std::lock_guard<std::mutex> serial(call_);
+    std::unique_lock<std::mutex> lk(m_);
+    dst_ = static_cast<char *>(dst);
+    src_ = static_cast<const char *>(src);
+    len_ = len;
+    pending_ = static_cast<int>(th_.size());
+    ++gen_;
+    cv_.notify_all();
+    lk.unlock();
+    part(0, static_cast<int>(th_.size()) + 1);  // the caller copies the first share
+    lk.lock();
+    done_.wait(lk, [&] { return pending_ == 0; });
+  }
+  ~CopyThreads() {
+    {
+      std::lock_guard<std::mutex> lk(m_);
+      stop_ = true;
+    }
+    cv_.notify_all();
+    for (auto &t : th_) t.join();
+  }
+
+ private:
+  CopyThreads() {
+    const int n = std::max(1, std::min(8, host_threads()) - 1);
+    for (int i = 0; i < n; ++i) th_.emplace_back([this, i] { run(i + 1); });
+  }
+  void part(int id, int parts) {
+    const size_t b = len_ * id / parts, e = len_ * (id + 1) / parts;
+    if (e > b) std::memcpy(dst_ + b, src_ + b, e - b);
+  }
+  void run(int id) {
+    uint64_t seen = 0;
+    std::unique_lock<std::mutex> lk(m_);
+    for (;;) {
+      cv_.wait(lk, [&] { return stop_ || gen_ != seen; });
+      if (stop_) return;
+      seen = gen_;
+      const int parts = static_cast<int>(th_.size()) + 1;
+      lk.unlock();
+      part(id, parts);
+      lk.lock();
+      if (--pending_ == 0) done_.notify_one();
+    }
+  }
+  std::vector<std::thread> th_;
+  std::mutex call_, m_;
+  std::condition_variable cv_, done_;
+  char *dst_ = nullptr;
+  const char *src_ = nullptr;
+  size_t len_ = 0;
+  int pending_ = 0;
+  uint64_t gen_ = 0;
+  bool stop_ = false;
+};
+
+constexpr size_t kStageChunk = size_t{32} << 20;  // bytes per pinned buffer
+constexpr size_t kStageMin = size_t{8} << 20;     // smaller uploads go directly
+
+struct Staging {
+  void *buf[2] = {nullptr, nullptr};
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  void release() {
+    for (int b = 0; b < 2; ++b) {
+      if (ev[b]) {
+        (void)hipEventSynchronize(ev[b]);
+        (void)hipEventDestroy(ev[b]);
+      }
+      if (buf[b]) (void)hipHostFree(buf[b]);
+      buf[b] = nullptr;
+      ev[b] = nullptr;
+    }
+  }
+};
+Staging &staging() {
+  thread_local Staging st;  // kept for the thread's calls (not freed at thread exit: the runtime may be gone)
+  return st;
+}
+
+}  // namespace
+
+void upload_bytes(void *dst, const void *src, size_t bytes, hipStream_t s) {
+  if (bytes == 0) return;
+  if (bytes < kStageMin) {
+    PM_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s));
+    return;
+  }
+  Staging &st = staging();
+  for (int b = 0; b < 2; ++b) {
+    if (!st.buf[b]) PM_HIP(hipHostMalloc(&st.buf[b], kStageChunk, 0));
+    if (!st.ev[b]) PM_HIP(hipEventCreateWithFlags(&st.ev[b], hipEventDisableTiming));
+  }
+  const char *sp = static_cast<const char *>(src);
+  char *dp = static_cast<char *>(dst);
+  size_t k = 0;
+  for (size_t off = 0; off < bytes; off += kStageChunk, ++k) {
+    const int b = static_cast<int>(k & 1);
+    const size_t len = std::min(kStageChunk, bytes - off);
+    PM_HIP(hipEventSynchronize(st.ev[b]));  // the last DMA out of this buffer is done
+    CopyThreads::get().copy(st.buf[b], sp + off, len);
+    PM_HIP(hipMemcpyAsync(dp + off, st.buf[b], len, hipMemcpyHostToDevice, s));
+    PM_HIP(hipEventRecord(st.ev[b], s));
+  }
+}
+
+void staging_release() { staging().release(); }
+
 }  // namespace pm
 
 extern "C" void pacmap_release_cached_memory(void) {
   pm::graph_cache_reset();
   pm::dev_trim();
+  pm::staging_release();
 }
