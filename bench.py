@@ -448,7 +448,9 @@ def grad_run(pm, steps, warmup):
             "config": {"workload": "optimiser of the 1000000x128 Zipf GMM fit: 35M generated pairs, 450 "
                                    "iterations in one hipGraph", "n": n, "pairs": pairs},
             "us_per_iter": round(u, 3), "bytes_per_iter": b_iter, "frac_hbm_peak": round(gbs / HBM_PEAK_GBS, 4),
-            "target_blocks": int(t["grad_target_blocks"]), "sliced_ell": int(t["grad_sell"])}
+            "target_blocks": int(t["grad_target_blocks"]), "sliced_ell": int(t["grad_sell"]),
+            "paged": int(t["grad_paged"]), "paged_words": int(t["grad_page_words"]),
+            "pages_per_range": int(t["grad_page_pages"])}
 
 
 def main():
@@ -557,7 +559,11 @@ def main():
     us_iter = opt_ms * 1e3 / it_total
     tb = int(tms[-1]["grad_target_blocks"])
     sell = int(tms[-1]["grad_sell"])
-    if sell:
+    paged = int(tms[-1]["grad_paged"])
+    if paged:
+        gname, gkey = ("grad_page_kernel + grad_tb_combine_kernel (paged target-blocked gradient: "
+                       "Y pages staged through LDS, then Adam)"), "grad_page"
+    elif sell:
         gname, gkey = ("grad_sell_kernel + grad_combine_elem_kernel (LDS-staged sliced-ELL gradient, "
                        "then Adam)"), "grad_sell"
     elif tb:
@@ -601,7 +607,9 @@ def main():
                       "frac_hbm_peak": round(b_iter * it_total / (opt_ms * 1e-3) / 1e9 /
                                              HBM_PEAK_GBS, 4),
                       "target_blocks": tb, "sliced_ell": sell,
-                      "sliced_ell_words": int(tms[-1]["grad_sell_entries"])},
+                      "sliced_ell_words": int(tms[-1]["grad_sell_entries"]), "paged": paged,
+                      "paged_words": int(tms[-1]["grad_page_words"]),
+                      "pages_per_range": int(tms[-1]["grad_page_pages"])},
         "breakdown_ms": breakdown,
         "knn": {"path": ["direct", "split-bf16 MFMA + certified rerank",
                          "bf16 MFMA + certified rerank", "fp8 MFMA, uncertified"][path],

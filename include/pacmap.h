@@ -292,6 +292,9 @@ typedef struct {
   uint64_t graph_reused;       /* 1: the previous call's optimiser graph was replayed */
   uint64_t knn_listed;         /* candidates listed, summed over the reranked rows */
   uint64_t knn_reranked;       /* of those, candidates given an exact distance */
+  uint64_t grad_paged;         /* 1: paged target-blocked gradient (Y pages staged through LDS) */
+  uint64_t grad_page_words;    /* its entry words including padding */
+  uint64_t grad_page_pages;    /* its pages per target range */
 } pacmap_timings;
 void pacmap_last_timings(pacmap_timings *out);
 /* Seam: the rows (ascending) that the last kNN on this thread (pacmap_knn or

@@ -342,21 +342,44 @@ bool use_sell(uint64_t n, uint64_t dims, SellPlan &plan) {
   return n >= 16384;
 }
 
+// The paged form of the target-blocked gradient (Y pages staged through LDS),
+// used in place of it when its plan fits.  PACMAP_GRAD_PAGED=0 turns it off
+// and =1 keeps it where its layout has many long runs (see build_grad_layout);
+// PACMAP_GRAD_PAGE_ROWS / PACMAP_GRAD_PAGE_POINTS cap the rows per chunk and
+// the points per page (tuning and tests).
+bool use_paged(uint64_t n, uint64_t dims, PagePlan &plan) {
+  if (n >= kMaxPoints) return false;
+  if (const char *e = std::getenv("PACMAP_GRAD_PAGED"))
+    if (e[0] == '0') return false;
+  uint32_t max_rows = 0, max_pp = 0;
+  if (const char *e = std::getenv("PACMAP_GRAD_PAGE_ROWS")) max_rows = static_cast<uint32_t>(std::strtoul(e, nullptr, 10));
+  if (const char *e = std::getenv("PACMAP_GRAD_PAGE_POINTS")) max_pp = static_cast<uint32_t>(std::strtoul(e, nullptr, 10));
+  if (!page_plan(static_cast<uint32_t>(n), static_cast<uint32_t>(dims), max_rows, max_pp, plan)) return false;
+  // every chunk copies all of Y through LDS once per iteration: past about
+  // 6 GB of copies (n around 1.5 million at 2 dims) the gather form is kept
+  return static_cast<uint64_t>(plan.chunks) * n * dims * 4 <= (6ull << 30);
+}
+
 // The gradient kernel for a CSR and its per-fit layout: the target-blocked
-// kernel when Y outgrows an XCD's L2, its LDS-staged (sliced ELL) form when a
-// target range fits in LDS, else the one-pass kernel (see kernels.hip).  The
-// optimiser and the gradient seam both build it, so they run the same kernel.
+// kernel when Y outgrows an XCD's L2 (in its paged form when the plan fits),
+// its LDS-staged (sliced ELL) form when a whole target range fits in LDS,
+// else the one-pass kernel (see kernels.hip, grad_paged.hip).  The optimiser
+// and the gradient seam both build it, so they run the same kernel.
 struct GradLayout {
-  bool tb = false, sell = false;
+  bool tb = false, sell = false, paged = false;
   SellPlan sp;
-  DevBuf<uint32_t> tb_ent, sell_perm, sell_off, sell_ent;
+  PagePlan pg;
+  DevBuf<uint32_t> tb_ent, sell_perm, sell_off, sell_ent, page_off, page_ent;
   DevBuf<uint16_t> tb_off;
   DevBuf<float> tb_part;
   uint32_t loss_blocks(uint32_t n) const {
+    if (paged) return page_grad_blocks(pg);
     return sell ? sell_grad_blocks(sp) : (tb ? grad_tb_blocks(n) : grad_blocks(n));
   }
-  const uint32_t *entries(const DevCsr &csr) const { return tb && !sell ? tb_ent.get() : csr.ent.get(); }
+  const uint32_t *entries(const DevCsr &csr) const { return tb && !sell && !paged ? tb_ent.get() : csr.ent.get(); }
   hipError_t launch(const GradArgs &a, uint32_t dims, bool adam, hipStream_t s) const {
+    if (paged)
+      return launch_grad_adam_paged(a, pg, page_off.get(), page_ent.get(), tb_part.get(), dims, adam, s);
     if (sell)
       return launch_grad_adam_sell(a, sp, sell_perm.get(), sell_off.get(), sell_ent.get(), tb_part.get(), dims,
                                    adam, s);
@@ -370,6 +393,7 @@ void build_grad_layout(const DevCsr &csr, uint64_t n, uint64_t dims, bool want, 
   SellPlan &sp = L.sp;
   bool tb = want && use_target_blocks(n, dims) && csr.nnz > 0;
   bool sell = want && !tb && csr.nnz > 0 && use_sell(n, dims, sp);
+  bool paged = tb && use_paged(n, dims, L.pg);
   if (tb || sell) {
     L.tb_ent.alloc(csr.nnz, kSlotTbEnt);
     L.tb_off.alloc(static_cast<size_t>(kTargetBlocks) * n, kSlotTbOff);
@@ -381,9 +405,45 @@ void build_grad_layout(const DevCsr &csr, uint64_t n, uint64_t dims, bool want, 
     download(&hb, bad.get(), 1, s);
     PM_HIP(hipStreamSynchronize(s));
     if (hb) {
-      tb = sell = false;  // a row too long for the u16 range offsets: one-pass kernel
+      tb = sell = paged = false;  // a row too long for the u16 range offsets: one-pass kernel
     } else {
       L.tb_part.alloc(static_cast<size_t>(kTargetBlocks) * nd, kSlotTbPart);
+    }
+  }
+  if (paged) {
+    PagePlan &pg = L.pg;
+    const uint32_t n32 = static_cast<uint32_t>(n);
+    const uint64_t runs = page_runs(pg);
+    L.page_off.alloc(runs + 1, kSlotPageOff);
+    DevBuf<uint32_t> rsz(runs), cur(runs), scr((runs + 1023) / 1024 + 1);
+    PM_HIP(launch_page_count(csr.row_ptr.get(), L.tb_ent.get(), L.tb_off.get(), n32, pg, rsz.get(), s));
+    // the scan is 32-bit and the gradient pass addresses words with 32-bit
+    // byte offsets: the run sizes are summed in 64 bits first, and a layout
+    // past kPageMaxWords keeps the tb kernel
+    std::vector<uint32_t> hsz(runs);
+    download(hsz.data(), rsz.get(), runs, s);
+    PM_HIP(hipStreamSynchronize(s));
+    uint64_t total = 0, long_runs = 0;
+    for (uint32_t v : hsz) {
+      total += v;
+      long_runs += v > 64u * kPageRunSteps ? 1u : 0u;
+    }
+    // a run longer than kPageRunSteps steps stalls its workgroup for a memory
+    // round trip (data whose neighbours sit close in index, hub rows): when
+    // more than 1% of the runs are, the gather form is faster
+    const char *force = std::getenv("PACMAP_GRAD_PAGED");
+    const bool forced = force && force[0] == '1';
+    if (total < kPageMaxWords && (forced || long_runs * 100 <= runs)) {
+      PM_HIP(launch_csr_scan(rsz.get(), static_cast<uint32_t>(runs), L.page_off.get(), cur.get(), scr.get(), s));
+      L.page_ent.alloc(total ? total : 1, kSlotPageEnt);
+      pg.words = static_cast<uint32_t>(total ? total : 1);
+      PM_HIP(launch_page_fill(csr.row_ptr.get(), L.tb_ent.get(), L.tb_off.get(), n32, pg, L.page_off.get(),
+                              L.page_ent.get(), s));
+      g_tm.grad_page_words = total;
+      g_tm.grad_page_pages = pg.pages;
+    } else {
+      paged = false;
+      L.page_off.release();
     }
   }
   if (sell) {
@@ -406,8 +466,10 @@ void build_grad_layout(const DevCsr &csr, uint64_t n, uint64_t dims, bool want, 
   }
   L.tb = tb;
   L.sell = sell;
+  L.paged = paged;
   g_tm.grad_target_blocks = tb ? kTargetBlocks : 0;
   g_tm.grad_sell = sell ? 1 : 0;
+  g_tm.grad_paged = paged ? 1 : 0;
 }
 
 // The initial Y is y (host), or y_dev (device, n x dims) when that is not

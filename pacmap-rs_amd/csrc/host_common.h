@@ -72,6 +72,8 @@ enum Slot : int {
   kSlotSellPerm,
   kSlotSellOff,
   kSlotSellEnt,
+  kSlotPageOff,
+  kSlotPageEnt,
   kSlotCount
 };
 void *dev_acquire_slot(int slot, size_t cls);

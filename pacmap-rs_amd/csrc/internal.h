@@ -137,6 +137,39 @@ hipError_t launch_tb_partition(const uint32_t *row_ptr, const uint32_t *ent, uin
                                uint16_t *off, uint32_t *bad, hipStream_t s);
 hipError_t launch_grad_adam_tb(const GradArgs &a, const uint16_t *off, float *part, uint32_t dims, bool adam,
                                hipStream_t s);
+// points per target range of the tb layout (a multiple of 4)
+uint32_t tb_span(uint32_t n);
+// pass 2 alone: part's kTargetBlocks range partials summed in range order,
+// then Adam (or the gradient written to grad_out)
+hipError_t launch_tb_combine(const GradArgs &a, const float *part, uint32_t dims, bool adam, hipStream_t s);
+// Paged form of the target-blocked gradient (grad_paged.hip), for
+// embeddings whose target range outgrows LDS: a workgroup owns a chunk of
+// rows (their Y and gradient sums in LDS) and walks its target range in
+// pages of Y staged through LDS.  Built once per fit from the tb_partition
+// output: count (rsz: page_runs() run sizes), exclusive scan of rsz into
+// roff (+1 total), fill (went: roff[last] words).  Each iteration runs
+// grad_page (loss_part: page_grad_blocks() floats) and the tb combine.
+struct PagePlan {
+  uint32_t span = 0;       // points per target range (tb_span)
+  uint32_t rc = 0;         // rows per chunk (a multiple of 4)
+  uint32_t chunks = 0;
+  uint32_t pp = 0;         // points per page (a multiple of 4)
+  uint32_t pages = 0;      // pages per range
+  uint32_t lds_bytes = 0;  // dynamic LDS per workgroup of the gradient pass
+  uint32_t words = 0;      // entry words (set once the layout is built)
+};
+// max_rows / max_page_points: 0 = no cap beyond the LDS budget
+bool page_plan(uint32_t n, uint32_t dims, uint32_t max_rows, uint32_t max_page_points, PagePlan &p);
+uint64_t page_runs(const PagePlan &p);          // (range, chunk, page, wave) runs
+uint32_t page_grad_blocks(const PagePlan &p);   // gradient pass workgroups (loss_part entries)
+constexpr uint32_t kPageMaxWords = 1u << 29;    // entry words addressable by the gradient pass
+constexpr uint32_t kPageRunSteps = 6;           // steps of a wave's run per page loaded ahead
+hipError_t launch_page_count(const uint32_t *row_ptr, const uint32_t *tb_ent, const uint16_t *off, uint32_t n,
+                             const PagePlan &p, uint32_t *rsz, hipStream_t s);
+hipError_t launch_page_fill(const uint32_t *row_ptr, const uint32_t *tb_ent, const uint16_t *off, uint32_t n,
+                            const PagePlan &p, const uint32_t *roff, uint32_t *went, hipStream_t s);
+hipError_t launch_grad_adam_paged(const GradArgs &a, const PagePlan &p, const uint32_t *roff, const uint32_t *went,
+                                  float *part, uint32_t dims, bool adam, hipStream_t s);
 // LDS-staged variant of the target-blocked gradient (sliced ELL, see
 // kernels.hip), used while a range of Y plus a row chunk fits in LDS.  Built
 // once per fit from the tb_partition output: sort (perm: kTargetBlocks x n,

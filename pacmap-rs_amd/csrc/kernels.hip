@@ -3,6 +3,7 @@
 // Built with -ffp-contract=off: the reference (Rust) never fuses a multiply
 // and an add, and the per-pair arithmetic here must round exactly like it.
 #include "internal.h"
+#include "grad_device.h"
 
 #include <cstdint>
 #include <type_traits>
@@ -572,78 +573,6 @@ hipError_t launch_scale_nn(const float *dist, const uint32_t *idx, const float *
 // and accumulates s * w1(D) * (y_row - y_other) exactly as the reference
 // computes each per-pair term; the 16 partials are combined by a fixed xor
 // butterfly, so the result is bit-identical from run to run (no atomics).
-// 1/x from v_rcp_f32 refined by one Newton step (x >= 1 here): within an ulp
-// of the correctly rounded quotient, at a quarter of the instructions of an
-// IEEE division.
-__device__ __forceinline__ float recip_nr(float x) {
-  const float r0 = __builtin_amdgcn_rcpf(x);
-  const float e = fmaf(-x, r0, 1.0f);
-  return fmaf(r0, e, r0);
-}
-
-// Per-type constants of the pair terms (gradient.rs:127-147), one set per
-// iteration: k = w * numerator constant with the far-pair sign folded in (so
-// every entry adds k rc^2 u), wt = w for the loss.  Kept as separate scalars:
-// selecting between members of an aggregate can turn into an indexed memory
-// load per entry.
-struct PairConsts {
-  float k_nn, k_mn, k_fp;
-  float w_nn, w_mn, w_fp;
-};
-__device__ __forceinline__ PairConsts pair_consts(const float (&w)[3]) {
-  PairConsts q;
-  q.k_nn = w[0] * 20.0f;
-  q.k_mn = w[1] * 20000.0f;
-  q.k_fp = -(w[2] * 2.0f);
-  q.w_nn = w[0];
-  q.w_mn = w[1];
-  q.w_fp = w[2];
-  return q;
-}
-
-// One CSR entry's contribution (gradient.rs:118-148) with the pair type's
-// constants already selected: w = entry word, c = 10 / 10000 / 1 and
-// k = its numerator constant (NN / MN / FP), yr = the row's point, yo = the
-// other point.  Every gradient kernel below ends here, so all of them round
-// each term identically.  D = 1 + |yr - yo|^2 is accumulated in the
-// reference's order with fused multiply-adds, as is the gradient (one
-// rounding per step instead of the reference's two); the coefficient is
-// k / (c + D)^2 with the reciprocal from recip_nr.
-template <int DIMS, bool LOSS>
-__device__ __forceinline__ void pair_term_ck(uint32_t w, float c, float k, const float (&yr)[DIMS],
-                                             const float *yo, float w_nn, float w_mn, float w_fp,
-                                             float (&g)[DIMS], float &loss) {
-  float u[DIMS];
-  float D = 1.0f;  // gradient.rs:121-125
-#pragma unroll
-  for (int d = 0; d < DIMS; ++d) {
-    u[d] = yr[d] - yo[d];
-    D = fmaf(u[d], u[d], D);
-  }
-  const float rc = recip_nr(c + D);
-  const float w1 = k * (rc * rc);  // gradient.rs:130 / 140 (sign folded into k)
-#pragma unroll
-  for (int d = 0; d < DIMS; ++d) g[d] = fmaf(w1, u[d], g[d]);
-  if (LOSS && (w & kOwnerBit)) {
-    const bool nn = w < (1u << 30), fp = w >= (2u << 30);
-    const float wt = nn ? w_nn : (fp ? w_fp : w_mn);
-    const float num = fp ? 1.0f : D;  // gradient.rs:129 / 139
-    loss = loss + wt * (num * rc);
-  }
-}
-
-// pair_term_ck with the type read from the word's top bits (0 NN, 1 MN,
-// 2 and 3 FP).
-template <int DIMS, bool LOSS>
-__device__ __forceinline__ void pair_term(uint32_t w, const float (&yr)[DIMS], const float *yo, float k_nn,
-                                          float k_mn, float k_fp, float w_nn, float w_mn, float w_fp,
-                                          float (&g)[DIMS], float &loss) {
-  const bool nn = w < (1u << 30), fp = w >= (2u << 30);
-  const float c = nn ? 10.0f : (fp ? 1.0f : 10000.0f);
-  const float k = nn ? k_nn : (fp ? k_fp : k_mn);
-  pair_term_ck<DIMS, LOSS>(w, c, k, yr, yo, w_nn, w_mn, w_fp, g, loss);
-}
-
 template <int DIMS, bool ADAM, bool LOSS, int U>
 __global__ __launch_bounds__(256) void grad_adam_kernel(const GradArgs a) {
   const int t = threadIdx.x, gl = t & 15, grp = t >> 4;
@@ -975,6 +904,31 @@ static hipError_t launch_tb_dims(const GradArgs &a, const uint16_t *off, float *
   return hipGetLastError();
 }
 
+template <int DIMS>
+static hipError_t launch_tb_combine_dims(const GradArgs &a, const float *part, bool adam, hipStream_t s) {
+  const uint32_t g2 = (a.n + 255) / 256;
+  if (g2 == 0) return hipSuccess;
+  if (adam)
+    grad_tb_combine_kernel<DIMS, true><<<g2, 256, 0, s>>>(a, part);
+  else
+    grad_tb_combine_kernel<DIMS, false><<<g2, 256, 0, s>>>(a, part);
+  return hipGetLastError();
+}
+
+hipError_t launch_tb_combine(const GradArgs &a, const float *part, uint32_t dims, bool adam, hipStream_t s) {
+  switch (dims) {
+    case 1: return launch_tb_combine_dims<1>(a, part, adam, s);
+    case 2: return launch_tb_combine_dims<2>(a, part, adam, s);
+    case 3: return launch_tb_combine_dims<3>(a, part, adam, s);
+    case 4: return launch_tb_combine_dims<4>(a, part, adam, s);
+    case 5: return launch_tb_combine_dims<5>(a, part, adam, s);
+    case 6: return launch_tb_combine_dims<6>(a, part, adam, s);
+    case 7: return launch_tb_combine_dims<7>(a, part, adam, s);
+    case 8: return launch_tb_combine_dims<8>(a, part, adam, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
 hipError_t launch_grad_adam_tb(const GradArgs &a, const uint16_t *off, float *part, uint32_t dims, bool adam,
                                hipStream_t s) {
   switch (dims) {
@@ -1017,10 +971,6 @@ hipError_t launch_grad_adam_tb(const GradArgs &a, const uint16_t *off, float *pa
 constexpr uint32_t kSellRows = 16;         // rows per slice (4 lanes each)
 constexpr uint32_t kSellMaxChunk = 4096;   // rows per chunk (LDS bitonic sort)
 constexpr uint32_t kSellThreads = 1024;
-constexpr uint32_t kLdsPerCu = 163840;
-// |yr - far|^2 <= 8 (1e18)^2 stays finite; 1 / (c + D) >= 1.2e-37 is a
-// normal number; its square is far below the smallest denormal.
-constexpr float kSellFar = 1.0e18f;
 // LDS table of (c, k) by pair type (static LDS, at a fixed address): a
 // shift, a mask and one broadcast LDS read replace the type's compares and
 // selects.
@@ -1029,16 +979,6 @@ constexpr uint32_t kSellTabBytes = 32;
 // start on a 16-byte boundary of LDS.
 constexpr uint32_t kSellFarSlots = 4;
 __host__ __device__ __forceinline__ uint32_t sell_pad(const SellPlan &p) { return (2u << 30) | p.span; }
-
-__device__ __forceinline__ uint32_t tb_seg_beg(const uint16_t *__restrict__ off, uint32_t n, uint32_t b,
-                                               uint32_t row) {
-  return b == 0 ? 0u : off[static_cast<size_t>(b) * n + row];
-}
-__device__ __forceinline__ uint32_t tb_seg_end(const uint32_t *__restrict__ row_ptr,
-                                               const uint16_t *__restrict__ off, uint32_t n, uint32_t b,
-                                               uint32_t row) {
-  return b + 1 == kTargetBlocks ? row_ptr[row + 1] - row_ptr[row] : off[static_cast<size_t>(b + 1) * n + row];
-}
 
 // Entries of SELL range b: tb ranges b g .. b g + g - 1 (g = 8 / R), which
 // tb_partition stored contiguously.
@@ -1200,31 +1140,6 @@ __device__ __forceinline__ float quad_swap(float v) {
   return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), PERM, 0xF, 0xF, false));
 }
 
-// Copies count floats from global memory to LDS with a kSellThreads
-// workgroup: 16-byte copies (both addresses 16-byte aligned), 8 in flight
-// per thread, then the remainder and the last < 4 floats.
-__device__ __forceinline__ void stage_copy(float *__restrict__ dst, const float *__restrict__ src, uint32_t count,
-                                           uint32_t t) {
-  constexpr uint32_t T = kSellThreads;
-  const uint32_t n4 = count / 4;
-  const float4 *s4 = reinterpret_cast<const float4 *>(src);
-  float4 *d4 = reinterpret_cast<float4 *>(dst);
-  uint32_t i0 = t;
-  for (; i0 + 7 * T < n4; i0 += 8 * T) {
-    const float4 v0 = s4[i0], v1 = s4[i0 + T], v2 = s4[i0 + 2 * T], v3 = s4[i0 + 3 * T];
-    const float4 v4 = s4[i0 + 4 * T], v5 = s4[i0 + 5 * T], v6 = s4[i0 + 6 * T], v7 = s4[i0 + 7 * T];
-    d4[i0] = v0;
-    d4[i0 + T] = v1;
-    d4[i0 + 2 * T] = v2;
-    d4[i0 + 3 * T] = v3;
-    d4[i0 + 4 * T] = v4;
-    d4[i0 + 5 * T] = v5;
-    d4[i0 + 6 * T] = v6;
-    d4[i0 + 7 * T] = v7;
-  }
-  for (; i0 < n4; i0 += T) d4[i0] = s4[i0];
-  if (t < count % 4) dst[n4 * 4 + t] = src[n4 * 4 + t];
-}
 
 // U: steps of a slice loaded ahead (4 with 8 ranges, 8 with 4 ranges: about
 // the longest rows of most slices)
@@ -1252,8 +1167,8 @@ __global__ __launch_bounds__(kSellThreads) void grad_sell_kernel(const GradArgs 
   {
     // lo and r0 are multiples of 4 points and the LDS areas start on
     // 16-byte boundaries, as stage_copy needs
-    stage_copy(sY, a.y_in + static_cast<size_t>(lo) * DIMS, nlo * DIMS, t);
-    stage_copy(sR, a.y_in + static_cast<size_t>(r0) * DIMS, rows * DIMS, t);
+    stage_copy<kSellThreads>(sY, a.y_in + static_cast<size_t>(lo) * DIMS, nlo * DIMS, t);
+    stage_copy<kSellThreads>(sR, a.y_in + static_cast<size_t>(r0) * DIMS, rows * DIMS, t);
     if (t < DIMS) sY[static_cast<size_t>(p.span) * DIMS + t] = kSellFar;
   }
   const PairConsts pc = pair_consts(a.w);

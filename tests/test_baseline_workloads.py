@@ -14,11 +14,13 @@ sizes in seconds, so the checks are split by stage:
   GPU's preprocessed data, bit for bit (all points at 70,000, a sample of
   points at 1,000,000);
 * gradient: one evaluation at full size through the optimiser's own kernel
-  (LDS-staged sliced ELL at 70,000, target-blocked at 1,000,000) against the oracle at
-  1e-5 relative, the loss against FP64;
+  (LDS-staged sliced ELL at 70,000, paged target-blocked at 1,000,000) against the
+  oracle at 1e-5 relative, the loss against FP64;
 * optimiser: 10 iterations from a Value initialisation on the full generated
   pair set against the oracle, with a bar set by how far other valid
-  summation orders of the oracle itself land (see assert_parity_large);
+  summation orders of the oracle itself land (see assert_parity_large); at
+  1,000,000 points every step against one oracle step from the GPU's
+  previous iterate (see assert_shadowing);
 * the full 450-iteration run: 10-NN preservation, same-cluster neighbour share
   and random-triplet accuracy of the GPU embedding within 0.01 of the
   oracle's embedding from the same initial Y and pairs.
@@ -41,7 +43,7 @@ import pacmap_mi355x as pm
 sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
 import bench  # noqa: E402  (the benchmark's data generators)
 
-from test_gpu_parity import (REL, assert_parity, assert_parity_large, assert_rel,  # noqa: E402
+from test_gpu_parity import (REL, assert_parity, assert_parity_large, assert_rel, assert_shadowing,  # noqa: E402
                              order_spread, other_orders)
 
 pytestmark = pytest.mark.gpu
@@ -216,18 +218,28 @@ def test_c5_knn_nn_and_sampled_pairs_bitwise(c5):
 
 
 def test_c5_optimizer_10_iterations_target_blocked(c5):
-    """10 iterations of the target-blocked gradient (the path at 1M points)
-    on all 35M generated pairs from a Value initialisation: 1e-5 against the
-    oracle."""
+    """10 iterations of the target-blocked gradient (the path at 1M points,
+    in its paged form) on all 35M generated pairs from a Value
+    initialisation.  Every step is held to 1e-5 against one oracle step from
+    the GPU's previous iterate (assert_shadowing), and the end point has no
+    more elements beyond 1e-5 than 3x another oracle order.  The end point's
+    norm is only reported: at a million points it is dominated by single
+    chaotic rows (measured: row 994139 is 9e-5 off after 3 iterations under
+    any order and 0.16 to 0.75 off after 10, under the gather and the paged
+    forms of the kernel), which says nothing about the kernel."""
     n = 1_000_000
     y0 = O.value_init(np.random.default_rng(4).normal(size=(n, 2)).astype(np.float32))
     it = (4, 3, 3)
-    y, _, _ = pm.optimize(y0, c5["nn"], c5["mn"], c5["fp"], 1.0, it, with_loss=False)
-    assert pm.last_timings()["grad_target_blocks"] == 8
+    y, snaps, _ = pm.optimize(y0, c5["nn"], c5["mn"], c5["fp"], 1.0, it, snapshots=list(range(1, 11)),
+                              with_loss=False)
+    t = pm.last_timings()
+    assert t["grad_target_blocks"] == 8 and t["grad_paged"] == 1
+    assert np.array_equal(snaps[-1], y)
+    assert_shadowing(snaps, y0, c5["nn"], c5["mn"], c5["fp"], 1.0, it, THREADS)
     ry, _, _ = O.optimize(y0, c5["nn"], c5["mn"], c5["fp"], 1.0, it, nthreads=THREADS)
-    assert_parity_large(y, ry, other_orders(y0, c5["nn"], c5["mn"], c5["fp"], it, THREADS))
+    assert_parity_large(y, ry, other_orders(y0, c5["nn"], c5["mn"], c5["fp"], it, THREADS), norm=False)
     assert pm.pacmap_grad(y, c5["nn"], c5["mn"], c5["fp"], 2.0, 500.0, 1.0) is not None
-    assert pm.last_timings()["grad_target_blocks"] == 8
+    assert pm.last_timings()["grad_paged"] == 1
     _gradient_parity(y, c5["nn"], c5["mn"], c5["fp"], 5)
     _gradient_parity(y, c5["nn"], c5["mn"], c5["fp"], 300)
 

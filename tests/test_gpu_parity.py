@@ -56,7 +56,7 @@ def other_orders(y0, nn, mn, fp, it, nthreads=1):
             O.optimize(y0, nn, mn, np.ascontiguousarray(fp[perm]), 1.0, it, nthreads=nthreads)[0]]
 
 
-def assert_parity_large(got, ref, alts, rel=REL):
+def assert_parity_large(got, ref, alts, rel=REL, norm=True):
     """10-iteration parity at sizes where valid summation orders of the
     reference itself disagree near the tolerance.  From a Value init (std
     1e-4) Adam's first steps are about lr * sign(g), so a gradient component
@@ -64,7 +64,9 @@ def assert_parity_large(got, ref, alts, rel=REL):
     `alts` are the oracle under other valid orders.  Bars: norm-wise error
     at most max(rel, 3x the largest norm-wise difference between oracle
     orders), and no more elements off by more than rel * max(|ref|, rms)
-    than 3x the most such elements of another oracle order, plus 10."""
+    than 3x the most such elements of another oracle order, plus 10.
+    norm=False keeps only the element count (for sizes where single chaotic
+    rows dominate the norm; assert_shadowing then bounds every step)."""
     got, ref = np.asarray(got, np.float64), np.asarray(ref, np.float64)
     nrm = np.linalg.norm(ref)
     rms = np.sqrt(np.mean(ref ** 2))
@@ -75,8 +77,46 @@ def assert_parity_large(got, ref, alts, rel=REL):
     g_out = int((np.abs(got - ref) > tol).sum())
     print(f"parity: norm-wise rel err {g_err:.2e} (oracle orders {a_err:.2e}), elements beyond "
           f"{rel:g} relative {g_out} (oracle orders {a_out}) of {ref.size}", flush=True)
-    assert g_err <= max(rel, 3.0 * a_err)
+    if norm:
+        assert g_err <= max(rel, 3.0 * a_err)
     assert g_out <= 3 * a_out + 10
+
+
+def assert_shadowing(snaps, y0, nn, mn, fp, lr, it, nthreads=1, rel=REL):
+    """Stepwise parity of an optimiser trajectory (snaps[k] = Y after
+    iteration k): every GPU iterate is compared with one oracle iteration
+    (gradient.rs:35-151, adam.rs:35-60, weights.rs:52-81) applied to the
+    GPU's previous iterate, with the Adam moments carried along the GPU's
+    trajectory.  Each step is held to 1e-5 without the chaotic growth that
+    end points accumulate over iterations: norm-wise error at most rel, and
+    no more elements beyond rel * max(|y|, rms) than 3x those of the oracle
+    under another valid order (every pair list reversed) on the same step,
+    plus 10 (Adam's first steps are about lr * sign(g), so a component
+    within rounding noise of zero takes either sign)."""
+    rev = [np.ascontiguousarray(a[::-1]) for a in (nn, mn, fp)]
+    prev = np.asarray(y0, np.float32)
+    m = np.zeros_like(prev)
+    v = np.zeros_like(prev)
+    for itr in range(int(sum(it))):
+        w = O.find_weights(1000.0, itr, it[0], it[1])
+        preds = []
+        for lists in ((nn, mn, fp), rev):
+            g = O.pacmap_grad(prev, *lists, *w, nthreads=nthreads)
+            p, mm, vv = prev.copy(), m.copy(), v.copy()
+            O.update_embedding_adam(p, g, mm, vv, np.float32(0.9), np.float32(0.999), lr, itr)
+            preds.append((p, mm, vv))
+        pred, m, v = preds[0]
+        ref = pred.astype(np.float64)
+        tol = rel * np.maximum(np.abs(ref), np.sqrt(np.mean(ref ** 2)))
+        got = np.asarray(snaps[itr], np.float64)
+        nerr = np.linalg.norm(got - ref) / np.linalg.norm(ref)
+        out = int((np.abs(got - ref) > tol).sum())
+        alt = int((np.abs(preds[1][0].astype(np.float64) - ref) > tol).sum())
+        print(f"  step {itr}: norm-wise rel err {nerr:.2e}, elements beyond {rel:g} {out} "
+              f"(oracle reversed order {alt})", flush=True)
+        assert nerr <= rel
+        assert out <= 3 * alt + 10
+        prev = np.asarray(snaps[itr], np.float32)
 
 
 def oracle_loss64(y0, nn, mn, fp, lr, iters):
@@ -448,18 +488,25 @@ def test_optimize_10_iterations_matches_oracle():
     assert_loss(loss, oracle_loss64(y0, nn, mn, fp, 1.0, it))
 
 
+@pytest.mark.parametrize("paged", [False, True])
 @pytest.mark.parametrize("dims", [2, 3])
-def test_optimize_target_blocked_matches_oracle(monkeypatch, dims):
+def test_optimize_target_blocked_matches_oracle(monkeypatch, dims, paged):
     """The two-pass target-blocked gradient (used when Y outgrows an XCD's L2)
-    forced on at a small n: same bar against the oracle, same loss, and
-    deterministic."""
+    forced on at a small n, in its gather form and in its paged form (pages
+    of 100 points, so each range is walked in 4 pages): same bar against the
+    oracle, same loss, and deterministic."""
     y0, nn, mn, fp = _problem(62, n=3000)
     if dims != 2:
         y0 = np.random.default_rng(63).normal(size=(y0.shape[0], dims)).astype(np.float32)
     it = (4, 3, 3)
     monkeypatch.setenv("PACMAP_GRAD_TARGET_BLOCKS", "1")
+    monkeypatch.setenv("PACMAP_GRAD_PAGED", "1" if paged else "0")
+    monkeypatch.setenv("PACMAP_GRAD_PAGE_POINTS", "100")
     y, _, loss = pm.optimize(y0, nn, mn, fp, 1.0, it)
-    assert pm.last_timings()["grad_target_blocks"] == 8
+    t = pm.last_timings()
+    assert t["grad_target_blocks"] == 8 and t["grad_paged"] == int(paged)
+    if paged:
+        assert t["grad_page_pages"] == 4
     y2, _, loss2 = pm.optimize(y0, nn, mn, fp, 1.0, it)
     assert np.array_equal(y, y2) and np.array_equal(loss, loss2)
     ry, _, _ = O.optimize(y0, nn, mn, fp, 1.0, it)
@@ -483,8 +530,9 @@ def test_optimize_lds_sliced_matches_target_blocked(monkeypatch, n, dims):
         y0 = np.random.default_rng(65).normal(size=(n, dims)).astype(np.float32)
     it = (4, 3, 3)
     monkeypatch.setenv("PACMAP_GRAD_TARGET_BLOCKS", "1")
+    monkeypatch.setenv("PACMAP_GRAD_PAGED", "0")
     yt, _, _ = pm.optimize(y0, nn, mn, fp, 1.0, it)
-    assert pm.last_timings()["grad_target_blocks"] == 8
+    assert pm.last_timings()["grad_target_blocks"] == 8 and pm.last_timings()["grad_paged"] == 0
     monkeypatch.setenv("PACMAP_GRAD_TARGET_BLOCKS", "0")
     if n < 16384:
         monkeypatch.setenv("PACMAP_GRAD_SELL", "1")
@@ -509,6 +557,79 @@ def test_optimize_lds_sliced_matches_target_blocked(monkeypatch, n, dims):
     monkeypatch.setenv("PACMAP_GRAD_SELL", "0")
     pm.optimize(y0, nn, mn, fp, 1.0, it)
     assert pm.last_timings()["grad_sell"] == 0
+
+
+@pytest.mark.parametrize("n,dims,rows,points", [
+    (3000, 2, 0, 64), (3000, 3, 96, 100), (777, 1, 0, 0), (20000, 2, 512, 300), (150000, 2, 0, 0)])
+def test_paged_gradient_matches_oracle_and_target_blocked(monkeypatch, n, dims, rows, points):
+    """The paged target-blocked gradient (rows' Y and sums in LDS, the target
+    range walked in pages of Y staged through LDS; the default wherever the
+    target-blocked kernel runs), with chunks and pages forced small so that
+    slices are partial and each range takes several pages: the gradient
+    matches the oracle at 1e-5, the loss the FP64 loss, the gather form of
+    the kernel within 1e-5, and it is bit-identical from run to run.  The
+    layout holds every entry once plus padding."""
+    rng = np.random.default_rng(70 + n + dims)
+    y = (rng.normal(size=(n, dims)) * 3).astype(np.float32)
+    nn, mn, fp = _pairs(rng, n, n * 10), _pairs(rng, n, n * 5), _pairs(rng, n, n * 20)
+    w = (2.0, 730.5, 1.0)
+    monkeypatch.setenv("PACMAP_GRAD_TARGET_BLOCKS", "1")
+    monkeypatch.setenv("PACMAP_GRAD_PAGED", "1")
+    if rows:
+        monkeypatch.setenv("PACMAP_GRAD_PAGE_ROWS", str(rows))
+    if points:
+        monkeypatch.setenv("PACMAP_GRAD_PAGE_POINTS", str(points))
+    print(f"paged gradient n={n} dims={dims} rows<={rows or 'plan'} points<={points or 'plan'}", flush=True)
+    g = pm.pacmap_grad(y, nn, mn, fp, *w)
+    t = pm.last_timings()
+    assert t["grad_paged"] == 1 and t["grad_target_blocks"] == 8
+    if points:
+        assert t["grad_page_pages"] >= 2
+    entries = 2 * int((nn[:, 0] != nn[:, 1]).sum() + (mn[:, 0] != mn[:, 1]).sum() + (fp[:, 0] != fp[:, 1]).sum())
+    assert t["grad_page_words"] >= entries
+    print(f"  pages {t['grad_page_pages']}, words {t['grad_page_words']} for {entries} entries", flush=True)
+    g2 = pm.pacmap_grad(y, nn, mn, fp, *w)
+    assert np.array_equal(g.view(np.uint32), g2.view(np.uint32))
+    ref = O.pacmap_grad(y, nn, mn, fp, *w)
+    assert_rel(g[:-1], ref[:-1])
+    ref64 = O.pacmap_grad_f64(y.astype(np.float64), nn, mn, fp, *w)
+    assert abs(g[-1, 0] - ref64[-1, 0]) <= REL * abs(ref64[-1, 0])
+    monkeypatch.setenv("PACMAP_GRAD_PAGED", "0")
+    gt = pm.pacmap_grad(y, nn, mn, fp, *w)
+    assert pm.last_timings()["grad_paged"] == 0
+    assert_rel(g[:-1], gt[:-1])
+
+
+def test_paged_gradient_hub_rows(monkeypatch):
+    """Hub rows in the paged layout: one point in 6,000 NN pairs and another
+    in 40,000 FP pairs; their entries fill the widest slices of every page.
+    Bit-identical for any order of the pair lists, the oracle's gradient at
+    1e-5, and 10 iterations within the oracle's order spread."""
+    n = 20000
+    rng = np.random.default_rng(77)
+    y = rng.normal(size=(n, 2)).astype(np.float32)
+    hub_a = np.stack([np.zeros(6000, np.int64), rng.integers(1, n, 6000)], 1)
+    hub_b = np.stack([rng.integers(0, n, 40000), np.ones(40000, np.int64)], 1)
+    nn = np.concatenate([_pairs(rng, n, n * 5), hub_a]).astype(np.uint32)
+    mn = _pairs(rng, n, n * 3)
+    fp = np.concatenate([_pairs(rng, n, n * 10), hub_b]).astype(np.uint32)
+    w = (2.0, 120.0, 1.0)
+    monkeypatch.setenv("PACMAP_GRAD_TARGET_BLOCKS", "1")
+    monkeypatch.setenv("PACMAP_GRAD_PAGED", "1")
+    monkeypatch.setenv("PACMAP_GRAD_PAGE_POINTS", "400")
+    g1 = pm.pacmap_grad(y, nn, mn, fp, *w)
+    assert pm.last_timings()["grad_paged"] == 1
+    perm = [rng.permutation(len(a)) for a in (nn, mn, fp)]
+    g2 = pm.pacmap_grad(y, nn[perm[0]], mn[perm[1]], fp[perm[2]], *w)
+    assert np.array_equal(g1[:-1].view(np.uint32), g2[:-1].view(np.uint32))
+    assert_rel(g1[:-1], O.pacmap_grad(y, nn, mn, fp, *w)[:-1])
+    it = (4, 3, 3)
+    y0 = O.value_init(y)
+    got, _, loss = pm.optimize(y0, nn, mn, fp, 1.0, it)
+    assert pm.last_timings()["grad_paged"] == 1
+    ry, _, _ = O.optimize(y0, nn, mn, fp, 1.0, it)
+    assert_parity_large(got, ry, other_orders(y0, nn, mn, fp, it))
+    assert_loss(loss, oracle_loss64(y0, nn, mn, fp, 1.0, it))
 
 
 def test_optimizer_graph_replayed_only_for_identical_launches():
