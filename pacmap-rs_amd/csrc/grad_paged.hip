@@ -37,8 +37,8 @@ namespace pm {
 
 constexpr uint32_t kPageThreads = 1024;                 // gradient pass workgroup
 constexpr uint32_t kPageWaves = kPageThreads / 64;      // runs per block
-constexpr uint32_t kPageBuildThreads = 256;             // layout passes
-constexpr uint32_t kPageMaxRows = 8188;                 // rows per chunk: 13-bit row field, dummy = rc
+constexpr uint32_t kPageBuildThreads = 512;             // layout passes
+constexpr uint32_t kPageMaxRows = 6400;                 // rows per chunk (13-bit row field, dummy = rc; layout LDS)
 constexpr uint32_t kPageMaxPoints = 16380;              // points per page: 14-bit target field, far = pp
 constexpr uint32_t kPageBuckets = 16;                   // count buckets of the layout sort
 constexpr uint32_t kPageTgtBits = 14;
@@ -121,11 +121,16 @@ __device__ __forceinline__ uint32_t page_bucket(uint32_t cnt) {
 }
 
 // ---------------------------------------------------------------------------
-// Layout passes: one workgroup per (range b, chunk c, page pg) block.
-// Both passes count each row's entries in the page and sort the rows the
-// same way (a stable counting sort by count bucket, largest counts first;
-// counts of 15 or more share the first bucket), so the fill pass writes
-// exactly the runs the count pass sized.
+// Layout passes: one workgroup per (range b, chunk c), walking the pages of
+// range b in order.  A row's range-b segment is sorted by entry word, so it
+// is six runs by the word's top three bits (pair type, owner), each
+// ascending by target; a page's entries of a row are the next entries of
+// each run below the page's end, and six cursors per row move through the
+// segment once.  For every page both passes count each row's entries and
+// sort the rows the same way (a stable counting sort by count bucket,
+// largest counts first; counts of 15 or more share the first bucket), so
+// the fill pass writes exactly the runs the count pass sized, each row's
+// entries in CSR order.
 // ---------------------------------------------------------------------------
 template <bool FILL>
 __global__ __launch_bounds__(kPageBuildThreads) void page_layout_kernel(const uint32_t *__restrict__ row_ptr,
@@ -134,120 +139,189 @@ __global__ __launch_bounds__(kPageBuildThreads) void page_layout_kernel(const ui
                                                                         PagePlan p, uint32_t *__restrict__ rsz,
                                                                         const uint32_t *__restrict__ roff,
                                                                         uint32_t *__restrict__ went) {
-  __shared__ uint32_t sCnt[kPageMaxRows];
+  constexpr uint32_t kSl = (kPageMaxRows + 63) / 64;
+  constexpr uint32_t kBw = kPageBuildThreads / 64;  // waves of the sort
+  constexpr int kRuns = 6;                          // (type, owner) runs of a segment
+  __shared__ uint32_t sBeg[kPageMaxRows];           // segment start (absolute)
+  __shared__ uint16_t sLen[kPageMaxRows];           // segment length
+  __shared__ uint16_t sCur[kRuns][kPageMaxRows];    // cursors into the runs (relative)
+  __shared__ uint16_t sCnt[kPageMaxRows];           // entries in the current page
   __shared__ uint16_t sRowAt[kPageMaxRows];
-  __shared__ uint32_t sW[(kPageMaxRows + 63) / 64];       // slice widths (steps)
-  __shared__ uint32_t sBefore[(kPageMaxRows + 63) / 64];  // steps before the slice in its wave's run
-  __shared__ uint8_t sWave[(kPageMaxRows + 63) / 64];
+  __shared__ uint32_t sW[kSl];                      // slice widths (steps)
+  __shared__ uint32_t sBefore[kSl];                 // steps before the slice in its wave's run
+  __shared__ uint8_t sWave[kSl];
   __shared__ uint32_t sLoad[kPageWaves];
-  const uint32_t blk = blockIdx.x;
-  const uint32_t pg = blk % p.pages, c = (blk / p.pages) % p.chunks, b = blk / (p.pages * p.chunks);
-  const uint32_t t = threadIdx.x, lane = t & 63;
+  __shared__ uint32_t sBk[kBw][kPageBuckets];       // per-wave bucket counts, then starts
+  const uint32_t c = blockIdx.x % p.chunks, b = blockIdx.x / p.chunks;
+  const uint32_t t = threadIdx.x, lane = t & 63, wv = t >> 6;
   const uint32_t r0 = c * p.rc, rows = min(p.rc, n - r0);
-  uint32_t plo, phi;
-  page_bounds(p, n, b, pg, plo, phi);
+  // run r of a row starts after the entries whose key (word >> 29) is below r
   for (uint32_t i = t; i < rows; i += kPageBuildThreads) {
     const uint32_t row = r0 + i;
     const uint32_t rb = row_ptr[row];
-    const uint32_t e0 = rb + tb_seg_beg(off, n, b, row), e1 = rb + tb_seg_end(row_ptr, off, n, b, row);
-    uint32_t cnt = 0;
-    for (uint32_t e = e0; e < e1; ++e) {
-      const uint32_t o = tb_ent[e] & kIdxMask;
-      cnt += (o >= plo && o < phi) ? 1u : 0u;
+    const uint32_t s0 = tb_seg_beg(off, n, b, row), len = tb_seg_end(row_ptr, off, n, b, row) - s0;
+    uint32_t st[kRuns];
+#pragma unroll
+    for (int r = 0; r < kRuns; ++r) st[r] = 0;
+    for (uint32_t k = 0; k < len; ++k) {
+      const uint32_t key = tb_ent[rb + s0 + k] >> 29;
+#pragma unroll
+      for (int r = 0; r < kRuns; ++r) st[r] += key < static_cast<uint32_t>(r) ? 1u : 0u;
     }
-    sCnt[i] = cnt;
+    sBeg[i] = rb + s0;
+    sLen[i] = static_cast<uint16_t>(len);
+#pragma unroll
+    for (int r = 0; r < kRuns; ++r) sCur[r][i] = static_cast<uint16_t>(st[r]);
   }
   __syncthreads();
-  if (t < 64) {
-    // wave 0: bucket totals, then each row's place (bucket start + earlier
-    // rows of its bucket), 64 rows at a time in row order
-    const uint64_t below = (1ull << lane) - 1ull;
-    uint32_t cur[kPageBuckets];
-#pragma unroll
-    for (uint32_t v = 0; v < kPageBuckets; ++v) cur[v] = 0;
-    for (uint32_t q0 = 0; q0 < rows; q0 += 64) {
-      const uint32_t i = q0 + lane;
-      const uint32_t bk = i < rows ? page_bucket(sCnt[i]) : kPageBuckets;
-#pragma unroll
-      for (uint32_t v = 0; v < kPageBuckets; ++v) cur[v] += __popcll(__ballot(bk == v));
-    }
-    uint32_t acc = 0;
-#pragma unroll
-    for (uint32_t v = 0; v < kPageBuckets; ++v) {
-      const uint32_t x = cur[v];
-      cur[v] = acc;
-      acc += x;
-    }
-    for (uint32_t q0 = 0; q0 < rows; q0 += 64) {
-      const uint32_t i = q0 + lane;
-      const uint32_t bk = i < rows ? page_bucket(sCnt[i]) : kPageBuckets;
-      uint32_t pos = 0;
-#pragma unroll
-      for (uint32_t v = 0; v < kPageBuckets; ++v) {
-        const uint64_t m = __ballot(bk == v);
-        if (bk == v) pos = cur[v] + __popcll(m & below);
-        cur[v] += __popcll(m);
-      }
-      if (i < rows) sRowAt[pos] = static_cast<uint16_t>(i);
-    }
-  }
-  __syncthreads();
-  // slice widths: the longest row of each slice of 64 places
+  // the sort's rows: wave wv owns rows [wv q, wv q + q) in order
+  const uint32_t q_per = (rows + kBw - 1) / kBw;
+  const uint32_t sq0 = min(wv * q_per, rows), sq1 = min(sq0 + q_per, rows);
+  const uint64_t below = (1ull << lane) - 1ull;
   const uint32_t nsl = (rows + 63) / 64;
-  for (uint32_t s = t; s < nsl; s += kPageBuildThreads) {
-    uint32_t w = 0;
-    const uint32_t qe = min(s * 64 + 64, rows);
-    for (uint32_t q = s * 64; q < qe; ++q) w = max(w, sCnt[sRowAt[q]]);
-    sW[s] = w;
-  }
-  __syncthreads();
-  // slices to waves, widest first to the least loaded wave (ties: the lower
-  // wave), so that every wave of the gradient pass has about the same steps
-  // in this page; a wave's slices are stored in slice order
-  if (t == 0) {
-    for (uint32_t w = 0; w < kPageWaves; ++w) sLoad[w] = 0;
-    for (uint32_t s = 0; s < nsl; ++s) {
-      uint32_t best = 0;
-      for (uint32_t w = 1; w < kPageWaves; ++w)
-        if (sLoad[w] < sLoad[best]) best = w;
-      sWave[s] = static_cast<uint8_t>(best);
-      sBefore[s] = sLoad[best];
-      sLoad[best] += sW[s];
+  const size_t run_base = (static_cast<size_t>(b) * p.chunks + c) * kPageWaves * p.pages;
+  for (uint32_t pg = 0; pg < p.pages; ++pg) {
+    uint32_t plo, phi;
+    page_bounds(p, n, b, pg, plo, phi);
+    // entries of each row below the page's end (the count pass moves the
+    // cursors here; the fill pass moves them as it writes)
+    for (uint32_t i = t; i < rows; i += kPageBuildThreads) {
+      const uint32_t beg = sBeg[i], len = sLen[i];
+      uint32_t cnt = 0;
+#pragma unroll
+      for (int r = 0; r < kRuns; ++r) {
+        uint32_t k = sCur[r][i];
+        const uint32_t k0 = k;
+        for (; k < len; ++k) {
+          const uint32_t w = tb_ent[beg + k];
+          if ((w >> 29) != static_cast<uint32_t>(r) || (w & kIdxMask) >= phi) break;
+        }
+        cnt += k - k0;
+        if (!FILL) sCur[r][i] = static_cast<uint16_t>(k);
+      }
+      sCnt[i] = static_cast<uint16_t>(cnt);
     }
-  }
-  __syncthreads();
-  // run of wave w: ((b chunks + c) 16 + w) pages + pg (a wave's runs of all
-  // pages are contiguous)
-  const size_t run0 = (static_cast<size_t>(b) * p.chunks + c) * kPageWaves * p.pages + pg;
-  if (!FILL) {
-    if (t < kPageWaves) rsz[run0 + static_cast<size_t>(t) * p.pages] = 64u * sLoad[t];
-    return;
-  }
-  for (uint32_t q = t; q < nsl * 64; q += kPageBuildThreads) {
-    const uint32_t s = q >> 6, j = q & 63, ws = sW[s];
-    if (ws == 0) continue;
-    uint32_t *dst = went + roff[run0 + static_cast<size_t>(sWave[s]) * p.pages] + 64u * sBefore[s] + j;
-    const uint32_t i = q < rows ? sRowAt[q] : p.rc;
-    const uint32_t cnt = q < rows ? sCnt[i] : 0u;
-    const uint32_t rf = (cnt ? i : p.rc) << kPageRowShift;
-    uint32_t k = 0;
-    if (cnt) {
-      const uint32_t row = r0 + i;
-      const uint32_t rb = row_ptr[row];
-      const uint32_t e0 = rb + tb_seg_beg(off, n, b, row), e1 = rb + tb_seg_end(row_ptr, off, n, b, row);
-      for (uint32_t e = e0; e < e1; ++e) {
-        const uint32_t w = tb_ent[e];
-        const uint32_t o = w & kIdxMask;
-        if (o < plo || o >= phi) continue;
-        dst[64u * k] = (w & ~kIdxMask) | rf | (o - plo) | (k + 1 == ws ? kPageLast : 0u);
-        ++k;
+    __syncthreads();
+    // stable counting sort by bucket: per-wave bucket counts, starts, places
+    {
+      uint32_t cur[kPageBuckets];
+#pragma unroll
+      for (uint32_t v = 0; v < kPageBuckets; ++v) cur[v] = 0;
+      for (uint32_t q0 = sq0; q0 < sq1; q0 += 64) {
+        const uint32_t i = q0 + lane;
+        const uint32_t bk = i < sq1 ? page_bucket(sCnt[i]) : kPageBuckets;
+#pragma unroll
+        for (uint32_t v = 0; v < kPageBuckets; ++v) cur[v] += __popcll(__ballot(bk == v));
+      }
+      if (lane < kPageBuckets) {
+        uint32_t x = 0;
+#pragma unroll
+        for (uint32_t v = 0; v < kPageBuckets; ++v)
+          if (v == lane) x = cur[v];
+        sBk[wv][lane] = x;
+      }
+      __syncthreads();
+      // exclusive scan of the counts in (bucket, wave) order by wave 0: lane
+      // l holds entries 2l and 2l + 1 of the kPageBuckets x kBw table
+      if (wv == 0) {
+        static_assert(kPageBuckets * kBw == 128, "two scan entries per lane");
+        const uint32_t e0 = 2 * lane, e1 = e0 + 1;
+        const uint32_t x0 = sBk[e0 % kBw][e0 / kBw], x1 = sBk[e1 % kBw][e1 / kBw];
+        uint32_t inc = x0 + x1;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+          const uint32_t y = __shfl_up(inc, o, 64);
+          if (static_cast<int>(lane) >= o) inc += y;
+        }
+        const uint32_t ex = inc - x0 - x1;
+        sBk[e0 % kBw][e0 / kBw] = ex;
+        sBk[e1 % kBw][e1 / kBw] = ex + x0;
+      }
+      __syncthreads();
+#pragma unroll
+      for (uint32_t v = 0; v < kPageBuckets; ++v) cur[v] = sBk[wv][v];
+      for (uint32_t q0 = sq0; q0 < sq1; q0 += 64) {
+        const uint32_t i = q0 + lane;
+        const uint32_t bk = i < sq1 ? page_bucket(sCnt[i]) : kPageBuckets;
+        uint32_t pos = 0;
+#pragma unroll
+        for (uint32_t v = 0; v < kPageBuckets; ++v) {
+          const uint64_t m = __ballot(bk == v);
+          if (bk == v) pos = cur[v] + __popcll(m & below);
+          cur[v] += __popcll(m);
+        }
+        if (i < sq1) sRowAt[pos] = static_cast<uint16_t>(i);
       }
     }
-    for (; k < ws; ++k) dst[64u * k] = (2u << 30) | rf | p.pp | (k + 1 == ws ? kPageLast : 0u);
+    __syncthreads();
+    // slice widths: the longest row of each slice of 64 places, one wave
+    // per slice
+    for (uint32_t sl = wv; sl < nsl; sl += kBw) {
+      const uint32_t q = sl * 64 + lane;
+      uint32_t w = q < rows ? static_cast<uint32_t>(sCnt[sRowAt[q]]) : 0u;
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) w = max(w, static_cast<uint32_t>(__shfl_xor(static_cast<int>(w), o, 64)));
+      if (lane == 0) sW[sl] = w;
+    }
+    __syncthreads();
+    // slices to waves, each to the least loaded wave so far (ties: the lower
+    // wave), so that every wave of the gradient pass has about the same
+    // steps in this page; a wave's slices are stored in slice order.  Wave
+    // 0 keeps the loads in lanes 0..15 and finds the minimum by shuffles.
+    if (wv == 0) {
+      uint32_t load = 0;
+      for (uint32_t sl = 0; sl < nsl; ++sl) {
+        uint32_t key = lane < kPageWaves ? (load << 4) | lane : 0xffffffffu;
+#pragma unroll
+        for (int o = 8; o > 0; o >>= 1) key = min(key, static_cast<uint32_t>(__shfl_xor(static_cast<int>(key), o, 64)));
+        const uint32_t best = key & 15u;
+        const uint32_t wsl = sW[sl];
+        if (lane == 0) {
+          sWave[sl] = static_cast<uint8_t>(best);
+          sBefore[sl] = key >> 4;
+        }
+        if (lane == best) load += wsl;
+      }
+      if (lane < kPageWaves) sLoad[lane] = load;
+    }
+    __syncthreads();
+    // run of wave w: ((b chunks + c) 16 + w) pages + pg (a wave's runs of
+    // all pages are contiguous)
+    const size_t run0 = run_base + pg;
+    if (!FILL) {
+      if (t < kPageWaves) rsz[run0 + static_cast<size_t>(t) * p.pages] = 64u * sLoad[t];
+    } else {
+      for (uint32_t q = t; q < nsl * 64; q += kPageBuildThreads) {
+        const uint32_t sl = q >> 6, j = q & 63, ws = sW[sl];
+        if (ws == 0) continue;
+        uint32_t *dst = went + roff[run0 + static_cast<size_t>(sWave[sl]) * p.pages] + 64u * sBefore[sl] + j;
+        const uint32_t i = q < rows ? sRowAt[q] : p.rc;
+        const uint32_t cnt = q < rows ? sCnt[i] : 0u;
+        const uint32_t rf = (cnt ? i : p.rc) << kPageRowShift;
+        uint32_t k = 0;
+        if (cnt) {
+          const uint32_t beg = sBeg[i], len = sLen[i];
+#pragma unroll
+          for (int r = 0; r < kRuns; ++r) {
+            uint32_t e = sCur[r][i];
+            for (; e < len; ++e) {
+              const uint32_t w = tb_ent[beg + e];
+              const uint32_t o = w & kIdxMask;
+              if ((w >> 29) != static_cast<uint32_t>(r) || o >= phi) break;
+              dst[64u * k] = (w & ~kIdxMask) | rf | (o - plo) | (k + 1 == ws ? kPageLast : 0u);
+              ++k;
+            }
+            sCur[r][i] = static_cast<uint16_t>(e);
+          }
+        }
+        for (; k < ws; ++k) dst[64u * k] = (2u << 30) | rf | p.pp | (k + 1 == ws ? kPageLast : 0u);
+      }
+    }
+    __syncthreads();
   }
 }
 
-static uint32_t page_layout_blocks(const PagePlan &p) { return kTargetBlocks * p.chunks * p.pages; }
+static uint32_t page_layout_blocks(const PagePlan &p) { return kTargetBlocks * p.chunks; }
 
 hipError_t launch_page_count(const uint32_t *row_ptr, const uint32_t *tb_ent, const uint16_t *off, uint32_t n,
                              const PagePlan &p, uint32_t *rsz, hipStream_t s) {
