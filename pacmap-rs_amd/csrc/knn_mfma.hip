@@ -329,22 +329,51 @@ void knn_mfma_kernel(
 
   uint4 pf[PF];
   float pa = 0.0f, pi = 0.0f;
+  // When a stage's half (hi or lo fragments) is a whole number of 256-thread
+  // rounds, the stage is fetched through buffer resources: slots past the
+  // range point past the fragment arrays and read zeros with no branch, and
+  // each round reads one half.  Otherwise one guarded load per element.
+  constexpr int PERH = CT * KCH * 64;
+  constexpr bool ROUNDS = PERH % 256 == 0;
+  constexpr int PFH = ROUNDS ? PERH / 256 : 1;
+  const uint32_t frag_bytes = ntiles * static_cast<uint32_t>(KCH) * 64u * 16u;
+  const __amdgpu_buffer_rsrc_t rsh =
+      __builtin_amdgcn_make_buffer_rsrc(const_cast<uint4 *>(fhi), 0, static_cast<int>(frag_bytes), 0x00020000);
+  const __amdgpu_buffer_rsrc_t rsl = __builtin_amdgcn_make_buffer_rsrc(
+      const_cast<uint4 *>(NH == 2 ? flo : fhi), 0, static_cast<int>(frag_bytes), 0x00020000);
   auto fetch = [&](uint32_t st) {
     const uint32_t s0 = slot_lo + st * CT;
+    if constexpr (ROUNDS) {
 #pragma unroll
-    for (int j = 0; j < PF; ++j) {
-      const int i = static_cast<int>(threadIdx.x) + j * 256;
-      uint4 v = make_uint4(0, 0, 0, 0);
-      if (i < PER) {
-        const int l = i & 63, h = (i >> 6) % NH, kc = (i / (64 * NH)) % KCH, ct = i / (64 * NH * KCH);
-        const uint32_t slot = s0 + ct;
-        const uint32_t tile = slot_tile(slot);
-        if (slot < slot_hi && tile < ntiles) {
-          const size_t o = (static_cast<size_t>(tile) * KCH + kc) * 64 + l;
-          v = h == 0 ? fhi[o] : flo[o];
+      for (int h = 0; h < NH; ++h)
+#pragma unroll
+        for (int j = 0; j < PFH; ++j) {
+          const int i = static_cast<int>(threadIdx.x) + j * 256;
+          const int l = i & 63, kc = (i >> 6) % KCH, ct = i / (64 * KCH);
+          const uint32_t slot = s0 + ct;
+          const uint32_t tile = slot_tile(slot);
+          const uint32_t o = (slot < slot_hi && tile < ntiles)
+                                 ? ((tile * static_cast<uint32_t>(KCH) + kc) * 64u + l) * 16u
+                                 : 0xfffffff0u;
+          const auto v = __builtin_amdgcn_raw_buffer_load_b128(h == 0 ? rsh : rsl, static_cast<int>(o), 0, 0);
+          pf[h * PFH + j] = make_uint4(v[0], v[1], v[2], v[3]);
         }
+    } else {
+#pragma unroll
+      for (int j = 0; j < PF; ++j) {
+        const int i = static_cast<int>(threadIdx.x) + j * 256;
+        uint4 v = make_uint4(0, 0, 0, 0);
+        if (i < PER) {
+          const int l = i & 63, h = (i >> 6) % NH, kc = (i / (64 * NH)) % KCH, ct = i / (64 * NH * KCH);
+          const uint32_t slot = s0 + ct;
+          const uint32_t tile = slot_tile(slot);
+          if (slot < slot_hi && tile < ntiles) {
+            const size_t o = (static_cast<size_t>(tile) * KCH + kc) * 64 + l;
+            v = h == 0 ? fhi[o] : flo[o];
+          }
+        }
+        pf[j] = v;
       }
-      pf[j] = v;
     }
     if (threadIdx.x < CT * 16) {
       const uint32_t slot = s0 + threadIdx.x / 16;
@@ -354,10 +383,21 @@ void knn_mfma_kernel(
     }
   };
   auto stash = [&](int buf) {
+    if constexpr (ROUNDS) {
 #pragma unroll
-    for (int j = 0; j < PF; ++j) {
-      const int i = static_cast<int>(threadIdx.x) + j * 256;
-      if (i < PER) (&sB[buf][0][0][0][0])[i] = pf[j];
+      for (int h = 0; h < NH; ++h)
+#pragma unroll
+        for (int j = 0; j < PFH; ++j) {
+          const int i = static_cast<int>(threadIdx.x) + j * 256;
+          const int l = i & 63, kc = (i >> 6) % KCH, ct = i / (64 * KCH);
+          sB[buf][ct][kc][h][l] = pf[h * PFH + j];
+        }
+    } else {
+#pragma unroll
+      for (int j = 0; j < PF; ++j) {
+        const int i = static_cast<int>(threadIdx.x) + j * 256;
+        if (i < PER) (&sB[buf][0][0][0][0])[i] = pf[j];
+      }
     }
     if (threadIdx.x < CT * 16) {
       (&sA[buf][0][0])[threadIdx.x] = pa;
@@ -410,6 +450,60 @@ void knn_mfma_kernel(
         }
       }
     };
+    // both candidate tiles of a pair, chunk by chunk, so that 2 RT
+    // independent accumulation chains interleave (each chain adds its
+    // products in the same order as products())
+    auto products2 = [&](int ct, floatx4 (&acc0)[RT], floatx4 (&acc1)[RT]) {
+#pragma unroll
+      for (int rt = 0; rt < RT; ++rt) {
+        acc0[rt] = floatx4{0.f, 0.f, 0.f, 0.f};
+        acc1[rt] = floatx4{0.f, 0.f, 0.f, 0.f};
+      }
+#pragma unroll
+      for (int kc = 0; kc < KCH; ++kc) {
+        if (FP8) {
+          intx8 bv[2];
+#pragma unroll
+          for (int u = 0; u < 2; ++u) {
+            const uint4 b0 = sB[buf][ct + u][kc][0][lane], b1 = sB[buf][ct + u][kc][NH - 1][lane];
+            bv[u] = intx8{static_cast<int>(b0.x), static_cast<int>(b0.y), static_cast<int>(b0.z),
+                          static_cast<int>(b0.w), static_cast<int>(b1.x), static_cast<int>(b1.y),
+                          static_cast<int>(b1.z), static_cast<int>(b1.w)};
+          }
+#pragma unroll
+          for (int rt = 0; rt < RT; ++rt) {
+            const uint4 a0 = __builtin_bit_cast(uint4, ahi[rt][kc]), a1 = __builtin_bit_cast(uint4, alo[rt][kc]);
+            const intx8 av = {static_cast<int>(a0.x), static_cast<int>(a0.y), static_cast<int>(a0.z),
+                              static_cast<int>(a0.w), static_cast<int>(a1.x), static_cast<int>(a1.y),
+                              static_cast<int>(a1.z), static_cast<int>(a1.w)};
+            acc0[rt] = __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(av, bv[0], acc0[rt], 0, 0, 0, 127, 0, 127);
+            acc1[rt] = __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(av, bv[1], acc1[rt], 0, 0, 0, 127, 0, 127);
+          }
+          continue;
+        }
+        const bf16x8 bh0 = __builtin_bit_cast(bf16x8, sB[buf][ct][kc][0][lane]);
+        const bf16x8 bh1 = __builtin_bit_cast(bf16x8, sB[buf][ct + 1][kc][0][lane]);
+#pragma unroll
+        for (int rt = 0; rt < RT; ++rt) {
+          acc0[rt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi[rt][kc], bh0, acc0[rt], 0, 0, 0);
+          acc1[rt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi[rt][kc], bh1, acc1[rt], 0, 0, 0);
+        }
+        if (SPLIT) {
+          const bf16x8 bl0 = __builtin_bit_cast(bf16x8, sB[buf][ct][kc][NH - 1][lane]);
+          const bf16x8 bl1 = __builtin_bit_cast(bf16x8, sB[buf][ct + 1][kc][NH - 1][lane]);
+#pragma unroll
+          for (int rt = 0; rt < RT; ++rt) {
+            acc0[rt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi[rt][kc], bl0, acc0[rt], 0, 0, 0);
+            acc1[rt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi[rt][kc], bl1, acc1[rt], 0, 0, 0);
+          }
+#pragma unroll
+          for (int rt = 0; rt < RT; ++rt) {
+            acc0[rt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(alo[rt][kc], bh0, acc0[rt], 0, 0, 0);
+            acc1[rt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(alo[rt][kc], bh1, acc1[rt], 0, 0, 0);
+          }
+        }
+      }
+    };
     auto epilogue = [&](int ct, uint32_t tile, const floatx4 (&acc)[RT]) {
       const uint32_t c = tile * 16 + col;
       const float ac = sA[buf][ct][col];
@@ -457,8 +551,7 @@ void knn_mfma_kernel(
         const uint32_t slot1 = slot0 + 1;
         const uint32_t tile1 = slot_tile(slot1);
         floatx4 acc1[RT];
-        products(ct, acc0);
-        products(ct + 1, acc1);
+        products2(ct, acc0, acc1);
         epilogue(ct, tile0, acc0);
         if (slot1 < slot_hi && tile1 < ntiles) epilogue(ct + 1, tile1, acc1);
       } else {
