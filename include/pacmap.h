@@ -251,8 +251,12 @@ int pacmap_dev_download(void *dst_host, const void *src_dev, uint64_t bytes, pac
  * reused by its later calls (no hipMalloc / hipFree per call), and so is the
  * thread's last optimiser graph (replayed when a call's launches are
  * identical).  Uploads of 8 MiB or more go through two 32 MiB pinned host
- * buffers per calling thread, also kept between calls.  This frees the
- * calling thread's caches. */
+ * buffers per calling thread, also kept between calls.  A thread's stream,
+ * graph, pinned buffers and cached blocks are released when the thread
+ * exits; an allocation that runs out of device memory first frees the idle
+ * cached blocks of every thread and retries.  This frees the idle cached
+ * blocks of every thread, and the calling thread's graph and pinned upload
+ * buffers. */
 void pacmap_release_cached_memory(void);
 /* HBM reference: a plain streaming device-to-device copy kernel run `repeats`
  * times; ms_out gets the average time per copy (HIP events).  Its rocprof

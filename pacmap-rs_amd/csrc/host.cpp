@@ -237,8 +237,11 @@ struct GraphCache {
     g = nullptr;
     key.clear();
   }
+  ~GraphCache() {
+    if (!process_exiting()) reset();
+  }
 };
-thread_local GraphCache g_graph;  // not destroyed at thread exit: the runtime may be gone by then
+thread_local GraphCache g_graph;  // the thread's kept graph, destroyed when the thread exits
 
 }  // namespace
 void graph_cache_reset() { g_graph.reset(); }

@@ -117,17 +117,35 @@ class DevBuf {
   int slot_ = kSlotNone;
 };
 
+// Thread-exit destructors release the thread's HIP resources unless the
+// process is already exiting (see pool.cpp); note_runtime_use registers the
+// hook that tells them.
+void note_runtime_use();
+bool process_exiting();
+
 // The calling thread's stream: one non-blocking stream per thread, created
-// on first use and kept for the life of the process (creating and
-// destroying one per call costs tens of microseconds).  Every entry point
-// orders all of its work on it and synchronises it before returning, so
-// work left by a call that failed still runs before the next call's.
+// on first use and destroyed when the thread exits (creating and destroying
+// one per call costs tens of microseconds).  Every entry point orders all of
+// its work on it and synchronises it before returning, so work left by a call
+// that failed still runs before the next call's.
+struct ThreadStream {
+  hipStream_t s = nullptr;
+  ~ThreadStream() {
+    if (s && !process_exiting()) {
+      (void)hipStreamSynchronize(s);
+      (void)hipStreamDestroy(s);
+    }
+  }
+};
 class Stream {
  public:
   Stream() {
-    static thread_local hipStream_t thread_stream = nullptr;
-    if (!thread_stream) PM_HIP(hipStreamCreateWithFlags(&thread_stream, hipStreamNonBlocking));
-    s_ = thread_stream;
+    static thread_local ThreadStream thread_stream;
+    if (!thread_stream.s) {
+      note_runtime_use();
+      PM_HIP(hipStreamCreateWithFlags(&thread_stream.s, hipStreamNonBlocking));
+    }
+    s_ = thread_stream.s;
   }
   Stream(const Stream &) = delete;
   Stream &operator=(const Stream &) = delete;
@@ -162,11 +180,21 @@ inline std::vector<PendingFlag> &pending_flags() {
   return v;
 }
 constexpr int kPinnedWords = 64;
-// kPinnedWords words of pinned host memory per thread, kept for the process
+// kPinnedWords words of pinned host memory per thread, freed when the thread
+// exits
+struct PinnedWords {
+  uint32_t *p = nullptr;
+  ~PinnedWords() {
+    if (p && !process_exiting()) (void)hipHostFree(p);
+  }
+};
 inline uint32_t *pinned_words() {
-  static thread_local uint32_t *p = nullptr;
-  if (!p) PM_HIP(hipHostMalloc(reinterpret_cast<void **>(&p), kPinnedWords * sizeof(uint32_t), 0));
-  return p;
+  static thread_local PinnedWords w;
+  if (!w.p) {
+    note_runtime_use();
+    PM_HIP(hipHostMalloc(reinterpret_cast<void **>(&w.p), kPinnedWords * sizeof(uint32_t), 0));
+  }
+  return w.p;
 }
 // Copies the device word `flag` into pinned memory on stream s; when the
 // entry point returns, a nonzero value fails it with (code, msg).

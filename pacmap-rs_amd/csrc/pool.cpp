@@ -1,6 +1,9 @@
 // pool.cpp -- per-thread cache of device memory blocks and the staged
 // host-to-device upload (see host_common.h).
 #include <algorithm>
+#include <atomic>
+#include <cstdlib>
+#include <vector>
 #include <condition_variable>
 #include <iterator>
 #include <map>
@@ -10,18 +13,70 @@
 #include "host_common.h"
 
 namespace pm {
+
+// Set by an exit hook registered on first use.  Thread-exit destructors run
+// before the process's static destructors (the HIP runtime's among them), so
+// they may release HIP resources; once the hook has run, static destruction
+// is under way and they leave them to the end of the process.
+namespace {
+std::atomic<bool> g_exiting{false};
+std::once_flag g_exit_hook;
+}  // namespace
+void note_runtime_use() {
+  std::call_once(g_exit_hook, [] { std::atexit([] { g_exiting.store(true); }); });
+}
+bool process_exiting() { return g_exiting.load(); }
+
 namespace {
 
+// One thread's cache of idle device blocks.  Every pool is listed in a
+// process-wide registry, so that an out-of-memory retry or
+// pacmap_release_cached_memory frees the idle blocks of every thread; each
+// pool has its own lock (uncontended on the owner's calls).
+struct Pool;
+struct Registry {
+  std::mutex mu;
+  std::vector<Pool *> pools;
+};
+Registry &registry() {
+  static Registry *r = new Registry;  // never destroyed: pools unregister at thread exit
+  return *r;
+}
+
 struct Pool {
+  std::mutex mu;
   std::multimap<size_t, void *> free_blocks;
   struct Kept {
     size_t cls = 0;
     void *p = nullptr;
   } slots[kSlotCount];
-  ~Pool() {
-    for (auto &kv : free_blocks) (void)hipFree(kv.second);
+  Pool() {
+    note_runtime_use();
+    Registry &r = registry();
+    std::lock_guard<std::mutex> lk(r.mu);
+    r.pools.push_back(this);
+  }
+  // moves every idle block (slots included) out of the pool
+  void take_all(std::vector<void *> &out) {
+    std::lock_guard<std::mutex> lk(mu);
+    for (auto &kv : free_blocks) out.push_back(kv.second);
+    free_blocks.clear();
     for (auto &k : slots)
-      if (k.p) (void)hipFree(k.p);
+      if (k.p) {
+        out.push_back(k.p);
+        k.p = nullptr;
+      }
+  }
+  ~Pool() {
+    {
+      Registry &r = registry();
+      std::lock_guard<std::mutex> lk(r.mu);
+      r.pools.erase(std::remove(r.pools.begin(), r.pools.end(), this), r.pools.end());
+    }
+    if (process_exiting()) return;
+    std::vector<void *> blocks;
+    take_all(blocks);
+    for (void *p : blocks) (void)hipFree(p);
   }
 };
 
@@ -46,17 +101,21 @@ size_t dev_size_class(size_t bytes) {
 // same order as the previous one gets the same blocks back (which also lets
 // the optimiser graph of that call be replayed).
 void *dev_acquire(size_t cls) {
-  auto &fb = pool().free_blocks;
-  const auto range = fb.equal_range(cls);
-  if (range.first != range.second) {
-    const auto it = std::prev(range.second);
-    void *p = it->second;
-    fb.erase(it);
-    return p;
+  Pool &pl = pool();
+  {
+    std::lock_guard<std::mutex> lk(pl.mu);
+    auto &fb = pl.free_blocks;
+    const auto range = fb.equal_range(cls);
+    if (range.first != range.second) {
+      const auto it = std::prev(range.second);
+      void *p = it->second;
+      fb.erase(it);
+      return p;
+    }
   }
   void *p = nullptr;
   hipError_t e = hipMalloc(&p, cls);
-  if (e != hipSuccess) {  // out of memory: give the cache back and retry once
+  if (e != hipSuccess) {  // out of memory: give every thread's idle blocks back and retry once
     (void)hipGetLastError();
     dev_trim();
     e = hipMalloc(&p, cls);
@@ -65,42 +124,57 @@ void *dev_acquire(size_t cls) {
   return p;
 }
 
-void dev_recycle(void *p, size_t cls) { pool().free_blocks.emplace(cls, p); }
+void dev_recycle(void *p, size_t cls) {
+  Pool &pl = pool();
+  std::lock_guard<std::mutex> lk(pl.mu);
+  pl.free_blocks.emplace(cls, p);
+}
 
 void *dev_acquire_slot(int slot, size_t cls) {
-  auto &k = pool().slots[slot];
-  if (k.p && k.cls == cls) {
-    void *p = k.p;
+  Pool &pl = pool();
+  void *old = nullptr;
+  size_t old_cls = 0;
+  {
+    std::lock_guard<std::mutex> lk(pl.mu);
+    auto &k = pl.slots[slot];
+    if (k.p && k.cls == cls) {
+      void *p = k.p;
+      k.p = nullptr;
+      return p;
+    }
+    old = k.p;
+    old_cls = k.cls;
     k.p = nullptr;
-    return p;
   }
-  if (k.p) dev_recycle(k.p, k.cls);
-  k.p = nullptr;
+  if (old) dev_recycle(old, old_cls);
   return dev_acquire(cls);
 }
 
 void dev_release_slot(int slot, void *p, size_t cls) {
-  auto &k = pool().slots[slot];
+  Pool &pl = pool();
+  std::lock_guard<std::mutex> lk(pl.mu);
+  auto &k = pl.slots[slot];
   if (k.p) {  // the slot already holds a block: this one joins the general cache
-    dev_recycle(p, cls);
+    pl.free_blocks.emplace(cls, p);
     return;
   }
   k.p = p;
   k.cls = cls;
 }
 
+// Frees the idle blocks of every thread's pool (blocks in use belong to
+// DevBufs, not to the pools).  The device is synchronised first: a block
+// can be released while work queued on its owner's stream still uses it.
 void dev_trim() {
-  auto &pl = pool();
-  for (auto &k : pl.slots)
-    if (k.p) {
-      pl.free_blocks.emplace(k.cls, k.p);
-      k.p = nullptr;
-    }
-  auto &fb = pl.free_blocks;
-  if (fb.empty()) return;
+  std::vector<void *> blocks;
+  {
+    Registry &r = registry();
+    std::lock_guard<std::mutex> lk(r.mu);
+    for (Pool *pl : r.pools) pl->take_all(blocks);
+  }
+  if (blocks.empty()) return;
   (void)hipDeviceSynchronize();
-  for (auto &kv : fb) (void)hipFree(kv.second);
-  fb.clear();
+  for (void *p : blocks) (void)hipFree(p);
 }
 
 // ---------------------------------------------------------------------------
@@ -184,6 +258,9 @@ constexpr size_t kStageMin = size_t{8} << 20;     // smaller uploads go directly
 struct Staging {
   void *buf[2] = {nullptr, nullptr};
   hipEvent_t ev[2] = {nullptr, nullptr};
+  ~Staging() {
+    if (!process_exiting()) release();
+  }
   void release() {
     for (int b = 0; b < 2; ++b) {
       if (ev[b]) {
@@ -197,7 +274,8 @@ struct Staging {
   }
 };
 Staging &staging() {
-  thread_local Staging st;  // kept for the thread's calls (not freed at thread exit: the runtime may be gone)
+  thread_local Staging st;  // kept for the thread's calls, freed when the thread exits
+  note_runtime_use();
   return st;
 }
 

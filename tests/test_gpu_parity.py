@@ -632,6 +632,43 @@ def test_paged_gradient_hub_rows(monkeypatch):
     assert_loss(loss, oracle_loss64(y0, nn, mn, fp, 1.0, it))
 
 
+def _device_free_bytes():
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    free, total = ctypes.c_size_t(), ctypes.c_size_t()
+    assert hip.hipMemGetInfo(ctypes.byref(free), ctypes.byref(total)) == 0
+    return free.value
+
+
+def test_thread_caches_released_at_thread_exit():
+    """Calls from short-lived threads: each thread gets its own stream, block
+    cache, graph and pinned buffers, computes the main thread's embedding bit
+    for bit, and returns what it held when it exits (free device memory after
+    eight threads have run and exited is within 64 MiB of before)."""
+    import threading
+    y0, nn, mn, fp = _problem(80, n=20000)
+    it = (4, 3, 3)
+    ref = pm.optimize(y0, nn, mn, fp, 1.0, it)[0]
+    before = _device_free_bytes()
+    for i in range(8):
+        out = {}
+
+        def work():
+            try:
+                out["y"] = pm.optimize(y0, nn, mn, fp, 1.0, it)[0]
+            except Exception as e:  # noqa: BLE001 - re-raised on the main thread
+                out["err"] = e
+
+        th = threading.Thread(target=work)
+        th.start()
+        th.join()
+        assert "err" not in out, out.get("err")
+        assert np.array_equal(out["y"], ref)
+    after = _device_free_bytes()
+    print(f"  device free before {before >> 20} MiB, after 8 threads {after >> 20} MiB", flush=True)
+    assert before - after < (64 << 20)
+
+
 def test_optimizer_graph_replayed_only_for_identical_launches():
     """A call whose launches match the previous call's replays its graph; the
     results equal a freshly captured run, and any change recaptures."""
