@@ -110,13 +110,26 @@ bool knn_mfma_path(const float *x_dev, uint64_t n, uint64_t d, uint32_t k, const
   // split the candidate range until there are enough (group, split) blocks
   // to keep every CU busy with a short tail
   const uint32_t groups = knn_cand_groups(static_cast<uint32_t>(ntiles));
-  const uint32_t nsplit = std::min<uint32_t>(16, std::max<uint32_t>(1, (4096 + groups - 1) / groups));
+  // symmetric pass (each tile pair of the upper triangle once, appends to
+  // both rows) unless PACMAP_KNN_SYM=0
+  bool sym = true;
+  if (const char *e = std::getenv("PACMAP_KNN_SYM")) sym = std::atoi(e) != 0;
+  const uint32_t nsplit =
+      sym ? 1u : std::min<uint32_t>(16, std::max<uint32_t>(1, (4096 + groups - 1) / groups));
   DevBuf<uint32_t> gcnt(n * nsplit), gcand(n * nsplit * capg * (split ? 2 : 1)), fb_list(n);
   DevBuf<float> gtau(n);
   DevBuf<unsigned long long> kstats(512);
   PM_HIP(launch_fill_zero(kstats.get(), kstats.bytes(), s));
   const void *flo_p = mode != 0 ? flo.get() : fhi.get();
   const float *rinv_p = mode == 2 ? rinv.get() : nullptr;
+  std::vector<uint32_t> items;
+  DevBuf<uint32_t> ditems;
+  if (sym) {
+    items = knn_sym_items(static_cast<uint32_t>(ntiles), std::max<uint32_t>(4096, 4 * groups));
+    ditems.alloc(items.size());
+    upload(ditems, items.data(), items.size(), s);
+    PM_HIP(launch_fill_zero(gcnt.get(), gcnt.bytes(), s));
+  }
   Timer tsamp, tk, tr;
   tsamp.start(s);
   PM_HIP(launch_knn_cand(0, fhi.get(), flo_p, arow.get(), rinv_p, static_cast<uint32_t>(n), kch,
@@ -124,9 +137,10 @@ bool knn_mfma_path(const float *x_dev, uint64_t n, uint64_t d, uint32_t k, const
                          gtau.get(), mode, s));
   tsamp.stop(s);
   tk.start(s);
-  PM_HIP(launch_knn_cand(1, fhi.get(), flo_p, arow.get(), rinv_p, static_cast<uint32_t>(n), kch,
+  PM_HIP(launch_knn_cand(sym ? 2 : 1, fhi.get(), flo_p, arow.get(), rinv_p, static_cast<uint32_t>(n), kch,
                          static_cast<uint32_t>(ntiles), ss, capg, nsplit, gcnt.get(), gcand.get(),
-                         gtau.get(), mode, s));
+                         gtau.get(), mode, s, sym ? ditems.get() : nullptr,
+                         static_cast<uint32_t>(items.size() / 3)));
   tk.stop(s);
   tr.start(s);
   // upper-bound pruning (certified split lists): see knn_rerank_kernel

@@ -23,6 +23,8 @@
 
 #include <cstdint>
 #include <type_traits>
+#include <vector>
+#include <algorithm>
 
 namespace pm {
 
@@ -251,6 +253,16 @@ __global__ void knn_rownorm_kernel(const float *__restrict__ x, uint64_t n, uint
 // PHASE 1 scans the candidate tiles of split blockIdx.y (of gridDim.y) and
 // appends every candidate with lb <= tau to the row's list of that split
 // (MODE 1: entries are (candidate, dot) pairs).
+// PHASE 2 computes each pair of tiles once: the block's item (items[3 b ..])
+// names a group g of 4 RT query tiles and a range of candidate tiles from
+// the upper triangle (tiles >= 4 RT g).  On the group's own tiles only the
+// query rows' lists are appended (every ordered pair of the group is
+// computed there); on later tiles a pair (q, c) is tested against both
+// tau_q and tau_c and appended to both rows' lists.  Appends are staged per
+// wave in LDS and flushed with one batch of global atomics on the rows'
+// counters (gcnt[row], lists of capg entries, nsplit = 1), so a list's order
+// depends on timing; the rerank sorts by (distance, index) and its pruning
+// bound is order-free, so the result does not.
 //
 // MODE 0: one bf16 product per multiply-add; MODE 1: split bf16 (hi hi + hi lo
 // + lo hi); MODE 2: FP8 E4M3 rows scaled by 1 / rinv (one 16x16x128 f8f6f4
@@ -262,7 +274,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MODE == 1 &
 void knn_mfma_kernel(
     const uint4 *__restrict__ fhi, const uint4 *__restrict__ flo, const float *__restrict__ arow,
     const float *__restrict__ rinv, uint32_t n, uint32_t ntiles, uint32_t ss, uint32_t capg,
-    uint32_t *__restrict__ gcnt, uint32_t *__restrict__ gcand, float *__restrict__ gtau) {
+    uint32_t *__restrict__ gcnt, uint32_t *__restrict__ gcand, float *__restrict__ gtau,
+    const uint32_t *__restrict__ items) {
   constexpr bool SPLIT = MODE == 1;
   constexpr bool FP8 = MODE == 2;
   constexpr int CT = (8 / KCH) > 0 ? (8 / KCH) : 1;  // candidate tiles per LDS stage
@@ -273,11 +286,18 @@ void knn_mfma_kernel(
   __shared__ float sA[2][CT][16];
   __shared__ float sI[2][CT][16];  // FP8: candidate rinv
   __shared__ uint32_t sCnt[4][16 * RT];
+  // PHASE 2: (a_c - tau_c) / 2 of the stage's candidates, and each wave's
+  // staged appends (row, candidate, dot)
+  constexpr uint32_t kApp = 128;
+  __shared__ float sT[PHASE == 2 ? 2 : 1][CT][16];
+  __shared__ uint32_t sApp[PHASE == 2 ? 4 : 1][3][PHASE == 2 ? kApp : 1];
 
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const uint32_t qtile0 = (blockIdx.x * 4 + wave) * RT;
+  const uint32_t group = PHASE == 2 ? items[3 * blockIdx.x] : blockIdx.x;
+  const uint32_t gtile0 = group * 4 * RT;  // the group's first query tile
+  const uint32_t qtile0 = (group * 4 + wave) * RT;
   const int rsub = lane >> 4, col = lane & 15;
-  const uint32_t split = PHASE == 0 ? 0 : blockIdx.y, nsplit = PHASE == 0 ? 1 : gridDim.y;
+  const uint32_t split = PHASE == 1 ? blockIdx.y : 0, nsplit = PHASE == 1 ? gridDim.y : 1;
 
   bf16x8 ahi[RT][KCH], alo[RT][KCH];
   float aq[RT][4], iq[RT][4];
@@ -317,18 +337,23 @@ void knn_mfma_kernel(
       best0[rt][r] = __builtin_huge_valf();
       best1[rt][r] = __builtin_huge_valf();
       const uint32_t q = qidx[rt][r];
-      tau[rt][r] = (PHASE == 1 && q < n) ? (aq[rt][r] - gtau[q]) * 0.5f : __builtin_huge_valf();
+      tau[rt][r] = (PHASE != 0 && q < n) ? (aq[rt][r] - gtau[q]) * 0.5f : __builtin_huge_valf();
     }
 
-  // candidate slots: sample tiles (phase 0) or this split's range (phase 1)
+  // candidate slots: sample tiles (phase 0), this split's range (phase 1)
+  // or the item's range (phase 2)
   const uint32_t nsample = (ntiles + ss - 1) / ss;
-  const uint32_t slot_lo = PHASE == 0 ? 0 : static_cast<uint32_t>(static_cast<uint64_t>(ntiles) * split / nsplit);
-  const uint32_t slot_hi = PHASE == 0 ? nsample : static_cast<uint32_t>(static_cast<uint64_t>(ntiles) * (split + 1) / nsplit);
+  const uint32_t slot_lo = PHASE == 0   ? 0
+                           : PHASE == 2 ? items[3 * blockIdx.x + 1]
+                                        : static_cast<uint32_t>(static_cast<uint64_t>(ntiles) * split / nsplit);
+  const uint32_t slot_hi = PHASE == 0   ? nsample
+                           : PHASE == 2 ? items[3 * blockIdx.x + 2]
+                                        : static_cast<uint32_t>(static_cast<uint64_t>(ntiles) * (split + 1) / nsplit);
   auto slot_tile = [&](uint32_t slot) -> uint32_t { return PHASE == 0 ? slot * ss : slot; };
   const uint32_t nstage = (slot_hi - slot_lo + CT - 1) / CT;
 
   uint4 pf[PF];
-  float pa = 0.0f, pi = 0.0f;
+  float pa = 0.0f, pi = 0.0f, pt = 0.0f;
   // When a stage's half (hi or lo fragments) is a whole number of 256-thread
   // rounds, the stage is fetched through buffer resources: slots past the
   // range point past the fragment arrays and read zeros with no branch, and
@@ -380,6 +405,11 @@ void knn_mfma_kernel(
       const uint32_t tile = slot_tile(slot);
       pa = (slot < slot_hi && tile < ntiles) ? arow[tile * 16 + (threadIdx.x & 15)] : 0.0f;
       if (FP8) pi = (slot < slot_hi && tile < ntiles) ? rinv[tile * 16 + (threadIdx.x & 15)] : 0.0f;
+      if (PHASE == 2) {
+        // rows past n carry a = NaN, so their half-threshold is NaN too
+        const uint32_t c = tile * 16 + (threadIdx.x & 15);
+        pt = (slot < slot_hi && tile < ntiles) ? (pa - (c < n ? gtau[c] : 0.0f)) * 0.5f : __builtin_nanf("");
+      }
     }
   };
   auto stash = [&](int buf) {
@@ -402,6 +432,51 @@ void knn_mfma_kernel(
     if (threadIdx.x < CT * 16) {
       (&sA[buf][0][0])[threadIdx.x] = pa;
       if (FP8) (&sI[buf][0][0])[threadIdx.x] = pi;
+      if (PHASE == 2) (&sT[buf & (PHASE == 2 ? 1 : 0)][0][0])[threadIdx.x] = pt;
+    }
+  };
+
+  // PHASE 2 appends: this wave's staged count (uniform) and the flush of
+  // its staged entries, every atomic of the batch issued before any slot
+  // is used
+  uint32_t napp = 0;
+  auto flush = [&]() __attribute__((always_inline)) {
+    if constexpr (PHASE != 2) return;
+    constexpr int R4 = kApp / 64;
+    uint32_t row[R4], cnd[R4], dv[R4], pos[R4];
+#pragma unroll
+    for (int i = 0; i < R4; ++i) {
+      const uint32_t e = i * 64 + lane;
+      const bool v = e < napp;
+      row[i] = v ? sApp[wave][0][e] : 0u;
+      cnd[i] = v ? sApp[wave][1][e] : 0u;
+      dv[i] = v ? sApp[wave][2][e] : 0u;
+      pos[i] = v ? atomicAdd(&gcnt[row[i]], 1u) : 0xffffffffu;
+    }
+#pragma unroll
+    for (int i = 0; i < R4; ++i)
+      if (pos[i] < capg) {
+        const size_t slot = static_cast<size_t>(row[i]) * capg + pos[i];
+        if (SPLIT)
+          reinterpret_cast<uint2 *>(gcand)[slot] = make_uint2(cnd[i], dv[i]);
+        else
+          gcand[slot] = cnd[i];
+      }
+    napp = 0;
+  };
+  // one test set (at most 64 appends): room is made first
+  auto append = [&](bool pass, uint32_t row, uint32_t cnd, float dot) __attribute__((always_inline)) {
+    if (napp > kApp - 64) flush();
+    const uint64_t m = __ballot(pass);
+    if (m) {
+      if (pass) {
+        const uint32_t e = napp + __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(m >> 32),
+                                                            __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(m), 0u));
+        sApp[wave][0][e] = row;
+        sApp[wave][1][e] = cnd;
+        sApp[wave][2][e] = __float_as_uint(dot);
+      }
+      napp += static_cast<uint32_t>(__popcll(m));
     }
   };
 
@@ -513,6 +588,22 @@ void knn_mfma_kernel(
       // the wave's own query tiles (a uniform condition)
       const float gc = ac * 0.5f;
       const bool diag = tile - qtile0 < static_cast<uint32_t>(RT);
+      if constexpr (PHASE == 2) {
+        // the group's own tiles: query rows only (uniform condition)
+        const bool own = tile - gtile0 < static_cast<uint32_t>(4 * RT);
+        const float tc = sT[buf & (PHASE == 2 ? 1 : 0)][ct][col];
+#pragma unroll
+        for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const uint32_t q = qidx[rt][r];
+            const float dot = FP8 ? acc[rt][r] * (iq[rt][r] * ic) : acc[rt][r];
+            append(dot >= tau[rt][r] + gc && (!diag || c != q), q, c, dot);
+            // tc + aq / 2 (the halving is exact, so the fused form rounds once, as the sum does)
+            if (!own) append(dot >= fmaf(aq[rt][r], 0.5f, tc), c, q, dot);
+          }
+        return;
+      }
 #pragma unroll
       for (int rt = 0; rt < RT; ++rt)
 #pragma unroll
@@ -563,6 +654,10 @@ void knn_mfma_kernel(
     __syncthreads();
   }
 
+  if (PHASE == 2) {
+    flush();
+    return;
+  }
   if (PHASE == 0) {
     // tau = 16th smallest of the 32 per-row values held by the row's 16 lanes
 #pragma unroll
@@ -879,15 +974,18 @@ template <int KCH, int MODE>
 static hipError_t cand_dispatch(int phase, const void *fhi, const void *flo, const float *arow,
                                 const float *rinv, uint32_t n, uint32_t ntiles, uint32_t ss, uint32_t capg,
                                 uint32_t nsplit, uint32_t *gcnt, uint32_t *gcand, float *gtau,
-                                hipStream_t s) {
+                                const uint32_t *items, uint32_t nitems, hipStream_t s) {
   const uint32_t groups = knn_cand_groups(ntiles);
   const uint4 *h = static_cast<const uint4 *>(fhi), *l = static_cast<const uint4 *>(flo);
   if (phase == 0)
     knn_mfma_kernel<KCH, kCandRT, MODE, 0><<<dim3(groups, 1), 256, 0, s>>>(h, l, arow, rinv, n, ntiles, ss,
-                                                                          capg, gcnt, gcand, gtau);
-  else
+                                                                          capg, gcnt, gcand, gtau, nullptr);
+  else if (phase == 1)
     knn_mfma_kernel<KCH, kCandRT, MODE, 1><<<dim3(groups, nsplit), 256, 0, s>>>(
-        h, l, arow, rinv, n, ntiles, ss, capg, gcnt, gcand, gtau);
+        h, l, arow, rinv, n, ntiles, ss, capg, gcnt, gcand, gtau, nullptr);
+  else if (nitems)
+    knn_mfma_kernel<KCH, kCandRT, MODE, 2><<<dim3(nitems, 1), 256, 0, s>>>(
+        h, l, arow, rinv, n, ntiles, ss, capg, gcnt, gcand, gtau, items);
   return hipGetLastError();
 }
 
@@ -895,22 +993,44 @@ template <int KCH>
 static hipError_t cand_dispatch_mode(int mode, int phase, const void *fhi, const void *flo,
                                      const float *arow, const float *rinv, uint32_t n, uint32_t ntiles,
                                      uint32_t ss, uint32_t capg, uint32_t nsplit, uint32_t *gcnt,
-                                     uint32_t *gcand, float *gtau, hipStream_t s) {
+                                     uint32_t *gcand, float *gtau, const uint32_t *items, uint32_t nitems,
+                                     hipStream_t s) {
   switch (mode) {
-    case 0: return cand_dispatch<KCH, 0>(phase, fhi, flo, arow, rinv, n, ntiles, ss, capg, nsplit, gcnt, gcand, gtau, s);
-    case 1: return cand_dispatch<KCH, 1>(phase, fhi, flo, arow, rinv, n, ntiles, ss, capg, nsplit, gcnt, gcand, gtau, s);
-    case 2: return cand_dispatch<KCH, 2>(phase, fhi, flo, arow, rinv, n, ntiles, ss, capg, nsplit, gcnt, gcand, gtau, s);
+    case 0: return cand_dispatch<KCH, 0>(phase, fhi, flo, arow, rinv, n, ntiles, ss, capg, nsplit, gcnt, gcand, gtau, items, nitems, s);
+    case 1: return cand_dispatch<KCH, 1>(phase, fhi, flo, arow, rinv, n, ntiles, ss, capg, nsplit, gcnt, gcand, gtau, items, nitems, s);
+    case 2: return cand_dispatch<KCH, 2>(phase, fhi, flo, arow, rinv, n, ntiles, ss, capg, nsplit, gcnt, gcand, gtau, items, nitems, s);
     default: return hipErrorInvalidValue;
   }
+}
+
+// Upper-triangle items of the symmetric candidate pass: group g (4 kCandRT
+// query tiles) scans candidate tiles [4 kCandRT g, ntiles) in pieces of
+// about the same number of tiles, so every workgroup has nearly equal work.
+std::vector<uint32_t> knn_sym_items(uint32_t ntiles, uint32_t target_items) {
+  const uint32_t groups = knn_cand_groups(ntiles), gt = 4 * kCandRT;
+  uint64_t tri = 0;
+  for (uint32_t g = 0; g < groups; ++g) tri += ntiles - g * gt;
+  uint64_t len = (tri + target_items - 1) / target_items;
+  len = std::max<uint64_t>(len, 2 * gt);
+  len = (len + 1) & ~uint64_t{1};  // whole stages of two candidate tiles
+  std::vector<uint32_t> it;
+  for (uint32_t g = 0; g < groups; ++g)
+    for (uint64_t lo = static_cast<uint64_t>(g) * gt; lo < ntiles; lo += len) {
+      it.push_back(g);
+      it.push_back(static_cast<uint32_t>(lo));
+      it.push_back(static_cast<uint32_t>(std::min<uint64_t>(lo + len, ntiles)));
+    }
+  return it;
 }
 
 hipError_t launch_knn_cand(int phase, const void *fhi, const void *flo, const float *arow, const float *rinv,
                            uint32_t n, uint32_t kch, uint32_t ntiles, uint32_t ss, uint32_t capg,
                            uint32_t nsplit, uint32_t *gcnt, uint32_t *gcand, float *gtau,
-                           int mode, hipStream_t s) {
+                           int mode, hipStream_t s, const uint32_t *items, uint32_t nitems) {
   if (mode == 2 && rinv == nullptr) return hipErrorInvalidValue;
+  if (phase == 2 && nitems && items == nullptr) return hipErrorInvalidValue;
 #define PM_CAND(K) \
-  case K: return cand_dispatch_mode<K>(mode, phase, fhi, flo, arow, rinv, n, ntiles, ss, capg, nsplit, gcnt, gcand, gtau, s);
+  case K: return cand_dispatch_mode<K>(mode, phase, fhi, flo, arow, rinv, n, ntiles, ss, capg, nsplit, gcnt, gcand, gtau, items, nitems, s);
   switch (kch) {
     PM_CAND(1)
     PM_CAND(2)
