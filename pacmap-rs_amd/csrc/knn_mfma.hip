@@ -275,7 +275,7 @@ void knn_mfma_kernel(
     const uint4 *__restrict__ fhi, const uint4 *__restrict__ flo, const float *__restrict__ arow,
     const float *__restrict__ rinv, uint32_t n, uint32_t ntiles, uint32_t ss, uint32_t capg,
     uint32_t *__restrict__ gcnt, uint32_t *__restrict__ gcand, float *__restrict__ gtau,
-    const uint32_t *__restrict__ items) {
+    const uint32_t *__restrict__ items, uint32_t sparse) {
   constexpr bool SPLIT = MODE == 1;
   constexpr bool FP8 = MODE == 2;
   constexpr int CT = (8 / KCH) > 0 ? (8 / KCH) : 1;  // candidate tiles per LDS stage
@@ -440,29 +440,25 @@ void knn_mfma_kernel(
   // its staged entries, every atomic of the batch issued before any slot
   // is used
   uint32_t napp = 0;
+  // the flush: every slot of a round is reserved by an atomic on its row's
+  // counter before any is written; slots past capg are dropped (the row's
+  // count then exceeds capg and the rerank rescans it)
   auto flush = [&]() __attribute__((always_inline)) {
-    if constexpr (PHASE != 2) return;
-    constexpr int R4 = kApp / 64;
-    uint32_t row[R4], cnd[R4], dv[R4], pos[R4];
-#pragma unroll
-    for (int i = 0; i < R4; ++i) {
-      const uint32_t e = i * 64 + lane;
-      const bool v = e < napp;
-      row[i] = v ? sApp[wave][0][e] : 0u;
-      cnd[i] = v ? sApp[wave][1][e] : 0u;
-      dv[i] = v ? sApp[wave][2][e] : 0u;
-      pos[i] = v ? atomicAdd(&gcnt[row[i]], 1u) : 0xffffffffu;
-    }
-#pragma unroll
-    for (int i = 0; i < R4; ++i)
-      if (pos[i] < capg) {
-        const size_t slot = static_cast<size_t>(row[i]) * capg + pos[i];
-        if (SPLIT)
-          reinterpret_cast<uint2 *>(gcand)[slot] = make_uint2(cnd[i], dv[i]);
-        else
-          gcand[slot] = cnd[i];
+    if constexpr (PHASE == 2) {
+#pragma unroll 1
+      for (uint32_t e = static_cast<uint32_t>(lane); e < napp; e += 64) {
+        const uint32_t row = sApp[wave][0][e], cnd = sApp[wave][1][e], dv = sApp[wave][2][e];
+        const uint32_t pos = atomicAdd(&gcnt[row], 1u);
+        if (pos < capg) {
+          const size_t slot = static_cast<size_t>(row) * capg + pos;
+          if (SPLIT)
+            reinterpret_cast<uint2 *>(gcand)[slot] = make_uint2(cnd, dv);
+          else
+            gcand[slot] = cnd;
+        }
       }
-    napp = 0;
+      napp = 0;
+    }
   };
   // one test set (at most 64 appends): room is made first
   auto append = [&](bool pass, uint32_t row, uint32_t cnd, float dot) __attribute__((always_inline)) {
@@ -589,17 +585,40 @@ void knn_mfma_kernel(
       const float gc = ac * 0.5f;
       const bool diag = tile - qtile0 < static_cast<uint32_t>(RT);
       if constexpr (PHASE == 2) {
-        // the group's own tiles: query rows only (uniform condition)
+        // the group's own tiles: query rows only (uniform condition).  All
+        // tests first; most tiles list nothing, and then one wave-wide test
+        // ends the epilogue.
         const bool own = tile - gtile0 < static_cast<uint32_t>(4 * RT);
         const float tc = sT[buf & (PHASE == 2 ? 1 : 0)][ct][col];
+        // tc + aq / 2: the halving is exact, so the fused form rounds once,
+        // as the sum does.  With `sparse` (few tiles list anything: large
+        // n), a wave-wide test of every set first ends most epilogues; it
+        // ignores the self pair, which only sends the wave down the listing
+        // path, where it is tested.
+        auto dot_of = [&](int rt, int r) __attribute__((always_inline)) {
+          return FP8 ? acc[rt][r] * (iq[rt][r] * ic) : acc[rt][r];
+        };
+        if (sparse) {
+          uint64_t any = 0;
+#pragma unroll
+          for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) any |= __ballot(dot_of(rt, r) >= tau[rt][r] + gc);
+          if (!own) {
+#pragma unroll
+            for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+              for (int r = 0; r < 4; ++r) any |= __ballot(dot_of(rt, r) >= fmaf(aq[rt][r], 0.5f, tc));
+          }
+          if (any == 0) return;
+        }
 #pragma unroll
         for (int rt = 0; rt < RT; ++rt)
 #pragma unroll
           for (int r = 0; r < 4; ++r) {
             const uint32_t q = qidx[rt][r];
-            const float dot = FP8 ? acc[rt][r] * (iq[rt][r] * ic) : acc[rt][r];
+            const float dot = dot_of(rt, r);
             append(dot >= tau[rt][r] + gc && (!diag || c != q), q, c, dot);
-            // tc + aq / 2 (the halving is exact, so the fused form rounds once, as the sum does)
             if (!own) append(dot >= fmaf(aq[rt][r], 0.5f, tc), c, q, dot);
           }
         return;
@@ -974,18 +993,18 @@ template <int KCH, int MODE>
 static hipError_t cand_dispatch(int phase, const void *fhi, const void *flo, const float *arow,
                                 const float *rinv, uint32_t n, uint32_t ntiles, uint32_t ss, uint32_t capg,
                                 uint32_t nsplit, uint32_t *gcnt, uint32_t *gcand, float *gtau,
-                                const uint32_t *items, uint32_t nitems, hipStream_t s) {
+                                const uint32_t *items, uint32_t nitems, uint32_t sparse, hipStream_t s) {
   const uint32_t groups = knn_cand_groups(ntiles);
   const uint4 *h = static_cast<const uint4 *>(fhi), *l = static_cast<const uint4 *>(flo);
   if (phase == 0)
     knn_mfma_kernel<KCH, kCandRT, MODE, 0><<<dim3(groups, 1), 256, 0, s>>>(h, l, arow, rinv, n, ntiles, ss,
-                                                                          capg, gcnt, gcand, gtau, nullptr);
+                                                                          capg, gcnt, gcand, gtau, nullptr, 0u);
   else if (phase == 1)
     knn_mfma_kernel<KCH, kCandRT, MODE, 1><<<dim3(groups, nsplit), 256, 0, s>>>(
-        h, l, arow, rinv, n, ntiles, ss, capg, gcnt, gcand, gtau, nullptr);
+        h, l, arow, rinv, n, ntiles, ss, capg, gcnt, gcand, gtau, nullptr, 0u);
   else if (nitems)
     knn_mfma_kernel<KCH, kCandRT, MODE, 2><<<dim3(nitems, 1), 256, 0, s>>>(
-        h, l, arow, rinv, n, ntiles, ss, capg, gcnt, gcand, gtau, items);
+        h, l, arow, rinv, n, ntiles, ss, capg, gcnt, gcand, gtau, items, sparse);
   return hipGetLastError();
 }
 
@@ -994,11 +1013,11 @@ static hipError_t cand_dispatch_mode(int mode, int phase, const void *fhi, const
                                      const float *arow, const float *rinv, uint32_t n, uint32_t ntiles,
                                      uint32_t ss, uint32_t capg, uint32_t nsplit, uint32_t *gcnt,
                                      uint32_t *gcand, float *gtau, const uint32_t *items, uint32_t nitems,
-                                     hipStream_t s) {
+                                     uint32_t sparse, hipStream_t s) {
   switch (mode) {
-    case 0: return cand_dispatch<KCH, 0>(phase, fhi, flo, arow, rinv, n, ntiles, ss, capg, nsplit, gcnt, gcand, gtau, items, nitems, s);
-    case 1: return cand_dispatch<KCH, 1>(phase, fhi, flo, arow, rinv, n, ntiles, ss, capg, nsplit, gcnt, gcand, gtau, items, nitems, s);
-    case 2: return cand_dispatch<KCH, 2>(phase, fhi, flo, arow, rinv, n, ntiles, ss, capg, nsplit, gcnt, gcand, gtau, items, nitems, s);
+    case 0: return cand_dispatch<KCH, 0>(phase, fhi, flo, arow, rinv, n, ntiles, ss, capg, nsplit, gcnt, gcand, gtau, items, nitems, sparse, s);
+    case 1: return cand_dispatch<KCH, 1>(phase, fhi, flo, arow, rinv, n, ntiles, ss, capg, nsplit, gcnt, gcand, gtau, items, nitems, sparse, s);
+    case 2: return cand_dispatch<KCH, 2>(phase, fhi, flo, arow, rinv, n, ntiles, ss, capg, nsplit, gcnt, gcand, gtau, items, nitems, sparse, s);
     default: return hipErrorInvalidValue;
   }
 }
@@ -1026,11 +1045,11 @@ std::vector<uint32_t> knn_sym_items(uint32_t ntiles, uint32_t target_items) {
 hipError_t launch_knn_cand(int phase, const void *fhi, const void *flo, const float *arow, const float *rinv,
                            uint32_t n, uint32_t kch, uint32_t ntiles, uint32_t ss, uint32_t capg,
                            uint32_t nsplit, uint32_t *gcnt, uint32_t *gcand, float *gtau,
-                           int mode, hipStream_t s, const uint32_t *items, uint32_t nitems) {
+                           int mode, hipStream_t s, const uint32_t *items, uint32_t nitems, bool sparse) {
   if (mode == 2 && rinv == nullptr) return hipErrorInvalidValue;
   if (phase == 2 && nitems && items == nullptr) return hipErrorInvalidValue;
 #define PM_CAND(K) \
-  case K: return cand_dispatch_mode<K>(mode, phase, fhi, flo, arow, rinv, n, ntiles, ss, capg, nsplit, gcnt, gcand, gtau, items, nitems, s);
+  case K: return cand_dispatch_mode<K>(mode, phase, fhi, flo, arow, rinv, n, ntiles, ss, capg, nsplit, gcnt, gcand, gtau, items, nitems, sparse, s);
   switch (kch) {
     PM_CAND(1)
     PM_CAND(2)
