@@ -630,22 +630,74 @@ struct PcaAxes {
   DevBuf<float> mean;  // d
 };
 
-void pca_axes_dev(const float *x_dev, uint64_t n, uint32_t d, uint32_t q, PcaAxes &out, hipStream_t s) {
+// FP64 column means and covariance sum of the rows of x, by the row chunks
+// of cov_chunks (see launch_pca_stats_chunks).
+struct PcaStats {
+  DevBuf<double> mean, cov, shift, colpart, covpart;
+  CovChunks c;
+  void alloc(uint64_t n, uint32_t d) {
+    c = cov_chunks(n, d);
+    mean.alloc(d);
+    shift.alloc(d);
+    cov.alloc(static_cast<size_t>(d) * d);
+    colpart.alloc(colsum_chunk_scratch_doubles(c, d));
+    covpart.alloc(cov_chunk_scratch_doubles(c, d));
+  }
+  void finish(uint64_t n, uint32_t d, hipStream_t s) {
+    PM_HIP(launch_pca_stats_finish(n, d, c, colpart.get(), shift.get(), covpart.get(), mean.get(), cov.get(), s));
+    colpart.release();
+    covpart.release();
+  }
+};
+
+// X already in device memory: every chunk in one launch.
+void pca_stats_dev(const float *x_dev, uint64_t n, uint32_t d, PcaStats &st, hipStream_t s) {
+  Timer tc;
+  tc.start(s);
+  st.alloc(n, d);
+  PM_HIP(launch_pca_stats_chunks(x_dev, n, d, st.c, 0, st.c.chunks, st.colpart.get(), st.shift.get(),
+                                 st.covpart.get(), s));
+  st.finish(n, d, s);
+  tc.stop(s);
+  tc.add_to(g_tm.pca_cov_ms);
+}
+
+// X in host memory: uploaded in the statistics' row chunks on the copy
+// stream; each chunk's sums run on s as soon as it lands, while the next
+// chunk is in flight.  The sums are the same bits as pca_stats_dev's.
+void upload_pca_stats(const float *x_host, uint64_t n, uint32_t d, float *x_dev, PcaStats &st, hipStream_t s) {
+  CopyStream cs;
+  st.alloc(n, d);
+  Timer tu, tc;
+  tu.start(s);
+  upload_rows_pieces(x_dev, x_host, n, static_cast<size_t>(d) * sizeof(float), st.c.rows, s, cs.get(),
+                     [&](uint32_t k, hipEvent_t landed) {
+                       PM_HIP(hipStreamWaitEvent(s, landed, 0));
+                       if (k + 1 == st.c.chunks) tc.start(s);  // the sums left after the last piece landed
+                       PM_HIP(launch_pca_stats_chunks(x_dev, n, d, st.c, k, 1, st.colpart.get(), st.shift.get(),
+                                                      st.covpart.get(), s));
+                     });
+  tu.stop(cs.get());
+  st.finish(n, d, s);
+  tc.stop(s);
+  tu.add_to(g_tm.upload_ms);
+  tc.add_to(g_tm.pca_cov_ms);
+}
+
+void pca_axes_dev(PcaStats &stats, uint64_t n, uint32_t d, uint32_t q, PcaAxes &out, hipStream_t s) {
   constexpr uint32_t kIters = 6;
   const uint32_t jmax = jacobi_max_size();
-  DevBuf<double> d_mean(d), d_scr(cov_scratch_doubles(n, d)), d_cov(static_cast<size_t>(d) * d);
-  Timer tc, te;
-  tc.start(s);
-  PM_HIP(launch_colmean_f64(x_dev, n, d, d_mean.get(), d_scr.get(), s));
-  PM_HIP(launch_cov_f64(x_dev, n, d, d_mean.get(), d_scr.get(), d_cov.get(), s));
-  tc.stop(s);
+  (void)n;
+  double *const d_cov = stats.cov.get();
+  double *const d_mean = stats.mean.get();
+  Timer te;
   te.start(s);
   DevBuf<double> V(static_cast<size_t>(d) * q);
   DevBuf<uint32_t> status(1);
   PM_HIP(launch_fill_zero(status.get(), status.bytes(), s));
   DevBuf<double> jwork(jacobi_work_doubles(std::min(d, jmax)));
   if (d <= jmax) {
-    PM_HIP(launch_jacobi(d_cov.get(), d, q, V.get(), jwork.get(), s));
+    PM_HIP(launch_jacobi(d_cov, d, q, V.get(), jwork.get(), s));
   } else {
     const uint32_t b = std::min<uint32_t>(std::min<uint32_t>(d, q + 12), jmax);
     if (q > b) fail(PACMAP_ERR_PCA, "too many principal components requested");
@@ -679,24 +731,29 @@ void pca_axes_dev(const float *x_dev, uint64_t n, uint32_t d, uint32_t q, PcaAxe
     // the covariance is exactly symmetric (cov_fin_kernel writes both halves
     // from one sum), so it is read as transposed: 128-byte rows per load
     for (uint32_t it = 0; it < kIters; ++it) {
-      gemm(d, b, d, d_cov.get(), d, true, Q.get(), b, Z.get(), b);
+      gemm(d, b, d, d_cov, d, true, Q.get(), b, Z.get(), b);
       orth(it + 1 == kIters);
     }
     // Rayleigh-Ritz: T = Q^T C Q, eigenvectors U, axes V = Q U
-    gemm(d, b, d, d_cov.get(), d, true, Q.get(), b, Z.get(), b);
+    gemm(d, b, d, d_cov, d, true, Q.get(), b, Z.get(), b);
     gemm(b, b, d, Q.get(), b, true, Z.get(), b, G.get(), b);
     PM_HIP(launch_jacobi(G.get(), b, q, U.get(), jwork.get(), s));
     gemm(d, q, b, Q.get(), b, false, U.get(), q, V.get(), q);
   }
   out.Vf.alloc(static_cast<size_t>(d) * q);
   out.mean.alloc(d);
-  PM_HIP(launch_sign_fix(V.get(), d, q, out.Vf.get(), d_mean.get(), out.mean.get(), s));
+  PM_HIP(launch_sign_fix(V.get(), d, q, out.Vf.get(), d_mean, out.mean.get(), s));
   te.stop(s);
   // checked when the entry point returns (the status stays 0 unless the
   // orthonormalisation failed even with a shift)
   defer_flag_check(status.get(), PACMAP_ERR_PCA, "subspace orthonormalisation failed", s);
-  tc.add_to(g_tm.pca_cov_ms);
   te.add_to(g_tm.pca_eig_ms);
+}
+
+void pca_axes_dev(const float *x_dev, uint64_t n, uint32_t d, uint32_t q, PcaAxes &out, hipStream_t s) {
+  PcaStats st;
+  pca_stats_dev(x_dev, n, d, st, s);
+  pca_axes_dev(st, n, d, q, out, s);
 }
 
 // out (n x q) = (x - mean) V on the device
@@ -836,14 +893,23 @@ void fit_impl(const float *x_host, const float *x_dev_in, uint64_t n, uint64_t d
   const hipStream_t s = stream.get();
   DevBuf<float> d_xin;
   const float *xd = x_dev_in;
+  // on the PCA path a host X is uploaded in row chunks whose FP64 sums run
+  // while the next chunk is in flight
+  PcaStats pca_stats;
+  bool have_stats = false;
   if (x_host) {
-    Timer tu;
-    tu.start(s);
     d_xin.alloc(n * d);
-    upload(d_xin, x_host, n * d, s);
-    tu.stop(s);
-    tu.add_to(g_tm.upload_ms);
     xd = d_xin.get();
+    if (pca_path) {
+      upload_pca_stats(x_host, n, static_cast<uint32_t>(d), d_xin.get(), pca_stats, s);
+      have_stats = true;
+    } else {
+      Timer tu;
+      tu.start(s);
+      upload(d_xin, x_host, n * d, s);
+      tu.stop(s);
+      tu.add_to(g_tm.upload_ms);
+    }
   }
 
   // --- preprocess_x (lib.rs:442-532) ---
@@ -857,7 +923,8 @@ void fit_impl(const float *x_host, const float *x_dev_in, uint64_t n, uint64_t d
     const uint32_t q = static_cast<uint32_t>(std::min<uint64_t>(100, n));  // lib.rs:457
     if (dims > q) fail(PACMAP_ERR_INVALID_ARGUMENT, "embedding_dimensions exceeds the PCA width");
     PcaAxes ax;
-    pca_axes_dev(xd, n, static_cast<uint32_t>(d), q, ax, s);
+    if (!have_stats) pca_stats_dev(xd, n, static_cast<uint32_t>(d), pca_stats, s);
+    pca_axes_dev(pca_stats, n, static_cast<uint32_t>(d), q, ax, s);
     pca_project_dev(xd, n, static_cast<uint32_t>(d), q, ax, d_xk, s);
     dk = q;
   } else {

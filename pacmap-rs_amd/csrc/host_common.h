@@ -156,6 +156,26 @@ class Stream {
   hipStream_t s_ = nullptr;
 };
 
+// A second per-thread stream for host-to-device copies that overlap the
+// work of the calling thread's stream (same lifetime rules).
+class CopyStream {
+ public:
+  CopyStream() {
+    static thread_local ThreadStream thread_stream;
+    if (!thread_stream.s) {
+      note_runtime_use();
+      PM_HIP(hipStreamCreateWithFlags(&thread_stream.s, hipStreamNonBlocking));
+    }
+    s_ = thread_stream.s;
+  }
+  CopyStream(const CopyStream &) = delete;
+  CopyStream &operator=(const CopyStream &) = delete;
+  hipStream_t get() const { return s_; }
+
+ private:
+  hipStream_t s_ = nullptr;
+};
+
 // Event pair timing a region of one stream.
 // Work settled when an entry point returns (settle_pending): phase timings,
 // and device flags copied into pinned host words.  Reading either as soon as
@@ -339,6 +359,14 @@ inline void require_device(const void *p, const char *what) {
 // thread's pinned staging buffers (pool.cpp); src may be reused when this
 // returns.  staging_release frees the thread's buffers.
 void upload_bytes(void *dst, const void *src, size_t bytes, hipStream_t s);
+// Uploads rows x row_bytes bytes in pieces of piece_rows whole rows (at most
+// 32 MiB) through the pinned staging buffers, the DMA on copy stream cs
+// (which first waits for the work queued on s).  After piece k is queued,
+// on_piece(k, landed) is called with an event recorded on cs after it; wait
+// on it (hipStreamWaitEvent) before the call returns to order work on the
+// piece.
+void upload_rows_pieces(void *dst, const void *src, uint64_t rows, size_t row_bytes, uint64_t piece_rows,
+                        hipStream_t s, hipStream_t cs, const std::function<void(uint32_t, hipEvent_t)> &on_piece);
 void staging_release();
 template <class T>
 void upload(DevBuf<T> &dst, const T *src, size_t count, hipStream_t s) {

@@ -221,8 +221,25 @@ hipError_t launch_sub_mean(float *x, uint64_t n, uint32_t d, const float *mean, 
 hipError_t launch_colmean_f64(const float *x, uint64_t n, uint32_t d, double *mean,
                               double *scratch, hipStream_t s);
 uint64_t cov_scratch_doubles(uint64_t n, uint32_t d);
-hipError_t launch_cov_f64(const float *x, uint64_t n, uint32_t d, const double *mean,
-                          double *scratch, double *cov, hipStream_t s);
+// PCA statistics by row chunks of kCovChunkBytes (the upload pieces of a
+// host X): chunks of `rows` rows, `per` covariance and `col_per` column-sum
+// splits each.  launch_pca_stats_chunks sums chunks k0 .. k0 + nk - 1 (the
+// first call must include chunk 0: it sets the shift); launch_pca_stats_finish
+// then gives the FP64 mean and the covariance sum about it.  Launching every
+// chunk at once or one chunk at a time gives the same bits.
+constexpr uint64_t kCovChunkBytes = uint64_t{32} << 20;
+struct CovChunks {
+  uint64_t rows = 0;
+  uint32_t chunks = 0, per = 0, col_per = 0;
+};
+CovChunks cov_chunks(uint64_t n, uint32_t d);
+uint64_t cov_chunk_scratch_doubles(const CovChunks &c, uint32_t d);
+uint64_t colsum_chunk_scratch_doubles(const CovChunks &c, uint32_t d);
+hipError_t launch_pca_stats_chunks(const float *x, uint64_t n, uint32_t d, const CovChunks &c, uint32_t k0,
+                                   uint32_t nk, double *colpart, double *shift, double *covpart, hipStream_t s);
+hipError_t launch_pca_stats_finish(uint64_t n, uint32_t d, const CovChunks &c, const double *colpart,
+                                   const double *shift, const double *covpart, double *mean, double *cov,
+                                   hipStream_t s);
 // out (n x q) = (x - mean) V; the workspace (project_workspace_floats) holds
 // V and the mean packed in the order the FP32-MFMA kernel stages them
 uint64_t project_workspace_floats(uint32_t d, uint32_t q);

@@ -1574,6 +1574,33 @@ __global__ void colsum_part_kernel(const float *__restrict__ x, uint64_t n, uint
   part[static_cast<size_t>(sp) * d + c] = s;
 }
 
+// Row chunks of the PCA statistics (the rows of one 32 MiB upload piece):
+// chunk k = k0 + blockIdx.y / per holds rows [k cr, min(n, (k + 1) cr)), and
+// split sp = blockIdx.y % per of it a fixed share of those rows; partial
+// slot (k per + sp).  One launch over every chunk (X in device memory) and
+// one launch per chunk as its upload lands give the same sums.
+__device__ __forceinline__ void chunk_split_rows(uint64_t n, uint64_t cr, uint32_t k0, uint32_t per, uint64_t &r0,
+                                                 uint64_t &r1, uint32_t &slot) {
+  const uint32_t k = k0 + blockIdx.y / per, sp = blockIdx.y % per;
+  const uint64_t c0 = static_cast<uint64_t>(k) * cr;
+  const uint64_t rows = c0 < n ? (n - c0 < cr ? n - c0 : cr) : 0;
+  r0 = c0 + rows * sp / per;
+  r1 = c0 + rows * (sp + 1) / per;
+  slot = k * per + sp;
+}
+
+__global__ void colsum_chunk_kernel(const float *__restrict__ x, uint64_t n, uint32_t d, uint64_t cr, uint32_t k0,
+                                    uint32_t per, double *__restrict__ part) {
+  const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= d) return;
+  uint64_t r0, r1;
+  uint32_t slot;
+  chunk_split_rows(n, cr, k0, per, r0, r1, slot);
+  double s = 0.0;
+  for (uint64_t r = r0; r < r1; ++r) s += static_cast<double>(x[r * d + c]);
+  part[static_cast<size_t>(slot) * d + c] = s;
+}
+
 // 16 columns per block; 16 thread groups stride the splits, then a fixed-order
 // LDS sum over the groups (deterministic).
 __global__ __launch_bounds__(256) void colmean_fin_kernel(const double *__restrict__ part, uint32_t d,
@@ -1616,9 +1643,10 @@ typedef double doublex4 __attribute__((ext_vector_type(4)));
 // products, whose lane l holds rows 4 (l / 16) + v).
 __global__ __launch_bounds__(256) void cov_part_kernel(const float *__restrict__ x, uint64_t n,
                                                        uint32_t d, const double *__restrict__ mean,
-                                                       uint32_t splits, double *__restrict__ part) {
+                                                       uint32_t splits, double *__restrict__ part,
+                                                       uint64_t cr, uint32_t k0) {
   // blockIdx.x enumerates the upper tiles (ti <= tj) row by row
-  const uint32_t tiles = (d + kCovTile - 1) / kCovTile, sp = blockIdx.y;
+  const uint32_t tiles = (d + kCovTile - 1) / kCovTile;
   uint32_t ti = 0, u = blockIdx.x;
   while (u >= tiles - ti) {
     u -= tiles - ti;
@@ -1631,7 +1659,10 @@ __global__ __launch_bounds__(256) void cov_part_kernel(const float *__restrict__
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
   const int wr = wave >> 1, wc = wave & 1;
   const uint32_t a0 = ti * kCovTile, b0 = tj * kCovTile;
-  const uint64_t r0 = n * sp / splits, r1 = n * (sp + 1) / splits;
+  // rows of this block's split of its chunk, and its partial slot
+  uint64_t r0, r1;
+  uint32_t sp;
+  chunk_split_rows(n, cr, k0, splits, r0, r1, sp);
   // staging: thread t moves column cc of rows rq + 4 f (f < 8) of a stage
   const int cc = t & 63, rq = t >> 6;
   const uint32_t ca = a0 + cc, cb = b0 + cc;
@@ -1694,28 +1725,70 @@ __global__ __launch_bounds__(256) void cov_part_kernel(const float *__restrict__
       }
 }
 
-// Sums the split partials of the upper triangle of tiles and writes both
-// (a, b) and (b, a); reads are coalesced along b.
-__global__ void cov_fin_kernel(const double *__restrict__ part, uint32_t d, uint32_t splits,
-                               double *__restrict__ cov) {
+// Chunked statistics (see chunk_split_rows).  The covariance is taken about
+// a shift (the first chunk's column mean) and corrected at the end:
+//   sum (x - mu)(x - mu)^T = sum (x - h)(x - h)^T - n (mu - h)(mu - h)^T,
+// which keeps the FP64 sums free of cancellation for data far from the
+// origin, and lets each chunk be summed before the others are in memory.
+CovChunks cov_chunks(uint64_t n, uint32_t d) {
+  CovChunks c;
+  c.rows = kCovChunkBytes / (static_cast<uint64_t>(d) * sizeof(float));
+  if (c.rows == 0) c.rows = 1;
+  c.chunks = static_cast<uint32_t>((n + c.rows - 1) / c.rows);
+  if (c.chunks == 0) c.chunks = 1;
+  const uint32_t total = cov_splits(n, d);
+  c.per = (total + c.chunks - 1) / c.chunks;
+  if (c.per < 1) c.per = 1;
+  c.col_per = colsum_splits(c.rows < n ? c.rows : n);
+  return c;
+}
+uint64_t cov_chunk_scratch_doubles(const CovChunks &c, uint32_t d) {
+  return static_cast<uint64_t>(c.chunks) * c.per * d * d;
+}
+uint64_t colsum_chunk_scratch_doubles(const CovChunks &c, uint32_t d) {
+  return static_cast<uint64_t>(c.chunks) * c.col_per * d;
+}
+
+__global__ void cov_fin_shift_kernel(const double *__restrict__ part, uint32_t d, uint32_t splits,
+                                     const double *__restrict__ mean, const double *__restrict__ shift, uint64_t n,
+                                     double *__restrict__ cov) {
   const uint64_t e = blockIdx.x * 256ull + threadIdx.x;
   if (e >= static_cast<uint64_t>(d) * d) return;
   const uint32_t a = static_cast<uint32_t>(e / d), b = static_cast<uint32_t>(e % d);
   if (b / kCovTile < a / kCovTile) return;  // lower tiles: written by their mirror
   double s = 0.0;
   for (uint32_t sp = 0; sp < splits; ++sp) s += part[static_cast<size_t>(sp) * d * d + e];
+  const double da = mean[a] - shift[a], db = mean[b] - shift[b];
+  s = s - static_cast<double>(n) * (da * db);
   cov[e] = s;
   if (b / kCovTile > a / kCovTile) cov[static_cast<uint64_t>(b) * d + a] = s;
 }
 
-hipError_t launch_cov_f64(const float *x, uint64_t n, uint32_t d, const double *mean,
-                          double *scratch, double *cov, hipStream_t s) {
-  const uint32_t splits = cov_splits(n, d);
+// Chunks k0 .. k0 + nk - 1: column-sum partials, then (k0 == 0) the shift
+// from the first chunk's sums, then covariance partials about the shift.
+hipError_t launch_pca_stats_chunks(const float *x, uint64_t n, uint32_t d, const CovChunks &c, uint32_t k0,
+                                   uint32_t nk, double *colpart, double *shift, double *covpart, hipStream_t s) {
+  if (nk == 0) return hipSuccess;
+  colsum_chunk_kernel<<<dim3((d + 63) / 64, nk * c.col_per), 64, 0, s>>>(x, n, d, c.rows, k0, c.col_per, colpart);
+  if (k0 == 0) {
+    const uint64_t rows0 = c.rows < n ? c.rows : n;
+    colmean_fin_kernel<<<(d + 15) / 16, 256, 0, s>>>(colpart, d, c.col_per, rows0, shift);
+  }
   const uint32_t tiles = (d + kCovTile - 1) / kCovTile;
-  dim3 g(tiles * (tiles + 1) / 2, splits);
-  cov_part_kernel<<<g, 256, 0, s>>>(x, n, d, mean, splits, scratch);
+  cov_part_kernel<<<dim3(tiles * (tiles + 1) / 2, nk * c.per), 256, 0, s>>>(x, n, d, shift, c.per, covpart,
+                                                                                   c.rows, k0);
+  return hipGetLastError();
+}
+
+// After every chunk: the mean from all column-sum partials and the
+// corrected covariance.
+hipError_t launch_pca_stats_finish(uint64_t n, uint32_t d, const CovChunks &c, const double *colpart,
+                                   const double *shift, const double *covpart, double *mean, double *cov,
+                                   hipStream_t s) {
+  colmean_fin_kernel<<<(d + 15) / 16, 256, 0, s>>>(colpart, d, c.chunks * c.col_per, n, mean);
   const uint64_t total = static_cast<uint64_t>(d) * d;
-  cov_fin_kernel<<<static_cast<uint32_t>((total + 255) / 256), 256, 0, s>>>(scratch, d, splits, cov);
+  cov_fin_shift_kernel<<<static_cast<uint32_t>((total + 255) / 256), 256, 0, s>>>(covpart, d, c.chunks * c.per, mean,
+                                                                                   shift, n, cov);
   return hipGetLastError();
 }
 

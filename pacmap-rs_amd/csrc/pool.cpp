@@ -10,6 +10,9 @@
 #include <mutex>
 #include <thread>
 
+#include <functional>
+#include <stdexcept>
+
 #include "host_common.h"
 
 namespace pm {
@@ -258,6 +261,7 @@ constexpr size_t kStageMin = size_t{8} << 20;     // smaller uploads go directly
 struct Staging {
   void *buf[2] = {nullptr, nullptr};
   hipEvent_t ev[2] = {nullptr, nullptr};
+  hipEvent_t order = nullptr;  // orders a copy stream after the calling stream
   ~Staging() {
     if (!process_exiting()) release();
   }
@@ -271,6 +275,8 @@ struct Staging {
       buf[b] = nullptr;
       ev[b] = nullptr;
     }
+    if (order) (void)hipEventDestroy(order);
+    order = nullptr;
   }
 };
 Staging &staging() {
@@ -302,6 +308,35 @@ void upload_bytes(void *dst, const void *src, size_t bytes, hipStream_t s) {
     CopyThreads::get().copy(st.buf[b], sp + off, len);
     PM_HIP(hipMemcpyAsync(dp + off, st.buf[b], len, hipMemcpyHostToDevice, s));
     PM_HIP(hipEventRecord(st.ev[b], s));
+  }
+}
+
+void upload_rows_pieces(void *dst, const void *src, uint64_t rows, size_t row_bytes, uint64_t piece_rows,
+                        hipStream_t s, hipStream_t cs, const std::function<void(uint32_t, hipEvent_t)> &on_piece) {
+  if (rows == 0) return;
+  if (piece_rows == 0 || piece_rows * row_bytes > kStageChunk) throw std::invalid_argument("upload piece too large");
+  Staging &st = staging();
+  for (int b = 0; b < 2; ++b) {
+    if (!st.buf[b]) PM_HIP(hipHostMalloc(&st.buf[b], kStageChunk, 0));
+    if (!st.ev[b]) PM_HIP(hipEventCreateWithFlags(&st.ev[b], hipEventDisableTiming));
+  }
+  // the copy stream starts after the work already queued on s (dst may be a
+  // block that work still reads)
+  if (!st.order) PM_HIP(hipEventCreateWithFlags(&st.order, hipEventDisableTiming));
+  PM_HIP(hipEventRecord(st.order, s));
+  PM_HIP(hipStreamWaitEvent(cs, st.order, 0));
+  const char *sp = static_cast<const char *>(src);
+  char *dp = static_cast<char *>(dst);
+  const uint64_t pieces = (rows + piece_rows - 1) / piece_rows;
+  for (uint64_t k = 0; k < pieces; ++k) {
+    const int b = static_cast<int>(k & 1);
+    const size_t off = static_cast<size_t>(k * piece_rows) * row_bytes;
+    const size_t len = static_cast<size_t>(std::min(piece_rows, rows - k * piece_rows)) * row_bytes;
+    PM_HIP(hipEventSynchronize(st.ev[b]));  // the last DMA out of this buffer is done
+    CopyThreads::get().copy(st.buf[b], sp + off, len);
+    PM_HIP(hipMemcpyAsync(dp + off, st.buf[b], len, hipMemcpyHostToDevice, cs));
+    PM_HIP(hipEventRecord(st.ev[b], cs));
+    on_piece(static_cast<uint32_t>(k), st.ev[b]);
   }
 }
 

@@ -909,6 +909,31 @@ def test_pca_initialisation_matches_fp64_eigensolve(n, d):
     assert err < 1e-4
 
 
+def test_pca_host_upload_in_chunks_matches_device_input():
+    """A host X on the PCA path is uploaded in 32 MiB row chunks whose FP64
+    sums run while the next chunk is in flight; X already in device memory
+    takes the same chunks in one launch.  The PCA initialisation, the pairs
+    and the embedding must be the same bits either way (here 3 chunks, the
+    last one short), and the axes must still be the FP64 ones, also for data
+    far from the origin (the sums are taken about the first chunk's mean)."""
+    rng = np.random.default_rng(77)
+    n, d = 60000, 320  # 32 MiB holds 26,214 rows of 320 floats
+    basis, _ = np.linalg.qr(rng.standard_normal((d, d)))
+    x = ((rng.standard_normal((n, d)) * (10.0 * 0.85 ** np.arange(d))) @ basis.T + 1000.0).astype(np.float32)
+    cfg = pm.Configuration(seed=3, num_iters=(0, 0, 0))
+    y_host, _ = pm.fit_transform(x, cfg)
+    y_dev, _ = pm.fit_transform(pm.DeviceArray(x), cfg)
+    assert np.array_equal(y_host.view(np.uint32), y_dev.view(np.uint32))
+    exp, _ = _pca_init_expected(x, 100)
+    err = np.abs(y_host - exp).max() / np.abs(exp).max()
+    print(f"chunked PCA init (mean 1000): max error {err:.2e}", flush=True)
+    assert err < 1e-4
+    cfg = pm.Configuration(seed=3, num_iters=(2, 2, 2))
+    y_host, _ = pm.fit_transform(x, cfg)
+    y_dev, _ = pm.fit_transform(pm.DeviceArray(x), cfg)
+    assert np.array_equal(y_host.view(np.uint32), y_dev.view(np.uint32))
+
+
 def test_pca_initialisation_rank_deficient():
     """n < d: the covariance has rank n - 1 < 100 requested axes; the
     orthonormalisation takes its shifted retry and the leading axes are still
