@@ -85,7 +85,7 @@ bool knn_mfma_path(const float *x_dev, uint64_t n, uint64_t d, uint32_t k, const
   const uint32_t kch = knn_chunks(d, mode);
   const uint64_t ntiles = (n + 15) / 16;
   const uint64_t rows_padded = ntiles * 16;
-  DevBuf<double> mean(d), scratch(cov_scratch_doubles(n, static_cast<uint32_t>(d)));
+  DevBuf<double> mean(d), scratch(colmean_scratch_doubles(n, static_cast<uint32_t>(d)));
   PM_HIP(launch_colmean_f64(x_dev, n, static_cast<uint32_t>(d), mean.get(), scratch.get(), s));
   const size_t frag_words = ntiles * kch * 64 * 4;  // 16 B per lane fragment
   DevBuf<uint32_t> fhi(frag_words), flo(mode != 0 ? frag_words : 4);

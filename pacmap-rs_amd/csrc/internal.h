@@ -220,7 +220,7 @@ hipError_t launch_sub_mean(float *x, uint64_t n, uint32_t d, const float *mean, 
 // PCA pieces (FP64 covariance, FP32 projection)
 hipError_t launch_colmean_f64(const float *x, uint64_t n, uint32_t d, double *mean,
                               double *scratch, hipStream_t s);
-uint64_t cov_scratch_doubles(uint64_t n, uint32_t d);
+uint64_t colmean_scratch_doubles(uint64_t n, uint32_t d);
 // PCA statistics by row chunks of kCovChunkBytes (the upload pieces of a
 // host X): chunks of `rows` rows, `per` covariance and `col_per` column-sum
 // splits each.  launch_pca_stats_chunks sums chunks k0 .. k0 + nk - 1 (the

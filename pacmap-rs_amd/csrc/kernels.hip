@@ -1531,22 +1531,10 @@ hipError_t launch_sub_mean(float *x, uint64_t n, uint32_t d, const float *mean, 
 // PCA initialisation pieces: FP64 column means and covariance, FP32 projection
 // ---------------------------------------------------------------------------
 constexpr uint32_t kCovTile = 64;
-constexpr uint32_t kCovRowsPerSplit = 2048;
-
-// Row splits of the covariance: about kCovRowsPerSplit rows each, rounded
-// so the (upper tile, split) blocks come closest to whole rounds of the
-// resident slots (256 CUs x 4 blocks of cov_part_kernel on MI355X); a
-// partly filled last round idles most of the chip for a whole block time.
+// Resident slots of cov_part_kernel on MI355X (256 CUs x 4 blocks): each
+// chunk's launch is given enough (upper tile, split) blocks to fill a round.
 constexpr uint64_t kCovSlots = 1024;
-static uint32_t cov_splits(uint64_t n, uint32_t d) {
-  const uint64_t tiles = (d + kCovTile - 1) / kCovTile, upper = tiles * (tiles + 1) / 2;
-  uint64_t s = (n + kCovRowsPerSplit - 1) / kCovRowsPerSplit;
-  const uint64_t rounds = (s * upper + kCovSlots / 2) / kCovSlots;
-  if (rounds >= 1 && rounds * kCovSlots >= upper) s = rounds * kCovSlots / upper;
-  if (s < 1) s = 1;
-  if (s > 64) s = 64;
-  return static_cast<uint32_t>(s);
-}
+constexpr uint64_t kCovChunkScratch = uint64_t{256} << 20;  // bytes of chunked covariance partials
 
 // row splits of the column sums: enough blocks to cover the chip
 static uint32_t colsum_splits(uint64_t n) {
@@ -1556,11 +1544,8 @@ static uint32_t colsum_splits(uint64_t n) {
   return static_cast<uint32_t>(s);
 }
 
-uint64_t cov_scratch_doubles(uint64_t n, uint32_t d) {
-  const uint64_t s = cov_splits(n, d);
-  const uint64_t cov = s * static_cast<uint64_t>(d) * d;
-  const uint64_t col = static_cast<uint64_t>(colsum_splits(n)) * d;
-  return cov > col ? cov : col;
+uint64_t colmean_scratch_doubles(uint64_t n, uint32_t d) {
+  return static_cast<uint64_t>(colsum_splits(n)) * d;
 }
 
 __global__ void colsum_part_kernel(const float *__restrict__ x, uint64_t n, uint32_t d,
@@ -1736,9 +1721,16 @@ CovChunks cov_chunks(uint64_t n, uint32_t d) {
   if (c.rows == 0) c.rows = 1;
   c.chunks = static_cast<uint32_t>((n + c.rows - 1) / c.rows);
   if (c.chunks == 0) c.chunks = 1;
-  const uint32_t total = cov_splits(n, d);
-  c.per = (total + c.chunks - 1) / c.chunks;
-  if (c.per < 1) c.per = 1;
+  // splits per chunk: enough (upper tile, split) blocks for a round of the
+  // resident slots in each chunk's launch, within kCovChunkScratch bytes of
+  // partials over all chunks
+  const uint64_t tiles = (d + kCovTile - 1) / kCovTile, upper = tiles * (tiles + 1) / 2;
+  const uint64_t want = (kCovSlots + upper - 1) / upper;
+  const uint64_t cap = kCovChunkScratch / (static_cast<uint64_t>(c.chunks) * d * d * sizeof(double));
+  uint64_t per = want < cap ? want : cap;
+  if (per < 1) per = 1;
+  if (per > 4096) per = 4096;
+  c.per = static_cast<uint32_t>(per);
   c.col_per = colsum_splits(c.rows < n ? c.rows : n);
   return c;
 }
@@ -1749,19 +1741,31 @@ uint64_t colsum_chunk_scratch_doubles(const CovChunks &c, uint32_t d) {
   return static_cast<uint64_t>(c.chunks) * c.col_per * d;
 }
 
-__global__ void cov_fin_shift_kernel(const double *__restrict__ part, uint32_t d, uint32_t splits,
-                                     const double *__restrict__ mean, const double *__restrict__ shift, uint64_t n,
-                                     double *__restrict__ cov) {
-  const uint64_t e = blockIdx.x * 256ull + threadIdx.x;
-  if (e >= static_cast<uint64_t>(d) * d) return;
+// 16 lanes per element: lane j sums slots j, j + 16, ... in order, then the
+// 16 sums are added in lane order (a fixed order for any slot count).
+__global__ __launch_bounds__(256) void cov_fin_shift_kernel(const double *__restrict__ part, uint32_t d,
+                                                            uint32_t splits, const double *__restrict__ mean,
+                                                            const double *__restrict__ shift, uint64_t n,
+                                                            double *__restrict__ cov) {
+  __shared__ double red[16][17];
+  const uint32_t j = threadIdx.x & 15, g = threadIdx.x >> 4;
+  const uint64_t e = blockIdx.x * 16ull + g;
+  const uint64_t dd = static_cast<uint64_t>(d) * d;
   const uint32_t a = static_cast<uint32_t>(e / d), b = static_cast<uint32_t>(e % d);
-  if (b / kCovTile < a / kCovTile) return;  // lower tiles: written by their mirror
+  // lower tiles: written by their mirror
+  const bool live = e < dd && b / kCovTile >= a / kCovTile;
   double s = 0.0;
-  for (uint32_t sp = 0; sp < splits; ++sp) s += part[static_cast<size_t>(sp) * d * d + e];
+  if (live)
+    for (uint32_t sp = j; sp < splits; sp += 16) s += part[static_cast<size_t>(sp) * dd + e];
+  red[g][j] = s;
+  __syncthreads();
+  if (!live || j != 0) return;
+  double t = 0.0;
+  for (int k = 0; k < 16; ++k) t += red[g][k];
   const double da = mean[a] - shift[a], db = mean[b] - shift[b];
-  s = s - static_cast<double>(n) * (da * db);
-  cov[e] = s;
-  if (b / kCovTile > a / kCovTile) cov[static_cast<uint64_t>(b) * d + a] = s;
+  t = t - static_cast<double>(n) * (da * db);
+  cov[e] = t;
+  if (b / kCovTile > a / kCovTile) cov[static_cast<uint64_t>(b) * d + a] = t;
 }
 
 // Chunks k0 .. k0 + nk - 1: column-sum partials, then (k0 == 0) the shift
@@ -1787,8 +1791,8 @@ hipError_t launch_pca_stats_finish(uint64_t n, uint32_t d, const CovChunks &c, c
                                    hipStream_t s) {
   colmean_fin_kernel<<<(d + 15) / 16, 256, 0, s>>>(colpart, d, c.chunks * c.col_per, n, mean);
   const uint64_t total = static_cast<uint64_t>(d) * d;
-  cov_fin_shift_kernel<<<static_cast<uint32_t>((total + 255) / 256), 256, 0, s>>>(covpart, d, c.chunks * c.per, mean,
-                                                                                   shift, n, cov);
+  cov_fin_shift_kernel<<<static_cast<uint32_t>((total + 15) / 16), 256, 0, s>>>(covpart, d, c.chunks * c.per, mean,
+                                                                                 shift, n, cov);
   return hipGetLastError();
 }
 
