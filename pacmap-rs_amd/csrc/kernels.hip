@@ -1281,16 +1281,21 @@ __global__ __launch_bounds__(kSellThreads) void grad_sell_kernel(const GradArgs 
   // two register sets in turn, so the prefetch needs no copies (and no wait
   // for the slice just requested)
   Slice qa, qb;
-  uint32_t sl = wave;
-  if (sl <= last_sl) fetch(sl, qa);
-  while (sl <= last_sl) {
-    fetch(sl + kStride, qb);
+  // A chunk's slices are sorted by length, longest first; turn k of wave w
+  // takes slice 16 k + w on even turns and 16 k + 15 - w on odd ones, so
+  // the waves that draw the longest slice of one turn draw the shortest of
+  // the next.  (Which wave computes a slice does not change its sums.)
+  auto slice_of = [&](uint32_t k) { return k * kStride + ((k & 1u) ? kStride - 1u - wave : wave); };
+  uint32_t turn = 0;
+  if (slice_of(0) <= last_sl) fetch(slice_of(0), qa);
+  while (slice_of(turn) <= last_sl) {
+    fetch(slice_of(turn + 1), qb);
     compute(qa);
-    sl += kStride;
-    if (sl > last_sl) break;
-    fetch(sl + kStride, qa);
+    ++turn;
+    if (slice_of(turn) > last_sl) break;
+    fetch(slice_of(turn + 1), qa);
     compute(qb);
-    sl += kStride;
+    ++turn;
   }
   if (LOSS) {
 #pragma unroll
