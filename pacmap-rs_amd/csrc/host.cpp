@@ -114,6 +114,14 @@ bool knn_mfma_path(const float *x_dev, uint64_t n, uint64_t d, uint32_t k, const
   // both rows) unless PACMAP_KNN_SYM=0
   bool sym = true;
   if (const char *e = std::getenv("PACMAP_KNN_SYM")) sym = std::atoi(e) != 0;
+  // candidate windows of at most 2 MiB of fragments (half an XCD's L2)
+  KnnSymPlan splan;
+  if (sym) {
+    const uint32_t tile_bytes = kch * (mode != 0 ? 2u : 1u) * 1024u;
+    splan = knn_sym_plan(static_cast<uint32_t>(ntiles), std::max<uint32_t>(4096, 4 * groups),
+                         std::max<uint32_t>(16, (2u << 20) / tile_bytes));
+    sym = splan.items != 0;
+  }
   const uint32_t nsplit =
       sym ? 1u : std::min<uint32_t>(16, std::max<uint32_t>(1, (4096 + groups - 1) / groups));
   DevBuf<uint32_t> gcnt(n * nsplit), gcand(n * nsplit * capg * (split ? 2 : 1)), fb_list(n);
@@ -122,20 +130,13 @@ bool knn_mfma_path(const float *x_dev, uint64_t n, uint64_t d, uint32_t k, const
   PM_HIP(launch_fill_zero(kstats.get(), kstats.bytes(), s));
   const void *flo_p = mode != 0 ? flo.get() : fhi.get();
   const float *rinv_p = mode == 2 ? rinv.get() : nullptr;
-  std::vector<uint32_t> items;
-  DevBuf<uint32_t> ditems;
   // A wave's epilogue tests 2 x 32 x 16 pairs of a tile pair; with about
   // 3.7 k listed candidates per row (k = 60: 222), it lists nothing with
   // probability exp(-1024 x 3.7 k / n): below 1/e past n = 3,800 k the
   // kernel first tests whether anything passes (70,000 points: 4 %,
   // 10^6 points: 80 % of the epilogues end there).
   const bool sparse = static_cast<double>(n) > 3800.0 * k;
-  if (sym) {
-    items = knn_sym_items(static_cast<uint32_t>(ntiles), std::max<uint32_t>(4096, 4 * groups));
-    ditems.alloc(items.size());
-    upload(ditems, items.data(), items.size(), s);
-    PM_HIP(launch_fill_zero(gcnt.get(), gcnt.bytes(), s));
-  }
+  if (sym) PM_HIP(launch_fill_zero(gcnt.get(), gcnt.bytes(), s));
   Timer tsamp, tk, tr;
   tsamp.start(s);
   PM_HIP(launch_knn_cand(0, fhi.get(), flo_p, arow.get(), rinv_p, static_cast<uint32_t>(n), kch,
@@ -145,8 +146,7 @@ bool knn_mfma_path(const float *x_dev, uint64_t n, uint64_t d, uint32_t k, const
   tk.start(s);
   PM_HIP(launch_knn_cand(sym ? 2 : 1, fhi.get(), flo_p, arow.get(), rinv_p, static_cast<uint32_t>(n), kch,
                          static_cast<uint32_t>(ntiles), ss, capg, nsplit, gcnt.get(), gcand.get(),
-                         gtau.get(), mode, s, sym ? ditems.get() : nullptr,
-                         static_cast<uint32_t>(items.size() / 3), sparse));
+                         gtau.get(), mode, s, splan, sparse));
   tk.stop(s);
   tr.start(s);
   // upper-bound pruning (certified split lists): see knn_rerank_kernel

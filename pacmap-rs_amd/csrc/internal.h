@@ -80,17 +80,23 @@ hipError_t launch_knn_rownorm(const float *x, uint64_t n, uint32_t d, const doub
 // ways.  Phase 2 (symmetric) computes each tile pair of the upper triangle
 // once and appends to both rows' lists: gcnt n counters (zeroed first),
 // gcand n x capg entries (nsplit = 1), one workgroup per item of
-// knn_sym_items (3 words each: group, first and end candidate tile).
+// knn_sym_plan.
 // sparse: most tile pairs list nothing (large n); a wave-wide test of the
 // whole tile then ends their epilogue early.
 // mode 0: bf16, 1: split bf16, 2: FP8 E4M3 with per-row scales (rinv).
 uint32_t knn_cand_groups(uint32_t ntiles);
-std::vector<uint32_t> knn_sym_items(uint32_t ntiles, uint32_t target_items);
+struct KnnSymPlan {
+  uint32_t gt = 0;     // query tiles per group
+  uint32_t L = 0;      // candidate tiles per window (m gt)
+  uint32_t m = 0;
+  uint32_t W = 0;      // windows
+  uint32_t items = 0;  // workgroups (0: no plan)
+};
+KnnSymPlan knn_sym_plan(uint32_t ntiles, uint32_t target_items, uint32_t max_window);
 hipError_t launch_knn_cand(int phase, const void *fhi, const void *flo, const float *arow, const float *rinv,
                            uint32_t n, uint32_t kch, uint32_t ntiles, uint32_t ss, uint32_t capg,
                            uint32_t nsplit, uint32_t *gcnt, uint32_t *gcand, float *gtau,
-                           int mode, hipStream_t s, const uint32_t *items = nullptr, uint32_t nitems = 0,
-                           bool sparse = false);
+                           int mode, hipStream_t s, const KnnSymPlan &sp = KnnSymPlan{}, bool sparse = false);
 // cert_scale < 0: accept every row with at least k listed candidates (no
 // certificate).  dots: the lists hold (candidate, dot) pairs (mode 1); with
 // ub_s1 > 0 as well, candidates whose lower bound exceeds the k-th smallest

@@ -239,6 +239,30 @@ __global__ void knn_rownorm_kernel(const float *__restrict__ x, uint64_t n, uint
   if (bad) atomicOr(nonfinite, 1u);
 }
 
+// Item b of the symmetric pass (see knn_sym_plan): windows of L = m gt
+// candidate tiles (gt = 4 kCandRT tiles per query group); window j < W - 1
+// holds the (j + 1) m groups whose first tile lies before its end, the last
+// window all G groups, so m j (j + 1) / 2 items precede window j.  Group g
+// of window j scans tiles [max(j L, g gt), min((j + 1) L, ntiles)).
+__device__ __forceinline__ void knn_sym_item(const KnnSymPlan &p, uint32_t ntiles, uint32_t b, uint32_t &g,
+                                             uint32_t &lo, uint32_t &hi) {
+  const uint64_t m = p.m, last = m * (p.W - 1) * p.W / 2;
+  uint64_t j;
+  if (b >= last) {
+    j = p.W - 1;
+    g = static_cast<uint32_t>(b - last);
+  } else {
+    j = static_cast<uint64_t>((sqrt(8.0 * static_cast<double>(b) / static_cast<double>(m) + 1.0) - 1.0) * 0.5);
+    while (m * (j + 1) * (j + 2) / 2 <= b) ++j;
+    while (m * j * (j + 1) / 2 > b) --j;
+    g = static_cast<uint32_t>(b - m * j * (j + 1) / 2);
+  }
+  const uint64_t w0 = j * p.L, w1 = min(w0 + p.L, static_cast<uint64_t>(ntiles));
+  const uint64_t first = static_cast<uint64_t>(g) * p.gt;
+  lo = static_cast<uint32_t>(w0 > first ? w0 : first);
+  hi = static_cast<uint32_t>(w1);
+}
+
 // ---------------------------------------------------------------------------
 // candidate kernel
 // ---------------------------------------------------------------------------
@@ -275,7 +299,7 @@ void knn_mfma_kernel(
     const uint4 *__restrict__ fhi, const uint4 *__restrict__ flo, const float *__restrict__ arow,
     const float *__restrict__ rinv, uint32_t n, uint32_t ntiles, uint32_t ss, uint32_t capg,
     uint32_t *__restrict__ gcnt, uint32_t *__restrict__ gcand, float *__restrict__ gtau,
-    const uint32_t *__restrict__ items, uint32_t sparse) {
+    KnnSymPlan sp, uint32_t sparse) {
   constexpr bool SPLIT = MODE == 1;
   constexpr bool FP8 = MODE == 2;
   constexpr int CT = (8 / KCH) > 0 ? (8 / KCH) : 1;  // candidate tiles per LDS stage
@@ -293,7 +317,8 @@ void knn_mfma_kernel(
   __shared__ uint32_t sApp[PHASE == 2 ? 4 : 1][3][PHASE == 2 ? kApp : 1];
 
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const uint32_t group = PHASE == 2 ? items[3 * blockIdx.x] : blockIdx.x;
+  uint32_t group = blockIdx.x, item_lo = 0, item_hi = 0;
+  if (PHASE == 2) knn_sym_item(sp, ntiles, blockIdx.x, group, item_lo, item_hi);
   const uint32_t gtile0 = group * 4 * RT;  // the group's first query tile
   const uint32_t qtile0 = (group * 4 + wave) * RT;
   const int rsub = lane >> 4, col = lane & 15;
@@ -344,10 +369,10 @@ void knn_mfma_kernel(
   // or the item's range (phase 2)
   const uint32_t nsample = (ntiles + ss - 1) / ss;
   const uint32_t slot_lo = PHASE == 0   ? 0
-                           : PHASE == 2 ? items[3 * blockIdx.x + 1]
+                           : PHASE == 2 ? item_lo
                                         : static_cast<uint32_t>(static_cast<uint64_t>(ntiles) * split / nsplit);
   const uint32_t slot_hi = PHASE == 0   ? nsample
-                           : PHASE == 2 ? items[3 * blockIdx.x + 2]
+                           : PHASE == 2 ? item_hi
                                         : static_cast<uint32_t>(static_cast<uint64_t>(ntiles) * (split + 1) / nsplit);
   auto slot_tile = [&](uint32_t slot) -> uint32_t { return PHASE == 0 ? slot * ss : slot; };
   const uint32_t nstage = (slot_hi - slot_lo + CT - 1) / CT;
@@ -993,18 +1018,18 @@ template <int KCH, int MODE>
 static hipError_t cand_dispatch(int phase, const void *fhi, const void *flo, const float *arow,
                                 const float *rinv, uint32_t n, uint32_t ntiles, uint32_t ss, uint32_t capg,
                                 uint32_t nsplit, uint32_t *gcnt, uint32_t *gcand, float *gtau,
-                                const uint32_t *items, uint32_t nitems, uint32_t sparse, hipStream_t s) {
+                                const KnnSymPlan &sp, uint32_t sparse, hipStream_t s) {
   const uint32_t groups = knn_cand_groups(ntiles);
   const uint4 *h = static_cast<const uint4 *>(fhi), *l = static_cast<const uint4 *>(flo);
   if (phase == 0)
     knn_mfma_kernel<KCH, kCandRT, MODE, 0><<<dim3(groups, 1), 256, 0, s>>>(h, l, arow, rinv, n, ntiles, ss,
-                                                                          capg, gcnt, gcand, gtau, nullptr, 0u);
+                                                                          capg, gcnt, gcand, gtau, KnnSymPlan{}, 0u);
   else if (phase == 1)
     knn_mfma_kernel<KCH, kCandRT, MODE, 1><<<dim3(groups, nsplit), 256, 0, s>>>(
-        h, l, arow, rinv, n, ntiles, ss, capg, gcnt, gcand, gtau, nullptr, 0u);
-  else if (nitems)
-    knn_mfma_kernel<KCH, kCandRT, MODE, 2><<<dim3(nitems, 1), 256, 0, s>>>(
-        h, l, arow, rinv, n, ntiles, ss, capg, gcnt, gcand, gtau, items, sparse);
+        h, l, arow, rinv, n, ntiles, ss, capg, gcnt, gcand, gtau, KnnSymPlan{}, 0u);
+  else if (sp.items)
+    knn_mfma_kernel<KCH, kCandRT, MODE, 2><<<dim3(sp.items, 1), 256, 0, s>>>(
+        h, l, arow, rinv, n, ntiles, ss, capg, gcnt, gcand, gtau, sp, sparse);
   return hipGetLastError();
 }
 
@@ -1012,44 +1037,50 @@ template <int KCH>
 static hipError_t cand_dispatch_mode(int mode, int phase, const void *fhi, const void *flo,
                                      const float *arow, const float *rinv, uint32_t n, uint32_t ntiles,
                                      uint32_t ss, uint32_t capg, uint32_t nsplit, uint32_t *gcnt,
-                                     uint32_t *gcand, float *gtau, const uint32_t *items, uint32_t nitems,
+                                     uint32_t *gcand, float *gtau, const KnnSymPlan &sp,
                                      uint32_t sparse, hipStream_t s) {
   switch (mode) {
-    case 0: return cand_dispatch<KCH, 0>(phase, fhi, flo, arow, rinv, n, ntiles, ss, capg, nsplit, gcnt, gcand, gtau, items, nitems, sparse, s);
-    case 1: return cand_dispatch<KCH, 1>(phase, fhi, flo, arow, rinv, n, ntiles, ss, capg, nsplit, gcnt, gcand, gtau, items, nitems, sparse, s);
-    case 2: return cand_dispatch<KCH, 2>(phase, fhi, flo, arow, rinv, n, ntiles, ss, capg, nsplit, gcnt, gcand, gtau, items, nitems, sparse, s);
+    case 0: return cand_dispatch<KCH, 0>(phase, fhi, flo, arow, rinv, n, ntiles, ss, capg, nsplit, gcnt, gcand, gtau, sp, sparse, s);
+    case 1: return cand_dispatch<KCH, 1>(phase, fhi, flo, arow, rinv, n, ntiles, ss, capg, nsplit, gcnt, gcand, gtau, sp, sparse, s);
+    case 2: return cand_dispatch<KCH, 2>(phase, fhi, flo, arow, rinv, n, ntiles, ss, capg, nsplit, gcnt, gcand, gtau, sp, sparse, s);
     default: return hipErrorInvalidValue;
   }
 }
 
-// Upper-triangle items of the symmetric candidate pass: group g (4 kCandRT
-// query tiles) scans candidate tiles [4 kCandRT g, ntiles) in pieces of
-// about the same number of tiles, so every workgroup has nearly equal work.
-std::vector<uint32_t> knn_sym_items(uint32_t ntiles, uint32_t target_items) {
-  const uint32_t groups = knn_cand_groups(ntiles), gt = 4 * kCandRT;
+// Plan of the symmetric pass: the candidate tiles are cut into windows of
+// L tiles (a multiple of the query group's gt tiles); group g takes, in
+// each window that reaches past its own first tile, the window's tiles from
+// that first tile on.  Items run window by window, so the workgroups
+// resident together stream the same candidate tiles and share them in L2.
+// L gives about target_items items and is at most max_window tiles.
+KnnSymPlan knn_sym_plan(uint32_t ntiles, uint32_t target_items, uint32_t max_window) {
+  KnnSymPlan p;
+  p.gt = 4 * kCandRT;
+  const uint64_t groups = (ntiles + p.gt - 1) / p.gt;
   uint64_t tri = 0;
-  for (uint32_t g = 0; g < groups; ++g) tri += ntiles - g * gt;
+  for (uint64_t g = 0; g < groups; ++g) tri += ntiles - g * p.gt;
   uint64_t len = (tri + target_items - 1) / target_items;
-  len = std::max<uint64_t>(len, 2 * gt);
-  len = (len + 1) & ~uint64_t{1};  // whole stages of two candidate tiles
-  std::vector<uint32_t> it;
-  for (uint32_t g = 0; g < groups; ++g)
-    for (uint64_t lo = static_cast<uint64_t>(g) * gt; lo < ntiles; lo += len) {
-      it.push_back(g);
-      it.push_back(static_cast<uint32_t>(lo));
-      it.push_back(static_cast<uint32_t>(std::min<uint64_t>(lo + len, ntiles)));
-    }
-  return it;
+  len = std::min<uint64_t>(len, max_window);
+  len = std::max<uint64_t>(len / p.gt, 2) * p.gt;
+  p.L = static_cast<uint32_t>(len);
+  p.m = p.L / p.gt;
+  p.W = static_cast<uint32_t>((ntiles + len - 1) / len);
+  if (ntiles == 0) p.W = 0;
+  const uint64_t items = p.W ? static_cast<uint64_t>(p.m) * (p.W - 1) * p.W / 2 + groups : 0;
+  p.items = items < (1ull << 31) ? static_cast<uint32_t>(items) : 0;  // 0: the caller takes the split pass
+  return p;
 }
 
 hipError_t launch_knn_cand(int phase, const void *fhi, const void *flo, const float *arow, const float *rinv,
                            uint32_t n, uint32_t kch, uint32_t ntiles, uint32_t ss, uint32_t capg,
                            uint32_t nsplit, uint32_t *gcnt, uint32_t *gcand, float *gtau,
-                           int mode, hipStream_t s, const uint32_t *items, uint32_t nitems, bool sparse) {
+                           int mode, hipStream_t s, const KnnSymPlan &sp, bool sparse) {
   if (mode == 2 && rinv == nullptr) return hipErrorInvalidValue;
-  if (phase == 2 && nitems && items == nullptr) return hipErrorInvalidValue;
+  if (phase == 2 && sp.items && (sp.L == 0 || sp.L % sp.gt != 0 || sp.m * sp.gt != sp.L || sp.W == 0 ||
+                                 sp.gt != 4 * kCandRT))
+    return hipErrorInvalidValue;
 #define PM_CAND(K) \
-  case K: return cand_dispatch_mode<K>(mode, phase, fhi, flo, arow, rinv, n, ntiles, ss, capg, nsplit, gcnt, gcand, gtau, items, nitems, sparse, s);
+  case K: return cand_dispatch_mode<K>(mode, phase, fhi, flo, arow, rinv, n, ntiles, ss, capg, nsplit, gcnt, gcand, gtau, sp, sparse, s);
   switch (kch) {
     PM_CAND(1)
     PM_CAND(2)
