@@ -35,10 +35,11 @@ HBM_PEAK_GBS = 8000.0
 # profiles/pmc_traffic_70000x784.md.  Keys: (workload, kernel).
 MEASURED_TRAFFIC = {
     ("70k", "grad"): 31_290_000,               # FETCH 14,387.8 KiB x 2.0 + WRITE 1,777.4 KiB
-    # grad_sell_kernel FETCH 14,983.1 KiB x 2.0 + WRITE 2,206.0 KiB, plus
-    # grad_combine_elem_kernel FETCH 1,928.6 KiB x 2.0 + WRITE 1,640.6 KiB
-    ("70k", "grad_sell"): 38_570_000,
-    ("70k", "knn_mfma_kernel"): 1_418_460_000,  # FETCH 459,788.2 KiB x 2.0 + WRITE 465,641.2 KiB
+    # grad_sell_kernel FETCH 14,983.5 KiB x 2.0 + WRITE 2,213.4 KiB, plus
+    # grad_combine_elem_kernel FETCH 1,927.1 KiB x 2.0 + WRITE 1,640.6 KiB
+    ("70k", "grad_sell"): 38_580_000,
+    # symmetric candidate pass: FETCH 666,107.8 KiB x 2.0 + WRITE 694,046.9 KiB
+    ("70k", "knn_mfma_kernel"): 2_074_890_000,
 }
 FP32_PEAK_TFLOPS = 157.3
 BF16_PEAK_TFLOPS = 2500.0
@@ -475,7 +476,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU design is "
-              "described in DESIGN.md and is not measured", file=sys.stderr)
+              "not measured here", file=sys.stderr)
         sys.exit(2)
     import pacmap_mi355x as pm
 
@@ -549,8 +550,9 @@ def main():
         kname, kkey, peak = ("knn_exact_kernel (direct FP32 in the reference's distance order)",
                              "knn_exact_kernel", FP32_PEAK_TFLOPS)
     else:
-        kname = ("knn_mfma_kernel phase 1 (split-bf16 MFMA candidates: 3 bf16 MFMAs per "
-                 "algorithmic multiply-add)" if path == 1 else "knn_mfma_kernel phase 1 (bf16 MFMA candidates)")
+        kname = ("knn_mfma_kernel symmetric pass (split-bf16 MFMA candidates: each pair of the upper "
+                 "triangle once, 3 bf16 MFMAs per multiply-add)" if path == 1
+                 else "knn_mfma_kernel symmetric pass (bf16 MFMA candidates)")
         kkey, peak = "knn_mfma_kernel", BF16_PEAK_TFLOPS
     opt_ms = float(np.mean([t["optimize_ms"] for t in tms]))
     pairs = n * (k_nn + n_mn + n_fp)
