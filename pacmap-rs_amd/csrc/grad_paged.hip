@@ -184,20 +184,28 @@ __global__ __launch_bounds__(kPageBuildThreads) void page_layout_kernel(const ui
     uint32_t plo, phi;
     page_bounds(p, n, b, pg, plo, phi);
     // entries of each row below the page's end (the count pass moves the
-    // cursors here; the fill pass moves them as it writes)
+    // cursors here; the fill pass moves them as it writes).  A row meets a
+    // page about 0.27 times on average, so the word at each run's cursor is
+    // usually already past the page: the six runs' words are loaded
+    // together and a run is followed further only when its word is inside.
     for (uint32_t i = t; i < rows; i += kPageBuildThreads) {
       const uint32_t beg = sBeg[i], len = sLen[i];
+      uint32_t k[kRuns], w[kRuns];
+#pragma unroll
+      for (int r = 0; r < kRuns; ++r) {
+        k[r] = sCur[r][i];
+        w[r] = k[r] < len ? tb_ent[beg + k[r]] : 0xffffffffu;
+      }
       uint32_t cnt = 0;
 #pragma unroll
       for (int r = 0; r < kRuns; ++r) {
-        uint32_t k = sCur[r][i];
-        const uint32_t k0 = k;
-        for (; k < len; ++k) {
-          const uint32_t w = tb_ent[beg + k];
-          if ((w >> 29) != static_cast<uint32_t>(r) || (w & kIdxMask) >= phi) break;
+        const uint32_t k0 = k[r];
+        while (k[r] < len && (w[r] >> 29) == static_cast<uint32_t>(r) && (w[r] & kIdxMask) < phi) {
+          ++k[r];
+          w[r] = k[r] < len ? tb_ent[beg + k[r]] : 0xffffffffu;
         }
-        cnt += k - k0;
-        if (!FILL) sCur[r][i] = static_cast<uint16_t>(k);
+        cnt += k[r] - k0;
+        if (!FILL) sCur[r][i] = static_cast<uint16_t>(k[r]);
       }
       sCnt[i] = static_cast<uint16_t>(cnt);
     }
@@ -300,18 +308,24 @@ __global__ __launch_bounds__(kPageBuildThreads) void page_layout_kernel(const ui
         const uint32_t rf = (cnt ? i : p.rc) << kPageRowShift;
         uint32_t k = 0;
         if (cnt) {
+          // as in the count: the six runs' cursor words loaded together
           const uint32_t beg = sBeg[i], len = sLen[i];
+          uint32_t e[kRuns], w[kRuns];
 #pragma unroll
           for (int r = 0; r < kRuns; ++r) {
-            uint32_t e = sCur[r][i];
-            for (; e < len; ++e) {
-              const uint32_t w = tb_ent[beg + e];
-              const uint32_t o = w & kIdxMask;
-              if ((w >> 29) != static_cast<uint32_t>(r) || o >= phi) break;
-              dst[64u * k] = (w & ~kIdxMask) | rf | (o - plo) | (k + 1 == ws ? kPageLast : 0u);
+            e[r] = sCur[r][i];
+            w[r] = e[r] < len ? tb_ent[beg + e[r]] : 0xffffffffu;
+          }
+#pragma unroll
+          for (int r = 0; r < kRuns; ++r) {
+            while (e[r] < len && (w[r] >> 29) == static_cast<uint32_t>(r) && (w[r] & kIdxMask) < phi) {
+              const uint32_t o = w[r] & kIdxMask;
+              dst[64u * k] = (w[r] & ~kIdxMask) | rf | (o - plo) | (k + 1 == ws ? kPageLast : 0u);
               ++k;
+              ++e[r];
+              w[r] = e[r] < len ? tb_ent[beg + e[r]] : 0xffffffffu;
             }
-            sCur[r][i] = static_cast<uint16_t>(e);
+            sCur[r][i] = static_cast<uint16_t>(e[r]);
           }
         }
         for (; k < ws; ++k) dst[64u * k] = (2u << 30) | rf | p.pp | (k + 1 == ws ? kPageLast : 0u);
