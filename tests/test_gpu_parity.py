@@ -199,6 +199,9 @@ def _data(kind, n, d, seed):
     if kind == "dups":
         base = rng.normal(size=(n // 4, d)).astype(np.float32)
         return np.repeat(base, 4, axis=0)[:n].copy()
+    if kind == "three":
+        base = rng.normal(size=(3, d)).astype(np.float32)
+        return base[rng.integers(0, 3, size=n)].copy()
     raise ValueError(kind)
 
 
@@ -221,6 +224,9 @@ MFMA_CASES = [
     (2600, 33, 100, "normal", 104),
     (3000, 12, 30, "dups", 105),
     (4000, 4, 60, "grid", 106),
+    # three distinct points: a tile epilogue lists up to 1,024 pairs, more
+    # than a wave's 128-entry staging area, so it flushes mid-epilogue
+    (2048, 8, 60, "three", 107),
 ]
 
 
@@ -237,6 +243,19 @@ def test_knn_mfma_paths_bitwise(n, d, k, kind, seed):
               f"kernel={t['knn_kernel_ms']:.3f} ms rerank={t['knn_rerank_ms']:.3f} ms", flush=True)
         assert np.array_equal(dist.view(np.uint32), ref_d.view(np.uint32)), mode
         assert np.array_equal(idx, ref_i), mode
+
+
+@pytest.mark.parametrize("n,d,k,kind,seed", [(4000, 100, 60, "normal", 121), (3000, 12, 30, "dups", 122)])
+def test_knn_split_pass_bitwise(monkeypatch, n, d, k, kind, seed):
+    """The candidate pass that scans every (query, candidate) pair in split
+    ranges (PACMAP_KNN_SYM=0) instead of the upper triangle once equals the
+    oracle bit for bit too."""
+    x = _data(kind, n, d, seed)
+    ref_i, ref_d = O.knn_rowwise(x, k, nthreads=16)
+    monkeypatch.setenv("PACMAP_KNN_SYM", "0")
+    idx, dist = pm.knn.find_k_nearest_neighbors(x, k)
+    assert np.array_equal(dist.view(np.uint32), ref_d.view(np.uint32))
+    assert np.array_equal(idx, ref_i)
 
 
 @pytest.mark.parametrize("n,d,k,seed", [(6000, 100, 60, 111), (5000, 256, 30, 112), (4000, 20, 10, 113)])
