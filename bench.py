@@ -460,6 +460,8 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="70k workload: skip the 10^6-point grad+Adam and the encoder measurements")
     ap.add_argument("--encoder-epochs", type=int, default=ENC["epochs"],
                     help="training epochs of the encoder workload")
     ap.add_argument("--encoder-no-plain", action="store_true",
@@ -626,7 +628,40 @@ def main():
     if not args.no_cpu_baseline:
         log("bench: timing the CPU oracle on a bounded sample")
         out["cpu_baseline"] = cpu_baseline(x, min(16, os.cpu_count() or 1))
+    if not large and not args.no_extras:
+        # The metric's other two parts, measured after the timed steps: the
+        # gradient + Adam rate of the 10^6-point optimiser, and the encoder
+        # against the plain optimiser on held-out points.
+        del xd
+        out["grad_adam_1m"] = extra(lambda: grad_extra(pm))
+        out["encoder_vs_plain"] = extra(lambda: encoder_extra(pm))
     print(json.dumps(out), flush=True)
+
+
+def extra(fn):
+    try:
+        return fn()
+    except Exception as e:  # reported in the line instead of losing it
+        log(f"bench: extra measurement failed: {e!r}")
+        return {"error": repr(e)}
+
+
+def grad_extra(pm):
+    log("bench: 1M optimiser (grad+Adam) on its generated pairs")
+    r = grad_run(pm, 2, 1)
+    return {"workload": r["config"]["workload"], "n": r["config"]["n"], "pairs": r["config"]["pairs"],
+            "bytes_per_iter": r["bytes_per_iter"], "us_per_iter": r["us_per_iter"], "GB_per_s": r["value"],
+            "frac_hbm_peak": r["frac_hbm_peak"], "paged": r["paged"]}
+
+
+def encoder_extra(pm):
+    log("bench: encoder vs plain optimiser (Swiss roll, 150000 / 100000)")
+    r = encoder_run(pm, plain=True)
+    e = r["encoder"]
+    return {"workload": r["config"]["workload"], "speedup_resident": r["value"],
+            "transform_resident_ms": e["transform_resident_ms"], "plain_fit_resident_ms": e["plain_fit_resident_ms"],
+            "knn10_preservation": e["knn10_preservation"], "triplet_accuracy": e["triplet_accuracy"],
+            "train_s": e["train_s"], "epochs": r["config"]["epochs"]}
 
 
 if __name__ == "__main__":
