@@ -136,12 +136,6 @@ bool knn_mfma_path(const float *x_dev, uint64_t n, uint64_t d, uint32_t k, const
   PM_HIP(launch_fill_zero(kstats.get(), kstats.bytes(), s));
   const void *flo_p = mode != 0 ? flo.get() : fhi.get();
   const float *rinv_p = mode == 2 ? rinv.get() : nullptr;
-  // A wave's epilogue tests 2 x 32 x 16 pairs of a tile pair; with about
-  // 3.7 k listed candidates per row (k = 60: 222), it lists nothing with
-  // probability exp(-1024 x 3.7 k / n): below 1/e past n = 3,800 k the
-  // kernel first tests whether anything passes (70,000 points: 4 %,
-  // 10^6 points: 80 % of the epilogues end there).
-  const bool sparse = static_cast<double>(n) > 3800.0 * k;
   if (sym) PM_HIP(launch_fill_zero(gcnt.get(), gcnt.bytes(), s));
   Timer tsamp, tk, tr;
   tsamp.start(s);
@@ -152,7 +146,7 @@ bool knn_mfma_path(const float *x_dev, uint64_t n, uint64_t d, uint32_t k, const
   tk.start(s);
   PM_HIP(launch_knn_cand(sym ? 2 : 1, fhi.get(), flo_p, arow.get(), rinv_p, static_cast<uint32_t>(n), kch,
                          static_cast<uint32_t>(ntiles), ss, capg, nsplit, gcnt.get(), gcand.get(),
-                         gtau.get(), mode, s, splan, sparse));
+                         gtau.get(), mode, s, splan));
   tk.stop(s);
   tr.start(s);
   // upper-bound pruning (certified split lists): see knn_rerank_kernel

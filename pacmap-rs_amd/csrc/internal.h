@@ -81,8 +81,6 @@ hipError_t launch_knn_rownorm(const float *x, uint64_t n, uint32_t d, const doub
 // once and appends to both rows' lists: gcnt n counters (zeroed first),
 // gcand n x capg entries (nsplit = 1), one workgroup per item of
 // knn_sym_plan.
-// sparse: most tile pairs list nothing (large n); a wave-wide test of the
-// whole tile then ends their epilogue early.
 // mode 0: bf16, 1: split bf16, 2: FP8 E4M3 with per-row scales (rinv).
 uint32_t knn_cand_groups(uint32_t ntiles);
 struct KnnSymPlan {
@@ -96,7 +94,7 @@ KnnSymPlan knn_sym_plan(uint32_t ntiles, uint32_t target_items, uint32_t max_win
 hipError_t launch_knn_cand(int phase, const void *fhi, const void *flo, const float *arow, const float *rinv,
                            uint32_t n, uint32_t kch, uint32_t ntiles, uint32_t ss, uint32_t capg,
                            uint32_t nsplit, uint32_t *gcnt, uint32_t *gcand, float *gtau,
-                           int mode, hipStream_t s, const KnnSymPlan &sp = KnnSymPlan{}, bool sparse = false);
+                           int mode, hipStream_t s, const KnnSymPlan &sp = KnnSymPlan{});
 // cert_scale < 0: accept every row with at least k listed candidates (no
 // certificate).  dots: the lists hold (candidate, dot) pairs (mode 1); with
 // ub_s1 > 0 as well, candidates whose lower bound exceeds the k-th smallest

@@ -299,7 +299,8 @@ void knn_mfma_kernel(
     const uint4 *__restrict__ fhi, const uint4 *__restrict__ flo, const float *__restrict__ arow,
     const float *__restrict__ rinv, uint32_t n, uint32_t ntiles, uint32_t ss, uint32_t capg,
     uint32_t *__restrict__ gcnt, uint32_t *__restrict__ gcand, float *__restrict__ gtau,
-    KnnSymPlan sp, uint32_t sparse) {
+    KnnSymPlan sp) {
+  static_assert(PHASE != 2 || RT <= 2, "phase 2 holds a lane's tests of two tiles in 32 bits");
   constexpr bool SPLIT = MODE == 1;
   constexpr bool FP8 = MODE == 2;
   constexpr int CT = (8 / KCH) > 0 ? (8 / KCH) : 1;  // candidate tiles per LDS stage
@@ -485,22 +486,6 @@ void knn_mfma_kernel(
       napp = 0;
     }
   };
-  // one test set (at most 64 appends): room is made first
-  auto append = [&](bool pass, uint32_t row, uint32_t cnd, float dot) __attribute__((always_inline)) {
-    if (napp > kApp - 64) flush();
-    const uint64_t m = __ballot(pass);
-    if (m) {
-      if (pass) {
-        const uint32_t e = napp + __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(m >> 32),
-                                                            __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(m), 0u));
-        sApp[wave][0][e] = row;
-        sApp[wave][1][e] = cnd;
-        sApp[wave][2][e] = __float_as_uint(dot);
-      }
-      napp += static_cast<uint32_t>(__popcll(m));
-    }
-  };
-
   if (nstage) {
     fetch(0);
     stash(0);
@@ -609,45 +594,6 @@ void knn_mfma_kernel(
       // the wave's own query tiles (a uniform condition)
       const float gc = ac * 0.5f;
       const bool diag = tile - qtile0 < static_cast<uint32_t>(RT);
-      if constexpr (PHASE == 2) {
-        // the group's own tiles: query rows only (uniform condition).  All
-        // tests first; most tiles list nothing, and then one wave-wide test
-        // ends the epilogue.
-        const bool own = tile - gtile0 < static_cast<uint32_t>(4 * RT);
-        const float tc = sT[buf & (PHASE == 2 ? 1 : 0)][ct][col];
-        // tc + aq / 2: the halving is exact, so the fused form rounds once,
-        // as the sum does.  With `sparse` (few tiles list anything: large
-        // n), a wave-wide test of every set first ends most epilogues; it
-        // ignores the self pair, which only sends the wave down the listing
-        // path, where it is tested.
-        auto dot_of = [&](int rt, int r) __attribute__((always_inline)) {
-          return FP8 ? acc[rt][r] * (iq[rt][r] * ic) : acc[rt][r];
-        };
-        if (sparse) {
-          uint64_t any = 0;
-#pragma unroll
-          for (int rt = 0; rt < RT; ++rt)
-#pragma unroll
-            for (int r = 0; r < 4; ++r) any |= __ballot(dot_of(rt, r) >= tau[rt][r] + gc);
-          if (!own) {
-#pragma unroll
-            for (int rt = 0; rt < RT; ++rt)
-#pragma unroll
-              for (int r = 0; r < 4; ++r) any |= __ballot(dot_of(rt, r) >= fmaf(aq[rt][r], 0.5f, tc));
-          }
-          if (any == 0) return;
-        }
-#pragma unroll
-        for (int rt = 0; rt < RT; ++rt)
-#pragma unroll
-          for (int r = 0; r < 4; ++r) {
-            const uint32_t q = qidx[rt][r];
-            const float dot = dot_of(rt, r);
-            append(dot >= tau[rt][r] + gc && (!diag || c != q), q, c, dot);
-            if (!own) append(dot >= fmaf(aq[rt][r], 0.5f, tc), c, q, dot);
-          }
-        return;
-      }
 #pragma unroll
       for (int rt = 0; rt < RT; ++rt)
 #pragma unroll
@@ -674,6 +620,80 @@ void knn_mfma_kernel(
           }
         }
     };
+    // PHASE 2: the tests of candidate tiles tile0 and (two) tile0 + 1, then
+    // one listing pass over both.  On the group's own tiles only the query
+    // rows' lists are tested (a uniform condition).  Lane bit
+    // u 8 RT + rt 4 + r lists (q, c) in q's list and bit u 8 RT + 4 RT + rt 4 + r
+    // lists (c, q) in c's list, for c = 16 (tile0 + u) + col.
+    auto list2 = [&](int ct, uint32_t tile0, const floatx4 (&acc0)[RT], const floatx4 (&acc1)[RT], bool two) {
+      float dv[2][RT * 4];
+      uint32_t bits = 0;
+#pragma unroll
+      for (int u = 0; u < 2; ++u) {
+        if (u == 1 && !two) break;
+        const uint32_t tile = tile0 + u;
+        const float ac = sA[buf][ct + u][col];
+        const float ic = FP8 ? sI[buf][ct + u][col] : 1.0f;
+        const float gc = ac * 0.5f;
+        const floatx4 (&acc)[RT] = u == 0 ? acc0 : acc1;
+#pragma unroll
+        for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) dv[u][rt * 4 + r] = FP8 ? acc[rt][r] * (iq[rt][r] * ic) : acc[rt][r];
+        uint32_t b = 0;
+#pragma unroll
+        for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) b |= dv[u][rt * 4 + r] >= tau[rt][r] + gc ? 1u << (rt * 4 + r) : 0u;
+        // self pairs: only on the wave's own query tiles (uniform), at
+        // most one per lane
+        if (tile - qtile0 < static_cast<uint32_t>(RT)) {
+          const int rr = col - 4 * rsub;
+          if (rr >= 0 && rr < 4) b &= ~(1u << ((tile - qtile0) * 4 + rr));
+        }
+        if (tile - gtile0 >= static_cast<uint32_t>(4 * RT)) {
+          // tc + aq / 2: the halving is exact, so the fused form rounds
+          // once, as the sum does
+          const float tc = sT[buf & (PHASE == 2 ? 1 : 0)][ct + u][col];
+#pragma unroll
+          for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+              b |= dv[u][rt * 4 + r] >= fmaf(aq[rt][r], 0.5f, tc) ? 1u << (4 * RT + rt * 4 + r) : 0u;
+        }
+        bits |= b << (u * 8 * RT);
+      }
+      uint64_t m = __ballot(bits != 0);
+      // rounds: each lane with bits left stages its lowest one (most tile
+      // pairs end here or after one round)
+      while (m) {
+        if (napp > kApp - 64) flush();
+        if (bits) {
+          const uint32_t i = __builtin_ctz(bits);
+          bits &= bits - 1u;
+          const uint32_t u = i / (8u * RT), k = i & (4u * RT - 1u);
+          const uint32_t di = u * 4u * RT + k;  // index into dv, by a tree of selects
+          float lv[RT * 8];
+#pragma unroll
+          for (int j = 0; j < RT * 8; ++j) lv[j] = dv[j / (RT * 4)][j % (RT * 4)];
+#pragma unroll
+          for (int w = RT * 4; w >= 1; w >>= 1)
+#pragma unroll
+            for (int j = 0; j < w; ++j) lv[j] = (di & static_cast<uint32_t>(w)) ? lv[j + w] : lv[j];
+          const float dot = lv[0];
+          const uint32_t c = (tile0 + u) * 16u + col;
+          const uint32_t q = (qtile0 + (k >> 2)) * 16u + 4u * rsub + (k & 3u);
+          const bool rev = (i & (4u * RT)) != 0;
+          const uint32_t e = napp + __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(m >> 32),
+                                                              __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(m), 0u));
+          sApp[wave][0][e] = rev ? c : q;
+          sApp[wave][1][e] = rev ? q : c;
+          sApp[wave][2][e] = __float_as_uint(dot);
+        }
+        napp += static_cast<uint32_t>(__popcll(m));
+        m = __ballot(bits != 0);
+      }
+    };
     // two candidate tiles at a time: their independent MFMA chains interleave
     // (slots past the range hold zero fragments; their epilogue is skipped)
 #pragma unroll 1
@@ -687,11 +707,18 @@ void knn_mfma_kernel(
         const uint32_t tile1 = slot_tile(slot1);
         floatx4 acc1[RT];
         products2(ct, acc0, acc1);
-        epilogue(ct, tile0, acc0);
-        if (slot1 < slot_hi && tile1 < ntiles) epilogue(ct + 1, tile1, acc1);
+        if constexpr (PHASE == 2) {
+          list2(ct, tile0, acc0, acc1, slot1 < slot_hi && tile1 < ntiles);
+        } else {
+          epilogue(ct, tile0, acc0);
+          if (slot1 < slot_hi && tile1 < ntiles) epilogue(ct + 1, tile1, acc1);
+        }
       } else {
         products(ct, acc0);
-        epilogue(ct, tile0, acc0);
+        if constexpr (PHASE == 2)
+          list2(ct, tile0, acc0, acc0, false);
+        else
+          epilogue(ct, tile0, acc0);
       }
     }
     if (st + 1 < nstage) stash(buf ^ 1);
@@ -1018,18 +1045,18 @@ template <int KCH, int MODE>
 static hipError_t cand_dispatch(int phase, const void *fhi, const void *flo, const float *arow,
                                 const float *rinv, uint32_t n, uint32_t ntiles, uint32_t ss, uint32_t capg,
                                 uint32_t nsplit, uint32_t *gcnt, uint32_t *gcand, float *gtau,
-                                const KnnSymPlan &sp, uint32_t sparse, hipStream_t s) {
+                                const KnnSymPlan &sp, hipStream_t s) {
   const uint32_t groups = knn_cand_groups(ntiles);
   const uint4 *h = static_cast<const uint4 *>(fhi), *l = static_cast<const uint4 *>(flo);
   if (phase == 0)
     knn_mfma_kernel<KCH, kCandRT, MODE, 0><<<dim3(groups, 1), 256, 0, s>>>(h, l, arow, rinv, n, ntiles, ss,
-                                                                          capg, gcnt, gcand, gtau, KnnSymPlan{}, 0u);
+                                                                          capg, gcnt, gcand, gtau, KnnSymPlan{});
   else if (phase == 1)
     knn_mfma_kernel<KCH, kCandRT, MODE, 1><<<dim3(groups, nsplit), 256, 0, s>>>(
-        h, l, arow, rinv, n, ntiles, ss, capg, gcnt, gcand, gtau, KnnSymPlan{}, 0u);
+        h, l, arow, rinv, n, ntiles, ss, capg, gcnt, gcand, gtau, KnnSymPlan{});
   else if (sp.items)
     knn_mfma_kernel<KCH, kCandRT, MODE, 2><<<dim3(sp.items, 1), 256, 0, s>>>(
-        h, l, arow, rinv, n, ntiles, ss, capg, gcnt, gcand, gtau, sp, sparse);
+        h, l, arow, rinv, n, ntiles, ss, capg, gcnt, gcand, gtau, sp);
   return hipGetLastError();
 }
 
@@ -1038,11 +1065,11 @@ static hipError_t cand_dispatch_mode(int mode, int phase, const void *fhi, const
                                      const float *arow, const float *rinv, uint32_t n, uint32_t ntiles,
                                      uint32_t ss, uint32_t capg, uint32_t nsplit, uint32_t *gcnt,
                                      uint32_t *gcand, float *gtau, const KnnSymPlan &sp,
-                                     uint32_t sparse, hipStream_t s) {
+                                     hipStream_t s) {
   switch (mode) {
-    case 0: return cand_dispatch<KCH, 0>(phase, fhi, flo, arow, rinv, n, ntiles, ss, capg, nsplit, gcnt, gcand, gtau, sp, sparse, s);
-    case 1: return cand_dispatch<KCH, 1>(phase, fhi, flo, arow, rinv, n, ntiles, ss, capg, nsplit, gcnt, gcand, gtau, sp, sparse, s);
-    case 2: return cand_dispatch<KCH, 2>(phase, fhi, flo, arow, rinv, n, ntiles, ss, capg, nsplit, gcnt, gcand, gtau, sp, sparse, s);
+    case 0: return cand_dispatch<KCH, 0>(phase, fhi, flo, arow, rinv, n, ntiles, ss, capg, nsplit, gcnt, gcand, gtau, sp, s);
+    case 1: return cand_dispatch<KCH, 1>(phase, fhi, flo, arow, rinv, n, ntiles, ss, capg, nsplit, gcnt, gcand, gtau, sp, s);
+    case 2: return cand_dispatch<KCH, 2>(phase, fhi, flo, arow, rinv, n, ntiles, ss, capg, nsplit, gcnt, gcand, gtau, sp, s);
     default: return hipErrorInvalidValue;
   }
 }
@@ -1074,13 +1101,13 @@ KnnSymPlan knn_sym_plan(uint32_t ntiles, uint32_t target_items, uint32_t max_win
 hipError_t launch_knn_cand(int phase, const void *fhi, const void *flo, const float *arow, const float *rinv,
                            uint32_t n, uint32_t kch, uint32_t ntiles, uint32_t ss, uint32_t capg,
                            uint32_t nsplit, uint32_t *gcnt, uint32_t *gcand, float *gtau,
-                           int mode, hipStream_t s, const KnnSymPlan &sp, bool sparse) {
+                           int mode, hipStream_t s, const KnnSymPlan &sp) {
   if (mode == 2 && rinv == nullptr) return hipErrorInvalidValue;
   if (phase == 2 && sp.items && (sp.L == 0 || sp.L % sp.gt != 0 || sp.m * sp.gt != sp.L || sp.W == 0 ||
                                  sp.gt != 4 * kCandRT))
     return hipErrorInvalidValue;
 #define PM_CAND(K) \
-  case K: return cand_dispatch_mode<K>(mode, phase, fhi, flo, arow, rinv, n, ntiles, ss, capg, nsplit, gcnt, gcand, gtau, sp, sparse, s);
+  case K: return cand_dispatch_mode<K>(mode, phase, fhi, flo, arow, rinv, n, ntiles, ss, capg, nsplit, gcnt, gcand, gtau, sp, s);
   switch (kch) {
     PM_CAND(1)
     PM_CAND(2)
