@@ -801,31 +801,38 @@ __global__ __launch_bounds__(256) void knn_rerank_kernel(
     for (int o = 1; o < 16; o <<= 1) maxc = max(maxc, __shfl_xor(maxc, o, 64));
     maxc = __shfl(maxc, 0, 64);  // lanes 0..15 hold the split sizes: the wave's bound
     const uint2 *gpair = reinterpret_cast<const uint2 *>(gcand);
-    for (uint32_t j0 = 0; j0 < maxc; j0 += 64) {
-      uint32_t v[16], dv[16];
+    // four loads in flight per lane: four 64-entry rounds of one list, or
+    // one round of four split lists
+    for (uint32_t j0 = 0; j0 < maxc; j0 += (nsplit == 1 ? 256u : 64u)) {
+      for (uint32_t sp0 = 0; sp0 < nsplit; sp0 += 4) {
+        uint32_t v[4], dv[4], dst[4];
 #pragma unroll
-      for (int sp = 0; sp < 16; ++sp) {
-        const uint32_t csp = __shfl(myc, sp, 64);
-        const size_t slot = (static_cast<size_t>(q) * nsplit + sp) * capg + j0 + lane;
-        v[sp] = 0u;
-        dv[sp] = 0u;
-        if (static_cast<uint32_t>(sp) < nsplit && j0 + lane < csp) {
-          if (dots) {
-            const uint2 pr = gpair[slot];
-            v[sp] = pr.x;
-            dv[sp] = pr.y;
-          } else {
-            v[sp] = gcand[slot];
+        for (int u = 0; u < 4; ++u) {
+          const uint32_t sp = nsplit == 1 ? 0u : sp0 + u;
+          const uint32_t j = j0 + lane + (nsplit == 1 ? 64u * u : 0u);
+          const uint32_t csp = __shfl(myc, static_cast<int>(sp & 15u), 64);
+          const uint32_t start = __shfl(incl, static_cast<int>(sp & 15u), 64) - csp;
+          const size_t slot = (static_cast<size_t>(q) * nsplit + sp) * capg + j;
+          v[u] = 0u;
+          dv[u] = 0u;
+          dst[u] = ~0u;
+          if (sp < nsplit && j < csp) {
+            dst[u] = start + j;
+            if (dots) {
+              const uint2 pr = gpair[slot];
+              v[u] = pr.x;
+              dv[u] = pr.y;
+            } else {
+              v[u] = gcand[slot];
+            }
           }
         }
-      }
 #pragma unroll
-      for (int sp = 0; sp < 16; ++sp) {
-        const uint32_t csp = __shfl(myc, sp, 64), start = __shfl(incl, sp, 64) - csp;
-        if (static_cast<uint32_t>(sp) < nsplit && j0 + lane < csp) {
-          list[start + j0 + lane] = v[sp];
-          if (dots) dotl[start + j0 + lane] = __uint_as_float(dv[sp]);
-        }
+        for (int u = 0; u < 4; ++u)
+          if (dst[u] != ~0u) {
+            list[dst[u]] = v[u];
+            if (dots) dotl[dst[u]] = __uint_as_float(dv[u]);
+          }
       }
     }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
@@ -845,13 +852,16 @@ __global__ __launch_bounds__(256) void knn_rerank_kernel(
   uint32_t cnt2 = cnt;
   if (dots && ub_s1 > 0.0) {
     const float aq = arow[q];
+    // theta: the k-th smallest of the rounded-up U values, found by
+    // bisection over their order-preserving 32-bit keys (32 rounds of one
+    // ballot count per register; no sort)
     auto theta_of = [&](auto rr) -> double {
       constexpr int R2 = decltype(rr)::value;
-      uint64_t v[R2];
+      uint32_t v[R2];
 #pragma unroll
       for (int r = 0; r < R2; ++r) {
         const uint32_t e = r * 64 + lane;
-        v[r] = ~0ull;
+        v[r] = ~0u;
         if (e < cnt) {
           const float dot = dotl[e], ac = arow[list[e]];
           const float lb = fmaf(-2.0f, dot, aq + ac);
@@ -859,11 +869,22 @@ __global__ __launch_bounds__(256) void knn_rerank_kernel(
               fmax((static_cast<double>(aq) + static_cast<double>(ac)) * ub_s1 - 2.0 * static_cast<double>(dot), 0.0) *
               ub_s2;
           dotl[e] = lb;  // the dot is not needed again
-          v[r] = (static_cast<uint64_t>(tkey_m(__double2float_ru(u))) << 32) | e;
+          v[r] = tkey_m(__double2float_ru(u));
         }
       }
-      wave_bitonic_m<R2>(v, lane);
-      return static_cast<double>(from_tkey_m(static_cast<uint32_t>(wave_pick_m<R2>(v, k - 1) >> 32)));
+      // smallest t with #{v <= t} >= k; k <= cnt keys are finite, so t < ~0
+      uint32_t lo = 0u, hi = ~0u;
+      while (lo < hi) {
+        const uint32_t mid = lo + ((hi - lo) >> 1);
+        uint32_t c = 0;
+#pragma unroll
+        for (int r = 0; r < R2; ++r) c += static_cast<uint32_t>(__popcll(__ballot(v[r] <= mid)));
+        if (c >= k)
+          hi = mid;
+        else
+          lo = mid + 1u;
+      }
+      return static_cast<double>(from_tkey_m(lo));
     };
     double theta;
     if (R >= 2 && cnt <= 128)
@@ -966,6 +987,52 @@ __global__ __launch_bounds__(256) void knn_rerank_kernel(
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+  // up to 128 keys: each key's rank is the number of smaller keys (keys are
+  // distinct: a candidate is listed once per row), counted against the
+  // keys read back from LDS; the first k go to their rank's slot
+  if (cnt2 <= 128) {
+    uint64_t v[2];
+    uint32_t rank[2] = {0u, 0u};
+#pragma unroll
+    for (int r = 0; r < 2; ++r) {
+      const uint32_t e = r * 64 + lane;
+      v[r] = e < cnt2 ? keys[e] : ~0ull;
+    }
+    uint32_t j = 0;
+    for (; j + 1 < cnt2; j += 2) {
+      const uint64_t a = keys[j], b = keys[j + 1];
+#pragma unroll
+      for (int r = 0; r < 2; ++r) rank[r] += (a < v[r] ? 1u : 0u) + (b < v[r] ? 1u : 0u);
+    }
+    if (j < cnt2) {
+      const uint64_t a = keys[j];
+#pragma unroll
+      for (int r = 0; r < 2; ++r) rank[r] += a < v[r] ? 1u : 0u;
+    }
+    // the k-th smallest key is held where the rank is k - 1
+    uint64_t kth = 0;
+#pragma unroll
+    for (int r = 0; r < 2; ++r) {
+      const uint64_t m = __ballot(r * 64 + lane < static_cast<int>(cnt2) && rank[r] == k - 1);
+      if (m) kth = shfl_u64m(v[r], static_cast<int>(__builtin_ctzll(m)));
+    }
+    const float dk = from_tkey_m(static_cast<uint32_t>(kth >> 32));
+    const float t = gtau[q];
+    const bool cert = all_listed || cert_scale < 0.0 ||
+                      (t > 0.0f && sqrt(static_cast<double>(t) * cert_scale) > static_cast<double>(dk));
+    if (!cert) {
+      if (lane == 0) fb_list[atomicAdd(fb_count, 1u)] = q;
+      return;
+    }
+#pragma unroll
+    for (int r = 0; r < 2; ++r) {
+      if (r * 64 + lane < static_cast<int>(cnt2) && rank[r] < k) {
+        oidx[static_cast<size_t>(q) * k + rank[r]] = static_cast<uint32_t>(v[r]);
+        odist[static_cast<size_t>(q) * k + rank[r]] = from_tkey_m(static_cast<uint32_t>(v[r] >> 32));
+      }
+    }
+    return;
+  }
   // sort only as many keys as remain (k <= cnt2 <= 64 R2)
   auto finish = [&](auto rr) {
     constexpr int R2 = decltype(rr)::value;
@@ -995,9 +1062,7 @@ __global__ __launch_bounds__(256) void knn_rerank_kernel(
       }
     }
   };
-  if (R >= 2 && cnt2 <= 128)
-    finish(std::integral_constant<int, (R >= 2 ? 2 : R)>{});
-  else if (R >= 4 && cnt2 <= 256)
+  if (R >= 4 && cnt2 <= 256)
     finish(std::integral_constant<int, (R >= 4 ? 4 : R)>{});
   else
     finish(std::integral_constant<int, R>{});
