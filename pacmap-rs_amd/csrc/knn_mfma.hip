@@ -594,20 +594,52 @@ void knn_mfma_kernel(
       // the wave's own query tiles (a uniform condition)
       const float gc = ac * 0.5f;
       const bool diag = tile - qtile0 < static_cast<uint32_t>(RT);
+      if constexpr (PHASE == 0) {
+        // every lb of the tile first, the self pair set to +inf (only on
+        // the wave's own query tiles, a uniform condition); one wave-wide
+        // test then ends most epilogues once the two smallest have settled
+        float lbv[RT][4];
+        bool any = false;
+        const int rr = col - 4 * rsub;
+#pragma unroll
+        for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const float dot = FP8 ? acc[rt][r] * (iq[rt][r] * ic) : acc[rt][r];
+            lbv[rt][r] = fmaf(-2.0f, dot, aq[rt][r] + ac);
+          }
+        if (diag) {
+#pragma unroll
+          for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+              if (tile - qtile0 == static_cast<uint32_t>(rt) && rr == r) lbv[rt][r] = __builtin_huge_valf();
+        }
+#pragma unroll
+        for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) any |= lbv[rt][r] < best1[rt][r];
+        if (__ballot(any) == 0) return;
+#pragma unroll
+        for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const float lb = lbv[rt][r];
+            if (lb < best1[rt][r]) {
+              const bool first = lb < best0[rt][r];
+              best1[rt][r] = first ? best0[rt][r] : lb;
+              best0[rt][r] = first ? lb : best0[rt][r];
+            }
+          }
+        return;
+      }
 #pragma unroll
       for (int rt = 0; rt < RT; ++rt)
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
           const uint32_t q = qidx[rt][r];
           const float dot = FP8 ? acc[rt][r] * (iq[rt][r] * ic) : acc[rt][r];
-          if (PHASE == 0) {
-            const float lb = fmaf(-2.0f, dot, aq[rt][r] + ac);
-            if (c != q && lb < best1[rt][r]) {
-              const bool first = lb < best0[rt][r];
-              best1[rt][r] = first ? best0[rt][r] : lb;
-              best0[rt][r] = first ? lb : best0[rt][r];
-            }
-          } else if (dot >= tau[rt][r] + gc && (!diag || c != q)) {
+          if (dot >= tau[rt][r] + gc && (!diag || c != q)) {
             const uint32_t pos = atomicAdd(&sCnt[wave][rt * 16 + 4 * rsub + r], 1u);
             if (pos < capg) {
               const size_t slot = (static_cast<size_t>(q) * nsplit + split) * capg + pos;
