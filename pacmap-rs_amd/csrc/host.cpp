@@ -16,8 +16,39 @@ namespace {
 
 thread_local pacmap_timings g_tm;
 // rows of the last kNN on this thread that the certificate sent to the exact
-// rescan (filled when the stream reaches the copy; read after the call returns)
-thread_local std::vector<uint32_t> g_rescanned;
+// rescan: copied into pinned memory on the stream (no host wait) and read
+// after the call returns, once the copy's event has completed
+struct RescannedRows {
+  uint32_t *rows = nullptr;  // pinned host memory
+  uint64_t cap = 0, count = 0;
+  hipEvent_t done = nullptr;
+  ~RescannedRows() {
+    if (process_exiting()) return;
+    if (rows) (void)hipHostFree(rows);
+    if (done) (void)hipEventDestroy(done);
+  }
+  void record(const uint32_t *dev_rows, uint64_t n, hipStream_t s) {
+    if (n > cap) {
+      if (done) PM_HIP(hipEventSynchronize(done));  // the last copy into rows is done
+      if (rows) PM_HIP(hipHostFree(rows));
+      rows = nullptr;
+      cap = 0;
+      note_runtime_use();
+      PM_HIP(hipHostMalloc(reinterpret_cast<void **>(&rows), n * sizeof(uint32_t), 0));
+      cap = n;
+    }
+    if (!done) PM_HIP(hipEventCreateWithFlags(&done, hipEventDisableTiming));
+    PM_HIP(hipMemcpyAsync(rows, dev_rows, n * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    PM_HIP(hipEventRecord(done, s));
+    count = n;
+  }
+  std::vector<uint32_t> read() const {
+    if (count == 0) return {};
+    PM_HIP(hipEventSynchronize(done));
+    return std::vector<uint32_t>(rows, rows + count);
+  }
+};
+thread_local RescannedRows g_rescanned;
 
 // ---------------------------------------------------------------------------
 // exact kNN on a device matrix (knn.rs:137-194)
@@ -101,10 +132,10 @@ bool knn_mfma_path(const float *x_dev, uint64_t n, uint64_t d, uint32_t k, const
   else
     PM_HIP(launch_knn_frag(x_dev, n, static_cast<uint32_t>(d), mean.get(), kch, ntiles, fhi.get(),
                            split ? flo.get() : nullptr, s));
+  // the finiteness flag is read with the rerank's results: non-finite
+  // input only makes the candidate passes list garbage (every access stays
+  // within its list), and the caller then runs the direct kernel
   uint32_t hflags[2] = {0, 0};
-  download(hflags, flags.get(), 2, s);
-  PM_HIP(hipStreamSynchronize(s));
-  if (hflags[0]) return false;
   const uint32_t ss = std::max<uint32_t>(1, (3 * k + 15) / 16);
   const uint32_t capg = (3.0 * k * 2.5 <= 512.0) ? 512 : 1024;
   // split the candidate range until there are enough (group, split) blocks
@@ -160,6 +191,7 @@ bool knn_mfma_path(const float *x_dev, uint64_t n, uint64_t d, uint32_t k, const
   std::vector<unsigned long long> hstats(512);
   download(hstats.data(), kstats.get(), 512, s);
   PM_HIP(hipStreamSynchronize(s));
+  if (hflags[0]) return false;
   tk.add_to(g_tm.knn_kernel_ms);
   g_tm.knn_kernel_launches += 1;
   tsamp.add_to(g_tm.knn_sample_ms);
@@ -185,8 +217,7 @@ bool knn_mfma_path(const float *x_dev, uint64_t n, uint64_t d, uint32_t k, const
     PM_HIP(launch_knn_rescan(xp, n, L, k, fb_list.get(), nfb, part.get(), S, idx.get(), dist.get(), s));
     tf.stop(s);
     tf.add_to(g_tm.knn_fallback_ms);
-    g_rescanned.resize(nfb);
-    download(g_rescanned.data(), fb_list.get(), nfb, s);
+    g_rescanned.record(fb_list.get(), nfb, s);
   }
   return true;
 }
@@ -200,7 +231,7 @@ void knn_device(const float *x_dev, uint64_t n, uint64_t d, uint32_t k, DevBuf<u
   if (d >= (1ull << 28)) fail(PACMAP_ERR_UNSUPPORTED, "too many features for the kNN kernel");
   idx.alloc(n * k);
   dist.alloc(n * k);
-  g_rescanned.clear();
+  g_rescanned.count = 0;
   if (k == 0) return;
   const KnnLayout L = knn_layout(static_cast<uint32_t>(d));
   DevBuf<float> xp(n * L.Dp);
@@ -684,6 +715,41 @@ void upload_pca_stats(const float *x_host, uint64_t n, uint32_t d, float *x_dev,
   tc.add_to(g_tm.pca_cov_ms);
 }
 
+// The subspace iteration's random start (uniform in [-0.5, 0.5), SmallRng
+// seeded 0x5eed, row-major d x b), drawn and uploaded once per thread and
+// shape, then kept on the device for the thread's lifetime.
+struct PcaStartBlock {
+  double *p = nullptr;
+  uint32_t d = 0, b = 0;
+  int dev = -1;
+  ~PcaStartBlock() {
+    if (p && !process_exiting()) (void)hipFree(p);
+  }
+};
+const double *pca_start_block(uint32_t d, uint32_t b) {
+  static thread_local PcaStartBlock c;
+  int dev = 0;
+  PM_HIP(hipGetDevice(&dev));
+  if (c.p && c.d == d && c.b == b && c.dev == dev) return c.p;
+  std::vector<double> q0(static_cast<size_t>(d) * b);
+  SmallRng rng = SmallRng::seed_from_u64(0x5eedull);
+  for (auto &e : q0) e = rng.next_f64() - 0.5;
+  if (c.p) {
+    PM_HIP(hipSetDevice(c.dev));
+    PM_HIP(hipDeviceSynchronize());  // earlier calls' copies out of it are done
+    PM_HIP(hipFree(c.p));
+    PM_HIP(hipSetDevice(dev));
+    c.p = nullptr;
+  }
+  note_runtime_use();
+  PM_HIP(hipMalloc(reinterpret_cast<void **>(&c.p), q0.size() * sizeof(double)));
+  PM_HIP(hipMemcpy(c.p, q0.data(), q0.size() * sizeof(double), hipMemcpyHostToDevice));
+  c.d = d;
+  c.b = b;
+  c.dev = dev;
+  return c.p;
+}
+
 void pca_axes_dev(PcaStats &stats, uint64_t n, uint32_t d, uint32_t q, PcaAxes &out, hipStream_t s) {
   constexpr uint32_t kIters = 6;
   const uint32_t jmax = jacobi_max_size();
@@ -701,12 +767,10 @@ void pca_axes_dev(PcaStats &stats, uint64_t n, uint32_t d, uint32_t q, PcaAxes &
   } else {
     const uint32_t b = std::min<uint32_t>(std::min<uint32_t>(d, q + 12), jmax);
     if (q > b) fail(PACMAP_ERR_PCA, "too many principal components requested");
-    std::vector<double> Q0(static_cast<size_t>(d) * b);
-    SmallRng rng = SmallRng::seed_from_u64(0x5eedull);
-    for (auto &e : Q0) e = rng.next_f64() - 0.5;
-    DevBuf<double> Q(Q0.size()), Z(Q0.size()), G(static_cast<size_t>(b) * b), Rinv(static_cast<size_t>(b) * b),
+    const size_t qn = static_cast<size_t>(d) * b;
+    DevBuf<double> Q(qn), Z(qn), G(static_cast<size_t>(b) * b), Rinv(static_cast<size_t>(b) * b),
         U(static_cast<size_t>(b) * q);
-    upload(Q, Q0.data(), Q0.size(), s);
+    PM_HIP(hipMemcpyAsync(Q.get(), pca_start_block(d, b), qn * sizeof(double), hipMemcpyDeviceToDevice, s));
     auto gemm = [&](uint32_t M, uint32_t N, uint32_t K, const double *A, uint32_t lda, bool tA, const double *B,
                     uint32_t ldb, double *C, uint32_t ldc) {
       PM_HIP(launch_gemm64(M, N, K, A, lda, tA, B, ldb, C, ldc, s));
@@ -1350,7 +1414,7 @@ void pacmap_last_timings(pacmap_timings *out) {
 }
 
 uint64_t pacmap_last_rescanned_rows(uint32_t *rows_out, uint64_t cap) {
-  std::vector<uint32_t> v = g_rescanned;
+  std::vector<uint32_t> v = g_rescanned.read();
   std::sort(v.begin(), v.end());
   if (rows_out)
     for (uint64_t i = 0; i < std::min<uint64_t>(cap, v.size()); ++i) rows_out[i] = v[i];
