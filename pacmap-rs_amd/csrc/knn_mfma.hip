@@ -392,7 +392,9 @@ void knn_mfma_kernel(
       __builtin_amdgcn_make_buffer_rsrc(const_cast<uint4 *>(fhi), 0, static_cast<int>(frag_bytes), 0x00020000);
   const __amdgpu_buffer_rsrc_t rsl = __builtin_amdgcn_make_buffer_rsrc(
       const_cast<uint4 *>(NH == 2 ? flo : fhi), 0, static_cast<int>(frag_bytes), 0x00020000);
-  auto fetch = [&](uint32_t st) {
+  // stage st into LDS buffer b (ROUNDS: straight to LDS, a wave's 64 lanes
+  // filling one 1 KB fragment block; otherwise into registers for stash)
+  auto fetch = [&](uint32_t st, int b) {
     const uint32_t s0 = slot_lo + st * CT;
     if constexpr (ROUNDS) {
 #pragma unroll
@@ -406,8 +408,10 @@ void knn_mfma_kernel(
           const uint32_t o = (slot < slot_hi && tile < ntiles)
                                  ? ((tile * static_cast<uint32_t>(KCH) + kc) * 64u + l) * 16u
                                  : 0xfffffff0u;
-          const auto v = __builtin_amdgcn_raw_buffer_load_b128(h == 0 ? rsh : rsl, static_cast<int>(o), 0, 0);
-          pf[h * PFH + j] = make_uint4(v[0], v[1], v[2], v[3]);
+          // reads past the fragments (slots past the range) write zeros
+          __builtin_amdgcn_raw_ptr_buffer_load_lds(h == 0 ? rsh : rsl,
+                                                   (__attribute__((address_space(3))) void *)&sB[b][ct][kc][h][0], 16,
+                                                   static_cast<int>(o), 0, 0, 0);
         }
     } else {
 #pragma unroll
@@ -440,14 +444,9 @@ void knn_mfma_kernel(
   };
   auto stash = [&](int buf) {
     if constexpr (ROUNDS) {
-#pragma unroll
-      for (int h = 0; h < NH; ++h)
-#pragma unroll
-        for (int j = 0; j < PFH; ++j) {
-          const int i = static_cast<int>(threadIdx.x) + j * 256;
-          const int l = i & 63, kc = (i >> 6) % KCH, ct = i / (64 * KCH);
-          sB[buf][ct][kc][h][l] = pf[h * PFH + j];
-        }
+      // this wave's fragment loads into LDS are complete before the barrier
+      // that publishes the stage
+      __builtin_amdgcn_s_waitcnt(0);
     } else {
 #pragma unroll
       for (int j = 0; j < PF; ++j) {
@@ -487,13 +486,13 @@ void knn_mfma_kernel(
     }
   };
   if (nstage) {
-    fetch(0);
+    fetch(0, 0);
     stash(0);
   }
   __syncthreads();
   for (uint32_t st = 0; st < nstage; ++st) {
     const int buf = st & 1;
-    if (st + 1 < nstage) fetch(st + 1);  // in flight while this stage computes
+    if (st + 1 < nstage) fetch(st + 1, buf ^ 1);  // in flight while this stage computes
     // products of candidate tile ct into acc (3 bf16 MFMAs per chunk when
     // split, one FP8 or bf16 MFMA otherwise)
     auto products = [&](int ct, floatx4 (&acc)[RT]) {
