@@ -623,7 +623,9 @@ def main():
                 "rerank_ms": round(float(np.mean([t["knn_rerank_ms"] for t in tms])), 3),
                 "fallback_ms": round(float(np.mean([t["knn_fallback_ms"] for t in tms])), 3),
                 "listed_per_row": round(float(tms[-1]["knn_listed"]) / max(1, n - int(tms[-1]["knn_rescanned_rows"])), 1),
-                "reranked_per_row": round(float(tms[-1]["knn_reranked"]) / max(1, n - int(tms[-1]["knn_rescanned_rows"])), 1)},
+                "reranked_per_row": round(float(tms[-1]["knn_reranked"]) / max(1, n - int(tms[-1]["knn_rescanned_rows"])), 1),
+                "tile_pairs_skipped": round(float(tms[-1]["knn_tile_pairs_skipped"]) /
+                                            max(1, int(tms[-1]["knn_tile_pairs"])), 4)},
     }
     if not args.no_cpu_baseline:
         log("bench: timing the CPU oracle on a bounded sample")

@@ -299,6 +299,8 @@ typedef struct {
   uint64_t grad_paged;         /* 1: paged target-blocked gradient (Y pages staged through LDS) */
   uint64_t grad_page_words;    /* its entry words including padding */
   uint64_t grad_page_pages;    /* its pages per target range */
+  uint64_t knn_tile_pairs;     /* symmetric kNN pass: (query group, candidate tile) pairs past the group's own */
+  uint64_t knn_tile_pairs_skipped; /* of those, skipped by their bounding boxes */
 } pacmap_timings;
 void pacmap_last_timings(pacmap_timings *out);
 /* Seam: the rows (ascending) that the last kNN on this thread (pacmap_knn or

@@ -120,22 +120,6 @@ bool knn_mfma_path(const float *x_dev, uint64_t n, uint64_t d, uint32_t k, const
   PM_HIP(launch_colmean_f64(x_dev, n, static_cast<uint32_t>(d), mean.get(), scratch.get(), s));
   const size_t frag_words = ntiles * kch * 64 * 4;  // 16 B per lane fragment
   DevBuf<uint32_t> fhi(frag_words), flo(mode != 0 ? frag_words : 4);
-  DevBuf<float> arow(rows_padded), rinv(mode == 2 ? rows_padded : 1);
-  DevBuf<uint32_t> flags(2);
-  PM_HIP(launch_fill_zero(flags.get(), flags.bytes(), s));
-  const double kappa = certify ? knn_kappa(kch * 32, split) : 0.0;
-  PM_HIP(launch_knn_rownorm(x_dev, n, static_cast<uint32_t>(d), mean.get(), static_cast<float>(1.0 - kappa),
-                            rows_padded, arow.get(), flags.get(), mode == 2 ? rinv.get() : nullptr, s));
-  if (mode == 2)
-    PM_HIP(launch_knn_frag8(x_dev, n, static_cast<uint32_t>(d), mean.get(), kch, ntiles, rinv.get(), fhi.get(),
-                            flo.get(), s));
-  else
-    PM_HIP(launch_knn_frag(x_dev, n, static_cast<uint32_t>(d), mean.get(), kch, ntiles, fhi.get(),
-                           split ? flo.get() : nullptr, s));
-  // the finiteness flag is read with the rerank's results: non-finite
-  // input only makes the candidate passes list garbage (every access stays
-  // within its list), and the caller then runs the direct kernel
-  uint32_t hflags[2] = {0, 0};
   const uint32_t ss = std::max<uint32_t>(1, (3 * k + 15) / 16);
   const uint32_t capg = (3.0 * k * 2.5 <= 512.0) ? 512 : 1024;
   // split the candidate range until there are enough (group, split) blocks
@@ -159,37 +143,85 @@ bool knn_mfma_path(const float *x_dev, uint64_t n, uint64_t d, uint32_t k, const
                          std::max<uint32_t>(16, window_bytes / tile_bytes));
     sym = splan.items != 0;
   }
+  // The candidate passes see the rows sorted by the Morton code of their
+  // first three coordinates, so that a tile's rows lie close together and
+  // the symmetric pass can skip tile pairs whose boxes lie apart
+  // (PACMAP_KNN_PRUNE=0: the rows' own order and no skipping).
+  bool prune = sym;
+  if (const char *e = std::getenv("PACMAP_KNN_PRUNE")) prune = prune && std::atoi(e) != 0;
+  DevBuf<uint32_t> perm(n);
+  {
+    DevBuf<uint32_t> omm(8), okey(n), okey2(n), oidx(n);
+    const size_t tb = knn_order_temp_bytes(n);
+    DevBuf<uint8_t> otemp(std::max<size_t>(tb, 16));
+    PM_HIP(launch_knn_order(x_dev, n, prune ? static_cast<uint32_t>(d) : 0u, omm.get(), okey.get(), okey2.get(),
+                            oidx.get(), perm.get(), otemp.get(), tb, s));
+  }
+  DevBuf<float> arow(rows_padded), arow_orig(n), rinv(mode == 2 ? rows_padded : 1);
+  DevBuf<uint32_t> flags(2);
+  PM_HIP(launch_fill_zero(flags.get(), flags.bytes(), s));
+  const double kappa = certify ? knn_kappa(kch * 32, split) : 0.0;
+  PM_HIP(launch_knn_rownorm(x_dev, n, static_cast<uint32_t>(d), mean.get(), static_cast<float>(1.0 - kappa),
+                            rows_padded, perm.get(), arow.get(), arow_orig.get(), flags.get(),
+                            mode == 2 ? rinv.get() : nullptr, s));
+  if (mode == 2)
+    PM_HIP(launch_knn_frag8(x_dev, n, static_cast<uint32_t>(d), mean.get(), kch, ntiles, rinv.get(), perm.get(),
+                            fhi.get(), flo.get(), s));
+  else
+    PM_HIP(launch_knn_frag(x_dev, n, static_cast<uint32_t>(d), mean.get(), kch, ntiles, perm.get(), fhi.get(),
+                           split ? flo.get() : nullptr, s));
+  DevBuf<float> box(prune ? ntiles * 16 : 1);
+  if (prune) PM_HIP(launch_knn_tilebox(x_dev, n, static_cast<uint32_t>(d), perm.get(), ntiles, box.get(), s));
+  // the finiteness flag is read with the rerank's results: non-finite
+  // input only makes the candidate passes list garbage (every access stays
+  // within its list), and the caller then runs the direct kernel
+  uint32_t hflags[2] = {0, 0};
   const uint32_t nsplit =
       sym ? 1u : std::min<uint32_t>(16, std::max<uint32_t>(1, (4096 + groups - 1) / groups));
   DevBuf<uint32_t> gcnt(n * nsplit), gcand(n * nsplit * capg * (split ? 2 : 1)), fb_list(n);
-  DevBuf<float> gtau(n);
-  DevBuf<unsigned long long> kstats(512);
+  DevBuf<float> gtau(n), gtau_orig(n), tmax(ntiles);
+  // 256 (listed, reranked) pairs by row mod 256, then the skipped tile pairs
+  DevBuf<unsigned long long> kstats(513);
   PM_HIP(launch_fill_zero(kstats.get(), kstats.bytes(), s));
   const void *flo_p = mode != 0 ? flo.get() : fhi.get();
   const float *rinv_p = mode == 2 ? rinv.get() : nullptr;
   if (sym) PM_HIP(launch_fill_zero(gcnt.get(), gcnt.bytes(), s));
   Timer tsamp, tk, tr;
   tsamp.start(s);
+  KnnOrder ord;
+  ord.perm = perm.get();
   PM_HIP(launch_knn_cand(0, fhi.get(), flo_p, arow.get(), rinv_p, static_cast<uint32_t>(n), kch,
                          static_cast<uint32_t>(ntiles), ss, capg, nsplit, gcnt.get(), gcand.get(),
-                         gtau.get(), mode, s));
+                         gtau.get(), mode, s, KnnSymPlan{}, ord));
+  // tau by row for the rerank; the largest tau per tile and the tile pairs
+  // each query group can skip
+  PM_HIP(launch_knn_tiletau(gtau.get(), perm.get(), n, ntiles, tmax.get(), gtau_orig.get(), s));
+  const uint32_t pwords = static_cast<uint32_t>((ntiles + 31) / 32);
+  DevBuf<uint32_t> pbits(prune ? static_cast<size_t>(groups) * pwords : 1);
+  if (prune) {
+    DevBuf<float> gbox(static_cast<size_t>(groups) * 17);
+    PM_HIP(launch_knn_prune(box.get(), tmax.get(), ntiles, static_cast<uint32_t>(d), pwords, gbox.get(), pbits.get(),
+                            kstats.get() + 512, s));
+    ord.prune = pbits.get();
+    ord.pwords = pwords;
+  }
   tsamp.stop(s);
   tk.start(s);
   PM_HIP(launch_knn_cand(sym ? 2 : 1, fhi.get(), flo_p, arow.get(), rinv_p, static_cast<uint32_t>(n), kch,
                          static_cast<uint32_t>(ntiles), ss, capg, nsplit, gcnt.get(), gcand.get(),
-                         gtau.get(), mode, s, splan));
+                         gtau.get(), mode, s, splan, ord));
   tk.stop(s);
   tr.start(s);
   // upper-bound pruning (certified split lists): see knn_rerank_kernel
   const double ub_s1 = split && certify ? knn_ub_scale1(kappa) : 0.0;
   PM_HIP(launch_knn_rerank(xp, L, static_cast<uint32_t>(n), k, capg, nsplit, gcnt.get(), gcand.get(),
-                           gtau.get(), certify ? knn_cert_scale(static_cast<uint32_t>(d)) : -1.0, arow.get(),
+                           gtau_orig.get(), certify ? knn_cert_scale(static_cast<uint32_t>(d)) : -1.0, arow_orig.get(),
                            split, ub_s1, knn_ub_scale2(static_cast<uint32_t>(d)), idx.get(), dist.get(),
                            flags.get() + 1, fb_list.get(), kstats.get(), s));
   tr.stop(s);
   download(hflags, flags.get(), 2, s);
-  std::vector<unsigned long long> hstats(512);
-  download(hstats.data(), kstats.get(), 512, s);
+  std::vector<unsigned long long> hstats(513);
+  download(hstats.data(), kstats.get(), 513, s);
   PM_HIP(hipStreamSynchronize(s));
   if (hflags[0]) return false;
   tk.add_to(g_tm.knn_kernel_ms);
@@ -202,6 +234,14 @@ bool knn_mfma_path(const float *x_dev, uint64_t n, uint64_t d, uint32_t k, const
     g_tm.knn_reranked += hstats[2 * i + 1];
   }
   g_tm.knn_path = mode == 1 ? 1 : mode == 0 ? 2 : 3;
+  if (sym) {
+    // (query group, candidate tile) pairs past the groups' own tiles, and
+    // those the symmetric pass skipped
+    const uint64_t gt = splan.gt;  // tiles per query group
+    for (uint64_t g = 0; g < groups; ++g)
+      g_tm.knn_tile_pairs += (g + 1) * gt < ntiles ? ntiles - (g + 1) * gt : 0;
+    g_tm.knn_tile_pairs_skipped = hstats[512];
+  }
   g_tm.knn_certified = certify ? 1 : 0;
   const uint32_t nfb = hflags[1];
   g_tm.knn_rescanned_rows += nfb;

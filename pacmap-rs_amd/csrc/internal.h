@@ -64,15 +64,37 @@ hipError_t launch_knn_rescan(const float *xp, uint64_t n, const KnnLayout &L, ui
                              uint32_t *idx, float *dist, hipStream_t s);
 
 // MFMA candidate path (knn_mfma.hip)
+// The candidate kernels see the rows in an order perm (position -> row, see
+// launch_knn_order); fragments, a and rinv are by position.
 hipError_t launch_knn_frag(const float *x, uint64_t n, uint32_t d, const double *mean, uint32_t kch,
-                           uint64_t ntiles, void *fhi, void *flo, hipStream_t s);
+                           uint64_t ntiles, const uint32_t *perm, void *fhi, void *flo, hipStream_t s);
 // FP8 E4M3 fragments of the centred rows scaled by 1 / rinv (128-feature chunks)
 hipError_t launch_knn_frag8(const float *x, uint64_t n, uint32_t d, const double *mean, uint32_t kch,
-                            uint64_t ntiles, const float *rinv, void *fa, void *fb, hipStream_t s);
-// arow = |c|^2 (1 - kappa); rinv (optional) = max |c| / 448 per row
+                            uint64_t ntiles, const float *rinv, const uint32_t *perm, void *fa, void *fb,
+                            hipStream_t s);
+// arow = |c|^2 (1 - kappa) by position, arow_orig the same by row; rinv
+// (optional) = max |c| / 448 by position
 hipError_t launch_knn_rownorm(const float *x, uint64_t n, uint32_t d, const double *mean,
-                              float one_minus_kappa, uint64_t rows_padded, float *arow,
-                              uint32_t *nonfinite, float *rinv, hipStream_t s);
+                              float one_minus_kappa, uint64_t rows_padded, const uint32_t *perm, float *arow,
+                              float *arow_orig, uint32_t *nonfinite, float *rinv, hipStream_t s);
+// perm: the rows sorted by the Morton code of their first 3 coordinates
+// (mm: 6 words; key, key_sorted, idx: n words; temp: knn_order_temp_bytes(n))
+size_t knn_order_temp_bytes(uint64_t n);
+hipError_t launch_knn_order(const float *x, uint64_t n, uint32_t d, uint32_t *mm, uint32_t *key, uint32_t *key_sorted,
+                            uint32_t *idx, uint32_t *perm, void *temp, size_t temp_bytes, hipStream_t s);
+// Tile-pair pruning of the symmetric pass: box (16 floats per tile) bounds
+// the first min(d, 8) coordinates of each tile's rows; after the tau
+// sample, tmax (a float per tile) is the tile's largest tau and gtau_orig
+// the taus by row; bits (words per query group) flags the candidate tiles
+// a group can skip.
+hipError_t launch_knn_tilebox(const float *x, uint64_t n, uint32_t d, const uint32_t *perm, uint64_t ntiles,
+                              float *box, hipStream_t s);
+hipError_t launch_knn_tiletau(const float *gtau, const uint32_t *perm, uint64_t n, uint64_t ntiles, float *tmax,
+                              float *gtau_orig, hipStream_t s);
+// gbox: 17 floats per query group (knn_cand_groups(ntiles)) of scratch;
+// count (zeroed by the caller) accumulates the skipped (group, tile) pairs
+hipError_t launch_knn_prune(const float *box, const float *tmax, uint64_t ntiles, uint32_t d, uint32_t words,
+                            float *gbox, uint32_t *bits, unsigned long long *count, hipStream_t s);
 // phase 0: tau per row from a 1/ss tile sample; phase 1: append candidates
 // with lb <= tau to per-(row, split) lists (gcnt: n x nsplit, gcand:
 // n x nsplit x capg words, twice that in mode 1, whose entries are
@@ -91,10 +113,17 @@ struct KnnSymPlan {
   uint32_t items = 0;  // workgroups (0: no plan)
 };
 KnnSymPlan knn_sym_plan(uint32_t ntiles, uint32_t target_items, uint32_t max_window);
+// perm maps positions to rows (lists and counters are by row, gtau by
+// position); prune (phase 2, optional) holds pwords words per query group.
+struct KnnOrder {
+  const uint32_t *perm = nullptr;
+  const uint32_t *prune = nullptr;
+  uint32_t pwords = 0;
+};
 hipError_t launch_knn_cand(int phase, const void *fhi, const void *flo, const float *arow, const float *rinv,
                            uint32_t n, uint32_t kch, uint32_t ntiles, uint32_t ss, uint32_t capg,
                            uint32_t nsplit, uint32_t *gcnt, uint32_t *gcand, float *gtau,
-                           int mode, hipStream_t s, const KnnSymPlan &sp = KnnSymPlan{});
+                           int mode, hipStream_t s, const KnnSymPlan &sp, const KnnOrder &o);
 // cert_scale < 0: accept every row with at least k listed candidates (no
 // certificate).  dots: the lists hold (candidate, dot) pairs (mode 1); with
 // ub_s1 > 0 as well, candidates whose lower bound exceeds the k-th smallest

@@ -22,6 +22,7 @@
 #include "internal.h"
 
 #include <cstdint>
+#include <hipcub/hipcub.hpp>
 #include <type_traits>
 #include <vector>
 #include <algorithm>
@@ -134,7 +135,8 @@ __device__ __forceinline__ float bf16_to_f(uint16_t h) { return __uint_as_float(
 // 16-byte load per lane fetches an A or B operand of v_mfma_f32_16x16x32_bf16.
 __global__ void knn_frag_kernel(const float *__restrict__ x, uint64_t n, uint32_t d,
                                 const double *__restrict__ mean, uint32_t kch, uint64_t ntiles,
-                                uint4 *__restrict__ fhi, uint4 *__restrict__ flo) {
+                                const uint32_t *__restrict__ perm, uint4 *__restrict__ fhi,
+                                uint4 *__restrict__ flo) {
   const uint64_t total = ntiles * kch * 64;
   for (uint64_t g = blockIdx.x * 256ull + threadIdx.x; g < total; g += gridDim.x * 256ull) {
     const uint32_t l = static_cast<uint32_t>(g & 63);
@@ -142,13 +144,14 @@ __global__ void knn_frag_kernel(const float *__restrict__ x, uint64_t n, uint32_
     const uint32_t kc = static_cast<uint32_t>(tk % kch);
     const uint64_t t = tk / kch;
     const uint64_t row = t * 16 + (l & 15);
+    const uint64_t src = row < n ? perm[row] : 0;
     const uint32_t f0 = kc * 32 + 8 * (l >> 4);
     uint16_t h[8], lo[8];
 #pragma unroll
     for (int e = 0; e < 8; ++e) {
       float c = 0.0f;
       const uint32_t f = f0 + e;
-      if (row < n && f < d) c = static_cast<float>(static_cast<double>(x[row * d + f]) - mean[f]);
+      if (row < n && f < d) c = static_cast<float>(static_cast<double>(x[src * d + f]) - mean[f]);
       h[e] = bf16_rn(c);
       lo[e] = bf16_rn(c - bf16_to_f(h[e]));
     }
@@ -183,8 +186,8 @@ __device__ __forceinline__ uint32_t f32_to_e4m3(float v) {
 // fb (the two halves of a v_mfma_*_16x16x128_f8f6f4 operand).
 __global__ void knn_frag8_kernel(const float *__restrict__ x, uint64_t n, uint32_t d,
                                  const double *__restrict__ mean, uint32_t kch, uint64_t ntiles,
-                                 const float *__restrict__ rinv, uint4 *__restrict__ fa,
-                                 uint4 *__restrict__ fb) {
+                                 const float *__restrict__ rinv, const uint32_t *__restrict__ perm,
+                                 uint4 *__restrict__ fa, uint4 *__restrict__ fb) {
   const uint64_t total = ntiles * kch * 64;
   for (uint64_t g = blockIdx.x * 256ull + threadIdx.x; g < total; g += gridDim.x * 256ull) {
     const uint32_t l = static_cast<uint32_t>(g & 63);
@@ -192,6 +195,7 @@ __global__ void knn_frag8_kernel(const float *__restrict__ x, uint64_t n, uint32
     const uint32_t kc = static_cast<uint32_t>(tk % kch);
     const uint64_t t = tk / kch;
     const uint64_t row = t * 16 + (l & 15);
+    const uint64_t src = row < n ? perm[row] : 0;
     const uint32_t f0 = kc * 128 + 32 * (l >> 4);
     const float ri = row < n ? rinv[row] : 0.0f;
     const float sc = ri > 0.0f ? 1.0f / ri : 0.0f;
@@ -203,7 +207,7 @@ __global__ void knn_frag8_kernel(const float *__restrict__ x, uint64_t n, uint32
       for (int e = 0; e < 4; ++e) {
         const uint32_t f = f0 + 4 * q + e;
         float c = 0.0f;
-        if (row < n && f < d) c = static_cast<float>(static_cast<double>(x[row * d + f]) - mean[f]);
+        if (row < n && f < d) c = static_cast<float>(static_cast<double>(x[src * d + f]) - mean[f]);
         word |= f32_to_e4m3(c * sc) << (8 * e);
       }
       w[q] = word;
@@ -213,9 +217,12 @@ __global__ void knn_frag8_kernel(const float *__restrict__ x, uint64_t n, uint32
   }
 }
 
+// Position r of the candidate order holds row perm[r]: arow[r] (and
+// arow_orig[perm[r]], the same value by row) = |c|^2 (1 - kappa).
 __global__ void knn_rownorm_kernel(const float *__restrict__ x, uint64_t n, uint32_t d,
                                    const double *__restrict__ mean, float one_minus_kappa,
-                                   uint64_t rows_padded, float *__restrict__ arow,
+                                   uint64_t rows_padded, const uint32_t *__restrict__ perm,
+                                   float *__restrict__ arow, float *__restrict__ arow_orig,
                                    uint32_t *__restrict__ nonfinite, float *__restrict__ rinv) {
   const uint64_t row = blockIdx.x * 256ull + threadIdx.x;
   if (row >= rows_padded) return;
@@ -224,19 +231,234 @@ __global__ void knn_rownorm_kernel(const float *__restrict__ x, uint64_t n, uint
     if (rinv) rinv[row] = 0.0f;
     return;
   }
+  const uint64_t src = perm[row];
   double s = 0.0;
   float mx = 0.0f;
   bool bad = false;
   for (uint32_t f = 0; f < d; ++f) {
-    const float xv = x[row * d + f];
+    const float xv = x[src * d + f];
     bad = bad || !isfinite(xv);
     const float c = static_cast<float>(static_cast<double>(xv) - mean[f]);
     s += static_cast<double>(c) * c;
     mx = fmaxf(mx, fabsf(c));
   }
-  arow[row] = static_cast<float>(s * static_cast<double>(one_minus_kappa));
+  const float a = static_cast<float>(s * static_cast<double>(one_minus_kappa));
+  arow[row] = a;
+  arow_orig[src] = a;
   if (rinv) rinv[row] = mx / 448.0f;
   if (bad) atomicOr(nonfinite, 1u);
+}
+
+// ---------------------------------------------------------------------------
+// candidate order and tile-pair pruning
+// ---------------------------------------------------------------------------
+// The candidate kernels work on the rows in the order of the Morton code of
+// their first kKnnOrderDims coordinates (each quantised to 10 bits between
+// its minimum and maximum), so that the 16 rows of a tile lie close
+// together there.  A tile's box is the range of each of its first
+// kKnnBoxDims coordinates.  The squared gaps between two boxes, summed,
+// bound the squared distance of every pair of their rows from below: a
+// candidate tile whose bound to a query group exceeds the largest tau of
+// both is skipped by the symmetric pass, as its tests would all fail.
+constexpr int kKnnOrderDims = 3;
+constexpr int kKnnBoxDims = 8;
+
+__device__ __forceinline__ uint32_t morton_spread(uint32_t v) {
+  v &= 0x3ffu;
+  v = (v | (v << 16)) & 0x030000ffu;
+  v = (v | (v << 8)) & 0x0300f00fu;
+  v = (v | (v << 4)) & 0x030c30c3u;
+  v = (v | (v << 2)) & 0x09249249u;
+  return v;
+}
+
+// mm: kKnnOrderDims minimum keys, then as many maximum keys (tkey_m order)
+__global__ void knn_order_init_kernel(uint32_t *mm) {
+  const int t = threadIdx.x;
+  if (t < kKnnOrderDims) {
+    mm[t] = 0xffffffffu;
+    mm[kKnnOrderDims + t] = 0u;
+  }
+}
+
+__global__ __launch_bounds__(256) void knn_order_minmax_kernel(const float *__restrict__ x, uint64_t n, uint32_t d,
+                                                               uint32_t kd, uint32_t *__restrict__ mm) {
+  uint32_t lo[kKnnOrderDims], hi[kKnnOrderDims];
+#pragma unroll
+  for (int j = 0; j < kKnnOrderDims; ++j) {
+    lo[j] = 0xffffffffu;
+    hi[j] = 0u;
+  }
+  for (uint64_t r = blockIdx.x * 256ull + threadIdx.x; r < n; r += gridDim.x * 256ull)
+#pragma unroll
+    for (int j = 0; j < kKnnOrderDims; ++j)
+      if (static_cast<uint32_t>(j) < kd) {
+        const uint32_t k = tkey_m(x[r * d + j]);
+        lo[j] = min(lo[j], k);
+        hi[j] = max(hi[j], k);
+      }
+#pragma unroll
+  for (int j = 0; j < kKnnOrderDims; ++j) {
+    for (int o = 32; o > 0; o >>= 1) {
+      lo[j] = min(lo[j], static_cast<uint32_t>(__shfl_xor(static_cast<int>(lo[j]), o, 64)));
+      hi[j] = max(hi[j], static_cast<uint32_t>(__shfl_xor(static_cast<int>(hi[j]), o, 64)));
+    }
+    if ((threadIdx.x & 63) == 0 && static_cast<uint32_t>(j) < kd) {
+      atomicMin(&mm[j], lo[j]);
+      atomicMax(&mm[kKnnOrderDims + j], hi[j]);
+    }
+  }
+}
+
+__global__ void knn_order_key_kernel(const float *__restrict__ x, uint64_t n, uint32_t d, uint32_t kd,
+                                     const uint32_t *__restrict__ mm, uint32_t *__restrict__ key,
+                                     uint32_t *__restrict__ idx) {
+  const uint64_t r = blockIdx.x * 256ull + threadIdx.x;
+  if (r >= n) return;
+  uint32_t code = 0;
+#pragma unroll
+  for (int j = 0; j < kKnnOrderDims; ++j) {
+    if (static_cast<uint32_t>(j) >= kd) break;
+    const float lo = from_tkey_m(mm[j]), hi = from_tkey_m(mm[kKnnOrderDims + j]);
+    const float v = x[r * d + j];
+    const float u = hi > lo ? (v - lo) / (hi - lo) * 1024.0f : 0.0f;
+    const uint32_t q = u >= 1023.0f ? 1023u : (u > 0.0f ? static_cast<uint32_t>(u) : 0u);
+    code |= morton_spread(q) << j;
+  }
+  key[r] = code;
+  idx[r] = static_cast<uint32_t>(r);
+}
+
+// box: per tile 2 kKnnBoxDims floats, the minima then the maxima of its
+// rows' first mb coordinates; one thread per (tile, coordinate)
+__global__ void knn_tilebox_kernel(const float *__restrict__ x, uint64_t n, uint32_t d, uint32_t mb,
+                                   const uint32_t *__restrict__ perm, uint64_t ntiles, float *__restrict__ box) {
+  const uint64_t g = blockIdx.x * 256ull + threadIdx.x;
+  const uint64_t t = g / mb;
+  const uint32_t j = static_cast<uint32_t>(g % mb);
+  if (t >= ntiles) return;
+  float lo = __builtin_huge_valf(), hi = -__builtin_huge_valf();
+  for (uint32_t i = 0; i < 16; ++i) {
+    const uint64_t r = t * 16 + i;
+    if (r < n) {
+      const float v = x[static_cast<uint64_t>(perm[r]) * d + j];
+      lo = fminf(lo, v);
+      hi = fmaxf(hi, v);
+    }
+  }
+  box[t * (2 * kKnnBoxDims) + j] = lo;
+  box[t * (2 * kKnnBoxDims) + kKnnBoxDims + j] = hi;
+}
+
+// After the tau sample: the largest tau of each tile, and tau by row
+// (gtau_orig[perm[r]] = gtau[r]) for the rerank.
+__global__ void knn_tiletau_kernel(const float *__restrict__ gtau, const uint32_t *__restrict__ perm, uint64_t n,
+                                   uint64_t ntiles, float *__restrict__ tmax, float *__restrict__ gtau_orig) {
+  const uint64_t t = blockIdx.x * 256ull + threadIdx.x;
+  if (t >= ntiles) return;
+  float m = -__builtin_huge_valf();
+  for (uint32_t i = 0; i < 16; ++i) {
+    const uint64_t r = t * 16 + i;
+    if (r < n) {
+      const float v = gtau[r];
+      m = fmaxf(m, v);
+      gtau_orig[perm[r]] = v;
+    }
+  }
+  tmax[t] = m;
+}
+
+// Per query group (gt tiles): the union of its tiles' boxes and its
+// largest tau (gbox: 2 kKnnBoxDims + 1 floats per group).
+__global__ void knn_groupbox_kernel(const float *__restrict__ box, const float *__restrict__ tmax, uint64_t ntiles,
+                                    uint32_t groups, uint32_t gt, uint32_t mb, float *__restrict__ gbox) {
+  const uint32_t g = blockIdx.x * 256u + threadIdx.x;
+  if (g >= groups) return;
+  float lo[kKnnBoxDims], hi[kKnnBoxDims];
+  float m = -__builtin_huge_valf();
+#pragma unroll
+  for (int j = 0; j < kKnnBoxDims; ++j) {
+    lo[j] = __builtin_huge_valf();
+    hi[j] = -__builtin_huge_valf();
+  }
+  for (uint32_t i = 0; i < gt; ++i) {
+    const uint64_t t = static_cast<uint64_t>(g) * gt + i;
+    if (t >= ntiles) break;
+#pragma unroll
+    for (int j = 0; j < kKnnBoxDims; ++j)
+      if (static_cast<uint32_t>(j) < mb) {
+        lo[j] = fminf(lo[j], box[t * (2 * kKnnBoxDims) + j]);
+        hi[j] = fmaxf(hi[j], box[t * (2 * kKnnBoxDims) + kKnnBoxDims + j]);
+      }
+    m = fmaxf(m, tmax[t]);
+  }
+  float *o = gbox + static_cast<size_t>(g) * (2 * kKnnBoxDims + 1);
+#pragma unroll
+  for (int j = 0; j < kKnnBoxDims; ++j) {
+    o[j] = lo[j];
+    o[kKnnBoxDims + j] = hi[j];
+  }
+  o[2 * kKnnBoxDims] = m;
+}
+
+// Workgroup (word w, 256 query groups): bit i of bits[g][w] is set when
+// tile 32 w + i, a tile after group g's own, can be skipped for the group:
+// the boxes' summed squared gaps exceed the larger of the group's and the
+// tile's largest tau.  Each FP32 gap is a correctly rounded difference, so
+// the sum is within a relative 10^-6 of the exact bound, inside the 10^-5
+// margin.  The word's 32 tile boxes are staged in LDS.
+__global__ __launch_bounds__(256) void knn_prune_kernel(const float *__restrict__ box, const float *__restrict__ tmax,
+                                                        const float *__restrict__ gbox, uint64_t ntiles,
+                                                        uint32_t groups, uint32_t gt, uint32_t mb, uint32_t words,
+                                                        uint32_t *__restrict__ bits,
+                                                        unsigned long long *__restrict__ count) {
+  __shared__ float sb[32][2 * kKnnBoxDims + 1];
+  __shared__ uint32_t scount;
+  if (threadIdx.x == 0) scount = 0;
+  const uint32_t w = blockIdx.x, t0 = w * 32u;
+  for (uint32_t i = threadIdx.x; i < 32u * (2 * kKnnBoxDims); i += 256u) {
+    const uint32_t ti = i / (2 * kKnnBoxDims), j = i % (2 * kKnnBoxDims);
+    const uint64_t t = static_cast<uint64_t>(t0) + ti;
+    sb[ti][j] = t < ntiles ? box[t * (2 * kKnnBoxDims) + j] : 0.0f;
+  }
+  if (threadIdx.x < 32u) {
+    const uint64_t t = static_cast<uint64_t>(t0) + threadIdx.x;
+    sb[threadIdx.x][2 * kKnnBoxDims] = t < ntiles ? tmax[t] : 0.0f;
+  }
+  __syncthreads();
+  const uint32_t g = blockIdx.y * 256u + threadIdx.x;
+  uint32_t out = 0;
+  const uint64_t first = (static_cast<uint64_t>(g) + 1) * gt;  // the first tile after the group's own
+  if (g < groups && static_cast<uint64_t>(t0) + 32 > first) {
+    const float *gb = gbox + static_cast<size_t>(g) * (2 * kKnnBoxDims + 1);
+    float glo[kKnnBoxDims], ghi[kKnnBoxDims];
+#pragma unroll
+    for (int j = 0; j < kKnnBoxDims; ++j) {
+      glo[j] = gb[j];
+      ghi[j] = gb[kKnnBoxDims + j];
+    }
+    const float gm = gb[2 * kKnnBoxDims];
+    for (uint32_t i = 0; i < 32; ++i) {
+      const uint64_t t = static_cast<uint64_t>(t0) + i;
+      if (t >= ntiles) break;
+      if (t < first) continue;
+      float b = 0.0f;
+#pragma unroll
+      for (int j = 0; j < kKnnBoxDims; ++j)
+        if (static_cast<uint32_t>(j) < mb) {
+          const float gap = fmaxf(0.0f, fmaxf(sb[i][j] - ghi[j], glo[j] - sb[i][kKnnBoxDims + j]));
+          b = fmaf(gap, gap, b);
+        }
+      if (b * (1.0f - 1e-5f) > fmaxf(gm, sb[i][2 * kKnnBoxDims])) out |= 1u << i;
+    }
+  }
+  if (g < groups) bits[static_cast<size_t>(g) * words + w] = out;
+  // skipped pairs, for the timings: one atomic per workgroup
+  uint32_t c = static_cast<uint32_t>(__popc(out));
+  for (int o = 32; o > 0; o >>= 1) c += static_cast<uint32_t>(__shfl_xor(static_cast<int>(c), o, 64));
+  if ((threadIdx.x & 63) == 0) atomicAdd(&scount, c);
+  __syncthreads();
+  if (threadIdx.x == 0 && scount) atomicAdd(count, static_cast<unsigned long long>(scount));
 }
 
 // Item b of the symmetric pass (see knn_sym_plan): windows of L = m gt
@@ -299,7 +521,7 @@ void knn_mfma_kernel(
     const uint4 *__restrict__ fhi, const uint4 *__restrict__ flo, const float *__restrict__ arow,
     const float *__restrict__ rinv, uint32_t n, uint32_t ntiles, uint32_t ss, uint32_t capg,
     uint32_t *__restrict__ gcnt, uint32_t *__restrict__ gcand, float *__restrict__ gtau,
-    KnnSymPlan sp) {
+    KnnSymPlan sp, const uint32_t *__restrict__ perm, const uint32_t *__restrict__ prune, uint32_t pwords) {
   static_assert(PHASE != 2 || RT <= 2, "phase 2 holds a lane's tests of two tiles in 32 bits");
   constexpr bool SPLIT = MODE == 1;
   constexpr bool FP8 = MODE == 2;
@@ -311,6 +533,10 @@ void knn_mfma_kernel(
   __shared__ float sA[2][CT][16];
   __shared__ float sI[2][CT][16];  // FP8: candidate rinv
   __shared__ uint32_t sCnt[4][16 * RT];
+  // rows by position (perm[position], ~0 past n): the wave's query rows and
+  // each stage's candidates
+  __shared__ uint32_t sQo[4][16 * RT];
+  __shared__ uint32_t sP[2][CT][16];
   // PHASE 2: (a_c - tau_c) / 2 of the stage's candidates, and each wave's
   // staged appends (row, candidate, dot)
   constexpr uint32_t kApp = 128;
@@ -350,6 +576,10 @@ void knn_mfma_kernel(
     }
   }
   for (int i = threadIdx.x; i < 4 * 16 * RT; i += 256) (&sCnt[0][0])[i] = 0;
+  if (lane < 16 * RT) {
+    const uint32_t q = qtile0 * 16 + static_cast<uint32_t>(lane);
+    sQo[wave][lane] = q < n ? perm[q] : 0xffffffffu;
+  }
 
   // Phase 1 tests lb = aq + ac - 2 dot <= tau as dot >= hq + ac / 2 with
   // hq = (aq - tau) / 2 per row (held in tau[]): two operations per pair.
@@ -380,6 +610,17 @@ void knn_mfma_kernel(
 
   uint4 pf[PF];
   float pa = 0.0f, pi = 0.0f, pt = 0.0f;
+  uint32_t pp = 0xffffffffu;
+  // phase 2: the stage's candidate tiles can all be skipped for this group
+  // (see knn_prune_kernel; a uniform condition)
+  auto stage_pruned = [&](uint32_t st) -> bool {
+    if (PHASE != 2 || pwords == 0) return false;
+    const uint32_t t0 = slot_lo + st * CT;
+    const uint32_t nv = min(static_cast<uint32_t>(CT), slot_hi - t0);
+    const uint32_t word = prune[static_cast<size_t>(group) * pwords + (t0 >> 5)];
+    const uint32_t mask = ((nv >= 32u ? 0u : (1u << nv)) - 1u) << (t0 & 31u);
+    return (word & mask) == mask;
+  };
   // When a stage's half (hi or lo fragments) is a whole number of 256-thread
   // rounds, the stage is fetched through buffer resources: slots past the
   // range point past the fragment arrays and read zeros with no branch, and
@@ -396,6 +637,7 @@ void knn_mfma_kernel(
   // filling one 1 KB fragment block; otherwise into registers for stash)
   auto fetch = [&](uint32_t st, int b) {
     const uint32_t s0 = slot_lo + st * CT;
+    if (stage_pruned(st)) return;
     if constexpr (ROUNDS) {
 #pragma unroll
       for (int h = 0; h < NH; ++h)
@@ -435,6 +677,10 @@ void knn_mfma_kernel(
       const uint32_t tile = slot_tile(slot);
       pa = (slot < slot_hi && tile < ntiles) ? arow[tile * 16 + (threadIdx.x & 15)] : 0.0f;
       if (FP8) pi = (slot < slot_hi && tile < ntiles) ? rinv[tile * 16 + (threadIdx.x & 15)] : 0.0f;
+      {
+        const uint32_t c = tile * 16 + (threadIdx.x & 15);
+        pp = (slot < slot_hi && tile < ntiles && c < n) ? perm[c] : 0xffffffffu;
+      }
       if (PHASE == 2) {
         // rows past n carry a = NaN, so their half-threshold is NaN too
         const uint32_t c = tile * 16 + (threadIdx.x & 15);
@@ -456,6 +702,7 @@ void knn_mfma_kernel(
     }
     if (threadIdx.x < CT * 16) {
       (&sA[buf][0][0])[threadIdx.x] = pa;
+      (&sP[buf][0][0])[threadIdx.x] = pp;
       if (FP8) (&sI[buf][0][0])[threadIdx.x] = pi;
       if (PHASE == 2) (&sT[buf & (PHASE == 2 ? 1 : 0)][0][0])[threadIdx.x] = pt;
     }
@@ -641,12 +888,13 @@ void knn_mfma_kernel(
           if (dot >= tau[rt][r] + gc && (!diag || c != q)) {
             const uint32_t pos = atomicAdd(&sCnt[wave][rt * 16 + 4 * rsub + r], 1u);
             if (pos < capg) {
-              const size_t slot = (static_cast<size_t>(q) * nsplit + split) * capg + pos;
+              const uint32_t qo = sQo[wave][rt * 16 + 4 * rsub + r], co = sP[buf][ct][col];
+              const size_t slot = (static_cast<size_t>(qo) * nsplit + split) * capg + pos;
               // split lists carry the pair's dot, for the rerank's upper bounds
               if (SPLIT)
-                reinterpret_cast<uint2 *>(gcand)[slot] = make_uint2(c, __float_as_uint(dot));
+                reinterpret_cast<uint2 *>(gcand)[slot] = make_uint2(co, __float_as_uint(dot));
               else
-                gcand[slot] = c;
+                gcand[slot] = co;
             }
           }
         }
@@ -712,8 +960,8 @@ void knn_mfma_kernel(
 #pragma unroll
             for (int j = 0; j < w; ++j) lv[j] = (di & static_cast<uint32_t>(w)) ? lv[j + w] : lv[j];
           const float dot = lv[0];
-          const uint32_t c = (tile0 + u) * 16u + col;
-          const uint32_t q = (qtile0 + (k >> 2)) * 16u + 4u * rsub + (k & 3u);
+          const uint32_t c = sP[buf][ct + u][col];
+          const uint32_t q = sQo[wave][(k >> 2) * 16u + 4u * rsub + (k & 3u)];
           const bool rev = (i & (4u * RT)) != 0;
           const uint32_t e = napp + __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(m >> 32),
                                                               __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(m), 0u));
@@ -727,8 +975,9 @@ void knn_mfma_kernel(
     };
     // two candidate tiles at a time: their independent MFMA chains interleave
     // (slots past the range hold zero fragments; their epilogue is skipped)
+    const bool skip = stage_pruned(st);
 #pragma unroll 1
-    for (int ct = 0; ct < CT; ct += 2) {
+    for (int ct = 0; ct < (skip ? 0 : CT); ct += 2) {
       const uint32_t slot0 = slot_lo + st * CT + ct;
       const uint32_t tile0 = slot_tile(slot0);
       if (slot0 >= slot_hi || tile0 >= ntiles) break;
@@ -781,7 +1030,8 @@ void knn_mfma_kernel(
         if (PHASE == 0)
           gtau[q] = tau[rt][r];
         else
-          gcnt[static_cast<size_t>(q) * nsplit + split] = sCnt[wave][rt * 16 + 4 * rsub + r];
+          gcnt[static_cast<size_t>(sQo[wave][rt * 16 + 4 * rsub + r]) * nsplit + split] =
+              sCnt[wave][rt * 16 + 4 * rsub + r];
       }
   }
 }
@@ -1103,33 +1353,34 @@ __global__ __launch_bounds__(256) void knn_rerank_kernel(
 // host launchers
 // ---------------------------------------------------------------------------
 hipError_t launch_knn_frag(const float *x, uint64_t n, uint32_t d, const double *mean, uint32_t kch,
-                           uint64_t ntiles, void *fhi, void *flo, hipStream_t s) {
+                           uint64_t ntiles, const uint32_t *perm, void *fhi, void *flo, hipStream_t s) {
   const uint64_t total = ntiles * kch * 64;
   uint64_t blocks = (total + 255) / 256;
   if (blocks > 65536) blocks = 65536;
   if (blocks == 0) return hipSuccess;
-  knn_frag_kernel<<<static_cast<uint32_t>(blocks), 256, 0, s>>>(x, n, d, mean, kch, ntiles,
+  knn_frag_kernel<<<static_cast<uint32_t>(blocks), 256, 0, s>>>(x, n, d, mean, kch, ntiles, perm,
                                                                 static_cast<uint4 *>(fhi),
                                                                 static_cast<uint4 *>(flo));
   return hipGetLastError();
 }
 
 hipError_t launch_knn_frag8(const float *x, uint64_t n, uint32_t d, const double *mean, uint32_t kch,
-                            uint64_t ntiles, const float *rinv, void *fa, void *fb, hipStream_t s) {
+                            uint64_t ntiles, const float *rinv, const uint32_t *perm, void *fa, void *fb,
+                            hipStream_t s) {
   const uint64_t total = ntiles * kch * 64;
   uint64_t blocks = (total + 255) / 256;
   if (blocks > 65536) blocks = 65536;
   if (blocks == 0) return hipSuccess;
-  knn_frag8_kernel<<<static_cast<uint32_t>(blocks), 256, 0, s>>>(x, n, d, mean, kch, ntiles, rinv,
+  knn_frag8_kernel<<<static_cast<uint32_t>(blocks), 256, 0, s>>>(x, n, d, mean, kch, ntiles, rinv, perm,
                                                                  static_cast<uint4 *>(fa), static_cast<uint4 *>(fb));
   return hipGetLastError();
 }
 
 hipError_t launch_knn_rownorm(const float *x, uint64_t n, uint32_t d, const double *mean,
-                              float one_minus_kappa, uint64_t rows_padded, float *arow,
-                              uint32_t *nonfinite, float *rinv, hipStream_t s) {
+                              float one_minus_kappa, uint64_t rows_padded, const uint32_t *perm, float *arow,
+                              float *arow_orig, uint32_t *nonfinite, float *rinv, hipStream_t s) {
   knn_rownorm_kernel<<<static_cast<uint32_t>((rows_padded + 255) / 256), 256, 0, s>>>(
-      x, n, d, mean, one_minus_kappa, rows_padded, arow, nonfinite, rinv);
+      x, n, d, mean, one_minus_kappa, rows_padded, perm, arow, arow_orig, nonfinite, rinv);
   return hipGetLastError();
 }
 
@@ -1137,22 +1388,78 @@ constexpr int kCandRT = 2;  // query tiles per wave in the candidate kernels
 
 uint32_t knn_cand_groups(uint32_t ntiles) { return (ntiles + 4 * kCandRT - 1) / (4 * kCandRT); }
 
+// 0 when the query fails (the sort then reports the error)
+size_t knn_order_temp_bytes(uint64_t n) {
+  size_t bytes = 0;
+  if (hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, static_cast<const uint32_t *>(nullptr),
+                                         static_cast<uint32_t *>(nullptr), static_cast<const uint32_t *>(nullptr),
+                                         static_cast<uint32_t *>(nullptr), static_cast<int>(n), 0,
+                                         10 * kKnnOrderDims) != hipSuccess)
+    return 0;
+  return bytes;
+}
+
+hipError_t launch_knn_order(const float *x, uint64_t n, uint32_t d, uint32_t *mm, uint32_t *key, uint32_t *key_sorted,
+                            uint32_t *idx, uint32_t *perm, void *temp, size_t temp_bytes, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  const uint32_t kd = d < static_cast<uint32_t>(kKnnOrderDims) ? d : static_cast<uint32_t>(kKnnOrderDims);
+  knn_order_init_kernel<<<1, 64, 0, s>>>(mm);
+  const uint64_t blocks = (n + 255) / 256;
+  knn_order_minmax_kernel<<<static_cast<uint32_t>(blocks < 2048 ? blocks : 2048), 256, 0, s>>>(x, n, d, kd, mm);
+  knn_order_key_kernel<<<static_cast<uint32_t>(blocks), 256, 0, s>>>(x, n, d, kd, mm, key, idx);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  return hipcub::DeviceRadixSort::SortPairs(temp, temp_bytes, key, key_sorted, idx, perm, static_cast<int>(n), 0,
+                                            10 * kKnnOrderDims, s);
+}
+
+uint32_t knn_box_dims(uint32_t d) { return d < static_cast<uint32_t>(kKnnBoxDims) ? d : static_cast<uint32_t>(kKnnBoxDims); }
+
+hipError_t launch_knn_tilebox(const float *x, uint64_t n, uint32_t d, const uint32_t *perm, uint64_t ntiles,
+                              float *box, hipStream_t s) {
+  const uint32_t mb = knn_box_dims(d);
+  const uint64_t total = ntiles * mb;
+  if (total == 0) return hipSuccess;
+  knn_tilebox_kernel<<<static_cast<uint32_t>((total + 255) / 256), 256, 0, s>>>(x, n, d, mb, perm, ntiles, box);
+  return hipGetLastError();
+}
+
+hipError_t launch_knn_tiletau(const float *gtau, const uint32_t *perm, uint64_t n, uint64_t ntiles, float *tmax,
+                              float *gtau_orig, hipStream_t s) {
+  if (ntiles == 0) return hipSuccess;
+  knn_tiletau_kernel<<<static_cast<uint32_t>((ntiles + 255) / 256), 256, 0, s>>>(gtau, perm, n, ntiles, tmax,
+                                                                                gtau_orig);
+  return hipGetLastError();
+}
+
+hipError_t launch_knn_prune(const float *box, const float *tmax, uint64_t ntiles, uint32_t d, uint32_t words,
+                            float *gbox, uint32_t *bits, unsigned long long *count, hipStream_t s) {
+  const uint32_t groups = knn_cand_groups(static_cast<uint32_t>(ntiles));
+  if (groups == 0 || words == 0) return hipSuccess;
+  knn_groupbox_kernel<<<(groups + 255) / 256, 256, 0, s>>>(box, tmax, ntiles, groups, 4 * kCandRT, knn_box_dims(d),
+                                                           gbox);
+  knn_prune_kernel<<<dim3(words, (groups + 255) / 256), 256, 0, s>>>(box, tmax, gbox, ntiles, groups, 4 * kCandRT,
+                                                                     knn_box_dims(d), words, bits, count);
+  return hipGetLastError();
+}
+
+
 template <int KCH, int MODE>
 static hipError_t cand_dispatch(int phase, const void *fhi, const void *flo, const float *arow,
                                 const float *rinv, uint32_t n, uint32_t ntiles, uint32_t ss, uint32_t capg,
                                 uint32_t nsplit, uint32_t *gcnt, uint32_t *gcand, float *gtau,
-                                const KnnSymPlan &sp, hipStream_t s) {
+                                const KnnSymPlan &sp, const KnnOrder &o, hipStream_t s) {
   const uint32_t groups = knn_cand_groups(ntiles);
   const uint4 *h = static_cast<const uint4 *>(fhi), *l = static_cast<const uint4 *>(flo);
   if (phase == 0)
-    knn_mfma_kernel<KCH, kCandRT, MODE, 0><<<dim3(groups, 1), 256, 0, s>>>(h, l, arow, rinv, n, ntiles, ss,
-                                                                          capg, gcnt, gcand, gtau, KnnSymPlan{});
+    knn_mfma_kernel<KCH, kCandRT, MODE, 0><<<dim3(groups, 1), 256, 0, s>>>(
+        h, l, arow, rinv, n, ntiles, ss, capg, gcnt, gcand, gtau, KnnSymPlan{}, o.perm, nullptr, 0u);
   else if (phase == 1)
     knn_mfma_kernel<KCH, kCandRT, MODE, 1><<<dim3(groups, nsplit), 256, 0, s>>>(
-        h, l, arow, rinv, n, ntiles, ss, capg, gcnt, gcand, gtau, KnnSymPlan{});
+        h, l, arow, rinv, n, ntiles, ss, capg, gcnt, gcand, gtau, KnnSymPlan{}, o.perm, nullptr, 0u);
   else if (sp.items)
     knn_mfma_kernel<KCH, kCandRT, MODE, 2><<<dim3(sp.items, 1), 256, 0, s>>>(
-        h, l, arow, rinv, n, ntiles, ss, capg, gcnt, gcand, gtau, sp);
+        h, l, arow, rinv, n, ntiles, ss, capg, gcnt, gcand, gtau, sp, o.perm, o.prune, o.prune ? o.pwords : 0u);
   return hipGetLastError();
 }
 
@@ -1160,12 +1467,12 @@ template <int KCH>
 static hipError_t cand_dispatch_mode(int mode, int phase, const void *fhi, const void *flo,
                                      const float *arow, const float *rinv, uint32_t n, uint32_t ntiles,
                                      uint32_t ss, uint32_t capg, uint32_t nsplit, uint32_t *gcnt,
-                                     uint32_t *gcand, float *gtau, const KnnSymPlan &sp,
+                                     uint32_t *gcand, float *gtau, const KnnSymPlan &sp, const KnnOrder &o,
                                      hipStream_t s) {
   switch (mode) {
-    case 0: return cand_dispatch<KCH, 0>(phase, fhi, flo, arow, rinv, n, ntiles, ss, capg, nsplit, gcnt, gcand, gtau, sp, s);
-    case 1: return cand_dispatch<KCH, 1>(phase, fhi, flo, arow, rinv, n, ntiles, ss, capg, nsplit, gcnt, gcand, gtau, sp, s);
-    case 2: return cand_dispatch<KCH, 2>(phase, fhi, flo, arow, rinv, n, ntiles, ss, capg, nsplit, gcnt, gcand, gtau, sp, s);
+    case 0: return cand_dispatch<KCH, 0>(phase, fhi, flo, arow, rinv, n, ntiles, ss, capg, nsplit, gcnt, gcand, gtau, sp, o, s);
+    case 1: return cand_dispatch<KCH, 1>(phase, fhi, flo, arow, rinv, n, ntiles, ss, capg, nsplit, gcnt, gcand, gtau, sp, o, s);
+    case 2: return cand_dispatch<KCH, 2>(phase, fhi, flo, arow, rinv, n, ntiles, ss, capg, nsplit, gcnt, gcand, gtau, sp, o, s);
     default: return hipErrorInvalidValue;
   }
 }
@@ -1197,13 +1504,14 @@ KnnSymPlan knn_sym_plan(uint32_t ntiles, uint32_t target_items, uint32_t max_win
 hipError_t launch_knn_cand(int phase, const void *fhi, const void *flo, const float *arow, const float *rinv,
                            uint32_t n, uint32_t kch, uint32_t ntiles, uint32_t ss, uint32_t capg,
                            uint32_t nsplit, uint32_t *gcnt, uint32_t *gcand, float *gtau,
-                           int mode, hipStream_t s, const KnnSymPlan &sp) {
+                           int mode, hipStream_t s, const KnnSymPlan &sp, const KnnOrder &o) {
   if (mode == 2 && rinv == nullptr) return hipErrorInvalidValue;
+  if (o.perm == nullptr) return hipErrorInvalidValue;
   if (phase == 2 && sp.items && (sp.L == 0 || sp.L % sp.gt != 0 || sp.m * sp.gt != sp.L || sp.W == 0 ||
                                  sp.gt != 4 * kCandRT))
     return hipErrorInvalidValue;
 #define PM_CAND(K) \
-  case K: return cand_dispatch_mode<K>(mode, phase, fhi, flo, arow, rinv, n, ntiles, ss, capg, nsplit, gcnt, gcand, gtau, sp, s);
+  case K: return cand_dispatch_mode<K>(mode, phase, fhi, flo, arow, rinv, n, ntiles, ss, capg, nsplit, gcnt, gcand, gtau, sp, o, s);
   switch (kch) {
     PM_CAND(1)
     PM_CAND(2)

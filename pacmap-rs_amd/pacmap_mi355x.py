@@ -81,7 +81,8 @@ class Timings(C.Structure):
         ("grad_target_blocks", C.c_uint64), ("grad_sell", C.c_uint64),
         ("grad_sell_entries", C.c_uint64), ("grad_sell_ranges", C.c_uint64), ("graph_reused", C.c_uint64),
         ("knn_listed", C.c_uint64), ("knn_reranked", C.c_uint64), ("grad_paged", C.c_uint64),
-        ("grad_page_words", C.c_uint64), ("grad_page_pages", C.c_uint64)]
+        ("grad_page_words", C.c_uint64), ("grad_page_pages", C.c_uint64), ("knn_tile_pairs", C.c_uint64),
+        ("knn_tile_pairs_skipped", C.c_uint64)]
 
 
 INIT_PCA, INIT_VALUE, INIT_RANDOM = 0, 1, 2

@@ -202,6 +202,9 @@ def _data(kind, n, d, seed):
     if kind == "three":
         base = rng.normal(size=(3, d)).astype(np.float32)
         return base[rng.integers(0, 3, size=n)].copy()
+    if kind == "clusters":  # 12 well-separated clusters, rows in random order
+        base = rng.normal(scale=12.0, size=(12, d)).astype(np.float32)
+        return (base[rng.integers(0, 12, size=n)] + rng.normal(size=(n, d))).astype(np.float32)
     raise ValueError(kind)
 
 
@@ -256,6 +259,27 @@ def test_knn_split_pass_bitwise(monkeypatch, n, d, k, kind, seed):
     idx, dist = pm.knn.find_k_nearest_neighbors(x, k)
     assert np.array_equal(dist.view(np.uint32), ref_d.view(np.uint32))
     assert np.array_equal(idx, ref_i)
+
+
+@pytest.mark.parametrize("n,d,k,seed", [(12000, 100, 60, 131), (9000, 40, 30, 132), (5000, 2, 10, 133)])
+def test_knn_pruned_pass_bitwise(monkeypatch, n, d, k, seed):
+    """The symmetric pass over rows in Morton order of their first three
+    coordinates skips the tile pairs whose boxes lie farther apart than both
+    rows' taus: on well-separated clusters most pairs are skipped, and the
+    neighbours equal the oracle's and the unpruned pass's bit for bit."""
+    x = _data("clusters", n, d, seed)
+    ref_i, ref_d = O.knn_rowwise(x, k, nthreads=16)
+    idx, dist = pm.knn.find_k_nearest_neighbors(x, k)
+    t = pm.last_timings()
+    print(f"pruned: {t['knn_tile_pairs_skipped']} of {t['knn_tile_pairs']} tile pairs, "
+          f"rescanned {t['knn_rescanned_rows']}", flush=True)
+    assert t["knn_tile_pairs"] > 0 and t["knn_tile_pairs_skipped"] > t["knn_tile_pairs"] // 2
+    assert np.array_equal(dist.view(np.uint32), ref_d.view(np.uint32))
+    assert np.array_equal(idx, ref_i)
+    monkeypatch.setenv("PACMAP_KNN_PRUNE", "0")
+    idx0, dist0 = pm.knn.find_k_nearest_neighbors(x, k)
+    assert pm.last_timings()["knn_tile_pairs_skipped"] == 0
+    assert np.array_equal(dist0.view(np.uint32), dist.view(np.uint32)) and np.array_equal(idx0, idx)
 
 
 @pytest.mark.parametrize("n,d,k,seed", [(6000, 100, 60, 111), (5000, 256, 30, 112), (4000, 20, 10, 113)])
