@@ -144,14 +144,14 @@ bool knn_mfma_path(const float *x_dev, uint64_t n, uint64_t d, uint32_t k, const
     sym = splan.items != 0;
   }
   // The candidate passes see the rows sorted by the Morton code of their
-  // first three coordinates, so that a tile's rows lie close together and
+  // first six coordinates, so that a tile's rows lie close together and
   // the symmetric pass can skip tile pairs whose boxes lie apart
   // (PACMAP_KNN_PRUNE=0: the rows' own order and no skipping).
   bool prune = sym;
   if (const char *e = std::getenv("PACMAP_KNN_PRUNE")) prune = prune && std::atoi(e) != 0;
   DevBuf<uint32_t> perm(n);
   {
-    DevBuf<uint32_t> omm(8), okey(n), okey2(n), oidx(n);
+    DevBuf<uint32_t> omm(16), okey(n), okey2(n), oidx(n);
     const size_t tb = knn_order_temp_bytes(n);
     DevBuf<uint8_t> otemp(std::max<size_t>(tb, 16));
     PM_HIP(launch_knn_order(x_dev, n, prune ? static_cast<uint32_t>(d) : 0u, omm.get(), okey.get(), okey2.get(),

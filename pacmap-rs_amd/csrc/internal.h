@@ -77,8 +77,8 @@ hipError_t launch_knn_frag8(const float *x, uint64_t n, uint32_t d, const double
 hipError_t launch_knn_rownorm(const float *x, uint64_t n, uint32_t d, const double *mean,
                               float one_minus_kappa, uint64_t rows_padded, const uint32_t *perm, float *arow,
                               float *arow_orig, uint32_t *nonfinite, float *rinv, hipStream_t s);
-// perm: the rows sorted by the Morton code of their first 3 coordinates
-// (mm: 6 words; key, key_sorted, idx: n words; temp: knn_order_temp_bytes(n))
+// perm: the rows sorted by the Morton code of their first 6 coordinates
+// (mm: 12 words; key, key_sorted, idx: n words; temp: knn_order_temp_bytes(n))
 size_t knn_order_temp_bytes(uint64_t n);
 hipError_t launch_knn_order(const float *x, uint64_t n, uint32_t d, uint32_t *mm, uint32_t *key, uint32_t *key_sorted,
                             uint32_t *idx, uint32_t *perm, void *temp, size_t temp_bytes, hipStream_t s);

@@ -253,23 +253,23 @@ __global__ void knn_rownorm_kernel(const float *__restrict__ x, uint64_t n, uint
 // candidate order and tile-pair pruning
 // ---------------------------------------------------------------------------
 // The candidate kernels work on the rows in the order of the Morton code of
-// their first kKnnOrderDims coordinates (each quantised to 10 bits between
-// its minimum and maximum), so that the 16 rows of a tile lie close
-// together there.  A tile's box is the range of each of its first
+// their first kKnnOrderDims coordinates (each quantised to kKnnOrderBits
+// bits between its minimum and maximum), so that the 16 rows of a tile lie
+// close together there.  A tile's box is the range of each of its first
 // kKnnBoxDims coordinates.  The squared gaps between two boxes, summed,
 // bound the squared distance of every pair of their rows from below: a
 // candidate tile whose bound to a query group exceeds the largest tau of
 // both is skipped by the symmetric pass, as its tests would all fail.
-constexpr int kKnnOrderDims = 3;
+constexpr int kKnnOrderDims = 6;
+constexpr int kKnnOrderBits = 30 / kKnnOrderDims;
 constexpr int kKnnBoxDims = 8;
 
+// bit b of v to bit b kKnnOrderDims
 __device__ __forceinline__ uint32_t morton_spread(uint32_t v) {
-  v &= 0x3ffu;
-  v = (v | (v << 16)) & 0x030000ffu;
-  v = (v | (v << 8)) & 0x0300f00fu;
-  v = (v | (v << 4)) & 0x030c30c3u;
-  v = (v | (v << 2)) & 0x09249249u;
-  return v;
+  uint32_t out = 0;
+#pragma unroll
+  for (int b = 0; b < kKnnOrderBits; ++b) out |= ((v >> b) & 1u) << (b * kKnnOrderDims);
+  return out;
 }
 
 // mm: kKnnOrderDims minimum keys, then as many maximum keys (tkey_m order)
@@ -321,8 +321,9 @@ __global__ void knn_order_key_kernel(const float *__restrict__ x, uint64_t n, ui
     if (static_cast<uint32_t>(j) >= kd) break;
     const float lo = from_tkey_m(mm[j]), hi = from_tkey_m(mm[kKnnOrderDims + j]);
     const float v = x[r * d + j];
-    const float u = hi > lo ? (v - lo) / (hi - lo) * 1024.0f : 0.0f;
-    const uint32_t q = u >= 1023.0f ? 1023u : (u > 0.0f ? static_cast<uint32_t>(u) : 0u);
+    constexpr float levels = static_cast<float>(1u << kKnnOrderBits);
+    const float u = hi > lo ? (v - lo) / (hi - lo) * levels : 0.0f;
+    const uint32_t q = u >= levels - 1.0f ? (1u << kKnnOrderBits) - 1u : (u > 0.0f ? static_cast<uint32_t>(u) : 0u);
     code |= morton_spread(q) << j;
   }
   key[r] = code;
@@ -1394,7 +1395,7 @@ size_t knn_order_temp_bytes(uint64_t n) {
   if (hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, static_cast<const uint32_t *>(nullptr),
                                          static_cast<uint32_t *>(nullptr), static_cast<const uint32_t *>(nullptr),
                                          static_cast<uint32_t *>(nullptr), static_cast<int>(n), 0,
-                                         10 * kKnnOrderDims) != hipSuccess)
+                                         kKnnOrderBits * kKnnOrderDims) != hipSuccess)
     return 0;
   return bytes;
 }
@@ -1410,7 +1411,7 @@ hipError_t launch_knn_order(const float *x, uint64_t n, uint32_t d, uint32_t *mm
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   return hipcub::DeviceRadixSort::SortPairs(temp, temp_bytes, key, key_sorted, idx, perm, static_cast<int>(n), 0,
-                                            10 * kKnnOrderDims, s);
+                                            kKnnOrderBits * kKnnOrderDims, s);
 }
 
 uint32_t knn_box_dims(uint32_t d) { return d < static_cast<uint32_t>(kKnnBoxDims) ? d : static_cast<uint32_t>(kKnnBoxDims); }

@@ -263,7 +263,7 @@ def test_knn_split_pass_bitwise(monkeypatch, n, d, k, kind, seed):
 
 @pytest.mark.parametrize("n,d,k,seed", [(12000, 100, 60, 131), (9000, 40, 30, 132), (5000, 2, 10, 133)])
 def test_knn_pruned_pass_bitwise(monkeypatch, n, d, k, seed):
-    """The symmetric pass over rows in Morton order of their first three
+    """The symmetric pass over rows in Morton order of their first six
     coordinates skips the tile pairs whose boxes lie farther apart than both
     rows' taus: on well-separated clusters most pairs are skipped, and the
     neighbours equal the oracle's and the unpruned pass's bit for bit."""
