@@ -218,30 +218,44 @@ __global__ void knn_frag8_kernel(const float *__restrict__ x, uint64_t n, uint32
 }
 
 // Position r of the candidate order holds row perm[r]: arow[r] (and
-// arow_orig[perm[r]], the same value by row) = |c|^2 (1 - kappa).
-__global__ void knn_rownorm_kernel(const float *__restrict__ x, uint64_t n, uint32_t d,
-                                   const double *__restrict__ mean, float one_minus_kappa,
-                                   uint64_t rows_padded, const uint32_t *__restrict__ perm,
-                                   float *__restrict__ arow, float *__restrict__ arow_orig,
-                                   uint32_t *__restrict__ nonfinite, float *__restrict__ rinv) {
-  const uint64_t row = blockIdx.x * 256ull + threadIdx.x;
-  if (row >= rows_padded) return;
+// arow_orig[perm[r]], the same value by row) = |c|^2 (1 - kappa).  L lanes
+// share a row and read it in coalesced runs; the double sum of the squares
+// is reduced over the L lanes before its one rounding to float.
+template <int L>
+__global__ __launch_bounds__(256) void knn_rownorm_kernel(const float *__restrict__ x, uint64_t n, uint32_t d,
+                                                          const double *__restrict__ mean, float one_minus_kappa,
+                                                          uint64_t rows_padded, const uint32_t *__restrict__ perm,
+                                                          float *__restrict__ arow, float *__restrict__ arow_orig,
+                                                          uint32_t *__restrict__ nonfinite, float *__restrict__ rinv) {
+  const uint64_t row = blockIdx.x * static_cast<uint64_t>(256 / L) + threadIdx.x / L;
+  const uint32_t j = threadIdx.x % L;
+  if (row >= rows_padded) return;  // uniform over the row's L lanes
   if (row >= n) {  // padding rows: NaN makes every lower bound against them fail
-    arow[row] = __builtin_nanf("");
-    if (rinv) rinv[row] = 0.0f;
+    if (j == 0) {
+      arow[row] = __builtin_nanf("");
+      if (rinv) rinv[row] = 0.0f;
+    }
     return;
   }
   const uint64_t src = perm[row];
+  const float *xr = x + src * d;
   double s = 0.0;
   float mx = 0.0f;
   bool bad = false;
-  for (uint32_t f = 0; f < d; ++f) {
-    const float xv = x[src * d + f];
+  for (uint32_t f = j; f < d; f += L) {
+    const float xv = xr[f];
     bad = bad || !isfinite(xv);
     const float c = static_cast<float>(static_cast<double>(xv) - mean[f]);
     s += static_cast<double>(c) * c;
     mx = fmaxf(mx, fabsf(c));
   }
+#pragma unroll
+  for (int o = L / 2; o > 0; o >>= 1) {
+    s += __shfl_xor(s, o, L);
+    mx = fmaxf(mx, __shfl_xor(mx, o, L));
+    bad = bad || __shfl_xor(static_cast<int>(bad), o, L) != 0;
+  }
+  if (j != 0) return;
   const float a = static_cast<float>(s * static_cast<double>(one_minus_kappa));
   arow[row] = a;
   arow_orig[src] = a;
@@ -297,16 +311,27 @@ __global__ __launch_bounds__(256) void knn_order_minmax_kernel(const float *__re
         lo[j] = min(lo[j], k);
         hi[j] = max(hi[j], k);
       }
+  // the workgroup's extremes first (LDS), then one atomic per coordinate
+  __shared__ uint32_t sl[4][2 * kKnnOrderDims];
+  const int wave = threadIdx.x >> 6;
 #pragma unroll
   for (int j = 0; j < kKnnOrderDims; ++j) {
     for (int o = 32; o > 0; o >>= 1) {
       lo[j] = min(lo[j], static_cast<uint32_t>(__shfl_xor(static_cast<int>(lo[j]), o, 64)));
       hi[j] = max(hi[j], static_cast<uint32_t>(__shfl_xor(static_cast<int>(hi[j]), o, 64)));
     }
-    if ((threadIdx.x & 63) == 0 && static_cast<uint32_t>(j) < kd) {
-      atomicMin(&mm[j], lo[j]);
-      atomicMax(&mm[kKnnOrderDims + j], hi[j]);
+    if ((threadIdx.x & 63) == 0) {
+      sl[wave][j] = lo[j];
+      sl[wave][kKnnOrderDims + j] = hi[j];
     }
+  }
+  __syncthreads();
+  if (threadIdx.x < static_cast<uint32_t>(kKnnOrderDims) && threadIdx.x < kd) {
+    const int j = threadIdx.x;
+    atomicMin(&mm[j], min(min(sl[0][j], sl[1][j]), min(sl[2][j], sl[3][j])));
+    atomicMax(&mm[kKnnOrderDims + j],
+              max(max(sl[0][kKnnOrderDims + j], sl[1][kKnnOrderDims + j]),
+                  max(sl[2][kKnnOrderDims + j], sl[3][kKnnOrderDims + j])));
   }
 }
 
@@ -1380,8 +1405,12 @@ hipError_t launch_knn_frag8(const float *x, uint64_t n, uint32_t d, const double
 hipError_t launch_knn_rownorm(const float *x, uint64_t n, uint32_t d, const double *mean,
                               float one_minus_kappa, uint64_t rows_padded, const uint32_t *perm, float *arow,
                               float *arow_orig, uint32_t *nonfinite, float *rinv, hipStream_t s) {
-  knn_rownorm_kernel<<<static_cast<uint32_t>((rows_padded + 255) / 256), 256, 0, s>>>(
-      x, n, d, mean, one_minus_kappa, rows_padded, perm, arow, arow_orig, nonfinite, rinv);
+  if (d >= 256)
+    knn_rownorm_kernel<64><<<static_cast<uint32_t>((rows_padded + 3) / 4), 256, 0, s>>>(
+        x, n, d, mean, one_minus_kappa, rows_padded, perm, arow, arow_orig, nonfinite, rinv);
+  else
+    knn_rownorm_kernel<16><<<static_cast<uint32_t>((rows_padded + 15) / 16), 256, 0, s>>>(
+        x, n, d, mean, one_minus_kappa, rows_padded, perm, arow, arow_orig, nonfinite, rinv);
   return hipGetLastError();
 }
 
@@ -1406,7 +1435,7 @@ hipError_t launch_knn_order(const float *x, uint64_t n, uint32_t d, uint32_t *mm
   const uint32_t kd = d < static_cast<uint32_t>(kKnnOrderDims) ? d : static_cast<uint32_t>(kKnnOrderDims);
   knn_order_init_kernel<<<1, 64, 0, s>>>(mm);
   const uint64_t blocks = (n + 255) / 256;
-  knn_order_minmax_kernel<<<static_cast<uint32_t>(blocks < 2048 ? blocks : 2048), 256, 0, s>>>(x, n, d, kd, mm);
+  knn_order_minmax_kernel<<<static_cast<uint32_t>(blocks < 512 ? blocks : 512), 256, 0, s>>>(x, n, d, kd, mm);
   knn_order_key_kernel<<<static_cast<uint32_t>(blocks), 256, 0, s>>>(x, n, d, kd, mm, key, idx);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
