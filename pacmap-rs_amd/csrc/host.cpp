@@ -170,6 +170,22 @@ bool knn_mfma_path(const float *x_dev, uint64_t n, uint64_t d, uint32_t k, const
   else
     PM_HIP(launch_knn_frag(x_dev, n, static_cast<uint32_t>(d), mean.get(), kch, ntiles, perm.get(), fhi.get(),
                            split ? flo.get() : nullptr, s));
+  // the tau sample: ceil(n / ss) rows spread over the input, in tiles of
+  // their own (in the candidate order a tile's rows are neighbours, and a
+  // sample of whole tiles would then set tau from a row's nearest tile)
+  const uint64_t srows = (n + ss - 1) / ss, stiles = (srows + 15) / 16;
+  DevBuf<uint32_t> srow(srows), sfhi(stiles * kch * 64 * 4), sflo(mode != 0 ? stiles * kch * 64 * 4 : 4);
+  DevBuf<float> sarow(stiles * 16), srinv(mode == 2 ? stiles * 16 : 1);
+  PM_HIP(launch_knn_sample_rows(n, srows, knn_sample_stride(n), srow.get(), s));
+  PM_HIP(launch_knn_rownorm(x_dev, srows, static_cast<uint32_t>(d), mean.get(), static_cast<float>(1.0 - kappa),
+                            stiles * 16, srow.get(), sarow.get(), arow_orig.get(), flags.get(),
+                            mode == 2 ? srinv.get() : nullptr, s));
+  if (mode == 2)
+    PM_HIP(launch_knn_frag8(x_dev, srows, static_cast<uint32_t>(d), mean.get(), kch, stiles, srinv.get(), srow.get(),
+                            sfhi.get(), sflo.get(), s));
+  else
+    PM_HIP(launch_knn_frag(x_dev, srows, static_cast<uint32_t>(d), mean.get(), kch, stiles, srow.get(), sfhi.get(),
+                           split ? sflo.get() : nullptr, s));
   DevBuf<float> box(prune ? ntiles * 16 : 1);
   if (prune) PM_HIP(launch_knn_tilebox(x_dev, n, static_cast<uint32_t>(d), perm.get(), ntiles, box.get(), s));
   // the finiteness flag is read with the rerank's results: non-finite
@@ -190,6 +206,11 @@ bool knn_mfma_path(const float *x_dev, uint64_t n, uint64_t d, uint32_t k, const
   tsamp.start(s);
   KnnOrder ord;
   ord.perm = perm.get();
+  ord.sample.fhi = reinterpret_cast<const uint4 *>(sfhi.get());
+  ord.sample.flo = reinterpret_cast<const uint4 *>(mode != 0 ? sflo.get() : sfhi.get());
+  ord.sample.arow = sarow.get();
+  ord.sample.rinv = mode == 2 ? srinv.get() : nullptr;
+  ord.sample.tiles = static_cast<uint32_t>(stiles);
   PM_HIP(launch_knn_cand(0, fhi.get(), flo_p, arow.get(), rinv_p, static_cast<uint32_t>(n), kch,
                          static_cast<uint32_t>(ntiles), ss, capg, nsplit, gcnt.get(), gcand.get(),
                          gtau.get(), mode, s, KnnSymPlan{}, ord));

@@ -368,6 +368,8 @@ void upload_bytes(void *dst, const void *src, size_t bytes, hipStream_t s);
 void upload_rows_pieces(void *dst, const void *src, uint64_t rows, size_t row_bytes, uint64_t piece_rows,
                         hipStream_t s, hipStream_t cs, const std::function<void(uint32_t, hipEvent_t)> &on_piece);
 void staging_release();
+// frees the calling thread's cached encoder transform weights
+void mlp_transform_cache_release();
 template <class T>
 void upload(DevBuf<T> &dst, const T *src, size_t count, hipStream_t s) {
   if (count) upload_bytes(dst.get(), src, count * sizeof(T), s);

@@ -115,11 +115,26 @@ struct KnnSymPlan {
 KnnSymPlan knn_sym_plan(uint32_t ntiles, uint32_t target_items, uint32_t max_window);
 // perm maps positions to rows (lists and counters are by row, gtau by
 // position); prune (phase 2, optional) holds pwords words per query group.
+// The tau sample's candidates: fragments and a values of `tiles` tiles of
+// rows spread over the whole input (knn_sample_rows), so that the 16 rows
+// of a sample tile are not neighbours in the candidate order.
+struct KnnSampleSet {
+  const uint4 *fhi = nullptr;
+  const uint4 *flo = nullptr;
+  const float *arow = nullptr;
+  const float *rinv = nullptr;
+  uint32_t tiles = 0;
+};
 struct KnnOrder {
   const uint32_t *perm = nullptr;
   const uint32_t *prune = nullptr;
   uint32_t pwords = 0;
+  KnnSampleSet sample;
 };
+// rows[p] = (stride p) mod n for p < m: m rows spread over [0, n) (stride
+// coprime to n, near 0.618 n)
+uint32_t knn_sample_stride(uint64_t n);
+hipError_t launch_knn_sample_rows(uint64_t n, uint64_t m, uint32_t stride, uint32_t *rows, hipStream_t s);
 hipError_t launch_knn_cand(int phase, const void *fhi, const void *flo, const float *arow, const float *rinv,
                            uint32_t n, uint32_t kch, uint32_t ntiles, uint32_t ss, uint32_t capg,
                            uint32_t nsplit, uint32_t *gcnt, uint32_t *gcand, float *gtau,

@@ -547,7 +547,8 @@ void knn_mfma_kernel(
     const uint4 *__restrict__ fhi, const uint4 *__restrict__ flo, const float *__restrict__ arow,
     const float *__restrict__ rinv, uint32_t n, uint32_t ntiles, uint32_t ss, uint32_t capg,
     uint32_t *__restrict__ gcnt, uint32_t *__restrict__ gcand, float *__restrict__ gtau,
-    KnnSymPlan sp, const uint32_t *__restrict__ perm, const uint32_t *__restrict__ prune, uint32_t pwords) {
+    KnnSymPlan sp, const uint32_t *__restrict__ perm, const uint32_t *__restrict__ prune, uint32_t pwords,
+    KnnSampleSet smp) {
   static_assert(PHASE != 2 || RT <= 2, "phase 2 holds a lane's tests of two tiles in 32 bits");
   constexpr bool SPLIT = MODE == 1;
   constexpr bool FP8 = MODE == 2;
@@ -624,14 +625,22 @@ void knn_mfma_kernel(
 
   // candidate slots: sample tiles (phase 0), this split's range (phase 1)
   // or the item's range (phase 2)
-  const uint32_t nsample = (ntiles + ss - 1) / ss;
+  // PHASE 0 scans the sample set's tiles (rows drawn across the whole
+  // input, so that a sample tile's rows are not neighbours), every other
+  // phase the query order's own tiles
+  const uint4 *__restrict__ cfhi = PHASE == 0 ? smp.fhi : fhi;
+  const uint4 *__restrict__ cflo = PHASE == 0 ? smp.flo : flo;
+  const float *__restrict__ carow = PHASE == 0 ? smp.arow : arow;
+  const float *__restrict__ crinv = PHASE == 0 ? smp.rinv : rinv;
+  const uint32_t ctiles = PHASE == 0 ? smp.tiles : ntiles;
+  const uint32_t nsample = ctiles;
   const uint32_t slot_lo = PHASE == 0   ? 0
                            : PHASE == 2 ? item_lo
                                         : static_cast<uint32_t>(static_cast<uint64_t>(ntiles) * split / nsplit);
   const uint32_t slot_hi = PHASE == 0   ? nsample
                            : PHASE == 2 ? item_hi
                                         : static_cast<uint32_t>(static_cast<uint64_t>(ntiles) * (split + 1) / nsplit);
-  auto slot_tile = [&](uint32_t slot) -> uint32_t { return PHASE == 0 ? slot * ss : slot; };
+  auto slot_tile = [&](uint32_t slot) -> uint32_t { return slot; };
   const uint32_t nstage = (slot_hi - slot_lo + CT - 1) / CT;
 
   uint4 pf[PF];
@@ -654,11 +663,11 @@ void knn_mfma_kernel(
   constexpr int PERH = CT * KCH * 64;
   constexpr bool ROUNDS = PERH % 256 == 0;
   constexpr int PFH = ROUNDS ? PERH / 256 : 1;
-  const uint32_t frag_bytes = ntiles * static_cast<uint32_t>(KCH) * 64u * 16u;
+  const uint32_t frag_bytes = ctiles * static_cast<uint32_t>(KCH) * 64u * 16u;
   const __amdgpu_buffer_rsrc_t rsh =
-      __builtin_amdgcn_make_buffer_rsrc(const_cast<uint4 *>(fhi), 0, static_cast<int>(frag_bytes), 0x00020000);
+      __builtin_amdgcn_make_buffer_rsrc(const_cast<uint4 *>(cfhi), 0, static_cast<int>(frag_bytes), 0x00020000);
   const __amdgpu_buffer_rsrc_t rsl = __builtin_amdgcn_make_buffer_rsrc(
-      const_cast<uint4 *>(NH == 2 ? flo : fhi), 0, static_cast<int>(frag_bytes), 0x00020000);
+      const_cast<uint4 *>(NH == 2 ? cflo : cfhi), 0, static_cast<int>(frag_bytes), 0x00020000);
   // stage st into LDS buffer b (ROUNDS: straight to LDS, a wave's 64 lanes
   // filling one 1 KB fragment block; otherwise into registers for stash)
   auto fetch = [&](uint32_t st, int b) {
@@ -673,7 +682,7 @@ void knn_mfma_kernel(
           const int l = i & 63, kc = (i >> 6) % KCH, ct = i / (64 * KCH);
           const uint32_t slot = s0 + ct;
           const uint32_t tile = slot_tile(slot);
-          const uint32_t o = (slot < slot_hi && tile < ntiles)
+          const uint32_t o = (slot < slot_hi && tile < ctiles)
                                  ? ((tile * static_cast<uint32_t>(KCH) + kc) * 64u + l) * 16u
                                  : 0xfffffff0u;
           // reads past the fragments (slots past the range) write zeros
@@ -690,9 +699,9 @@ void knn_mfma_kernel(
           const int l = i & 63, h = (i >> 6) % NH, kc = (i / (64 * NH)) % KCH, ct = i / (64 * NH * KCH);
           const uint32_t slot = s0 + ct;
           const uint32_t tile = slot_tile(slot);
-          if (slot < slot_hi && tile < ntiles) {
+          if (slot < slot_hi && tile < ctiles) {
             const size_t o = (static_cast<size_t>(tile) * KCH + kc) * 64 + l;
-            v = h == 0 ? fhi[o] : flo[o];
+            v = h == 0 ? cfhi[o] : cflo[o];
           }
         }
         pf[j] = v;
@@ -701,11 +710,11 @@ void knn_mfma_kernel(
     if (threadIdx.x < CT * 16) {
       const uint32_t slot = s0 + threadIdx.x / 16;
       const uint32_t tile = slot_tile(slot);
-      pa = (slot < slot_hi && tile < ntiles) ? arow[tile * 16 + (threadIdx.x & 15)] : 0.0f;
-      if (FP8) pi = (slot < slot_hi && tile < ntiles) ? rinv[tile * 16 + (threadIdx.x & 15)] : 0.0f;
+      pa = (slot < slot_hi && tile < ctiles) ? carow[tile * 16 + (threadIdx.x & 15)] : 0.0f;
+      if (FP8) pi = (slot < slot_hi && tile < ctiles) ? crinv[tile * 16 + (threadIdx.x & 15)] : 0.0f;
       {
         const uint32_t c = tile * 16 + (threadIdx.x & 15);
-        pp = (slot < slot_hi && tile < ntiles && c < n) ? perm[c] : 0xffffffffu;
+        pp = (PHASE != 0 && slot < slot_hi && tile < ntiles && c < n) ? perm[c] : 0xffffffffu;
       }
       if (PHASE == 2) {
         // rows past n carry a = NaN, so their half-threshold is NaN too
@@ -867,12 +876,12 @@ void knn_mfma_kernel(
       const float gc = ac * 0.5f;
       const bool diag = tile - qtile0 < static_cast<uint32_t>(RT);
       if constexpr (PHASE == 0) {
-        // every lb of the tile first, the self pair set to +inf (only on
-        // the wave's own query tiles, a uniform condition); one wave-wide
-        // test then ends most epilogues once the two smallest have settled
+        // every lb of the tile first; one wave-wide test then ends most
+        // epilogues once the two smallest have settled.  A sampled query
+        // row meets itself once (lb about 0): its tau is then the 15th
+        // smallest of the others, still a valid threshold
         float lbv[RT][4];
         bool any = false;
-        const int rr = col - 4 * rsub;
 #pragma unroll
         for (int rt = 0; rt < RT; ++rt)
 #pragma unroll
@@ -880,13 +889,6 @@ void knn_mfma_kernel(
             const float dot = FP8 ? acc[rt][r] * (iq[rt][r] * ic) : acc[rt][r];
             lbv[rt][r] = fmaf(-2.0f, dot, aq[rt][r] + ac);
           }
-        if (diag) {
-#pragma unroll
-          for (int rt = 0; rt < RT; ++rt)
-#pragma unroll
-            for (int r = 0; r < 4; ++r)
-              if (tile - qtile0 == static_cast<uint32_t>(rt) && rr == r) lbv[rt][r] = __builtin_huge_valf();
-        }
 #pragma unroll
         for (int rt = 0; rt < RT; ++rt)
 #pragma unroll
@@ -1006,7 +1008,7 @@ void knn_mfma_kernel(
     for (int ct = 0; ct < (skip ? 0 : CT); ct += 2) {
       const uint32_t slot0 = slot_lo + st * CT + ct;
       const uint32_t tile0 = slot_tile(slot0);
-      if (slot0 >= slot_hi || tile0 >= ntiles) break;
+      if (slot0 >= slot_hi || tile0 >= ctiles) break;
       floatx4 acc0[RT];
       if (ct + 1 < CT) {
         const uint32_t slot1 = slot0 + 1;
@@ -1014,10 +1016,10 @@ void knn_mfma_kernel(
         floatx4 acc1[RT];
         products2(ct, acc0, acc1);
         if constexpr (PHASE == 2) {
-          list2(ct, tile0, acc0, acc1, slot1 < slot_hi && tile1 < ntiles);
+          list2(ct, tile0, acc0, acc1, slot1 < slot_hi && tile1 < ctiles);
         } else {
           epilogue(ct, tile0, acc0);
-          if (slot1 < slot_hi && tile1 < ntiles) epilogue(ct + 1, tile1, acc1);
+          if (slot1 < slot_hi && tile1 < ctiles) epilogue(ct + 1, tile1, acc1);
         }
       } else {
         products(ct, acc0);
@@ -1443,6 +1445,32 @@ hipError_t launch_knn_order(const float *x, uint64_t n, uint32_t d, uint32_t *mm
                                             kKnnOrderBits * kKnnOrderDims, s);
 }
 
+__global__ void knn_sample_rows_kernel(uint64_t n, uint64_t m, uint32_t stride, uint32_t *__restrict__ rows) {
+  const uint64_t p = blockIdx.x * 256ull + threadIdx.x;
+  if (p < m) rows[p] = static_cast<uint32_t>((static_cast<uint64_t>(stride) * p) % n);
+}
+
+uint32_t knn_sample_stride(uint64_t n) {
+  if (n < 3) return 1u;
+  auto gcd = [](uint64_t a, uint64_t b) {
+    while (b) {
+      const uint64_t t = a % b;
+      a = b;
+      b = t;
+    }
+    return a;
+  };
+  uint64_t a = static_cast<uint64_t>(0.6180339887498949 * static_cast<double>(n)) | 1u;
+  while (gcd(a, n) != 1) a += 2;
+  return static_cast<uint32_t>(a % n);
+}
+
+hipError_t launch_knn_sample_rows(uint64_t n, uint64_t m, uint32_t stride, uint32_t *rows, hipStream_t s) {
+  if (m == 0) return hipSuccess;
+  knn_sample_rows_kernel<<<static_cast<uint32_t>((m + 255) / 256), 256, 0, s>>>(n, m, stride, rows);
+  return hipGetLastError();
+}
+
 uint32_t knn_box_dims(uint32_t d) { return d < static_cast<uint32_t>(kKnnBoxDims) ? d : static_cast<uint32_t>(kKnnBoxDims); }
 
 hipError_t launch_knn_tilebox(const float *x, uint64_t n, uint32_t d, const uint32_t *perm, uint64_t ntiles,
@@ -1483,13 +1511,14 @@ static hipError_t cand_dispatch(int phase, const void *fhi, const void *flo, con
   const uint4 *h = static_cast<const uint4 *>(fhi), *l = static_cast<const uint4 *>(flo);
   if (phase == 0)
     knn_mfma_kernel<KCH, kCandRT, MODE, 0><<<dim3(groups, 1), 256, 0, s>>>(
-        h, l, arow, rinv, n, ntiles, ss, capg, gcnt, gcand, gtau, KnnSymPlan{}, o.perm, nullptr, 0u);
+        h, l, arow, rinv, n, ntiles, ss, capg, gcnt, gcand, gtau, KnnSymPlan{}, o.perm, nullptr, 0u, o.sample);
   else if (phase == 1)
     knn_mfma_kernel<KCH, kCandRT, MODE, 1><<<dim3(groups, nsplit), 256, 0, s>>>(
-        h, l, arow, rinv, n, ntiles, ss, capg, gcnt, gcand, gtau, KnnSymPlan{}, o.perm, nullptr, 0u);
+        h, l, arow, rinv, n, ntiles, ss, capg, gcnt, gcand, gtau, KnnSymPlan{}, o.perm, nullptr, 0u, KnnSampleSet{});
   else if (sp.items)
     knn_mfma_kernel<KCH, kCandRT, MODE, 2><<<dim3(sp.items, 1), 256, 0, s>>>(
-        h, l, arow, rinv, n, ntiles, ss, capg, gcnt, gcand, gtau, sp, o.perm, o.prune, o.prune ? o.pwords : 0u);
+        h, l, arow, rinv, n, ntiles, ss, capg, gcnt, gcand, gtau, sp, o.perm, o.prune, o.prune ? o.pwords : 0u,
+        KnnSampleSet{});
   return hipGetLastError();
 }
 

@@ -6,6 +6,7 @@
 // W3 (dims x H), b3 (dims); y = W3 gelu(W2 gelu(W1 x + b1) + b2) + b3.
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 #include <numeric>
 
 #include "host_common.h"
@@ -42,11 +43,13 @@ struct MlpShape {
 struct MlpWeights {
   DevBuf<float> p;
   DevBuf<uint16_t> w1b, w2b, w2tb;
-  MlpWeights(const MlpShape &sh, const float *host, hipStream_t s)
+  // transposed: also W2^T in bf16 (the backward pass's operand; a
+  // transform does without it)
+  MlpWeights(const MlpShape &sh, const float *host, hipStream_t s, bool transposed = true)
       : p(sh.total), w1b(static_cast<size_t>(sh.H) * sh.Kp), w2b(static_cast<size_t>(sh.H) * sh.H),
-        w2tb(static_cast<size_t>(sh.H) * sh.H) {
+        w2tb(transposed ? static_cast<size_t>(sh.H) * sh.H : 1) {
     upload(p, host, sh.total, s);
-    pack(sh, s);
+    PM_HIP(launch_mlp_pack(p.get(), sh.d, sh.Kp, sh.H, w1b.get(), w2b.get(), transposed ? w2tb.get() : nullptr, s));
   }
   void pack(const MlpShape &sh, hipStream_t s) {
     PM_HIP(launch_mlp_pack(p.get(), sh.d, sh.Kp, sh.H, w1b.get(), w2b.get(), w2tb.get(), s));
@@ -157,6 +160,69 @@ void fill_init(const MlpShape &sh, uint64_t seed, float *p) {
 }
 
 }  // namespace
+}  // namespace pm
+
+namespace pm {
+// The fused transform's device weights (fp32 parameters, bf16 W1 and W2),
+// kept per thread: a call whose parameters are byte-equal to the previous
+// call's (same shape, same device) reuses them instead of uploading and
+// converting again.  One block of device memory, replaced when the
+// parameters change; freed with the thread (pacmap_release_cached_memory
+// frees the calling thread's).
+struct TransformWeights {
+  int dev = -1;
+  uint32_t d = 0, H = 0, dims = 0;
+  std::vector<float> host;
+  void *blk = nullptr;
+  float *p = nullptr;
+  uint16_t *w1b = nullptr, *w2b = nullptr;
+  void release() {
+    host.clear();
+    if (!blk) return;
+    int cur = 0;
+    PM_HIP(hipGetDevice(&cur));
+    PM_HIP(hipSetDevice(dev));
+    PM_HIP(hipDeviceSynchronize());  // no earlier call still reads it
+    PM_HIP(hipFree(blk));
+    PM_HIP(hipSetDevice(cur));
+    blk = nullptr;
+  }
+  ~TransformWeights() {
+    if (blk && !process_exiting()) (void)hipFree(blk);
+  }
+};
+thread_local TransformWeights t_transform_weights;
+
+void mlp_transform_cache_release() { t_transform_weights.release(); }
+
+const TransformWeights &transform_weights(const MlpShape &sh, const float *params, hipStream_t s) {
+  TransformWeights &c = t_transform_weights;
+  int dev = 0;
+  PM_HIP(hipGetDevice(&dev));
+  const size_t bytes = sh.total * sizeof(float);
+  if (c.blk && c.dev == dev && c.d == sh.d && c.H == sh.H && c.dims == sh.dims && c.host.size() == sh.total &&
+      std::memcmp(c.host.data(), params, bytes) == 0)
+    return c;
+  c.release();  // a failure below leaves no match behind
+  const size_t pb = (bytes + 255) & ~size_t{255};
+  const size_t w1 = (static_cast<size_t>(sh.H) * sh.Kp * sizeof(uint16_t) + 255) & ~size_t{255};
+  const size_t w2 = static_cast<size_t>(sh.H) * sh.H * sizeof(uint16_t);
+  note_runtime_use();
+  PM_HIP(hipMalloc(&c.blk, pb + w1 + w2));
+  c.dev = dev;
+  c.p = static_cast<float *>(c.blk);
+  c.w1b = reinterpret_cast<uint16_t *>(static_cast<char *>(c.blk) + pb);
+  c.w2b = reinterpret_cast<uint16_t *>(static_cast<char *>(c.blk) + pb + w1);
+  PM_HIP(hipMemcpyAsync(c.p, params, bytes, hipMemcpyHostToDevice, s));
+  PM_HIP(launch_mlp_pack(c.p, sh.d, sh.Kp, sh.H, c.w1b, c.w2b, nullptr, s));
+  PM_HIP(hipStreamSynchronize(s));
+  c.d = sh.d;
+  c.H = sh.H;
+  c.dims = sh.dims;
+  c.host.assign(params, params + sh.total);
+  return c;
+}
+
 }  // namespace pm
 
 using namespace pm;
@@ -305,15 +371,16 @@ static int mlp_transform_impl(const float *x_host, const float *x_dev, uint64_t 
       upload(dxb, x_host, n * d, s);
       xd = dxb.get();
     }
-    MlpWeights W(sh, params, s);
     if (mlp_transform_fused_ok(sh.H, sh.dims, sh.Kp)) {
+      const TransformWeights &T = transform_weights(sh, params, s);
       DevBuf<float> Y(n * dims);
-      PM_HIP(launch_mlp_transform(xd, n, sh.d, sh.Kp, W.w1b.get(), W.p.get() + sh.ob1, W.w2b.get(), W.p.get() + sh.ob2,
-                                  W.p.get() + sh.oW3, W.p.get() + sh.ob3, sh.dims, Y.get(), s));
+      PM_HIP(launch_mlp_transform(xd, n, sh.d, sh.Kp, T.w1b, T.p + sh.ob1, T.w2b, T.p + sh.ob2, T.p + sh.oW3,
+                                  T.p + sh.ob3, sh.dims, Y.get(), s));
       download(y_out, Y.get(), n * dims, s);
       PM_HIP(hipStreamSynchronize(s));
       return;
     }
+    MlpWeights W(sh, params, s, false);
     const uint32_t Mt = static_cast<uint32_t>(std::min<uint64_t>(kTransformRows, (n + 63) / 64 * 64));
     DevBuf<uint16_t> xb(static_cast<size_t>(Mt) * sh.Kp), h1b(static_cast<size_t>(Mt) * sh.H);
     DevBuf<float> h2f(static_cast<size_t>(Mt) * sh.H), Y(n * dims);

@@ -348,4 +348,5 @@ extern "C" void pacmap_release_cached_memory(void) {
   pm::graph_cache_reset();
   pm::dev_trim();
   pm::staging_release();
+  pm::mlp_transform_cache_release();
 }

@@ -606,7 +606,7 @@ class mlp:  # noqa: N801 -- namespace of the parametric encoder entry points
             x = np.ascontiguousarray(x, np.float32)
         n, d = x.shape
         p = np.ascontiguousarray(params, np.float32)
-        y = np.zeros((n, dims), np.float32)
+        y = np.empty((n, dims), np.float32)  # every element is written
         st = Status()
         fn = lib().pacmap_mlp_transform_dev if dev else lib().pacmap_mlp_transform
         _check(fn(x.ptr if dev else x.ctypes.data, n, d, hidden, dims, p.ctypes.data, y.ctypes.data,

@@ -354,7 +354,8 @@ def test_c4_encoder_gain_over_plain_method():
             ts.append(time.perf_counter() - t0)
         return out, min(ts)
 
-    y_enc, t_enc = best(lambda: pm.mlp.transform(xd_te, params), 7)
+    # the 0.2 ms transform is timed over more repetitions than the 20 ms fit
+    y_enc, t_enc = best(lambda: pm.mlp.transform(xd_te, params), 21)
     (y_plain, _), t_plain = best(lambda: pm.fit_transform(xd_raw, cfg), 3)
     p_enc, p_plain = _preservation(xp_te, y_enc), _preservation(xp_te, y_plain)
     tr_enc = bench.triplet_accuracy(xp_te, y_enc, np.random.default_rng(7))
