@@ -216,7 +216,9 @@ int pacmap_mlp_fit(const float *x, uint64_t n, uint64_t d, uint64_t dims, const 
                    uint64_t n_nn, const uint32_t *pair_mn, uint64_t n_mn, const uint32_t *pair_fp,
                    uint64_t n_fp, const pacmap_mlp_config *cfg, float *params, float *epoch_loss,
                    pacmap_status *st);
-/* y_out (n x dims) = encoder(x); x on the host or (``_dev``) in device memory. */
+/* y_out (n x dims) = encoder(x); x on the host or (``_dev``) in device memory.
+ * The converted weights of a thread's last call are kept and reused while
+ * params stay byte-equal (pacmap_release_cached_memory frees them). */
 int pacmap_mlp_transform(const float *x, uint64_t n, uint64_t d, uint64_t hidden, uint64_t dims,
                          const float *params, float *y_out, pacmap_status *st);
 int pacmap_mlp_transform_dev(const float *x_dev, uint64_t n, uint64_t d, uint64_t hidden,
@@ -255,8 +257,8 @@ int pacmap_dev_download(void *dst_host, const void *src_dev, uint64_t bytes, pac
  * graph, pinned buffers and cached blocks are released when the thread
  * exits; an allocation that runs out of device memory first frees the idle
  * cached blocks of every thread and retries.  This frees the idle cached
- * blocks of every thread, and the calling thread's graph and pinned upload
- * buffers. */
+ * blocks of every thread, and the calling thread's graph, pinned upload
+ * buffers and cached encoder transform weights. */
 void pacmap_release_cached_memory(void);
 /* HBM reference: a plain streaming device-to-device copy kernel run `repeats`
  * times; ms_out gets the average time per copy (HIP events).  Its rocprof
