@@ -161,6 +161,31 @@ def test_mlp_fit_deterministic_and_loss_falls():
 
 
 @pytest.mark.gpu
+def test_mlp_transform_kept_weights_follow_parameter_changes():
+    """The fused transform keeps its converted weights between calls while
+    the parameters stay byte-equal: a change made in place to the same array
+    (a trained step), a different shape, and a release of the cached memory
+    must each give the result of a first call on those parameters."""
+    import pacmap_mi355x as pm
+    rng = np.random.default_rng(21)
+    n, d = 777, 100
+    x = pm.DeviceArray(rng.normal(size=(n, d)).astype(np.float32))
+    p = pm.mlp.init(d, 256, 2, seed=4)
+    y1 = pm.mlp.transform(x, p)
+    assert np.array_equal(pm.mlp.transform(x, p), y1)  # reused weights, same bits
+    p[5] += 0.25  # W1 entry, changed in place
+    p[-1] += 1.0  # output bias
+    y2 = pm.mlp.transform(x, p)
+    assert not np.array_equal(y2, y1)
+    pm.release_cached_memory()
+    assert np.array_equal(pm.mlp.transform(x, p), y2)  # rebuilt from scratch: same bits
+    q = pm.mlp.init(d, 256, 3, seed=4)  # another shape
+    y3 = pm.mlp.transform(x, q, dims=3)
+    assert y3.shape == (n, 3)
+    assert np.array_equal(pm.mlp.transform(x, p), y2)
+
+
+@pytest.mark.gpu
 def test_generate_pairs_match_fit_transform_pipeline():
     """generate_pairs returns the pairs fit_transform optimises: with them as
     AllProvided (and the same Value init) the two embeddings are identical."""
