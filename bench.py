@@ -38,8 +38,8 @@ MEASURED_TRAFFIC = {
     # grad_sell_kernel FETCH 14,983.5 KiB x 2.0 + WRITE 2,213.4 KiB, plus
     # grad_combine_elem_kernel FETCH 1,927.1 KiB x 2.0 + WRITE 1,640.6 KiB
     ("70k", "grad_sell"): 38_580_000,
-    # symmetric candidate pass: FETCH 736,962.9 KiB x 2.0 + WRITE 781,193.0 KiB
-    ("70k", "knn_mfma_kernel"): 2_309_240_000,
+    # symmetric candidate pass: FETCH 741,790.8 KiB x 2.0 + WRITE 807,758.9 KiB
+    ("70k", "knn_mfma_kernel"): 2_346_330_000,
 }
 FP32_PEAK_TFLOPS = 157.3
 BF16_PEAK_TFLOPS = 2500.0
